@@ -1,0 +1,164 @@
+/*
+ * OpenSTC-MI355X C ABI: the per-step data-parallel hot path of the
+ * structured-grid reacting-flow solver (reconstruction + Riemann flux,
+ * central viscous flux with thermo/transport, finite-rate chemistry with the
+ * point-implicit source, SSP-RK3 update, CFL dt, residual, boundary
+ * conditions) on one gfx950 GPU.
+ *
+ * Conventions: every entry point returns 0 on success and a nonzero status
+ * otherwise (no exception crosses the ABI); ostc_last_error() gives the
+ * calling thread's message.  One host thread drives one solver; all device
+ * work is ordered on the solver's HIP stream and only calls returning host
+ * scalars synchronise.  Field arrays use the reference's DZone layout
+ * (src/gxl_lib/Array.hpp:141-155): component-major, i fastest, ngg ghost
+ * layers per side.  The reference exposes these operations as C++ template
+ * launchers over (Block&, DZone*, DParameter*); each entry below names the
+ * one it replaces.
+ */
+#ifndef OSTC_H
+#define OSTC_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct ostc_solver ostc_solver;
+
+/* field identifiers for ostc_field_info / ostc_upload / ostc_download */
+enum ostc_field {
+  OSTC_BV = 0,       /* rho,u,v,w,p,T (6)            ghosted  Field.h:31 */
+  OSTC_SV = 1,       /* Y_1..Y_ns, k, omega (n_scalar) ghosted Field.h:32 */
+  OSTC_CV = 2,       /* rho, rho u, .., rho E, rho Y (n_var) ghosted Field.h:30 */
+  OSTC_DQ = 3,       /* RHS * J (n_var)              no ghosts Field.h:53 */
+  OSTC_VEL = 4,      /* |V|                          ghosted */
+  OSTC_ACOUSTIC = 5, /* speed of sound               ghosted */
+  OSTC_MACH = 6,
+  OSTC_MUL = 7,      /* laminar viscosity */
+  OSTC_COND = 8,     /* thermal conductivity */
+  OSTC_RHOD = 9,     /* rho D_k (n_spec) */
+  OSTC_GAMMA = 10,
+  OSTC_CP = 11,
+  OSTC_DT_LOCAL = 12,/* no ghosts */
+  OSTC_QN = 13,      /* cv at t^n */
+  OSTC_BV_LAST = 14, /* residual work array (4), no ghosts */
+  OSTC_JAC = 15,     /* cell Jacobian J = det(dx/dxi), ghosted */
+  OSTC_METRIC = 16,  /* 9 components d xi_r/d x_c, component-major, ghosted */
+  OSTC_CHEM_JAC = 17 /* n_spec^2 EPI Jacobian, no ghosts */
+};
+
+/* Version of this ABI (bumped on any signature change). */
+int ostc_abi_version(void);
+/* Message of the last failed call on this thread ("" if none). */
+const char *ostc_last_error(void);
+
+/* Reads <case_dir>/input_files/setup/*.txt in the reference order with its
+ * grammar and derived keys, and (species != 0) the CHEMKIN, therm and tran
+ * files named there; selects HIP device `device` (device < 0 opens the
+ * solver host-only: setup files, tables and grid metrics, no device work).
+ * Replaces: Parameter::Parameter (Parameter.cpp:8-52), Species/Reaction
+ * constructors (ChemData.cpp:11-632), DParameter::DParameter (DParameter.cu:7-159). */
+int ostc_init(const char *case_dir, int device, ostc_solver **out);
+int ostc_destroy(ostc_solver *s);
+
+/* Integer / real setup keys after derivation (e.g. "ngg", "n_var", "n_spec",
+ * "n_reac", "inviscid_type"). */
+int ostc_get_int(ostc_solver *s, const char *key, int *value);
+int ostc_get_real(ostc_solver *s, const char *key, double *value);
+/* Mechanism tables as parsed (names as in DParameter: "mw", "t_low",
+ * "high_coeff", "stoi_f", "label", "A", "Ea2", "third_body_coeff", ...).
+ * `out` holds `count` doubles (integer tables are converted). */
+int ostc_get_table(ostc_solver *s, const char *name, double *out, int64_t count);
+
+/* Adds one structured block from its interior node coordinates (i fastest,
+ * length mx*my*mz); builds ghost nodes and metrics.
+ * Replaces: Mesh::read_grid + init_ghost_grid + compute_jac_metric
+ * (Mesh.cpp:599-686, 790-944, 259-521). Returns the block index in *blk. */
+int ostc_add_block(ostc_solver *s, int mx, int my, int mz, const double *x, const double *y, const double *z,
+                   int *blk);
+/* One boundary-file line "i1 i2 j1 j2 k1 k2 label" (0-based).
+ * Replaces: Mesh::read_boundary (Mesh.cpp:686-706). */
+int ostc_add_boundary(ostc_solver *s, int blk, int i1, int i2, int j1, int j2, int k1, int k2, int label);
+/* Reads <case_dir>/input_files/boundary_condition/boundary   0.txt for all blocks. */
+int ostc_read_boundary_file(ostc_solver *s);
+/* Allocates the device zones and resolves boundary labels against the
+ * setup-file structs.  Replaces: Field::setup_device_memory (Field.cu:133-251),
+ * DBoundCond::initialize_bc_on_GPU (BoundCond.cu:167-384). */
+int ostc_finalize(ostc_solver *s);
+
+/* Host copies of a block's Jacobian (ghosted) and 9 metric components
+ * (component-major), available before ostc_finalize and in host-only mode.
+ * Either pointer may be NULL. */
+int ostc_block_geometry(ostc_solver *s, int blk, double *jac, double *metric);
+
+/* Field shape: element count, ghost layers, components. */
+int ostc_field_info(ostc_solver *s, int blk, int field, int64_t *count, int *ghosts, int *ncomp);
+int ostc_upload(ostc_solver *s, int blk, int field, const double *host);
+int ostc_download(ostc_solver *s, int blk, int field, double *host);
+/* Device address of a field (for callers that keep their data in HBM). */
+int ostc_field_device_ptr(ostc_solver *s, int blk, int field, double **dptr);
+
+/* Uniform initial field from the setup files' default_init inflow.
+ * Replaces: initialize_from_start (Initialize.cu:5-47) for groups_init = 1. */
+int ostc_initialize_uniform(ostc_solver *s, int blk);
+/* Reference inflow state of struct `name`: out[0..6] = rho,u,v,w,p,T,|V|,
+ * out[7..] = scalars.  Replaces: Inflow::Inflow (BoundCond.cpp:7-122). */
+int ostc_inflow_state(ostc_solver *s, const char *name, double *out, int n);
+/* Start-of-run sequence on every block: 2-D k-copy, boundary conditions,
+ * |V|, properties, last-step store, cv from bv on all points.
+ * Replaces: Driver::initialize_computation (Driver.cu:40-98) + RK.cuh:38-48. */
+int ostc_initialize(ostc_solver *s);
+
+/* dq = source + inviscid + viscous for one block.
+ * Replaces: compute_source (SourceTerm.cuh:11-37), compute_inviscid_flux
+ * (SchemeSelector.cuh:30-47), compute_viscous_flux (SchemeSelector.cuh:49-69). */
+int ostc_rhs(ostc_solver *s, int blk);
+/* Local CFL dt on every block and the global minimum, returned to the host.
+ * Replaces: local_time_step (TimeAdvanceFunc.cuh:50-100) + global_time_step
+ * (TimeAdvanceFunc.cu:155-180). */
+int ostc_global_dt(ostc_solver *s, double *dt);
+/* One SSP-RK3 stage update with a given dt.  Replaces: update_cv_and_bv_rk (RK.cuh:197-253). */
+int ostc_rk3_stage(ostc_solver *s, int blk, double dt, int rk);
+/* Boundary conditions in the reference priority order.
+ * Replaces: DBoundCond::apply_boundary_conditions (BoundCond.cuh:673-822). */
+int ostc_apply_bc(ostc_solver *s, int blk, int with_cv);
+/* Physical properties on [-1, m] (+ 2-D k-copy first when with_k_copy).
+ * Replaces: update_physical_properties (FieldOperation.cuh:112-146). */
+int ostc_update_properties(ostc_solver *s, int blk);
+/* `nsteps` full SSP-RK3 steps with the dt recomputed on the device every
+ * stage and the residual every step, no host synchronisation inside.
+ * dt_hist (3*nsteps) and res_hist (5*nsteps: 4 residuals, err_max) may be NULL.
+ * Replaces: RK3_bv (RK.cuh:19-195) without IO/monitors. */
+int ostc_step_rk3(ostc_solver *s, int nsteps, double *dt_hist, double *res_hist);
+/* Physical time and step counter. */
+int ostc_time(ostc_solver *s, double *t, int *step);
+/* 1 = flux arithmetic in FP32 (state, thermo, Newton T and accumulation stay FP64). */
+int ostc_set_flux_precision(ostc_solver *s, int fp32);
+
+/* One point-implicit chemistry step dt of n isolated constant-volume cells:
+ * rho[n], T[n], Y[n_spec*n] (component-major) in, dY[n_spec*n], dT[n] out.
+ * All pointers are DEVICE pointers.  mode 0 = FP64 point-implicit integrator;
+ * mode 1 = trained MLP surrogate with per-cell fallback to mode 0
+ * (*n_fallback receives the fallback count).  New entry point: the reference
+ * reaches this arithmetic only inside compute_source + EPI
+ * (FiniteRateChem.cu:7-63, 234-256). */
+int ostc_chem_batch(ostc_solver *s, int64_t n, const double *rho, const double *T, const double *Y, double dt,
+                    double *dY, double *dT, int mode, int64_t *n_fallback);
+
+/* Device memory helpers for callers without their own allocator. */
+int ostc_dev_alloc(ostc_solver *s, int64_t bytes, void **dptr);
+int ostc_dev_free(ostc_solver *s, void *dptr);
+int ostc_memcpy_h2d(ostc_solver *s, void *dst, const void *src, int64_t bytes);
+int ostc_memcpy_d2h(ostc_solver *s, void *dst, const void *src, int64_t bytes);
+int ostc_sync(ostc_solver *s);
+
+/* Kernel timing with HIP events on the solver stream (0 off, 1 on) and a
+ * text summary "name calls total_ms" per kernel family. */
+int ostc_profile(ostc_solver *s, int enable);
+int ostc_profile_report(ostc_solver *s, char *buf, int len);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

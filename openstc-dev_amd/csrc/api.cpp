@@ -1,0 +1,1030 @@
+// C ABI of the solver (include/ostc.h): set-up from the reference's input
+// files, device zones in the DZone layout, the stage operations and the
+// fused RK3 driver (RK.cuh:19-195 without IO), all stream-ordered on one
+// HIP stream per solver.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <map>
+#include <memory>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "../../include/ostc.h"
+#include "device_common.h"
+#include "flux_kernels.h"
+#include "ostc_internal.h"
+#include "surrogate.h"
+
+using namespace ostc;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(const std::string &msg) {
+  g_err = msg;
+  return 1;
+}
+
+#define HIPCHK(expr)                                                                          \
+  do {                                                                                        \
+    hipError_t e_ = (expr);                                                                   \
+    if (e_ != hipSuccess) return fail(std::string(#expr) + ": " + hipGetErrorString(e_));   \
+  } while (0)
+
+struct Inflow {
+  std::string name;
+  int label = 5;
+  double rho = 0, u = 0, v = 0, w = 0, p = 0, t = 0, vel = 0, mach = 0, mw = kMwAir;
+  std::vector<double> sv;
+};
+
+struct WallBC {
+  int label = 2;
+  bool isothermal = true;
+  double t = 300;
+};
+
+struct Block {
+  int mx = 0, my = 0, mz = 0;
+  std::vector<double> jac, metric;  // host copies (ghosted, component-major metric)
+  std::vector<BoundaryFace> faces;
+  std::vector<BCDesc> bcs;          // in application order
+  ZoneView z{};
+  std::vector<void *> mem;
+};
+
+struct Prof {
+  std::string name;
+  hipEvent_t a, b;
+};
+
+}  // namespace
+
+struct ostc_solver {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string case_dir;
+  Params prm;
+  Mechanism mech;
+  DMech *d_mech = nullptr;
+  DParams P{};
+  int ngg = 2, n_var = 5, n_spec = 0, n_scalar = 0;
+  std::vector<Block> blocks;
+  std::vector<Inflow> inflows;
+  std::vector<WallBC> walls;
+  bool finalized = false;
+  bool fp32 = false;
+  StepScratch w{};
+  double t_phys = 0;
+  int step = 0;
+  bool prof = false;
+  std::vector<Prof> pending;
+  std::map<std::string, std::pair<long, double>> stats;
+  std::vector<void *> user_mem;
+  SurrogateState *sur = nullptr;
+
+  ~ostc_solver() {
+    for (auto &b : blocks)
+      for (void *p : b.mem) (void)hipFree(p);
+    for (void *p : user_mem) (void)hipFree(p);
+    if (d_mech) (void)hipFree(d_mech);
+    (void)hipFree(w.partial);
+    (void)hipFree(w.counter);
+    (void)hipFree(w.dt);
+    (void)hipFree(w.dt_hist);
+    (void)hipFree(w.res_hist);
+    (void)hipFree(w.res_scale);
+    for (auto &p : pending) {
+      (void)hipEventDestroy(p.a);
+      (void)hipEventDestroy(p.b);
+    }
+    surrogate_destroy(sur);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+
+  // RAII timing of one launch group
+  struct Timer {
+    ostc_solver *s;
+    Prof p{};
+    bool on;
+    Timer(ostc_solver *sv, const char *name) : s(sv), on(sv->prof) {
+      if (!on) return;
+      p.name = name;
+      (void)hipEventCreate(&p.a);
+      (void)hipEventCreate(&p.b);
+      (void)hipEventRecord(p.a, s->stream);
+    }
+    ~Timer() {
+      if (!on) return;
+      (void)hipEventRecord(p.b, s->stream);
+      s->pending.push_back(p);
+    }
+  };
+
+  void drain_profile() {
+    if (pending.empty()) return;
+    (void)hipStreamSynchronize(stream);
+    for (auto &p : pending) {
+      float ms = 0;
+      (void)hipEventElapsedTime(&ms, p.a, p.b);
+      auto &e = stats[p.name];
+      e.first += 1;
+      e.second += ms;
+      (void)hipEventDestroy(p.a);
+      (void)hipEventDestroy(p.b);
+    }
+    pending.clear();
+  }
+};
+
+namespace {
+
+// BoundCond.cpp:7-122 Inflow::Inflow for the keys a struct may carry.
+int make_inflow(ostc_solver *s, const std::string &name, Inflow &in) {
+  auto it = s->prm.st.find(name);
+  if (it == s->prm.st.end()) return fail("setup struct '" + name + "' not found");
+  const Struct &info = it->second;
+  auto get_r = [&](const char *k, double &v) {
+    auto f = info.find(k);
+    if (f == info.end()) return false;
+    if (std::holds_alternative<double>(f->second))
+      v = std::get<double>(f->second);
+    else if (std::holds_alternative<int>(f->second))
+      v = std::get<int>(f->second);
+    return true;
+  };
+  in.name = name;
+  auto lf = info.find("label");
+  if (lf != info.end() && std::holds_alternative<int>(lf->second)) in.label = std::get<int>(lf->second);
+  double mach = -1, pressure = 101325, temperature = -1, velocity = 0, density = -1, u = 1, v = 0, w = 0;
+  get_r("mach", mach);
+  get_r("pressure", pressure);
+  get_r("temperature", temperature);
+  get_r("velocity", velocity);
+  get_r("density", density);
+  get_r("u", u);
+  get_r("v", v);
+  get_r("w", w);
+  const int ns = s->n_spec;
+  in.sv.assign(std::max(s->n_scalar, 0), 0.0);
+  double mw = kMwAir;
+  if (ns > 0) {
+    for (int l = 0; l < ns; ++l) get_r(s->mech.names[l].c_str(), in.sv[l]);
+    double acc = 0;
+    for (int l = 0; l < ns; ++l) acc += in.sv[l] / s->mech.mw[l];
+    mw = 1 / acc;
+  }
+  if (temperature < 0) temperature = pressure * mw / (density * kRu);
+  double gamma = kGammaAir;
+  if (ns > 0) {
+    double cp = 0, cv = 0;
+    for (int l = 0; l < ns; ++l) {
+      // Species::compute_cp (ChemData.cpp:110-128)
+      const double tt = temperature < s->mech.t_low[l] ? s->mech.t_low[l] : temperature;
+      const double *c = (temperature < s->mech.t_low[l] || tt < s->mech.t_mid[l]) ? &s->mech.lo[l * 7]
+                                                                                  : &s->mech.hi[l * 7];
+      double cpi;
+      if (temperature < s->mech.t_low[l]) {
+        const double tt2 = tt * tt, tt3 = tt2 * tt, tt4 = tt3 * tt;
+        cpi = c[0] + c[1] * tt + c[2] * tt2 + c[3] * tt3 + c[4] * tt4;
+      } else {
+        const double t2 = temperature * temperature, t3 = t2 * temperature, t4 = t3 * temperature;
+        cpi = c[0] + c[1] * tt + c[2] * t2 + c[3] * t3 + c[4] * t4;
+      }
+      cpi *= kRu / s->mech.mw[l];
+      cp += in.sv[l] * cpi;
+      cv += in.sv[l] * (cpi - kRu / s->mech.mw[l]);
+    }
+    gamma = cp / cv;
+  }
+  const double c = std::sqrt(gamma * kRu / mw * temperature);
+  if (mach < 0)
+    mach = velocity / c;
+  else
+    velocity = mach * c;
+  u *= velocity;
+  v *= velocity;
+  w *= velocity;
+  if (density < 0) density = pressure * mw / (kRu * temperature);
+  in.rho = density;
+  in.u = u;
+  in.v = v;
+  in.w = w;
+  in.p = pressure;
+  in.t = temperature;
+  in.vel = velocity;
+  in.mach = mach;
+  in.mw = mw;
+  return 0;
+}
+
+int upload_mech(ostc_solver *s) {
+  DMech h{};
+  const Mechanism &m = s->mech;
+  h.n_spec = m.n_spec;
+  h.n_reac = m.n_reac;
+  for (int i = 0; i < m.n_spec; ++i) {
+    h.mw[i] = m.mw[i];
+    h.mw_inv[i] = 1.0 / m.mw[i];
+    h.t_low[i] = m.t_low[i];
+    h.t_mid[i] = m.t_mid[i];
+    for (int c = 0; c < 7; ++c) {
+      h.hi[i * 7 + c] = m.hi[i * 7 + c];
+      h.lo[i * 7 + c] = m.lo[i * 7 + c];
+    }
+    h.LJ_potent_inv[i] = m.LJ_potent_inv[i];
+    h.vis_coeff[i] = m.vis_coeff[i];
+  }
+  const int ns = m.n_spec, nr = m.n_reac;
+  for (int i = 0; i < ns * ns; ++i) {
+    h.WjDivWi_to_One4th[i] = m.WjDivWi_to_One4th[i];
+    h.sqrt_WiDivWjPl1Mul8[i] = m.sqrt_WiDivWjPl1Mul8[i];
+    h.binary_diffusivity_coeff[i] = m.binary_diffusivity_coeff[i];
+    h.kb_over_eps_jk[i] = m.kb_over_eps_jk[i];
+  }
+  for (int r = 0; r < nr; ++r) {
+    h.label[r] = m.label[r];
+    h.order[r] = m.order[r];
+    h.A[r] = m.A[r];
+    h.b[r] = m.b[r];
+    h.Ea[r] = m.Ea[r];
+    h.A2[r] = m.A2[r];
+    h.b2[r] = m.b2[r];
+    h.Ea2[r] = m.Ea2[r];
+    h.troe_alpha[r] = m.troe_alpha[r];
+    h.troe_t3[r] = m.troe_t3[r];
+    h.troe_t1[r] = m.troe_t1[r];
+    h.troe_t2[r] = m.troe_t2[r];
+  }
+  for (int i = 0; i < nr * ns; ++i) {
+    h.stoi_f[i] = m.stoi_f[i];
+    h.stoi_b[i] = m.stoi_b[i];
+    h.third_body_coeff[i] = m.third_body_coeff[i];
+  }
+  HIPCHK(hipMalloc(&s->d_mech, sizeof(DMech)));
+  HIPCHK(hipMemcpy(s->d_mech, &h, sizeof(DMech), hipMemcpyHostToDevice));
+  return 0;
+}
+
+struct FieldDesc {
+  double *ptr;
+  int64_t count;
+  int ghosts, ncomp;
+};
+
+FieldDesc field_desc(ostc_solver *s, Block &b, int f) {
+  ZoneView &z = b.z;
+  const int64_t sg = z.sz, s0 = z.sz0;
+  switch (f) {
+    case OSTC_BV: return {z.bv, sg * 6, z.g, 6};
+    case OSTC_SV: return {z.sv, sg * s->n_scalar, z.g, s->n_scalar};
+    case OSTC_CV: return {z.cv, sg * s->n_var, z.g, s->n_var};
+    case OSTC_DQ: return {z.dq, s0 * s->n_var, 0, s->n_var};
+    case OSTC_VEL: return {z.vel, sg, z.g, 1};
+    case OSTC_ACOUSTIC: return {z.acoustic, sg, z.g, 1};
+    case OSTC_MACH: return {z.mach, sg, z.g, 1};
+    case OSTC_MUL: return {z.mul, sg, z.g, 1};
+    case OSTC_COND: return {z.cond, sg, z.g, 1};
+    case OSTC_RHOD: return {z.rhoD, sg * s->n_spec, z.g, s->n_spec};
+    case OSTC_GAMMA: return {z.gam, s->n_spec ? sg : 0, z.g, 1};
+    case OSTC_CP: return {z.cp, s->n_spec ? sg : 0, z.g, 1};
+    case OSTC_DT_LOCAL: return {z.dt_local, s0, 0, 1};
+    case OSTC_QN: return {z.qn, sg * s->n_var, z.g, s->n_var};
+    case OSTC_BV_LAST: return {z.bv_last, s0 * 4, 0, 4};
+    case OSTC_JAC: return {const_cast<double *>(z.jac), sg, z.g, 1};
+    case OSTC_METRIC: return {const_cast<double *>(z.metric), sg * 9, z.g, 9};
+    case OSTC_CHEM_JAC: return {z.chem_jac, s->P.chemSrcMethod == 1 ? s0 * s->n_spec * s->n_spec : 0, 0,
+                                s->n_spec * s->n_spec};
+  }
+  return {nullptr, -1, 0, 0};
+}
+
+int check_blk(ostc_solver *s, int blk) {
+  if (!s) return fail("null solver");
+  if (blk < 0 || blk >= (int)s->blocks.size()) return fail("block index out of range");
+  if (!s->finalized) return fail("ostc_finalize has not been called");
+  return 0;
+}
+
+// Boundary::register_boundary (Mesh.cpp:14-31)
+void register_face(BoundaryFace &b, int ngg, int dim) {
+  int s[3] = {b.raw[0], b.raw[2], b.raw[4]}, e[3] = {b.raw[1], b.raw[3], b.raw[5]};
+  b.face = 0;
+  b.direction = 1;
+  for (int d = 0; d < dim; ++d)
+    if (s[d] == e[d]) {
+      b.face = d;
+      if (s[d] == 0) b.direction = -1;
+      break;
+    }
+  for (int d = 0; d < dim; ++d) {
+    if (d == b.face) continue;
+    if (e[d] < s[d]) std::swap(s[d], e[d]);
+    s[d] -= ngg;
+    e[d] += ngg;
+  }
+  if (dim == 2) {
+    s[2] = -ngg;
+    e[2] = ngg;
+  }
+  for (int d = 0; d < 3; ++d) {
+    b.rs[d] = s[d];
+    b.re[d] = e[d];
+  }
+}
+
+int run_bcs(ostc_solver *s, Block &b, bool with_cv) {
+  ostc_solver::Timer t(s, "boundary");
+  for (auto &bc : b.bcs) HIPCHK(launch_bc(b.z, s->P, s->d_mech, bc, with_cv, s->stream));
+  return 0;
+}
+
+int rhs_block(ostc_solver *s, Block &b) {
+  bool init = true;
+  if (s->P.mix && s->P.reaction == 1) {
+    ostc_solver::Timer t(s, "source");
+    HIPCHK(launch_source(b.z, s->P, s->d_mech, s->stream));
+    init = false;
+  }
+  ostc_solver::Timer t(s, "flux_sweeps");
+  HIPCHK(launch_flux_sweeps(b.z, s->P, s->d_mech, init, s->fp32, s->stream));
+  return 0;
+}
+
+int properties_block(ostc_solver *s, Block &b) {
+  ostc_solver::Timer t(s, "properties");
+  HIPCHK(launch_properties(b.z, s->P, s->d_mech, s->stream));
+  return 0;
+}
+
+int ensure_hist(ostc_solver *s, int nsteps) {
+  static thread_local int cap = 0;
+  (void)cap;
+  return nsteps < 0 ? fail("negative step count") : 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ostc_abi_version(void) { return 1; }
+
+const char *ostc_last_error(void) { return g_err.c_str(); }
+
+int ostc_init(const char *case_dir, int device, ostc_solver **out) {
+  if (!case_dir || !out) return fail("null argument");
+  auto s = std::make_unique<ostc_solver>();
+  s->case_dir = case_dir;
+  s->device = device;
+  read_setup(s->case_dir, s->prm);
+  const Params &p = s->prm;
+  const int species = p.get_int("species", 0), reaction = p.get_int("reaction", 0);
+  if (species != 0 && species != 1) return fail("species = " + std::to_string(species) + " is not supported");
+  if (reaction == 2) return fail("flamelet model (reaction = 2) is not supported");
+  if (p.get_bool("turbulence", false)) return fail("turbulent runs are not supported by this build");
+  if (species == 1) {
+    const std::string in = s->case_dir + "/input_files";
+    auto e = read_mechanism(in, p.get_string("mechanism_file", "chemistry/combustion_mechanism.inp"),
+                            p.get_string("therm_file", "chemistry/therm.dat"),
+                            p.get_string("transport_file", "chemistry/tran.dat"), reaction == 1, s->mech);
+    if (!e.empty()) return fail(e);
+    s->n_spec = s->mech.n_spec;
+  }
+  // Field.cu:4-60: variables per cell
+  s->n_scalar = s->n_spec;
+  s->n_var = 5 + s->n_spec;
+  s->ngg = p.get_int("ngg", 2);
+  s->prm.i["n_spec"] = s->n_spec;
+  s->prm.i["n_reac"] = s->mech.n_reac;
+  s->prm.i["n_var"] = s->n_var;
+  s->prm.i["n_scalar"] = s->n_scalar;
+  DParams &P = s->P;
+  P.g = s->ngg;
+  P.nv = s->n_var;
+  P.ns = s->n_spec;
+  P.nsc = s->n_scalar;
+  P.mix = species ? 1 : 0;
+  P.reaction = reaction;
+  P.chemSrcMethod = p.get_int("chemSrcMethod", 0);
+  P.inviscid_scheme = p.get_int("inviscid_scheme", 3);
+  P.reconstruction = p.get_int("reconstruction", 2);
+  P.limiter = p.get_int("limiter", 0);
+  P.viscous = p.get_int("viscous_order", 2) != 0;
+  P.inviscid_type = p.get_int("inviscid_type", 0);
+  P.gradP = p.get_bool("gradPInDiffusionFlux", false) ? 1 : 0;
+  P.cfl = p.get_real("cfl", 1.0);
+  P.Pr = p.get_real("prandtl_number", 0.72);
+  P.Sc = p.get_real("schmidt_number", 0.9);
+  P.Prt = p.get_real("turbulent_prandtl_number", 0.9);
+  P.Sct = p.get_real("turbulent_schmidt_number", 0.9);
+  if (P.inviscid_type == 2) return fail("the Roe scheme (inviscid_scheme = 2 without WENO) is not supported yet");
+  if (P.inviscid_type == 1 && P.mix) return fail("AWENO for multi-species mixtures has no reference definition");
+  if (P.mix && P.ns != 9) return fail("this build is compiled for 9-species mechanisms");
+  if (device < 0) {  // host-only: setup files, tables and grid metrics, no device work
+    *out = s.release();
+    return 0;
+  }
+  int ndev = 0;
+  HIPCHK(hipGetDeviceCount(&ndev));
+  if (device < 0 || device >= ndev) return fail("HIP device " + std::to_string(device) + " not available");
+  HIPCHK(hipSetDevice(device));
+  HIPCHK(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+  if (upload_mech(s.get())) return 1;
+  *out = s.release();
+  return 0;
+}
+
+int ostc_destroy(ostc_solver *s) {
+  if (s && s->device < 0) {
+    delete s;
+  } else if (s) {
+    (void)hipSetDevice(s->device);
+    (void)hipStreamSynchronize(s->stream);
+    delete s;
+  }
+  return 0;
+}
+
+int ostc_get_int(ostc_solver *s, const char *key, int *value) {
+  if (!s || !key || !value) return fail("null argument");
+  auto it = s->prm.i.find(key);
+  if (it != s->prm.i.end()) {
+    *value = it->second;
+    return 0;
+  }
+  auto ib = s->prm.b.find(key);
+  if (ib != s->prm.b.end()) {
+    *value = ib->second ? 1 : 0;
+    return 0;
+  }
+  return fail(std::string("integer key '") + key + "' not set");
+}
+
+int ostc_get_real(ostc_solver *s, const char *key, double *value) {
+  if (!s || !key || !value) return fail("null argument");
+  auto it = s->prm.r.find(key);
+  if (it == s->prm.r.end()) return fail(std::string("real key '") + key + "' not set");
+  *value = it->second;
+  return 0;
+}
+
+int ostc_get_table(ostc_solver *s, const char *name, double *out, int64_t count) {
+  if (!s || !name || !out) return fail("null argument");
+  const Mechanism &m = s->mech;
+  const std::map<std::string, const std::vector<double> *> dt = {
+      {"mw", &m.mw}, {"t_low", &m.t_low}, {"t_mid", &m.t_mid}, {"t_high", &m.t_high}, {"high_coeff", &m.hi},
+      {"low_coeff", &m.lo}, {"LJ_potent_inv", &m.LJ_potent_inv}, {"vis_coeff", &m.vis_coeff},
+      {"WjDivWi_to_One4th", &m.WjDivWi_to_One4th}, {"sqrt_WiDivWjPl1Mul8", &m.sqrt_WiDivWjPl1Mul8},
+      {"binary_diffusivity_coeff", &m.binary_diffusivity_coeff}, {"kb_over_eps_jk", &m.kb_over_eps_jk},
+      {"A", &m.A}, {"b", &m.b}, {"Ea", &m.Ea}, {"A2", &m.A2}, {"b2", &m.b2}, {"Ea2", &m.Ea2},
+      {"third_body_coeff", &m.third_body_coeff}, {"troe_alpha", &m.troe_alpha}, {"troe_t3", &m.troe_t3},
+      {"troe_t1", &m.troe_t1}, {"troe_t2", &m.troe_t2}};
+  const std::map<std::string, const std::vector<int> *> it = {
+      {"label", &m.label}, {"stoi_f", &m.stoi_f}, {"stoi_b", &m.stoi_b}, {"order", &m.order},
+      {"elem_comp", &m.elem_comp}};
+  if (auto f = dt.find(name); f != dt.end()) {
+    if ((int64_t)f->second->size() != count) return fail("table size mismatch for " + std::string(name));
+    std::copy(f->second->begin(), f->second->end(), out);
+    return 0;
+  }
+  if (auto f = it.find(name); f != it.end()) {
+    if ((int64_t)f->second->size() != count) return fail("table size mismatch for " + std::string(name));
+    for (int64_t i = 0; i < count; ++i) out[i] = (*f->second)[i];
+    return 0;
+  }
+  return fail(std::string("unknown table ") + name);
+}
+
+int ostc_add_block(ostc_solver *s, int mx, int my, int mz, const double *x, const double *y, const double *z,
+                   int *blk) {
+  if (!s || !x || !y || !z) return fail("null argument");
+  if (s->finalized) return fail("blocks must be added before ostc_finalize");
+  if (mx < 2 * s->ngg + 2 || my < 2 * s->ngg + 2 || mz < 1) return fail("block too small for the ghost depth");
+  if (!s->blocks.empty() && ((mz == 1) != (s->blocks[0].mz == 1))) return fail("mixed 2-D / 3-D blocks");
+  Block b;
+  b.mx = mx;
+  b.my = my;
+  b.mz = mz;
+  std::string err;
+  if (!compute_metrics(mx, my, mz, s->ngg, x, y, z, b.jac, b.metric, err)) return fail(err);
+  s->blocks.push_back(std::move(b));
+  if (blk) *blk = (int)s->blocks.size() - 1;
+  return 0;
+}
+
+int ostc_add_boundary(ostc_solver *s, int blk, int i1, int i2, int j1, int j2, int k1, int k2, int label) {
+  if (!s) return fail("null solver");
+  if (blk < 0 || blk >= (int)s->blocks.size()) return fail("block index out of range");
+  BoundaryFace f;
+  const int r[6] = {i1, i2, j1, j2, k1, k2};
+  std::copy(r, r + 6, f.raw);
+  f.label = label;
+  s->blocks[blk].faces.push_back(f);
+  return 0;
+}
+
+int ostc_read_boundary_file(ostc_solver *s) {
+  if (!s) return fail("null solver");
+  std::ifstream f(s->case_dir + "/input_files/boundary_condition/boundary   0.txt");
+  if (!f) return fail("boundary file not found");
+  std::string line;
+  for (size_t b = 0; b < s->blocks.size(); ++b) {
+    if (!std::getline(f, line)) return fail("boundary file ends early");
+    const int n = std::stoi(line);
+    for (int q = 0; q < n; ++q) {
+      if (!std::getline(f, line)) return fail("boundary file ends early");
+      std::istringstream ls(line);
+      int v[7] = {0, 0, 0, 0, 0, 0, 0};
+      for (int &x : v) ls >> x;
+      if (ostc_add_boundary(s, (int)b, v[0], v[1], v[2], v[3], v[4], v[5], v[6])) return 1;
+    }
+  }
+  return 0;
+}
+
+int ostc_finalize(ostc_solver *s) {
+  if (!s) return fail("null solver");
+  if (s->finalized) return fail("already finalized");
+  if (s->device < 0) return fail("solver was opened host-only (device < 0)");
+  if (s->blocks.empty()) return fail("no block added");
+  HIPCHK(hipSetDevice(s->device));
+  const int dim = s->blocks[0].mz == 1 ? 2 : 3;
+  s->P.dim = dim;
+  // BoundCond.cu:167-280: label -> type, named structs first, then defaults
+  std::vector<int> labels;
+  for (auto &b : s->blocks)
+    for (auto &f : b.faces)
+      if (std::find(labels.begin(), labels.end(), f.label) == labels.end()) labels.push_back(f.label);
+  std::map<int, int> type_of;
+  std::vector<std::pair<int, int>> reg;  // (type, label) in registration order
+  auto bca = s->prm.sa.find("boundary_conditions");
+  if (bca != s->prm.sa.end()) {
+    for (auto &name : bca->second) {
+      auto st = s->prm.st.find(name);
+      if (st == s->prm.st.end()) return fail("boundary condition struct '" + name + "' missing");
+      const Struct &info = st->second;
+      auto lf = info.find("label");
+      auto tf = info.find("type");
+      if (lf == info.end() || tf == info.end()) return fail("struct '" + name + "' needs label and type");
+      const int label = std::get<int>(lf->second);
+      auto li = std::find(labels.begin(), labels.end(), label);
+      if (li == labels.end()) continue;
+      labels.erase(li);
+      const std::string type = std::get<std::string>(tf->second);
+      int t = 0;
+      if (type == "wall") t = BC_WALL;
+      else if (type == "inflow") t = BC_INFLOW;
+      else if (type == "outflow") t = BC_OUTFLOW;
+      else if (type == "symmetry") t = BC_SYMMETRY;
+      else if (type == "farfield") t = BC_FARFIELD;
+      else if (type == "subsonic_inflow") t = BC_SUBSONIC;
+      else if (type == "back_pressure") t = BC_BACKPRESSURE;
+      else return fail("unknown boundary type '" + type + "'");
+      type_of[label] = t;
+      reg.push_back({t, label});
+      if (t == BC_INFLOW) {
+        Inflow in;
+        if (make_inflow(s, name, in)) return 1;
+        in.label = label;
+        s->inflows.push_back(in);
+      } else if (t == BC_WALL) {
+        WallBC wb;
+        wb.label = label;
+        auto th = info.find("thermal_type");
+        if (th != info.end()) wb.isothermal = std::get<std::string>(th->second) == "isothermal";
+        auto tw = info.find("temperature");
+        if (wb.isothermal && tw != info.end()) wb.t = std::get<double>(tw->second);
+        s->walls.push_back(wb);
+      }
+    }
+  }
+  for (int lab : labels) {
+    if (lab == 2 || lab == 3 || lab == 4 || lab == 5 || lab == 6 || lab == 7 || lab == 9) {
+      type_of[lab] = lab;
+      reg.push_back({lab, lab});
+      if (lab == 2) {
+        WallBC wb;
+        wb.label = 2;
+        s->walls.push_back(wb);
+      }
+      if (lab == 5) return fail("inflow label 5 has no setup struct");
+    } else {
+      return fail("boundary label " + std::to_string(lab) + " has no type");
+    }
+  }
+  for (auto &[t, l] : reg)
+    if (t == BC_FARFIELD || t == BC_SUBSONIC || t == BC_BACKPRESSURE)
+      return fail("boundary type " + std::to_string(t) + " is not supported by this build");
+  // allocate zones
+  const int g = s->ngg;
+  int64_t max_cells = 0;
+  for (auto &b : s->blocks) {
+    ZoneView &z = b.z;
+    z.mx = b.mx;
+    z.my = b.my;
+    z.mz = b.mz;
+    z.g = g;
+    const Layout L(b.mx, b.my, b.mz, g), L0(b.mx, b.my, b.mz, 0);
+    z.d2 = L.d2;
+    z.d1 = L.d1;
+    z.sz = L.sz;
+    z.e2 = L0.d2;
+    z.e1 = L0.d1;
+    z.sz0 = L0.sz;
+    max_cells = std::max<int64_t>(max_cells, L0.sz);
+    auto alloc = [&](double **p, int64_t n) -> int {
+      n = std::max<int64_t>(n, 1);
+      HIPCHK(hipMalloc((void **)p, n * sizeof(double)));
+      HIPCHK(hipMemset(*p, 0, n * sizeof(double)));
+      b.mem.push_back(*p);
+      return 0;
+    };
+    const int nv = s->n_var, ns = s->n_spec;
+    double *jac = nullptr, *met = nullptr;
+    if (alloc(&z.bv, L.sz * 6) || alloc(&z.sv, L.sz * std::max(s->n_scalar, 1)) || alloc(&z.cv, L.sz * nv) ||
+        alloc(&z.qn, L.sz * nv) || alloc(&z.vel, L.sz) || alloc(&z.acoustic, L.sz) || alloc(&z.mach, L.sz) ||
+        alloc(&z.mul, L.sz) || alloc(&z.cond, L.sz) || alloc(&z.rhoD, ns ? L.sz * ns : 1) ||
+        alloc(&z.gam, ns ? L.sz : 1) || alloc(&z.cp, ns ? L.sz : 1) || alloc(&jac, L.sz) ||
+        alloc(&met, L.sz * 9) || alloc(&z.dq, L0.sz * nv) || alloc(&z.dt_local, L0.sz) ||
+        alloc(&z.bv_last, L0.sz * 4) ||
+        alloc(&z.chem_jac, s->P.chemSrcMethod == 1 ? L0.sz * ns * ns : 1))
+      return 1;
+    z.jac = jac;
+    z.metric = met;
+    HIPCHK(hipMemcpy(jac, b.jac.data(), L.sz * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(met, b.metric.data(), L.sz * 9 * sizeof(double), hipMemcpyHostToDevice));
+    // boundaries: register, trim walls (BoundCond.cu:369-384), order by priority
+    for (auto &f : b.faces) {
+      register_face(f, g, dim);
+      f.type = type_of.at(f.label);
+      if (f.type == BC_WALL)
+        for (int q = 0; q < 3; ++q) {
+          if (q == f.face) continue;
+          f.rs[q] += g;
+          f.re[q] -= g;
+        }
+    }
+    static const int prio[] = {BC_FARFIELD, BC_OUTFLOW, BC_BACKPRESSURE, BC_INFLOW, BC_SUBSONIC, BC_SYMMETRY, BC_WALL};
+    for (int t : prio)
+      for (auto &[rt, rl] : reg) {
+        if (rt != t) continue;
+        for (auto &f : b.faces) {
+          if (f.label != rl) continue;
+          BCDesc d{};
+          d.type = t;
+          d.face = f.face;
+          d.dir = f.direction;
+          for (int q = 0; q < 3; ++q) {
+            d.rs[q] = f.rs[q];
+            d.re[q] = f.re[q];
+          }
+          if (t == BC_INFLOW) {
+            const Inflow *in = nullptr;
+            for (auto &x : s->inflows)
+              if (x.label == rl) in = &x;
+            if (!in) return fail("no inflow state for label " + std::to_string(rl));
+            d.rho = in->rho;
+            d.u = in->u;
+            d.v = in->v;
+            d.w = in->w;
+            d.p = in->p;
+            d.t = in->t;
+            d.vel = in->vel;
+            for (size_t l = 0; l < in->sv.size() && l < (size_t)kMS + 4; ++l) d.sv[l] = in->sv[l];
+          } else if (t == BC_WALL) {
+            d.t_wall = 300;
+            d.isothermal = 1;
+            for (auto &x : s->walls)
+              if (x.label == rl) {
+                d.t_wall = x.t;
+                d.isothermal = x.isothermal ? 1 : 0;
+              }
+          }
+          b.bcs.push_back(d);
+        }
+      }
+  }
+  // per-run scratch: partials, ticket counter, dt slot, residual scale
+  s->w.max_blocks = (int)((max_cells + 255) / 256) + 1;
+  HIPCHK(hipMalloc(&s->w.partial, sizeof(double) * 4 * s->w.max_blocks));
+  HIPCHK(hipMalloc(&s->w.counter, sizeof(unsigned)));
+  HIPCHK(hipMemset(s->w.counter, 0, sizeof(unsigned)));
+  HIPCHK(hipMalloc(&s->w.dt, sizeof(double)));
+  HIPCHK(hipMalloc(&s->w.res_scale, sizeof(double) * 4));
+  s->finalized = true;
+  return 0;
+}
+
+int ostc_block_geometry(ostc_solver *s, int blk, double *jac, double *metric) {
+  if (!s) return fail("null solver");
+  if (blk < 0 || blk >= (int)s->blocks.size()) return fail("block index out of range");
+  const Block &b = s->blocks[blk];
+  if (jac) std::copy(b.jac.begin(), b.jac.end(), jac);
+  if (metric) std::copy(b.metric.begin(), b.metric.end(), metric);
+  return 0;
+}
+
+int ostc_field_info(ostc_solver *s, int blk, int field, int64_t *count, int *ghosts, int *ncomp) {
+  if (check_blk(s, blk)) return 1;
+  FieldDesc d = field_desc(s, s->blocks[blk], field);
+  if (d.count < 0) return fail("unknown field id");
+  if (count) *count = d.count;
+  if (ghosts) *ghosts = d.ghosts;
+  if (ncomp) *ncomp = d.ncomp;
+  return 0;
+}
+
+int ostc_upload(ostc_solver *s, int blk, int field, const double *host) {
+  if (check_blk(s, blk)) return 1;
+  FieldDesc d = field_desc(s, s->blocks[blk], field);
+  if (d.count < 0) return fail("unknown field id");
+  if (field == OSTC_JAC || field == OSTC_METRIC) return fail("geometry is derived from the grid");
+  if (d.count == 0) return 0;
+  HIPCHK(hipMemcpyAsync(d.ptr, host, d.count * sizeof(double), hipMemcpyHostToDevice, s->stream));
+  HIPCHK(hipStreamSynchronize(s->stream));
+  return 0;
+}
+
+int ostc_download(ostc_solver *s, int blk, int field, double *host) {
+  if (check_blk(s, blk)) return 1;
+  FieldDesc d = field_desc(s, s->blocks[blk], field);
+  if (d.count < 0) return fail("unknown field id");
+  if (d.count == 0) return 0;
+  HIPCHK(hipMemcpyAsync(host, d.ptr, d.count * sizeof(double), hipMemcpyDeviceToHost, s->stream));
+  HIPCHK(hipStreamSynchronize(s->stream));
+  return 0;
+}
+
+int ostc_field_device_ptr(ostc_solver *s, int blk, int field, double **dptr) {
+  if (check_blk(s, blk)) return 1;
+  FieldDesc d = field_desc(s, s->blocks[blk], field);
+  if (d.count < 0) return fail("unknown field id");
+  *dptr = d.ptr;
+  return 0;
+}
+
+int ostc_inflow_state(ostc_solver *s, const char *name, double *out, int n) {
+  if (!s || !name || !out) return fail("null argument");
+  Inflow in;
+  if (make_inflow(s, name, in)) return 1;
+  const double v[7] = {in.rho, in.u, in.v, in.w, in.p, in.t, in.vel};
+  for (int i = 0; i < n && i < 7; ++i) out[i] = v[i];
+  for (int i = 7; i < n && i - 7 < (int)in.sv.size(); ++i) out[i] = in.sv[i - 7];
+  return 0;
+}
+
+int ostc_initialize_uniform(ostc_solver *s, int blk) {
+  if (check_blk(s, blk)) return 1;
+  Inflow in;
+  if (make_inflow(s, s->prm.get_string("default_init", s->prm.get_string("reference_state", "")), in)) return 1;
+  Block &b = s->blocks[blk];
+  const int64_t sz = b.z.sz;
+  std::vector<double> bv(sz * 6), sv(sz * std::max(s->n_scalar, 0));
+  const double v[6] = {in.rho, in.u, in.v, in.w, in.p, in.t};
+  for (int l = 0; l < 6; ++l) std::fill(bv.begin() + l * sz, bv.begin() + (l + 1) * sz, v[l]);
+  for (int l = 0; l < s->n_scalar; ++l) std::fill(sv.begin() + l * sz, sv.begin() + (l + 1) * sz, in.sv[l]);
+  if (ostc_upload(s, blk, OSTC_BV, bv.data())) return 1;
+  if (s->n_scalar > 0 && ostc_upload(s, blk, OSTC_SV, sv.data())) return 1;
+  return 0;
+}
+
+int ostc_initialize(ostc_solver *s) {
+  if (check_blk(s, 0)) return 1;
+  HIPCHK(hipSetDevice(s->device));
+  for (auto &b : s->blocks) {
+    if (s->P.dim == 2) HIPCHK(launch_eliminate_k(b.z, s->P, false, s->stream));
+    if (run_bcs(s, b, false)) return 1;
+    HIPCHK(launch_velocity(b.z, s->stream));
+    if (properties_block(s, b)) return 1;
+    HIPCHK(launch_store_last(b.z, s->stream));
+    HIPCHK(launch_cv_from_bv(b.z, s->P, s->d_mech, s->stream));
+  }
+  s->t_phys = 0;
+  s->step = 0;
+  HIPCHK(hipStreamSynchronize(s->stream));
+  return 0;
+}
+
+int ostc_rhs(ostc_solver *s, int blk) {
+  if (check_blk(s, blk)) return 1;
+  if (rhs_block(s, s->blocks[blk])) return 1;
+  HIPCHK(hipStreamSynchronize(s->stream));
+  return 0;
+}
+
+int ostc_global_dt(ostc_solver *s, double *dt) {
+  if (check_blk(s, 0)) return 1;
+  double best = 1e+6;
+  for (auto &b : s->blocks) {
+    HIPCHK(launch_local_dt(b.z, s->P, s->w, s->w.dt, s->stream));
+    double v = 0;
+    HIPCHK(hipMemcpyAsync(&v, s->w.dt, sizeof(double), hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    best = std::min(best, v);
+  }
+  if (dt) *dt = best;
+  return 0;
+}
+
+int ostc_rk3_stage(ostc_solver *s, int blk, double dt, int rk) {
+  if (check_blk(s, blk)) return 1;
+  if (rk < 0 || rk > 2) return fail("rk stage must be 0, 1 or 2");
+  Block &b = s->blocks[blk];
+  if (rk == 0)
+    HIPCHK(hipMemcpyAsync(b.z.qn, b.z.cv, b.z.sz * s->n_var * sizeof(double), hipMemcpyDeviceToDevice, s->stream));
+  HIPCHK(hipMemcpyAsync(s->w.dt, &dt, sizeof(double), hipMemcpyHostToDevice, s->stream));
+  HIPCHK(launch_rk_update(b.z, s->P, s->d_mech, s->w.dt, rk, s->stream));
+  HIPCHK(hipStreamSynchronize(s->stream));
+  return 0;
+}
+
+int ostc_apply_bc(ostc_solver *s, int blk, int with_cv) {
+  if (check_blk(s, blk)) return 1;
+  if (run_bcs(s, s->blocks[blk], with_cv != 0)) return 1;
+  HIPCHK(hipStreamSynchronize(s->stream));
+  return 0;
+}
+
+int ostc_update_properties(ostc_solver *s, int blk) {
+  if (check_blk(s, blk)) return 1;
+  Block &b = s->blocks[blk];
+  if (s->P.dim == 2) HIPCHK(launch_eliminate_k(b.z, s->P, true, s->stream));
+  if (properties_block(s, b)) return 1;
+  HIPCHK(hipStreamSynchronize(s->stream));
+  return 0;
+}
+
+int ostc_step_rk3(ostc_solver *s, int nsteps, double *dt_hist, double *res_hist) {
+  if (check_blk(s, 0)) return 1;
+  if (ensure_hist(s, nsteps)) return 1;
+  if (nsteps == 0) return 0;
+  HIPCHK(hipSetDevice(s->device));
+  if (s->blocks.size() != 1) return fail("the fused RK3 driver runs single-block cases");
+  Block &b = s->blocks[0];
+  double *d_dt = nullptr, *d_res = nullptr;
+  HIPCHK(hipMalloc(&d_dt, sizeof(double) * 3 * nsteps));
+  HIPCHK(hipMalloc(&d_res, sizeof(double) * 5 * nsteps));
+  s->w.dt_hist = d_dt;
+  s->w.res_hist = d_res;
+  const bool fixed = s->prm.get_bool("fixed_time_step", false);
+  const double fixed_dt = s->prm.get_real("dt", 1e-5);
+  const int os = std::max(1, s->prm.get_int("output_screen", 1));
+  int rc = 0;
+  auto body = [&]() -> int {
+    if (fixed) HIPCHK(hipMemcpyAsync(s->w.dt, &fixed_dt, sizeof(double), hipMemcpyHostToDevice, s->stream));
+    for (int it = 0; it < nsteps; ++it) {
+      const int step = s->step + 1 + it;
+      if (step % os == 0) HIPCHK(launch_store_last(b.z, s->stream));
+      HIPCHK(hipMemcpyAsync(b.z.qn, b.z.cv, b.z.sz * s->n_var * sizeof(double), hipMemcpyDeviceToDevice,
+                            s->stream));
+      for (int rk = 0; rk < 3; ++rk) {
+        if (rhs_block(s, b)) return 1;
+        if (!fixed) {
+          ostc_solver::Timer t(s, "local_dt");
+          HIPCHK(launch_local_dt(b.z, s->P, s->w, s->w.dt, s->stream));
+        }
+        HIPCHK(launch_copy_dt(s->w.dt, d_dt + it * 3 + rk, s->stream));
+        {
+          ostc_solver::Timer t(s, "rk_update");
+          HIPCHK(launch_rk_update(b.z, s->P, s->d_mech, s->w.dt, rk, s->stream));
+        }
+        if (run_bcs(s, b, true)) return 1;
+        if (s->P.dim == 2) HIPCHK(launch_eliminate_k(b.z, s->P, true, s->stream));
+        if (properties_block(s, b)) return 1;
+      }
+      if (step % os == 0 || step == 1) {
+        ostc_solver::Timer t(s, "residual");
+        HIPCHK(launch_residual(b.z, s->w, it, step == 1 ? 1 : 0, s->stream));
+      }
+    }
+    return 0;
+  };
+  rc = body();
+  std::vector<double> hdt(3 * nsteps), hres(5 * nsteps, NAN);
+  if (!rc) {
+    hipError_t e = hipMemcpyAsync(hdt.data(), d_dt, sizeof(double) * 3 * nsteps, hipMemcpyDeviceToHost, s->stream);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(hres.data(), d_res, sizeof(double) * 5 * nsteps, hipMemcpyDeviceToHost, s->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+    if (e != hipSuccess) rc = fail(std::string("step_rk3: ") + hipGetErrorString(e));
+  }
+  (void)hipFree(d_dt);
+  (void)hipFree(d_res);
+  s->w.dt_hist = nullptr;
+  s->w.res_hist = nullptr;
+  if (rc) return rc;
+  for (int it = 0; it < nsteps; ++it) {
+    const int step = s->step + 1 + it;
+    s->t_phys += hdt[it * 3 + 2];
+    if (!(step % os == 0 || step == 1))
+      for (int l = 0; l < 5; ++l) hres[it * 5 + l] = NAN;
+    else if (std::isnan(hres[it * 5 + 4]))
+      return fail("NaN residual at step " + std::to_string(step));
+  }
+  s->step += nsteps;
+  if (dt_hist) std::copy(hdt.begin(), hdt.end(), dt_hist);
+  if (res_hist) std::copy(hres.begin(), hres.end(), res_hist);
+  s->drain_profile();
+  return 0;
+}
+
+int ostc_time(ostc_solver *s, double *t, int *step) {
+  if (!s) return fail("null solver");
+  if (t) *t = s->t_phys;
+  if (step) *step = s->step;
+  return 0;
+}
+
+int ostc_set_flux_precision(ostc_solver *s, int fp32) {
+  if (!s) return fail("null solver");
+  s->fp32 = fp32 != 0;
+  return 0;
+}
+
+int ostc_chem_batch(ostc_solver *s, int64_t n, const double *rho, const double *T, const double *Y, double dt,
+                    double *dY, double *dT, int mode, int64_t *n_fallback) {
+  if (!s) return fail("null solver");
+  if (s->n_spec != 9 || s->mech.n_reac <= 0) return fail("chemistry batch needs the 9-species reacting mechanism");
+  if (n < 0) return fail("negative cell count");
+  HIPCHK(hipSetDevice(s->device));
+  if (mode == 0) {
+    ostc_solver::Timer t(s, "chem_batch_fp64");
+    HIPCHK(launch_chem_batch(s->d_mech, n, rho, T, Y, dt, dY, dT, s->stream));
+    if (n_fallback) *n_fallback = 0;
+  } else if (mode == 1) {
+    if (!s->sur || !surrogate_ready(s->sur)) return fail("no trained surrogate loaded");
+    int64_t nf = 0;
+    std::string err;
+    if (surrogate_infer(s->sur, s->d_mech, n, rho, T, Y, dt, dY, dT, &nf, s->stream, err)) return fail(err);
+    if (n_fallback) *n_fallback = nf;
+  } else {
+    return fail("mode must be 0 (point-implicit) or 1 (surrogate)");
+  }
+  HIPCHK(hipStreamSynchronize(s->stream));
+  s->drain_profile();
+  return 0;
+}
+
+int ostc_dev_alloc(ostc_solver *s, int64_t bytes, void **dptr) {
+  if (!s || !dptr) return fail("null argument");
+  HIPCHK(hipSetDevice(s->device));
+  HIPCHK(hipMalloc(dptr, std::max<int64_t>(bytes, 8)));
+  s->user_mem.push_back(*dptr);
+  return 0;
+}
+
+int ostc_dev_free(ostc_solver *s, void *dptr) {
+  if (!s) return fail("null solver");
+  auto it = std::find(s->user_mem.begin(), s->user_mem.end(), dptr);
+  if (it == s->user_mem.end()) return fail("pointer not allocated by ostc_dev_alloc");
+  s->user_mem.erase(it);
+  HIPCHK(hipFree(dptr));
+  return 0;
+}
+
+int ostc_memcpy_h2d(ostc_solver *s, void *dst, const void *src, int64_t bytes) {
+  if (!s) return fail("null solver");
+  HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s->stream));
+  HIPCHK(hipStreamSynchronize(s->stream));
+  return 0;
+}
+
+int ostc_memcpy_d2h(ostc_solver *s, void *dst, const void *src, int64_t bytes) {
+  if (!s) return fail("null solver");
+  HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, s->stream));
+  HIPCHK(hipStreamSynchronize(s->stream));
+  return 0;
+}
+
+int ostc_sync(ostc_solver *s) {
+  if (!s) return fail("null solver");
+  HIPCHK(hipStreamSynchronize(s->stream));
+  return 0;
+}
+
+int ostc_profile(ostc_solver *s, int enable) {
+  if (!s) return fail("null solver");
+  s->drain_profile();
+  s->prof = enable != 0;
+  if (enable) s->stats.clear();
+  return 0;
+}
+
+int ostc_profile_report(ostc_solver *s, char *buf, int len) {
+  if (!s || !buf || len <= 0) return fail("null argument");
+  s->drain_profile();
+  std::ostringstream o;
+  for (auto &[name, v] : s->stats) o << name << " " << v.first << " " << v.second << "\n";
+  const std::string r = o.str();
+  std::snprintf(buf, (size_t)len, "%s", r.c_str());
+  return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,238 @@
+// Per-point finite-rate chemistry, mixture transport and the point-implicit
+// solve, FP64, templated on the species count so every per-lane array has a
+// compile-time size and stays in registers.
+#pragma once
+
+#include "device_common.h"
+
+namespace ostc {
+
+__host__ __device__ constexpr double kPatmD() { return 101325.0; }
+
+__device__ inline double d_arrhenius(double t, double A, double b, double Ea) { return A * pow(t, b) * exp(-Ea / t); }
+
+// FiniteRateChem.cu:65-149 for reaction r: kf (Arrhenius + duplicate, third
+// body, Lindemann, Troe) and kb = kf / Kc (or the REV set, 0 if irreversible).
+template <int NS>
+__device__ inline void d_rate(const DMech *__restrict__ m, int r, double t, const double *c, const double *g,
+                              double temp_t, double &kf, double &kb) {
+  const int type = m->label[r];
+  double k = d_arrhenius(t, m->A[r], m->b[r], m->Ea[r]);
+  if (type == 3) k += d_arrhenius(t, m->A2[r], m->b2[r], m->Ea2[r]);
+  if (type > 3) {
+    double cc = 0;
+#pragma unroll
+    for (int l = 0; l < NS; ++l) cc += c[l] * m->third_body_coeff[r * NS + l];
+    if (type == 4) {
+      k *= cc;
+    } else {
+      const double klow = d_arrhenius(t, m->A2[r], m->b2[r], m->Ea2[r]);
+      const double khigh = k;
+      const double pr = klow * cc / khigh;
+      double F = 1.0;
+      if (type > 5) {
+        double fc = (1 - m->troe_alpha[r]) * exp(-t / m->troe_t3[r]) + m->troe_alpha[r] * exp(-t / m->troe_t1[r]);
+        if (type == 7) fc += exp(-m->troe_t2[r] / t);
+        const double lfc = log10(fc);
+        const double c2 = -0.4 - 0.67 * lfc;
+        const double nn = 0.75 - 1.27 * lfc;
+        const double lpr = log10(pr);
+        const double tempo = (lpr + c2) / (nn - 0.14 * (lpr + c2));
+        const double pp = lfc / (1.0 + tempo * tempo);
+        F = pow(10.0, pp);
+      }
+      k = khigh * pr / (1.0 + pr) * F;
+    }
+  }
+  kf = k;
+  if (type == 0) {
+    kb = 0;
+  } else if (type == 2) {
+    kb = d_arrhenius(t, m->A2[r], m->b2[r], m->Ea2[r]);
+  } else {
+    double dg = 0;
+#pragma unroll
+    for (int l = 0; l < NS; ++l) dg += g[l] * (m->stoi_b[r * NS + l] - m->stoi_f[r * NS + l]);
+    const double kc = pow(temp_t, (double)m->order[r]) * exp(-dg);
+    kb = k / kc;
+  }
+}
+
+// FiniteRateChem.cu:7-63 + 151-217: omega [kg/m3/s] and, when jac != null,
+// the exact-point-implicit Jacobian d omega_m / d(rho Y_n).  Rates, progress
+// and source are fused per reaction so no per-reaction array is kept.
+template <int NS>
+__device__ inline void d_chem_source(const DMech *__restrict__ m, double rho, double t, const double *Y, double *omega,
+                                     double *jac) {
+  double c[NS], g[NS];
+#pragma unroll
+  for (int l = 0; l < NS; ++l) c[l] = rho * Y[l] / m->mw[l] * 1e-3;
+  d_gibbs_rt<NS>(t, m, g);
+  const double temp_t = (kPatmD() / kRuD * 1e-3) / t;
+  const int nr = m->n_reac;
+  double creation[NS], destr[NS];
+#pragma unroll
+  for (int l = 0; l < NS; ++l) {
+    creation[l] = 0;
+    destr[l] = 0;
+  }
+  if (jac) {
+#pragma unroll
+    for (int a = 0; a < NS * NS; ++a) jac[a] = 0;
+  }
+  for (int r = 0; r < nr; ++r) {
+    double kf, kb;
+    d_rate<NS>(m, r, t, c, g, temp_t, kf, kb);
+    double q1 = 1.0, q2 = 1.0;
+#pragma unroll
+    for (int l = 0; l < NS; ++l) {
+      q1 *= pow(c[l], (double)m->stoi_f[r * NS + l]);
+      q2 *= pow(c[l], (double)m->stoi_b[r * NS + l]);
+    }
+    q1 *= kf;
+    q2 = m->label[r] != 0 ? q2 * kb : 0.0;
+#pragma unroll
+    for (int l = 0; l < NS; ++l) {
+      const int nf = m->stoi_f[r * NS + l], nb = m->stoi_b[r * NS + l];
+      creation[l] += q2 * nf + q1 * nb;
+      destr[l] += q1 * nf + q2 * nb;
+    }
+    if (jac) {
+#pragma unroll
+      for (int n = 0; n < NS; ++n) {
+        const double dn = m->stoi_f[r * NS + n] * q1 - m->stoi_b[r * NS + n] * q2;
+#pragma unroll
+        for (int a = 0; a < NS; ++a) jac[a * NS + n] += (m->stoi_b[r * NS + a] - m->stoi_f[r * NS + a]) * dn;
+      }
+    }
+  }
+#pragma unroll
+  for (int l = 0; l < NS; ++l) omega[l] = creation[l] * (m->mw[l] * 1e+3) - destr[l] * (m->mw[l] * 1e+3);
+  if (jac) {
+#pragma unroll
+    for (int n = 0; n < NS; ++n) {
+      const bool on = Y[n] > 1e-25;
+#pragma unroll
+      for (int a = 0; a < NS; ++a) jac[a * NS + n] = on ? m->mw[a] * (jac[a * NS + n] / (rho * Y[n])) * 1e+3 : 0.0;
+    }
+  }
+}
+
+// FiniteRateChem.cu:364-418 solve_chem_system: Crout-style LU with row
+// pivoting (first maximum wins), then substitution; rhs overwritten.
+template <int N>
+__device__ inline void d_lu_solve(double *A, double *b) {
+  int piv[N];
+#pragma unroll
+  for (int n = 0; n < N; ++n) {
+    int ik = n;
+#pragma unroll
+    for (int r = n; r < N; ++r) {
+#pragma unroll
+      for (int t = 0; t < n; ++t) A[r * N + n] -= A[r * N + t] * A[t * N + n];
+      if (fabs(A[r * N + n]) > fabs(A[ik * N + n])) ik = r;
+    }
+    piv[n] = ik;
+    if (ik != n) {
+#pragma unroll
+      for (int t = 0; t < N; ++t) {
+        // row swap through selects keeps the indexing static
+#pragma unroll
+        for (int r = n + 1; r < N; ++r) {
+          if (r == ik) {
+            const double tmp = A[r * N + t];
+            A[r * N + t] = A[n * N + t];
+            A[n * N + t] = tmp;
+          }
+        }
+      }
+    }
+#pragma unroll
+    for (int p = n + 1; p < N; ++p)
+#pragma unroll
+      for (int t = 0; t < n; ++t) A[n * N + p] -= A[n * N + t] * A[t * N + p];
+#pragma unroll
+    for (int r = n + 1; r < N; ++r) A[r * N + n] /= A[n * N + n];
+  }
+#pragma unroll
+  for (int r = 0; r < N; ++r) {
+    const int t = piv[r];
+#pragma unroll
+    for (int q = r + 1; q < N; ++q) {
+      if (q == t) {
+        const double tmp = b[q];
+        b[q] = b[r];
+        b[r] = tmp;
+      }
+    }
+  }
+#pragma unroll
+  for (int r = 1; r < N; ++r)
+#pragma unroll
+    for (int t = 0; t < r; ++t) b[r] -= A[r * N + t] * b[t];
+  b[N - 1] /= A[N * N - 1];
+#pragma unroll
+  for (int r = N - 2; r >= 0; --r) {
+#pragma unroll
+    for (int t = r + 1; t < N; ++t) b[r] -= A[r * N + t] * b[t];
+    b[r] /= A[r * N + r];
+  }
+}
+
+// Transport.cu:40-110 compute_transport_property
+template <int NS>
+__device__ inline void d_transport(const DMech *__restrict__ m, double t, double p, double rho, const double *Y,
+                                   const double *cp, double &mu, double &lam, double *rhoD) {
+  double mwi = 0;
+#pragma unroll
+  for (int l = 0; l < NS; ++l) mwi += Y[l] / m->mw[l];
+  const double mw_total = 1 / mwi;
+  double X[NS], vis[NS];
+  const double st = sqrt(t);
+#pragma unroll
+  for (int l = 0; l < NS; ++l) {
+    X[l] = Y[l] * mw_total / m->mw[l];
+    const double tdl = t * m->LJ_potent_inv[l];
+    const double ci = 1.147 * pow(tdl, -0.145) + pow(tdl + 0.5, -2.0);
+    vis[l] = m->vis_coeff[l] * st / ci;
+  }
+  double viscosity = 0, conductivity = 0;
+#pragma unroll
+  for (int a = 0; a < NS; ++a) {
+    double vt = 0;
+#pragma unroll
+    for (int b = 0; b < NS; ++b) {
+      double pf = 1.0;
+      if (a != b) {
+        const double num = 1 + sqrt(vis[a] / vis[b]) * m->WjDivWi_to_One4th[a * NS + b];
+        pf = num * num * m->sqrt_WiDivWjPl1Mul8[a * NS + b];
+      }
+      vt += pf * X[b];
+    }
+    const double ct = 1.065 * vt - 0.065 * X[a];
+    viscosity += vis[a] * X[a] / vt;
+    const double la = vis[a] * (cp[a] + 1.25 * kRuD / m->mw[a]);
+    conductivity += la * X[a] / ct;
+  }
+  mu = viscosity;
+  lam = conductivity;
+  const double t32p = t * st / p;
+#pragma unroll
+  for (int l = 0; l < NS; ++l) {
+    double num = 0, den = 0;
+#pragma unroll
+    for (int n = 0; n < NS; ++n) {
+      if (l != n) {
+        const int a = l < n ? l : n, b = l < n ? n : l;
+        const double tr = t * m->kb_over_eps_jk[a * NS + b];
+        const double od = 1.0 / pow(tr, 0.145) + 1.0 / ((tr + 0.5) * (tr + 0.5));
+        const double dij = m->binary_diffusivity_coeff[a * NS + b] * t32p / od;
+        num += (X[n] + 1e-12) * m->mw[n];
+        den += (X[n] + 1e-12) / dij;
+      }
+    }
+    rhoD[l] = rho * num / (den * mw_total);
+  }
+}
+
+}  // namespace ostc

@@ -1,0 +1,179 @@
+// Device-side data views and per-point physics shared by the gfx950
+// kernels.  All state is FP64 in HBM (reference DZone layout); the flux
+// arithmetic is templated on its working precision.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace ostc {
+
+constexpr int kMS = 20;   // species capacity of the device tables
+constexpr int kMR = 50;   // reaction capacity of the device tables
+
+// DParameter.cuh:13-92 content, fixed capacity, row-major packed with the
+// live n_spec / n_reac.  Lives in device memory; kernels read it with
+// wave-uniform addresses.
+struct DMech {
+  int n_spec, n_reac;
+  double mw[kMS], mw_inv[kMS], t_low[kMS], t_mid[kMS], hi[kMS * 7], lo[kMS * 7];
+  double LJ_potent_inv[kMS], vis_coeff[kMS];
+  double WjDivWi_to_One4th[kMS * kMS], sqrt_WiDivWjPl1Mul8[kMS * kMS];
+  double binary_diffusivity_coeff[kMS * kMS], kb_over_eps_jk[kMS * kMS];
+  int label[kMR], stoi_f[kMR * kMS], stoi_b[kMR * kMS], order[kMR];
+  double A[kMR], b[kMR], Ea[kMR], A2[kMR], b2[kMR], Ea2[kMR];
+  double third_body_coeff[kMR * kMS];
+  double troe_alpha[kMR], troe_t3[kMR], troe_t1[kMR], troe_t2[kMR];
+};
+
+// Scheme switches and constants (the scalar part of DParameter).
+struct DParams {
+  int mx, my, mz, g, dim;
+  int nv, ns, nsc;
+  int mix, reaction, chemSrcMethod;
+  int inviscid_scheme, reconstruction, limiter, viscous, inviscid_type, gradP;
+  double cfl, Pr, Sc, Prt, Sct;
+};
+
+// Raw device pointers of one block (DZone, Field.h:13-71).  bv/sv/cv/qn and
+// the property arrays use the ngg-ghosted layout; dq/dt_local/bv_last use
+// the ghost-free layout.  metric is 9 component-major arrays
+// (index (r*3+c)*sz + p  =  d xi_r / d x_c).
+struct ZoneView {
+  int mx, my, mz, g;
+  int64_t d2, d1, sz;      // ghosted
+  int64_t e2, e1, sz0;     // ghost-free
+  double *bv, *sv, *cv, *qn;
+  double *vel, *acoustic, *mach, *mul, *cond, *rhoD, *gam, *cp;
+  const double *jac, *metric;
+  double *dq, *dt_local, *bv_last, *chem_jac;
+
+  __host__ __device__ int64_t at(int i, int j, int k) const {
+    return (int64_t)(k + g) * d1 + (int64_t)(j + g) * d2 + (i + g);
+  }
+  __host__ __device__ int64_t at0(int i, int j, int k) const { return (int64_t)k * e1 + (int64_t)j * e2 + i; }
+};
+
+constexpr double kRuD = 8314.462618;
+constexpr double kGam = 1.4;
+constexpr double kMwAirD = 28.9647;
+
+// ---------------------------------------------------------------- thermo
+// Thermo.cu:5-79.  Below t_low: h extrapolated with cp(t_low), cp and g/RT
+// frozen at t_low.
+
+template <int NS>
+__device__ inline void d_enthalpy(double t, const DMech *__restrict__ m, double *h) {
+  const double t2 = t * t, t3 = t2 * t, t4 = t3 * t, t5 = t4 * t;
+#pragma unroll
+  for (int i = 0; i < NS; ++i) {
+    if (t < m->t_low[i]) {
+      const double tt = m->t_low[i];
+      const double tt2 = tt * tt, tt3 = tt2 * tt, tt4 = tt3 * tt, tt5 = tt4 * tt;
+      const double *c = &m->lo[i * 7];
+      double hh = c[0] * tt + 0.5 * c[1] * tt2 + c[2] * tt3 / 3 + 0.25 * c[3] * tt4 + 0.2 * c[4] * tt5 + c[5];
+      const double cp = c[0] + c[1] * tt + c[2] * tt2 + c[3] * tt3 + c[4] * tt4;
+      h[i] = hh + cp * (t - tt);
+    } else {
+      const double *c = t < m->t_mid[i] ? &m->lo[i * 7] : &m->hi[i * 7];
+      h[i] = c[0] * t + 0.5 * c[1] * t2 + c[2] * t3 / 3 + 0.25 * c[3] * t4 + 0.2 * c[4] * t5 + c[5];
+    }
+    h[i] *= kRuD / m->mw[i];
+  }
+}
+
+template <int NS>
+__device__ inline void d_enthalpy_cp(double t, const DMech *__restrict__ m, double *h, double *cp) {
+  const double t2 = t * t, t3 = t2 * t, t4 = t3 * t, t5 = t4 * t;
+#pragma unroll
+  for (int i = 0; i < NS; ++i) {
+    if (t < m->t_low[i]) {
+      const double tt = m->t_low[i];
+      const double tt2 = tt * tt, tt3 = tt2 * tt, tt4 = tt3 * tt, tt5 = tt4 * tt;
+      const double *c = &m->lo[i * 7];
+      const double hh = c[0] * tt + 0.5 * c[1] * tt2 + c[2] * tt3 / 3 + 0.25 * c[3] * tt4 + 0.2 * c[4] * tt5 + c[5];
+      cp[i] = c[0] + c[1] * tt + c[2] * tt2 + c[3] * tt3 + c[4] * tt4;
+      h[i] = hh + cp[i] * (t - tt);
+    } else {
+      const double *c = t < m->t_mid[i] ? &m->lo[i * 7] : &m->hi[i * 7];
+      h[i] = c[0] * t + 0.5 * c[1] * t2 + c[2] * t3 / 3 + 0.25 * c[3] * t4 + 0.2 * c[4] * t5 + c[5];
+      cp[i] = c[0] + c[1] * t + c[2] * t2 + c[3] * t3 + c[4] * t4;
+    }
+    h[i] *= kRuD / m->mw[i];
+    cp[i] *= kRuD / m->mw[i];
+  }
+}
+
+template <int NS>
+__device__ inline void d_cp(double t, const DMech *__restrict__ m, double *cp) {
+  const double t2 = t * t, t3 = t2 * t, t4 = t3 * t;
+#pragma unroll
+  for (int i = 0; i < NS; ++i) {
+    const double tt = t < m->t_low[i] ? m->t_low[i] : t;
+    const double *c = (t < m->t_low[i] || t < m->t_mid[i]) ? &m->lo[i * 7] : &m->hi[i * 7];
+    if (t < m->t_low[i]) {
+      const double tt2 = tt * tt, tt3 = tt2 * tt, tt4 = tt3 * tt;
+      cp[i] = c[0] + c[1] * tt + c[2] * tt2 + c[3] * tt3 + c[4] * tt4;
+    } else {
+      cp[i] = c[0] + c[1] * t + c[2] * t2 + c[3] * t3 + c[4] * t4;
+    }
+    cp[i] *= kRuD / m->mw[i];
+  }
+}
+
+template <int NS>
+__device__ inline void d_gibbs_rt(double t, const DMech *__restrict__ m, double *g) {
+  const double t2 = t * t, t3 = t2 * t, t4 = t3 * t, t_inv = 1 / t, log_t = log(t);
+#pragma unroll
+  for (int i = 0; i < NS; ++i) {
+    if (t < m->t_low[i]) {
+      const double tt = m->t_low[i];
+      const double tt2 = tt * tt, tt3 = tt2 * tt, tt4 = tt3 * tt, tt_inv = 1 / tt, log_tt = log(tt);
+      const double *c = &m->lo[i * 7];
+      g[i] = c[0] * (1.0 - log_tt) - 0.5 * c[1] * tt - c[2] * tt2 / 6.0 - c[3] * tt3 / 12.0 - c[4] * tt4 * 0.05 +
+             c[5] * tt_inv - c[6];
+    } else {
+      const double *c = t < m->t_mid[i] ? &m->lo[i * 7] : &m->hi[i * 7];
+      g[i] = c[0] * (1.0 - log_t) - 0.5 * c[1] * t - c[2] * t2 / 6.0 - c[3] * t3 / 12.0 - c[4] * t4 * 0.05 +
+             c[5] * t_inv - c[6];
+    }
+  }
+}
+
+// Transport.cu:6-9
+__device__ inline double d_sutherland(double t) { return 1.716e-5 * pow(t / 273, 1.5) * (273 + 111) / (t + 111); }
+
+// FieldOperation.cu:3-39: Newton on e(T) from the warm start; tolerance
+// |1 - T1/T| <= 1e-3, at most 1000 iterations.
+template <int NS>
+__device__ inline double d_temperature(const DMech *__restrict__ m, double rho, double rhoE, double u, double v,
+                                       double w, const double *Y, double t, double &p) {
+  double mw = 0;
+#pragma unroll
+  for (int l = 0; l < NS; ++l) mw += Y[l] / m->mw[l];
+  mw = 1 / mw;
+  const double gas = kRuD / mw;
+  const double e = rhoE / rho - 0.5 * (u * u + v * v + w * w);
+  double err = 1;
+  int iter = 0;
+  while (err > 1e-3 && iter++ < 1000) {
+    double h[NS > 0 ? NS : 1], cp[NS > 0 ? NS : 1];
+    d_enthalpy_cp<NS>(t, m, h, cp);
+    double cpt = 0, ht = 0;
+#pragma unroll
+    for (int l = 0; l < NS; ++l) {
+      cpt += cp[l] * Y[l];
+      ht += h[l] * Y[l];
+    }
+    const double et = ht - gas * t;
+    const double cv = cpt - gas;
+    const double t1 = t - (et - e) / cv;
+    err = fabs(1 - t1 / t);
+    t = t1;
+  }
+  p = rho * t * gas;
+  return t;
+}
+
+}  // namespace ostc

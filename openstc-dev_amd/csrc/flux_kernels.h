@@ -1,0 +1,50 @@
+// Launch wrappers of the device kernels (host-callable, stream-ordered).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "device_common.h"
+
+namespace ostc {
+
+struct BCDesc {
+  int type, face, dir;
+  int rs[3], re[3];
+  double rho, u, v, w, p, t, vel;
+  double sv[kMS + 4];
+  double t_wall;
+  int isothermal;
+};
+
+// per-step scratch kept on the device so a run needs no host round trip
+struct StepScratch {
+  double *partial;         // per-block partials (>= 4 * max_blocks)
+  unsigned int *counter;   // last-block ticket
+  double *dt;              // current stage dt
+  double *dt_hist;         // 3 per step
+  double *res_hist;        // 5 per step (4 residuals + err_max)
+  double *res_scale;       // 4
+  int max_blocks;
+};
+
+hipError_t launch_flux_sweeps(const ZoneView &z, const DParams &P, const DMech *m, bool init, bool fp32,
+                              hipStream_t s);
+hipError_t launch_source(const ZoneView &z, const DParams &P, const DMech *m, hipStream_t s);
+hipError_t launch_local_dt(const ZoneView &z, const DParams &P, const StepScratch &w, double *dt_out,
+                           hipStream_t s);
+hipError_t launch_rk_update(const ZoneView &z, const DParams &P, const DMech *m, const double *dt, int rk,
+                            hipStream_t s);
+hipError_t launch_bc(const ZoneView &z, const DParams &P, const DMech *m, const BCDesc &bc, bool with_cv,
+                     hipStream_t s);
+hipError_t launch_eliminate_k(const ZoneView &z, const DParams &P, bool with_cv, hipStream_t s);
+hipError_t launch_properties(const ZoneView &z, const DParams &P, const DMech *m, hipStream_t s);
+hipError_t launch_velocity(const ZoneView &z, hipStream_t s);
+hipError_t launch_cv_from_bv(const ZoneView &z, const DParams &P, const DMech *m, hipStream_t s);
+hipError_t launch_store_last(const ZoneView &z, hipStream_t s);
+// residual of the current step into res_hist[slot*5 ..]; first != 0 sets the scale
+hipError_t launch_residual(const ZoneView &z, const StepScratch &w, int slot, int first, hipStream_t s);
+hipError_t launch_copy_dt(const double *src, double *dst, hipStream_t s);
+hipError_t launch_chem_batch(const DMech *m, int64_t n, const double *rho, const double *T, const double *Y,
+                             double dt, double *dY, double *dT, hipStream_t s);
+
+}  // namespace ostc

@@ -1,0 +1,168 @@
+// Host-side grid set-up for one structured block: ghost nodes by point
+// reflection with parallelogram edges/corners (Mesh.cpp:790-944; 2-D
+// k-ghosts at z0+k, Mesh.cpp:848) and the Jacobian / metrics by second-order
+// central differences (Mesh.cpp:259-521).  One-time work; the result is
+// uploaded as component-major arrays for coalesced device reads.
+#include <cstring>
+
+#include "ostc_internal.h"
+
+namespace ostc {
+
+namespace {
+
+void ghost_nodes(int mx, int my, int mz, int ngg, const Layout &L, double *A, bool is_z) {
+  auto a = [&](int i, int j, int k) -> double & { return A[L.at(i, j, k)]; };
+  const int G = ngg + 1;
+  for (int k = 0; k < mz; ++k)
+    for (int j = 0; j < my; ++j) {
+      for (int i = 1; i <= G; ++i) a(-i, j, k) = 2 * a(0, j, k) - a(i, j, k);
+      for (int i = mx; i < mx + G; ++i) a(i, j, k) = 2 * a(mx - 1, j, k) - a(2 * (mx - 1) - i, j, k);
+    }
+  for (int k = 0; k < mz; ++k)
+    for (int i = 0; i < mx; ++i) {
+      for (int j = 1; j <= G; ++j) a(i, -j, k) = 2 * a(i, 0, k) - a(i, j, k);
+      for (int j = my; j < my + G; ++j) a(i, j, k) = 2 * a(i, my - 1, k) - a(i, 2 * (my - 1) - j, k);
+    }
+  if (mz > 1) {
+    for (int j = 0; j < my; ++j)
+      for (int i = 0; i < mx; ++i) {
+        for (int k = 1; k <= G; ++k) a(i, j, -k) = 2 * a(i, j, 0) - a(i, j, k);
+        for (int k = mz; k < mz + G; ++k) a(i, j, k) = 2 * a(i, j, mz - 1) - a(i, j, 2 * (mz - 1) - k);
+      }
+  } else {
+    for (int j = 0; j < my; ++j)
+      for (int i = 0; i < mx; ++i) {
+        for (int k = 1; k <= G; ++k) a(i, j, -k) = a(i, j, 0) - (is_z ? k : 0);
+        for (int k = mz; k < mz + G; ++k) a(i, j, k) = a(i, j, mz - 1) + (is_z ? k - mz + 1 : 0);
+      }
+  }
+  // edges and corners: x-y edges over the interior k range
+  for (int i = 1; i <= G; ++i)
+    for (int j = 1; j <= G; ++j)
+      for (int k = 0; k < mz; ++k) {
+        const int i1 = -i, j1 = -j, i2 = mx - 1 + i, j2 = my - 1 + j;
+        a(i1, j1, k) = a(i1, 0, k) + a(0, j1, k) - a(0, 0, k);
+        a(i2, j2, k) = a(i2, my - 1, k) + a(mx - 1, j2, k) - a(mx - 1, my - 1, k);
+        a(i1, j2, k) = a(i1, my - 1, k) + a(0, j2, k) - a(0, my - 1, k);
+        a(i2, j1, k) = a(i2, 0, k) + a(mx - 1, j1, k) - a(mx - 1, 0, k);
+      }
+  // x-z edges over the interior j range
+  for (int i = 1; i <= G; ++i)
+    for (int j = 0; j < my; ++j)
+      for (int k = 1; k <= G; ++k) {
+        const int i1 = -i, k1 = -k, i2 = mx - 1 + i, k2 = mz - 1 + k;
+        a(i1, j, k1) = a(i1, j, 0) + a(0, j, k1) - a(0, j, 0);
+        a(i2, j, k2) = a(i2, j, mz - 1) + a(mx - 1, j, k2) - a(mx - 1, j, mz - 1);
+        a(i1, j, k2) = a(i1, j, mz - 1) + a(0, j, k2) - a(0, j, mz - 1);
+        a(i2, j, k1) = a(i2, j, 0) + a(mx - 1, j, k1) - a(mx - 1, j, 0);
+      }
+  // y-z edges and corners over the full i range
+  for (int i = -G; i < mx + G; ++i)
+    for (int j = 1; j <= G; ++j)
+      for (int k = 1; k <= G; ++k) {
+        const int j1 = -j, k1 = -k, j2 = my - 1 + j, k2 = mz - 1 + k;
+        a(i, j1, k1) = a(i, j1, 0) + a(i, 0, k1) - a(i, 0, 0);
+        a(i, j2, k2) = a(i, j2, mz - 1) + a(i, my - 1, k2) - a(i, my - 1, mz - 1);
+        a(i, j1, k2) = a(i, j1, mz - 1) + a(i, 0, k2) - a(i, 0, mz - 1);
+        a(i, j2, k1) = a(i, j2, 0) + a(i, my - 1, k1) - a(i, my - 1, 0);
+      }
+}
+
+struct Metric9 {
+  double J;
+  double m[9];
+};
+
+bool point_metric(const Layout &Lc, const double *X, const double *Y, const double *Z, int i, int j, int k,
+                  Metric9 &out) {
+  auto d = [&](const double *A, int di, int dj, int dk) {
+    return (A[Lc.at(i + di, j + dj, k + dk)] - A[Lc.at(i - di, j - dj, k - dk)]) * 0.5;
+  };
+  const double x1 = d(X, 1, 0, 0), y1 = d(Y, 1, 0, 0), z1 = d(Z, 1, 0, 0);
+  const double x2 = d(X, 0, 1, 0), y2 = d(Y, 0, 1, 0), z2 = d(Z, 0, 1, 0);
+  const double x3 = d(X, 0, 0, 1), y3 = d(Y, 0, 0, 1), z3 = d(Z, 0, 0, 1);
+  const double J = x1 * y2 * z3 + x2 * y3 * z1 + x3 * y1 * z2 - x1 * y3 * z2 - x2 * y1 * z3 - x3 * y2 * z1;
+  if (J <= 0) return false;
+  out.J = J;
+  out.m[0] = (y2 * z3 - y3 * z2) / J;
+  out.m[1] = (x3 * z2 - x2 * z3) / J;
+  out.m[2] = (x2 * y3 - x3 * y2) / J;
+  out.m[3] = (y3 * z1 - y1 * z3) / J;
+  out.m[4] = (x1 * z3 - x3 * z1) / J;
+  out.m[5] = (x3 * y1 - x1 * y3) / J;
+  out.m[6] = (y1 * z2 - y2 * z1) / J;
+  out.m[7] = (x2 * z1 - x1 * z2) / J;
+  out.m[8] = (x1 * y2 - x2 * y1) / J;
+  return true;
+}
+
+}  // namespace
+
+bool compute_metrics(int mx, int my, int mz, int ngg, const double *x, const double *y, const double *z,
+                     std::vector<double> &jac, std::vector<double> &metric, std::string &err) {
+  const Layout Lc(mx, my, mz, ngg + 1), L(mx, my, mz, ngg);
+  std::vector<double> X(Lc.sz, 0), Y(Lc.sz, 0), Z(Lc.sz, 0);
+  for (int k = 0; k < mz; ++k)
+    for (int j = 0; j < my; ++j)
+      for (int i = 0; i < mx; ++i) {
+        const size_t s = ((size_t)k * my + j) * mx + i;
+        X[Lc.at(i, j, k)] = x[s];
+        Y[Lc.at(i, j, k)] = y[s];
+        Z[Lc.at(i, j, k)] = z[s];
+      }
+  ghost_nodes(mx, my, mz, ngg, Lc, X.data(), false);
+  ghost_nodes(mx, my, mz, ngg, Lc, Y.data(), false);
+  ghost_nodes(mx, my, mz, ngg, Lc, Z.data(), true);
+  // interim AoS store, repacked component-major at the end
+  std::vector<Metric9> M(L.sz);
+  auto put = [&](int i, int j, int k, int di, int dj, int dk) {
+    Metric9 v;
+    if (point_metric(Lc, X.data(), Y.data(), Z.data(), i, j, k, v))
+      M[L.at(i, j, k)] = v;
+    else
+      M[L.at(i, j, k)] = M[L.at(i + di, j + dj, k + dk)];
+  };
+  // interior (a non-positive Jacobian here is an error, Mesh.cpp:282-288)
+  for (int i = 1; i < mx - 1; ++i)
+    for (int j = 1; j < my - 1; ++j) {
+      const int k0 = mz == 1 ? 0 : 1, k1 = mz == 1 ? 1 : mz - 1;
+      for (int k = k0; k < k1; ++k) {
+        Metric9 v;
+        if (!point_metric(Lc, X.data(), Y.data(), Z.data(), i, j, k, v)) {
+          err = "negative Jacobian at interior point (" + std::to_string(i) + "," + std::to_string(j) + "," +
+                std::to_string(k) + ")";
+          return false;
+        }
+        M[L.at(i, j, k)] = v;
+      }
+    }
+  // the six ghost slabs in the reference's order, each falling back to its inner neighbour
+  for (int i = 1; i < mx - 1; ++i)
+    for (int j = 1; j < my - 1; ++j)
+      for (int k = 0; k >= -ngg; --k) put(i, j, k, 0, 0, 1);
+  for (int i = 1; i < mx - 1; ++i)
+    for (int j = 1; j < my - 1; ++j)
+      for (int k = mz - 1; k < mz + ngg; ++k) put(i, j, k, 0, 0, -1);
+  for (int i = 1; i < mx - 1; ++i)
+    for (int j = 0; j >= -ngg; --j)
+      for (int k = -ngg; k < mz + ngg; ++k) put(i, j, k, 0, 1, 0);
+  for (int i = 1; i < mx - 1; ++i)
+    for (int j = my - 1; j < my + ngg; ++j)
+      for (int k = -ngg; k < mz + ngg; ++k) put(i, j, k, 0, -1, 0);
+  for (int i = 0; i >= -ngg; --i)
+    for (int j = -ngg; j < my + ngg; ++j)
+      for (int k = -ngg; k < mz + ngg; ++k) put(i, j, k, 1, 0, 0);
+  for (int i = mx - 1; i < mx + ngg; ++i)
+    for (int j = -ngg; j < my + ngg; ++j)
+      for (int k = -ngg; k < mz + ngg; ++k) put(i, j, k, -1, 0, 0);
+  jac.assign(L.sz, 0);
+  metric.assign((size_t)L.sz * 9, 0);
+  for (int64_t p = 0; p < L.sz; ++p) {
+    jac[p] = M[p].J;
+    for (int c = 0; c < 9; ++c) metric[(size_t)c * L.sz + p] = M[p].m[c];
+  }
+  return true;
+}
+
+}  // namespace ostc

@@ -1,0 +1,677 @@
+// Point-wise and reduction kernels of the RK3 stage: chemical source,
+// CFL time step with an on-device global minimum, the SSP-RK3 update with
+// the FP64 Newton temperature inversion, boundary conditions, 2-D k-copies,
+// physical properties, the residual and the chemistry-batch integrator.
+#include "device_chem.h"
+#include "device_common.h"
+#include "flux_kernels.h"
+
+namespace ostc {
+
+namespace {
+
+constexpr int kTB = 256;
+
+__device__ inline void cell_of(const ZoneView &z, int64_t t, int &i, int &j, int &k) {
+  i = (int)(t % z.mx);
+  const int64_t r = t / z.mx;
+  j = (int)(r % z.my);
+  k = (int)(r / z.my);
+}
+
+__device__ inline double wave_min(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o, 64));
+  return v;
+}
+
+__device__ inline double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// ------------------------------------------------------------- source
+// SourceTerm.cuh:11-37 (finite-rate part): dq = 0 for the flow components,
+// dq(5+l) = J omega_l, and the chemistry Jacobian (EPI) or its diagonal (DA).
+template <int NS>
+__global__ void __launch_bounds__(kTB) k_source(ZoneView z, DParams P, const DMech *__restrict__ m) {
+  const int64_t t = (int64_t)blockIdx.x * kTB + threadIdx.x;
+  const int64_t n = (int64_t)z.mx * z.my * z.mz;
+  if (t >= n) return;
+  int i, j, k;
+  cell_of(z, t, i, j, k);
+  const int64_t p = z.at(i, j, k), q = z.at0(i, j, k);
+  double Y[NS], om[NS];
+#pragma unroll
+  for (int l = 0; l < NS; ++l) Y[l] = z.sv[(int64_t)l * z.sz + p];
+  const double rho = z.bv[p], T = z.bv[5 * z.sz + p], J = z.jac[p];
+  if (P.chemSrcMethod == 1) {
+    double jac[NS * NS];
+    d_chem_source<NS>(m, rho, T, Y, om, jac);
+#pragma unroll
+    for (int c = 0; c < NS * NS; ++c) z.chem_jac[(int64_t)c * z.sz0 + q] = jac[c];
+  } else {
+    d_chem_source<NS>(m, rho, T, Y, om, nullptr);
+  }
+#pragma unroll
+  for (int l = 0; l < 5; ++l) z.dq[(int64_t)l * z.sz0 + q] = 0.0;
+#pragma unroll
+  for (int l = 0; l < NS; ++l) z.dq[(int64_t)(5 + l) * z.sz0 + q] = J * om[l];
+}
+
+// ------------------------------------------------------------- dt
+// TimeAdvanceFunc.cuh:50-100 local_time_step, then the global minimum
+// (TimeAdvanceFunc.cu:155-201) by a last-block reduction: no host round trip.
+template <int MIX>
+__global__ void __launch_bounds__(kTB) k_local_dt(ZoneView z, DParams P, double *partial, unsigned *counter,
+                                                   double *dt_out) {
+  const int64_t t = (int64_t)blockIdx.x * kTB + threadIdx.x;
+  const int64_t n = (int64_t)z.mx * z.my * z.mz;
+  double dtl = 1e+6;
+  if (t < n) {
+    int i, j, k;
+    cell_of(z, t, i, j, k);
+    const int64_t p = z.at(i, j, k);
+    double mm[9];
+#pragma unroll
+    for (int c = 0; c < 9; ++c) mm[c] = z.metric[c * z.sz + p];
+    const double gxi = sqrt(mm[0] * mm[0] + mm[1] * mm[1] + mm[2] * mm[2]);
+    const double geta = sqrt(mm[3] * mm[3] + mm[4] * mm[4] + mm[5] * mm[5]);
+    const double gzeta = sqrt(mm[6] * mm[6] + mm[7] * mm[7] + mm[8] * mm[8]);
+    const double u = z.bv[z.sz + p], v = z.bv[2 * z.sz + p], w = z.bv[3 * z.sz + p];
+    const double U = u * mm[0] + v * mm[1] + w * mm[2];
+    const double V = u * mm[3] + v * mm[4] + w * mm[5];
+    const double W = u * mm[6] + v * mm[7] + w * mm[8];
+    const double a = z.acoustic[p];
+    const double r0 = fabs(U) + a * gxi, r1 = fabs(V) + a * geta;
+    const double r2 = P.dim == 3 ? fabs(W) + a * gzeta : 0.0;
+    const double sri = r0 + r1 + r2;
+    const double gamma = MIX ? z.gam[p] : kGam;
+    const double c1 = fmax(gamma, 4.0 / 3.0) / z.bv[p];
+    const double c2 = z.mul[p] / P.Pr;
+    double srv = gxi * gxi + geta * geta;
+    if (P.dim == 3) srv += gzeta * gzeta;
+    srv *= c1 * c2;
+    dtl = P.cfl / (sri + srv);
+    z.dt_local[z.at0(i, j, k)] = dtl;
+  }
+  __shared__ double sm[kTB / 64];
+  __shared__ bool last;
+  double v = wave_min(dtl);
+  if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double b = sm[0];
+#pragma unroll
+    for (int w = 1; w < kTB / 64; ++w) b = fmin(b, sm[w]);
+    partial[blockIdx.x] = b;
+    __threadfence();
+    const unsigned ticket = atomicAdd(counter, 1u);
+    last = ticket == gridDim.x - 1;
+  }
+  __syncthreads();
+  if (!last) return;
+  __threadfence();
+  double r = 1e+6;
+  for (unsigned b = threadIdx.x; b < gridDim.x; b += kTB) r = fmin(r, __hip_atomic_load(&partial[b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+  r = wave_min(r);
+  if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = r;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double b = sm[0];
+#pragma unroll
+    for (int w = 1; w < kTB / 64; ++w) b = fmin(b, sm[w]);
+    *dt_out = b;
+    *counter = 0u;
+  }
+}
+
+// ------------------------------------------------------------- RK update
+// RK.cuh:10-14 (SSP-RK3 a, b, c) and RK.cuh:197-253 update_cv_and_bv_rk.
+template <int NS>
+__global__ void __launch_bounds__(kTB) k_rk_update(ZoneView z, DParams P, const DMech *__restrict__ m,
+                                                    const double *__restrict__ dtp, int rk) {
+  const int64_t t = (int64_t)blockIdx.x * kTB + threadIdx.x;
+  const int64_t n = (int64_t)z.mx * z.my * z.mz;
+  if (t >= n) return;
+  int i, j, k;
+  cell_of(z, t, i, j, k);
+  const double A = rk == 0 ? 1.0 : (rk == 1 ? 0.75 : 1.0 / 3.0);
+  const double B = rk == 0 ? 0.0 : (rk == 1 ? 0.25 : 2.0 / 3.0);
+  const double C = rk == 0 ? 1.0 : (rk == 1 ? 0.25 : 2.0 / 3.0);
+  const int64_t p = z.at(i, j, k), q = z.at0(i, j, k);
+  const double dtj = *dtp / z.jac[p];
+  constexpr int NV = 5 + NS;
+  double cv[NV];
+#pragma unroll
+  for (int l = 0; l < NV; ++l) {
+    const int64_t o = (int64_t)l * z.sz + p;
+    cv[l] = A * z.qn[o] + B * z.cv[o] + C * dtj * z.dq[(int64_t)l * z.sz0 + q];
+  }
+  if (z.mz == 1) cv[3] = 0;
+#pragma unroll
+  for (int l = 0; l < NV; ++l) z.cv[(int64_t)l * z.sz + p] = cv[l];
+  const double ri = 1.0 / cv[0];
+  const double u = cv[1] * ri, v = cv[2] * ri, w = cv[3] * ri;
+  z.bv[p] = cv[0];
+  z.bv[z.sz + p] = u;
+  z.bv[2 * z.sz + p] = v;
+  z.bv[3 * z.sz + p] = w;
+  const double V2 = u * u + v * v + w * w;
+  if constexpr (NS > 0) {
+    double Y[NS];
+#pragma unroll
+    for (int l = 0; l < NS; ++l) {
+      Y[l] = cv[5 + l] * ri;
+      z.sv[(int64_t)l * z.sz + p] = Y[l];
+    }
+    double pr;
+    const double T = d_temperature<NS>(m, cv[0], cv[4], u, v, w, Y, z.bv[5 * z.sz + p], pr);
+    z.bv[5 * z.sz + p] = T;
+    z.bv[4 * z.sz + p] = pr;
+  } else {
+    const double pr = (kGam - 1) * (cv[4] - 0.5 * cv[0] * V2);
+    z.bv[4 * z.sz + p] = pr;
+    z.bv[5 * z.sz + p] = pr * kMwAirD * ri / kRuD;
+  }
+  z.vel[p] = sqrt(V2);
+}
+
+// FieldOperation.cuh:18-110 compute_total_energy / compute_cv_from_bv_1_point
+template <int NS>
+__device__ inline void cv_point(const ZoneView &z, const DMech *__restrict__ m, int64_t p) {
+  const double rho = z.bv[p], u = z.bv[z.sz + p], v = z.bv[2 * z.sz + p], w = z.bv[3 * z.sz + p];
+  z.cv[p] = rho;
+  z.cv[z.sz + p] = rho * u;
+  z.cv[2 * z.sz + p] = rho * v;
+  z.cv[3 * z.sz + p] = rho * w;
+  const double V2 = u * u + v * v + w * w;
+  double te = 0.5 * V2;
+  if constexpr (NS > 0) {
+    double h[NS];
+    d_enthalpy<NS>(z.bv[5 * z.sz + p], m, h);
+#pragma unroll
+    for (int l = 0; l < NS; ++l) {
+      const double y = z.sv[(int64_t)l * z.sz + p];
+      z.cv[(int64_t)(5 + l) * z.sz + p] = rho * y;
+      te += h[l] * y;
+    }
+    te *= rho;
+    te -= z.bv[4 * z.sz + p];
+  } else {
+    te *= rho;
+    te += z.bv[4 * z.sz + p] / (kGam - 1);
+  }
+  z.cv[4 * z.sz + p] = te;
+  z.vel[p] = sqrt(V2);
+}
+
+template <int NS>
+__global__ void __launch_bounds__(kTB) k_cv_from_bv(ZoneView z, const DMech *__restrict__ m) {
+  const int X = z.mx + 2 * z.g, Y = z.my + 2 * z.g, Z = z.mz + 2 * z.g;
+  const int64_t t = (int64_t)blockIdx.x * kTB + threadIdx.x;
+  if (t >= (int64_t)X * Y * Z) return;
+  cv_point<NS>(z, m, t);
+}
+
+__global__ void __launch_bounds__(kTB) k_velocity(ZoneView z) {
+  const int X = z.mx + 2 * z.g, Y = z.my + 2 * z.g, Z = z.mz + 2 * z.g;
+  const int64_t p = (int64_t)blockIdx.x * kTB + threadIdx.x;
+  if (p >= (int64_t)X * Y * Z) return;
+  const double u = z.bv[z.sz + p], v = z.bv[2 * z.sz + p], w = z.bv[3 * z.sz + p];
+  z.vel[p] = sqrt(u * u + v * v + w * w);
+}
+
+// ------------------------------------------------------------- BCs
+// BoundCond.cuh:52-552: one thread per face point of a registered range,
+// writing the face point (where the reference does) and its ngg ghosts.
+template <int NS, bool WITH_CV>
+__global__ void __launch_bounds__(kTB) k_bc(ZoneView z, DParams P, const DMech *__restrict__ m, BCDesc b) {
+  const int nx = b.re[0] - b.rs[0] + 1, ny = b.re[1] - b.rs[1] + 1, nz = b.re[2] - b.rs[2] + 1;
+  const int64_t t = (int64_t)blockIdx.x * kTB + threadIdx.x;
+  if (t >= (int64_t)nx * ny * nz) return;
+  const int i = b.rs[0] + (int)(t % nx), j = b.rs[1] + (int)((t / nx) % ny), k = b.rs[2] + (int)(t / ((int64_t)nx * ny));
+  int d[3] = {0, 0, 0};
+  d[b.face] = b.dir;
+  const int64_t so[3] = {1, z.d2, z.d1};
+  const int64_t sd = d[0] * so[0] + d[1] * so[1] + d[2] * so[2];  // outward step
+  const int64_t p = z.at(i, j, k);
+  double *bv = z.bv, *sv = z.sv;
+  const int64_t S = z.sz;
+  if (b.type == 3) {
+    // symmetry (BoundCond.cuh:52-118); never refreshes ghost cv
+    const int64_t pi = p - sd;
+    double kx = z.metric[(b.face * 3 + 0) * S + p], ky = z.metric[(b.face * 3 + 1) * S + p],
+           kz = z.metric[(b.face * 3 + 2) * S + p];
+    const double km = sqrt(kx * kx + ky * ky + kz * kz);
+    kx /= km;
+    ky /= km;
+    kz /= km;
+    const double u1 = bv[S + pi], v1 = bv[2 * S + pi], w1 = bv[3 * S + pi];
+    double uk = kx * u1 + ky * v1 + kz * w1;
+    const double ut = u1 - kx * uk, vt = v1 - ky * uk, wt = w1 - kz * uk;
+    bv[S + p] = ut;
+    bv[2 * S + p] = vt;
+    bv[3 * S + p] = wt;
+    z.vel[p] = sqrt(ut * ut + vt * vt + wt * wt);
+    bv[p] = bv[pi];
+    bv[4 * S + p] = bv[4 * S + pi];
+    bv[5 * S + p] = bv[5 * S + pi];
+    for (int l = 0; l < P.nsc; ++l) sv[l * S + p] = sv[l * S + pi];
+    for (int g = 1; g <= z.g; ++g) {
+      const int64_t pg = p + g * sd, pm = p - g * sd;
+      bv[pg] = bv[pm];
+      const double u = bv[S + pm], v = bv[2 * S + pm], w = bv[3 * S + pm];
+      uk = kx * u + ky * v + kz * w;
+      const double ug = u - 2 * uk * kx, vg = v - 2 * uk * ky, wg = w - 2 * uk * kz;
+      bv[S + pg] = ug;
+      bv[2 * S + pg] = vg;
+      bv[3 * S + pg] = wg;
+      z.vel[pg] = sqrt(ug * ug + vg * vg + wg * wg);
+      bv[4 * S + pg] = bv[4 * S + pm];
+      bv[5 * S + pg] = bv[5 * S + pm];
+      for (int l = 0; l < P.nsc; ++l) sv[l * S + pg] = sv[l * S + pm];
+    }
+  } else if (b.type == 6) {
+    // outflow (BoundCond.cuh:120-148)
+    for (int g = 1; g <= z.g; ++g) {
+      const int64_t pg = p + g * sd;
+#pragma unroll
+      for (int l = 0; l < 6; ++l) bv[l * S + pg] = bv[l * S + p];
+      for (int l = 0; l < P.nsc; ++l) sv[l * S + pg] = sv[l * S + p];
+      if (WITH_CV) cv_point<NS>(z, m, pg);
+    }
+  } else if (b.type == 5) {
+    // inflow (BoundCond.cuh:150-212)
+    for (int g = 0; g <= z.g; ++g) {
+      const int64_t pg = p + g * sd;
+      bv[pg] = b.rho;
+      bv[S + pg] = b.u;
+      bv[2 * S + pg] = b.v;
+      bv[3 * S + pg] = b.w;
+      bv[4 * S + pg] = b.p;
+      bv[5 * S + pg] = b.t;
+      for (int l = 0; l < P.nsc; ++l) sv[l * S + pg] = b.sv[l];
+      if (g == 0) z.vel[pg] = b.vel;
+      if (WITH_CV) cv_point<NS>(z, m, pg);
+    }
+  } else if (b.type == 2) {
+    // no-slip wall (BoundCond.cuh:424-552)
+    const int64_t pi = p - sd;
+    double tw = b.isothermal ? b.t_wall : bv[5 * S + pi];
+    const double pw = bv[4 * S + pi];
+    double mw = kMwAirD;
+    if constexpr (NS > 0) {
+      double s = 0;
+#pragma unroll
+      for (int l = 0; l < NS; ++l) {
+        const double y = sv[l * S + pi];
+        sv[l * S + p] = y;
+        s += y / m->mw[l];
+      }
+      mw = 1 / s;
+    }
+    bv[p] = pw * mw / (tw * kRuD);
+    bv[S + p] = 0;
+    bv[2 * S + p] = 0;
+    bv[3 * S + p] = 0;
+    bv[4 * S + p] = pw;
+    bv[5 * S + p] = tw;
+    z.vel[p] = 0;
+    if (WITH_CV) cv_point<NS>(z, m, p);
+    for (int g = 1; g <= z.g; ++g) {
+      const int64_t pa = p - g * sd, pg = p + g * sd;
+      const double ui = bv[S + pa], vi = bv[2 * S + pa], wi = bv[3 * S + pa], pii = bv[4 * S + pa],
+                   ti = bv[5 * S + pa];
+      double tg = ti;
+      if (b.isothermal) {
+        tg = 2 * tw - ti;
+        if (tg <= 0.1 * tw) tg = tw;
+      }
+      if constexpr (NS > 0) {
+        double s = 0;
+#pragma unroll
+        for (int l = 0; l < NS; ++l) {
+          const double y = sv[l * S + pa];
+          sv[l * S + pg] = y;
+          s += y / m->mw[l];
+        }
+        mw = 1 / s;
+      }
+      bv[pg] = pii * mw / (tg * kRuD);
+      bv[S + pg] = -ui;
+      bv[2 * S + pg] = -vi;
+      bv[3 * S + pg] = -wi;
+      bv[4 * S + pg] = pii;
+      bv[5 * S + pg] = tg;
+      if (WITH_CV) cv_point<NS>(z, m, pg);
+    }
+  }
+}
+
+// FieldOperation.cuh:372-400 eliminate_k_gradient (2-D runs)
+template <bool WITH_CV>
+__global__ void __launch_bounds__(kTB) k_eliminate_k(ZoneView z, DParams P) {
+  const int X = z.mx + 2 * z.g, Y = z.my + 2 * z.g;
+  const int64_t t = (int64_t)blockIdx.x * kTB + threadIdx.x;
+  if (t >= (int64_t)X * Y) return;
+  const int i = (int)(t % X) - z.g, j = (int)(t / X) - z.g;
+  const int64_t p0 = z.at(i, j, 0);
+  for (int k = 1; k <= z.g; ++k) {
+    const int64_t pu = p0 + k * z.d1, pd = p0 - k * z.d1;
+#pragma unroll
+    for (int l = 0; l < 6; ++l) {
+      const double v = z.bv[l * z.sz + p0];
+      z.bv[l * z.sz + pu] = v;
+      z.bv[l * z.sz + pd] = v;
+    }
+    for (int l = 0; l < P.nsc; ++l) {
+      const double v = z.sv[l * z.sz + p0];
+      z.sv[l * z.sz + pu] = v;
+      z.sv[l * z.sz + pd] = v;
+    }
+    if (WITH_CV)
+      for (int l = 0; l < P.nv; ++l) {
+        const double v = z.cv[l * z.sz + p0];
+        z.cv[l * z.sz + pu] = v;
+        z.cv[l * z.sz + pd] = v;
+      }
+  }
+}
+
+// FieldOperation.cuh:112-146 update_physical_properties on [-1, m]
+template <int NS>
+__global__ void __launch_bounds__(kTB) k_properties(ZoneView z, DParams P, const DMech *__restrict__ m) {
+  const int X = z.mx + 2, Y = z.my + 2, Z = z.mz + 2;
+  const int64_t t = (int64_t)blockIdx.x * kTB + threadIdx.x;
+  if (t >= (int64_t)X * Y * Z) return;
+  const int i = (int)(t % X) - 1, j = (int)((t / X) % Y) - 1, k = (int)(t / ((int64_t)X * Y)) - 1;
+  const int64_t p = z.at(i, j, k);
+  const double T = z.bv[5 * z.sz + p];
+  if constexpr (NS > 0) {
+    double Yk[NS], cp[NS];
+    d_cp<NS>(T, m, cp);
+    double mwi = 0, cpt = 0, cvt = 0;
+#pragma unroll
+    for (int l = 0; l < NS; ++l) {
+      Yk[l] = z.sv[(int64_t)l * z.sz + p];
+      mwi += Yk[l] / m->mw[l];
+      cpt += Yk[l] * cp[l];
+      cvt += Yk[l] * (cp[l] - kRuD / m->mw[l]);
+    }
+    const double mw = 1 / mwi;
+    const double gam = cpt / cvt;
+    z.cp[p] = cpt;
+    z.gam[p] = gam;
+    z.acoustic[p] = sqrt(gam * kRuD * T / mw);
+    double mu, lam, rd[NS];
+    d_transport<NS>(m, T, z.bv[4 * z.sz + p], z.bv[p], Yk, cp, mu, lam, rd);
+    z.mul[p] = mu;
+    z.cond[p] = lam;
+#pragma unroll
+    for (int l = 0; l < NS; ++l) z.rhoD[(int64_t)l * z.sz + p] = rd[l];
+  } else {
+    const double c_temp = kGam * kRuD / kMwAirD;
+    const double cp = kGam * kRuD / kMwAirD / (kGam - 1);
+    z.acoustic[p] = sqrt(c_temp * T);
+    const double mu = d_sutherland(T);
+    z.mul[p] = mu;
+    z.cond[p] = mu * cp / P.Pr;
+  }
+  z.mach[p] = z.vel[p] / z.acoustic[p];
+}
+
+// TimeAdvanceFunc.cu:6-17 store_last_step
+__global__ void __launch_bounds__(kTB) k_store_last(ZoneView z) {
+  const int64_t t = (int64_t)blockIdx.x * kTB + threadIdx.x;
+  const int64_t n = (int64_t)z.mx * z.my * z.mz;
+  if (t >= n) return;
+  int i, j, k;
+  cell_of(z, t, i, j, k);
+  const int64_t p = z.at(i, j, k), q = z.at0(i, j, k);
+  z.bv_last[q] = z.bv[p];
+  z.bv_last[z.sz0 + q] = z.vel[p];
+  z.bv_last[2 * z.sz0 + q] = z.bv[4 * z.sz + p];
+  z.bv_last[3 * z.sz0 + q] = z.bv[5 * z.sz + p];
+}
+
+// Residual.cuh:57-161 + TimeAdvanceFunc.cu:19-33: squared change of
+// (rho, |V|, p, T), summed; the last block finishes the RMS, the step-1
+// scale (floor 1e-20) and err_max, all on the device.
+__global__ void __launch_bounds__(kTB) k_residual(ZoneView z, double *partial, unsigned *counter, double *res_scale,
+                                                   double *out, int first) {
+  const int64_t t = (int64_t)blockIdx.x * kTB + threadIdx.x;
+  const int64_t n = (int64_t)z.mx * z.my * z.mz;
+  double s[4] = {0, 0, 0, 0};
+  if (t < n) {
+    int i, j, k;
+    cell_of(z, t, i, j, k);
+    const int64_t p = z.at(i, j, k), q = z.at0(i, j, k);
+    const double cur[4] = {z.bv[p], z.vel[p], z.bv[4 * z.sz + p], z.bv[5 * z.sz + p]};
+#pragma unroll
+    for (int l = 0; l < 4; ++l) {
+      const double d = cur[l] - z.bv_last[l * z.sz0 + q];
+      z.bv_last[l * z.sz0 + q] = d * d;
+      s[l] = d * d;
+    }
+  }
+  __shared__ double sm[4][kTB / 64];
+  __shared__ bool last;
+#pragma unroll
+  for (int l = 0; l < 4; ++l) {
+    const double v = wave_sum(s[l]);
+    if ((threadIdx.x & 63) == 0) sm[l][threadIdx.x >> 6] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int l = 0; l < 4; ++l) {
+      double b = 0;
+#pragma unroll
+      for (int w = 0; w < kTB / 64; ++w) b += sm[l][w];
+      partial[(size_t)l * gridDim.x + blockIdx.x] = b;
+    }
+    __threadfence();
+    last = atomicAdd(counter, 1u) == gridDim.x - 1;
+  }
+  __syncthreads();
+  if (!last) return;
+  __threadfence();
+  double r[4] = {0, 0, 0, 0};
+  for (unsigned b = threadIdx.x; b < gridDim.x; b += kTB)
+#pragma unroll
+    for (int l = 0; l < 4; ++l)
+      r[l] += __hip_atomic_load(&partial[(size_t)l * gridDim.x + b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __syncthreads();
+#pragma unroll
+  for (int l = 0; l < 4; ++l) {
+    const double v = wave_sum(r[l]);
+    if ((threadIdx.x & 63) == 0) sm[l][threadIdx.x >> 6] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double res[4];
+#pragma unroll
+    for (int l = 0; l < 4; ++l) {
+      double b = 0;
+#pragma unroll
+      for (int w = 0; w < kTB / 64; ++w) b += sm[l][w];
+      res[l] = sqrt(b / (double)n);
+      if (first) res_scale[l] = res[l] < 1e-20 ? 1e-20 : res[l];
+      res[l] /= res_scale[l];
+      out[l] = res[l];
+    }
+    double e = res[0];
+#pragma unroll
+    for (int l = 1; l < 4; ++l)
+      if (fabs(res[l]) > e) e = res[l];
+    out[4] = e;
+    *counter = 0u;
+  }
+}
+
+__global__ void k_copy1(const double *src, double *dst) { *dst = *src; }
+
+// ------------------------------------------------------------- chemistry batch
+// One point-implicit step of isolated constant-volume cells: omega and the
+// exact Jacobian at frozen T, (I - dt J) x = omega, rho Y += dt x, then the
+// Newton temperature at the unchanged internal energy (the composition of
+// FiniteRateChem.cu:7-63/234-256 and FieldOperation.cuh:221-296 at V = 0).
+template <int NS>
+__global__ void __launch_bounds__(128) k_chem_batch(const DMech *__restrict__ m, int64_t n,
+                                                    const double *__restrict__ rho_a, const double *__restrict__ T_a,
+                                                    const double *__restrict__ Y_a, double dt, double *__restrict__ dY,
+                                                    double *__restrict__ dT) {
+  const int64_t c = (int64_t)blockIdx.x * 128 + threadIdx.x;
+  if (c >= n) return;
+  const double rho = rho_a[c], T = T_a[c];
+  double Y[NS], om[NS], J[NS * NS], h[NS];
+#pragma unroll
+  for (int l = 0; l < NS; ++l) Y[l] = Y_a[(int64_t)l * n + c];
+  double mwi = 0;
+#pragma unroll
+  for (int l = 0; l < NS; ++l) mwi += Y[l] / m->mw[l];
+  const double mw = 1 / mwi;
+  const double p_old = rho * T * (kRuD / mw);
+  d_enthalpy<NS>(T, m, h);
+  double rhoE = 0;
+#pragma unroll
+  for (int l = 0; l < NS; ++l) rhoE += h[l] * Y[l];
+  rhoE *= rho;
+  rhoE -= p_old;
+  d_chem_source<NS>(m, rho, T, Y, om, J);
+#pragma unroll
+  for (int a = 0; a < NS; ++a)
+#pragma unroll
+    for (int b = 0; b < NS; ++b) J[a * NS + b] = (a == b ? 1.0 : 0.0) - dt * J[a * NS + b];
+  d_lu_solve<NS>(J, om);
+  const double ri = 1.0 / rho;
+  double Yn[NS];
+#pragma unroll
+  for (int l = 0; l < NS; ++l) Yn[l] = ri * (rho * Y[l] + om[l] * dt);
+  double pn;
+  const double Tn = d_temperature<NS>(m, rho, rhoE, 0.0, 0.0, 0.0, Yn, T, pn);
+#pragma unroll
+  for (int l = 0; l < NS; ++l) dY[(int64_t)l * n + c] = Yn[l] - Y[l];
+  dT[c] = Tn - T;
+}
+
+inline unsigned nblk(int64_t n, int tb) { return (unsigned)((n + tb - 1) / tb); }
+
+}  // namespace
+
+hipError_t launch_source(const ZoneView &z, const DParams &P, const DMech *m, hipStream_t s) {
+  const int64_t n = (int64_t)z.mx * z.my * z.mz;
+  if (P.ns != 9) return hipErrorNotSupported;
+  hipLaunchKernelGGL(k_source<9>, dim3(nblk(n, kTB)), dim3(kTB), 0, s, z, P, m);
+  return hipGetLastError();
+}
+
+hipError_t launch_local_dt(const ZoneView &z, const DParams &P, const StepScratch &w, double *dt_out, hipStream_t s) {
+  const int64_t n = (int64_t)z.mx * z.my * z.mz;
+  if ((int64_t)nblk(n, kTB) > w.max_blocks) return hipErrorInvalidValue;
+  if (P.mix)
+    hipLaunchKernelGGL(k_local_dt<1>, dim3(nblk(n, kTB)), dim3(kTB), 0, s, z, P, w.partial, w.counter, dt_out);
+  else
+    hipLaunchKernelGGL(k_local_dt<0>, dim3(nblk(n, kTB)), dim3(kTB), 0, s, z, P, w.partial, w.counter, dt_out);
+  return hipGetLastError();
+}
+
+hipError_t launch_rk_update(const ZoneView &z, const DParams &P, const DMech *m, const double *dt, int rk,
+                            hipStream_t s) {
+  const int64_t n = (int64_t)z.mx * z.my * z.mz;
+  if (P.ns == 0)
+    hipLaunchKernelGGL(k_rk_update<0>, dim3(nblk(n, kTB)), dim3(kTB), 0, s, z, P, m, dt, rk);
+  else if (P.ns == 9)
+    hipLaunchKernelGGL(k_rk_update<9>, dim3(nblk(n, kTB)), dim3(kTB), 0, s, z, P, m, dt, rk);
+  else
+    return hipErrorNotSupported;
+  return hipGetLastError();
+}
+
+hipError_t launch_bc(const ZoneView &z, const DParams &P, const DMech *m, const BCDesc &bc, bool with_cv,
+                     hipStream_t s) {
+  const int64_t n = (int64_t)(bc.re[0] - bc.rs[0] + 1) * (bc.re[1] - bc.rs[1] + 1) * (bc.re[2] - bc.rs[2] + 1);
+  if (n <= 0) return hipSuccess;
+  const dim3 g(nblk(n, kTB)), b(kTB);
+  if (P.ns == 0) {
+    if (with_cv)
+      hipLaunchKernelGGL((k_bc<0, true>), g, b, 0, s, z, P, m, bc);
+    else
+      hipLaunchKernelGGL((k_bc<0, false>), g, b, 0, s, z, P, m, bc);
+  } else if (P.ns == 9) {
+    if (with_cv)
+      hipLaunchKernelGGL((k_bc<9, true>), g, b, 0, s, z, P, m, bc);
+    else
+      hipLaunchKernelGGL((k_bc<9, false>), g, b, 0, s, z, P, m, bc);
+  } else {
+    return hipErrorNotSupported;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_eliminate_k(const ZoneView &z, const DParams &P, bool with_cv, hipStream_t s) {
+  const int64_t n = (int64_t)(z.mx + 2 * z.g) * (z.my + 2 * z.g);
+  if (with_cv)
+    hipLaunchKernelGGL(k_eliminate_k<true>, dim3(nblk(n, kTB)), dim3(kTB), 0, s, z, P);
+  else
+    hipLaunchKernelGGL(k_eliminate_k<false>, dim3(nblk(n, kTB)), dim3(kTB), 0, s, z, P);
+  return hipGetLastError();
+}
+
+hipError_t launch_properties(const ZoneView &z, const DParams &P, const DMech *m, hipStream_t s) {
+  const int64_t n = (int64_t)(z.mx + 2) * (z.my + 2) * (z.mz + 2);
+  if (P.ns == 0)
+    hipLaunchKernelGGL(k_properties<0>, dim3(nblk(n, kTB)), dim3(kTB), 0, s, z, P, m);
+  else if (P.ns == 9)
+    hipLaunchKernelGGL(k_properties<9>, dim3(nblk(n, kTB)), dim3(kTB), 0, s, z, P, m);
+  else
+    return hipErrorNotSupported;
+  return hipGetLastError();
+}
+
+hipError_t launch_velocity(const ZoneView &z, hipStream_t s) {
+  const int64_t n = (int64_t)(z.mx + 2 * z.g) * (z.my + 2 * z.g) * (z.mz + 2 * z.g);
+  hipLaunchKernelGGL(k_velocity, dim3(nblk(n, kTB)), dim3(kTB), 0, s, z);
+  return hipGetLastError();
+}
+
+hipError_t launch_cv_from_bv(const ZoneView &z, const DParams &P, const DMech *m, hipStream_t s) {
+  const int64_t n = (int64_t)(z.mx + 2 * z.g) * (z.my + 2 * z.g) * (z.mz + 2 * z.g);
+  if (P.ns == 0)
+    hipLaunchKernelGGL(k_cv_from_bv<0>, dim3(nblk(n, kTB)), dim3(kTB), 0, s, z, m);
+  else if (P.ns == 9)
+    hipLaunchKernelGGL(k_cv_from_bv<9>, dim3(nblk(n, kTB)), dim3(kTB), 0, s, z, m);
+  else
+    return hipErrorNotSupported;
+  return hipGetLastError();
+}
+
+hipError_t launch_store_last(const ZoneView &z, hipStream_t s) {
+  const int64_t n = (int64_t)z.mx * z.my * z.mz;
+  hipLaunchKernelGGL(k_store_last, dim3(nblk(n, kTB)), dim3(kTB), 0, s, z);
+  return hipGetLastError();
+}
+
+hipError_t launch_residual(const ZoneView &z, const StepScratch &w, int slot, int first, hipStream_t s) {
+  const int64_t n = (int64_t)z.mx * z.my * z.mz;
+  if ((int64_t)nblk(n, kTB) * 4 > (int64_t)w.max_blocks * 4) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_residual, dim3(nblk(n, kTB)), dim3(kTB), 0, s, z, w.partial, w.counter, w.res_scale,
+                     w.res_hist + (size_t)slot * 5, first);
+  return hipGetLastError();
+}
+
+hipError_t launch_copy_dt(const double *src, double *dst, hipStream_t s) {
+  hipLaunchKernelGGL(k_copy1, dim3(1), dim3(1), 0, s, src, dst);
+  return hipGetLastError();
+}
+
+hipError_t launch_chem_batch(const DMech *m, int64_t n, const double *rho, const double *T, const double *Y, double dt,
+                             double *dY, double *dT, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_chem_batch<9>, dim3(nblk(n, 128)), dim3(128), 0, s, m, n, rho, T, Y, dt, dY, dT);
+  return hipGetLastError();
+}
+
+}  // namespace ostc

@@ -1,0 +1,279 @@
+"""Python front-end of the OpenSTC-MI355X C ABI (include/ostc.h).
+
+Loads the in-tree libostc.so with ctypes; there is no Python compute path:
+if the library is missing or the GPU is unavailable every call raises.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+PKG = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(PKG, "libostc.so")
+
+FIELDS = dict(bv=0, sv=1, cv=2, dq=3, vel=4, acoustic=5, mach=6, mul=7, cond=8, rhoD=9, gamma=10, cp=11,
+              dt_local=12, qn=13, bv_last=14, jac=15, metric=16, chem_jac=17)
+
+
+class OstcError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise OstcError(f"{LIB_PATH} not built; run openstc-dev_amd/build.py")
+        L = C.CDLL(LIB_PATH)
+        vp, dp, ip = C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int)
+        sig = {
+            "ostc_abi_version": ([], C.c_int),
+            "ostc_last_error": ([], C.c_char_p),
+            "ostc_init": ([C.c_char_p, C.c_int, C.POINTER(vp)], C.c_int),
+            "ostc_destroy": ([vp], C.c_int),
+            "ostc_get_int": ([vp, C.c_char_p, ip], C.c_int),
+            "ostc_get_real": ([vp, C.c_char_p, dp], C.c_int),
+            "ostc_get_table": ([vp, C.c_char_p, dp, C.c_int64], C.c_int),
+            "ostc_add_block": ([vp, C.c_int, C.c_int, C.c_int, dp, dp, dp, ip], C.c_int),
+            "ostc_add_boundary": ([vp] + [C.c_int] * 8, C.c_int),
+            "ostc_read_boundary_file": ([vp], C.c_int),
+            "ostc_finalize": ([vp], C.c_int),
+            "ostc_block_geometry": ([vp, C.c_int, dp, dp], C.c_int),
+            "ostc_field_info": ([vp, C.c_int, C.c_int, C.POINTER(C.c_int64), ip, ip], C.c_int),
+            "ostc_upload": ([vp, C.c_int, C.c_int, dp], C.c_int),
+            "ostc_download": ([vp, C.c_int, C.c_int, dp], C.c_int),
+            "ostc_field_device_ptr": ([vp, C.c_int, C.c_int, C.POINTER(vp)], C.c_int),
+            "ostc_initialize_uniform": ([vp, C.c_int], C.c_int),
+            "ostc_inflow_state": ([vp, C.c_char_p, dp, C.c_int], C.c_int),
+            "ostc_initialize": ([vp], C.c_int),
+            "ostc_rhs": ([vp, C.c_int], C.c_int),
+            "ostc_global_dt": ([vp, dp], C.c_int),
+            "ostc_rk3_stage": ([vp, C.c_int, C.c_double, C.c_int], C.c_int),
+            "ostc_apply_bc": ([vp, C.c_int, C.c_int], C.c_int),
+            "ostc_update_properties": ([vp, C.c_int], C.c_int),
+            "ostc_step_rk3": ([vp, C.c_int, dp, dp], C.c_int),
+            "ostc_time": ([vp, dp, ip], C.c_int),
+            "ostc_set_flux_precision": ([vp, C.c_int], C.c_int),
+            "ostc_chem_batch": ([vp, C.c_int64, vp, vp, vp, C.c_double, vp, vp, C.c_int,
+                                 C.POINTER(C.c_int64)], C.c_int),
+            "ostc_dev_alloc": ([vp, C.c_int64, C.POINTER(vp)], C.c_int),
+            "ostc_dev_free": ([vp, vp], C.c_int),
+            "ostc_memcpy_h2d": ([vp, vp, vp, C.c_int64], C.c_int),
+            "ostc_memcpy_d2h": ([vp, vp, vp, C.c_int64], C.c_int),
+            "ostc_sync": ([vp], C.c_int),
+            "ostc_profile": ([vp, C.c_int], C.c_int),
+            "ostc_profile_report": ([vp, C.c_char_p, C.c_int], C.c_int),
+        }
+        for name, (args, res) in sig.items():
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = res
+        _lib = L
+    return _lib
+
+
+def exported_symbols():
+    return [n for n in dir(lib()) if n.startswith("ostc_")]
+
+
+def _chk(rc):
+    if rc != 0:
+        raise OstcError(lib().ostc_last_error().decode())
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+class DeviceArray:
+    """A device buffer owned by a Solver (ostc_dev_alloc)."""
+
+    def __init__(self, solver, nbytes):
+        self.s = solver
+        self.nbytes = int(nbytes)
+        p = C.c_void_p()
+        _chk(lib().ostc_dev_alloc(solver.h, self.nbytes, C.byref(p)))
+        self.ptr = p
+
+    @classmethod
+    def from_host(cls, solver, arr):
+        arr = np.ascontiguousarray(arr, dtype=np.float64)
+        d = cls(solver, arr.nbytes)
+        _chk(lib().ostc_memcpy_h2d(solver.h, d.ptr, arr.ctypes.data_as(C.c_void_p), arr.nbytes))
+        return d
+
+    def to_host(self, shape):
+        out = np.empty(shape, dtype=np.float64)
+        _chk(lib().ostc_memcpy_d2h(self.s.h, out.ctypes.data_as(C.c_void_p), self.ptr, out.nbytes))
+        return out
+
+    def free(self):
+        if self.ptr:
+            _chk(lib().ostc_dev_free(self.s.h, self.ptr))
+            self.ptr = None
+
+
+class Solver:
+    """One solver instance over a reference-format case directory."""
+
+    def __init__(self, case_dir, device=0):
+        h = C.c_void_p()
+        _chk(lib().ostc_init(case_dir.encode(), device, C.byref(h)))
+        self.h = h
+        self.blocks = []
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().ostc_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def get_int(self, key):
+        v = C.c_int()
+        _chk(lib().ostc_get_int(self.h, key.encode(), C.byref(v)))
+        return v.value
+
+    def get_real(self, key):
+        v = C.c_double()
+        _chk(lib().ostc_get_real(self.h, key.encode(), C.byref(v)))
+        return v.value
+
+    def table(self, name, count):
+        out = np.zeros(count)
+        _chk(lib().ostc_get_table(self.h, name.encode(), _dp(out), count))
+        return out
+
+    def add_block(self, mx, my, mz, x, y, z):
+        x = np.ascontiguousarray(x, dtype=np.float64).ravel()
+        y = np.ascontiguousarray(y, dtype=np.float64).ravel()
+        z = np.ascontiguousarray(z, dtype=np.float64).ravel()
+        b = C.c_int()
+        _chk(lib().ostc_add_block(self.h, mx, my, mz, _dp(x), _dp(y), _dp(z), C.byref(b)))
+        self.blocks.append((mx, my, mz))
+        return b.value
+
+    def add_boundary(self, blk, raw, label):
+        _chk(lib().ostc_add_boundary(self.h, blk, *[int(v) for v in raw], int(label)))
+
+    def read_boundary_file(self):
+        _chk(lib().ostc_read_boundary_file(self.h))
+
+    def finalize(self):
+        _chk(lib().ostc_finalize(self.h))
+
+    def geometry(self, blk=0):
+        mx, my, mz = self.blocks[blk]
+        g = self.get_int("ngg")
+        n = (mx + 2 * g) * (my + 2 * g) * (mz + 2 * g)
+        jac, met = np.zeros(n), np.zeros(9 * n)
+        _chk(lib().ostc_block_geometry(self.h, blk, _dp(jac), _dp(met)))
+        return jac, met
+
+    def field_info(self, field, blk=0):
+        n, g, c = C.c_int64(), C.c_int(), C.c_int()
+        _chk(lib().ostc_field_info(self.h, blk, FIELDS[field], C.byref(n), C.byref(g), C.byref(c)))
+        return n.value, g.value, c.value
+
+    def download(self, field, blk=0):
+        n, g, c = self.field_info(field, blk)
+        out = np.zeros(n)
+        if n:
+            _chk(lib().ostc_download(self.h, blk, FIELDS[field], _dp(out)))
+        return out
+
+    def upload(self, field, arr, blk=0):
+        n, g, c = self.field_info(field, blk)
+        a = np.ascontiguousarray(arr, dtype=np.float64).ravel()
+        if a.size != n:
+            raise OstcError(f"{field}: expected {n} values, got {a.size}")
+        _chk(lib().ostc_upload(self.h, blk, FIELDS[field], _dp(a)))
+
+    def device_ptr(self, field, blk=0):
+        p = C.c_void_p()
+        _chk(lib().ostc_field_device_ptr(self.h, blk, FIELDS[field], C.byref(p)))
+        return p.value
+
+    def initialize_uniform(self, blk=0):
+        _chk(lib().ostc_initialize_uniform(self.h, blk))
+
+    def inflow_state(self, name, n=7):
+        out = np.zeros(n)
+        _chk(lib().ostc_inflow_state(self.h, name.encode(), _dp(out), n))
+        return out
+
+    def initialize(self):
+        _chk(lib().ostc_initialize(self.h))
+
+    def rhs(self, blk=0):
+        _chk(lib().ostc_rhs(self.h, blk))
+
+    def global_dt(self):
+        v = C.c_double()
+        _chk(lib().ostc_global_dt(self.h, C.byref(v)))
+        return v.value
+
+    def rk3_stage(self, dt, rk, blk=0):
+        _chk(lib().ostc_rk3_stage(self.h, blk, dt, rk))
+
+    def apply_bc(self, with_cv, blk=0):
+        _chk(lib().ostc_apply_bc(self.h, blk, 1 if with_cv else 0))
+
+    def update_properties(self, blk=0):
+        _chk(lib().ostc_update_properties(self.h, blk))
+
+    def step(self, nsteps):
+        dt = np.zeros(3 * nsteps)
+        res = np.zeros(5 * nsteps)
+        _chk(lib().ostc_step_rk3(self.h, nsteps, _dp(dt), _dp(res)))
+        return dt.reshape(nsteps, 3), res.reshape(nsteps, 5)
+
+    def time(self):
+        t, s = C.c_double(), C.c_int()
+        _chk(lib().ostc_time(self.h, C.byref(t), C.byref(s)))
+        return t.value, s.value
+
+    def set_flux_fp32(self, on):
+        _chk(lib().ostc_set_flux_precision(self.h, 1 if on else 0))
+
+    def chem_batch(self, n, rho, T, Y, dt, dY, dT, mode=0):
+        """rho/T/Y/dY/dT: DeviceArray (Y, dY component-major n_spec x n)."""
+        nf = C.c_int64()
+        _chk(lib().ostc_chem_batch(self.h, n, rho.ptr, T.ptr, Y.ptr, dt, dY.ptr, dT.ptr, mode, C.byref(nf)))
+        return nf.value
+
+    def sync(self):
+        _chk(lib().ostc_sync(self.h))
+
+    def profile(self, on):
+        _chk(lib().ostc_profile(self.h, 1 if on else 0))
+
+    def profile_report(self):
+        buf = C.create_string_buffer(1 << 16)
+        _chk(lib().ostc_profile_report(self.h, buf, len(buf)))
+        out = {}
+        for line in buf.value.decode().splitlines():
+            name, calls, ms = line.split()
+            out[name] = (int(calls), float(ms))
+        return out
+
+
+def solver_for_case(case, root, device=0, initial=True):
+    """Build a solver for a generated case dict (see ostc.cases)."""
+    from . import cases as _cases
+    _cases.write_case_dir(case, root)
+    s = Solver(root, device)
+    s.add_block(case["mx"], case["my"], case["mz"], case["x"], case["y"], case["z"])
+    s.read_boundary_file()
+    s.finalize()
+    if initial:
+        bv, sv = _cases.initial_fields(case)
+        s.upload("bv", bv)
+        if sv.shape[0]:
+            s.upload("sv", sv)
+        s.initialize()
+    return s

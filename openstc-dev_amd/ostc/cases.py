@@ -1,0 +1,278 @@
+"""Seeded workload generators for the solver's benchmark and parity cases.
+
+Each builder returns a plain dict describing a single-block case in the
+reference's own input vocabulary: setup-file keys (Parameter.cpp:53-163
+format), boundary-file lines (Mesh.cpp:686-706 format: i1 i2 j1 j2 k1 k2
+label, 0-based), interior node coordinates (i fastest) and the seeded initial
+primitive field.  write_case_dir() lays the case out as a reference case
+directory (input_files/setup/*.txt, input_files/boundary_condition/,
+input_files/chemistry/) so the solver reads it through its setup-file
+interface.
+
+Random numbers: splitmix64 -> uniform [0,1) (top 53 bits), Gaussian by
+Box-Muller; identical streams feed the GPU run and the CPU baseline.
+"""
+import os
+import shutil
+
+import numpy as np
+
+R_U = 8314.462618
+GAMMA_AIR = 1.4
+MW_AIR = 28.9647
+PKG = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+CHEM_DIR = os.path.join(PKG, "input_files", "chemistry")
+
+_MASK = (1 << 64) - 1
+
+
+class SplitMix64:
+    """splitmix64 (Steele, Lea, Flood 2014) vectorised over a counter."""
+
+    def __init__(self, seed):
+        self.state = np.uint64(seed & _MASK)
+
+    def u64(self, n):
+        with np.errstate(over="ignore"):
+            ctr = np.arange(1, n + 1, dtype=np.uint64) * np.uint64(0x9E3779B97F4A7C15)
+            z = self.state + ctr
+            self.state = np.uint64(z[-1]) if n else self.state
+            z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+            z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+            z = z ^ (z >> np.uint64(31))
+        return z
+
+    def uniform(self, n):
+        return (self.u64(n) >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)
+
+    def normal(self, n):
+        m = (n + 1) // 2
+        u1 = self.uniform(m)
+        u2 = self.uniform(m)
+        u1 = np.where(u1 <= 0.0, 1e-300, u1)
+        r = np.sqrt(-2.0 * np.log(u1))
+        z = np.concatenate([r * np.cos(2 * np.pi * u2), r * np.sin(2 * np.pi * u2)])
+        return z[:n]
+
+
+def sutherland(t):
+    """Transport.cu:6-9."""
+    return 1.716e-5 * (t / 273) ** 1.5 * (273 + 111) / (t + 111)
+
+
+def air_inflow(mach, p, t):
+    mw = MW_AIR
+    c = np.sqrt(GAMMA_AIR * R_U / mw * t)
+    return dict(rho=p * mw / (R_U * t), u=mach * c, v=0.0, w=0.0, p=p, t=t, vel=mach * c)
+
+
+def ghost_shape(mx, my, mz, g):
+    return (mz + 2 * g, my + 2 * g, mx + 2 * g)
+
+
+def _setup_common(extra):
+    base = {
+        "int initial": 0, "int species": 0, "int reaction": 0, "int chemSrcMethod": 0, "bool turbulence": 0,
+        "int turbulence_method": 1, "int RANS_model": 2, "int turb_implicit": 1, "int compressibility_correction": 0,
+        "bool gradPInDiffusionFlux": 0, "bool if_collect_statistics": 0, "int start_collect_statistics_iter": 0,
+        "string mechanism_file": "chemistry/combustion_mechanism.inp", "string therm_file": "chemistry/therm.dat",
+        "string transport_file": "chemistry/tran.dat", "bool parallel": 0, "int output_file": 100000,
+        "int output_screen": 1, "bool steady": 0, "int implicit_method": 0, "int DPLUR_inner_step": 3,
+        "real convergence_criteria": 1e-7, "int temporal_scheme": 3, "bool fixed_time_step": 0, "real dt": 1e-5,
+        "real total_simulation_time": 1e10, "int inviscid_scheme": 3, "int reconstruction": 2, "int limiter": 0,
+        "real entropy_fix_factor": 0.125, "int viscous_order": 2, "int diffusivity_method": 0,
+        "real schmidt_number": 0.9, "real prandtl_number": 0.72, "real turbulent_prandtl_number": 0.9,
+        "real turbulent_schmidt_number": 0.9, "real c_chi": 1.0, "int groups_init": 1, "int if_monitor": 0,
+        "real gridScale": 1.0, "int wall_distance": 0,
+    }
+    base.update(extra)
+    return base
+
+
+def c1_case(nx=200, ny=100, seed=1001, steps=100):
+    """2-D inviscid air, 200x100 Cartesian (spacing 1 mm), AUSM+ with
+    MUSCL-minmod, SSP-RK3 at CFL 0.5.  Mach 2 supersonic inflow (101325 Pa,
+    300 K) at i=0, supersonic outflow at i=max, symmetry at j=0 and j=max.
+    Initial state: uniform inflow; interior density scaled by
+    (1 + 0.01 N(0,1)) at fixed pressure, T from the equation of state."""
+    dx = 1e-3
+    xi = np.arange(nx) * dx
+    yj = np.arange(ny) * dx
+    X, Y = np.meshgrid(xi, yj, indexing="xy")  # (ny, nx)
+    x = X[None, :, :].copy()
+    y = Y[None, :, :].copy()
+    z = np.zeros_like(x)
+    inflow = air_inflow(2.0, 101325.0, 300.0)
+    setup = _setup_common({
+        "int dimension": 2, "real cfl": 0.5, "int inviscid_scheme": 3, "int reconstruction": 2, "int limiter": 0,
+        "int viscous_order": 0, "int total_step": steps,
+        "array string boundary_conditions": ["freestream", "outflow", "symmetry"],
+        "string reference_state": "freestream", "string default_init": "freestream",
+        "struct freestream": [("string", "type", "inflow"), ("int", "label", 5), ("real", "mach", 2.0),
+                              ("real", "pressure", 101325.0), ("real", "temperature", 300.0), ("real", "u", 1.0),
+                              ("real", "v", 0.0), ("real", "w", 0.0)],
+        "struct outflow": [("string", "type", "outflow"), ("int", "label", 6)],
+        "struct symmetry": [("string", "type", "symmetry"), ("int", "label", 3)],
+    })
+    bcs = [
+        dict(type=5, raw=(0, 0, 0, ny - 1, 0, 0), label=5),
+        dict(type=6, raw=(nx - 1, nx - 1, 0, ny - 1, 0, 0), label=6),
+        dict(type=3, raw=(0, nx - 1, 0, 0, 0, 0), label=3),
+        dict(type=3, raw=(0, nx - 1, ny - 1, ny - 1, 0, 0), label=3),
+    ]
+    scheme = dict(mix=0, inviscid_scheme=3, reconstruction=2, limiter=0, viscous_order=0, cfl=0.5,
+                  fixed_time_step=0, output_screen=1)
+    return dict(name="c1", mx=nx, my=ny, mz=1, ngg=2, x=x, y=y, z=z, setup=setup, bcs=bcs, scheme=scheme,
+                inflow=inflow, seed=seed, perturb="density", steps=steps)
+
+
+def c2_grid(nx, ny, nz, lx=2.0, ly=1.0, lz=1.0, ratio=1.05, amp=0.05):
+    """Curvilinear 3-D grid: wall-normal (j) geometric stretching by `ratio`,
+    then x' = x + amp sin(2 pi y) sin(2 pi z)."""
+    xi = np.linspace(0.0, lx, nx)
+    steps = ratio ** np.arange(ny - 1)
+    yj = np.concatenate([[0.0], np.cumsum(steps)])
+    yj *= ly / yj[-1]
+    zk = np.linspace(0.0, lz, nz)
+    Z, Y, X = np.meshgrid(zk, yj, xi, indexing="ij")  # (nz, ny, nx)
+    X = X + amp * np.sin(2 * np.pi * Y) * np.sin(2 * np.pi * Z)
+    return X.copy(), Y.copy(), Z.copy()
+
+
+def c2_case(nx=256, ny=128, nz=128, seed=1002, steps=20, mach=2.5, t_inf=300.0, re_unit=1e5, t_wall=300.0,
+            reconstruction=4, inviscid_scheme=2, perturb=True):
+    """3-D laminar Navier-Stokes in air on the sinusoidally mapped,
+    wall-normal-stretched curvilinear grid; Roe + AWENO-5 requested
+    (the reference runs AUSM+ for that pair), SSP-RK3 at CFL 0.5.  Mach 2.5,
+    Re = 1e5 per metre at T = 300 K (Sutherland viscosity), isothermal no-slip
+    wall at 300 K on j=0; supersonic inflow i=0, outflow i=max and j=max,
+    symmetry k=0 and k=max."""
+    x, y, z = c2_grid(nx, ny, nz)
+    c = np.sqrt(GAMMA_AIR * R_U / MW_AIR * t_inf)
+    u = mach * c
+    rho = re_unit * sutherland(t_inf) / u
+    p = rho * R_U / MW_AIR * t_inf
+    inflow = air_inflow(mach, p, t_inf)
+    setup = _setup_common({
+        "int dimension": 3, "real cfl": 0.5, "int inviscid_scheme": inviscid_scheme,
+        "int reconstruction": reconstruction, "int limiter": 0, "int viscous_order": 2, "int total_step": steps,
+        "array string boundary_conditions": ["freestream", "outflow", "symmetry", "wall"],
+        "string reference_state": "freestream", "string default_init": "freestream",
+        "struct freestream": [("string", "type", "inflow"), ("int", "label", 5), ("real", "mach", mach),
+                              ("real", "pressure", p), ("real", "temperature", t_inf), ("real", "u", 1.0),
+                              ("real", "v", 0.0), ("real", "w", 0.0)],
+        "struct outflow": [("string", "type", "outflow"), ("int", "label", 6)],
+        "struct symmetry": [("string", "type", "symmetry"), ("int", "label", 3)],
+        "struct wall": [("string", "type", "wall"), ("int", "label", 2), ("string", "thermal_type", "isothermal"),
+                        ("real", "temperature", t_wall)],
+    })
+    bcs = [
+        dict(type=5, raw=(0, 0, 0, ny - 1, 0, nz - 1), label=5),
+        dict(type=6, raw=(nx - 1, nx - 1, 0, ny - 1, 0, nz - 1), label=6),
+        dict(type=2, raw=(0, nx - 1, 0, 0, 0, nz - 1), label=2, t_wall=t_wall, isothermal=True),
+        dict(type=6, raw=(0, nx - 1, ny - 1, ny - 1, 0, nz - 1), label=6),
+        dict(type=3, raw=(0, nx - 1, 0, ny - 1, 0, 0), label=3),
+        dict(type=3, raw=(0, nx - 1, 0, ny - 1, nz - 1, nz - 1), label=3),
+    ]
+    scheme = dict(mix=0, inviscid_scheme=inviscid_scheme, reconstruction=reconstruction, limiter=0, viscous_order=2,
+                  cfl=0.5, fixed_time_step=0, output_screen=1)
+    return dict(name="c2", mx=nx, my=ny, mz=nz, ngg=3 if reconstruction == 4 else 2, x=x, y=y, z=z, setup=setup,
+                bcs=bcs, scheme=scheme, inflow=inflow, seed=seed, perturb="density" if perturb else None,
+                steps=steps)
+
+
+def initial_fields(case):
+    """Uniform inflow everywhere (ghosts included), then the seeded interior
+    density perturbation at fixed pressure.  Returns bv (6, ...) and sv
+    (n_scalar, ...) in the ghosted reference layout."""
+    mx, my, mz, g = case["mx"], case["my"], case["mz"], case["ngg"]
+    shp = ghost_shape(mx, my, mz, g)
+    st = case["inflow"]
+    bv = np.empty((6,) + shp)
+    for l, key in enumerate(("rho", "u", "v", "w", "p", "t")):
+        bv[l] = st[key]
+    nsc = len(st.get("sv", []))
+    sv = np.empty((nsc,) + shp)
+    for l in range(nsc):
+        sv[l] = st["sv"][l]
+    if case.get("perturb") == "density":
+        rng = SplitMix64(case["seed"])
+        n = mx * my * mz
+        f = 1.0 + 0.01 * rng.normal(n).reshape(mz, my, mx)
+        sl = (slice(g, g + mz), slice(g, g + my), slice(g, g + mx))
+        rho = bv[0][sl] * f
+        bv[0][sl] = rho
+        mw = st.get("mw", MW_AIR)
+        bv[5][sl] = bv[4][sl] * mw / (rho * R_U)
+    return bv, sv
+
+
+def chem_batch_inputs(n, seed=1004, species=("H2", "H", "O2", "O", "OH", "HO2", "H2O2", "H2O", "N2"), mw=None):
+    """Chemistry-only batch: T ~ U(900, 2400) K, p ~ logU(0.5, 5) atm,
+    equivalence ratio ln(phi) ~ N(ln 0.9487, 0.576) clipped to [0.3, 3],
+    H2/O2/N2 from phi with air O2:N2 = 0.233:0.767 by mass, radicals
+    H/O/OH/HO2 ~ logU(1e-6, 1e-3) mass fraction, H2O = H2O2 = 0, renormalised;
+    rho from the equation of state.  Returns (rho, T, p, Y[ns, n])."""
+    rng = SplitMix64(seed)
+    t = 900.0 + 1500.0 * rng.uniform(n)
+    p = 101325.0 * np.exp(np.log(0.5) + (np.log(5.0) - np.log(0.5)) * rng.uniform(n))
+    phi = np.exp(np.log(0.9487) + 0.576 * rng.normal(n))
+    phi = np.clip(phi, 0.3, 3.0)
+    idx = {s: i for i, s in enumerate(species)}
+    ns = len(species)
+    Y = np.zeros((ns, n))
+    # stoichiometric H2/O2 mass ratio: 2 * 2.016 / 31.998
+    st = 2 * 2.016 / 31.998
+    y_air = 1.0 / (1.0 + phi * st * (1.0 / 0.233))
+    y_h2 = 1.0 - y_air
+    Y[idx["H2"]] = y_h2
+    Y[idx["O2"]] = 0.233 * y_air
+    Y[idx["N2"]] = 0.767 * y_air
+    for s in ("H", "O", "OH", "HO2"):
+        Y[idx[s]] = np.exp(np.log(1e-6) + (np.log(1e-3) - np.log(1e-6)) * rng.uniform(n))
+    Y /= Y.sum(axis=0, keepdims=True)
+    if mw is None:
+        mw = np.array([2.016, 1.008, 31.998, 15.999, 17.007, 33.006, 34.014, 18.015, 28.014])
+    mwinv = (Y / mw[:, None]).sum(axis=0)
+    rho = p / (R_U * t * mwinv)
+    return rho, t, p, Y
+
+
+def _fmt(v):
+    if isinstance(v, float):
+        return repr(v)
+    return str(v)
+
+
+def write_case_dir(case, root):
+    """Write a reference-format case directory for `case` under `root`."""
+    setup_dir = os.path.join(root, "input_files", "setup")
+    bc_dir = os.path.join(root, "input_files", "boundary_condition")
+    os.makedirs(setup_dir, exist_ok=True)
+    os.makedirs(bc_dir, exist_ok=True)
+    chem = os.path.join(root, "input_files", "chemistry")
+    if not os.path.exists(chem):
+        shutil.copytree(CHEM_DIR, chem)
+    lines = []
+    for key, val in case["setup"].items():
+        kind, name = key.split(" ", 1)
+        if kind == "struct":
+            lines.append(f"struct {name} {{")
+            for t, k, v in val:
+                lines.append(f"    {t} {k} = {_fmt(v)}")
+            lines.append("}")
+        elif kind == "array":
+            t, nm = name.split(" ", 1)
+            lines.append(f"array {t} {nm} {{")
+            lines.append("    " + " ".join(_fmt(v) for v in val))
+            lines.append("}")
+        else:
+            lines.append(f"{kind} {name} = {_fmt(val)}")
+    with open(os.path.join(setup_dir, "default_settings.txt"), "w") as f:
+        f.write("// generated case: " + case["name"] + "\n" + "\n".join(lines) + "\n")
+    with open(os.path.join(bc_dir, "boundary   0.txt"), "w") as f:
+        f.write(f"{len(case['bcs']):>12d}\n")
+        for b in case["bcs"]:
+            r = b["raw"]
+            f.write("".join(f"{v:>6d}" for v in r) + f"{b['label']:>6d}\n")
+    return root

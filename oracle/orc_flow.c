@@ -1,0 +1,1400 @@
+/*
+ * OpenSTC CPU oracle, part 2: grid metrics, boundary conditions, inviscid
+ * (MUSCL/NND/1st + AUSM+/HLLC/LF, AWENO-5 + CDS) and viscous fluxes, the
+ * CFL time step, the SSP-RK3 update and the residual.  TEST INFRASTRUCTURE
+ * ONLY (see ostc_oracle.h).
+ */
+#include "ostc_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define R_U 8314.462618     /* Constants.h:17 */
+#define GAMMA_AIR 1.4       /* Constants.h:20 */
+#define MW_AIR 28.9647      /* Constants.h:25 */
+
+typedef struct {
+  int g;
+  int64_t d2, d1, sz;
+} lay;
+
+static lay mklay(int mx, int my, int mz, int g) {
+  lay L;
+  L.g = g;
+  L.d2 = mx + 2 * g;
+  L.d1 = (int64_t)(my + 2 * g) * L.d2;
+  L.sz = L.d1 * (mz + 2 * g);
+  return L;
+}
+
+static inline int64_t at(lay L, int i, int j, int k) {
+  return (int64_t)(k + L.g) * L.d1 + (int64_t)(j + L.g) * L.d2 + (i + L.g);
+}
+
+struct orc_zone {
+  int mx, my, mz, g, nv, ns, nsc, dim;
+  orc_cfg cfg;
+  orc_mech mech;
+  lay Lc, Lg, L0;
+  double *x, *y, *z, *jac, *metric;
+  double *bv, *sv, *cv, *qn, *vel, *acoustic, *mach, *mul, *cond, *rhoD, *gam, *cp;
+  double *dq, *dt_local, *bv_last, *chem_jac, *inv_sr, *visc_sr;
+  /* registered boundaries (range after Boundary::register_boundary + wall trim) */
+  int nbc;
+  int bface[ORC_MAXBC], bdir[ORC_MAXBC], rs[ORC_MAXBC][3], re[ORC_MAXBC][3];
+  int order[ORC_MAXBC];
+  double dt, t_phys;
+  int step;
+  double res[4], res_scale[4];
+  int inviscid_type;
+};
+
+#define M9(zz, p, r, c) ((zz)->metric[(p) * 9 + ((r) - 1) * 3 + ((c) - 1)])
+
+/* ------------------------------------------------------------------ grid */
+
+/* Mesh.cpp:790-944 init_ghost_grid (single block, no inner/parallel faces):
+ * reflected ghost nodes, 2-D k-ghosts at z0+k, parallelogram edges/corners. */
+static void ghost_grid(int mx, int my, int mz, int ngg, int dim, lay L, double *X, double *Y, double *Z) {
+  double *C[3] = {X, Y, Z};
+  for (int c = 0; c < 3; ++c) {
+    double *A = C[c];
+#define A_(i, j, k) A[at(L, i, j, k)]
+    for (int k = 0; k < mz; ++k)
+      for (int j = 0; j < my; ++j) {
+        for (int i = -1; i >= -ngg - 1; --i) A_(i, j, k) = 2 * A_(0, j, k) - A_(-i, j, k);
+        for (int i = mx; i < mx + ngg + 1; ++i) A_(i, j, k) = 2 * A_(mx - 1, j, k) - A_(2 * (mx - 1) - i, j, k);
+      }
+    for (int k = 0; k < mz; ++k)
+      for (int i = 0; i < mx; ++i) {
+        for (int j = -1; j >= -ngg - 1; --j) A_(i, j, k) = 2 * A_(i, 0, k) - A_(i, -j, k);
+        for (int j = my; j < my + ngg + 1; ++j) A_(i, j, k) = 2 * A_(i, my - 1, k) - A_(i, 2 * (my - 1) - j, k);
+      }
+    if (dim == 3) {
+      for (int j = 0; j < my; ++j)
+        for (int i = 0; i < mx; ++i) {
+          for (int k = -1; k >= -ngg - 1; --k) A_(i, j, k) = 2 * A_(i, j, 0) - A_(i, j, -k);
+          for (int k = mz; k < mz + ngg + 1; ++k) A_(i, j, k) = 2 * A_(i, j, mz - 1) - A_(i, j, 2 * (mz - 1) - k);
+        }
+    } else {
+      for (int j = 0; j < my; ++j)
+        for (int i = 0; i < mx; ++i) {
+          for (int k = -1; k >= -ngg - 1; --k) A_(i, j, k) = A_(i, j, 0) + (c == 2 ? k : 0);
+          for (int k = mz; k < mz + ngg + 1; ++k) A_(i, j, k) = A_(i, j, mz - 1) + (c == 2 ? k - mz + 1 : 0);
+        }
+    }
+    for (int i = 1; i <= ngg + 1; ++i)
+      for (int j = 1; j <= ngg + 1; ++j)
+        for (int k = 0; k < mz; ++k) {
+          const int i1 = -i, j1 = -j, i2 = mx - 1 + i, j2 = my - 1 + j;
+          A_(i1, j1, k) = A_(i1, 0, k) + A_(0, j1, k) - A_(0, 0, k);
+          A_(i2, j2, k) = A_(i2, my - 1, k) + A_(mx - 1, j2, k) - A_(mx - 1, my - 1, k);
+          A_(i1, j2, k) = A_(i1, my - 1, k) + A_(0, j2, k) - A_(0, my - 1, k);
+          A_(i2, j1, k) = A_(i2, 0, k) + A_(mx - 1, j1, k) - A_(mx - 1, 0, k);
+        }
+    for (int i = 1; i <= ngg + 1; ++i)
+      for (int j = 0; j < my; ++j)
+        for (int k = 1; k <= ngg + 1; ++k) {
+          const int i1 = -i, k1 = -k, i2 = mx - 1 + i, k2 = mz - 1 + k;
+          A_(i1, j, k1) = A_(i1, j, 0) + A_(0, j, k1) - A_(0, j, 0);
+          A_(i2, j, k2) = A_(i2, j, mz - 1) + A_(mx - 1, j, k2) - A_(mx - 1, j, mz - 1);
+          A_(i1, j, k2) = A_(i1, j, mz - 1) + A_(0, j, k2) - A_(0, j, mz - 1);
+          A_(i2, j, k1) = A_(i2, j, 0) + A_(mx - 1, j, k1) - A_(mx - 1, j, 0);
+        }
+    for (int i = -ngg - 1; i <= mx + ngg; ++i)
+      for (int j = 1; j <= ngg + 1; ++j)
+        for (int k = 1; k <= ngg + 1; ++k) {
+          const int j1 = -j, k1 = -k, j2 = my - 1 + j, k2 = mz - 1 + k;
+          A_(i, j1, k1) = A_(i, j1, 0) + A_(i, 0, k1) - A_(i, 0, 0);
+          A_(i, j2, k2) = A_(i, j2, mz - 1) + A_(i, my - 1, k2) - A_(i, my - 1, mz - 1);
+          A_(i, j1, k2) = A_(i, j1, mz - 1) + A_(i, 0, k2) - A_(i, 0, mz - 1);
+          A_(i, j2, k1) = A_(i, j2, 0) + A_(i, my - 1, k1) - A_(i, my - 1, 0);
+        }
+#undef A_
+  }
+}
+
+/* one point of Mesh.cpp:259-521: central differences of x,y,z; J and the
+ * metrics d(xi_r)/d(x_c).  Returns 0 when J <= 0. */
+static int metric_point(lay Lc, const double *X, const double *Y, const double *Z, int i, int j, int k, double *jac,
+                        double *m) {
+#define P(A, a, b, c) A[at(Lc, a, b, c)]
+  const double dxd1 = (P(X, i + 1, j, k) - P(X, i - 1, j, k)) * 0.5, dyd1 = (P(Y, i + 1, j, k) - P(Y, i - 1, j, k)) * 0.5,
+               dzd1 = (P(Z, i + 1, j, k) - P(Z, i - 1, j, k)) * 0.5;
+  const double dxd2 = (P(X, i, j + 1, k) - P(X, i, j - 1, k)) * 0.5, dyd2 = (P(Y, i, j + 1, k) - P(Y, i, j - 1, k)) * 0.5,
+               dzd2 = (P(Z, i, j + 1, k) - P(Z, i, j - 1, k)) * 0.5;
+  const double dxd3 = (P(X, i, j, k + 1) - P(X, i, j, k - 1)) * 0.5, dyd3 = (P(Y, i, j, k + 1) - P(Y, i, j, k - 1)) * 0.5,
+               dzd3 = (P(Z, i, j, k + 1) - P(Z, i, j, k - 1)) * 0.5;
+#undef P
+  const double J = dxd1 * dyd2 * dzd3 + dxd2 * dyd3 * dzd1 + dxd3 * dyd1 * dzd2 - dxd1 * dyd3 * dzd2 -
+                   dxd2 * dyd1 * dzd3 - dxd3 * dyd2 * dzd1;
+  if (J <= 0) return 0;
+  *jac = J;
+  m[0] = (dyd2 * dzd3 - dyd3 * dzd2) / J;
+  m[1] = (dxd3 * dzd2 - dxd2 * dzd3) / J;
+  m[2] = (dxd2 * dyd3 - dxd3 * dyd2) / J;
+  m[3] = (dyd3 * dzd1 - dyd1 * dzd3) / J;
+  m[4] = (dxd1 * dzd3 - dxd3 * dzd1) / J;
+  m[5] = (dxd3 * dyd1 - dxd1 * dyd3) / J;
+  m[6] = (dyd1 * dzd2 - dyd2 * dzd1) / J;
+  m[7] = (dxd2 * dzd1 - dxd1 * dzd2) / J;
+  m[8] = (dxd1 * dyd2 - dxd2 * dyd1) / J;
+  return 1;
+}
+
+/* Mesh.cpp:259-521 compute_jac_metric: interior, then the six ghost slabs in
+ * the reference's order; a non-positive J copies the inner neighbour. */
+static int metrics(int mx, int my, int mz, int ngg, lay Lc, lay L, const double *X, const double *Y, const double *Z,
+                   double *jac, double *met) {
+#define DO(i, j, k, di, dj, dk)                                                              \
+  do {                                                                                       \
+    const int64_t p_ = at(L, i, j, k);                                                       \
+    if (!metric_point(Lc, X, Y, Z, i, j, k, &jac[p_], &met[p_ * 9])) {                       \
+      const int64_t q_ = at(L, (i) + (di), (j) + (dj), (k) + (dk));                          \
+      jac[p_] = jac[q_];                                                                     \
+      memcpy(&met[p_ * 9], &met[q_ * 9], 9 * sizeof(double));                                \
+    }                                                                                        \
+  } while (0)
+  for (int i = 1; i < mx - 1; ++i)
+    for (int j = 1; j < my - 1; ++j) {
+      int kmin = 1, kmax = mz - 1;
+      if (mz == 1) {
+        kmin = 0;
+        kmax = 1;
+      }
+      for (int k = kmin; k < kmax; ++k) {
+        const int64_t p = at(L, i, j, k);
+        if (!metric_point(Lc, X, Y, Z, i, j, k, &jac[p], &met[p * 9])) return -1;
+      }
+    }
+  for (int i = 1; i < mx - 1; ++i)
+    for (int j = 1; j < my - 1; ++j)
+      for (int k = 0; k >= -ngg; --k) DO(i, j, k, 0, 0, 1);
+  for (int i = 1; i < mx - 1; ++i)
+    for (int j = 1; j < my - 1; ++j)
+      for (int k = mz - 1; k < mz + ngg; ++k) DO(i, j, k, 0, 0, -1);
+  for (int i = 1; i < mx - 1; ++i)
+    for (int j = 0; j >= -ngg; --j)
+      for (int k = -ngg; k < mz + ngg; ++k) DO(i, j, k, 0, 1, 0);
+  for (int i = 1; i < mx - 1; ++i)
+    for (int j = my - 1; j < my + ngg; ++j)
+      for (int k = -ngg; k < mz + ngg; ++k) DO(i, j, k, 0, -1, 0);
+  for (int i = 0; i >= -ngg; --i)
+    for (int j = -ngg; j < my + ngg; ++j)
+      for (int k = -ngg; k < mz + ngg; ++k) DO(i, j, k, 1, 0, 0);
+  for (int i = mx - 1; i < mx + ngg; ++i)
+    for (int j = -ngg; j < my + ngg; ++j)
+      for (int k = -ngg; k < mz + ngg; ++k) DO(i, j, k, -1, 0, 0);
+#undef DO
+  return 0;
+}
+
+void orc_grid_metrics(int mx, int my, int mz, int ngg, const double *x, const double *y, const double *z, double *xg,
+                      double *yg, double *zg, double *jac, double *metric) {
+  lay Lc = mklay(mx, my, mz, ngg + 1), L = mklay(mx, my, mz, ngg);
+  const int dim = mz == 1 ? 2 : 3;
+  for (int k = 0; k < mz; ++k)
+    for (int j = 0; j < my; ++j)
+      for (int i = 0; i < mx; ++i) {
+        const int64_t s = ((int64_t)k * my + j) * mx + i;
+        xg[at(Lc, i, j, k)] = x[s];
+        yg[at(Lc, i, j, k)] = y[s];
+        zg[at(Lc, i, j, k)] = z[s];
+      }
+  ghost_grid(mx, my, mz, ngg, dim, Lc, xg, yg, zg);
+  metrics(mx, my, mz, ngg, Lc, L, xg, yg, zg, jac, metric);
+}
+
+/* --------------------------------------------------------------- zone */
+
+static void *cal(int64_t n) { return calloc((size_t)(n > 0 ? n : 1), sizeof(double)); }
+
+orc_zone *orc_zone_create(int mx, int my, int mz, int ngg, const orc_cfg *cfg, const orc_mech *mech, const double *x,
+                          const double *y, const double *z) {
+  orc_zone *zz = (orc_zone *)calloc(1, sizeof(orc_zone));
+  zz->mx = mx;
+  zz->my = my;
+  zz->mz = mz;
+  zz->g = ngg;
+  zz->dim = mz == 1 ? 2 : 3;
+  zz->cfg = *cfg;
+  if (mech) zz->mech = *mech;
+  zz->ns = cfg->mix ? zz->mech.n_spec : 0;
+  zz->nsc = zz->ns;
+  zz->nv = 5 + zz->nsc;
+  zz->Lc = mklay(mx, my, mz, ngg + 1);
+  zz->Lg = mklay(mx, my, mz, ngg);
+  zz->L0 = mklay(mx, my, mz, 0);
+  const int64_t sg = zz->Lg.sz, s0 = zz->L0.sz, sc = zz->Lc.sz;
+  zz->x = cal(sc);
+  zz->y = cal(sc);
+  zz->z = cal(sc);
+  zz->jac = cal(sg);
+  zz->metric = cal(sg * 9);
+  zz->bv = cal(sg * 6);
+  zz->sv = cal(sg * (zz->nsc ? zz->nsc : 1));
+  zz->cv = cal(sg * zz->nv);
+  zz->qn = cal(sg * zz->nv);
+  zz->vel = cal(sg);
+  zz->acoustic = cal(sg);
+  zz->mach = cal(sg);
+  zz->mul = cal(sg);
+  zz->cond = cal(sg);
+  zz->rhoD = cal(sg * (zz->ns ? zz->ns : 1));
+  zz->gam = cal(sg);
+  zz->cp = cal(sg);
+  zz->dq = cal(s0 * zz->nv);
+  zz->dt_local = cal(s0);
+  zz->bv_last = cal(s0 * 4);
+  zz->chem_jac = cal(s0 * (zz->ns ? zz->ns * zz->ns : 1));
+  zz->inv_sr = cal(s0 * 3);
+  zz->visc_sr = cal(s0);
+  orc_grid_metrics(mx, my, mz, ngg, x, y, z, zz->x, zz->y, zz->z, zz->jac, zz->metric);
+  /* inviscid_type: Parameter.cpp:165-219 deduce_known_info */
+  zz->inviscid_type = 0;
+  if (cfg->inviscid_scheme == 2) zz->inviscid_type = 2;
+  if (cfg->reconstruction == 4 || cfg->reconstruction == 5) zz->inviscid_type = 1;
+  /* boundaries: Mesh.cpp:14-31 register_boundary, BoundCond.cu:369-384 wall trim */
+  zz->nbc = cfg->n_bc;
+  for (int b = 0; b < cfg->n_bc; ++b) {
+    const orc_bc *B = &cfg->bc[b];
+    int s[3] = {B->raw[0], B->raw[2], B->raw[4]}, e[3] = {B->raw[1], B->raw[3], B->raw[5]};
+    int face = 0, dir = 1;
+    for (int d = 0; d < zz->dim; ++d)
+      if (s[d] == e[d]) {
+        face = d;
+        if (s[d] == 0) dir = -1;
+        break;
+      }
+    for (int d = 0; d < zz->dim; ++d) {
+      if (d == face) continue;
+      if (e[d] < s[d]) {
+        int t = e[d];
+        e[d] = s[d];
+        s[d] = t;
+      }
+      s[d] -= ngg;
+      e[d] += ngg;
+    }
+    if (zz->dim == 2) {
+      s[2] = -ngg;
+      e[2] = ngg;
+    }
+    if (B->type == 2)
+      for (int q = 0; q < 3; ++q) {
+        if (q == face) continue;
+        s[q] += ngg;
+        e[q] -= ngg;
+      }
+    zz->bface[b] = face;
+    zz->bdir[b] = dir;
+    for (int d = 0; d < 3; ++d) {
+      zz->rs[b][d] = s[d];
+      zz->re[b][d] = e[d];
+    }
+  }
+  /* application order: BoundCond.cuh:673-822 (outflow, inflow, symmetry, wall;
+   * file order within one type) */
+  const int prio[4] = {6, 5, 3, 2};
+  int n = 0;
+  for (int p = 0; p < 4; ++p)
+    for (int b = 0; b < cfg->n_bc; ++b)
+      if (cfg->bc[b].type == prio[p]) zz->order[n++] = b;
+  zz->nbc = n;
+  return zz;
+}
+
+void orc_zone_destroy(orc_zone *zz) {
+  if (!zz) return;
+  double *ptrs[] = {zz->x, zz->y, zz->z, zz->jac, zz->metric, zz->bv, zz->sv, zz->cv, zz->qn, zz->vel, zz->acoustic,
+                    zz->mach, zz->mul, zz->cond, zz->rhoD, zz->gam, zz->cp, zz->dq, zz->dt_local, zz->bv_last,
+                    zz->chem_jac, zz->inv_sr, zz->visc_sr};
+  for (size_t i = 0; i < sizeof(ptrs) / sizeof(ptrs[0]); ++i) free(ptrs[i]);
+  free(zz);
+}
+
+double *orc_zone_field(orc_zone *zz, int id, int64_t *count) {
+  const int64_t sg = zz->Lg.sz, s0 = zz->L0.sz, sc = zz->Lc.sz;
+  switch (id) {
+    case ORC_X: *count = sc; return zz->x;
+    case ORC_Y: *count = sc; return zz->y;
+    case ORC_Z: *count = sc; return zz->z;
+    case ORC_JAC: *count = sg; return zz->jac;
+    case ORC_METRIC: *count = sg * 9; return zz->metric;
+    case ORC_BV: *count = sg * 6; return zz->bv;
+    case ORC_SV: *count = sg * zz->nsc; return zz->sv;
+    case ORC_CV: *count = sg * zz->nv; return zz->cv;
+    case ORC_DQ: *count = s0 * zz->nv; return zz->dq;
+    case ORC_VEL: *count = sg; return zz->vel;
+    case ORC_ACOUSTIC: *count = sg; return zz->acoustic;
+    case ORC_MACH: *count = sg; return zz->mach;
+    case ORC_MUL: *count = sg; return zz->mul;
+    case ORC_COND: *count = sg; return zz->cond;
+    case ORC_RHOD: *count = sg * zz->ns; return zz->rhoD;
+    case ORC_GAMMA: *count = sg; return zz->gam;
+    case ORC_CP: *count = sg; return zz->cp;
+    case ORC_DT_LOCAL: *count = s0; return zz->dt_local;
+    case ORC_QN: *count = sg * zz->nv; return zz->qn;
+    case ORC_BV_LAST: *count = s0 * 4; return zz->bv_last;
+    case ORC_CHEM_JAC: *count = s0 * zz->ns * zz->ns; return zz->chem_jac;
+  }
+  *count = 0;
+  return NULL;
+}
+
+void orc_zone_state(orc_zone *zz, double *t_phys, int *step) {
+  *t_phys = zz->t_phys;
+  *step = zz->step;
+}
+
+#define BV(l, i, j, k) zz->bv[(l) * zz->Lg.sz + at(zz->Lg, i, j, k)]
+#define SV(l, i, j, k) zz->sv[(l) * zz->Lg.sz + at(zz->Lg, i, j, k)]
+#define CV(l, i, j, k) zz->cv[(l) * zz->Lg.sz + at(zz->Lg, i, j, k)]
+#define QN(l, i, j, k) zz->qn[(l) * zz->Lg.sz + at(zz->Lg, i, j, k)]
+#define DQ(l, i, j, k) zz->dq[(l) * zz->L0.sz + at(zz->L0, i, j, k)]
+#define S1(a, i, j, k) zz->a[at(zz->Lg, i, j, k)]
+
+/* FieldOperation.cuh:18-41 compute_total_energy (also sets |V|) */
+static void total_energy(orc_zone *zz, int i, int j, int k) {
+  const double u = BV(1, i, j, k), v = BV(2, i, j, k), w = BV(3, i, j, k);
+  const double V2 = u * u + v * v + w * w;
+  double te = 0.5 * V2;
+  if (zz->cfg.mix) {
+    double h[ORC_MAXS];
+    orc_enthalpy(BV(5, i, j, k), &zz->mech, h);
+    for (int l = 0; l < zz->ns; ++l) te += h[l] * SV(l, i, j, k);
+    te *= BV(0, i, j, k);
+    te -= BV(4, i, j, k);
+  } else {
+    te *= BV(0, i, j, k);
+    te += BV(4, i, j, k) / (GAMMA_AIR - 1);
+  }
+  CV(4, i, j, k) = te;
+  S1(vel, i, j, k) = sqrt(V2);
+}
+
+/* FieldOperation.cuh:83-110 compute_cv_from_bv_1_point */
+static void cv_from_bv_point(orc_zone *zz, int i, int j, int k) {
+  const double rho = BV(0, i, j, k);
+  CV(0, i, j, k) = rho;
+  CV(1, i, j, k) = rho * BV(1, i, j, k);
+  CV(2, i, j, k) = rho * BV(2, i, j, k);
+  CV(3, i, j, k) = rho * BV(3, i, j, k);
+  for (int l = 0; l < zz->nsc; ++l) CV(5 + l, i, j, k) = rho * SV(l, i, j, k);
+  total_energy(zz, i, j, k);
+}
+
+/* ------------------------------------------------------------ boundaries */
+
+/* BoundCond.cuh:52-118 apply_symmetry (never refreshes ghost cv) */
+static void bc_symmetry(orc_zone *zz, int b, int i, int j, int k) {
+  const int face = zz->bface[b];
+  int dir[3] = {0, 0, 0};
+  dir[face] = zz->bdir[b];
+  const int ii0 = i - dir[0], ij0 = j - dir[1], ik0 = k - dir[2];
+  const double *m = &zz->metric[at(zz->Lg, i, j, k) * 9];
+  double kx = m[face * 3 + 0], ky = m[face * 3 + 1], kz = m[face * 3 + 2];
+  const double km = sqrt(kx * kx + ky * ky + kz * kz);
+  kx /= km;
+  ky /= km;
+  kz /= km;
+  const double u1 = BV(1, ii0, ij0, ik0), v1 = BV(2, ii0, ij0, ik0), w1 = BV(3, ii0, ij0, ik0);
+  double u_k = kx * u1 + ky * v1 + kz * w1;
+  const double ut = u1 - kx * u_k, vt = v1 - ky * u_k, wt = w1 - kz * u_k;
+  BV(1, i, j, k) = ut;
+  BV(2, i, j, k) = vt;
+  BV(3, i, j, k) = wt;
+  S1(vel, i, j, k) = sqrt(ut * ut + vt * vt + wt * wt);
+  BV(0, i, j, k) = BV(0, ii0, ij0, ik0);
+  BV(4, i, j, k) = BV(4, ii0, ij0, ik0);
+  BV(5, i, j, k) = BV(5, ii0, ij0, ik0);
+  for (int l = 0; l < zz->nsc; ++l) SV(l, i, j, k) = SV(l, ii0, ij0, ik0);
+  for (int g = 1; g <= zz->g; ++g) {
+    const int gi = i + g * dir[0], gj = j + g * dir[1], gk = k + g * dir[2];
+    const int ii = i - g * dir[0], ij = j - g * dir[1], ik = k - g * dir[2];
+    BV(0, gi, gj, gk) = BV(0, ii, ij, ik);
+    const double u = BV(1, ii, ij, ik), v = BV(2, ii, ij, ik), w = BV(3, ii, ij, ik);
+    u_k = kx * u + ky * v + kz * w;
+    BV(1, gi, gj, gk) = u - 2 * u_k * kx;
+    BV(2, gi, gj, gk) = v - 2 * u_k * ky;
+    BV(3, gi, gj, gk) = w - 2 * u_k * kz;
+    S1(vel, gi, gj, gk) = sqrt(BV(1, gi, gj, gk) * BV(1, gi, gj, gk) + BV(2, gi, gj, gk) * BV(2, gi, gj, gk) +
+                               BV(3, gi, gj, gk) * BV(3, gi, gj, gk));
+    BV(4, gi, gj, gk) = BV(4, ii, ij, ik);
+    BV(5, gi, gj, gk) = BV(5, ii, ij, ik);
+    for (int l = 0; l < zz->nsc; ++l) SV(l, gi, gj, gk) = SV(l, ii, ij, ik);
+  }
+}
+
+/* BoundCond.cuh:120-148 apply_outflow */
+static void bc_outflow(orc_zone *zz, int b, int i, int j, int k, int with_cv) {
+  int dir[3] = {0, 0, 0};
+  dir[zz->bface[b]] = zz->bdir[b];
+  for (int g = 1; g <= zz->g; ++g) {
+    const int gi = i + g * dir[0], gj = j + g * dir[1], gk = k + g * dir[2];
+    for (int l = 0; l < 6; ++l) BV(l, gi, gj, gk) = BV(l, i, j, k);
+    for (int l = 0; l < zz->nsc; ++l) SV(l, gi, gj, gk) = SV(l, i, j, k);
+    if (with_cv) cv_from_bv_point(zz, gi, gj, gk);
+  }
+}
+
+/* BoundCond.cuh:150-212 apply_inflow */
+static void bc_inflow(orc_zone *zz, const orc_bc *B, int b, int i, int j, int k, int with_cv) {
+  int dir[3] = {0, 0, 0};
+  dir[zz->bface[b]] = zz->bdir[b];
+  BV(0, i, j, k) = B->rho;
+  BV(1, i, j, k) = B->u;
+  BV(2, i, j, k) = B->v;
+  BV(3, i, j, k) = B->w;
+  BV(4, i, j, k) = B->p;
+  BV(5, i, j, k) = B->t;
+  for (int l = 0; l < zz->nsc; ++l) SV(l, i, j, k) = B->sv[l];
+  S1(vel, i, j, k) = B->vel;
+  if (with_cv) cv_from_bv_point(zz, i, j, k);
+  for (int g = 1; g <= zz->g; ++g) {
+    const int gi = i + g * dir[0], gj = j + g * dir[1], gk = k + g * dir[2];
+    BV(0, gi, gj, gk) = B->rho;
+    BV(1, gi, gj, gk) = B->u;
+    BV(2, gi, gj, gk) = B->v;
+    BV(3, gi, gj, gk) = B->w;
+    BV(4, gi, gj, gk) = B->p;
+    BV(5, gi, gj, gk) = B->t;
+    for (int l = 0; l < zz->nsc; ++l) SV(l, gi, gj, gk) = B->sv[l];
+    if (with_cv) cv_from_bv_point(zz, gi, gj, gk);
+  }
+}
+
+/* BoundCond.cuh:424-552 apply_wall (no-slip; isothermal ghost T = 2Tw - Ti,
+ * clipped at 0.1 Tw; rho from p and T) */
+static void bc_wall(orc_zone *zz, const orc_bc *B, int b, int i, int j, int k, int with_cv) {
+  int dir[3] = {0, 0, 0};
+  dir[zz->bface[b]] = zz->bdir[b];
+  double t_wall = B->t_wall;
+  const int i0 = i - dir[0], j0 = j - dir[1], k0 = k - dir[2];
+  if (!B->isothermal) t_wall = BV(5, i0, j0, k0);
+  const double p = BV(4, i0, j0, k0);
+  double mw = MW_AIR;
+  if (zz->cfg.mix) {
+    mw = 0;
+    for (int l = 0; l < zz->ns; ++l) {
+      SV(l, i, j, k) = SV(l, i0, j0, k0);
+      mw += SV(l, i, j, k) / zz->mech.mw[l];
+    }
+    mw = 1 / mw;
+  }
+  const double rho_wall = p * mw / (t_wall * R_U);
+  BV(0, i, j, k) = rho_wall;
+  BV(1, i, j, k) = 0;
+  BV(2, i, j, k) = 0;
+  BV(3, i, j, k) = 0;
+  BV(4, i, j, k) = p;
+  BV(5, i, j, k) = t_wall;
+  S1(vel, i, j, k) = 0;
+  if (with_cv) cv_from_bv_point(zz, i, j, k);
+  for (int g = 1; g <= zz->g; ++g) {
+    const int ai = i - g * dir[0], aj = j - g * dir[1], ak = k - g * dir[2];
+    const int gi = i + g * dir[0], gj = j + g * dir[1], gk = k + g * dir[2];
+    const double ui = BV(1, ai, aj, ak), vi = BV(2, ai, aj, ak), wi = BV(3, ai, aj, ak);
+    const double pi = BV(4, ai, aj, ak), ti = BV(5, ai, aj, ak);
+    double tg = ti;
+    if (B->isothermal) {
+      tg = 2 * t_wall - ti;
+      if (tg <= 0.1 * t_wall) tg = t_wall;
+    }
+    if (zz->cfg.mix) {
+      mw = 0;
+      for (int l = 0; l < zz->ns; ++l) {
+        SV(l, gi, gj, gk) = SV(l, ai, aj, ak);
+        mw += SV(l, gi, gj, gk) / zz->mech.mw[l];
+      }
+      mw = 1 / mw;
+    }
+    const double rho_g = pi * mw / (tg * R_U);
+    BV(0, gi, gj, gk) = rho_g;
+    BV(1, gi, gj, gk) = -ui;
+    BV(2, gi, gj, gk) = -vi;
+    BV(3, gi, gj, gk) = -wi;
+    BV(4, gi, gj, gk) = pi;
+    BV(5, gi, gj, gk) = tg;
+    if (with_cv) cv_from_bv_point(zz, gi, gj, gk);
+  }
+}
+
+/* BoundCond.cuh:673-822 apply_boundary_conditions: one type after another
+ * in priority order (a later one overwrites shared corner points). */
+static void apply_bcs(orc_zone *zz, int with_cv) {
+  for (int n = 0; n < zz->nbc; ++n) {
+    const int b = zz->order[n];
+    const orc_bc *B = &zz->cfg.bc[b];
+    for (int k = zz->rs[b][2]; k <= zz->re[b][2]; ++k)
+      for (int j = zz->rs[b][1]; j <= zz->re[b][1]; ++j)
+        for (int i = zz->rs[b][0]; i <= zz->re[b][0]; ++i) {
+          switch (B->type) {
+            case 3: bc_symmetry(zz, b, i, j, k); break;
+            case 5: bc_inflow(zz, B, b, i, j, k, with_cv); break;
+            case 6: bc_outflow(zz, b, i, j, k, with_cv); break;
+            case 2: bc_wall(zz, B, b, i, j, k, with_cv); break;
+            default: break;
+          }
+        }
+  }
+}
+
+/* FieldOperation.cuh:372-400 eliminate_k_gradient (2-D) */
+static void eliminate_k_gradient(orc_zone *zz, int with_cv) {
+  const int g = zz->g;
+  for (int j = -g; j < zz->my + g; ++j)
+    for (int i = -g; i < zz->mx + g; ++i)
+      for (int k = 1; k <= g; ++k) {
+        for (int l = 0; l < 6; ++l) {
+          BV(l, i, j, k) = BV(l, i, j, 0);
+          BV(l, i, j, -k) = BV(l, i, j, 0);
+        }
+        for (int l = 0; l < zz->nsc; ++l) {
+          SV(l, i, j, k) = SV(l, i, j, 0);
+          SV(l, i, j, -k) = SV(l, i, j, 0);
+        }
+        if (with_cv)
+          for (int l = 0; l < zz->nv; ++l) {
+            CV(l, i, j, k) = CV(l, i, j, 0);
+            CV(l, i, j, -k) = CV(l, i, j, 0);
+          }
+      }
+}
+
+/* FieldOperation.cuh:112-146 update_physical_properties on [-1, m] */
+static void update_properties(orc_zone *zz) {
+  for (int k = -1; k <= zz->mz; ++k)
+    for (int j = -1; j <= zz->my; ++j)
+      for (int i = -1; i <= zz->mx; ++i) {
+        const double t = BV(5, i, j, k);
+        if (zz->cfg.mix) {
+          double cp[ORC_MAXS], Y[ORC_MAXS];
+          orc_cp(t, &zz->mech, cp);
+          double mw = 0, cp_tot = 0, cvv = 0;
+          for (int l = 0; l < zz->ns; ++l) {
+            Y[l] = SV(l, i, j, k);
+            mw += Y[l] / zz->mech.mw[l];
+            cp_tot += Y[l] * cp[l];
+            cvv += Y[l] * (cp[l] - R_U / zz->mech.mw[l]);
+          }
+          mw = 1 / mw;
+          S1(cp, i, j, k) = cp_tot;
+          S1(gam, i, j, k) = cp_tot / cvv;
+          S1(acoustic, i, j, k) = sqrt(S1(gam, i, j, k) * R_U * t / mw);
+          double rhoD[ORC_MAXS];
+          orc_transport(&zz->mech, t, BV(4, i, j, k), BV(0, i, j, k), Y, cp, &S1(mul, i, j, k), &S1(cond, i, j, k),
+                        rhoD);
+          for (int l = 0; l < zz->ns; ++l) zz->rhoD[l * zz->Lg.sz + at(zz->Lg, i, j, k)] = rhoD[l];
+        } else {
+          const double c_temp = GAMMA_AIR * R_U / MW_AIR;
+          const double cp = GAMMA_AIR * R_U / MW_AIR / (GAMMA_AIR - 1);
+          S1(acoustic, i, j, k) = sqrt(c_temp * t);
+          S1(mul, i, j, k) = orc_sutherland(t);
+          S1(cond, i, j, k) = S1(mul, i, j, k) * cp / zz->cfg.Pr;
+        }
+        S1(mach, i, j, k) = S1(vel, i, j, k) / S1(acoustic, i, j, k);
+      }
+}
+
+/* Driver.cu:40-98 initialize_computation followed by RK.cuh:38-48
+ * (store_last_step, compute_cv_from_bv on every point). */
+void orc_zone_initialize(orc_zone *zz) {
+  const int g = zz->g;
+  if (zz->dim == 2) eliminate_k_gradient(zz, 0);
+  apply_bcs(zz, 0);
+  for (int k = -g; k < zz->mz + g; ++k)
+    for (int j = -g; j < zz->my + g; ++j)
+      for (int i = -g; i < zz->mx + g; ++i)
+        S1(vel, i, j, k) = sqrt(BV(1, i, j, k) * BV(1, i, j, k) + BV(2, i, j, k) * BV(2, i, j, k) +
+                                BV(3, i, j, k) * BV(3, i, j, k));
+  update_properties(zz);
+  for (int k = 0; k < zz->mz; ++k)
+    for (int j = 0; j < zz->my; ++j)
+      for (int i = 0; i < zz->mx; ++i) {
+        const int64_t p = at(zz->L0, i, j, k);
+        zz->bv_last[p] = BV(0, i, j, k);
+        zz->bv_last[zz->L0.sz + p] = S1(vel, i, j, k);
+        zz->bv_last[2 * zz->L0.sz + p] = BV(4, i, j, k);
+        zz->bv_last[3 * zz->L0.sz + p] = BV(5, i, j, k);
+      }
+  for (int k = -g; k < zz->mz + g; ++k)
+    for (int j = -g; j < zz->my + g; ++j)
+      for (int i = -g; i < zz->mx + g; ++i) cv_from_bv_point(zz, i, j, k);
+  zz->step = 0;
+  zz->t_phys = 0;
+}
+
+/* ---------------------------------------------------------- inviscid */
+
+/* gxl_lib/Math.hpp:13-15 sgn (sgn(0) = +1) and Limiter.cuh:13-16 minmod */
+static double minmod(double a, double b) {
+  const double sa = a < 0 ? -1 : 1, sb = b < 0 ? -1 : 1;
+  return 0.5 * (sa + sb) * fmin(fabs(a), fabs(b));
+}
+
+/* Reconstruction.cuh:20-72 + Reconstruction.cu:5-46.  pv is a line buffer of
+ * nv entries per point; ip is the left point of the face.  Appends E (and
+ * gamma for mixtures) at pv_l[nv], pv_l[nv+1]. */
+static void reconstruct(const orc_zone *zz, const double *pv, int ip, double *pl, double *pr) {
+  const int nv = zz->nv;
+  const double kappa = 1.0 / 3.0;
+  for (int l = 0; l < nv; ++l) {
+    const double q0 = pv[(ip - 1) * nv + l], q1 = pv[ip * nv + l], q2 = pv[(ip + 1) * nv + l],
+                 q3 = pv[(ip + 2) * nv + l];
+    switch (zz->cfg.reconstruction) {
+      case 2: {
+        const double di = q1 - q0, di1 = q2 - q1, di2 = q3 - q2;
+        const double dnl = minmod(di, di1), dpl = minmod(di1, di), dnr = minmod(di1, di2), dpr = minmod(di2, di1);
+        pl[l] = q1 + 0.25 * ((1 - kappa) * dnl + (1 + kappa) * dpl);
+        pr[l] = q2 - 0.25 * ((1 - kappa) * dpr + (1 + kappa) * dnr);
+        break;
+      }
+      case 3: {
+        const double di = q1 - q0, di1 = q2 - q1, di2 = q3 - q2;
+        pl[l] = q1 + 0.5 * minmod(di, di1);
+        pr[l] = q2 - 0.5 * minmod(di1, di2);
+        break;
+      }
+      default:
+        pl[l] = q1;
+        pr[l] = q2;
+    }
+  }
+  if (zz->cfg.mix) {
+    const orc_mech *m = &zz->mech;
+    double el = 0.5 * (pl[1] * pl[1] + pl[2] * pl[2] + pl[3] * pl[3]);
+    double er = 0.5 * (pr[1] * pr[1] + pr[2] * pr[2] + pr[3] * pr[3]);
+    double mil = 0, mir = 0;
+    for (int l = 0; l < zz->ns; ++l) {
+      mil += pl[5 + l] / m->mw[l];
+      mir += pr[5 + l] / m->mw[l];
+    }
+    const double tl = pl[4] / (pl[0] * R_U * mil), tr = pr[4] / (pr[0] * R_U * mir);
+    double hl[ORC_MAXS], hr[ORC_MAXS], cpl_i[ORC_MAXS], cpr_i[ORC_MAXS];
+    orc_enthalpy_and_cp(tl, m, hl, cpl_i);
+    orc_enthalpy_and_cp(tr, m, hr, cpr_i);
+    double cpl = 0, cpr = 0, cvl = 0, cvr = 0;
+    for (int l = 0; l < zz->ns; ++l) {
+      cpl += cpl_i[l] * pl[l + 5];
+      cpr += cpr_i[l] * pr[l + 5];
+      cvl += pl[l + 5] * (cpl_i[l] - R_U / m->mw[l]);
+      cvr += pr[l + 5] * (cpr_i[l] - R_U / m->mw[l]);
+      el += hl[l] * pl[l + 5];
+      er += hr[l] * pr[l + 5];
+    }
+    pl[nv] = pl[0] * el - pl[4];
+    pr[nv] = pr[0] * er - pr[4];
+    pl[nv + 1] = cpl / cvl;
+    pr[nv + 1] = cpr / cvr;
+  } else {
+    const double el = 0.5 * (pl[1] * pl[1] + pl[2] * pl[2] + pl[3] * pl[3]);
+    const double er = 0.5 * (pr[1] * pr[1] + pr[2] * pr[2] + pr[3] * pr[3]);
+    pl[nv] = el * pl[0] + pl[4] / (GAMMA_AIR - 1);
+    pr[nv] = er * pr[0] + pr[4] / (GAMMA_AIR - 1);
+  }
+}
+
+/* AusmP.cu:6-94 compute_ausmPlus_flux; met/jac are the line buffers, ip the left point. */
+static void flux_ausm(const orc_zone *zz, const double *pl, const double *pr, const double *met, const double *jac,
+                      int ip, double *f) {
+  const int nv = zz->nv;
+  const double *ml = &met[ip * 3], *mr = &met[(ip + 1) * 3];
+  const double jl = jac[ip], jr = jac[ip + 1];
+  const double k1 = 0.5 * (jl * ml[0] + jr * mr[0]);
+  const double k2 = 0.5 * (jl * ml[1] + jr * mr[1]);
+  const double k3 = 0.5 * (jl * ml[2] + jr * mr[2]);
+  const double gk = sqrt(k1 * k1 + k2 * k2 + k3 * k3);
+  const double ul = (k1 * pl[1] + k2 * pl[2] + k3 * pl[3]) / gk;
+  const double ur = (k1 * pr[1] + k2 * pr[2] + k3 * pr[3]) / gk;
+  const double pL = pl[4], pR = pr[4], rl = pl[0], rr = pr[0];
+  double gl = GAMMA_AIR, gr = GAMMA_AIR;
+  if (zz->cfg.mix) {
+    gl = pl[nv + 1];
+    gr = pr[nv + 1];
+  }
+  const double c = 0.5 * (sqrt(gl * pL / rl) + sqrt(gr * pR / rr));
+  const double mlc = ul / c, mrc = ur / c;
+  double mlp, mrn, plp, prn;
+  const double alpha = 3 / 16.0;
+  if (fabs(mlc) > 1) {
+    mlp = 0.5 * (mlc + fabs(mlc));
+    plp = mlp / mlc;
+  } else {
+    const double a = (mlc + 1) * (mlc + 1) * 0.25;
+    const double b = (mlc * mlc - 1) * (mlc * mlc - 1);
+    mlp = a + 0.125 * b;
+    plp = a * (2 - mlc) + alpha * mlc * b;
+  }
+  if (fabs(mrc) > 1) {
+    mrn = 0.5 * (mrc - fabs(mrc));
+    prn = mrn / mrc;
+  } else {
+    const double a = (mrc - 1) * (mrc - 1) * 0.25;
+    const double b = (mrc * mrc - 1) * (mrc * mrc - 1);
+    mrn = -a - 0.125 * b;
+    prn = a * (2 + mrc) - alpha * mrc * b;
+  }
+  const double p_coeff = plp * pL + prn * pR;
+  const double m_half = mlp + mrn;
+  const double mpos = 0.5 * (m_half + fabs(m_half)), mneg = 0.5 * (m_half - fabs(m_half));
+  const double mass = c * (rl * mpos + rr * mneg);
+  const double coeff = mass * gk;
+  const double *s = mass >= 0 ? pl : pr;
+  f[0] = coeff;
+  f[1] = coeff * s[1] + p_coeff * k1;
+  f[2] = coeff * s[2] + p_coeff * k2;
+  f[3] = coeff * s[3] + p_coeff * k3;
+  f[4] = coeff * (s[nv] + s[4]) / s[0];
+  for (int l = 5; l < nv; ++l) f[l] = coeff * s[l];
+}
+
+/* HLLC.cu:10-143 compute_hllc_flux */
+static void flux_hllc(const orc_zone *zz, const double *pl, const double *pr, const double *met, const double *jac,
+                      int ip, double *f) {
+  const int nv = zz->nv;
+  const double rlc = sqrt(pl[0]) / (sqrt(pl[0]) + sqrt(pr[0])), rrc = sqrt(pr[0]) / (sqrt(pl[0]) + sqrt(pr[0]));
+  const double ut = pl[1] * rlc + pr[1] * rrc, vt = pl[2] * rlc + pr[2] * rrc, wt = pl[3] * rlc + pr[3] * rrc;
+  const double hl = (pl[nv] + pl[4]) / pl[0], hr = (pr[nv] + pr[4]) / pr[0];
+  const double ht = hl * rlc + hr * rrc;
+  const double ekt = 0.5 * (ut * ut + vt * vt + wt * wt);
+  double gamma = GAMMA_AIR, ct;
+  double svm[ORC_MAXS + 4];
+  for (int l = 0; l < nv - 5; ++l) svm[l] = rlc * pl[l + 5] + rrc * pr[l + 5];
+  if (!zz->cfg.mix) {
+    ct = sqrt((gamma - 1) * (ht - ekt));
+  } else {
+    const orc_mech *m = &zz->mech;
+    double mw_inv = 0;
+    for (int l = 0; l < zz->ns; ++l) mw_inv += svm[l] / m->mw[l];
+    const double tl = pl[4] / pl[0], tr = pr[4] / pr[0];
+    const double t = (rlc * tl + rrc * tr) / (R_U * mw_inv);
+    double h_i[ORC_MAXS], cp_i[ORC_MAXS];
+    orc_enthalpy_and_cp(t, m, h_i, cp_i);
+    double cvv = 0, cpp = 0;
+    for (int l = 0; l < zz->ns; ++l) {
+      cpp += cp_i[l] * svm[l];
+      cvv += svm[l] * (cp_i[l] - R_U / m->mw[l]);
+    }
+    gamma = cpp / cvv;
+    ct = sqrt(gamma * R_U * mw_inv * t);
+  }
+  const double kx = 0.5 * (met[ip * 3] + met[(ip + 1) * 3]);
+  const double ky = 0.5 * (met[ip * 3 + 1] + met[(ip + 1) * 3 + 1]);
+  const double kz = 0.5 * (met[ip * 3 + 2] + met[(ip + 1) * 3 + 2]);
+  const double gK = sqrt(kx * kx + ky * ky + kz * kz);
+  const double Utb = (kx * ut + ky * vt + kz * wt) / gK;
+  const double ja = 0.5 * (jac[ip] + jac[ip + 1]);
+  double gl = GAMMA_AIR, gr = GAMMA_AIR;
+  if (zz->cfg.mix) {
+    gl = pl[nv + 1];
+    gr = pr[nv + 1];
+  }
+  const double Ul = kx * pl[1] + ky * pl[2] + kz * pl[3];
+  const double cl = sqrt(gl * pl[4] / pl[0]);
+  const double sl = fmin(Ul / gK - cl, Utb - ct);
+  if (sl >= 0) {
+    const double rU = pl[0] * Ul;
+    f[0] = ja * rU;
+    f[1] = ja * (rU * pl[1] + pl[4] * kx);
+    f[2] = ja * (rU * pl[2] + pl[4] * ky);
+    f[3] = ja * (rU * pl[3] + pl[4] * kz);
+    f[4] = ja * ((pl[nv] + pl[4]) * Ul);
+    for (int l = 5; l < nv; ++l) f[l] = ja * rU * pl[l];
+    return;
+  }
+  const double Ur = kx * pr[1] + ky * pr[2] + kz * pr[3];
+  const double cr = sqrt(gr * pr[4] / pr[0]);
+  const double sr = fmax(Ur / gK + cr, Utb + ct);
+  if (sr < 0) {
+    const double rU = pr[0] * Ur;
+    f[0] = ja * rU;
+    f[1] = ja * (rU * pr[1] + pr[4] * kx);
+    f[2] = ja * (rU * pr[2] + pr[4] * ky);
+    f[3] = ja * (rU * pr[3] + pr[4] * kz);
+    f[4] = ja * ((pr[nv] + pr[4]) * Ur);
+    for (int l = 5; l < nv; ++l) f[l] = ja * rU * pr[l];
+    return;
+  }
+  const double sm = ((pr[0] * Ur * (sr - Ur / gK) - pl[0] * Ul * (sl - Ul / gK)) / gK + pl[4] - pr[4]) /
+                    (pr[0] * (sr - Ur / gK) - pl[0] * (sl - Ul / gK));
+  const double pm = pl[0] * (sl - Ul / gK) * (sm - Ul / gK) + pl[4];
+  if (sm >= 0) {
+    const double pC = 1.0 / (sl - sm);
+    const double Q = ja * pC * sm * (sl * gK - Ul) * pl[0];
+    f[0] = Q;
+    const double dP = (sl * pm - sm * pl[4]) * pC * ja;
+    f[1] = Q * pl[1] + dP * kx;
+    f[2] = Q * pl[2] + dP * ky;
+    f[3] = Q * pl[3] + dP * kz;
+    f[4] = Q * pl[nv] / pl[0] + pC * ja * (sl * pm * sm * gK - sm * pl[4] * Ul);
+    for (int l = 5; l < nv; ++l) f[l] = Q * pl[l];
+  } else {
+    const double pC = 1.0 / (sr - sm);
+    const double Q = ja * pC * sm * (sr * gK - Ur) * pr[0];
+    f[0] = Q;
+    const double dP = (sr * pm - sm * pr[4]) * pC * ja;
+    f[1] = Q * pr[1] + dP * kx;
+    f[2] = Q * pr[2] + dP * ky;
+    f[3] = Q * pr[3] + dP * kz;
+    f[4] = Q * pr[nv] / pr[0] + pC * ja * (sr * pm * sm * gK - sm * pr[4] * Ur);
+    for (int l = 5; l < nv; ++l) f[l] = Q * pr[l];
+  }
+}
+
+/* LaxFriedrichs.cu:14-49 compute_lf_flux<Air> */
+static void flux_lf(const orc_zone *zz, const double *pl, const double *pr, const double *met, const double *jac, int ip,
+                    double *f) {
+  const int nv = zz->nv;
+  const double kx = 0.5 * (met[ip * 3] + met[(ip + 1) * 3]);
+  const double ky = 0.5 * (met[ip * 3 + 1] + met[(ip + 1) * 3 + 1]);
+  const double kz = 0.5 * (met[ip * 3 + 2] + met[(ip + 1) * 3 + 2]);
+  const double gK = sqrt(kx * kx + ky * ky + kz * kz);
+  const double Ukl = pl[1] * kx + pl[2] * ky + pl[3] * kz, Ukr = pr[1] * kx + pr[2] * ky + pr[3] * kz;
+  const double cl = sqrt(GAMMA_AIR * pl[4] / pl[0]), cr = sqrt(GAMMA_AIR * pr[4] / pr[0]);
+  const double sr = fmax(fabs(Ukl) + cl * gK, fabs(Ukr) + cr * gK);
+  const double hj = 0.5 * 0.5 * (jac[ip] + jac[ip + 1]);
+  const double rUl = pl[0] * Ukl, rUr = pr[0] * Ukr;
+  f[0] = (rUl + rUr - sr * (pr[0] - pl[0])) * hj;
+  f[1] = (rUl * pl[1] + rUr * pr[1] + kx * (pl[4] + pr[4]) - sr * (pr[1] * pr[0] - pl[1] * pl[0])) * hj;
+  f[2] = (rUl * pl[2] + rUr * pr[2] + ky * (pl[4] + pr[4]) - sr * (pr[2] * pr[0] - pl[2] * pl[0])) * hj;
+  f[3] = (rUl * pl[3] + rUr * pr[3] + kz * (pl[4] + pr[4]) - sr * (pr[3] * pr[0] - pl[3] * pl[0])) * hj;
+  f[4] = ((pl[nv] + pl[4]) * Ukl + (pr[nv] + pr[4]) * Ukr - sr * (pr[nv] - pl[nv])) * hj;
+}
+
+static void riemann(const orc_zone *zz, const double *pl, const double *pr, const double *met, const double *jac,
+                    int ip, double *f) {
+  switch (zz->cfg.inviscid_scheme) {
+    case 1: flux_lf(zz, pl, pr, met, jac, ip, f); break;
+    case 4: flux_hllc(zz, pl, pr, met, jac, ip, f); break;
+    default: flux_ausm(zz, pl, pr, met, jac, ip, f); break;
+  }
+}
+
+/* AWENO.cu:151-206 WENO5: characteristic projection of the 6-point stencil
+ * and the alternative-WENO weights (linear 1/16, 5/8, 5/16; tau5 = (b0-b2)^2) */
+static void weno5(const double *L, const double *cvb, int ip, int row, double *vm, double *vp) {
+  double v[6] = {0, 0, 0, 0, 0, 0};
+  for (int s = -2; s < 4; ++s) {
+    const double *U = &cvb[(ip + s) * 7];
+    for (int c = 0; c < 5; ++c) v[s + 2] += L[row * 5 + c] * U[c];
+  }
+  double v0 = 0.125 * (3 * v[0] - 10 * v[1] + 15 * v[2]);
+  double v1 = 0.125 * (-v[1] + 6 * v[2] + 3 * v[3]);
+  double v2 = 0.125 * (3 * v[2] + 6 * v[3] - v[4]);
+  const double t12 = 13.0 / 12;
+  double b0 = t12 * (v[0] + v[2] - 2 * v[1]) * (v[0] + v[2] - 2 * v[1]) +
+              0.25 * (v[0] + 3 * v[2] - 4 * v[1]) * (v[0] + 3 * v[2] - 4 * v[1]);
+  double b1 = t12 * (v[1] + v[3] - 2 * v[2]) * (v[1] + v[3] - 2 * v[2]) + 0.25 * (v[3] - v[1]) * (v[3] - v[1]);
+  double b2 = t12 * (v[2] + v[4] - 2 * v[3]) * (v[2] + v[4] - 2 * v[3]) +
+              0.25 * (3 * v[2] + v[4] - 4 * v[3]) * (3 * v[2] + v[4] - 4 * v[3]);
+  double tau = (b0 - b2) * (b0 - b2);
+  const double eps = 1e-6;
+  const double o16 = 1.0 / 16, f8 = 5.0 / 8, f16 = 5.0 / 16;
+  double a0 = o16 + o16 * tau / ((eps + b0) * (eps + b0));
+  double a1 = f8 + f8 * tau / ((eps + b1) * (eps + b1));
+  double a2 = f16 + f16 * tau / ((eps + b2) * (eps + b2));
+  *vm = (a0 * v0 + a1 * v1 + a2 * v2) / (a0 + a1 + a2);
+  v0 = v1;
+  v1 = v2;
+  v2 = 0.125 * (15 * v[3] - 10 * v[4] + 3 * v[5]);
+  b0 = t12 * (v[1] + v[3] - 2 * v[2]) * (v[1] + v[3] - 2 * v[2]) +
+       0.25 * (v[1] + 3 * v[3] - 4 * v[2]) * (v[1] + 3 * v[3] - 4 * v[2]);
+  b1 = t12 * (v[2] + v[4] - 2 * v[3]) * (v[2] + v[4] - 2 * v[3]) + 0.25 * (v[4] - v[2]) * (v[4] - v[2]);
+  b2 = t12 * (v[3] + v[5] - 2 * v[4]) * (v[3] + v[5] - 2 * v[4]) +
+       0.25 * (3 * v[3] + v[5] - 4 * v[4]) * (3 * v[3] + v[5] - 4 * v[4]);
+  tau = (b0 - b2) * (b0 - b2);
+  a0 = f16 + f16 * tau / ((eps + b0) * (eps + b0));
+  a1 = f8 + f8 * tau / ((eps + b1) * (eps + b1));
+  a2 = o16 + o16 * tau / ((eps + b2) * (eps + b2));
+  *vp = (a0 * v0 + a1 * v1 + a2 * v2) / (a0 + a1 + a2);
+}
+
+/* AWENO.cu:17-149 AWENO_interpolation<Air>: Roe average at the face, left
+ * eigenvectors along the normalised averaged metric, WENO5 per
+ * characteristic, right eigenvectors back to conservative, then primitives.
+ * cvb holds (rho, rho u, rho v, rho w, rho E, p, T) per point. */
+static void aweno_air(const double *cvb, const double *met, int ip, double *pl, double *pr) {
+  const double *cl = &cvb[ip * 7], *cr = &cvb[(ip + 1) * 7];
+  const double rlc = sqrt(cl[0]) / (sqrt(cl[0]) + sqrt(cr[0]));
+  const double rrc = sqrt(cr[0]) / (sqrt(cl[0]) + sqrt(cr[0]));
+  const double um = rlc * cl[1] / cl[0] + rrc * cr[1] / cr[0];
+  const double vm = rlc * cl[2] / cl[0] + rrc * cr[2] / cr[0];
+  const double wm = rlc * cl[3] / cl[0] + rrc * cr[3] / cr[0];
+  const double ekm = 0.5 * (um * um + vm * vm + wm * wm);
+  const double gm1 = GAMMA_AIR - 1;
+  const double hl = (cl[4] + cl[5]) / cl[0], hr = (cr[4] + cr[5]) / cr[0];
+  const double hm = rlc * hl + rrc * hr;
+  const double cm2 = gm1 * (hm - ekm);
+  const double cm = sqrt(cm2);
+  double kx = 0.5 * (met[ip * 3] + met[(ip + 1) * 3]);
+  double ky = 0.5 * (met[ip * 3 + 1] + met[(ip + 1) * 3 + 1]);
+  double kz = 0.5 * (met[ip * 3 + 2] + met[(ip + 1) * 3 + 2]);
+  const double gK = sqrt(kx * kx + ky * ky + kz * kz);
+  kx /= gK;
+  ky /= gK;
+  kz /= gK;
+  const double Uk = kx * um + ky * vm + kz * wm;
+  const double alpha = gm1 * ekm;
+  double LR[25];
+  LR[0] = (alpha + Uk * cm) / cm2 * 0.5;
+  LR[1] = -(gm1 * um + kx * cm) / cm2 * 0.5;
+  LR[2] = -(gm1 * vm + ky * cm) / cm2 * 0.5;
+  LR[3] = -(gm1 * wm + kz * cm) / cm2 * 0.5;
+  LR[4] = gm1 / cm2 * 0.5;
+  LR[5] = kx * (1 - alpha / cm2) - (kz * vm - ky * wm) / cm;
+  LR[6] = kx * gm1 * um / cm2;
+  LR[7] = (kx * gm1 * vm + kz * cm) / cm2;
+  LR[8] = (kx * gm1 * wm - ky * cm) / cm2;
+  LR[9] = -kx * gm1 / cm2;
+  LR[10] = ky * (1 - alpha / cm2) - (kx * wm - kz * um) / cm;
+  LR[11] = (ky * gm1 * um - kz * cm) / cm2;
+  LR[12] = ky * gm1 * vm / cm2;
+  LR[13] = (ky * gm1 * wm + kx * cm) / cm2;
+  LR[14] = -ky * gm1 / cm2;
+  LR[15] = kz * (1 - alpha / cm2) - (ky * um - kx * vm) / cm;
+  LR[16] = (kz * gm1 * um + ky * cm) / cm2;
+  LR[17] = (kz * gm1 * vm - kx * cm) / cm2;
+  LR[18] = kz * gm1 * wm / cm2;
+  LR[19] = -kz * gm1 / cm2;
+  LR[20] = (alpha - Uk * cm) / cm2 * 0.5;
+  LR[21] = -(gm1 * um - kx * cm) / cm2 * 0.5;
+  LR[22] = -(gm1 * vm - ky * cm) / cm2 * 0.5;
+  LR[23] = -(gm1 * wm - kz * cm) / cm2 * 0.5;
+  LR[24] = gm1 / cm2 * 0.5;
+  double vp[5], vmn[5];
+  for (int l = 0; l < 5; ++l) weno5(LR, cvb, ip, l, &vmn[l], &vp[l]);
+  LR[0] = 1.0;
+  LR[1] = kx;
+  LR[2] = ky;
+  LR[3] = kz;
+  LR[4] = 1.0;
+  LR[5] = um - kx * cm;
+  LR[6] = kx * um;
+  LR[7] = ky * um - kz * cm;
+  LR[8] = kz * um + ky * cm;
+  LR[9] = um + kx * cm;
+  LR[10] = vm - ky * cm;
+  LR[11] = kx * vm + kz * cm;
+  LR[12] = ky * vm;
+  LR[13] = kz * vm - kx * cm;
+  LR[14] = vm + ky * cm;
+  LR[15] = wm - kz * cm;
+  LR[16] = kx * wm - ky * cm;
+  LR[17] = ky * wm + kx * cm;
+  LR[18] = kz * wm;
+  LR[19] = wm + kz * cm;
+  LR[20] = hm - Uk * cm;
+  LR[21] = kx * alpha / gm1 + (kz * vm - ky * wm) * cm;
+  LR[22] = ky * alpha / gm1 + (kx * wm - kz * um) * cm;
+  LR[23] = kz * alpha / gm1 + (ky * um - kx * vm) * cm;
+  LR[24] = hm + Uk * cm;
+  double ql[5], qr[5];
+  for (int a = 0; a < 5; ++a) {
+    ql[a] = 0;
+    qr[a] = 0;
+    for (int b = 0; b < 5; ++b) {
+      ql[a] += LR[a * 5 + b] * vmn[b];
+      qr[a] += LR[a * 5 + b] * vp[b];
+    }
+  }
+  pl[0] = ql[0];
+  pl[1] = ql[1] / ql[0];
+  pl[2] = ql[2] / ql[0];
+  pl[3] = ql[3] / ql[0];
+  pl[4] = gm1 * (ql[4] - 0.5 * (pl[1] * pl[1] + pl[2] * pl[2] + pl[3] * pl[3]) * ql[0]);
+  pl[5] = ql[4];
+  pr[0] = qr[0];
+  pr[1] = qr[1] / qr[0];
+  pr[2] = qr[2] / qr[0];
+  pr[3] = qr[3] / qr[0];
+  pr[4] = gm1 * (qr[4] - 0.5 * (pr[1] * pr[1] + pr[2] * pr[2] + pr[3] * pr[3]) * qr[0]);
+  pr[5] = qr[4];
+}
+
+/* line gather helpers: position s on line (dir, a, b) -> (i,j,k) */
+static void line_ijk(int dir, int a, int b, int s, int *i, int *j, int *k) {
+  if (dir == 0) { *i = s; *j = a; *k = b; }
+  else if (dir == 1) { *i = a; *j = s; *k = b; }
+  else { *i = a; *j = b; *k = s; }
+}
+
+/* InviscidScheme.cu:10-158 (pv path), :160-311 (AWENO path) and
+ * AWENO.cu:213-264 (CDSPart1D<Air>) for one direction. */
+static void inviscid_dir(orc_zone *zz, int dir) {
+  const int mext[3] = {zz->mx, zz->my, zz->mz};
+  const int n = mext[dir], g = zz->g, nv = zz->nv;
+  const int oa = dir == 0 ? 1 : 0, ob = dir == 2 ? 1 : 2;
+  const int na = mext[oa], nb = mext[ob];
+  const int npt = n + 2 * g;
+  double *pv = malloc(sizeof(double) * npt * (nv + 2) + 8);
+  double *cvb = malloc(sizeof(double) * npt * 7 + 8);
+  double *met = malloc(sizeof(double) * npt * 3 + 8);
+  double *jac = malloc(sizeof(double) * npt + 8);
+  double *fc = malloc(sizeof(double) * (n + 1) * nv + 8);
+  double *fn = malloc(sizeof(double) * npt * 5 + 8);
+  for (int b = 0; b < nb; ++b)
+    for (int a = 0; a < na; ++a) {
+      for (int s = -g; s < n + g; ++s) {
+        int i, j, k;
+        line_ijk(dir, a, b, s, &i, &j, &k);
+        const int q = s + g;
+        const int64_t p = at(zz->Lg, i, j, k);
+        for (int l = 0; l < 5; ++l) pv[q * nv + l] = BV(l, i, j, k);
+        for (int l = 0; l < zz->nsc; ++l) pv[q * nv + 5 + l] = SV(l, i, j, k);
+        for (int l = 0; l < 5; ++l) cvb[q * 7 + l] = CV(l, i, j, k);
+        cvb[q * 7 + 5] = BV(4, i, j, k);
+        cvb[q * 7 + 6] = BV(5, i, j, k);
+        for (int c = 0; c < 3; ++c) met[q * 3 + c] = zz->metric[p * 9 + dir * 3 + c];
+        jac[q] = zz->jac[p];
+      }
+      /* face fluxes for faces s+1/2, s = -1 .. n-1 */
+      for (int s = -1; s < n; ++s) {
+        const int q = s + g;
+        double pl[ORC_MAXS + 16], pr[ORC_MAXS + 16];
+        if (zz->inviscid_type == 1)
+          aweno_air(cvb, met, q, pl, pr);
+        else
+          reconstruct(zz, pv, q, pl, pr);
+        riemann(zz, pl, pr, met, jac, q, &fc[(s + 1) * nv]);
+      }
+      for (int s = 0; s < n; ++s) {
+        int i, j, k;
+        line_ijk(dir, a, b, s, &i, &j, &k);
+        for (int l = 0; l < nv; ++l) DQ(l, i, j, k) -= fc[(s + 1) * nv + l] - fc[s * nv + l];
+      }
+      if (zz->inviscid_type == 1) {
+        /* CDSPart1D<Air>: node fluxes on [-g, n-g), correction on [0, n-2g) */
+        for (int s = -g; s < n - g; ++s) {
+          int i, j, k;
+          line_ijk(dir, a, b, s, &i, &j, &k);
+          const int q = s + g;
+          const double *mm = &met[q * 3];
+          const double J = jac[q];
+          const double u = BV(1, i, j, k), v = BV(2, i, j, k), w = BV(3, i, j, k);
+          const double Uk = mm[0] * u + mm[1] * v + mm[2] * w;
+          const double rU = BV(0, i, j, k) * Uk;
+          const double p = BV(4, i, j, k);
+          fn[q * 5 + 0] = rU * J;
+          fn[q * 5 + 1] = J * (rU * u + mm[0] * p);
+          fn[q * 5 + 2] = J * (rU * v + mm[1] * p);
+          fn[q * 5 + 3] = J * (rU * w + mm[2] * p);
+          fn[q * 5 + 4] = J * Uk * (CV(4, i, j, k) + p);
+        }
+        const double c1 = 19.0 / 3840, c2 = -13.0 / 320, c3 = 17.0 / 256;
+        for (int s = 0; s < n - 2 * g; ++s) {
+          int i, j, k;
+          line_ijk(dir, a, b, s, &i, &j, &k);
+          const int q = s + g;
+          for (int l = 0; l < nv; ++l)
+            DQ(l, i, j, k) -= c1 * (fn[(q + 3) * 5 + l] - fn[(q - 3) * 5 + l]) +
+                              c2 * (fn[(q + 2) * 5 + l] - fn[(q - 2) * 5 + l]) +
+                              c3 * (fn[(q + 1) * 5 + l] - fn[(q - 1) * 5 + l]);
+        }
+      }
+    }
+  free(pv);
+  free(cvb);
+  free(met);
+  free(jac);
+  free(fc);
+  free(fn);
+}
+
+/* ----------------------------------------------------------- viscous */
+
+/* ViscousScheme.cuh:22-288 (fv), 290-557 (gv), 559-821 (hv): second-order
+ * central face gradients, Stokes stress, Fourier heat flux and (mixtures)
+ * mixture-averaged diffusion with the correction velocity.  Face between
+ * p0 and p0 + e_dir. */
+static void viscous_face(orc_zone *zz, int dir, int i, int j, int k, double *fv) {
+  const int nv = zz->nv, ns = zz->ns;
+  int e[3] = {0, 0, 0};
+  e[dir] = 1;
+  const int i1 = i + e[0], j1 = j + e[1], k1 = k + e[2];
+  const double *m = &zz->metric[at(zz->Lg, i, j, k) * 9];
+  const double *m1 = &zz->metric[at(zz->Lg, i1, j1, k1) * 9];
+  double mm[9];
+  for (int c = 0; c < 9; ++c) mm[c] = 0.5 * (m[c] + m1[c]);
+  /* derivative along computational direction d of field f at the face */
+#define FLD(l, a, b, c) (l < 6 ? BV(l, a, b, c) : SV(l - 6, a, b, c))
+  double d_c[3][ORC_MAXS + 8];
+  const int nf = 6 + zz->nsc;
+  for (int l = 0; l < nf; ++l) {
+    if (l == 0 || l == 4) {
+      d_c[0][l] = d_c[1][l] = d_c[2][l] = 0;
+      if (!(l == 4 && zz->cfg.gradPInDiffusionFlux)) continue;
+    }
+    for (int d = 0; d < 3; ++d) {
+      if (d == dir) {
+        d_c[d][l] = FLD(l, i1, j1, k1) - FLD(l, i, j, k);
+      } else {
+        int o[3] = {0, 0, 0};
+        o[d] = 1;
+        d_c[d][l] = 0.25 * (FLD(l, i + o[0], j + o[1], k + o[2]) - FLD(l, i - o[0], j - o[1], k - o[2]) +
+                            FLD(l, i1 + o[0], j1 + o[1], k1 + o[2]) - FLD(l, i1 - o[0], j1 - o[1], k1 - o[2]));
+      }
+    }
+  }
+  /* chain rule: f_x = f_xi xi_x + f_eta eta_x + f_zeta zeta_x */
+#define GX(l, c) (d_c[0][l] * mm[0 * 3 + (c)] + d_c[1][l] * mm[1 * 3 + (c)] + d_c[2][l] * mm[2 * 3 + (c)])
+  const double u_x = GX(1, 0), u_y = GX(1, 1), u_z = GX(1, 2);
+  const double v_x = GX(2, 0), v_y = GX(2, 1), v_z = GX(2, 2);
+  const double w_x = GX(3, 0), w_y = GX(3, 1), w_z = GX(3, 2);
+  const double mul = 0.5 * (S1(mul, i, j, k) + S1(mul, i1, j1, k1));
+  const double vis = mul;
+  const double txx = vis * (4 * u_x - 2 * v_y - 2 * w_z) / 3.0;
+  const double tyy = vis * (4 * v_y - 2 * u_x - 2 * w_z) / 3.0;
+  const double tzz = vis * (4 * w_z - 2 * u_x - 2 * v_y) / 3.0;
+  const double txy = vis * (u_y + v_x), txz = vis * (u_z + w_x), tyz = vis * (v_z + w_y);
+  const double J0 = zz->jac[at(zz->Lg, i, j, k)], J1 = zz->jac[at(zz->Lg, i1, j1, k1)];
+  const double kx = 0.5 * (m[dir * 3 + 0] * J0 + m1[dir * 3 + 0] * J1);
+  const double ky = 0.5 * (m[dir * 3 + 1] * J0 + m1[dir * 3 + 1] * J1);
+  const double kz = 0.5 * (m[dir * 3 + 2] * J0 + m1[dir * 3 + 2] * J1);
+  fv[0] = 0;
+  fv[1] = kx * txx + ky * txy + kz * txz;
+  fv[2] = kx * txy + ky * tyy + kz * tyz;
+  fv[3] = kx * txz + ky * tyz + kz * tzz;
+  const double um = 0.5 * (BV(1, i, j, k) + BV(1, i1, j1, k1));
+  const double vm = 0.5 * (BV(2, i, j, k) + BV(2, i1, j1, k1));
+  const double wm = 0.5 * (BV(3, i, j, k) + BV(3, i1, j1, k1));
+  const double t_x = GX(5, 0), t_y = GX(5, 1), t_z = GX(5, 2);
+  const double cond = 0.5 * (S1(cond, i, j, k) + S1(cond, i1, j1, k1));
+  fv[4] = um * fv[1] + vm * fv[2] + wm * fv[3] + cond * (kx * t_x + ky * t_y + kz * t_z);
+  if (zz->cfg.mix) {
+    const orc_mech *me = &zz->mech;
+    double diffusivity[ORC_MAXS], yk[ORC_MAXS], ddf[ORC_MAXS];
+    double sum_gy_w = 0, sum_rhoDY = 0, corr = 0, mw_tot = 0;
+    for (int l = 0; l < ns; ++l) {
+      yk[l] = 0.5 * (SV(l, i, j, k) + SV(l, i1, j1, k1));
+      diffusivity[l] = 0.5 * (zz->rhoD[l * zz->Lg.sz + at(zz->Lg, i, j, k)] +
+                              zz->rhoD[l * zz->Lg.sz + at(zz->Lg, i1, j1, k1)]);
+      const double y_x = GX(6 + l, 0), y_y = GX(6 + l, 1), y_z = GX(6 + l, 2);
+      const double gy = kx * y_x + ky * y_y + kz * y_z;
+      ddf[l] = gy;
+      corr += diffusivity[l] * gy;
+      sum_gy_w += gy / me->mw[l];
+      mw_tot += yk[l] / me->mw[l];
+      sum_rhoDY += diffusivity[l] * yk[l];
+    }
+    mw_tot = 1.0 / mw_tot;
+    corr -= mw_tot * sum_rhoDY * sum_gy_w;
+    if (zz->cfg.gradPInDiffusionFlux) {
+      const double p_x = GX(4, 0), p_y = GX(4, 1), p_z = GX(4, 2);
+      const double gp = (kx * p_x + ky * p_y + kz * p_z) / (0.5 * (BV(4, i1, j1, k1) + BV(4, i, j, k)));
+      for (int l = 0; l < ns; ++l) {
+        ddf[l] += (mw_tot / me->mw[l] - 1) * yk[l] * gp;
+        corr += (mw_tot / me->mw[l] - 1) * yk[l] * gp * diffusivity[l];
+      }
+    }
+    double h[ORC_MAXS];
+    const double tm = 0.5 * (BV(5, i, j, k) + BV(5, i1, j1, k1));
+    orc_enthalpy(tm, me, h);
+    for (int l = 0; l < ns; ++l) {
+      const double dflux = diffusivity[l] * (ddf[l] - mw_tot * yk[l] * sum_gy_w) - yk[l] * corr;
+      fv[5 + l] = dflux;
+      fv[4] += h[l] * dflux;
+    }
+  }
+  (void)nv;
+#undef GX
+#undef FLD
+}
+
+/* SchemeSelector.cuh:49-157 compute_viscous_flux: dq += fv(i+1/2) - fv(i-1/2) */
+static void viscous_dir(orc_zone *zz, int dir) {
+  const int nv = zz->nv;
+  const int mext[3] = {zz->mx, zz->my, zz->mz};
+  const int n = mext[dir];
+  const int oa = dir == 0 ? 1 : 0, ob = dir == 2 ? 1 : 2;
+  double *fv = malloc(sizeof(double) * (n + 1) * nv);
+  for (int b = 0; b < mext[ob]; ++b)
+    for (int a = 0; a < mext[oa]; ++a) {
+      for (int s = -1; s < n; ++s) {
+        int i, j, k;
+        line_ijk(dir, a, b, s, &i, &j, &k);
+        viscous_face(zz, dir, i, j, k, &fv[(s + 1) * nv]);
+      }
+      for (int s = 0; s < n; ++s) {
+        int i, j, k;
+        line_ijk(dir, a, b, s, &i, &j, &k);
+        for (int l = 0; l < nv; ++l) DQ(l, i, j, k) += fv[(s + 1) * nv + l] - fv[s * nv + l];
+      }
+    }
+  free(fv);
+}
+
+/* SourceTerm.cuh:11-37 compute_source (finite-rate chemistry part) */
+static void source(orc_zone *zz) {
+  if (!(zz->cfg.mix && zz->cfg.reaction == 1)) return;
+  const int ns = zz->ns;
+  for (int k = 0; k < zz->mz; ++k)
+    for (int j = 0; j < zz->my; ++j)
+      for (int i = 0; i < zz->mx; ++i) {
+        double Y[ORC_MAXS], om[ORC_MAXS], jc[ORC_MAXS * ORC_MAXS], jd[ORC_MAXS];
+        for (int l = 0; l < ns; ++l) Y[l] = SV(l, i, j, k);
+        const int meth = zz->cfg.chemSrcMethod;
+        orc_chem_source_point(&zz->mech, BV(0, i, j, k), BV(5, i, j, k), Y, om, meth == 1 ? jc : NULL,
+                              meth == 2 ? jd : NULL);
+        const double J = zz->jac[at(zz->Lg, i, j, k)];
+        for (int l = 0; l < ns; ++l) DQ(5 + l, i, j, k) += J * om[l];
+        const int64_t p = at(zz->L0, i, j, k);
+        if (meth == 1)
+          for (int c = 0; c < ns * ns; ++c) zz->chem_jac[c * zz->L0.sz + p] = jc[c];
+        if (meth == 2)
+          for (int c = 0; c < ns; ++c) zz->chem_jac[c * zz->L0.sz + p] = jd[c];
+      }
+}
+
+void orc_zone_rhs(orc_zone *zz) {
+  memset(zz->dq, 0, sizeof(double) * zz->L0.sz * zz->nv);
+  source(zz);
+  for (int d = 0; d < zz->dim; ++d) inviscid_dir(zz, d);
+  if (zz->cfg.viscous_order != 0)
+    for (int d = 0; d < zz->dim; ++d) viscous_dir(zz, d);
+}
+
+/* ----------------------------------------------------------- time */
+
+/* TimeAdvanceFunc.cuh:50-100 local_time_step + TimeAdvanceFunc.cu:155-201 global min */
+double orc_zone_global_dt(orc_zone *zz) {
+  double dt = 1e+6;
+  for (int k = 0; k < zz->mz; ++k)
+    for (int j = 0; j < zz->my; ++j)
+      for (int i = 0; i < zz->mx; ++i) {
+        const double *m = &zz->metric[at(zz->Lg, i, j, k) * 9];
+        const double gxi = sqrt(m[0] * m[0] + m[1] * m[1] + m[2] * m[2]);
+        const double geta = sqrt(m[3] * m[3] + m[4] * m[4] + m[5] * m[5]);
+        const double gzeta = sqrt(m[6] * m[6] + m[7] * m[7] + m[8] * m[8]);
+        const double u = BV(1, i, j, k), v = BV(2, i, j, k), w = BV(3, i, j, k);
+        const double U = u * m[0] + v * m[1] + w * m[2];
+        const double V = u * m[3] + v * m[4] + w * m[5];
+        const double W = u * m[6] + v * m[7] + w * m[8];
+        const double a = S1(acoustic, i, j, k);
+        const int64_t p = at(zz->L0, i, j, k);
+        double *isr = &zz->inv_sr[p * 3];
+        isr[0] = fabs(U) + a * gxi;
+        isr[1] = fabs(V) + a * geta;
+        isr[2] = 0;
+        if (zz->dim == 3) isr[2] = fabs(W) + a * gzeta;
+        const double sri = isr[0] + isr[1] + isr[2];
+        double gamma = GAMMA_AIR;
+        if (zz->cfg.mix) gamma = S1(gam, i, j, k);
+        const double c1 = fmax(gamma, 4.0 / 3.0) / BV(0, i, j, k);
+        const double c2 = S1(mul, i, j, k) / zz->cfg.Pr;
+        double srv = gxi * gxi + geta * geta;
+        if (zz->dim == 3) srv += gzeta * gzeta;
+        srv *= c1 * c2;
+        zz->visc_sr[p] = srv;
+        zz->dt_local[p] = zz->cfg.cfl / (sri + srv);
+        if (zz->dt_local[p] < dt) dt = zz->dt_local[p];
+      }
+  return dt;
+}
+
+/* RK.cuh:10-14 SSPRK3 coefficients and RK.cuh:197-253 update_cv_and_bv_rk */
+static void rk_update(orc_zone *zz, double dt, int rk) {
+  static const double A[3] = {1.0, 0.75, 1.0 / 3.0}, B[3] = {0.0, 0.25, 2.0 / 3.0}, C[3] = {1.0, 0.25, 2.0 / 3.0};
+  for (int k = 0; k < zz->mz; ++k)
+    for (int j = 0; j < zz->my; ++j)
+      for (int i = 0; i < zz->mx; ++i) {
+        const double dtj = dt / zz->jac[at(zz->Lg, i, j, k)];
+        for (int l = 0; l < zz->nv; ++l)
+          CV(l, i, j, k) = A[rk] * QN(l, i, j, k) + B[rk] * CV(l, i, j, k) + C[rk] * dtj * DQ(l, i, j, k);
+        if (zz->mz == 1) CV(3, i, j, k) = 0;
+        BV(0, i, j, k) = CV(0, i, j, k);
+        const double ri = 1.0 / CV(0, i, j, k);
+        BV(1, i, j, k) = CV(1, i, j, k) * ri;
+        BV(2, i, j, k) = CV(2, i, j, k) * ri;
+        BV(3, i, j, k) = CV(3, i, j, k) * ri;
+        const double V2 = BV(1, i, j, k) * BV(1, i, j, k) + BV(2, i, j, k) * BV(2, i, j, k) +
+                          BV(3, i, j, k) * BV(3, i, j, k);
+        for (int l = 0; l < zz->nsc; ++l) SV(l, i, j, k) = CV(5 + l, i, j, k) * ri;
+        if (zz->cfg.mix) {
+          double Y[ORC_MAXS], p;
+          for (int l = 0; l < zz->ns; ++l) Y[l] = SV(l, i, j, k);
+          const double v3[3] = {BV(1, i, j, k), BV(2, i, j, k), BV(3, i, j, k)};
+          const double t = orc_temperature_newton(&zz->mech, BV(0, i, j, k), CV(4, i, j, k), v3, Y, BV(5, i, j, k),
+                                                  &p, NULL);
+          BV(5, i, j, k) = t;
+          BV(4, i, j, k) = p;
+        } else {
+          BV(4, i, j, k) = (GAMMA_AIR - 1) * (CV(4, i, j, k) - 0.5 * BV(0, i, j, k) * V2);
+          BV(5, i, j, k) = BV(4, i, j, k) * MW_AIR * ri / R_U;
+        }
+        S1(vel, i, j, k) = sqrt(V2);
+      }
+}
+
+static void store_last_step(orc_zone *zz) {
+  for (int k = 0; k < zz->mz; ++k)
+    for (int j = 0; j < zz->my; ++j)
+      for (int i = 0; i < zz->mx; ++i) {
+        const int64_t p = at(zz->L0, i, j, k);
+        zz->bv_last[p] = BV(0, i, j, k);
+        zz->bv_last[zz->L0.sz + p] = S1(vel, i, j, k);
+        zz->bv_last[2 * zz->L0.sz + p] = BV(4, i, j, k);
+        zz->bv_last[3 * zz->L0.sz + p] = BV(5, i, j, k);
+      }
+}
+
+/* Residual.cuh:57-161 + TimeAdvanceFunc.cu:19-33: RMS of the change of
+ * (rho, |V|, p, T), scaled by the first step's value (floor 1e-20). */
+static double residual(orc_zone *zz, int step) {
+  double s[4] = {0, 0, 0, 0};
+  const int64_t n = (int64_t)zz->mx * zz->my * zz->mz;
+  for (int k = 0; k < zz->mz; ++k)
+    for (int j = 0; j < zz->my; ++j)
+      for (int i = 0; i < zz->mx; ++i) {
+        const int64_t p = at(zz->L0, i, j, k);
+        const double cur[4] = {BV(0, i, j, k), S1(vel, i, j, k), BV(4, i, j, k), BV(5, i, j, k)};
+        for (int l = 0; l < 4; ++l) {
+          const double d = cur[l] - zz->bv_last[l * zz->L0.sz + p];
+          zz->bv_last[l * zz->L0.sz + p] = d * d;
+          s[l] += d * d;
+        }
+      }
+  for (int l = 0; l < 4; ++l) zz->res[l] = sqrt(s[l] / (double)n);
+  if (step == 1)
+    for (int l = 0; l < 4; ++l) {
+      zz->res_scale[l] = zz->res[l];
+      if (zz->res_scale[l] < 1e-20) zz->res_scale[l] = 1e-20;
+    }
+  for (int l = 0; l < 4; ++l) zz->res[l] /= zz->res_scale[l];
+  double err = zz->res[0];
+  for (int l = 1; l < 4; ++l)
+    if (fabs(zz->res[l]) > err) err = zz->res[l];
+  return err;
+}
+
+/* RK.cuh:89-193 RK3_bv loop (single block, no IO/monitors) */
+void orc_zone_run_rk3(orc_zone *zz, int nsteps, double *dt_hist, double *res_hist) {
+  const int64_t ncv = zz->Lg.sz * zz->nv;
+  const int os = zz->cfg.output_screen > 0 ? zz->cfg.output_screen : 1;
+  for (int it = 0; it < nsteps; ++it) {
+    const int step = ++zz->step;
+    double dt = zz->cfg.fixed_time_step ? zz->cfg.dt : 1e+6;
+    if (step % os == 0) store_last_step(zz);
+    memcpy(zz->qn, zz->cv, sizeof(double) * ncv);
+    for (int rk = 0; rk < 3; ++rk) {
+      orc_zone_rhs(zz);
+      if (!zz->cfg.fixed_time_step) dt = orc_zone_global_dt(zz);
+      rk_update(zz, dt, rk);
+      apply_bcs(zz, 1);
+      if (zz->dim == 2) eliminate_k_gradient(zz, 1);
+      update_properties(zz);
+      if (dt_hist) dt_hist[it * 3 + rk] = dt;
+    }
+    zz->t_phys += dt;
+    if (step % os == 0 || step == 1) {
+      const double err = residual(zz, step);
+      if (res_hist) {
+        for (int l = 0; l < 4; ++l) res_hist[it * 5 + l] = zz->res[l];
+        res_hist[it * 5 + 4] = err;
+      }
+    } else if (res_hist) {
+      for (int l = 0; l < 5; ++l) res_hist[it * 5 + l] = NAN;
+    }
+  }
+}

@@ -1,0 +1,365 @@
+"""ctypes front-end of the C oracle (oracle/orc_*.c).  TEST INFRASTRUCTURE ONLY.
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg import
+this module; the product never does.  It loads oracle/_build/liboracle.so
+(built by oracle/build.sh), fills the mechanism struct from the golden dump
+of the reference's own CHEMKIN readers (tests/golden/h2o2_9sp_tables.txt),
+restates the reference's inflow-state construction (BoundCond.cpp:7-122) and
+drives the flow oracle.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+LIB = os.path.join(HERE, "_build", "liboracle.so")
+TABLES = os.path.join(ROOT, "tests", "golden", "h2o2_9sp_tables.txt")
+
+MAXS, MAXR, MAXBC = 20, 50, 32
+R_U = 8314.462618
+GAMMA_AIR = 1.4
+MW_AIR = 28.9647
+
+
+def build():
+    subprocess.run(["bash", os.path.join(HERE, "build.sh")], check=True)
+
+
+class Mech(C.Structure):
+    _fields_ = [("n_spec", C.c_int), ("n_reac", C.c_int),
+                ("mw", C.c_double * MAXS), ("t_low", C.c_double * MAXS), ("t_mid", C.c_double * MAXS),
+                ("t_high", C.c_double * MAXS), ("hi", C.c_double * (MAXS * 7)), ("lo", C.c_double * (MAXS * 7)),
+                ("LJ_potent_inv", C.c_double * MAXS), ("vis_coeff", C.c_double * MAXS),
+                ("WjDivWi_to_One4th", C.c_double * (MAXS * MAXS)),
+                ("sqrt_WiDivWjPl1Mul8", C.c_double * (MAXS * MAXS)),
+                ("binary_diffusivity_coeff", C.c_double * (MAXS * MAXS)),
+                ("kb_over_eps_jk", C.c_double * (MAXS * MAXS)),
+                ("label", C.c_int * MAXR), ("stoi_f", C.c_int * (MAXR * MAXS)), ("stoi_b", C.c_int * (MAXR * MAXS)),
+                ("order", C.c_int * MAXR),
+                ("A", C.c_double * MAXR), ("b", C.c_double * MAXR), ("Ea", C.c_double * MAXR),
+                ("A2", C.c_double * MAXR), ("b2", C.c_double * MAXR), ("Ea2", C.c_double * MAXR),
+                ("third_body_coeff", C.c_double * (MAXR * MAXS)),
+                ("troe_alpha", C.c_double * MAXR), ("troe_t3", C.c_double * MAXR), ("troe_t1", C.c_double * MAXR),
+                ("troe_t2", C.c_double * MAXR)]
+
+
+class BC(C.Structure):
+    _fields_ = [("type", C.c_int), ("raw", C.c_int * 6),
+                ("rho", C.c_double), ("u", C.c_double), ("v", C.c_double), ("w", C.c_double), ("p", C.c_double),
+                ("t", C.c_double), ("vel", C.c_double), ("sv", C.c_double * (MAXS + 4)),
+                ("t_wall", C.c_double), ("isothermal", C.c_int)]
+
+
+class Cfg(C.Structure):
+    _fields_ = [("mix", C.c_int), ("reaction", C.c_int), ("chemSrcMethod", C.c_int), ("inviscid_scheme", C.c_int),
+                ("reconstruction", C.c_int), ("limiter", C.c_int), ("viscous_order", C.c_int),
+                ("fixed_time_step", C.c_int), ("output_screen", C.c_int), ("gradPInDiffusionFlux", C.c_int),
+                ("cfl", C.c_double), ("Pr", C.c_double), ("Sc", C.c_double), ("Prt", C.c_double),
+                ("Sct", C.c_double), ("dt", C.c_double), ("entropy_fix_factor", C.c_double),
+                ("n_bc", C.c_int), ("bc", BC * MAXBC)]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            build()
+        L = C.CDLL(LIB)
+        dp = C.POINTER(C.c_double)
+        L.orc_enthalpy.argtypes = [C.c_double, C.POINTER(Mech), dp]
+        L.orc_enthalpy_and_cp.argtypes = [C.c_double, C.POINTER(Mech), dp, dp]
+        L.orc_cp.argtypes = [C.c_double, C.POINTER(Mech), dp]
+        L.orc_gibbs_rt.argtypes = [C.c_double, C.POINTER(Mech), dp]
+        L.orc_sutherland.argtypes = [C.c_double]
+        L.orc_sutherland.restype = C.c_double
+        L.orc_transport.argtypes = [C.POINTER(Mech), C.c_double, C.c_double, C.c_double, dp, dp, dp, dp, dp]
+        L.orc_forward_rate.argtypes = [C.POINTER(Mech), C.c_double, dp, dp]
+        L.orc_backward_rate.argtypes = [C.POINTER(Mech), C.c_double, dp, dp]
+        L.orc_rate_of_progress.argtypes = [C.POINTER(Mech), dp, dp, dp, dp, dp, dp]
+        L.orc_chem_source_point.argtypes = [C.POINTER(Mech), C.c_double, C.c_double, dp, dp, dp, dp]
+        L.orc_lu_solve.argtypes = [dp, dp, C.c_int]
+        L.orc_temperature_newton.argtypes = [C.POINTER(Mech), C.c_double, C.c_double, dp, dp, C.c_double, dp,
+                                             C.POINTER(C.c_int)]
+        L.orc_temperature_newton.restype = C.c_double
+        L.orc_chem_batch.argtypes = [C.POINTER(Mech), C.c_int64, dp, dp, dp, C.c_double, dp, dp]
+        L.orc_zone_create.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(Cfg), C.POINTER(Mech), dp, dp,
+                                      dp]
+        L.orc_zone_create.restype = C.c_void_p
+        L.orc_zone_destroy.argtypes = [C.c_void_p]
+        L.orc_zone_field.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int64)]
+        L.orc_zone_field.restype = dp
+        L.orc_zone_initialize.argtypes = [C.c_void_p]
+        L.orc_zone_rhs.argtypes = [C.c_void_p]
+        L.orc_zone_global_dt.argtypes = [C.c_void_p]
+        L.orc_zone_global_dt.restype = C.c_double
+        L.orc_zone_run_rk3.argtypes = [C.c_void_p, C.c_int, dp, dp]
+        L.orc_zone_state.argtypes = [C.c_void_p, dp, C.POINTER(C.c_int)]
+        L.orc_grid_metrics.argtypes = [C.c_int] * 4 + [dp] * 7
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+# ----------------------------------------------------------------- tables
+
+def load_tables(path=TABLES):
+    """Parse the golden dump into a dict of numpy arrays / lists."""
+    t = {}
+    with open(path) as f:
+        for line in f:
+            if line.startswith("#") or not line.strip():
+                continue
+            parts = line.split()
+            key, n, vals = parts[0], int(parts[1]), parts[2:]
+            if key in ("names", "elements"):
+                t[key] = vals
+            elif key in ("n_spec", "n_reac"):
+                t[key] = int(vals[0])
+            elif key in ("label", "stoi_f", "stoi_b", "order", "elem_comp"):
+                t[key] = np.array([int(v) for v in vals], dtype=np.int64)
+            else:
+                t[key] = np.array([float(v) for v in vals], dtype=np.float64)
+            assert n == len(vals), key
+    return t
+
+
+def make_mech(t=None):
+    if t is None:
+        t = load_tables()
+    m = Mech()
+    ns, nr = t["n_spec"], t["n_reac"]
+    m.n_spec, m.n_reac = ns, nr
+    for key in ("mw", "t_low", "t_mid", "t_high", "LJ_potent_inv", "vis_coeff"):
+        getattr(m, key)[:ns] = list(t[key])
+    m.hi[:ns * 7] = list(t["high_coeff"])
+    m.lo[:ns * 7] = list(t["low_coeff"])
+    for key in ("WjDivWi_to_One4th", "sqrt_WiDivWjPl1Mul8", "binary_diffusivity_coeff", "kb_over_eps_jk"):
+        getattr(m, key)[:ns * ns] = list(t[key])
+    for key in ("label", "order"):
+        getattr(m, key)[:nr] = [int(v) for v in t[key]]
+    for key in ("stoi_f", "stoi_b"):
+        getattr(m, key)[:nr * ns] = [int(v) for v in t[key]]
+    for key in ("A", "b", "Ea", "A2", "b2", "Ea2", "troe_alpha", "troe_t3", "troe_t1", "troe_t2"):
+        getattr(m, key)[:nr] = list(t[key])
+    m.third_body_coeff[:nr * ns] = list(t["third_body_coeff"])
+    return m
+
+
+# ------------------------------------------------------------- point calls
+
+def enthalpy(m, T):
+    h = np.zeros(MAXS)
+    lib().orc_enthalpy(T, C.byref(m), _p(h))
+    return h[:m.n_spec].copy()
+
+
+def enthalpy_cp(m, T):
+    h, cp = np.zeros(MAXS), np.zeros(MAXS)
+    lib().orc_enthalpy_and_cp(T, C.byref(m), _p(h), _p(cp))
+    return h[:m.n_spec].copy(), cp[:m.n_spec].copy()
+
+
+def cp(m, T):
+    c = np.zeros(MAXS)
+    lib().orc_cp(T, C.byref(m), _p(c))
+    return c[:m.n_spec].copy()
+
+
+def gibbs_rt(m, T):
+    g = np.zeros(MAXS)
+    lib().orc_gibbs_rt(T, C.byref(m), _p(g))
+    return g[:m.n_spec].copy()
+
+
+def sutherland(T):
+    return lib().orc_sutherland(T)
+
+
+def transport(m, T, p, rho, Y):
+    Y = np.ascontiguousarray(Y, dtype=np.float64)
+    c = cp(m, T)
+    mu, lam = C.c_double(), C.c_double()
+    rhoD = np.zeros(MAXS)
+    c_full = np.zeros(MAXS)
+    c_full[:m.n_spec] = c
+    lib().orc_transport(C.byref(m), T, p, rho, _p(Y), _p(c_full), C.byref(mu), C.byref(lam), _p(rhoD))
+    return mu.value, lam.value, rhoD[:m.n_spec].copy()
+
+
+def chem_source(m, rho, T, Y, want_jac=True):
+    Y = np.ascontiguousarray(Y, dtype=np.float64)
+    om = np.zeros(MAXS)
+    jac = np.zeros(MAXS * MAXS)
+    jd = np.zeros(MAXS)
+    lib().orc_chem_source_point(C.byref(m), rho, T, _p(Y), _p(om), _p(jac) if want_jac else None, _p(jd))
+    ns = m.n_spec
+    return om[:ns].copy(), jac[:ns * ns].reshape(ns, ns).copy(), jd[:ns].copy()
+
+
+def rates(m, T, c):
+    c = np.ascontiguousarray(c, dtype=np.float64)
+    kf, kb = np.zeros(MAXR), np.zeros(MAXR)
+    lib().orc_forward_rate(C.byref(m), T, _p(c), _p(kf))
+    lib().orc_backward_rate(C.byref(m), T, _p(kf), _p(kb))
+    return kf[:m.n_reac].copy(), kb[:m.n_reac].copy()
+
+
+def progress(m, kf, kb, c):
+    kf = np.ascontiguousarray(kf, dtype=np.float64)
+    kb = np.ascontiguousarray(kb, dtype=np.float64)
+    c = np.ascontiguousarray(c, dtype=np.float64)
+    q, q1, q2 = np.zeros(MAXR), np.zeros(MAXR), np.zeros(MAXR)
+    lib().orc_rate_of_progress(C.byref(m), _p(kf), _p(kb), _p(c), _p(q), _p(q1), _p(q2))
+    nr = m.n_reac
+    return q[:nr].copy(), q1[:nr].copy(), q2[:nr].copy()
+
+
+def lu_solve(A, b):
+    A = np.ascontiguousarray(A, dtype=np.float64).copy()
+    b = np.ascontiguousarray(b, dtype=np.float64).copy()
+    lib().orc_lu_solve(_p(A), _p(b), len(b))
+    return b
+
+
+def temperature_newton(m, rho, rhoE, vel3, Y, t_start):
+    Y = np.ascontiguousarray(Y, dtype=np.float64)
+    v = np.ascontiguousarray(vel3, dtype=np.float64)
+    p = C.c_double()
+    it = C.c_int()
+    t = lib().orc_temperature_newton(C.byref(m), rho, rhoE, _p(v), _p(Y), t_start, C.byref(p), C.byref(it))
+    return t, p.value, it.value
+
+
+def chem_batch(m, rho, T, Y, dt):
+    """Y: (n_spec, n) SoA.  Returns dY (n_spec, n), dT (n)."""
+    rho = np.ascontiguousarray(rho, dtype=np.float64)
+    T = np.ascontiguousarray(T, dtype=np.float64)
+    Y = np.ascontiguousarray(Y, dtype=np.float64)
+    n = rho.shape[0]
+    dY = np.zeros_like(Y)
+    dT = np.zeros(n)
+    lib().orc_chem_batch(C.byref(m), n, _p(rho), _p(T), _p(Y), dt, _p(dY), _p(dT))
+    return dY, dT
+
+
+# ------------------------------------------------------------- inflow
+
+def inflow_state(mach, pressure, temperature, direction=(1.0, 0.0, 0.0), Y=None, m=None):
+    """BoundCond.cpp:7-122 Inflow::Inflow for the (mach, pressure, temperature)
+    combination: returns dict(rho,u,v,w,p,t,vel,sv)."""
+    if Y is None:
+        mw = MW_AIR
+        gamma = GAMMA_AIR
+        sv = np.zeros(0)
+    else:
+        sv = np.asarray(Y, dtype=np.float64)
+        mw = 0.0
+        for i in range(m.n_spec):
+            mw += sv[i] / m.mw[i]
+        mw = 1 / mw
+        cpi = cp(m, temperature)
+        cpt, cvt = 0.0, 0.0
+        for i in range(m.n_spec):
+            cpt += sv[i] * cpi[i]
+            cvt += sv[i] * (cpi[i] - R_U / m.mw[i])
+        gamma = cpt / cvt
+    c = np.sqrt(gamma * R_U / mw * temperature)
+    velocity = mach * c
+    u, v, w = (d * velocity for d in direction)
+    density = pressure * mw / (R_U * temperature)
+    return dict(rho=density, u=u, v=v, w=w, p=pressure, t=temperature, vel=velocity, sv=sv, mw=mw, gamma=gamma)
+
+
+# ------------------------------------------------------------- flow zone
+
+FIELDS = dict(x=0, y=1, z=2, jac=3, metric=4, bv=5, sv=6, cv=7, dq=8, vel=9, acoustic=10, mach=11, mul=12, cond=13,
+              rhoD=14, gamma=15, cp=16, dt_local=17, qn=18, bv_last=19, chem_jac=20)
+
+
+def make_cfg(case):
+    """case: dict with scheme keys and 'bcs' list of dict(type, raw, state...)."""
+    c = Cfg()
+    c.mix = case.get("mix", 0)
+    c.reaction = case.get("reaction", 0)
+    c.chemSrcMethod = case.get("chemSrcMethod", 0)
+    c.inviscid_scheme = case.get("inviscid_scheme", 3)
+    c.reconstruction = case.get("reconstruction", 2)
+    c.limiter = case.get("limiter", 0)
+    c.viscous_order = case.get("viscous_order", 2)
+    c.fixed_time_step = case.get("fixed_time_step", 0)
+    c.output_screen = case.get("output_screen", 1)
+    c.gradPInDiffusionFlux = case.get("gradPInDiffusionFlux", 0)
+    c.cfl = case.get("cfl", 0.5)
+    c.Pr = case.get("prandtl_number", 0.72)
+    c.Sc = case.get("schmidt_number", 0.9)
+    c.Prt = case.get("turbulent_prandtl_number", 0.9)
+    c.Sct = case.get("turbulent_schmidt_number", 0.9)
+    c.dt = case.get("dt", 1e-5)
+    c.entropy_fix_factor = case.get("entropy_fix_factor", 0.125)
+    c.n_bc = len(case["bcs"])
+    for n, b in enumerate(case["bcs"]):
+        B = c.bc[n]
+        B.type = b["type"]
+        B.raw[:] = list(b["raw"])
+        st = b.get("state")
+        if st is not None:
+            B.rho, B.u, B.v, B.w, B.p, B.t, B.vel = (st[k] for k in ("rho", "u", "v", "w", "p", "t", "vel"))
+            B.sv[:len(st["sv"])] = list(st["sv"])
+        B.t_wall = b.get("t_wall", 300.0)
+        B.isothermal = 1 if b.get("isothermal", True) else 0
+    return c
+
+
+class Zone:
+    """Flow oracle zone: reference DZone layout arrays in numpy views."""
+
+    def __init__(self, mx, my, mz, ngg, case, x, y, z, mech=None):
+        self.mx, self.my, self.mz, self.g = mx, my, mz, ngg
+        self.cfg = make_cfg(case)
+        self.mech = mech if mech is not None else Mech()
+        self._x = np.ascontiguousarray(x, dtype=np.float64).ravel()
+        self._y = np.ascontiguousarray(y, dtype=np.float64).ravel()
+        self._z = np.ascontiguousarray(z, dtype=np.float64).ravel()
+        self.h = lib().orc_zone_create(mx, my, mz, ngg, C.byref(self.cfg), C.byref(self.mech), _p(self._x),
+                                       _p(self._y), _p(self._z))
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_zone_destroy(self.h)
+            self.h = None
+
+    def field(self, name):
+        n = C.c_int64()
+        ptr = lib().orc_zone_field(self.h, FIELDS[name], C.byref(n))
+        if n.value == 0:
+            return np.zeros(0)
+        return np.ctypeslib.as_array(ptr, shape=(n.value,))
+
+    def initialize(self):
+        lib().orc_zone_initialize(self.h)
+
+    def rhs(self):
+        lib().orc_zone_rhs(self.h)
+
+    def global_dt(self):
+        return lib().orc_zone_global_dt(self.h)
+
+    def run(self, nsteps):
+        dt = np.zeros(nsteps * 3)
+        res = np.zeros(nsteps * 5)
+        lib().orc_zone_run_rk3(self.h, nsteps, _p(dt), _p(res))
+        return dt.reshape(nsteps, 3), res.reshape(nsteps, 5)
+
+    def state(self):
+        t, s = C.c_double(), C.c_int()
+        lib().orc_zone_state(self.h, C.byref(t), C.byref(s))
+        return t.value, s.value

@@ -1,0 +1,117 @@
+"""GPU parity of the air path (C1 2-D MUSCL/AUSM+, C2 3-D AWENO-5/CDS/viscous)
+against the FP64 CPU oracle on identical seeded inputs.
+
+Bars: FP64 RHS and fields within 1e-12 (normwise, per component) after one
+evaluation / step; 1e-9 after 100 steps (C1) and 20 steps (reduced C2)."""
+import numpy as np
+import pytest
+
+import ostc
+from ostc import cases
+from helpers import oracle_zone, per_component, relerr
+
+pytestmark = pytest.mark.gpu
+
+
+def _solver(case, tmp_path):
+    return ostc.solver_for_case(case, str(tmp_path / case["name"]))
+
+
+def _compare_state(s, z, case, tol, label):
+    bv_p, bv_o = s.download("bv"), z.field("bv")
+    errs = per_component(bv_p, bv_o, 6)
+    print(f"{label} bv err", ["%.2e" % e for e in errs], flush=True)
+    assert max(errs) < tol, (label, "bv", errs)
+    cv_p, cv_o = s.download("cv"), z.field("cv")
+    errs = per_component(cv_p, cv_o, 5)
+    print(f"{label} cv err", ["%.2e" % e for e in errs], flush=True)
+    assert max(errs) < tol, (label, "cv", errs)
+    for f in ("vel", "acoustic", "mul", "cond", "mach"):
+        e = relerr(s.download(f), z.field(f))
+        assert e < tol, (label, f, e)
+
+
+def test_c1_init_and_rhs(tmp_path):
+    case = cases.c1_case(60, 30)
+    s = _solver(case, tmp_path)
+    z = oracle_zone(case)
+    jac_p, met_p = s.download("jac"), s.download("metric")
+    assert relerr(jac_p, z.field("jac")) < 1e-14
+    met_o = z.field("metric").reshape(-1, 9).T.ravel()
+    assert relerr(met_p, met_o) < 1e-14
+    _compare_state(s, z, case, 1e-14, "c1-init")
+    s.rhs()
+    z.rhs()
+    errs = per_component(s.download("dq"), z.field("dq"), 5)
+    print("c1 rhs err", ["%.2e" % e for e in errs], flush=True)
+    assert max(errs) < 1e-12
+    dt_p, dt_o = s.global_dt(), z.global_dt()
+    assert abs(dt_p - dt_o) / dt_o < 1e-14
+    assert relerr(s.download("dt_local"), z.field("dt_local")) < 1e-14
+
+
+def test_c1_one_step(tmp_path):
+    case = cases.c1_case()
+    s = _solver(case, tmp_path)
+    z = oracle_zone(case)
+    dt_p, res_p = s.step(1)
+    dt_o, res_o = z.run(1)
+    assert relerr(dt_p, dt_o) < 1e-12
+    _compare_state(s, z, case, 1e-12, "c1-1step")
+    assert relerr(res_p[:, :4], res_o[:, :4]) < 1e-12
+
+
+def test_c1_hundred_steps(tmp_path):
+    case = cases.c1_case()
+    s = _solver(case, tmp_path)
+    z = oracle_zone(case)
+    dt_p, res_p = s.step(100)
+    dt_o, res_o = z.run(100)
+    print("c1 100-step dt err %.2e" % relerr(dt_p, dt_o), flush=True)
+    assert relerr(dt_p, dt_o) < 1e-9
+    _compare_state(s, z, case, 1e-9, "c1-100")
+    # residual history: relative error per entry
+    rel = np.abs(res_p[:, :4] - res_o[:, :4]) / np.maximum(np.abs(res_o[:, :4]), 1e-30)
+    print("c1 100-step residual history max rel %.2e" % rel.max(), flush=True)
+    assert rel.max() < 1e-9
+    t_p, step_p = s.time()
+    assert step_p == 100
+    assert abs(t_p - z.state()[0]) / z.state()[0] < 1e-12
+
+
+def test_c2_small_rhs_and_steps(tmp_path):
+    case = cases.c2_case(40, 24, 20)
+    s = _solver(case, tmp_path)
+    z = oracle_zone(case)
+    _compare_state(s, z, case, 1e-14, "c2-init")
+    s.rhs()
+    z.rhs()
+    errs = per_component(s.download("dq"), z.field("dq"), 5)
+    print("c2 rhs err", ["%.2e" % e for e in errs], flush=True)
+    assert max(errs) < 1e-12
+    dt_p, _ = s.step(1)
+    dt_o, _ = z.run(1)
+    _compare_state(s, z, case, 1e-12, "c2-1step")
+    dt_p, res_p = s.step(19)
+    dt_o, res_o = z.run(19)
+    _compare_state(s, z, case, 1e-9, "c2-20step")
+    assert relerr(dt_p, dt_o) < 1e-9
+
+
+def test_c2_small_mixed_precision_error(tmp_path):
+    """FP32 flux arithmetic against the FP64 oracle: per-field L2 / Linf
+    relative errors after 10 steps are reported and bounded."""
+    case = cases.c2_case(40, 24, 20)
+    s = _solver(case, tmp_path)
+    s.set_flux_fp32(True)
+    z = oracle_zone(case)
+    s.step(10)
+    z.run(10)
+    bv_p = s.download("bv").reshape(6, -1)
+    bv_o = z.field("bv").reshape(6, -1)
+    for l, name in enumerate(("rho", "u", "v", "w", "p", "T")):
+        d = bv_p[l] - bv_o[l]
+        linf = np.max(np.abs(d)) / np.max(np.abs(bv_o[l]))
+        l2 = np.linalg.norm(d) / np.linalg.norm(bv_o[l])
+        print(f"mixed {name}: L2 {l2:.2e} Linf {linf:.2e}", flush=True)
+        assert linf < 1e-3
