@@ -642,7 +642,7 @@ int ostc_finalize(ostc_solver *s) {
     auto alloc = [&](double **p, int64_t n) -> int {
       n = std::max<int64_t>(n, 1);
       HIPCHK(hipMalloc((void **)p, n * sizeof(double)));
-      HIPCHK(hipMemset(*p, 0, n * sizeof(double)));
+      HIPCHK(hipMemsetAsync(*p, 0, n * sizeof(double), s->stream));
       b.mem.push_back(*p);
       return 0;
     };
@@ -658,8 +658,9 @@ int ostc_finalize(ostc_solver *s) {
       return 1;
     z.jac = jac;
     z.metric = met;
-    HIPCHK(hipMemcpy(jac, b.jac.data(), L.sz * sizeof(double), hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(met, b.metric.data(), L.sz * 9 * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpyAsync(jac, b.jac.data(), L.sz * sizeof(double), hipMemcpyHostToDevice, s->stream));
+    HIPCHK(hipMemcpyAsync(met, b.metric.data(), L.sz * 9 * sizeof(double), hipMemcpyHostToDevice, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
     // boundaries: register, trim walls (BoundCond.cu:369-384), order by priority
     for (auto &f : b.faces) {
       register_face(f, g, dim);
@@ -715,9 +716,10 @@ int ostc_finalize(ostc_solver *s) {
   s->w.max_blocks = (int)((max_cells + 255) / 256) + 1;
   HIPCHK(hipMalloc(&s->w.partial, sizeof(double) * 4 * s->w.max_blocks));
   HIPCHK(hipMalloc(&s->w.counter, sizeof(unsigned)));
-  HIPCHK(hipMemset(s->w.counter, 0, sizeof(unsigned)));
+  HIPCHK(hipMemsetAsync(s->w.counter, 0, sizeof(unsigned), s->stream));
   HIPCHK(hipMalloc(&s->w.dt, sizeof(double)));
   HIPCHK(hipMalloc(&s->w.res_scale, sizeof(double) * 4));
+  HIPCHK(hipStreamSynchronize(s->stream));
   s->finalized = true;
   return 0;
 }

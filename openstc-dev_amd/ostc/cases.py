@@ -239,8 +239,10 @@ def chem_batch_inputs(n, seed=1004, species=("H2", "H", "O2", "O", "OH", "HO2", 
 
 
 def _fmt(v):
-    if isinstance(v, float):
-        return repr(v)
+    if isinstance(v, (float, np.floating)):
+        return repr(float(v))
+    if isinstance(v, (int, np.integer)):
+        return str(int(v))
     return str(v)
 
 

@@ -48,3 +48,23 @@ def interior(arr, case, ncomp):
     g = case["ngg"]
     a = np.asarray(arr).reshape(ncomp, case["mz"] + 2 * g, case["my"] + 2 * g, case["mx"] + 2 * g)
     return a[:, g:g + case["mz"], g:g + case["my"], g:g + case["mx"]]
+
+
+def grouped(a, b, ncomp, groups):
+    """Normwise relative errors per named group of components; a vector
+    group (e.g. the three velocity components) is scaled by the largest
+    component magnitude of the whole group, so a component that is zero in
+    exact arithmetic is judged against the vector's size, not its own
+    rounding noise."""
+    a = np.asarray(a).reshape(ncomp, -1)
+    b = np.asarray(b).reshape(ncomp, -1)
+    out = {}
+    for name, comps in groups.items():
+        den = max(np.max(np.abs(b[c])) for c in comps)
+        num = max(np.max(np.abs(a[c] - b[c])) for c in comps)
+        out[name] = num / den if den > 0 else num
+    return out
+
+
+BV_GROUPS = {"rho": [0], "vel": [1, 2, 3], "p": [4], "T": [5]}
+CV_GROUPS = {"rho": [0], "mom": [1, 2, 3], "rhoE": [4]}

@@ -8,7 +8,7 @@ import pytest
 
 import ostc
 from ostc import cases
-from helpers import oracle_zone, per_component, relerr
+from helpers import BV_GROUPS, CV_GROUPS, grouped, oracle_zone, relerr
 
 pytestmark = pytest.mark.gpu
 
@@ -18,14 +18,12 @@ def _solver(case, tmp_path):
 
 
 def _compare_state(s, z, case, tol, label):
-    bv_p, bv_o = s.download("bv"), z.field("bv")
-    errs = per_component(bv_p, bv_o, 6)
-    print(f"{label} bv err", ["%.2e" % e for e in errs], flush=True)
-    assert max(errs) < tol, (label, "bv", errs)
-    cv_p, cv_o = s.download("cv"), z.field("cv")
-    errs = per_component(cv_p, cv_o, 5)
-    print(f"{label} cv err", ["%.2e" % e for e in errs], flush=True)
-    assert max(errs) < tol, (label, "cv", errs)
+    errs = grouped(s.download("bv"), z.field("bv"), 6, BV_GROUPS)
+    print(f"{label} bv err", {k: "%.2e" % v for k, v in errs.items()}, flush=True)
+    assert max(errs.values()) < tol, (label, "bv", errs)
+    errs = grouped(s.download("cv"), z.field("cv"), 5, CV_GROUPS)
+    print(f"{label} cv err", {k: "%.2e" % v for k, v in errs.items()}, flush=True)
+    assert max(errs.values()) < tol, (label, "cv", errs)
     for f in ("vel", "acoustic", "mul", "cond", "mach"):
         e = relerr(s.download(f), z.field(f))
         assert e < tol, (label, f, e)
@@ -42,9 +40,9 @@ def test_c1_init_and_rhs(tmp_path):
     _compare_state(s, z, case, 1e-14, "c1-init")
     s.rhs()
     z.rhs()
-    errs = per_component(s.download("dq"), z.field("dq"), 5)
-    print("c1 rhs err", ["%.2e" % e for e in errs], flush=True)
-    assert max(errs) < 1e-12
+    errs = grouped(s.download("dq"), z.field("dq"), 5, CV_GROUPS)
+    print("c1 rhs err", {k: "%.2e" % v for k, v in errs.items()}, flush=True)
+    assert max(errs.values()) < 1e-12
     dt_p, dt_o = s.global_dt(), z.global_dt()
     assert abs(dt_p - dt_o) / dt_o < 1e-14
     assert relerr(s.download("dt_local"), z.field("dt_local")) < 1e-14
@@ -70,10 +68,15 @@ def test_c1_hundred_steps(tmp_path):
     print("c1 100-step dt err %.2e" % relerr(dt_p, dt_o), flush=True)
     assert relerr(dt_p, dt_o) < 1e-9
     _compare_state(s, z, case, 1e-9, "c1-100")
-    # residual history: relative error per entry
-    rel = np.abs(res_p[:, :4] - res_o[:, :4]) / np.maximum(np.abs(res_o[:, :4]), 1e-30)
-    print("c1 100-step residual history max rel %.2e" % rel.max(), flush=True)
+    # residual history.  In this case |V| and p are constant in exact
+    # arithmetic (a density perturbation advected at uniform u and p), so
+    # their residuals are rounding noise on both sides; the density and
+    # temperature residuals carry the signal and are compared entry by entry.
+    sig = [0, 3]
+    rel = np.abs(res_p[:, sig] - res_o[:, sig]) / np.abs(res_o[:, sig])
+    print("c1 100-step residual history (rho, T) max rel %.2e" % rel.max(), flush=True)
     assert rel.max() < 1e-9
+    assert np.isfinite(res_p).all()
     t_p, step_p = s.time()
     assert step_p == 100
     assert abs(t_p - z.state()[0]) / z.state()[0] < 1e-12
@@ -86,9 +89,9 @@ def test_c2_small_rhs_and_steps(tmp_path):
     _compare_state(s, z, case, 1e-14, "c2-init")
     s.rhs()
     z.rhs()
-    errs = per_component(s.download("dq"), z.field("dq"), 5)
-    print("c2 rhs err", ["%.2e" % e for e in errs], flush=True)
-    assert max(errs) < 1e-12
+    errs = grouped(s.download("dq"), z.field("dq"), 5, CV_GROUPS)
+    print("c2 rhs err", {k: "%.2e" % v for k, v in errs.items()}, flush=True)
+    assert max(errs.values()) < 1e-12
     dt_p, _ = s.step(1)
     dt_o, _ = z.run(1)
     _compare_state(s, z, case, 1e-12, "c2-1step")
@@ -109,9 +112,9 @@ def test_c2_small_mixed_precision_error(tmp_path):
     z.run(10)
     bv_p = s.download("bv").reshape(6, -1)
     bv_o = z.field("bv").reshape(6, -1)
-    for l, name in enumerate(("rho", "u", "v", "w", "p", "T")):
-        d = bv_p[l] - bv_o[l]
-        linf = np.max(np.abs(d)) / np.max(np.abs(bv_o[l]))
-        l2 = np.linalg.norm(d) / np.linalg.norm(bv_o[l])
+    for name, comps in BV_GROUPS.items():
+        d = bv_p[comps] - bv_o[comps]
+        linf = np.max(np.abs(d)) / np.max(np.abs(bv_o[comps]))
+        l2 = np.linalg.norm(d) / np.linalg.norm(bv_o[comps])
         print(f"mixed {name}: L2 {l2:.2e} Linf {linf:.2e}", flush=True)
         assert linf < 1e-3
