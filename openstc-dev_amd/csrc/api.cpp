@@ -353,8 +353,11 @@ int rhs_block(ostc_solver *s, Block &b) {
     HIPCHK(launch_source(b.z, s->P, s->d_mech, s->stream));
     init = false;
   }
-  ostc_solver::Timer t(s, "flux_sweeps");
-  HIPCHK(launch_flux_sweeps(b.z, s->P, s->d_mech, init, s->fp32, s->stream));
+  static const char *names[3] = {"sweep_i", "sweep_j", "sweep_k"};
+  for (int d = 0; d < s->P.dim; ++d) {
+    ostc_solver::Timer t(s, names[d]);
+    HIPCHK(launch_flux_sweep(b.z, s->P, s->d_mech, d, init && d == 0, s->fp32, s->stream));
+  }
   return 0;
 }
 

@@ -12,9 +12,9 @@
 //  * i-sweep: a wavefront owns 64 consecutive faces (the 63 cells between
 //    them); the left face of a cell comes from the neighbouring lane through
 //    a cross-lane shuffle.  Loads along i are coalesced.
-//  * j- and k-sweeps: lanes run along i (coalesced), each thread marches a
-//    segment of kSeg cells along the sweep direction, carrying the previous
-//    face flux in registers (one extra face per segment).
+//  * j- and k-sweeps: lanes run along i (coalesced); the 8 wavefronts of a
+//    workgroup own 8 consecutive faces along the sweep direction and pass
+//    face fluxes to the next row through LDS (7 cells per 8 faces).
 //
 // The G form of the CDS term: with node fluxes f, the reference's
 //   c1 (f[+3]-f[-3]) + c2 (f[+2]-f[-2]) + c3 (f[+1]-f[-1])
@@ -28,8 +28,6 @@
 namespace ostc {
 
 namespace {
-
-constexpr int kSeg = 16;
 
 template <typename R>
 __device__ inline R minmod(R a, R b) {
@@ -609,67 +607,89 @@ __global__ void __launch_bounds__(256) sweep_i(ZoneView z, DParams P, const DMec
   if (live && lane > 0) add_dq<NS, R>(z, z.at0(i, j, k), init != 0, F, Fp, i < ncds, G, Gp);
 }
 
-// j- and k-direction sweeps: lanes along i, each thread marches kSeg cells.
+// j- and k-direction sweeps: lanes along i (coalesced), kRows wavefronts
+// per workgroup own kRows consecutive faces along the sweep direction; each
+// thread computes one face and the left face of its cell comes from the
+// previous row through LDS (one redundant face per kRows - 1 cells).
+template <int NS>
+struct JkRows {
+  // 8 rows (512 threads, <= 256 VGPRs) for air; 4 for mixtures, whose face
+  // arithmetic needs more registers than a 512-thread block allows
+  static constexpr int value = NS > 0 ? 4 : 8;
+};
+
 template <int DIR, int NS, int SCHEME, typename R>
-__global__ void __launch_bounds__(256) sweep_jk(ZoneView z, DParams P, const DMech *__restrict__ m, int init) {
+__global__ void __launch_bounds__(64 * JkRows<NS>::value) sweep_jk(ZoneView z, DParams P,
+                                                                   const DMech *__restrict__ m, int init) {
   constexpr int NV = 5 + NS;
-  const int i = blockIdx.x * 64 + threadIdx.x;
-  const int o = blockIdx.y * blockDim.y + threadIdx.y;  // k for DIR 1, j for DIR 2
+  constexpr int kRows = JkRows<NS>::value;
+  __shared__ R sF[kRows][NV][64];
+  __shared__ R sG[kRows][5][64];
+  const int lane = threadIdx.x, r = threadIdx.y;
+  const int i = blockIdx.x * 64 + lane;
+  const int o = blockIdx.y;  // k for DIR 1, j for DIR 2
   const int n = DIR == 1 ? z.my : z.mz;
-  const int s0 = blockIdx.z * kSeg;
-  if (i >= z.mx || o >= (DIR == 1 ? z.mz : z.my) || s0 >= n) return;
+  const int s = blockIdx.z * (kRows - 1) + r - 1;  // this thread's face is s + 1/2
+  const bool live = i < z.mx && s < n;
   const int64_t st = DIR == 1 ? z.d2 : z.d1;
-  const int64_t st0 = DIR == 1 ? z.e2 : z.e1;
   const int ncds = SCHEME == 1 ? n - 2 * z.g : 0;
-  auto pidx = [&](int s) { return DIR == 1 ? z.at(i, s, o) : z.at(i, o, s); };
-  const int64_t q00 = DIR == 1 ? z.at0(i, 0, o) : z.at0(i, o, 0);
-  R Fp[NV], Gp[5], F[NV], G[5];
+  R F[NV], G[5];
 #pragma unroll
-  for (int l = 0; l < 5; ++l) Gp[l] = R(0);
-  face_flux<NS, SCHEME, R>(z, P, m, DIR, pidx(s0 - 1), st, s0 - 1 <= ncds - 1, Fp, Gp);
-  const int s1 = min(s0 + kSeg, n);
-  for (int s = s0; s < s1; ++s) {
+  for (int l = 0; l < NV; ++l) F[l] = R(0);
 #pragma unroll
-    for (int l = 0; l < 5; ++l) G[l] = R(0);
-    face_flux<NS, SCHEME, R>(z, P, m, DIR, pidx(s), st, s <= ncds - 1, F, G);
-    add_dq<NS, R>(z, q00 + (int64_t)s * st0, init != 0, F, Fp, s < ncds, G, Gp);
+  for (int l = 0; l < 5; ++l) G[l] = R(0);
+  if (live) {
+    const int64_t p0 = DIR == 1 ? z.at(i, s, o) : z.at(i, o, s);
+    face_flux<NS, SCHEME, R>(z, P, m, DIR, p0, st, s <= ncds - 1, F, G);
+  }
 #pragma unroll
-    for (int l = 0; l < NV; ++l) Fp[l] = F[l];
+  for (int l = 0; l < NV; ++l) sF[r][l][lane] = F[l];
 #pragma unroll
-    for (int l = 0; l < 5; ++l) Gp[l] = G[l];
+  for (int l = 0; l < 5; ++l) sG[r][l][lane] = G[l];
+  __syncthreads();
+  if (live && r > 0) {
+    R Fp[NV], Gp[5];
+#pragma unroll
+    for (int l = 0; l < NV; ++l) Fp[l] = sF[r - 1][l][lane];
+#pragma unroll
+    for (int l = 0; l < 5; ++l) Gp[l] = sG[r - 1][l][lane];
+    const int64_t q = DIR == 1 ? z.at0(i, s, o) : z.at0(i, o, s);
+    add_dq<NS, R>(z, q, init != 0, F, Fp, s < ncds, G, Gp);
   }
 }
 
 }  // namespace
 
 template <int NS, int SCHEME, typename R>
-hipError_t launch_sweeps_t(const ZoneView &z, const DParams &P, const DMech *m, bool init, hipStream_t s) {
-  const dim3 tb(64, 4, 1);
-  {
+hipError_t launch_sweep_t(const ZoneView &z, const DParams &P, const DMech *m, int dir, bool init, hipStream_t s) {
+  if (dir == 0) {
     const dim3 gr((z.mx + 62) / 63, (z.my + 3) / 4, z.mz);
-    hipLaunchKernelGGL((sweep_i<NS, SCHEME, R>), gr, tb, 0, s, z, P, m, init ? 1 : 0);
-  }
-  {
-    const dim3 gr((z.mx + 63) / 64, (z.mz + 3) / 4, (z.my + kSeg - 1) / kSeg);
-    hipLaunchKernelGGL((sweep_jk<1, NS, SCHEME, R>), gr, tb, 0, s, z, P, m, 0);
-  }
-  if (z.mz > 1) {
-    const dim3 gr((z.mx + 63) / 64, (z.my + 3) / 4, (z.mz + kSeg - 1) / kSeg);
-    hipLaunchKernelGGL((sweep_jk<2, NS, SCHEME, R>), gr, tb, 0, s, z, P, m, 0);
+    hipLaunchKernelGGL((sweep_i<NS, SCHEME, R>), gr, dim3(64, 4, 1), 0, s, z, P, m, init ? 1 : 0);
+  } else if (dir == 1) {
+    constexpr int kRows = JkRows<NS>::value;
+    const dim3 gr((z.mx + 63) / 64, z.mz, (z.my + kRows - 2) / (kRows - 1));
+    hipLaunchKernelGGL((sweep_jk<1, NS, SCHEME, R>), gr, dim3(64, kRows, 1), 0, s, z, P, m, init ? 1 : 0);
+  } else {
+    constexpr int kRows = JkRows<NS>::value;
+    const dim3 gr((z.mx + 63) / 64, z.my, (z.mz + kRows - 2) / (kRows - 1));
+    hipLaunchKernelGGL((sweep_jk<2, NS, SCHEME, R>), gr, dim3(64, kRows, 1), 0, s, z, P, m, init ? 1 : 0);
   }
   return hipGetLastError();
 }
 
-hipError_t launch_flux_sweeps(const ZoneView &z, const DParams &P, const DMech *m, bool init, bool fp32,
-                              hipStream_t s) {
+hipError_t launch_flux_sweep(const ZoneView &z, const DParams &P, const DMech *m, int dir, bool init, bool fp32,
+                             hipStream_t s) {
   const bool aweno = P.inviscid_type == 1;
   if (P.ns == 0) {
     if (aweno)
-      return fp32 ? launch_sweeps_t<0, 1, float>(z, P, m, init, s) : launch_sweeps_t<0, 1, double>(z, P, m, init, s);
-    return fp32 ? launch_sweeps_t<0, 0, float>(z, P, m, init, s) : launch_sweeps_t<0, 0, double>(z, P, m, init, s);
+      return fp32 ? launch_sweep_t<0, 1, float>(z, P, m, dir, init, s)
+                  : launch_sweep_t<0, 1, double>(z, P, m, dir, init, s);
+    return fp32 ? launch_sweep_t<0, 0, float>(z, P, m, dir, init, s)
+                : launch_sweep_t<0, 0, double>(z, P, m, dir, init, s);
   }
   if (P.ns == 9 && !aweno)
-    return fp32 ? launch_sweeps_t<9, 0, float>(z, P, m, init, s) : launch_sweeps_t<9, 0, double>(z, P, m, init, s);
+    return fp32 ? launch_sweep_t<9, 0, float>(z, P, m, dir, init, s)
+                : launch_sweep_t<9, 0, double>(z, P, m, dir, init, s);
   return hipErrorNotSupported;
 }
 

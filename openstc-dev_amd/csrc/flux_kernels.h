@@ -27,8 +27,9 @@ struct StepScratch {
   int max_blocks;
 };
 
-hipError_t launch_flux_sweeps(const ZoneView &z, const DParams &P, const DMech *m, bool init, bool fp32,
-                              hipStream_t s);
+// one direction (0 = i, 1 = j, 2 = k); init: the sweep writes dq instead of accumulating
+hipError_t launch_flux_sweep(const ZoneView &z, const DParams &P, const DMech *m, int dir, bool init, bool fp32,
+                             hipStream_t s);
 hipError_t launch_source(const ZoneView &z, const DParams &P, const DMech *m, hipStream_t s);
 hipError_t launch_local_dt(const ZoneView &z, const DParams &P, const StepScratch &w, double *dt_out,
                            hipStream_t s);

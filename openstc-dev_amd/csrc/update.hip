@@ -6,6 +6,8 @@
 #include "device_common.h"
 #include "flux_kernels.h"
 
+#include <algorithm>
+
 namespace ostc {
 
 namespace {
@@ -62,14 +64,16 @@ __global__ void __launch_bounds__(kTB) k_source(ZoneView z, DParams P, const DMe
 
 // ------------------------------------------------------------- dt
 // TimeAdvanceFunc.cuh:50-100 local_time_step, then the global minimum
-// (TimeAdvanceFunc.cu:155-201) by a last-block reduction: no host round trip.
+// (TimeAdvanceFunc.cu:155-201) in two passes on the device: a grid-stride
+// pass writes one partial per block, a one-block pass reduces them.  No
+// atomics, no host round trip; min is order-independent, hence exact.
+constexpr int kRedBlocks = 1024;
+
 template <int MIX>
-__global__ void __launch_bounds__(kTB) k_local_dt(ZoneView z, DParams P, double *partial, unsigned *counter,
-                                                   double *dt_out) {
-  const int64_t t = (int64_t)blockIdx.x * kTB + threadIdx.x;
+__global__ void __launch_bounds__(kTB) k_local_dt(ZoneView z, DParams P, double *partial) {
   const int64_t n = (int64_t)z.mx * z.my * z.mz;
-  double dtl = 1e+6;
-  if (t < n) {
+  double best = 1e+6;
+  for (int64_t t = (int64_t)blockIdx.x * kTB + threadIdx.x; t < n; t += (int64_t)gridDim.x * kTB) {
     int i, j, k;
     cell_of(z, t, i, j, k);
     const int64_t p = z.at(i, j, k);
@@ -93,28 +97,26 @@ __global__ void __launch_bounds__(kTB) k_local_dt(ZoneView z, DParams P, double 
     double srv = gxi * gxi + geta * geta;
     if (P.dim == 3) srv += gzeta * gzeta;
     srv *= c1 * c2;
-    dtl = P.cfl / (sri + srv);
+    const double dtl = P.cfl / (sri + srv);
     z.dt_local[z.at0(i, j, k)] = dtl;
+    best = fmin(best, dtl);
   }
   __shared__ double sm[kTB / 64];
-  __shared__ bool last;
-  double v = wave_min(dtl);
+  const double v = wave_min(best);
   if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = v;
   __syncthreads();
   if (threadIdx.x == 0) {
-    double b = sm[0];
+    double r = sm[0];
 #pragma unroll
-    for (int w = 1; w < kTB / 64; ++w) b = fmin(b, sm[w]);
-    partial[blockIdx.x] = b;
-    __threadfence();
-    const unsigned ticket = atomicAdd(counter, 1u);
-    last = ticket == gridDim.x - 1;
+    for (int w = 1; w < kTB / 64; ++w) r = fmin(r, sm[w]);
+    partial[blockIdx.x] = r;
   }
-  __syncthreads();
-  if (!last) return;
-  __threadfence();
+}
+
+__global__ void __launch_bounds__(kTB) k_min_final(const double *partial, int nb, double *out) {
   double r = 1e+6;
-  for (unsigned b = threadIdx.x; b < gridDim.x; b += kTB) r = fmin(r, __hip_atomic_load(&partial[b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+  for (int b = threadIdx.x; b < nb; b += kTB) r = fmin(r, partial[b]);
+  __shared__ double sm[kTB / 64];
   r = wave_min(r);
   if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = r;
   __syncthreads();
@@ -122,8 +124,7 @@ __global__ void __launch_bounds__(kTB) k_local_dt(ZoneView z, DParams P, double 
     double b = sm[0];
 #pragma unroll
     for (int w = 1; w < kTB / 64; ++w) b = fmin(b, sm[w]);
-    *dt_out = b;
-    *counter = 0u;
+    *out = b;
   }
 }
 
@@ -437,14 +438,12 @@ __global__ void __launch_bounds__(kTB) k_store_last(ZoneView z) {
 }
 
 // Residual.cuh:57-161 + TimeAdvanceFunc.cu:19-33: squared change of
-// (rho, |V|, p, T), summed; the last block finishes the RMS, the step-1
-// scale (floor 1e-20) and err_max, all on the device.
-__global__ void __launch_bounds__(kTB) k_residual(ZoneView z, double *partial, unsigned *counter, double *res_scale,
-                                                   double *out, int first) {
-  const int64_t t = (int64_t)blockIdx.x * kTB + threadIdx.x;
+// (rho, |V|, p, T) summed per block (grid-stride), then one block finishes
+// the RMS, the step-1 scale (floor 1e-20) and err_max on the device.
+__global__ void __launch_bounds__(kTB) k_residual(ZoneView z, double *partial) {
   const int64_t n = (int64_t)z.mx * z.my * z.mz;
   double s[4] = {0, 0, 0, 0};
-  if (t < n) {
+  for (int64_t t = (int64_t)blockIdx.x * kTB + threadIdx.x; t < n; t += (int64_t)gridDim.x * kTB) {
     int i, j, k;
     cell_of(z, t, i, j, k);
     const int64_t p = z.at(i, j, k), q = z.at0(i, j, k);
@@ -453,11 +452,10 @@ __global__ void __launch_bounds__(kTB) k_residual(ZoneView z, double *partial, u
     for (int l = 0; l < 4; ++l) {
       const double d = cur[l] - z.bv_last[l * z.sz0 + q];
       z.bv_last[l * z.sz0 + q] = d * d;
-      s[l] = d * d;
+      s[l] += d * d;
     }
   }
   __shared__ double sm[4][kTB / 64];
-  __shared__ bool last;
 #pragma unroll
   for (int l = 0; l < 4; ++l) {
     const double v = wave_sum(s[l]);
@@ -472,18 +470,16 @@ __global__ void __launch_bounds__(kTB) k_residual(ZoneView z, double *partial, u
       for (int w = 0; w < kTB / 64; ++w) b += sm[l][w];
       partial[(size_t)l * gridDim.x + blockIdx.x] = b;
     }
-    __threadfence();
-    last = atomicAdd(counter, 1u) == gridDim.x - 1;
   }
-  __syncthreads();
-  if (!last) return;
-  __threadfence();
+}
+
+__global__ void __launch_bounds__(kTB) k_residual_final(const double *partial, int nb, int64_t n, double *res_scale,
+                                                         double *out, int first) {
   double r[4] = {0, 0, 0, 0};
-  for (unsigned b = threadIdx.x; b < gridDim.x; b += kTB)
+  for (int b = threadIdx.x; b < nb; b += kTB)
 #pragma unroll
-    for (int l = 0; l < 4; ++l)
-      r[l] += __hip_atomic_load(&partial[(size_t)l * gridDim.x + b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  __syncthreads();
+    for (int l = 0; l < 4; ++l) r[l] += partial[(size_t)l * nb + b];
+  __shared__ double sm[4][kTB / 64];
 #pragma unroll
   for (int l = 0; l < 4; ++l) {
     const double v = wave_sum(r[l]);
@@ -507,7 +503,6 @@ __global__ void __launch_bounds__(kTB) k_residual(ZoneView z, double *partial, u
     for (int l = 1; l < 4; ++l)
       if (fabs(res[l]) > e) e = res[l];
     out[4] = e;
-    *counter = 0u;
   }
 }
 
@@ -570,11 +565,13 @@ hipError_t launch_source(const ZoneView &z, const DParams &P, const DMech *m, hi
 
 hipError_t launch_local_dt(const ZoneView &z, const DParams &P, const StepScratch &w, double *dt_out, hipStream_t s) {
   const int64_t n = (int64_t)z.mx * z.my * z.mz;
-  if ((int64_t)nblk(n, kTB) > w.max_blocks) return hipErrorInvalidValue;
+  const int nb = (int)std::min<int64_t>(kRedBlocks, nblk(n, kTB));
+  if (nb > w.max_blocks) return hipErrorInvalidValue;
   if (P.mix)
-    hipLaunchKernelGGL(k_local_dt<1>, dim3(nblk(n, kTB)), dim3(kTB), 0, s, z, P, w.partial, w.counter, dt_out);
+    hipLaunchKernelGGL(k_local_dt<1>, dim3(nb), dim3(kTB), 0, s, z, P, w.partial);
   else
-    hipLaunchKernelGGL(k_local_dt<0>, dim3(nblk(n, kTB)), dim3(kTB), 0, s, z, P, w.partial, w.counter, dt_out);
+    hipLaunchKernelGGL(k_local_dt<0>, dim3(nb), dim3(kTB), 0, s, z, P, w.partial);
+  hipLaunchKernelGGL(k_min_final, dim3(1), dim3(kTB), 0, s, (const double *)w.partial, nb, dt_out);
   return hipGetLastError();
 }
 
@@ -656,8 +653,10 @@ hipError_t launch_store_last(const ZoneView &z, hipStream_t s) {
 
 hipError_t launch_residual(const ZoneView &z, const StepScratch &w, int slot, int first, hipStream_t s) {
   const int64_t n = (int64_t)z.mx * z.my * z.mz;
-  if ((int64_t)nblk(n, kTB) * 4 > (int64_t)w.max_blocks * 4) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(k_residual, dim3(nblk(n, kTB)), dim3(kTB), 0, s, z, w.partial, w.counter, w.res_scale,
+  const int nb = (int)std::min<int64_t>(kRedBlocks, nblk(n, kTB));
+  if (nb > w.max_blocks) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_residual, dim3(nb), dim3(kTB), 0, s, z, w.partial);
+  hipLaunchKernelGGL(k_residual_final, dim3(1), dim3(kTB), 0, s, (const double *)w.partial, nb, n, w.res_scale,
                      w.res_hist + (size_t)slot * 5, first);
   return hipGetLastError();
 }
