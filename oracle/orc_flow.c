@@ -915,7 +915,8 @@ static void weno5(const double *L, const double *cvb, int ip, int row, double *v
  * eigenvectors along the normalised averaged metric, WENO5 per
  * characteristic, right eigenvectors back to conservative, then primitives.
  * cvb holds (rho, rho u, rho v, rho w, rho E, p, T) per point. */
-static void aweno_air(const double *cvb, const double *met, int ip, double *pl, double *pr) {
+static void aweno_air_m(const double *cvb, const double *met, int ip, double *pl, double *pr, double *Lout,
+                        double *Rout) {
   const double *cl = &cvb[ip * 7], *cr = &cvb[(ip + 1) * 7];
   const double rlc = sqrt(cl[0]) / (sqrt(cl[0]) + sqrt(cr[0]));
   const double rrc = sqrt(cr[0]) / (sqrt(cl[0]) + sqrt(cr[0]));
@@ -964,6 +965,7 @@ static void aweno_air(const double *cvb, const double *met, int ip, double *pl, 
   LR[23] = -(gm1 * wm - kz * cm) / cm2 * 0.5;
   LR[24] = gm1 / cm2 * 0.5;
   double vp[5], vmn[5];
+  if (Lout) memcpy(Lout, LR, sizeof(LR));
   for (int l = 0; l < 5; ++l) weno5(LR, cvb, ip, l, &vmn[l], &vp[l]);
   LR[0] = 1.0;
   LR[1] = kx;
@@ -990,6 +992,7 @@ static void aweno_air(const double *cvb, const double *met, int ip, double *pl, 
   LR[22] = ky * alpha / gm1 + (kx * wm - kz * um) * cm;
   LR[23] = kz * alpha / gm1 + (ky * um - kx * vm) * cm;
   LR[24] = hm + Uk * cm;
+  if (Rout) memcpy(Rout, LR, sizeof(LR));
   double ql[5], qr[5];
   for (int a = 0; a < 5; ++a) {
     ql[a] = 0;
@@ -1011,6 +1014,10 @@ static void aweno_air(const double *cvb, const double *met, int ip, double *pl, 
   pr[3] = qr[3] / qr[0];
   pr[4] = gm1 * (qr[4] - 0.5 * (pr[1] * pr[1] + pr[2] * pr[2] + pr[3] * pr[3]) * qr[0]);
   pr[5] = qr[4];
+}
+
+static void aweno_air(const double *cvb, const double *met, int ip, double *pl, double *pr) {
+  aweno_air_m(cvb, met, ip, pl, pr, NULL, NULL);
 }
 
 /* line gather helpers: position s on line (dir, a, b) -> (i,j,k) */
@@ -1294,8 +1301,11 @@ double orc_zone_global_dt(orc_zone *zz) {
 }
 
 /* RK.cuh:10-14 SSPRK3 coefficients and RK.cuh:197-253 update_cv_and_bv_rk */
+static const double RK_A[3] = {1.0, 0.75, 1.0 / 3.0}, RK_B[3] = {0.0, 0.25, 2.0 / 3.0},
+                    RK_C[3] = {1.0, 0.25, 2.0 / 3.0};
+
 static void rk_update(orc_zone *zz, double dt, int rk) {
-  static const double A[3] = {1.0, 0.75, 1.0 / 3.0}, B[3] = {0.0, 0.25, 2.0 / 3.0}, C[3] = {1.0, 0.25, 2.0 / 3.0};
+  const double *A = RK_A, *B = RK_B, *C = RK_C;
   for (int k = 0; k < zz->mz; ++k)
     for (int j = 0; j < zz->my; ++j)
       for (int i = 0; i < zz->mx; ++i) {
@@ -1396,5 +1406,59 @@ void orc_zone_run_rk3(orc_zone *zz, int nsteps, double *dt_hist, double *res_his
     } else if (res_hist) {
       for (int l = 0; l < 5; ++l) res_hist[it * 5 + l] = NAN;
     }
+  }
+}
+
+/* ------------------------------------------------------- test hooks */
+/* Point-level access to the oracle's own face routines so the tests can pin
+ * them against closed forms (consistency, supersonic upwinding, exactness of
+ * WENO5 on low-order data, L R = I, the RK3 stability polynomial). */
+
+/* air face flux: pl/pr = (rho,u,v,w,p) on each side, met6 = metric row of
+ * the sweep direction at the left and right points, jac2 their Jacobians */
+void orc_face_flux_air(int scheme, const double *pl5, const double *pr5, const double *met6, const double *jac2,
+                       double *f5) {
+  orc_zone zz;
+  memset(&zz, 0, sizeof(zz));
+  zz.nv = 5;
+  zz.cfg.inviscid_scheme = scheme;
+  double pl[8], pr[8];
+  for (int l = 0; l < 5; ++l) {
+    pl[l] = pl5[l];
+    pr[l] = pr5[l];
+  }
+  pl[5] = 0.5 * (pl[1] * pl[1] + pl[2] * pl[2] + pl[3] * pl[3]) * pl[0] + pl[4] / (GAMMA_AIR - 1);
+  pr[5] = 0.5 * (pr[1] * pr[1] + pr[2] * pr[2] + pr[3] * pr[3]) * pr[0] + pr[4] / (GAMMA_AIR - 1);
+  riemann(&zz, pl, pr, met6, jac2, 0, f5);
+}
+
+/* WENO5 of scalar data v[0..5] at points i-2..i+3: v-(i+1/2), v+(i+1/2) */
+void orc_weno5_scalar(const double *v6, double *vm, double *vp) {
+  double L[25] = {0};
+  double cvb[6 * 7] = {0};
+  L[0] = 1.0;
+  for (int s = 0; s < 6; ++s) cvb[s * 7] = v6[s];
+  weno5(L, cvb, 2, 0, vm, vp);
+}
+
+/* AWENO characteristic matrices at the face between two states
+ * c7 = (rho, rho u, rho v, rho w, rho E, p, T) and the face metric rows */
+void orc_aweno_matrices(const double *cl7, const double *cr7, const double *met6, double *L25, double *R25) {
+  double cvb[6 * 7];
+  for (int s = 0; s < 6; ++s)
+    for (int c = 0; c < 7; ++c) cvb[s * 7 + c] = s < 3 ? cl7[c] : cr7[c];
+  double met[18];
+  for (int s = 0; s < 6; ++s)
+    for (int c = 0; c < 3; ++c) met[s * 3 + c] = s < 3 ? met6[c] : met6[3 + c];
+  double pl[8], pr[8];
+  aweno_air_m(cvb, met, 2, pl, pr, L25, R25);
+}
+
+/* the SSP-RK3 coefficients used by the stage update */
+void orc_ssprk3_coeffs(double *a, double *b, double *c) {
+  for (int r = 0; r < 3; ++r) {
+    a[r] = RK_A[r];
+    b[r] = RK_B[r];
+    c[r] = RK_C[r];
   }
 }

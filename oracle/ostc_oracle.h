@@ -131,6 +131,13 @@ void orc_zone_state(orc_zone *zz, double *t_phys, int *step);
 void orc_grid_metrics(int mx, int my, int mz, int ngg, const double *x, const double *y, const double *z,
                       double *xg, double *yg, double *zg, double *jac, double *metric);
 
+/* point-level test hooks (see orc_flow.c) */
+void orc_face_flux_air(int scheme, const double *pl5, const double *pr5, const double *met6, const double *jac2,
+                       double *f5);
+void orc_weno5_scalar(const double *v6, double *vm, double *vp);
+void orc_aweno_matrices(const double *cl7, const double *cr7, const double *met6, double *L25, double *R25);
+void orc_ssprk3_coeffs(double *a, double *b, double *c);
+
 #ifdef __cplusplus
 }
 #endif
