@@ -964,7 +964,8 @@ int ostc_chem_batch(ostc_solver *s, int64_t n, const double *rho, const double *
     HIPCHK(launch_chem_batch(s->d_mech, n, rho, T, Y, dt, dY, dT, s->stream));
     if (n_fallback) *n_fallback = 0;
   } else if (mode == 1) {
-    if (!s->sur || !surrogate_ready(s->sur)) return fail("no trained surrogate loaded");
+    if (!s->sur || !surrogate_ready(s->sur)) return fail("no trained surrogate: train or load one first");
+    ostc_solver::Timer t(s, "chem_batch_surrogate");
     int64_t nf = 0;
     std::string err;
     if (surrogate_infer(s->sur, s->d_mech, n, rho, T, Y, dt, dY, dT, &nf, s->stream, err)) return fail(err);
@@ -975,6 +976,137 @@ int ostc_chem_batch(ostc_solver *s, int64_t n, const double *rho, const double *
   HIPCHK(hipStreamSynchronize(s->stream));
   s->drain_profile();
   return 0;
+}
+
+static int need_sur(ostc_solver *s) {
+  if (!s) return fail("null solver");
+  if (s->n_spec != 9 || s->mech.n_reac <= 0) return fail("the surrogate needs the 9-species reacting mechanism");
+  if (s->device < 0) return fail("the surrogate needs a device");
+  if (!s->sur) {
+    std::string err;
+    if (surrogate_create(s->mech, s->device, &s->sur, err)) return fail(err);
+  }
+  return 0;
+}
+
+#define SURCALL(expr)                  \
+  do {                                 \
+    std::string err;                   \
+    if (expr) return fail(err);        \
+  } while (0)
+
+int ostc_surrogate_init(ostc_solver *s, uint64_t seed) {
+  if (need_sur(s)) return 1;
+  SURCALL(surrogate_init_params(s->sur, seed, s->stream, err));
+  return 0;
+}
+
+int ostc_surrogate_param_count(ostc_solver *s, int64_t *n) {
+  if (need_sur(s)) return 1;
+  if (!n) return fail("null argument");
+  *n = surrogate_param_count(s->sur);
+  return 0;
+}
+
+int ostc_surrogate_get_params(ostc_solver *s, float *host) {
+  if (need_sur(s)) return 1;
+  if (!host) return fail("null argument");
+  SURCALL(surrogate_get_params(s->sur, host, s->stream, err));
+  return 0;
+}
+
+int ostc_surrogate_set_params(ostc_solver *s, const float *host) {
+  if (need_sur(s)) return 1;
+  if (!host) return fail("null argument");
+  SURCALL(surrogate_set_params(s->sur, host, s->stream, err));
+  return 0;
+}
+
+int ostc_surrogate_forward(ostc_solver *s, int64_t B, const float *x, float *y) {
+  if (need_sur(s)) return 1;
+  if (!x || !y) return fail("null argument");
+  SURCALL(surrogate_forward(s->sur, B, x, y, s->stream, err));
+  return 0;
+}
+
+int ostc_surrogate_gradient(ostc_solver *s, int64_t B, const float *x, const float *t, float *grad, double *loss) {
+  if (need_sur(s)) return 1;
+  if (!x || !t || !grad) return fail("null argument");
+  SURCALL(surrogate_gradient(s->sur, B, x, t, grad, loss, s->stream, err));
+  return 0;
+}
+
+int ostc_surrogate_train(ostc_solver *s, int64_t n, const double *rho, const double *T, const double *Y, double dt,
+                         const ostc_surrogate_train_cfg *cfg, double *loss_hist) {
+  if (need_sur(s)) return 1;
+  SurrogateTrainCfg c;
+  if (cfg) {
+    c.epochs = cfg->epochs;
+    c.batch = cfg->batch;
+    c.lr = cfg->lr;
+    c.lr_min = cfg->lr_min;
+    c.beta1 = cfg->beta1;
+    c.beta2 = cfg->beta2;
+    c.eps = cfg->eps;
+    c.seed = cfg->seed;
+  }
+  if (c.epochs < 1) return fail("epochs must be >= 1");
+  ostc_solver::Timer t(s, "surrogate_train");
+  SURCALL(surrogate_train(s->sur, s->d_mech, n, rho, T, Y, dt, c, loss_hist, s->stream, err));
+  return 0;
+}
+
+int ostc_surrogate_set_gate(ostc_solver *s, const ostc_surrogate_gate *g) {
+  if (need_sur(s)) return 1;
+  if (!g) return fail("null argument");
+  if (!(g->cons_tol >= 0) || !(g->dy_max >= 0) || !(g->dT_max >= 0) || !(g->margin >= 0) || !(g->src_max >= 0))
+    return fail("gate thresholds must be non-negative");
+  SurrogateGate q;
+  q.cons_tol = g->cons_tol;
+  q.dy_max = g->dy_max;
+  q.dT_max = g->dT_max;
+  q.margin = g->margin;
+  q.src_max = g->src_max;
+  surrogate_set_gate(s->sur, q);
+  return 0;
+}
+
+int ostc_surrogate_get_gate(ostc_solver *s, ostc_surrogate_gate *g) {
+  if (need_sur(s)) return 1;
+  if (!g) return fail("null argument");
+  const SurrogateGate q = surrogate_get_gate(s->sur);
+  g->cons_tol = q.cons_tol;
+  g->dy_max = q.dy_max;
+  g->dT_max = q.dT_max;
+  g->margin = q.margin;
+  g->src_max = q.src_max;
+  return 0;
+}
+
+int ostc_surrogate_norm(ostc_solver *s, double *out) {
+  if (need_sur(s)) return 1;
+  if (!out) return fail("null argument");
+  return surrogate_get_norm(s->sur, out);
+}
+
+int ostc_surrogate_save(ostc_solver *s, const char *path) {
+  if (need_sur(s)) return 1;
+  if (!path) return fail("null path");
+  SURCALL(surrogate_save(s->sur, path, s->stream, err));
+  return 0;
+}
+
+int ostc_surrogate_load(ostc_solver *s, const char *path) {
+  if (need_sur(s)) return 1;
+  if (!path) return fail("null path");
+  SURCALL(surrogate_load(s->sur, path, s->stream, err));
+  return 0;
+}
+
+int ostc_surrogate_last_stats(ostc_solver *s, double *out) {
+  if (need_sur(s)) return 1;
+  if (!out) return fail("null argument");
+  return surrogate_last_stats(s->sur, out);
 }
 
 int ostc_dev_alloc(ostc_solver *s, int64_t bytes, void **dptr) {

@@ -20,8 +20,6 @@
 
 namespace ostc {
 
-namespace {
-
 // Standard atomic weights as tabulated by the reference (Element.cpp:13-132);
 // only the symbols a combustion mechanism may name are listed.
 double atomic_weight(const std::string &e) {
@@ -32,6 +30,8 @@ double atomic_weight(const std::string &e) {
   auto it = w.find(e);
   return it == w.end() ? -1.0 : it->second;
 }
+
+namespace {
 
 void upper(std::string &s) {
   for (auto &c : s) c = (char)std::toupper((unsigned char)c);

@@ -63,6 +63,9 @@ struct Mechanism {
   std::vector<double> troe_alpha, troe_t3, troe_t1, troe_t2;
 };
 
+// Atomic weight of an element symbol (upper case), -1 if unknown.
+double atomic_weight(const std::string &e);
+
 // ChemData.cpp:11-685 semantics.  Paths are relative to <case>/input_files/.
 // Returns an empty string on success or an error message.
 std::string read_mechanism(const std::string &input_dir, const std::string &mech_file, const std::string &therm_file,

@@ -1,23 +1,1070 @@
-// MLP chemistry surrogate.  Until a network is trained or loaded the
-// surrogate reports itself as not ready and ostc_chem_batch(mode 1) fails.
+// MLP chemistry surrogate: training-set generation with the FP64
+// point-implicit integrator, BF16-MFMA training (forward, backward, Adam),
+// gated inference and the per-cell FP64 fallback.
 #include "surrogate.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <memory>
+#include <numeric>
+#include <vector>
+
+#include "device_chem.h"
+#include "flux_kernels.h"
+#include "mlp.h"
 
 namespace ostc {
 
-struct SurrogateState {
-  bool ready = false;
+namespace {
+
+constexpr int kDin[kSurLayers] = {11, 256, 256, 256};
+constexpr int kDout[kSurLayers] = {256, 256, 256, kSurOut};
+constexpr int kKpad[kSurLayers] = {16, 256, 256, 256};
+constexpr int kLd[kSurLayers] = {24, 264, 264, 264};
+constexpr int kOutPad[kSurLayers] = {256, 256, 256, 16};
+constexpr int kXld = 24, kHld = 264, kYld = 16, kRawLd = 16;
+constexpr int kNS = 9;
+constexpr double kYFloor = 1e-12;
+constexpr int64_t kInferChunk = 1 << 20;
+
+// Standardisation, conservation rows and gate thresholds (kernel argument).
+struct SurNorm {
+  double x_mean[kSurIn], x_istd[kSurIn], x_lo[kSurIn], x_hi[kSurIn];
+  double c_mean[kSurOut], c_std[kSurOut];  // of u = asinh(c / c_scale)
+  double c_scale[kSurOut];
+  double basis[kNS * kSurNull];  // dY_k = sum_j basis[k * 6 + j] c_j
+  double cons[4 * kNS];          // element mass-fraction rows, then the mass row
+  int n_cons;
+  double cons_tol, dy_max, dT_max, src_max, dt;
 };
+
+// (log(Y + 1e-12), T, log p) of one cell; p from the equation of state as
+// in the point-implicit batch kernel
+__device__ inline void raw_features(const DMech *__restrict__ m, double rho, double T, const double *Y, double *x) {
+  double mwi = 0;
+#pragma unroll
+  for (int l = 0; l < kNS; ++l) mwi += Y[l] / m->mw[l];
+  const double mw = 1 / mwi;
+  const double p = rho * T * (kRuD / mw);
+#pragma unroll
+  for (int l = 0; l < kNS; ++l) x[l] = log(fmax(Y[l], 0.0) + kYFloor);
+  x[9] = T;
+  x[10] = log(p);
+}
+
+__device__ inline void load_cell(int64_t n, int64_t c, const double *__restrict__ Y_a, double *Y) {
+#pragma unroll
+  for (int l = 0; l < kNS; ++l) Y[l] = Y_a[(int64_t)l * n + c];
+}
+
+// FP32 explicit chemical increment of a cell, max_k |dt omega_k / rho|: the
+// stiffness indicator of the gate.  The finite-rate formulas of the FP64
+// integrator (FiniteRateChem.cu:65-194) evaluated in log form so that no
+// FP32 intermediate overflows: ln kf = ln A + b ln T - Ea / T, third body,
+// Lindemann / Troe fall-off, ln kb = ln kf - ln Kc, ln q = ln k + sum nu ln c.
+__device__ inline float explicit_increment(const DMech *__restrict__ m, double rho_d, double T_d, const double *Y,
+                                           double dt) {
+  const float t = (float)T_d, rho = (float)rho_d;
+  const float lt = __logf(t), ti = 1.f / t;
+  float lc[kNS], c[kNS], g[kNS];
+#pragma unroll
+  for (int l = 0; l < kNS; ++l) {
+    c[l] = (float)(rho_d * Y[l] / m->mw[l] * 1e-3);
+    lc[l] = __logf(c[l]);  // -inf for an absent species
+    const float tt = fmaxf(t, (float)m->t_low[l]);
+    const double *a = tt < (float)m->t_mid[l] ? &m->lo[l * 7] : &m->hi[l * 7];
+    const float tt2 = tt * tt;
+    g[l] = (float)a[0] * (1.f - __logf(tt)) - 0.5f * (float)a[1] * tt - (float)a[2] * tt2 * (1.f / 6.f) -
+           (float)a[3] * tt2 * tt * (1.f / 12.f) - (float)a[4] * tt2 * tt2 * 0.05f + (float)a[5] / tt - (float)a[6];
+  }
+  const float ltemp = __logf((float)(101325.0 / kRuD * 1e-3)) - lt;
+  float w[kNS];
+#pragma unroll
+  for (int l = 0; l < kNS; ++l) w[l] = 0.f;
+  for (int r = 0; r < m->n_reac; ++r) {
+    const int type = m->label[r];
+    float lk = __logf((float)m->A[r]) + (float)m->b[r] * lt - (float)m->Ea[r] * ti;
+    if (type == 3) {
+      const float lk2 = __logf((float)m->A2[r]) + (float)m->b2[r] * lt - (float)m->Ea2[r] * ti;
+      const float hi = fmaxf(lk, lk2);
+      lk = hi + __logf(1.f + __expf(fminf(lk, lk2) - hi));
+    }
+    if (type > 3) {
+      float cc = 0.f;
+#pragma unroll
+      for (int l = 0; l < kNS; ++l) cc += c[l] * (float)m->third_body_coeff[r * kNS + l];
+      if (type == 4) {
+        lk += __logf(cc);
+      } else {
+        const float llow = __logf((float)m->A2[r]) + (float)m->b2[r] * lt - (float)m->Ea2[r] * ti;
+        const float lpr_e = llow + __logf(cc) - lk;  // ln Pr
+        float lF = 0.f;
+        if (type > 5) {
+          const float al = (float)m->troe_alpha[r];
+          float fc = (1.f - al) * __expf(-t / (float)m->troe_t3[r]) + al * __expf(-t / (float)m->troe_t1[r]);
+          if (type == 7) fc += __expf(-(float)m->troe_t2[r] * ti);
+          const float lfc = __log10f(fc);
+          const float c2 = -0.4f - 0.67f * lfc, nn = 0.75f - 1.27f * lfc;
+          const float lpr = lpr_e * 0.43429448f;
+          const float tp = (lpr + c2) / (nn - 0.14f * (lpr + c2));
+          lF = lfc / (1.f + tp * tp) * 2.30258509f;
+        }
+        // k = khigh * Pr / (1 + Pr) * F
+        lk = lk + lpr_e - (lpr_e > 0.f ? lpr_e + __logf(1.f + __expf(-lpr_e)) : __logf(1.f + __expf(lpr_e))) + lF;
+      }
+    }
+    float l1 = lk, l2;
+    float dg = 0.f, lq2 = 0.f;
+#pragma unroll
+    for (int l = 0; l < kNS; ++l) {
+      const int nf = m->stoi_f[r * kNS + l], nb = m->stoi_b[r * kNS + l];
+      if (nf) l1 += (float)nf * lc[l];
+      if (nb) lq2 += (float)nb * lc[l];
+      dg += g[l] * (float)(nb - nf);
+    }
+    float q = __expf(l1);
+    if (type == 2) {
+      q -= __expf(__logf((float)m->A2[r]) + (float)m->b2[r] * lt - (float)m->Ea2[r] * ti + lq2);
+    } else if (type != 0) {
+      l2 = lk - (float)m->order[r] * ltemp + dg + lq2;
+      q -= __expf(l2);
+    }
+#pragma unroll
+    for (int l = 0; l < kNS; ++l) w[l] += (float)(m->stoi_b[r * kNS + l] - m->stoi_f[r * kNS + l]) * q;
+  }
+  float mx = 0.f;
+#pragma unroll
+  for (int l = 0; l < kNS; ++l) mx = fmaxf(mx, fabsf(w[l] * (float)(m->mw[l] * 1e3)));
+  return (float)dt * mx / rho;
+}
+
+// raw features F [n x 16] and raw targets Tt [n x 16]: null-space coordinates of dY and dT
+__global__ void k_sur_raw(const DMech *__restrict__ m, int64_t n, const double *__restrict__ rho,
+                          const double *__restrict__ T, const double *__restrict__ Y_a, const double *__restrict__ dY,
+                          const double *__restrict__ dT, SurNorm nm, float *__restrict__ F, float *__restrict__ Tt) {
+  const int64_t c = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (c >= n) return;
+  double Y[kNS], x[kSurIn], d[kNS];
+  load_cell(n, c, Y_a, Y);
+  raw_features(m, rho[c], T[c], Y, x);
+#pragma unroll
+  for (int i = 0; i < kRawLd; ++i) F[c * kRawLd + i] = i < kSurIn ? (float)x[i] : 0.f;
+  load_cell(n, c, dY, d);
+#pragma unroll
+  for (int j = 0; j < kSurNull; ++j) {
+    double s = 0;
+#pragma unroll
+    for (int k = 0; k < kNS; ++k) s += nm.basis[k * kSurNull + j] * d[k];
+    Tt[c * kRawLd + j] = (float)s;
+  }
+  Tt[c * kRawLd + 6] = (float)dT[c];
+#pragma unroll
+  for (int i = kSurOut; i < kRawLd; ++i) Tt[c * kRawLd + i] = 0.f;
+}
+
+// per-block column sums, sums of squares, minima and maxima of A [n x 16]
+__global__ void __launch_bounds__(256) k_colstats(const float *__restrict__ A, int64_t n, double *__restrict__ part) {
+  const int col = threadIdx.x & 15, rl = threadIdx.x >> 4;
+  double s = 0, q = 0, lo = 1e300, hi = -1e300;
+  for (int64_t r = (int64_t)blockIdx.x * 16 + rl; r < n; r += (int64_t)gridDim.x * 16) {
+    const double v = A[r * kRawLd + col];
+    s += v;
+    q += v * v;
+    lo = fmin(lo, v);
+    hi = fmax(hi, v);
+  }
+  __shared__ double sh[4][16][16];
+  sh[0][rl][col] = s;
+  sh[1][rl][col] = q;
+  sh[2][rl][col] = lo;
+  sh[3][rl][col] = hi;
+  __syncthreads();
+  if (rl == 0) {
+    for (int r = 1; r < 16; ++r) {
+      s += sh[0][r][col];
+      q += sh[1][r][col];
+      lo = fmin(lo, sh[2][r][col]);
+      hi = fmax(hi, sh[3][r][col]);
+    }
+    double *o = part + ((int64_t)blockIdx.x * 16 + col) * 4;
+    o[0] = s;
+    o[1] = q;
+    o[2] = lo;
+    o[3] = hi;
+  }
+}
+
+// u = asinh(c / scale) of the raw targets, in place
+__global__ void k_sur_transform(int64_t n, float *__restrict__ Tt, SurNorm nm) {
+  const int64_t c = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (c >= n) return;
+#pragma unroll
+  for (int j = 0; j < kSurOut; ++j) Tt[c * kRawLd + j] = (float)asinh((double)Tt[c * kRawLd + j] / nm.c_scale[j]);
+}
+
+__device__ inline void write_x_row(__bf16 *__restrict__ X, int64_t r, const double *xs) {
+  __bf16 *o = X + r * kXld;
+#pragma unroll
+  for (int i = 0; i < kXld; ++i) o[i] = (__bf16)(i < kSurIn ? (float)xs[i] : (i == 16 ? 1.f : 0.f));
+}
+
+// standardise: F -> X (BF16, augmented), Tt in place
+__global__ void k_sur_standardize(int64_t n, const float *__restrict__ F, float *__restrict__ Tt, SurNorm nm,
+                                  __bf16 *__restrict__ X) {
+  const int64_t c = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (c >= n) return;
+  double xs[kSurIn];
+#pragma unroll
+  for (int i = 0; i < kSurIn; ++i) xs[i] = ((double)F[c * kRawLd + i] - nm.x_mean[i]) * nm.x_istd[i];
+  write_x_row(X, c, xs);
+#pragma unroll
+  for (int j = 0; j < kSurOut; ++j) Tt[c * kRawLd + j] = (float)(((double)Tt[c * kRawLd + j] - nm.c_mean[j]) / nm.c_std[j]);
+}
+
+// host-supplied standardised inputs [B x 11] -> X (augmented BF16)
+__global__ void k_pack_x(int64_t B, const float *__restrict__ x, __bf16 *__restrict__ X) {
+  const int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (r >= B) return;
+  double xs[kSurIn];
+#pragma unroll
+  for (int i = 0; i < kSurIn; ++i) xs[i] = x[r * kSurIn + i];
+  write_x_row(X, r, xs);
+}
+
+// standardised network inputs of cells c0 .. c0+count
+__global__ void k_sur_features(const DMech *__restrict__ m, int64_t n, int64_t c0, int64_t count,
+                               const double *__restrict__ rho, const double *__restrict__ T,
+                               const double *__restrict__ Y_a, SurNorm nm, __bf16 *__restrict__ X) {
+  const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (t >= count) return;
+  const int64_t c = c0 + t;
+  double Y[kNS], x[kSurIn];
+  load_cell(n, c, Y_a, Y);
+  raw_features(m, rho[c], T[c], Y, x);
+#pragma unroll
+  for (int i = 0; i < kSurIn; ++i) x[i] = (x[i] - nm.x_mean[i]) * nm.x_istd[i];
+  write_x_row(X, t, x);
+}
+
+// Decode the network output of cells c0 .. c0+count, apply the gate and
+// either write (dY, dT) or append the cell to the fallback list.
+//   gate: inputs inside the training range, finite outputs, |dY| and |dT|
+//   within the accepted activity, FP32 explicit increment dt |omega| / rho
+//   within src_max (stiff cells, where the point-implicit step itself is
+//   far from linear, stay with the integrator), element / mass conservation error of the
+//   written state (conservation rows of dY plus the mass that mass fractions
+//   outside [0, 1] would need clipped) <= cons_tol, converged temperature.
+// The outputs are u_j = asinh(c_j / scale_j) standardised; c_j are the
+// null-space coordinates of dY (j < 6) and dT (j = 6).
+// T is the Newton temperature at the unchanged internal energy started
+// from T + dT_net, as in the point-implicit batch kernel.
+__global__ void __launch_bounds__(256) k_sur_finish(const DMech *__restrict__ m, SurNorm nm, int64_t n, int64_t c0,
+                                                    int64_t count, const double *__restrict__ rho_a,
+                                                    const double *__restrict__ T_a, const double *__restrict__ Y_a,
+                                                    const float *__restrict__ out, int force_fallback,
+                                                    double *__restrict__ dY, double *__restrict__ dT,
+                                                    int *__restrict__ fb_count, int *__restrict__ fb_idx) {
+  const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  const bool active = t < count;
+  const int64_t c = c0 + t;
+  bool ok = active && !force_fallback;
+  if (ok) {
+    const double rho = rho_a[c], T = T_a[c];
+    double Y[kNS], x[kSurIn];
+    load_cell(n, c, Y_a, Y);
+    raw_features(m, rho, T, Y, x);
+#pragma unroll
+    for (int i = 0; i < kSurIn; ++i) ok = ok && x[i] >= nm.x_lo[i] && x[i] <= nm.x_hi[i];
+    double cj[kSurOut];
+#pragma unroll
+    for (int j = 0; j < kSurOut; ++j) cj[j] = nm.c_scale[j] * sinh(nm.c_mean[j] + nm.c_std[j] * (double)out[t * kYld + j]);
+    double dy[kNS];
+    double amax = 0;
+#pragma unroll
+    for (int k = 0; k < kNS; ++k) {
+      double s = 0;
+#pragma unroll
+      for (int j = 0; j < kSurNull; ++j) s += nm.basis[k * kSurNull + j] * cj[j];
+      dy[k] = s;
+      amax = fmax(amax, fabs(s));
+    }
+    const double dTn = cj[6];
+    ok = ok && isfinite(amax) && isfinite(dTn) && amax <= nm.dy_max && fabs(dTn) <= nm.dT_max;
+    if (ok) {
+      const float xi = explicit_increment(m, rho, T, Y, nm.dt);
+      ok = xi <= (float)nm.src_max;  // false for NaN as well
+    }
+    double err = 0, neg = 0;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      double s = 0;
+#pragma unroll
+      for (int k = 0; k < kNS; ++k) s += nm.cons[r * kNS + k] * dy[k];
+      if (r < nm.n_cons) err = fmax(err, fabs(s));
+    }
+#pragma unroll
+    for (int k = 0; k < kNS; ++k) neg += fmax(0.0, -(Y[k] + dy[k])) + fmax(0.0, Y[k] + dy[k] - 1.0);
+    ok = ok && fmax(err, neg) <= nm.cons_tol;
+    if (ok) {
+      double mwi = 0, h[kNS];
+#pragma unroll
+      for (int l = 0; l < kNS; ++l) mwi += Y[l] / m->mw[l];
+      const double mw = 1 / mwi;
+      const double p_old = rho * T * (kRuD / mw);
+      d_enthalpy<kNS>(T, m, h);
+      double rhoE = 0;
+#pragma unroll
+      for (int l = 0; l < kNS; ++l) rhoE += h[l] * Y[l];
+      rhoE = rhoE * rho - p_old;
+      double Yn[kNS];
+#pragma unroll
+      for (int l = 0; l < kNS; ++l) Yn[l] = Y[l] + dy[l];
+      const double t0 = fmin(fmax(T + dTn, 0.5 * T), 2.0 * T + 1000.0);
+      double pn;
+      const double Tn = d_temperature<kNS>(m, rho, rhoE, 0.0, 0.0, 0.0, Yn, t0, pn);
+      ok = isfinite(Tn) && Tn > 0;
+      if (ok) {
+#pragma unroll
+        for (int l = 0; l < kNS; ++l) dY[(int64_t)l * n + c] = dy[l];
+        dT[c] = Tn - T;
+      }
+    }
+  }
+  // wave-aggregated append of the fallback cells
+  const bool fb = active && !ok;
+  const unsigned long long mask = __ballot(fb);
+  if (mask) {
+    const int lane = threadIdx.x & 63;
+    const int leader = __ffsll((long long)mask) - 1;
+    int base = 0;
+    if (lane == leader) base = atomicAdd(fb_count, (int)__popcll(mask));
+    base = __shfl(base, leader, 64);
+    if (fb) fb_idx[base + (int)__popcll(mask & ((1ull << lane) - 1ull))] = (int)c;
+  }
+}
+
+inline unsigned nb256(int64_t n) { return (unsigned)((n + 255) / 256); }
+
+uint64_t splitmix(uint64_t &s) {
+  uint64_t z = (s += 0x9E3779B97F4A7C15ull);
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+double unit(uint64_t &s) { return (double)(splitmix(s) >> 11) * (1.0 / 9007199254740992.0); }
+
+}  // namespace
+
+struct SurrogateState {
+  int device = 0;
+  SurNorm norm{};
+  SurrogateGate gate{};
+  double x_min[kSurIn]{}, x_max[kSurIn]{};
+  bool ready = false;
+  double dt_train = 0;
+  int64_t woff[kSurLayers]{}, nw = 0;
+  float *w = nullptr, *g = nullptr, *m1 = nullptr, *m2 = nullptr;
+  __bf16 *wb = nullptr;
+  float *aug = nullptr;  // [1, 0 x 7]: the augmented columns of an activation row
+  int adam_t = 0;
+  // training workspace for batches up to tcap rows
+  int64_t tcap = 0;
+  __bf16 *tX = nullptr, *tH[3] = {};
+  float *tT = nullptr, *tZ[3] = {}, *tY = nullptr, *tG = nullptr, *tD[2] = {}, *loss = nullptr;
+  // inference workspace for chunks up to icap rows, fallback list up to fcap cells
+  int64_t icap = 0, fcap = 0;
+  __bf16 *iX = nullptr, *iH[2] = {};
+  float *iOut = nullptr;
+  int *fb_idx = nullptr, *fb_count = nullptr;
+  hipEvent_t ev[3] = {};
+  double last_ms[2] = {0, 0};
+  int64_t last_fallback = 0;
+
+  void free_train() {
+    for (void *p : {(void *)tX, (void *)tH[0], (void *)tH[1], (void *)tH[2], (void *)tT, (void *)tZ[0],
+                    (void *)tZ[1], (void *)tZ[2], (void *)tY, (void *)tG, (void *)tD[0], (void *)tD[1]})
+      (void)hipFree(p);
+    tX = nullptr;
+    tH[0] = tH[1] = tH[2] = nullptr;
+    tT = tY = tG = nullptr;
+    tZ[0] = tZ[1] = tZ[2] = nullptr;
+    tD[0] = tD[1] = nullptr;
+    tcap = 0;
+  }
+  void free_infer() {
+    for (void *p : {(void *)iX, (void *)iH[0], (void *)iH[1], (void *)iOut}) (void)hipFree(p);
+    iX = nullptr;
+    iH[0] = iH[1] = nullptr;
+    iOut = nullptr;
+    icap = 0;
+  }
+  ~SurrogateState() {
+    (void)hipSetDevice(device);
+    free_train();
+    free_infer();
+    for (void *p : {(void *)w, (void *)g, (void *)m1, (void *)m2, (void *)wb, (void *)aug, (void *)loss,
+                    (void *)fb_idx, (void *)fb_count})
+      (void)hipFree(p);
+    for (auto &e : ev)
+      if (e) (void)hipEventDestroy(e);
+  }
+};
+
+namespace {
+
+#define SCHK(expr)                                                                  \
+  do {                                                                              \
+    hipError_t e_ = (expr);                                                         \
+    if (e_ != hipSuccess) {                                                         \
+      err = std::string(#expr) + ": " + hipGetErrorString(e_);                      \
+      return 1;                                                                     \
+    }                                                                               \
+  } while (0)
+
+template <typename T>
+hipError_t dalloc(T **p, int64_t count) {
+  return hipMalloc((void **)p, (size_t)std::max<int64_t>(count, 1) * sizeof(T));
+}
+
+int ensure_train_ws(SurrogateState *s, int64_t B, hipStream_t st, std::string &err) {
+  if (B <= s->tcap) return 0;
+  s->free_train();
+  SCHK(dalloc(&s->tX, B * kXld));
+  for (int l = 0; l < 3; ++l) {
+    SCHK(dalloc(&s->tH[l], B * kHld));
+    SCHK(launch_fill_cols_bf16(s->tH[l], B, kHld, 256, s->aug, st));
+    SCHK(dalloc(&s->tZ[l], B * kHld));
+  }
+  SCHK(dalloc(&s->tT, B * kYld));
+  SCHK(dalloc(&s->tY, B * kYld));
+  SCHK(hipMemsetAsync(s->tY, 0, (size_t)B * kYld * sizeof(float), st));
+  SCHK(dalloc(&s->tG, B * kYld));
+  SCHK(dalloc(&s->tD[0], B * kHld));
+  SCHK(dalloc(&s->tD[1], B * kHld));
+  s->tcap = B;
+  return 0;
+}
+
+int ensure_infer_ws(SurrogateState *s, int64_t chunk, int64_t n, hipStream_t st, std::string &err) {
+  if (chunk > s->icap) {
+    s->free_infer();
+    SCHK(dalloc(&s->iX, chunk * kXld));
+    for (int l = 0; l < 2; ++l) {
+      SCHK(dalloc(&s->iH[l], chunk * kHld));
+      SCHK(launch_fill_cols_bf16(s->iH[l], chunk, kHld, 256, s->aug, st));
+    }
+    SCHK(dalloc(&s->iOut, chunk * kYld));
+    s->icap = chunk;
+  }
+  if (n > s->fcap) {
+    (void)hipFree(s->fb_idx);
+    s->fb_idx = nullptr;
+    SCHK(dalloc(&s->fb_idx, n));
+    s->fcap = n;
+  }
+  return 0;
+}
+
+int split_for(int M, int N, int K) {
+  const int tiles = ((M + 63) / 64) * ((N + 63) / 64);
+  int sp = std::max(1, 512 / tiles);
+  return std::max(1, std::min(sp, K / 256));
+}
+
+// forward pass; Z (saved pre-activations) may be null (inference)
+int forward(SurrogateState *s, int64_t M, const __bf16 *X, __bf16 *const *H, float *const *Z, float *Y, bool bf16w,
+            hipStream_t st, std::string &err) {
+  const void *A = X;
+  for (int l = 0; l < kSurLayers; ++l) {
+    GemmArgs g;
+    g.M = (int)M;
+    g.N = kDout[l];
+    g.K = kLd[l];
+    g.lda = kLd[l];
+    g.ldb = kLd[l];
+    g.transB = 1;
+    const void *Bw = bf16w ? (const void *)(s->wb + s->woff[l]) : (const void *)(s->w + s->woff[l]);
+    if (l < kSurLayers - 1) {
+      g.mode = EPI_GELU;
+      g.Z = Z ? Z[l] : nullptr;
+      g.Cb = H[l];
+      g.ldc = kHld;
+    } else {
+      g.mode = EPI_STORE;
+      g.Cf = Y;
+      g.ldc = kYld;
+    }
+    SCHK(launch_gemm(g, A, true, Bw, bf16w, 1, st));
+    A = H[l];
+  }
+  return 0;
+}
+
+// loss + backward into s->g (zeroed here); forward must have filled tZ, tH, tY
+int backward(SurrogateState *s, int64_t B, const __bf16 *X, const float *T, hipStream_t st, std::string &err) {
+  SCHK(hipMemsetAsync(s->g, 0, (size_t)s->nw * sizeof(float), st));
+  SCHK(launch_mse(s->tY, T, (int)B, kSurOut, kYld, s->tG, s->loss, st));
+  // dZ of the current layer: G for the output layer, then tD ping-pong
+  const float *dZ = s->tG;
+  int ldz = kYld;
+  for (int l = kSurLayers - 1; l >= 0; --l) {
+    // weight (and bias) gradient: dW_l = dZ^T [h, 1]
+    GemmArgs gw;
+    gw.M = kDout[l];
+    gw.N = kKpad[l] + 1;
+    gw.K = (int)B;
+    gw.transA = 1;
+    gw.lda = ldz;
+    gw.ldb = l == 0 ? kXld : kHld;
+    gw.mode = EPI_ATOMIC;
+    gw.Cf = s->g + s->woff[l];
+    gw.ldc = kLd[l];
+    const void *Hin = l == 0 ? (const void *)X : (const void *)s->tH[l - 1];
+    SCHK(launch_gemm(gw, dZ, false, Hin, true, split_for(gw.M, gw.N, gw.K), st));
+    if (l == 0) break;
+    // dZ_{l-1} = (dZ_l W_l) * GELU'(Z_{l-1})
+    GemmArgs gd;
+    gd.M = (int)B;
+    gd.N = kDin[l];
+    gd.K = kOutPad[l];
+    gd.lda = ldz;
+    gd.ldb = kLd[l];
+    gd.mode = EPI_GELU_GRAD;
+    gd.Zin = s->tZ[l - 1];
+    float *dst = s->tD[(l + 1) & 1];
+    gd.Cf = dst;
+    gd.ldc = kHld;
+    SCHK(launch_gemm(gd, dZ, false, s->w + s->woff[l], false, 1, st));
+    dZ = dst;
+    ldz = kHld;
+  }
+  return 0;
+}
+
+void finish_norm(SurrogateState *s) {
+  for (int i = 0; i < kSurIn; ++i) {
+    const double r = s->x_max[i] - s->x_min[i];
+    s->norm.x_lo[i] = s->x_min[i] - s->gate.margin * r - 1e-6;
+    s->norm.x_hi[i] = s->x_max[i] + s->gate.margin * r + 1e-6;
+  }
+  s->norm.cons_tol = s->gate.cons_tol;
+  s->norm.dy_max = s->gate.dy_max;
+  s->norm.dT_max = s->gate.dT_max;
+  s->norm.src_max = s->gate.src_max;
+  s->norm.dt = s->dt_train;
+}
+
+}  // namespace
+
+int surrogate_create(const Mechanism &mech, int device, SurrogateState **out, std::string &err) {
+  *out = nullptr;
+  if (mech.n_spec != kNS) {
+    err = "the surrogate is built for 9-species mechanisms";
+    return 1;
+  }
+  auto s = std::make_unique<SurrogateState>();
+  s->device = device;
+  SCHK(hipSetDevice(device));
+  // conservation rows: mass fraction of each element per unit dY_k, then mass
+  const int ne = (int)mech.elements.size();
+  std::vector<std::vector<double>> rows;
+  for (int j = 0; j < ne; ++j) {
+    const double aw = atomic_weight(mech.elements[j]);
+    if (aw <= 0) {
+      err = "unknown element " + mech.elements[j];
+      return 1;
+    }
+    std::vector<double> r(kNS);
+    for (int k = 0; k < kNS; ++k) r[k] = mech.elem_comp[(size_t)k * ne + j] * aw / mech.mw[k];
+    rows.push_back(r);
+  }
+  rows.push_back(std::vector<double>(kNS, 1.0));
+  if ((int)rows.size() > 4) {
+    err = "too many elements for the surrogate's conservation gate";
+    return 1;
+  }
+  s->norm.n_cons = (int)rows.size();
+  for (size_t r = 0; r < rows.size(); ++r)
+    for (int k = 0; k < kNS; ++k) s->norm.cons[r * kNS + k] = rows[r][k];
+  // orthonormal basis of the row space (modified Gram-Schmidt), then of its complement
+  std::vector<std::vector<double>> q;
+  auto orth = [&](std::vector<double> v, double tol) {
+    const double n0 = std::sqrt(std::inner_product(v.begin(), v.end(), v.begin(), 0.0));
+    for (int pass = 0; pass < 2; ++pass)
+      for (auto &u : q) {
+        const double d = std::inner_product(u.begin(), u.end(), v.begin(), 0.0);
+        for (int k = 0; k < kNS; ++k) v[k] -= d * u[k];
+      }
+    const double nn = std::sqrt(std::inner_product(v.begin(), v.end(), v.begin(), 0.0));
+    if (nn <= tol * n0) return false;
+    for (auto &e : v) e /= nn;
+    q.push_back(v);
+    return true;
+  };
+  for (auto &r : rows) orth(r, 1e-10);
+  const size_t rank = q.size();
+  int nnull = 0;
+  for (int k = 0; k < kNS && nnull < kSurNull + 1; ++k) {
+    std::vector<double> e(kNS, 0.0);
+    e[k] = 1;
+    if (orth(e, 1e-8)) ++nnull;
+  }
+  if (nnull > kSurNull) {
+    err = "conservation null space larger than the network's 6 outputs";
+    return 1;
+  }
+  for (int j = 0; j < kSurNull; ++j)
+    for (int k = 0; k < kNS; ++k) s->norm.basis[k * kSurNull + j] = j < nnull ? q[rank + j][k] : 0.0;
+  for (int i = 0; i < kSurIn; ++i) {
+    s->norm.x_mean[i] = 0;
+    s->norm.x_istd[i] = 1;
+    s->x_min[i] = -1e300;
+    s->x_max[i] = 1e300;
+  }
+  for (int j = 0; j < kSurOut; ++j) {
+    s->norm.c_mean[j] = 0;
+    s->norm.c_std[j] = 1;
+    s->norm.c_scale[j] = 1;
+  }
+  finish_norm(s.get());
+  // parameters
+  int64_t off = 0;
+  for (int l = 0; l < kSurLayers; ++l) {
+    s->woff[l] = off;
+    off += (int64_t)kOutPad[l] * kLd[l];
+  }
+  s->nw = off;
+  SCHK(dalloc(&s->w, off));
+  SCHK(dalloc(&s->g, off));
+  SCHK(dalloc(&s->m1, off));
+  SCHK(dalloc(&s->m2, off));
+  SCHK(dalloc(&s->wb, off));
+  SCHK(hipMemset(s->w, 0, off * sizeof(float)));
+  SCHK(dalloc(&s->aug, 8));
+  const float aug[8] = {1, 0, 0, 0, 0, 0, 0, 0};
+  SCHK(hipMemcpy(s->aug, aug, sizeof(aug), hipMemcpyHostToDevice));
+  SCHK(dalloc(&s->loss, 1));
+  SCHK(dalloc(&s->fb_count, 1));
+  for (auto &e : s->ev) SCHK(hipEventCreate(&e));
+  *out = s.release();
+  return 0;
+}
 
 void surrogate_destroy(SurrogateState *s) { delete s; }
 
 bool surrogate_ready(const SurrogateState *s) { return s && s->ready; }
 
-int surrogate_infer(SurrogateState *s, const DMech *, int64_t, const double *, const double *, const double *, double,
-                    double *, double *, int64_t *, hipStream_t, std::string &err) {
-  if (!surrogate_ready(s)) {
-    err = "surrogate not ready";
+int64_t surrogate_param_count(const SurrogateState *s) { return s ? s->nw : 0; }
+
+void surrogate_set_gate(SurrogateState *s, const SurrogateGate &g) {
+  s->gate = g;
+  finish_norm(s);
+}
+
+SurrogateGate surrogate_get_gate(const SurrogateState *s) { return s->gate; }
+
+int surrogate_get_norm(const SurrogateState *s, double *o) {
+  for (int i = 0; i < kSurIn; ++i) {
+    o[i] = s->norm.x_mean[i];
+    o[11 + i] = 1.0 / s->norm.x_istd[i];
+    o[22 + i] = s->x_min[i];
+    o[33 + i] = s->x_max[i];
+  }
+  for (int j = 0; j < kSurOut; ++j) {
+    o[44 + j] = s->norm.c_mean[j];
+    o[51 + j] = s->norm.c_std[j];
+  }
+  o[58] = s->dt_train;
+  for (int k = 0; k < kNS * kSurNull; ++k) o[59 + k] = s->norm.basis[k];
+  for (int j = 0; j < kSurOut; ++j) o[113 + j] = s->norm.c_scale[j];
+  return 0;
+}
+
+static int upload_params(SurrogateState *s, const std::vector<float> &h, hipStream_t st, std::string &err) {
+  SCHK(hipMemcpyAsync(s->w, h.data(), (size_t)s->nw * sizeof(float), hipMemcpyHostToDevice, st));
+  SCHK(hipMemsetAsync(s->m1, 0, (size_t)s->nw * sizeof(float), st));
+  SCHK(hipMemsetAsync(s->m2, 0, (size_t)s->nw * sizeof(float), st));
+  SCHK(launch_to_bf16(s->w, s->wb, s->nw, st));
+  SCHK(hipStreamSynchronize(st));
+  s->adam_t = 0;
+  return 0;
+}
+
+int surrogate_init_params(SurrogateState *s, uint64_t seed, hipStream_t st, std::string &err) {
+  SCHK(hipSetDevice(s->device));
+  std::vector<float> h((size_t)s->nw, 0.f);
+  uint64_t rng = seed;
+  for (int l = 0; l < kSurLayers; ++l) {
+    // uniform with variance 2 / fan_in (GELU hidden layers), 1 / fan_in (output layer)
+    const double a = std::sqrt((l < kSurLayers - 1 ? 6.0 : 3.0) / kDin[l]);
+    for (int r = 0; r < kDout[l]; ++r)
+      for (int c = 0; c < kDin[l]; ++c) h[(size_t)(s->woff[l] + (int64_t)r * kLd[l] + c)] = (float)(a * (2 * unit(rng) - 1));
+  }
+  return upload_params(s, h, st, err);
+}
+
+int surrogate_get_params(SurrogateState *s, float *host, hipStream_t st, std::string &err) {
+  SCHK(hipSetDevice(s->device));
+  SCHK(hipMemcpyAsync(host, s->w, (size_t)s->nw * sizeof(float), hipMemcpyDeviceToHost, st));
+  SCHK(hipStreamSynchronize(st));
+  return 0;
+}
+
+int surrogate_set_params(SurrogateState *s, const float *host, hipStream_t st, std::string &err) {
+  SCHK(hipSetDevice(s->device));
+  std::vector<float> h(host, host + s->nw);
+  // keep the padding exactly zero
+  for (int l = 0; l < kSurLayers; ++l)
+    for (int r = 0; r < kOutPad[l]; ++r)
+      for (int c = 0; c < kLd[l]; ++c)
+        if (r >= kDout[l] || (c >= kDin[l] && c != kKpad[l])) h[(size_t)(s->woff[l] + (int64_t)r * kLd[l] + c)] = 0.f;
+  return upload_params(s, h, st, err);
+}
+
+static int stage_batch(SurrogateState *s, int64_t B, const float *x, const float *t, hipStream_t st,
+                       std::string &err) {
+  if (B <= 0 || B % 8) {
+    err = "batch size must be a positive multiple of 8";
     return 1;
   }
+  if (ensure_train_ws(s, B, st, err)) return 1;
+  float *xin = nullptr;
+  SCHK(dalloc(&xin, B * kSurIn));
+  SCHK(hipMemcpyAsync(xin, x, (size_t)B * kSurIn * sizeof(float), hipMemcpyHostToDevice, st));
+  hipLaunchKernelGGL(k_pack_x, dim3(nb256(B)), dim3(256), 0, st, B, (const float *)xin, s->tX);
+  SCHK(hipGetLastError());
+  if (t) {
+    std::vector<float> tp((size_t)B * kYld, 0.f);
+    for (int64_t r = 0; r < B; ++r)
+      for (int j = 0; j < kSurOut; ++j) tp[(size_t)(r * kYld + j)] = t[r * kSurOut + j];
+    SCHK(hipMemcpyAsync(s->tT, tp.data(), tp.size() * sizeof(float), hipMemcpyHostToDevice, st));
+    SCHK(hipStreamSynchronize(st));
+  } else {
+    SCHK(hipStreamSynchronize(st));
+  }
+  SCHK(hipFree(xin));
+  return 0;
+}
+
+int surrogate_forward(SurrogateState *s, int64_t B, const float *x, float *y, hipStream_t st, std::string &err) {
+  SCHK(hipSetDevice(s->device));
+  if (stage_batch(s, B, x, nullptr, st, err)) return 1;
+  if (forward(s, B, s->tX, s->tH, s->tZ, s->tY, false, st, err)) return 1;
+  std::vector<float> yp((size_t)B * kYld);
+  SCHK(hipMemcpyAsync(yp.data(), s->tY, yp.size() * sizeof(float), hipMemcpyDeviceToHost, st));
+  SCHK(hipStreamSynchronize(st));
+  for (int64_t r = 0; r < B; ++r)
+    for (int j = 0; j < kSurOut; ++j) y[r * kSurOut + j] = yp[(size_t)(r * kYld + j)];
+  return 0;
+}
+
+int surrogate_gradient(SurrogateState *s, int64_t B, const float *x, const float *t, float *grad, double *loss,
+                       hipStream_t st, std::string &err) {
+  SCHK(hipSetDevice(s->device));
+  if (stage_batch(s, B, x, t, st, err)) return 1;
+  SCHK(hipMemsetAsync(s->loss, 0, sizeof(float), st));
+  if (forward(s, B, s->tX, s->tH, s->tZ, s->tY, false, st, err)) return 1;
+  if (backward(s, B, s->tX, s->tT, st, err)) return 1;
+  float l = 0;
+  SCHK(hipMemcpyAsync(&l, s->loss, sizeof(float), hipMemcpyDeviceToHost, st));
+  SCHK(hipMemcpyAsync(grad, s->g, (size_t)s->nw * sizeof(float), hipMemcpyDeviceToHost, st));
+  SCHK(hipStreamSynchronize(st));
+  if (loss) *loss = l;
+  return 0;
+}
+
+int surrogate_train(SurrogateState *s, const DMech *m, int64_t n, const double *rho, const double *T,
+                    const double *Y, double dt, const SurrogateTrainCfg &cfg, double *loss_hist, hipStream_t st,
+                    std::string &err) {
+  SCHK(hipSetDevice(s->device));
+  const int64_t B = cfg.batch;
+  if (B <= 0 || B % 8 || n < B || n >= (int64_t)1 << 31) {
+    err = "training needs a batch that is a positive multiple of 8, and batch <= n < 2^31";
+    return 1;
+  }
+  if (!(dt > 0)) {
+    err = "dt must be positive";
+    return 1;
+  }
+  // 1. targets from the FP64 point-implicit integrator
+  double *dY = nullptr, *dTt = nullptr;
+  SCHK(dalloc(&dY, n * kNS));
+  SCHK(dalloc(&dTt, n));
+  SCHK(launch_chem_batch(m, n, rho, T, Y, dt, dY, dTt, st));
+  // 2. raw features / targets and their statistics
+  float *F = nullptr, *Tt = nullptr;
+  SCHK(dalloc(&F, n * kRawLd));
+  SCHK(dalloc(&Tt, n * kRawLd));
+  hipLaunchKernelGGL(k_sur_raw, dim3(nb256(n)), dim3(256), 0, st, m, n, rho, T, Y, (const double *)dY,
+                     (const double *)dTt, s->norm, F, Tt);
+  SCHK(hipGetLastError());
+  SCHK(hipFree(dY));
+  SCHK(hipFree(dTt));
+  // asinh scale of each target: the median |c_j| of an evenly strided sample
+  {
+    const int64_t ns = std::min<int64_t>(n, 65536), stride = n / ns;
+    std::vector<float> smp((size_t)ns * kRawLd);
+    SCHK(hipMemcpy2DAsync(smp.data(), kRawLd * sizeof(float), Tt, (size_t)stride * kRawLd * sizeof(float),
+                          kRawLd * sizeof(float), (size_t)ns, hipMemcpyDeviceToHost, st));
+    SCHK(hipStreamSynchronize(st));
+    std::vector<double> col((size_t)ns);
+    for (int j = 0; j < kSurOut; ++j) {
+      double amax = 0;
+      for (int64_t r = 0; r < ns; ++r) {
+        col[(size_t)r] = std::fabs((double)smp[(size_t)(r * kRawLd + j)]);
+        amax = std::max(amax, col[(size_t)r]);
+      }
+      std::nth_element(col.begin(), col.begin() + ns / 2, col.end());
+      double med = col[(size_t)(ns / 2)];
+      if (!(med > 1e-9 * amax) || !(med > 0)) med = amax > 0 ? amax : 1.0;
+      s->norm.c_scale[j] = med;
+    }
+  }
+  hipLaunchKernelGGL(k_sur_transform, dim3(nb256(n)), dim3(256), 0, st, n, Tt, s->norm);
+  SCHK(hipGetLastError());
+  const int nblk = 1024;
+  double *part = nullptr;
+  SCHK(dalloc(&part, (int64_t)nblk * 16 * 4 * 2));
+  hipLaunchKernelGGL(k_colstats, dim3(nblk), dim3(256), 0, st, (const float *)F, n, part);
+  hipLaunchKernelGGL(k_colstats, dim3(nblk), dim3(256), 0, st, (const float *)Tt, n, part + nblk * 16 * 4);
+  SCHK(hipGetLastError());
+  std::vector<double> hp((size_t)nblk * 16 * 4 * 2);
+  SCHK(hipMemcpyAsync(hp.data(), part, hp.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+  SCHK(hipStreamSynchronize(st));
+  SCHK(hipFree(part));
+  auto stats = [&](int which, int col, double &mean, double &sd, double &lo, double &hi) {
+    double sm = 0, sq = 0;
+    lo = 1e300;
+    hi = -1e300;
+    for (int b = 0; b < nblk; ++b) {
+      const double *o = &hp[(size_t)(((which * nblk + b) * 16 + col) * 4)];
+      sm += o[0];
+      sq += o[1];
+      lo = std::min(lo, o[2]);
+      hi = std::max(hi, o[3]);
+    }
+    mean = sm / (double)n;
+    sd = std::sqrt(std::max(0.0, sq / (double)n - mean * mean));
+    if (!(sd > 1e-9 * (1.0 + std::fabs(mean)))) sd = 1.0;  // constant column
+  };
+  for (int i = 0; i < kSurIn; ++i) {
+    double sd;
+    stats(0, i, s->norm.x_mean[i], sd, s->x_min[i], s->x_max[i]);
+    s->norm.x_istd[i] = 1.0 / sd;
+  }
+  for (int j = 0; j < kSurOut; ++j) {
+    double lo, hi;
+    stats(1, j, s->norm.c_mean[j], s->norm.c_std[j], lo, hi);
+  }
+  finish_norm(s);
+  // 3. standardised training set
+  __bf16 *Xall = nullptr;
+  SCHK(dalloc(&Xall, n * kXld));
+  hipLaunchKernelGGL(k_sur_standardize, dim3(nb256(n)), dim3(256), 0, st, n, (const float *)F, Tt, s->norm, Xall);
+  SCHK(hipGetLastError());
+  SCHK(hipFree(F));
+  if (ensure_train_ws(s, B, st, err)) return 1;
+  int *perm = nullptr;
+  SCHK(dalloc(&perm, n));
+  std::vector<int> hperm((size_t)n);
+  const int64_t steps = n / B;
+  const int64_t total = steps * cfg.epochs;
+  int64_t gstep = 0;
+  SCHK(hipMemsetAsync(s->m1, 0, (size_t)s->nw * sizeof(float), st));
+  SCHK(hipMemsetAsync(s->m2, 0, (size_t)s->nw * sizeof(float), st));
+  s->adam_t = 0;
+  s->ready = false;
+  uint64_t rng = cfg.seed ^ 0x5DEECE66Dull;
+  for (int64_t e = 0; e < cfg.epochs; ++e) {
+    for (int64_t i = 0; i < n; ++i) hperm[(size_t)i] = (int)i;
+    for (int64_t i = n - 1; i > 0; --i) std::swap(hperm[(size_t)i], hperm[(size_t)(splitmix(rng) % (uint64_t)(i + 1))]);
+    SCHK(hipMemcpyAsync(perm, hperm.data(), (size_t)n * sizeof(int), hipMemcpyHostToDevice, st));
+    SCHK(hipMemsetAsync(s->loss, 0, sizeof(float), st));
+    for (int64_t k = 0; k < steps; ++k, ++gstep) {
+      const int *idx = perm + k * B;
+      SCHK(launch_gather_rows_bf16(Xall, idx, (int)B, kXld, s->tX, st));
+      SCHK(launch_gather_rows_f32(Tt, idx, (int)B, kYld, s->tT, st));
+      if (forward(s, B, s->tX, s->tH, s->tZ, s->tY, false, st, err)) return 1;
+      if (backward(s, B, s->tX, s->tT, st, err)) return 1;
+      const double frac = total > 1 ? (double)gstep / (double)(total - 1) : 1.0;
+      const double lr = cfg.lr_min + 0.5 * (cfg.lr - cfg.lr_min) * (1 + std::cos(M_PI * frac));
+      SCHK(launch_adam(s->w, s->g, s->m1, s->m2, s->nw, (float)lr, (float)cfg.beta1, (float)cfg.beta2,
+                       (float)cfg.eps, ++s->adam_t, st));
+    }
+    float l = 0;
+    SCHK(hipMemcpyAsync(&l, s->loss, sizeof(float), hipMemcpyDeviceToHost, st));
+    SCHK(hipStreamSynchronize(st));
+    if (loss_hist) loss_hist[e] = (double)l / (double)steps;
+  }
+  SCHK(launch_to_bf16(s->w, s->wb, s->nw, st));
+  SCHK(hipStreamSynchronize(st));
+  SCHK(hipFree(perm));
+  SCHK(hipFree(Xall));
+  SCHK(hipFree(Tt));
+  s->dt_train = dt;
+  finish_norm(s);
+  s->ready = true;
+  return 0;
+}
+
+int surrogate_infer(SurrogateState *s, const DMech *m, int64_t n, const double *rho, const double *T,
+                    const double *Y, double dt, double *dY, double *dT, int64_t *n_fallback, hipStream_t st,
+                    std::string &err) {
+  if (!surrogate_ready(s)) {
+    err = "surrogate not trained or loaded";
+    return 1;
+  }
+  if (n >= (int64_t)1 << 31) {
+    err = "surrogate batches are limited to 2^31 - 1 cells";
+    return 1;
+  }
+  SCHK(hipSetDevice(s->device));
+  if (n <= 0) {
+    if (n_fallback) *n_fallback = 0;
+    return 0;
+  }
+  // a network trained for another dt is not used: every cell falls back
+  const int force = std::fabs(dt - s->dt_train) > 1e-12 * s->dt_train ? 1 : 0;
+  const int64_t chunk = std::min<int64_t>(n, kInferChunk);
+  if (ensure_infer_ws(s, (chunk + 7) / 8 * 8, n, st, err)) return 1;
+  SCHK(hipEventRecord(s->ev[0], st));
+  SCHK(hipMemsetAsync(s->fb_count, 0, sizeof(int), st));
+  for (int64_t c0 = 0; c0 < n; c0 += chunk) {
+    const int64_t cnt = std::min(chunk, n - c0);
+    if (!force) {
+      hipLaunchKernelGGL(k_sur_features, dim3(nb256(cnt)), dim3(256), 0, st, m, n, c0, cnt, rho, T, Y, s->norm,
+                         s->iX);
+      SCHK(hipGetLastError());
+      __bf16 *H[3] = {s->iH[0], s->iH[1], s->iH[0]};
+      if (forward(s, cnt, s->iX, H, nullptr, s->iOut, true, st, err)) return 1;
+    }
+    hipLaunchKernelGGL(k_sur_finish, dim3(nb256(cnt)), dim3(256), 0, st, m, s->norm, n, c0, cnt, rho, T, Y,
+                       (const float *)s->iOut, force, dY, dT, s->fb_count, s->fb_idx);
+    SCHK(hipGetLastError());
+  }
+  SCHK(hipEventRecord(s->ev[1], st));
+  int nf = 0;
+  SCHK(hipMemcpyAsync(&nf, s->fb_count, sizeof(int), hipMemcpyDeviceToHost, st));
+  SCHK(hipStreamSynchronize(st));
+  if (nf < 0 || nf > n) {
+    err = "fallback count out of range";
+    return 1;
+  }
+  SCHK(launch_chem_batch_subset(m, n, nf, s->fb_idx, rho, T, Y, dt, dY, dT, st));
+  SCHK(hipEventRecord(s->ev[2], st));
+  SCHK(hipEventSynchronize(s->ev[2]));
+  float a = 0, b = 0;
+  SCHK(hipEventElapsedTime(&a, s->ev[0], s->ev[1]));
+  SCHK(hipEventElapsedTime(&b, s->ev[1], s->ev[2]));
+  s->last_ms[0] = a;
+  s->last_ms[1] = b;
+  s->last_fallback = nf;
+  if (n_fallback) *n_fallback = nf;
+  return 0;
+}
+
+// ---------------------------------------------------------------- file IO
+namespace {
+const char kMagic[8] = {'O', 'S', 'T', 'C', 'S', 'U', 'R', '2'};
+}
+
+int surrogate_save(SurrogateState *s, const std::string &path, hipStream_t st, std::string &err) {
+  if (!surrogate_ready(s)) {
+    err = "nothing to save: surrogate not trained or loaded";
+    return 1;
+  }
+  std::vector<float> w((size_t)s->nw);
+  if (surrogate_get_params(s, w.data(), st, err)) return 1;
+  std::ofstream f(path, std::ios::binary);
+  if (!f) {
+    err = "cannot open " + path;
+    return 1;
+  }
+  const int32_t dims[4] = {kSurIn, kSurOut, kSurLayers, 256};
+  f.write(kMagic, 8);
+  f.write((const char *)dims, sizeof(dims));
+  f.write((const char *)&s->dt_train, sizeof(double));
+  f.write((const char *)s->norm.x_mean, sizeof(s->norm.x_mean));
+  f.write((const char *)s->norm.x_istd, sizeof(s->norm.x_istd));
+  f.write((const char *)s->x_min, sizeof(s->x_min));
+  f.write((const char *)s->x_max, sizeof(s->x_max));
+  f.write((const char *)s->norm.c_mean, sizeof(s->norm.c_mean));
+  f.write((const char *)s->norm.c_std, sizeof(s->norm.c_std));
+  f.write((const char *)s->norm.c_scale, sizeof(s->norm.c_scale));
+  f.write((const char *)s->norm.basis, sizeof(s->norm.basis));
+  const int64_t nw = s->nw;
+  f.write((const char *)&nw, sizeof(nw));
+  f.write((const char *)w.data(), (std::streamsize)(w.size() * sizeof(float)));
+  if (!f) {
+    err = "write failed: " + path;
+    return 1;
+  }
+  return 0;
+}
+
+int surrogate_load(SurrogateState *s, const std::string &path, hipStream_t st, std::string &err) {
+  std::ifstream f(path, std::ios::binary);
+  if (!f) {
+    err = "cannot open " + path;
+    return 1;
+  }
+  char magic[8];
+  int32_t dims[4];
+  f.read(magic, 8);
+  f.read((char *)dims, sizeof(dims));
+  if (!f || std::memcmp(magic, kMagic, 8) != 0 || dims[0] != kSurIn || dims[1] != kSurOut ||
+      dims[2] != kSurLayers || dims[3] != 256) {
+    err = path + ": not a surrogate file of this network shape";
+    return 1;
+  }
+  SurNorm nm = s->norm;
+  double dt = 0, xmin[kSurIn], xmax[kSurIn], basis[kNS * kSurNull];
+  f.read((char *)&dt, sizeof(double));
+  f.read((char *)nm.x_mean, sizeof(nm.x_mean));
+  f.read((char *)nm.x_istd, sizeof(nm.x_istd));
+  f.read((char *)xmin, sizeof(xmin));
+  f.read((char *)xmax, sizeof(xmax));
+  f.read((char *)nm.c_mean, sizeof(nm.c_mean));
+  f.read((char *)nm.c_std, sizeof(nm.c_std));
+  f.read((char *)nm.c_scale, sizeof(nm.c_scale));
+  f.read((char *)basis, sizeof(basis));
+  int64_t nw = 0;
+  f.read((char *)&nw, sizeof(nw));
+  if (!f || nw != s->nw) {
+    err = path + ": truncated or wrong parameter count";
+    return 1;
+  }
+  for (int k = 0; k < kNS * kSurNull; ++k)
+    if (std::fabs(basis[k] - s->norm.basis[k]) > 1e-12) {
+      err = path + ": trained for a mechanism with other species or elements";
+      return 1;
+    }
+  std::vector<float> w((size_t)nw);
+  f.read((char *)w.data(), (std::streamsize)(w.size() * sizeof(float)));
+  if (!f) {
+    err = path + ": truncated parameters";
+    return 1;
+  }
+  if (surrogate_set_params(s, w.data(), st, err)) return 1;
+  s->norm = nm;
+  std::copy(xmin, xmin + kSurIn, s->x_min);
+  std::copy(xmax, xmax + kSurIn, s->x_max);
+  s->dt_train = dt;
+  finish_norm(s);
+  s->ready = true;
+  return 0;
+}
+
+int surrogate_last_stats(const SurrogateState *s, double *out3) {
+  out3[0] = (double)s->last_fallback;
+  out3[1] = s->last_ms[0];
+  out3[2] = s->last_ms[1];
   return 0;
 }
 

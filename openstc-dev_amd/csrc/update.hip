@@ -513,13 +513,15 @@ __global__ void k_copy1(const double *src, double *dst) { *dst = *src; }
 // exact Jacobian at frozen T, (I - dt J) x = omega, rho Y += dt x, then the
 // Newton temperature at the unchanged internal energy (the composition of
 // FiniteRateChem.cu:7-63/234-256 and FieldOperation.cuh:221-296 at V = 0).
+// With idx, thread t handles cell idx[t] for t < count; otherwise cell t < n.
 template <int NS>
-__global__ void __launch_bounds__(128) k_chem_batch(const DMech *__restrict__ m, int64_t n,
-                                                    const double *__restrict__ rho_a, const double *__restrict__ T_a,
-                                                    const double *__restrict__ Y_a, double dt, double *__restrict__ dY,
-                                                    double *__restrict__ dT) {
-  const int64_t c = (int64_t)blockIdx.x * 128 + threadIdx.x;
-  if (c >= n) return;
+__global__ void __launch_bounds__(128) k_chem_batch(const DMech *__restrict__ m, int64_t n, int64_t count,
+                                                    const int *__restrict__ idx, const double *__restrict__ rho_a,
+                                                    const double *__restrict__ T_a, const double *__restrict__ Y_a,
+                                                    double dt, double *__restrict__ dY, double *__restrict__ dT) {
+  const int64_t t = (int64_t)blockIdx.x * 128 + threadIdx.x;
+  if (t >= count) return;
+  const int64_t c = idx ? (int64_t)idx[t] : t;
   const double rho = rho_a[c], T = T_a[c];
   double Y[NS], om[NS], J[NS * NS], h[NS];
 #pragma unroll
@@ -669,7 +671,18 @@ hipError_t launch_copy_dt(const double *src, double *dst, hipStream_t s) {
 hipError_t launch_chem_batch(const DMech *m, int64_t n, const double *rho, const double *T, const double *Y, double dt,
                              double *dY, double *dT, hipStream_t s) {
   if (n <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_chem_batch<9>, dim3(nblk(n, 128)), dim3(128), 0, s, m, n, rho, T, Y, dt, dY, dT);
+  hipLaunchKernelGGL(k_chem_batch<9>, dim3(nblk(n, 128)), dim3(128), 0, s, m, n, n, (const int *)nullptr, rho, T, Y,
+                     dt, dY, dT);
+  return hipGetLastError();
+}
+
+hipError_t launch_chem_batch_subset(const DMech *m, int64_t n, int64_t count, const int *idx, const double *rho,
+                                    const double *T, const double *Y, double dt, double *dY, double *dT,
+                                    hipStream_t s) {
+  if (count <= 0) return hipSuccess;
+  if (count > n) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_chem_batch<9>, dim3(nblk(count, 128)), dim3(128), 0, s, m, n, count, idx, rho, T, Y, dt, dY,
+                     dT);
   return hipGetLastError();
 }
 

@@ -22,6 +22,18 @@ class OstcError(RuntimeError):
 _lib = None
 
 
+class TrainCfg(C.Structure):
+    """ostc_surrogate_train_cfg"""
+    _fields_ = [("epochs", C.c_int64), ("batch", C.c_int64), ("lr", C.c_double), ("lr_min", C.c_double),
+                ("beta1", C.c_double), ("beta2", C.c_double), ("eps", C.c_double), ("seed", C.c_uint64)]
+
+
+class Gate(C.Structure):
+    """ostc_surrogate_gate"""
+    _fields_ = [("cons_tol", C.c_double), ("dy_max", C.c_double), ("dT_max", C.c_double), ("margin", C.c_double),
+                ("src_max", C.c_double)]
+
+
 def lib():
     global _lib
     if _lib is None:
@@ -29,6 +41,7 @@ def lib():
             raise OstcError(f"{LIB_PATH} not built; run openstc-dev_amd/build.py")
         L = C.CDLL(LIB_PATH)
         vp, dp, ip = C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int)
+        fp = C.POINTER(C.c_float)
         sig = {
             "ostc_abi_version": ([], C.c_int),
             "ostc_last_error": ([], C.c_char_p),
@@ -66,6 +79,19 @@ def lib():
             "ostc_sync": ([vp], C.c_int),
             "ostc_profile": ([vp, C.c_int], C.c_int),
             "ostc_profile_report": ([vp, C.c_char_p, C.c_int], C.c_int),
+            "ostc_surrogate_init": ([vp, C.c_uint64], C.c_int),
+            "ostc_surrogate_param_count": ([vp, C.POINTER(C.c_int64)], C.c_int),
+            "ostc_surrogate_get_params": ([vp, fp], C.c_int),
+            "ostc_surrogate_set_params": ([vp, fp], C.c_int),
+            "ostc_surrogate_forward": ([vp, C.c_int64, fp, fp], C.c_int),
+            "ostc_surrogate_gradient": ([vp, C.c_int64, fp, fp, fp, dp], C.c_int),
+            "ostc_surrogate_train": ([vp, C.c_int64, vp, vp, vp, C.c_double, C.POINTER(TrainCfg), dp], C.c_int),
+            "ostc_surrogate_set_gate": ([vp, C.POINTER(Gate)], C.c_int),
+            "ostc_surrogate_get_gate": ([vp, C.POINTER(Gate)], C.c_int),
+            "ostc_surrogate_norm": ([vp, dp], C.c_int),
+            "ostc_surrogate_save": ([vp, C.c_char_p], C.c_int),
+            "ostc_surrogate_load": ([vp, C.c_char_p], C.c_int),
+            "ostc_surrogate_last_stats": ([vp, dp], C.c_int),
         }
         for name, (args, res) in sig.items():
             f = getattr(L, name)
@@ -86,6 +112,10 @@ def _chk(rc):
 
 def _dp(a):
     return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _fp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
 
 
 class DeviceArray:
@@ -123,6 +153,7 @@ class Solver:
         h = C.c_void_p()
         _chk(lib().ostc_init(case_dir.encode(), device, C.byref(h)))
         self.h = h
+        self.root = case_dir
         self.blocks = []
 
     def close(self):
@@ -248,6 +279,74 @@ class Solver:
 
     def sync(self):
         _chk(lib().ostc_sync(self.h))
+
+    # ---- MLP chemistry surrogate
+    def surrogate_init(self, seed):
+        _chk(lib().ostc_surrogate_init(self.h, seed))
+
+    def surrogate_param_count(self):
+        n = C.c_int64()
+        _chk(lib().ostc_surrogate_param_count(self.h, C.byref(n)))
+        return n.value
+
+    def surrogate_params(self):
+        w = np.empty(self.surrogate_param_count(), dtype=np.float32)
+        _chk(lib().ostc_surrogate_get_params(self.h, _fp(w)))
+        return w
+
+    def surrogate_set_params(self, w):
+        w = np.ascontiguousarray(w, dtype=np.float32)
+        if w.size != self.surrogate_param_count():
+            raise OstcError("parameter count mismatch")
+        _chk(lib().ostc_surrogate_set_params(self.h, _fp(w)))
+
+    def surrogate_forward(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        y = np.empty((x.shape[0], 7), dtype=np.float32)
+        _chk(lib().ostc_surrogate_forward(self.h, x.shape[0], _fp(x), _fp(y)))
+        return y
+
+    def surrogate_gradient(self, x, t):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        t = np.ascontiguousarray(t, dtype=np.float32)
+        g = np.empty(self.surrogate_param_count(), dtype=np.float32)
+        loss = C.c_double()
+        _chk(lib().ostc_surrogate_gradient(self.h, x.shape[0], _fp(x), _fp(t), _fp(g), C.byref(loss)))
+        return g, loss.value
+
+    def surrogate_train(self, n, rho, T, Y, dt, epochs=8, batch=8192, lr=2e-3, lr_min=2e-5, seed=2004,
+                        beta1=0.9, beta2=0.999, eps=1e-8):
+        """Train on n device states (DeviceArray); returns the per-epoch loss."""
+        cfg = TrainCfg(epochs, batch, lr, lr_min, beta1, beta2, eps, seed)
+        hist = np.zeros(epochs)
+        _chk(lib().ostc_surrogate_train(self.h, n, rho.ptr, T.ptr, Y.ptr, dt, C.byref(cfg), _dp(hist)))
+        return hist
+
+    def surrogate_gate(self, **kw):
+        g = Gate()
+        _chk(lib().ostc_surrogate_get_gate(self.h, C.byref(g)))
+        for k, v in kw.items():
+            setattr(g, k, v)
+        if kw:
+            _chk(lib().ostc_surrogate_set_gate(self.h, C.byref(g)))
+        return {k: getattr(g, k) for k, _ in Gate._fields_}
+
+    def surrogate_norm(self):
+        o = np.zeros(120)
+        _chk(lib().ostc_surrogate_norm(self.h, _dp(o)))
+        return dict(x_mean=o[0:11], x_std=o[11:22], x_min=o[22:33], x_max=o[33:44], u_mean=o[44:51],
+                    u_std=o[51:58], dt=o[58], basis=o[59:113].reshape(9, 6), c_scale=o[113:120])
+
+    def surrogate_save(self, path):
+        _chk(lib().ostc_surrogate_save(self.h, path.encode()))
+
+    def surrogate_load(self, path):
+        _chk(lib().ostc_surrogate_load(self.h, path.encode()))
+
+    def surrogate_last_stats(self):
+        o = np.zeros(3)
+        _chk(lib().ostc_surrogate_last_stats(self.h, _dp(o)))
+        return dict(n_fallback=int(o[0]), net_ms=o[1], fallback_ms=o[2])
 
     def profile(self, on):
         _chk(lib().ostc_profile(self.h, 1 if on else 0))
