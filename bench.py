@@ -1,10 +1,14 @@
 """Benchmark: Mcell-updates/s of the SSP-RK3 step on the 3-D laminar
 Navier-Stokes air case (256x128x128 curvilinear, AWENO-5 + CDS, AUSM+,
 second-order viscous, CFL dt recomputed every stage, residual every step) on
-one MI355X, FP64.  Also reported: the same step with FP32 flux arithmetic,
-the FP64 point-implicit chemistry batch (2^22 cells, dt = 1e-7 s) in
-cells/s, the roofline of the dominant kernel and a serial CPU baseline
-(the oracle) on a bounded sample.
+one MI355X, FP64.  Also reported: the same step with FP32 flux arithmetic;
+the 2-D reacting 9-species mixing layer (512x256, HLLC + MUSCL, viscous,
+finite-rate source) in FP64 and mixed precision; the chemistry batch (2^22
+cells, dt = 1e-7 s) with the FP64 point-implicit integrator and with the
+BF16-MFMA surrogate + FP64 fallback (trained first, untimed, on 2^24 pairs
+from the integrator) and the surrogate's error against the integrator; the
+rooflines of the dominant sweep (HBM), the plain chemistry (FP64) and the
+surrogate (MFMA); a serial CPU baseline (the oracle) on a bounded sample.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W]
 
@@ -27,11 +31,27 @@ import numpy as np  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
+BF16_MFMA_PEAK_TFLOPS = 2500.0
+
+# surrogate network FLOP per cell: 2 (11*256 + 256*256 + 256*256 + 256*7)
+SURROGATE_FLOP_PER_CELL = 2 * (11 * 256 + 2 * 256 * 256 + 256 * 7)
 
 # compulsory HBM bytes per cell of one j- (or k-) direction sweep launch:
 # reads cv(5) + bv(6) + mu + lambda + J + metric(9) = 23 doubles, read and
 # write dq(5) = 10 doubles; halo re-reads are not counted
 SWEEP_JK_BYTES_PER_CELL = 8 * (23 + 10)
+
+
+def chem_flop_per_cell(labels, ns=9):
+    """Static operation count of one point-implicit cell step as the FP64
+    kernel executes it (+ - * / = 1, FMA = 2, exp/log/pow = 1): state and
+    energy 154, concentrations and g/RT 143, per reaction 325 (Arrhenius 5,
+    kb 21, progress with pow over all species 38, creation/destruction 72,
+    Jacobian row updates 189) plus the third-body (19), duplicate (6) and
+    Troe fall-off (78) extras, source and Jacobian scaling 279, I - dt J 162,
+    9x9 LU with substitution 650, new Y 27, two Newton iterations 542."""
+    per = {0: 325, 1: 325, 2: 325, 3: 331, 4: 344, 5: 403, 6: 403, 7: 406}
+    return 154 + 143 + sum(per.get(int(l), 325) for l in labels) + 279 + 162 + 650 + 27 + 542 + 10
 
 
 def log(msg):
@@ -62,6 +82,32 @@ def cpu_baseline_chem(n=1 << 20):
     return n / (t1 - t0), f"{n} cells of the seeded batch", t1 - t0
 
 
+def c3_solver(d, nx, ny):
+    """Reacting mixing-layer solver with its seeded initial field."""
+    import ostc
+    from ostc import cases
+    case = cases.c3_case(nx, ny)
+    s = ostc.solver_for_case(case, os.path.join(d, "c3"), initial=False)
+    states = {}
+    for name in ("air_stream", "fuel_stream"):
+        o = s.inflow_state(name, 16)
+        states[name] = dict(rho=o[0], u=o[1], v=o[2], w=o[3], p=o[4], t=o[5], vel=o[6], sv=o[7:16])
+    bv, sv = cases.c3_fields(case, states, s.table("mw", 9))
+    s.upload("bv", bv)
+    s.upload("sv", sv)
+    s.initialize()
+    return s
+
+
+def time_steps(s, steps, warmup):
+    s.step(warmup)
+    s.sync()
+    t0 = time.perf_counter()
+    s.step(steps)
+    s.sync()
+    return time.perf_counter() - t0
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -71,7 +117,10 @@ def main():
     ap.add_argument("--ny", type=int, default=128)
     ap.add_argument("--nz", type=int, default=128)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-c3", action="store_true")
     ap.add_argument("--chem-cells", type=int, default=1 << 22)
+    ap.add_argument("--train-pairs", type=int, default=1 << 24)
+    ap.add_argument("--epochs", type=int, default=16)
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build runs on one GPU; multi-GPU sharding is described in DESIGN.md "
@@ -82,7 +131,6 @@ def main():
     from ostc import cases
 
     cells = a.nx * a.ny * a.nz
-    result = {}
     with tempfile.TemporaryDirectory() as d:
         case = cases.c2_case(a.nx, a.ny, a.nz)
         log(f"bench: building case {a.nx}x{a.ny}x{a.nz}")
@@ -101,19 +149,27 @@ def main():
         fp64_s = t1 - t0
         # FP32-flux variant on the same state path
         s.set_flux_fp32(True)
-        s.step(a.warmup)
-        s.sync()
         log(f"bench: timing {a.steps} mixed-precision steps")
-        t2 = time.perf_counter()
-        s.step(a.steps)
-        s.sync()
-        t3 = time.perf_counter()
-        mixed_s = t3 - t2
+        mixed_s = time_steps(s, a.steps, a.warmup)
         finite = bool(np.isfinite(res_h).all())
         s.close()
 
-        # chemistry batch (C4 inputs), FP64 point-implicit
-        chem = None
+        c3 = None
+        if not a.no_c3:
+            log("bench: reacting mixing layer 512x256")
+            s3 = c3_solver(d, 512, 256)
+            c3_cells = 512 * 256
+            t64 = time_steps(s3, a.steps, a.warmup)
+            s3.set_flux_fp32(True)
+            t32 = time_steps(s3, a.steps, a.warmup)
+            s3.close()
+            c3 = {"workload": "2-D reacting H2/air mixing layer 512x256, 9 species / 19 reactions, HLLC + MUSCL, "
+                              "viscous, explicit finite-rate source, SSP-RK3",
+                  "fp64_mcell_per_s": c3_cells * a.steps / t64 / 1e6, "fp64_ms_per_step": t64 / a.steps * 1e3,
+                  "mixed_mcell_per_s": c3_cells * a.steps / t32 / 1e6, "mixed_ms_per_step": t32 / a.steps * 1e3}
+
+        # chemistry batch: FP64 point-implicit, then the surrogate
+        chem = sur = None
         if a.chem_cells > 0:
             cc = cases.c1_case(20, 12)
             cc["setup"]["int species"] = 1
@@ -122,10 +178,12 @@ def main():
             root = os.path.join(d, "chem")
             cases.write_case_dir(cc, root)
             sc = ostc.Solver(root, 0)
+            labels = sc.table("label", 19)
             n = a.chem_cells
             rho, T, p, Y = cases.chem_batch_inputs(n, seed=1004)
             dr, dT_, dY_ = (ostc.DeviceArray.from_host(sc, v) for v in (rho, T, Y))
             oY, oT = ostc.DeviceArray(sc, Y.nbytes), ostc.DeviceArray(sc, T.nbytes)
+            log("bench: chemistry batch FP64")
             sc.chem_batch(n, dr, dT_, dY_, 1e-7, oY, oT, mode=0)
             sc.profile(True)
             reps = 3
@@ -135,9 +193,56 @@ def main():
             sc.sync()
             t5 = time.perf_counter()
             cp = sc.profile_report()
+            sc.profile(False)
+            Y0, T0 = oY.to_host(Y.shape), oT.to_host(T.shape)
+            k_ms = cp["chem_batch_fp64"][1] / cp["chem_batch_fp64"][0]
+            fpc = chem_flop_per_cell(labels)
+            chem = dict(cells=n, cells_per_s=n * reps / (t5 - t4), kernel_ms=k_ms, flop_per_cell_est=fpc,
+                        roofline={"bound": "fp64", "achieved": fpc * n / (k_ms * 1e-3) / 1e12,
+                                  "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                                  "frac": fpc * n / (k_ms * 1e-3) / 1e12 / FP64_PEAK_TFLOPS})
+            if a.train_pairs > 0:
+                log(f"bench: surrogate training on {a.train_pairs} pairs x {a.epochs} epochs (untimed)")
+                r2, t2, p2, y2 = cases.chem_batch_inputs(a.train_pairs, seed=2004)
+                tr = [ostc.DeviceArray.from_host(sc, v) for v in (r2, t2, y2)]
+                del r2, t2, p2, y2
+                sc.surrogate_init(11)
+                t6 = time.perf_counter()
+                hist = sc.surrogate_train(a.train_pairs, tr[0], tr[1], tr[2], 1e-7, epochs=a.epochs, batch=8192,
+                                          seed=2004)
+                t7 = time.perf_counter()
+                for v in tr:
+                    v.free()
+                log("bench: surrogate inference")
+                sc.chem_batch(n, dr, dT_, dY_, 1e-7, oY, oT, mode=1)
+                net_ms, fb_ms, nfs = [], [], []
+                t8 = time.perf_counter()
+                for _ in range(reps):
+                    nf = sc.chem_batch(n, dr, dT_, dY_, 1e-7, oY, oT, mode=1)
+                    st = sc.surrogate_last_stats()
+                    net_ms.append(st["net_ms"])
+                    fb_ms.append(st["fallback_ms"])
+                    nfs.append(nf)
+                sc.sync()
+                t9 = time.perf_counter()
+                Y1, T1 = oY.to_host(Y.shape), oT.to_host(T.shape)
+                err = np.abs(Y1 - Y0)
+                Tn0, Tn1 = T + T0, T + T1
+                net = float(np.mean(net_ms))
+                achieved = SURROGATE_FLOP_PER_CELL * n / (net * 1e-3) / 1e12
+                sur = dict(cells=n, cells_per_s=n * reps / (t9 - t8),
+                           speedup_vs_fp64=(n * reps / (t9 - t8)) / chem["cells_per_s"],
+                           fallback_fraction=nfs[-1] / n, network_and_gate_ms=net, fallback_ms=float(np.mean(fb_ms)),
+                           major_species_linf=float(err[[0, 2, 7, 8]].max()), all_species_linf=float(err.max()),
+                           T_rel_linf=float(np.max(np.abs(Tn1 - Tn0) / Tn0)),
+                           peak_T_rel_err=float(abs(Tn1.max() - Tn0.max()) / Tn0.max()),
+                           train_pairs=a.train_pairs, epochs=a.epochs, train_s=t7 - t6, final_loss=float(hist[-1]),
+                           gate=sc.surrogate_gate(),
+                           roofline={"bound": "mfma", "kernel": "k_sur_fused<128>", "achieved": achieved,
+                                     "peak": BF16_MFMA_PEAK_TFLOPS, "unit": "TFLOP/s",
+                                     "frac": achieved / BF16_MFMA_PEAK_TFLOPS,
+                                     "flop_per_cell": SURROGATE_FLOP_PER_CELL})
             sc.close()
-            chem = dict(cells_per_s=n * reps / (t5 - t4), kernel_ms=cp["chem_batch_fp64"][1] / cp["chem_batch_fp64"][0],
-                        cells=n)
 
     ms_per_step = fp64_s / a.steps * 1e3
     value = cells * a.steps / fp64_s / 1e6
@@ -183,7 +288,9 @@ def main():
         "cpu_baseline": cpu,
         "mixed_fp32_flux": {"value": cells * a.steps / mixed_s / 1e6, "unit": "Mcell-updates/s",
                             "ms_per_step": mixed_s / a.steps * 1e3},
+        "reacting_mixing_layer": c3,
         "chem_point_implicit_fp64": chem,
+        "chem_surrogate": sur,
         "kernel_avg_ms": stage_ms,
         "residual_finite": finite,
     }

@@ -1103,6 +1103,21 @@ int ostc_surrogate_load(ostc_solver *s, const char *path) {
   return 0;
 }
 
+int ostc_surrogate_set_kernel(ostc_solver *s, int kernel) {
+  if (need_sur(s)) return 1;
+  if (kernel < 0 || kernel > 2) return fail("kernel must be 0 (layer GEMMs), 1 (fused, 128 cells) or 2 (fused, 64)");
+  surrogate_set_kernel(s->sur, kernel);
+  return 0;
+}
+
+int ostc_surrogate_network_outputs(ostc_solver *s, int64_t n, const double *rho, const double *T, const double *Y,
+                                   float *host_out) {
+  if (need_sur(s)) return 1;
+  if (!host_out) return fail("null argument");
+  SURCALL(surrogate_network_outputs(s->sur, s->d_mech, n, rho, T, Y, host_out, s->stream, err));
+  return 0;
+}
+
 int ostc_surrogate_last_stats(ostc_solver *s, double *out) {
   if (need_sur(s)) return 1;
   if (!out) return fail("null argument");

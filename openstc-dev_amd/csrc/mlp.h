@@ -7,6 +7,17 @@
 
 namespace ostc {
 
+typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8;
+typedef __attribute__((__vector_size__(4 * sizeof(float)))) float f32x4;
+
+// GELU with the error function
+__device__ inline float gelu(float x) { return 0.5f * x * (1.0f + erff(x * 0.70710678118654752f)); }
+__device__ inline float gelu_grad(float x) {
+  const float cdf = 0.5f * (1.0f + erff(x * 0.70710678118654752f));
+  const float pdf = 0.39894228040143268f * expf(-0.5f * x * x);
+  return cdf + x * pdf;
+}
+
 enum GemmEpilogue {
   EPI_STORE = 0,      // C = acc (+ bias)
   EPI_GELU = 1,       // Z = acc (+ bias) saved if Z != null, C = GELU(Z)

@@ -26,21 +26,11 @@ namespace ostc {
 
 namespace {
 
-typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8;
-typedef __attribute__((__vector_size__(4 * sizeof(float)))) float f32x4;
-
 constexpr int BM = 64, BN = 64, BK = 32, LDK = 40;
 
 __device__ inline __bf16 to_bf(float x) { return (__bf16)x; }
 __device__ inline float from_bf(__bf16 x) { return (float)x; }
 
-// GELU with the error function and its derivative
-__device__ inline float gelu(float x) { return 0.5f * x * (1.0f + erff(x * 0.70710678118654752f)); }
-__device__ inline float gelu_grad(float x) {
-  const float cdf = 0.5f * (1.0f + erff(x * 0.70710678118654752f));
-  const float pdf = 0.39894228040143268f * expf(-0.5f * x * x);
-  return cdf + x * pdf;
-}
 
 template <typename T>
 __device__ inline void load8(const T *p, float *v) {

@@ -78,6 +78,11 @@ int surrogate_save(SurrogateState *s, const std::string &path, hipStream_t st, s
 int surrogate_load(SurrogateState *s, const std::string &path, hipStream_t st, std::string &err);
 // standardisation: x mean/std/min/max (4 x 11), u mean/std (2 x 7), dt, basis (54), asinh scales (7)
 int surrogate_get_norm(const SurrogateState *s, double *out120);
+// inference kernel: 1 fused (128 cells per workgroup, default), 2 fused (64), 0 layer-by-layer GEMMs
+void surrogate_set_kernel(SurrogateState *s, int kernel);
+// raw (standardised) network outputs [n x 7] of device states, copied to the host
+int surrogate_network_outputs(SurrogateState *s, const DMech *m, int64_t n, const double *rho, const double *T,
+                              const double *Y, float *host_out, hipStream_t st, std::string &err);
 // fallback count, network+gate ms and fallback ms of the last inference
 int surrogate_last_stats(const SurrogateState *s, double *out3);
 int surrogate_infer(SurrogateState *s, const DMech *m, int64_t n, const double *rho, const double *T,

@@ -40,6 +40,7 @@ struct SurNorm {
   double basis[kNS * kSurNull];  // dY_k = sum_j basis[k * 6 + j] c_j
   double cons[4 * kNS];          // element mass-fraction rows, then the mass row
   int n_cons;
+  int force;  // 1: every cell falls back
   double cons_tol, dy_max, dT_max, src_max, dt;
 };
 
@@ -263,6 +264,89 @@ __global__ void k_sur_features(const DMech *__restrict__ m, int64_t n, int64_t c
 // null-space coordinates of dY (j < 6) and dT (j = 6).
 // T is the Newton temperature at the unchanged internal energy started
 // from T + dT_net, as in the point-implicit batch kernel.
+// Decode one cell's network output o[0..6], apply the gate and, if the cell
+// is accepted, write (dY, dT); returns false when the cell must fall back.
+__device__ inline bool finish_cell(const DMech *__restrict__ m, const SurNorm &nm, int64_t n, int64_t c,
+                                   const double *__restrict__ rho_a, const double *__restrict__ T_a,
+                                   const double *__restrict__ Y_a, const float *o, double *__restrict__ dY,
+                                   double *__restrict__ dT) {
+  bool ok = true;
+  const double rho = rho_a[c], T = T_a[c];
+  double Y[kNS], x[kSurIn];
+  load_cell(n, c, Y_a, Y);
+  raw_features(m, rho, T, Y, x);
+#pragma unroll
+  for (int i = 0; i < kSurIn; ++i) ok = ok && x[i] >= nm.x_lo[i] && x[i] <= nm.x_hi[i];
+  double cj[kSurOut];
+#pragma unroll
+  for (int j = 0; j < kSurOut; ++j) cj[j] = nm.c_scale[j] * sinh(nm.c_mean[j] + nm.c_std[j] * (double)o[j]);
+  double dy[kNS];
+  double amax = 0;
+#pragma unroll
+  for (int k = 0; k < kNS; ++k) {
+    double s = 0;
+#pragma unroll
+    for (int j = 0; j < kSurNull; ++j) s += nm.basis[k * kSurNull + j] * cj[j];
+    dy[k] = s;
+    amax = fmax(amax, fabs(s));
+  }
+  const double dTn = cj[6];
+  ok = ok && isfinite(amax) && isfinite(dTn) && amax <= nm.dy_max && fabs(dTn) <= nm.dT_max;
+  if (ok) {
+    const float xi = explicit_increment(m, rho, T, Y, nm.dt);
+    ok = xi <= (float)nm.src_max;  // false for NaN as well
+  }
+  double err = 0, neg = 0;
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    double s = 0;
+#pragma unroll
+    for (int k = 0; k < kNS; ++k) s += nm.cons[r * kNS + k] * dy[k];
+    if (r < nm.n_cons) err = fmax(err, fabs(s));
+  }
+#pragma unroll
+  for (int k = 0; k < kNS; ++k) neg += fmax(0.0, -(Y[k] + dy[k])) + fmax(0.0, Y[k] + dy[k] - 1.0);
+  ok = ok && fmax(err, neg) <= nm.cons_tol;
+  if (ok) {
+    double mwi = 0, h[kNS];
+#pragma unroll
+    for (int l = 0; l < kNS; ++l) mwi += Y[l] / m->mw[l];
+    const double mw = 1 / mwi;
+    const double p_old = rho * T * (kRuD / mw);
+    d_enthalpy<kNS>(T, m, h);
+    double rhoE = 0;
+#pragma unroll
+    for (int l = 0; l < kNS; ++l) rhoE += h[l] * Y[l];
+    rhoE = rhoE * rho - p_old;
+    double Yn[kNS];
+#pragma unroll
+    for (int l = 0; l < kNS; ++l) Yn[l] = Y[l] + dy[l];
+    const double t0 = fmin(fmax(T + dTn, 0.5 * T), 2.0 * T + 1000.0);
+    double pn;
+    const double Tn = d_temperature<kNS>(m, rho, rhoE, 0.0, 0.0, 0.0, Yn, t0, pn);
+    ok = isfinite(Tn) && Tn > 0;
+    if (ok) {
+#pragma unroll
+      for (int l = 0; l < kNS; ++l) dY[(int64_t)l * n + c] = dy[l];
+      dT[c] = Tn - T;
+    }
+  }
+  return ok;
+}
+
+// wave-aggregated append of the cells with fb set to the fallback list
+__device__ inline void append_fallback(bool fb, int64_t c, int *__restrict__ fb_count, int *__restrict__ fb_idx) {
+  const unsigned long long mask = __ballot(fb);
+  if (mask) {
+    const int lane = threadIdx.x & 63;
+    const int leader = __ffsll((long long)mask) - 1;
+    int base = 0;
+    if (lane == leader) base = atomicAdd(fb_count, (int)__popcll(mask));
+    base = __shfl(base, leader, 64);
+    if (fb) fb_idx[base + (int)__popcll(mask & ((1ull << lane) - 1ull))] = (int)c;
+  }
+}
+
 __global__ void __launch_bounds__(256) k_sur_finish(const DMech *__restrict__ m, SurNorm nm, int64_t n, int64_t c0,
                                                     int64_t count, const double *__restrict__ rho_a,
                                                     const double *__restrict__ T_a, const double *__restrict__ Y_a,
@@ -272,79 +356,162 @@ __global__ void __launch_bounds__(256) k_sur_finish(const DMech *__restrict__ m,
   const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
   const bool active = t < count;
   const int64_t c = c0 + t;
-  bool ok = active && !force_fallback;
-  if (ok) {
-    const double rho = rho_a[c], T = T_a[c];
-    double Y[kNS], x[kSurIn];
-    load_cell(n, c, Y_a, Y);
-    raw_features(m, rho, T, Y, x);
+  bool ok = false;
+  if (active && !force_fallback) ok = finish_cell(m, nm, n, c, rho_a, T_a, Y_a, out + t * kYld, dY, dT);
+  append_fallback(active && !ok, c, fb_count, fb_idx);
+}
+
+// ------------------------------------------------------------ fused inference
+// The whole network of R cells in one workgroup of 4 wavefronts:
+// features -> 4 layers (activations resident in LDS, weight K-slices of 32
+// double-buffered through LDS) -> gate, Newton T and fallback list.
+// Hidden layers: wave w computes output columns [64 w, 64 w + 64) for all R
+// rows (R/16 x 4 tiles of v_mfma_f32_16x16x32_bf16, FP32 accumulation);
+// the output layer (7 columns in one 16-wide tile) splits the rows over the
+// waves.  Same BF16 roundings as the layer-by-layer path: BF16 operands,
+// FP32 accumulation, the bias taken from the BF16 weight copy, added in FP32.
+constexpr int kLdw = 40;  // LDS row of one weight slice: 32 k + 8 pad (BF16)
+
+template <int R>
+struct FusedLds {
+  __bf16 act[R][kHld];
+  __bf16 w[2][256][kLdw];
+  float out[R][8];
+};
+
+struct Slice {
+  uint4 v[4];
+};
+
+// row n = tid of W (leading dimension ld), k in [k0, k0 + 32); zero at k >= kmax
+__device__ inline Slice load_slice(const __bf16 *__restrict__ W, int ld, int k0, int kmax) {
+  Slice r;
+  const __bf16 *p = W + (int64_t)threadIdx.x * ld + k0;
 #pragma unroll
-    for (int i = 0; i < kSurIn; ++i) ok = ok && x[i] >= nm.x_lo[i] && x[i] <= nm.x_hi[i];
-    double cj[kSurOut];
+  for (int q = 0; q < 4; ++q) r.v[q] = (k0 + 8 * q < kmax) ? *(const uint4 *)(p + 8 * q) : make_uint4(0, 0, 0, 0);
+  return r;
+}
+
+__device__ inline void store_slice(__bf16 (*w)[kLdw], const Slice &r) {
 #pragma unroll
-    for (int j = 0; j < kSurOut; ++j) cj[j] = nm.c_scale[j] * sinh(nm.c_mean[j] + nm.c_std[j] * (double)out[t * kYld + j]);
-    double dy[kNS];
-    double amax = 0;
+  for (int q = 0; q < 4; ++q) *(uint4 *)&w[threadIdx.x][8 * q] = r.v[q];
+}
+
+template <int R>
+__global__ void __launch_bounds__(256) k_sur_fused(const DMech *__restrict__ m, SurNorm nm, int64_t n, int64_t count,
+                                                   const double *__restrict__ rho_a, const double *__restrict__ T_a,
+                                                   const double *__restrict__ Y_a, const __bf16 *__restrict__ wb,
+                                                   int64_t off1, int64_t off2, int64_t off3, double *__restrict__ dY,
+                                                   double *__restrict__ dT, int *__restrict__ fb_count,
+                                                   int *__restrict__ fb_idx, float *__restrict__ raw_out) {
+  static_assert(R % 64 == 0 && R <= 256, "rows per workgroup");
+  constexpr int RT = R / 16;
+  __shared__ FusedLds<R> L;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int lr = lane & 15, lk = 8 * (lane >> 4);
+  const int64_t r0 = (int64_t)blockIdx.x * R;
+
+  // features of the R cells into act[:, 0:32) (zeros beyond the 11 inputs)
+  Slice pre = load_slice(wb, kLd[0], 0, kLd[0]);
+  for (int t = tid; t < R; t += 256) {
+    const int64_t c = r0 + t;
+    double x[kSurIn];
+    if (c < count) {
+      double Y[kNS];
+      load_cell(n, c, Y_a, Y);
+      raw_features(m, rho_a[c], T_a[c], Y, x);
 #pragma unroll
-    for (int k = 0; k < kNS; ++k) {
-      double s = 0;
+      for (int i = 0; i < kSurIn; ++i) x[i] = (x[i] - nm.x_mean[i]) * nm.x_istd[i];
+    } else {
 #pragma unroll
-      for (int j = 0; j < kSurNull; ++j) s += nm.basis[k * kSurNull + j] * cj[j];
-      dy[k] = s;
-      amax = fmax(amax, fabs(s));
+      for (int i = 0; i < kSurIn; ++i) x[i] = 0;
     }
-    const double dTn = cj[6];
-    ok = ok && isfinite(amax) && isfinite(dTn) && amax <= nm.dy_max && fabs(dTn) <= nm.dT_max;
-    if (ok) {
-      const float xi = explicit_increment(m, rho, T, Y, nm.dt);
-      ok = xi <= (float)nm.src_max;  // false for NaN as well
+#pragma unroll
+    for (int i = 0; i < 32; ++i) L.act[t][i] = (__bf16)(i < kSurIn ? (float)x[i] : 0.f);
+  }
+  // the layer-0 weights carry the bias in column 16 and zeros in 11..15; the
+  // activations are zero there, so the slice contributes W x only
+  store_slice(L.w[0], pre);
+  __syncthreads();
+
+  f32x4 acc[RT][4];
+  for (int l = 0; l < 3; ++l) {
+    const __bf16 *Wc = l == 0 ? wb : (l == 1 ? wb + off1 : wb + off2);
+    const int ld = l == 0 ? kLd[0] : kHld, kp = l == 0 ? kKpad[0] : 256, nk = l == 0 ? 1 : 8;
+#pragma unroll
+    for (int a = 0; a < RT; ++a)
+#pragma unroll
+      for (int b = 0; b < 4; ++b) acc[a][b] = f32x4{0.f, 0.f, 0.f, 0.f};
+    for (int ks = 0; ks < nk; ++ks) {
+      const int buf = ks & 1;
+      Slice nxt;
+      if (ks + 1 < nk) nxt = load_slice(Wc, ld, 32 * (ks + 1), kp);
+      bf16x8 bfr[4];
+#pragma unroll
+      for (int b = 0; b < 4; ++b) bfr[b] = *(const bf16x8 *)&L.w[buf][wave * 64 + b * 16 + lr][lk];
+#pragma unroll
+      for (int a = 0; a < RT; ++a) {
+        const bf16x8 af = *(const bf16x8 *)&L.act[a * 16 + lr][32 * ks + lk];
+#pragma unroll
+        for (int b = 0; b < 4; ++b) acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af, bfr[b], acc[a][b], 0, 0, 0);
+      }
+      if (ks + 1 < nk) store_slice(L.w[buf ^ 1], nxt);
+      __syncthreads();
     }
-    double err = 0, neg = 0;
+    // first slice of the next hidden layer while the epilogue runs
+    if (l < 2) pre = load_slice(l == 0 ? wb + off1 : wb + off2, kHld, 0, 256);
+    float bias[4];
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      double s = 0;
+    for (int b = 0; b < 4; ++b) bias[b] = (float)Wc[(int64_t)(wave * 64 + b * 16 + lr) * ld + kp];
+    // every wave has finished reading act (barrier at the end of the K loop)
 #pragma unroll
-      for (int k = 0; k < kNS; ++k) s += nm.cons[r * kNS + k] * dy[k];
-      if (r < nm.n_cons) err = fmax(err, fabs(s));
-    }
+    for (int a = 0; a < RT; ++a)
 #pragma unroll
-    for (int k = 0; k < kNS; ++k) neg += fmax(0.0, -(Y[k] + dy[k])) + fmax(0.0, Y[k] + dy[k] - 1.0);
-    ok = ok && fmax(err, neg) <= nm.cons_tol;
-    if (ok) {
-      double mwi = 0, h[kNS];
+      for (int b = 0; b < 4; ++b)
 #pragma unroll
-      for (int l = 0; l < kNS; ++l) mwi += Y[l] / m->mw[l];
-      const double mw = 1 / mwi;
-      const double p_old = rho * T * (kRuD / mw);
-      d_enthalpy<kNS>(T, m, h);
-      double rhoE = 0;
+        for (int e = 0; e < 4; ++e)
+          L.act[a * 16 + 4 * (lane >> 4) + e][wave * 64 + b * 16 + lr] = (__bf16)gelu(acc[a][b][e] + bias[b]);
+    if (l < 2) store_slice(L.w[0], pre);
+    __syncthreads();
+  }
+
+  // output layer: 7 columns (one 16-wide tile, rows 7..15 of W3 are zero)
+  {
+    const __bf16 *W3 = wb + off3;
+    constexpr int RW = RT / 4;  // row tiles per wave
+    f32x4 o[RW];
 #pragma unroll
-      for (int l = 0; l < kNS; ++l) rhoE += h[l] * Y[l];
-      rhoE = rhoE * rho - p_old;
-      double Yn[kNS];
+    for (int a = 0; a < RW; ++a) o[a] = f32x4{0.f, 0.f, 0.f, 0.f};
+    for (int ks = 0; ks < 8; ++ks) {
+      const bf16x8 bfr = *(const bf16x8 *)&W3[(int64_t)lr * kLd[3] + 32 * ks + lk];
 #pragma unroll
-      for (int l = 0; l < kNS; ++l) Yn[l] = Y[l] + dy[l];
-      const double t0 = fmin(fmax(T + dTn, 0.5 * T), 2.0 * T + 1000.0);
-      double pn;
-      const double Tn = d_temperature<kNS>(m, rho, rhoE, 0.0, 0.0, 0.0, Yn, t0, pn);
-      ok = isfinite(Tn) && Tn > 0;
-      if (ok) {
-#pragma unroll
-        for (int l = 0; l < kNS; ++l) dY[(int64_t)l * n + c] = dy[l];
-        dT[c] = Tn - T;
+      for (int a = 0; a < RW; ++a) {
+        const bf16x8 af = *(const bf16x8 *)&L.act[(wave * RW + a) * 16 + lr][32 * ks + lk];
+        o[a] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af, bfr, o[a], 0, 0, 0);
       }
     }
+    const float b3 = lr < kSurOut ? (float)W3[(int64_t)lr * kLd[3] + kKpad[3]] : 0.f;
+#pragma unroll
+    for (int a = 0; a < RW; ++a)
+#pragma unroll
+      for (int e = 0; e < 4; ++e)
+        if (lr < 8) L.out[(wave * RW + a) * 16 + 4 * (lane >> 4) + e][lr] = o[a][e] + b3;
   }
-  // wave-aggregated append of the fallback cells
-  const bool fb = active && !ok;
-  const unsigned long long mask = __ballot(fb);
-  if (mask) {
-    const int lane = threadIdx.x & 63;
-    const int leader = __ffsll((long long)mask) - 1;
-    int base = 0;
-    if (lane == leader) base = atomicAdd(fb_count, (int)__popcll(mask));
-    base = __shfl(base, leader, 64);
-    if (fb) fb_idx[base + (int)__popcll(mask & ((1ull << lane) - 1ull))] = (int)c;
+  __syncthreads();
+
+  for (int t0 = 0; t0 < R; t0 += 256) {
+    const int t = t0 + tid;
+    const int64_t c = r0 + t;
+    const bool active = t < R && c < count;
+    if (raw_out) {
+      if (active)
+#pragma unroll
+        for (int j = 0; j < kSurOut; ++j) raw_out[c * kSurOut + j] = L.out[t][j];
+      continue;
+    }
+    bool ok = false;
+    if (active && !nm.force) ok = finish_cell(m, nm, n, c, rho_a, T_a, Y_a, L.out[t], dY, dT);
+    append_fallback(active && !ok, c, fb_count, fb_idx);
   }
 }
 
@@ -382,6 +549,7 @@ struct SurrogateState {
   __bf16 *iX = nullptr, *iH[2] = {};
   float *iOut = nullptr;
   int *fb_idx = nullptr, *fb_count = nullptr;
+  int kernel = 1;  // inference: 1 fused (128 rows), 2 fused (64 rows), 0 layer-by-layer GEMMs
   hipEvent_t ev[3] = {};
   double last_ms[2] = {0, 0};
   int64_t last_fallback = 0;
@@ -913,6 +1081,55 @@ int surrogate_train(SurrogateState *s, const DMech *m, int64_t n, const double *
   return 0;
 }
 
+static int launch_fused(SurrogateState *s, const DMech *m, const SurNorm &nm, int64_t n, const double *rho,
+                        const double *T, const double *Y, double *dY, double *dT, float *raw, hipStream_t st,
+                        std::string &err) {
+  if (s->kernel == 2) {
+    hipLaunchKernelGGL(k_sur_fused<64>, dim3((unsigned)((n + 63) / 64)), dim3(256), 0, st, m, nm, n, n, rho, T, Y,
+                       (const __bf16 *)s->wb, s->woff[1], s->woff[2], s->woff[3], dY, dT, s->fb_count, s->fb_idx, raw);
+  } else {
+    hipLaunchKernelGGL(k_sur_fused<128>, dim3((unsigned)((n + 127) / 128)), dim3(256), 0, st, m, nm, n, n, rho, T, Y,
+                       (const __bf16 *)s->wb, s->woff[1], s->woff[2], s->woff[3], dY, dT, s->fb_count, s->fb_idx, raw);
+  }
+  SCHK(hipGetLastError());
+  return 0;
+}
+
+void surrogate_set_kernel(SurrogateState *s, int kernel) { s->kernel = kernel; }
+
+int surrogate_network_outputs(SurrogateState *s, const DMech *m, int64_t n, const double *rho, const double *T,
+                              const double *Y, float *host_out, hipStream_t st, std::string &err) {
+  SCHK(hipSetDevice(s->device));
+  if (n <= 0) return 0;
+  if (n >= (int64_t)1 << 31) {
+    err = "batches are limited to 2^31 - 1 cells";
+    return 1;
+  }
+  float *raw = nullptr;
+  SCHK(dalloc(&raw, n * kSurOut));
+  if (s->kernel != 0) {
+    if (launch_fused(s, m, s->norm, n, rho, T, Y, nullptr, nullptr, raw, st, err)) return 1;
+  } else {
+    const int64_t chunk = std::min<int64_t>(n, kInferChunk);
+    if (ensure_infer_ws(s, (chunk + 7) / 8 * 8, 0, st, err)) return 1;
+    std::vector<float> tmp;
+    for (int64_t c0 = 0; c0 < n; c0 += chunk) {
+      const int64_t cnt = std::min(chunk, n - c0);
+      hipLaunchKernelGGL(k_sur_features, dim3(nb256(cnt)), dim3(256), 0, st, m, n, c0, cnt, rho, T, Y, s->norm,
+                         s->iX);
+      SCHK(hipGetLastError());
+      __bf16 *H[3] = {s->iH[0], s->iH[1], s->iH[0]};
+      if (forward(s, cnt, s->iX, H, nullptr, s->iOut, true, st, err)) return 1;
+      SCHK(hipMemcpy2DAsync(raw + c0 * kSurOut, kSurOut * sizeof(float), s->iOut, kYld * sizeof(float),
+                            kSurOut * sizeof(float), (size_t)cnt, hipMemcpyDeviceToDevice, st));
+    }
+  }
+  SCHK(hipMemcpyAsync(host_out, raw, (size_t)n * kSurOut * sizeof(float), hipMemcpyDeviceToHost, st));
+  SCHK(hipStreamSynchronize(st));
+  SCHK(hipFree(raw));
+  return 0;
+}
+
 int surrogate_infer(SurrogateState *s, const DMech *m, int64_t n, const double *rho, const double *T,
                     const double *Y, double dt, double *dY, double *dT, int64_t *n_fallback, hipStream_t st,
                     std::string &err) {
@@ -932,10 +1149,15 @@ int surrogate_infer(SurrogateState *s, const DMech *m, int64_t n, const double *
   // a network trained for another dt is not used: every cell falls back
   const int force = std::fabs(dt - s->dt_train) > 1e-12 * s->dt_train ? 1 : 0;
   const int64_t chunk = std::min<int64_t>(n, kInferChunk);
-  if (ensure_infer_ws(s, (chunk + 7) / 8 * 8, n, st, err)) return 1;
+  if (ensure_infer_ws(s, s->kernel == 0 ? (chunk + 7) / 8 * 8 : 0, n, st, err)) return 1;
   SCHK(hipEventRecord(s->ev[0], st));
   SCHK(hipMemsetAsync(s->fb_count, 0, sizeof(int), st));
-  for (int64_t c0 = 0; c0 < n; c0 += chunk) {
+  if (s->kernel != 0) {
+    SurNorm nm = s->norm;
+    nm.force = force;
+    if (launch_fused(s, m, nm, n, rho, T, Y, dY, dT, nullptr, st, err)) return 1;
+  }
+  for (int64_t c0 = 0; c0 < n && s->kernel == 0; c0 += chunk) {
     const int64_t cnt = std::min(chunk, n - c0);
     if (!force) {
       hipLaunchKernelGGL(k_sur_features, dim3(nb256(cnt)), dim3(256), 0, st, m, n, c0, cnt, rho, T, Y, s->norm,

@@ -92,6 +92,8 @@ def lib():
             "ostc_surrogate_save": ([vp, C.c_char_p], C.c_int),
             "ostc_surrogate_load": ([vp, C.c_char_p], C.c_int),
             "ostc_surrogate_last_stats": ([vp, dp], C.c_int),
+            "ostc_surrogate_set_kernel": ([vp, C.c_int], C.c_int),
+            "ostc_surrogate_network_outputs": ([vp, C.c_int64, vp, vp, vp, fp], C.c_int),
         }
         for name, (args, res) in sig.items():
             f = getattr(L, name)
@@ -342,6 +344,15 @@ class Solver:
 
     def surrogate_load(self, path):
         _chk(lib().ostc_surrogate_load(self.h, path.encode()))
+
+    def surrogate_set_kernel(self, kernel):
+        """1 fused (128 cells per workgroup), 2 fused (64), 0 one GEMM launch per layer."""
+        _chk(lib().ostc_surrogate_set_kernel(self.h, kernel))
+
+    def surrogate_network_outputs(self, n, rho, T, Y):
+        out = np.empty((n, 7), dtype=np.float32)
+        _chk(lib().ostc_surrogate_network_outputs(self.h, n, rho.ptr, T.ptr, Y.ptr, _fp(out)))
+        return out
 
     def surrogate_last_stats(self):
         o = np.zeros(3)

@@ -181,6 +181,86 @@ def c2_case(nx=256, ny=128, nz=128, seed=1002, steps=20, mach=2.5, t_inf=300.0, 
                 steps=steps)
 
 
+def c3_case(nx=512, ny=256, seed=1003, steps=20, dx=1e-4, delta=5e-4, viscous_order=2):
+    """2-D reacting H2/air supersonic mixing layer with the 9-species,
+    19-reaction H2-O2 mechanism (input_files/chemistry): upper stream air
+    (Y_O2 = 0.233, Y_N2 = 0.767) at 1500 K and Mach 2, lower stream
+    H2/N2 (0.3/0.7 by mass) at 300 K and Mach 1.2, both at 101325 Pa; a
+    tanh profile of thickness `delta` across the centre line plus 1 % seeded
+    velocity noise (see c3_fields).  HLLC with MUSCL-minmod (the reference's
+    AWENO interpolation has no mixture form), second-order viscous fluxes
+    with mixture-averaged transport, explicit finite-rate source inside
+    SSP-RK3 at CFL 0.5.  Inflow i=0 (air above the centre line, fuel below,
+    labels 5 and 51), outflow i=max, symmetry j=0 and j=max."""
+    xi = np.arange(nx) * dx
+    yj = np.arange(ny) * dx
+    X, Y = np.meshgrid(xi, yj, indexing="xy")
+    x = X[None, :, :].copy()
+    y = Y[None, :, :].copy()
+    z = np.zeros_like(x)
+    air = [("real", "O2", 0.233), ("real", "N2", 0.767)]
+    fuel = [("real", "H2", 0.3), ("real", "N2", 0.7)]
+    common = [("real", "pressure", 101325.0), ("real", "u", 1.0), ("real", "v", 0.0), ("real", "w", 0.0)]
+    setup = _setup_common({
+        "int dimension": 2, "int species": 1, "int reaction": 1, "int chemSrcMethod": 0, "real cfl": 0.5,
+        "int inviscid_scheme": 4, "int reconstruction": 2, "int limiter": 0, "int viscous_order": viscous_order,
+        "int total_step": steps,
+        "array string boundary_conditions": ["air_stream", "fuel_stream", "outflow", "symmetry"],
+        "string reference_state": "air_stream", "string default_init": "air_stream",
+        "struct air_stream": [("string", "type", "inflow"), ("int", "label", 5), ("real", "mach", 2.0),
+                              ("real", "temperature", 1500.0)] + common + air,
+        "struct fuel_stream": [("string", "type", "inflow"), ("int", "label", 51), ("real", "mach", 1.2),
+                               ("real", "temperature", 300.0)] + common + fuel,
+        "struct outflow": [("string", "type", "outflow"), ("int", "label", 6)],
+        "struct symmetry": [("string", "type", "symmetry"), ("int", "label", 3)],
+    })
+    half = ny // 2
+    bcs = [
+        dict(type=5, raw=(0, 0, half, ny - 1, 0, 0), label=5, stream="air_stream"),
+        dict(type=5, raw=(0, 0, 0, half - 1, 0, 0), label=51, stream="fuel_stream"),
+        dict(type=6, raw=(nx - 1, nx - 1, 0, ny - 1, 0, 0), label=6),
+        dict(type=3, raw=(0, nx - 1, 0, 0, 0, 0), label=3),
+        dict(type=3, raw=(0, nx - 1, ny - 1, ny - 1, 0, 0), label=3),
+    ]
+    scheme = dict(mix=1, reaction=1, chemSrcMethod=0, inviscid_scheme=4, reconstruction=2, limiter=0,
+                  viscous_order=viscous_order, cfl=0.5, fixed_time_step=0, output_screen=1)
+    return dict(name="c3", mx=nx, my=ny, mz=1, ngg=2, x=x, y=y, z=z, setup=setup, bcs=bcs, scheme=scheme,
+                seed=seed, steps=steps, dx=dx, delta=delta, y0=0.5 * (ny - 1) * dx)
+
+
+def c3_fields(case, states, mw):
+    """Initial bv (6, ...) and sv (9, ...) of the mixing layer: with
+    f = (1 + tanh((y - y0) / delta)) / 2, u, T and Y blend from the lower
+    (fuel) to the upper (air) stream state, p is uniform, rho follows from
+    the equation of state, and interior u is scaled by (1 + 0.01 N(0,1))
+    from the seeded generator.  `states` maps the two stream names to dicts
+    with rho, u, p, t and sv (the inflow states the solver derives from the
+    set-up structs); `mw` holds the species molecular weights."""
+    mx, my, mz, g = case["mx"], case["my"], case["mz"], case["ngg"]
+    shp = ghost_shape(mx, my, mz, g)
+    up, lo = states["air_stream"], states["fuel_stream"]
+    yc = (np.arange(my + 2 * g) - g) * case["dx"]
+    f = 0.5 * (1.0 + np.tanh((yc - case["y0"]) / case["delta"]))[None, :, None]
+    f = np.broadcast_to(f, shp)
+    bv = np.empty((6,) + shp)
+    bv[1] = lo["u"] + f * (up["u"] - lo["u"])
+    bv[2] = 0.0
+    bv[3] = 0.0
+    bv[4] = up["p"]
+    bv[5] = lo["t"] + f * (up["t"] - lo["t"])
+    ns = len(mw)
+    sv = np.empty((ns,) + shp)
+    for l in range(ns):
+        sv[l] = lo["sv"][l] + f * (up["sv"][l] - lo["sv"][l])
+    mwinv = sum(sv[l] / mw[l] for l in range(ns))
+    bv[0] = bv[4] / (R_U * bv[5] * mwinv)
+    rng = SplitMix64(case["seed"])
+    noise = 1.0 + 0.01 * rng.normal(mx * my * mz).reshape(mz, my, mx)
+    sl = (slice(g, g + mz), slice(g, g + my), slice(g, g + mx))
+    bv[1][sl] = bv[1][sl] * noise
+    return bv, sv
+
+
 def initial_fields(case):
     """Uniform inflow everywhere (ghosts included), then the seeded interior
     density perturbation at fixed pressure.  Returns bv (6, ...) and sv

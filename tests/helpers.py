@@ -11,7 +11,8 @@ def oracle_zone(case, mech=None, bv=None, sv=None):
     for b in case["bcs"]:
         d = dict(type=b["type"], raw=b["raw"])
         if b["type"] == 5:
-            d["state"] = dict(case["inflow"], sv=list(case["inflow"].get("sv", [])))
+            st = b.get("state", case.get("inflow"))
+            d["state"] = dict(st, sv=list(st.get("sv", [])))
         if b["type"] == 2:
             d["t_wall"] = b.get("t_wall", 300.0)
             d["isothermal"] = b.get("isothermal", True)
