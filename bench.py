@@ -33,8 +33,8 @@ HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
 BF16_MFMA_PEAK_TFLOPS = 2500.0
 
-# surrogate network FLOP per cell: 2 (11*256 + 256*256 + 256*256 + 256*7)
-SURROGATE_FLOP_PER_CELL = 2 * (11 * 256 + 2 * 256 * 256 + 256 * 7)
+# surrogate network FLOP per cell: 2 (11*256 + 256*256 + 256*256 + 256*8)
+SURROGATE_FLOP_PER_CELL = 2 * (11 * 256 + 2 * 256 * 256 + 256 * 8)
 
 # compulsory HBM bytes per cell of one j- (or k-) direction sweep launch:
 # reads cv(5) + bv(6) + mu + lambda + J + metric(9) = 23 doubles, read and

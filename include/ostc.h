@@ -147,14 +147,15 @@ int ostc_chem_batch(ostc_solver *s, int64_t n, const double *rho, const double *
                     double *dY, double *dT, int mode, int64_t *n_fallback);
 
 /* ---- MLP chemistry surrogate (no counterpart in the reference) ----
- * A 11 -> 256 -> 256 -> 256 -> 7 GELU network, BF16 MFMA with FP32
+ * A 11 -> 256 -> 256 -> 256 -> 8 GELU network, BF16 MFMA with FP32
  * accumulation, mapping (log(Y+1e-12) of the 9 species, T, log p) of a cell
  * to its point-implicit increment over one dt: the 6 coordinates of dY in an
  * orthonormal basis of the null space of the element/mass conservation rows,
- * and dT, each encoded as asinh(c / scale), standardised.  Parameters are FP32, one array; layer l is a row-major
+ * dT, and sigma_min(I - dt J) (the step's conditioning, for the gate), each
+ * encoded as asinh(c / scale), standardised.  Parameters are FP32, one array; layer l is a row-major
  * [out_pad x ld] matrix with the bias in column kpad:
  *   l=0: 256 x 24 (kpad 16, 11 inputs)   l=1,2: 256 x 264 (kpad 256)
- *   l=3: 16 x 264 (kpad 256, 7 outputs); everything else is zero.
+ *   l=3: 16 x 264 (kpad 256, 8 outputs); everything else is zero.
  * Network inputs/outputs of the forward/gradient hooks are standardised. */
 typedef struct {
   int64_t epochs, batch;          /* batch: multiple of 8 */
@@ -167,7 +168,7 @@ typedef struct {
   double cons_tol;                /* fallback above this element/mass conservation error (1e-6) */
   double dy_max, dT_max;          /* fallback above these |dY_k|, |dT| [K] predicted */
   double margin;                  /* input range widening, fraction of the training range */
-  double src_max;                 /* fallback above this explicit increment dt max|omega_k| / rho (FP32) */
+  double sig_min;                 /* fallback below this predicted sigma_min(I - dt J) */
 } ostc_surrogate_gate;
 
 /* Random initial parameters (uniform, variance 2/fan_in; 1/fan_in for the output layer). */
@@ -175,10 +176,10 @@ int ostc_surrogate_init(ostc_solver *s, uint64_t seed);
 int ostc_surrogate_param_count(ostc_solver *s, int64_t *n);
 int ostc_surrogate_get_params(ostc_solver *s, float *host);
 int ostc_surrogate_set_params(ostc_solver *s, const float *host);
-/* Forward pass of the training path (FP32 master weights): x [B x 11], y [B x 7], host arrays. */
+/* Forward pass of the training path (FP32 master weights): x [B x 11], y [B x 8], host arrays. */
 int ostc_surrogate_forward(ostc_solver *s, int64_t B, const float *x, float *y);
 /* One forward + backward pass of the training kernels on host arrays x [B x 11],
- * t [B x 7] (B multiple of 8): loss = mean (y - t)^2, grad = d loss / d params. */
+ * t [B x 8] (B multiple of 8): loss = mean (y - t)^2, grad = d loss / d params. */
 int ostc_surrogate_gradient(ostc_solver *s, int64_t B, const float *x, const float *t, float *grad, double *loss);
 /* Train on n device states (rho, T, Y component-major): targets from the FP64
  * point-implicit integrator with step dt, standardisation fitted to the set,
@@ -187,15 +188,15 @@ int ostc_surrogate_train(ostc_solver *s, int64_t n, const double *rho, const dou
                          const ostc_surrogate_train_cfg *cfg, double *loss_hist);
 int ostc_surrogate_set_gate(ostc_solver *s, const ostc_surrogate_gate *g);
 int ostc_surrogate_get_gate(ostc_solver *s, ostc_surrogate_gate *g);
-/* out[120]: x mean(11) std(11) min(11) max(11), u mean(7) std(7), dt, basis(9x6 row-major),
- * asinh scales(7); network output j decodes to c_j = scale_j sinh(mean_j + std_j out_j). */
+/* out[123]: x mean(11) std(11) min(11) max(11), u mean(8) std(8), dt, basis(9x6 row-major),
+ * asinh scales(8); network output j decodes to c_j = scale_j sinh(mean_j + std_j out_j). */
 int ostc_surrogate_norm(ostc_solver *s, double *out);
 int ostc_surrogate_save(ostc_solver *s, const char *path);
 int ostc_surrogate_load(ostc_solver *s, const char *path);
 /* Inference kernel of ostc_chem_batch mode 1: 1 = fused network + gate, 128 cells per
  * workgroup (default); 2 = fused, 64 cells; 0 = one GEMM launch per layer. */
 int ostc_surrogate_set_kernel(ostc_solver *s, int kernel);
-/* Standardised network outputs out[n x 7] (host) of n device states, without the gate. */
+/* Standardised network outputs out[n x 8] (host) of n device states, without the gate. */
 int ostc_surrogate_network_outputs(ostc_solver *s, int64_t n, const double *rho, const double *T, const double *Y,
                                    float *host_out);
 /* Last surrogate chem batch: out[3] = fallback cells, network+gate ms, fallback ms. */

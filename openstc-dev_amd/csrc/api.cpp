@@ -423,11 +423,11 @@ int ostc_init(const char *case_dir, int device, ostc_solver **out) {
   P.inviscid_type = p.get_int("inviscid_type", 0);
   P.gradP = p.get_bool("gradPInDiffusionFlux", false) ? 1 : 0;
   P.cfl = p.get_real("cfl", 1.0);
+  P.efix = p.get_real("entropy_fix_factor", 0.125);
   P.Pr = p.get_real("prandtl_number", 0.72);
   P.Sc = p.get_real("schmidt_number", 0.9);
   P.Prt = p.get_real("turbulent_prandtl_number", 0.9);
   P.Sct = p.get_real("turbulent_schmidt_number", 0.9);
-  if (P.inviscid_type == 2) return fail("the Roe scheme (inviscid_scheme = 2 without WENO) is not supported yet");
   if (P.inviscid_type == 1 && P.mix) return fail("AWENO for multi-species mixtures has no reference definition");
   if (P.mix && P.ns != 9) return fail("this build is compiled for 9-species mechanisms");
   if (device < 0) {  // host-only: setup files, tables and grid metrics, no device work
@@ -1059,14 +1059,14 @@ int ostc_surrogate_train(ostc_solver *s, int64_t n, const double *rho, const dou
 int ostc_surrogate_set_gate(ostc_solver *s, const ostc_surrogate_gate *g) {
   if (need_sur(s)) return 1;
   if (!g) return fail("null argument");
-  if (!(g->cons_tol >= 0) || !(g->dy_max >= 0) || !(g->dT_max >= 0) || !(g->margin >= 0) || !(g->src_max >= 0))
+  if (!(g->cons_tol >= 0) || !(g->dy_max >= 0) || !(g->dT_max >= 0) || !(g->margin >= 0) || !(g->sig_min >= 0))
     return fail("gate thresholds must be non-negative");
   SurrogateGate q;
   q.cons_tol = g->cons_tol;
   q.dy_max = g->dy_max;
   q.dT_max = g->dT_max;
   q.margin = g->margin;
-  q.src_max = g->src_max;
+  q.sig_min = g->sig_min;
   surrogate_set_gate(s->sur, q);
   return 0;
 }
@@ -1079,7 +1079,7 @@ int ostc_surrogate_get_gate(ostc_solver *s, ostc_surrogate_gate *g) {
   g->dy_max = q.dy_max;
   g->dT_max = q.dT_max;
   g->margin = q.margin;
-  g->src_max = q.src_max;
+  g->sig_min = q.sig_min;
   return 0;
 }
 

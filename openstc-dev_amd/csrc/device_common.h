@@ -34,6 +34,7 @@ struct DParams {
   int mix, reaction, chemSrcMethod;
   int inviscid_scheme, reconstruction, limiter, viscous, inviscid_type, gradP;
   double cfl, Pr, Sc, Prt, Sct;
+  double efix;  // Roe entropy-fix factor
 };
 
 // Raw device pointers of one block (DZone, Field.h:13-71).  bv/sv/cv/qn and

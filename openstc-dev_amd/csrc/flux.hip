@@ -383,6 +383,130 @@ __device__ inline void hllc(const DMech *__restrict__ m, const R *pl, const R *p
   for (int l = 5; l < NV; ++l) f[l] = Q * q[l];
 }
 
+// Roe.cu:60-87: entropy-fix delta of one point, eps (|U| + |V| (+|W|) +
+// a * mean |grad xi|); 2-D drops the third direction.
+template <typename R>
+__device__ inline R efix_point(const ZoneView &z, const DParams &P, int64_t p) {
+  R mm[9];
+#pragma unroll
+  for (int c = 0; c < 9; ++c) mm[c] = (R)z.metric[c * z.sz + p];
+  const R u = (R)z.bv[1 * z.sz + p], v = (R)z.bv[2 * z.sz + p], w = (R)z.bv[3 * z.sz + p];
+  const R U = fabs(u * mm[0] + v * mm[1] + w * mm[2]);
+  const R V = fabs(u * mm[3] + v * mm[4] + w * mm[5]);
+  const R W = fabs(u * mm[6] + v * mm[7] + w * mm[8]);
+  const R kx = sqrt(mm[0] * mm[0] + mm[1] * mm[1] + mm[2] * mm[2]);
+  const R ky = sqrt(mm[3] * mm[3] + mm[4] * mm[4] + mm[5] * mm[5]);
+  const R kz = sqrt(mm[6] * mm[6] + mm[7] * mm[7] + mm[8] * mm[8]);
+  const R a = (R)z.acoustic[p];
+  if (P.dim == 2) return (R)P.efix * (U + V + a * R(0.5) * (kx + ky));
+  return (R)P.efix * (U + V + W + a * (kx + ky + kz) / R(3));
+}
+
+// Roe.cu:186-375: half sum of the left and right fluxes, each with its own
+// J * metric, minus 1/2 |A| dQ from the characteristic decomposition at the
+// Roe average (species rows included; the mixture's gamma and c from the
+// Roe-averaged temperature, FP64), Harten's entropy fix with the face
+// average of the point deltas d0, d1.
+template <int NS, typename R>
+__device__ inline void roe(const DMech *__restrict__ m, const R *pl, const R *pr, const R *mk0, const R *mk1, R j0,
+                           R j1, R d0, R d1, R *f) {
+  constexpr int NV = 5 + NS;
+#pragma unroll
+  for (int l = 0; l < NV; ++l) f[l] = R(0);
+#pragma unroll
+  for (int sd = 0; sd < 2; ++sd) {
+    const R *q = sd ? pr : pl;
+    const R *mk = sd ? mk1 : mk0;
+    const R J = sd ? j1 : j0;
+    const R Jx = J * mk[0], Jy = J * mk[1], Jz = J * mk[2];
+    const R Uk = q[1] * Jx + q[2] * Jy + q[3] * Jz;
+    const R co = Uk * q[0];
+    f[0] += R(0.5) * co;
+    f[1] += R(0.5) * (co * q[1] + q[4] * Jx);
+    f[2] += R(0.5) * (co * q[2] + q[4] * Jy);
+    f[3] += R(0.5) * (co * q[3] + q[4] * Jz);
+    f[4] += R(0.5) * Uk * (q[4] + q[NV]);
+#pragma unroll
+    for (int l = 5; l < NV; ++l) f[l] += R(0.5) * co * q[l];
+  }
+  const R dl = sqrt(pl[0]), dr = sqrt(pr[0]), id = R(1) / (dl + dr);
+  const R u = (dl * pl[1] + dr * pr[1]) * id, v = (dl * pl[2] + dr * pr[2]) * id, w = (dl * pl[3] + dr * pr[3]) * id;
+  const R ek = R(0.5) * (u * u + v * v + w * w);
+  const R h = (dl * (pl[NV] + pl[4]) / pl[0] + dr * (pr[NV] + pr[4]) / pr[0]) * id;
+  R gamma = R(kGam), c, mw = R(kMwAirD);
+  R svm[NS > 0 ? NS : 1], hi[NS > 0 ? NS : 1];
+  if constexpr (NS == 0) {
+    c = sqrt((gamma - R(1)) * (h - ek));
+  } else {
+    double mwi = 0;
+#pragma unroll
+    for (int l = 0; l < NS; ++l) {
+      svm[l] = (dl * pl[l + 5] + dr * pr[l + 5]) * id;
+      mwi += (double)svm[l] / m->mw[l];
+    }
+    const double t = (double)((dl * (pl[4] / pl[0]) + dr * (pr[4] / pr[0])) * id) / (kRuD * mwi);
+    double hh[NS], cc[NS], cp = 0, cv = 0;
+    d_enthalpy_cp<NS>(t, m, hh, cc);
+#pragma unroll
+    for (int l = 0; l < NS; ++l) {
+      cp += cc[l] * (double)svm[l];
+      cv += (double)svm[l] * (cc[l] - kRuD / m->mw[l]);
+      hi[l] = (R)hh[l];
+    }
+    gamma = (R)(cp / cv);
+    c = (R)sqrt(cp / cv * kRuD * mwi * t);
+    mw = (R)(1.0 / mwi);
+  }
+  R kx = R(0.5) * (mk0[0] + mk1[0]), ky = R(0.5) * (mk0[1] + mk1[1]), kz = R(0.5) * (mk0[2] + mk1[2]);
+  const R gk = sqrt(kx * kx + ky * ky + kz * kz);
+  R Uk = kx * u + ky * v + kz * w;
+  R lam[3] = {Uk - gk * c, Uk, Uk + gk * c};
+  const R d = R(0.5) * (d0 + d1);
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    lam[a] = fabs(lam[a]);
+    if (lam[a] < d) lam[a] = R(0.5) * (lam[a] * lam[a] / d + d);
+  }
+  kx /= gk;
+  ky /= gk;
+  kz /= gk;
+  Uk /= gk;
+  const R ja = R(0.5) * (j0 + j1);
+  R dq[NV];
+  dq[0] = ja * (pr[0] - pl[0]);
+#pragma unroll
+  for (int l = 1; l < NV; ++l) dq[l] = ja * (pr[0] * pr[l] - pl[0] * pl[l]);
+  dq[4] = ja * (pr[NV] - pl[NV]);
+  const R g1 = gamma - R(1), ic2 = R(1) / (c * c);
+  R c1 = g1 * (ek * dq[0] - u * dq[1] - v * dq[2] - w * dq[3] + dq[4]) * ic2;
+  R c2 = (kx * dq[1] + ky * dq[2] + kz * dq[3] - Uk * dq[0]) / c;
+#pragma unroll
+  for (int l = 0; l < NS; ++l) c1 += (mw / (R)m->mw[l] - hi[l] * g1 * ic2) * dq[l + 5];
+  R c3 = dq[0] - c1;
+  R b[NV];
+  b[0] = -lam[0] * (R(0.5) * (c1 - c2));
+  b[1] = -lam[1] * (kx * c3 - ((kz * v - ky * w) * dq[0] - kz * dq[2] + ky * dq[3]) / c);
+  b[2] = -lam[1] * (ky * c3 - ((kx * w - kz * u) * dq[0] - kx * dq[3] + kz * dq[1]) / c);
+  b[3] = -lam[1] * (kz * c3 - ((ky * u - kx * v) * dq[0] - ky * dq[1] + kx * dq[2]) / c);
+  b[4] = -lam[2] * (R(0.5) * (c1 + c2));
+#pragma unroll
+  for (int l = 0; l < NS; ++l) b[l + 5] = -lam[1] * (dq[l + 5] - svm[l] * dq[0]);
+  const R c0 = kx * b[1] + ky * b[2] + kz * b[3];
+  c1 = c0 + b[0] + b[4];
+  c2 = c * (b[4] - b[0]);
+  c3 = R(0);
+#pragma unroll
+  for (int l = 0; l < NS; ++l) c3 += (hi[l] - mw / (R)m->mw[l] * c * c / g1) * b[l + 5];
+  f[0] += R(0.5) * c1;
+  f[1] += R(0.5) * (u * c1 + kx * c2 - c * (kz * b[2] - ky * b[3]));
+  f[2] += R(0.5) * (v * c1 + ky * c2 - c * (kx * b[3] - kz * b[1]));
+  f[3] += R(0.5) * (w * c1 + kz * c2 - c * (ky * b[1] - kx * b[2]));
+  f[4] += R(0.5) * (h * c1 + Uk * c2 - c * c * c0 / g1 +
+                    c * ((kz * v - ky * w) * b[1] + (kx * w - kz * u) * b[2] + (ky * u - kx * v) * b[3]) + c3);
+#pragma unroll
+  for (int l = 0; l < NS; ++l) f[5 + l] += R(0.5) * (b[l + 5] + svm[l] * c1);
+}
+
 // LaxFriedrichs.cu:14-49 (air only; the reference's generic version is empty)
 template <int NS, typename R>
 __device__ inline void lax_friedrichs(const R *pl, const R *pr, const R *mk0, const R *mk1, R j0, R j1, R *f) {
@@ -556,9 +680,17 @@ __device__ inline void face_flux(const ZoneView &z, const DParams &P, const DMec
   } else {
     recon_pv<NS, R>(z, P, m, p0, st, pl, pr);
   }
+  // inviscid_scheme 2 with AWENO (inviscid_type 1) runs AUSM+, as the
+  // reference's AWENO switch has no Roe case (InviscidScheme.cu:290-303)
   switch (P.inviscid_scheme) {
     case 1: lax_friedrichs<NS, R>(pl, pr, mk0, mk1, j0, j1, F); break;
     case 4: hllc<NS, R>(m, pl, pr, mk0, mk1, j0, j1, F); break;
+    case 2:
+      if constexpr (SCHEME == 0) {
+        roe<NS, R>(m, pl, pr, mk0, mk1, j0, j1, efix_point<R>(z, P, p0), efix_point<R>(z, P, p0 + st), F);
+        break;
+      }
+      [[fallthrough]];
     default: ausm_plus<NS, R>(pl, pr, mk0, mk1, j0, j1, F); break;
   }
   if (P.viscous) {

@@ -47,6 +47,9 @@ hipError_t launch_residual(const ZoneView &z, const StepScratch &w, int slot, in
 hipError_t launch_copy_dt(const double *src, double *dst, hipStream_t s);
 hipError_t launch_chem_batch(const DMech *m, int64_t n, const double *rho, const double *T, const double *Y,
                              double dt, double *dY, double *dT, hipStream_t s);
+// smallest singular value of I - dt J per cell (conditioning of the point-implicit step)
+hipError_t launch_epi_sigma(const DMech *m, int64_t n, const double *rho, const double *T, const double *Y, double dt,
+                            double *sig, hipStream_t s);
 // the same for the cells idx[0..count) of arrays of stride n (idx values < n)
 hipError_t launch_chem_batch_subset(const DMech *m, int64_t n, int64_t count, const int *idx, const double *rho,
                                     const double *T, const double *Y, double dt, double *dY, double *dT,

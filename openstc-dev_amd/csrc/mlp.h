@@ -10,12 +10,16 @@ namespace ostc {
 typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8;
 typedef __attribute__((__vector_size__(4 * sizeof(float)))) float f32x4;
 
-// GELU with the error function
-__device__ inline float gelu(float x) { return 0.5f * x * (1.0f + erff(x * 0.70710678118654752f)); }
+// GELU, tanh form: x (1 + tanh u) / 2 = x / (1 + exp(-2u)),
+// u = sqrt(2/pi) (x + 0.044715 x^3); and its derivative
+__device__ inline float gelu(float x) {
+  const float u2 = 1.5957691216057308f * (x + 0.044715f * x * x * x);
+  return x / (1.0f + __expf(-u2));
+}
 __device__ inline float gelu_grad(float x) {
-  const float cdf = 0.5f * (1.0f + erff(x * 0.70710678118654752f));
-  const float pdf = 0.39894228040143268f * expf(-0.5f * x * x);
-  return cdf + x * pdf;
+  const float u2 = 1.5957691216057308f * (x + 0.044715f * x * x * x);
+  const float s = 1.0f / (1.0f + __expf(-u2));
+  return s + x * s * (1.0f - s) * 1.5957691216057308f * (1.0f + 0.134145f * x * x);
 }
 
 enum GemmEpilogue {

@@ -1,15 +1,16 @@
 // MLP chemistry surrogate (no counterpart in the reference): a 3x256 GELU
-// network mapping (log Y, T, log p) of a cell to its point-implicit
+// (tanh form) network mapping (log Y, T, log p) of a cell to its point-implicit
 // increments (dY, dT) over one dt, trained and run with BF16 MFMA and FP32
 // accumulation, with a per-cell fallback to the FP64 integrator.
 //
-// Network: 11 -> 256 -> 256 -> 256 -> 7.  The 7 outputs encode the 6
+// Network: 11 -> 256 -> 256 -> 256 -> 8.  The 8 outputs encode the 6
 // coordinates c_j of dY in an orthonormal basis of the null space of the
 // element/mass conservation rows (so dY conserves elements and mass to FP64
-// rounding by construction) and dT, each as u_j = asinh(c_j / s_j) with s_j
-// the median |c_j| of the training set (the point-implicit increments span
-// many decades).  Inputs and u are standardised with the mean and standard
-// deviation of the training set.
+// rounding by construction), dT, and sigma, the smallest singular value of
+// the point-implicit matrix I - dt J (the step's conditioning, which the gate
+// uses), each as u_j = asinh(c_j / s_j) with s_j the median |c_j| of the
+// training set (the point-implicit increments span many decades).  Inputs and
+// u are standardised with the mean and standard deviation of the training set.
 //
 // Parameter layout (FP32, one contiguous array): layer l is a row-major
 // [out_pad_l x ld_l] matrix W_l with the bias in column kpad_l
@@ -18,7 +19,7 @@
 //   0  11   16    24   256  256
 //   1  256  256   264  256  256
 //   2  256  256   264  256  256
-//   3  256  256   264  7    16
+//   3  256  256   264  8    16
 // Entries outside (out x in) and the bias column are zero.
 #pragma once
 
@@ -47,11 +48,14 @@ struct SurrogateGate {
   // at dt = 1e-7 s) go to the FP64 integrator
   double dy_max = 3e-3;
   double dT_max = 50.0;
-  double src_max = 1e-2;   // largest FP32 explicit increment dt max_k |omega_k| / rho accepted
   double margin = 0.01;    // input range widening, as a fraction of the training range
+  // smallest predicted sigma_min(I - dt J) accepted: near-singular cells, where
+  // the point-implicit step is close to a pole and not smooth, stay with the
+  // integrator
+  double sig_min = 0.08;
 };
 
-constexpr int kSurIn = 11, kSurOut = 7, kSurNull = 6, kSurLayers = 4;
+constexpr int kSurIn = 11, kSurOut = 8, kSurNull = 6, kSurLayers = 4;
 
 // create for a 9-species mechanism (element tables give the conservation rows)
 int surrogate_create(const Mechanism &mech, int device, SurrogateState **out, std::string &err);
@@ -61,11 +65,11 @@ int64_t surrogate_param_count(const SurrogateState *s);
 int surrogate_init_params(SurrogateState *s, uint64_t seed, hipStream_t st, std::string &err);
 int surrogate_get_params(SurrogateState *s, float *host, hipStream_t st, std::string &err);
 int surrogate_set_params(SurrogateState *s, const float *host, hipStream_t st, std::string &err);
-// one forward+backward pass on standardised inputs x [B x 11] and targets t [B x 7]
+// one forward+backward pass on standardised inputs x [B x 11] and targets t [B x 8]
 // (host arrays); grad gets d loss / d params in the parameter layout
 int surrogate_gradient(SurrogateState *s, int64_t B, const float *x, const float *t, float *grad, double *loss,
                        hipStream_t st, std::string &err);
-// standardised forward pass of the training path (FP32 master weights): y [B x 7]
+// standardised forward pass of the training path (FP32 master weights): y [B x 8]
 int surrogate_forward(SurrogateState *s, int64_t B, const float *x, float *y, hipStream_t st, std::string &err);
 // generate targets with the FP64 integrator on the device states, fit the
 // standardisation and train; loss_hist[e] = mean training loss of epoch e
@@ -76,11 +80,11 @@ void surrogate_set_gate(SurrogateState *s, const SurrogateGate &g);
 SurrogateGate surrogate_get_gate(const SurrogateState *s);
 int surrogate_save(SurrogateState *s, const std::string &path, hipStream_t st, std::string &err);
 int surrogate_load(SurrogateState *s, const std::string &path, hipStream_t st, std::string &err);
-// standardisation: x mean/std/min/max (4 x 11), u mean/std (2 x 7), dt, basis (54), asinh scales (7)
-int surrogate_get_norm(const SurrogateState *s, double *out120);
+// standardisation: x mean/std/min/max (4 x 11), u mean/std (2 x 8), dt, basis (54), asinh scales (8)
+int surrogate_get_norm(const SurrogateState *s, double *out123);
 // inference kernel: 1 fused (128 cells per workgroup, default), 2 fused (64), 0 layer-by-layer GEMMs
 void surrogate_set_kernel(SurrogateState *s, int kernel);
-// raw (standardised) network outputs [n x 7] of device states, copied to the host
+// raw (standardised) network outputs [n x 8] of device states, copied to the host
 int surrogate_network_outputs(SurrogateState *s, const DMech *m, int64_t n, const double *rho, const double *T,
                               const double *Y, float *host_out, hipStream_t st, std::string &err);
 // fallback count, network+gate ms and fallback ms of the last inference

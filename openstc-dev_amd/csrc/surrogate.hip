@@ -40,8 +40,7 @@ struct SurNorm {
   double basis[kNS * kSurNull];  // dY_k = sum_j basis[k * 6 + j] c_j
   double cons[4 * kNS];          // element mass-fraction rows, then the mass row
   int n_cons;
-  int force;  // 1: every cell falls back
-  double cons_tol, dy_max, dT_max, src_max, dt;
+  double cons_tol, dy_max, dT_max, sig_min, dt;
 };
 
 // (log(Y + 1e-12), T, log p) of one cell; p from the equation of state as
@@ -63,91 +62,12 @@ __device__ inline void load_cell(int64_t n, int64_t c, const double *__restrict_
   for (int l = 0; l < kNS; ++l) Y[l] = Y_a[(int64_t)l * n + c];
 }
 
-// FP32 explicit chemical increment of a cell, max_k |dt omega_k / rho|: the
-// stiffness indicator of the gate.  The finite-rate formulas of the FP64
-// integrator (FiniteRateChem.cu:65-194) evaluated in log form so that no
-// FP32 intermediate overflows: ln kf = ln A + b ln T - Ea / T, third body,
-// Lindemann / Troe fall-off, ln kb = ln kf - ln Kc, ln q = ln k + sum nu ln c.
-__device__ inline float explicit_increment(const DMech *__restrict__ m, double rho_d, double T_d, const double *Y,
-                                           double dt) {
-  const float t = (float)T_d, rho = (float)rho_d;
-  const float lt = __logf(t), ti = 1.f / t;
-  float lc[kNS], c[kNS], g[kNS];
-#pragma unroll
-  for (int l = 0; l < kNS; ++l) {
-    c[l] = (float)(rho_d * Y[l] / m->mw[l] * 1e-3);
-    lc[l] = __logf(c[l]);  // -inf for an absent species
-    const float tt = fmaxf(t, (float)m->t_low[l]);
-    const double *a = tt < (float)m->t_mid[l] ? &m->lo[l * 7] : &m->hi[l * 7];
-    const float tt2 = tt * tt;
-    g[l] = (float)a[0] * (1.f - __logf(tt)) - 0.5f * (float)a[1] * tt - (float)a[2] * tt2 * (1.f / 6.f) -
-           (float)a[3] * tt2 * tt * (1.f / 12.f) - (float)a[4] * tt2 * tt2 * 0.05f + (float)a[5] / tt - (float)a[6];
-  }
-  const float ltemp = __logf((float)(101325.0 / kRuD * 1e-3)) - lt;
-  float w[kNS];
-#pragma unroll
-  for (int l = 0; l < kNS; ++l) w[l] = 0.f;
-  for (int r = 0; r < m->n_reac; ++r) {
-    const int type = m->label[r];
-    float lk = __logf((float)m->A[r]) + (float)m->b[r] * lt - (float)m->Ea[r] * ti;
-    if (type == 3) {
-      const float lk2 = __logf((float)m->A2[r]) + (float)m->b2[r] * lt - (float)m->Ea2[r] * ti;
-      const float hi = fmaxf(lk, lk2);
-      lk = hi + __logf(1.f + __expf(fminf(lk, lk2) - hi));
-    }
-    if (type > 3) {
-      float cc = 0.f;
-#pragma unroll
-      for (int l = 0; l < kNS; ++l) cc += c[l] * (float)m->third_body_coeff[r * kNS + l];
-      if (type == 4) {
-        lk += __logf(cc);
-      } else {
-        const float llow = __logf((float)m->A2[r]) + (float)m->b2[r] * lt - (float)m->Ea2[r] * ti;
-        const float lpr_e = llow + __logf(cc) - lk;  // ln Pr
-        float lF = 0.f;
-        if (type > 5) {
-          const float al = (float)m->troe_alpha[r];
-          float fc = (1.f - al) * __expf(-t / (float)m->troe_t3[r]) + al * __expf(-t / (float)m->troe_t1[r]);
-          if (type == 7) fc += __expf(-(float)m->troe_t2[r] * ti);
-          const float lfc = __log10f(fc);
-          const float c2 = -0.4f - 0.67f * lfc, nn = 0.75f - 1.27f * lfc;
-          const float lpr = lpr_e * 0.43429448f;
-          const float tp = (lpr + c2) / (nn - 0.14f * (lpr + c2));
-          lF = lfc / (1.f + tp * tp) * 2.30258509f;
-        }
-        // k = khigh * Pr / (1 + Pr) * F
-        lk = lk + lpr_e - (lpr_e > 0.f ? lpr_e + __logf(1.f + __expf(-lpr_e)) : __logf(1.f + __expf(lpr_e))) + lF;
-      }
-    }
-    float l1 = lk, l2;
-    float dg = 0.f, lq2 = 0.f;
-#pragma unroll
-    for (int l = 0; l < kNS; ++l) {
-      const int nf = m->stoi_f[r * kNS + l], nb = m->stoi_b[r * kNS + l];
-      if (nf) l1 += (float)nf * lc[l];
-      if (nb) lq2 += (float)nb * lc[l];
-      dg += g[l] * (float)(nb - nf);
-    }
-    float q = __expf(l1);
-    if (type == 2) {
-      q -= __expf(__logf((float)m->A2[r]) + (float)m->b2[r] * lt - (float)m->Ea2[r] * ti + lq2);
-    } else if (type != 0) {
-      l2 = lk - (float)m->order[r] * ltemp + dg + lq2;
-      q -= __expf(l2);
-    }
-#pragma unroll
-    for (int l = 0; l < kNS; ++l) w[l] += (float)(m->stoi_b[r * kNS + l] - m->stoi_f[r * kNS + l]) * q;
-  }
-  float mx = 0.f;
-#pragma unroll
-  for (int l = 0; l < kNS; ++l) mx = fmaxf(mx, fabsf(w[l] * (float)(m->mw[l] * 1e3)));
-  return (float)dt * mx / rho;
-}
-
-// raw features F [n x 16] and raw targets Tt [n x 16]: null-space coordinates of dY and dT
+// raw features F [n x 16] and raw targets Tt [n x 16]: null-space coordinates
+// of dY, dT and sigma_min(I - dt J)
 __global__ void k_sur_raw(const DMech *__restrict__ m, int64_t n, const double *__restrict__ rho,
                           const double *__restrict__ T, const double *__restrict__ Y_a, const double *__restrict__ dY,
-                          const double *__restrict__ dT, SurNorm nm, float *__restrict__ F, float *__restrict__ Tt) {
+                          const double *__restrict__ dT, const double *__restrict__ sig, SurNorm nm,
+                          float *__restrict__ F, float *__restrict__ Tt) {
   const int64_t c = (int64_t)blockIdx.x * 256 + threadIdx.x;
   if (c >= n) return;
   double Y[kNS], x[kSurIn], d[kNS];
@@ -164,6 +84,7 @@ __global__ void k_sur_raw(const DMech *__restrict__ m, int64_t n, const double *
     Tt[c * kRawLd + j] = (float)s;
   }
   Tt[c * kRawLd + 6] = (float)dT[c];
+  Tt[c * kRawLd + 7] = (float)sig[c];
 #pragma unroll
   for (int i = kSurOut; i < kRawLd; ++i) Tt[c * kRawLd + i] = 0.f;
 }
@@ -255,13 +176,13 @@ __global__ void k_sur_features(const DMech *__restrict__ m, int64_t n, int64_t c
 // Decode the network output of cells c0 .. c0+count, apply the gate and
 // either write (dY, dT) or append the cell to the fallback list.
 //   gate: inputs inside the training range, finite outputs, |dY| and |dT|
-//   within the accepted activity, FP32 explicit increment dt |omega| / rho
-//   within src_max (stiff cells, where the point-implicit step itself is
-//   far from linear, stay with the integrator), element / mass conservation error of the
+//   within the accepted activity, predicted sigma_min(I - dt J) >= sig_min
+//   (near-singular cells, where the point-implicit step is close to a pole,
+//   stay with the integrator), element / mass conservation error of the
 //   written state (conservation rows of dY plus the mass that mass fractions
 //   outside [0, 1] would need clipped) <= cons_tol, converged temperature.
 // The outputs are u_j = asinh(c_j / scale_j) standardised; c_j are the
-// null-space coordinates of dY (j < 6) and dT (j = 6).
+// null-space coordinates of dY (j < 6), dT (j = 6) and sigma_min (j = 7).
 // T is the Newton temperature at the unchanged internal energy started
 // from T + dT_net, as in the point-implicit batch kernel.
 // Decode one cell's network output o[0..6], apply the gate and, if the cell
@@ -292,10 +213,7 @@ __device__ inline bool finish_cell(const DMech *__restrict__ m, const SurNorm &n
   }
   const double dTn = cj[6];
   ok = ok && isfinite(amax) && isfinite(dTn) && amax <= nm.dy_max && fabs(dTn) <= nm.dT_max;
-  if (ok) {
-    const float xi = explicit_increment(m, rho, T, Y, nm.dt);
-    ok = xi <= (float)nm.src_max;  // false for NaN as well
-  }
+  ok = ok && cj[7] >= nm.sig_min;  // false for NaN as well
   double err = 0, neg = 0;
 #pragma unroll
   for (int r = 0; r < 4; ++r) {
@@ -350,69 +268,48 @@ __device__ inline void append_fallback(bool fb, int64_t c, int *__restrict__ fb_
 __global__ void __launch_bounds__(256) k_sur_finish(const DMech *__restrict__ m, SurNorm nm, int64_t n, int64_t c0,
                                                     int64_t count, const double *__restrict__ rho_a,
                                                     const double *__restrict__ T_a, const double *__restrict__ Y_a,
-                                                    const float *__restrict__ out, int force_fallback,
+                                                    const float *__restrict__ out, int ld, int force_fallback,
                                                     double *__restrict__ dY, double *__restrict__ dT,
                                                     int *__restrict__ fb_count, int *__restrict__ fb_idx) {
   const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
   const bool active = t < count;
   const int64_t c = c0 + t;
   bool ok = false;
-  if (active && !force_fallback) ok = finish_cell(m, nm, n, c, rho_a, T_a, Y_a, out + t * kYld, dY, dT);
+  if (active && !force_fallback) ok = finish_cell(m, nm, n, c, rho_a, T_a, Y_a, out + t * ld, dY, dT);
   append_fallback(active && !ok, c, fb_count, fb_idx);
 }
 
-// ------------------------------------------------------------ fused inference
-// The whole network of R cells in one workgroup of 4 wavefronts:
-// features -> 4 layers (activations resident in LDS, weight K-slices of 32
-// double-buffered through LDS) -> gate, Newton T and fallback list.
-// Hidden layers: wave w computes output columns [64 w, 64 w + 64) for all R
-// rows (R/16 x 4 tiles of v_mfma_f32_16x16x32_bf16, FP32 accumulation);
-// the output layer (7 columns in one 16-wide tile) splits the rows over the
-// waves.  Same BF16 roundings as the layer-by-layer path: BF16 operands,
-// FP32 accumulation, the bias taken from the BF16 weight copy, added in FP32.
-constexpr int kLdw = 40;  // LDS row of one weight slice: 32 k + 8 pad (BF16)
-
-template <int R>
-struct FusedLds {
-  __bf16 act[R][kHld];
-  __bf16 w[2][256][kLdw];
-  float out[R][8];
-};
-
-struct Slice {
-  uint4 v[4];
-};
-
-// row n = tid of W (leading dimension ld), k in [k0, k0 + 32); zero at k >= kmax
-__device__ inline Slice load_slice(const __bf16 *__restrict__ W, int ld, int k0, int kmax) {
-  Slice r;
-  const __bf16 *p = W + (int64_t)threadIdx.x * ld + k0;
-#pragma unroll
-  for (int q = 0; q < 4; ++q) r.v[q] = (k0 + 8 * q < kmax) ? *(const uint4 *)(p + 8 * q) : make_uint4(0, 0, 0, 0);
-  return r;
-}
-
-__device__ inline void store_slice(__bf16 (*w)[kLdw], const Slice &r) {
-#pragma unroll
-  for (int q = 0; q < 4; ++q) *(uint4 *)&w[threadIdx.x][8 * q] = r.v[q];
-}
-
-template <int R>
-__global__ void __launch_bounds__(256) k_sur_fused(const DMech *__restrict__ m, SurNorm nm, int64_t n, int64_t count,
-                                                   const double *__restrict__ rho_a, const double *__restrict__ T_a,
-                                                   const double *__restrict__ Y_a, const __bf16 *__restrict__ wb,
-                                                   int64_t off1, int64_t off2, int64_t off3, double *__restrict__ dY,
-                                                   double *__restrict__ dT, int *__restrict__ fb_count,
-                                                   int *__restrict__ fb_idx, float *__restrict__ raw_out) {
+// ------------------------------------------------------------ fused network
+// The whole network of R cells in one workgroup of 4 wavefronts: features ->
+// 3 hidden layers -> output layer, activations resident in LDS, weights read
+// as MFMA B fragments straight from L2 (every workgroup reads the same 280 kB
+// of BF16 weights) with a two-K-step register prefetch, so the K loop has no
+// barrier.  Hidden layers: wave w computes output columns [64 w, 64 w + 64)
+// for all R rows (R/16 x 4 tiles of v_mfma_f32_16x16x32_bf16, FP32
+// accumulation); the output layer (8 columns, one 16-wide tile) splits the
+// rows over the waves.  Same roundings as the layer-by-layer path: BF16
+// operands, FP32 accumulation, the bias from the BF16 weight copy added in
+// FP32.  Writes the standardised outputs out[c * 8 + j]; the gate runs in
+// k_sur_finish.  LDS: R x 264 BF16 (66 kB for R = 128), two workgroups per CU.
+template <int R, int PF>
+__global__ void __launch_bounds__(256, 2) k_sur_net(const DMech *__restrict__ m, SurNorm nm, int64_t n,
+                                                    int64_t count, const double *__restrict__ rho_a,
+                                                    const double *__restrict__ T_a, const double *__restrict__ Y_a,
+                                                    const __bf16 *__restrict__ wb, int64_t off1, int64_t off2,
+                                                    int64_t off3, float *__restrict__ out) {
   static_assert(R % 64 == 0 && R <= 256, "rows per workgroup");
   constexpr int RT = R / 16;
-  __shared__ FusedLds<R> L;
+  __shared__ __bf16 act[R][kHld];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int lr = lane & 15, lk = 8 * (lane >> 4);
   const int64_t r0 = (int64_t)blockIdx.x * R;
+  const bf16x8 zero8 = {};
 
-  // features of the R cells into act[:, 0:32) (zeros beyond the 11 inputs)
-  Slice pre = load_slice(wb, kLd[0], 0, kLd[0]);
+  // layer-0 B fragments (K = 24: 11 weights, zeros, the bias column, zeros)
+  bf16x8 b0[4];
+#pragma unroll
+  for (int b = 0; b < 4; ++b)
+    b0[b] = lk < kLd[0] ? *(const bf16x8 *)(wb + (int64_t)(wave * 64 + b * 16 + lr) * kLd[0] + lk) : zero8;
   for (int t = tid; t < R; t += 256) {
     const int64_t c = r0 + t;
     double x[kSurIn];
@@ -426,92 +323,95 @@ __global__ void __launch_bounds__(256) k_sur_fused(const DMech *__restrict__ m, 
 #pragma unroll
       for (int i = 0; i < kSurIn; ++i) x[i] = 0;
     }
+    // zeros beyond the 11 inputs: the bias column of W0 meets a zero here
 #pragma unroll
-    for (int i = 0; i < 32; ++i) L.act[t][i] = (__bf16)(i < kSurIn ? (float)x[i] : 0.f);
+    for (int i = 0; i < 32; ++i) act[t][i] = (__bf16)(i < kSurIn ? (float)x[i] : 0.f);
   }
-  // the layer-0 weights carry the bias in column 16 and zeros in 11..15; the
-  // activations are zero there, so the slice contributes W x only
-  store_slice(L.w[0], pre);
   __syncthreads();
 
   f32x4 acc[RT][4];
-  for (int l = 0; l < 3; ++l) {
-    const __bf16 *Wc = l == 0 ? wb : (l == 1 ? wb + off1 : wb + off2);
-    const int ld = l == 0 ? kLd[0] : kHld, kp = l == 0 ? kKpad[0] : 256, nk = l == 0 ? 1 : 8;
-#pragma unroll
-    for (int a = 0; a < RT; ++a)
-#pragma unroll
-      for (int b = 0; b < 4; ++b) acc[a][b] = f32x4{0.f, 0.f, 0.f, 0.f};
-    for (int ks = 0; ks < nk; ++ks) {
-      const int buf = ks & 1;
-      Slice nxt;
-      if (ks + 1 < nk) nxt = load_slice(Wc, ld, 32 * (ks + 1), kp);
-      bf16x8 bfr[4];
-#pragma unroll
-      for (int b = 0; b < 4; ++b) bfr[b] = *(const bf16x8 *)&L.w[buf][wave * 64 + b * 16 + lr][lk];
-#pragma unroll
-      for (int a = 0; a < RT; ++a) {
-        const bf16x8 af = *(const bf16x8 *)&L.act[a * 16 + lr][32 * ks + lk];
-#pragma unroll
-        for (int b = 0; b < 4; ++b) acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af, bfr[b], acc[a][b], 0, 0, 0);
-      }
-      if (ks + 1 < nk) store_slice(L.w[buf ^ 1], nxt);
-      __syncthreads();
-    }
-    // first slice of the next hidden layer while the epilogue runs
-    if (l < 2) pre = load_slice(l == 0 ? wb + off1 : wb + off2, kHld, 0, 256);
+  auto epilogue = [&](const __bf16 *W, int ld, int kp) {
     float bias[4];
 #pragma unroll
-    for (int b = 0; b < 4; ++b) bias[b] = (float)Wc[(int64_t)(wave * 64 + b * 16 + lr) * ld + kp];
-    // every wave has finished reading act (barrier at the end of the K loop)
+    for (int b = 0; b < 4; ++b) bias[b] = (float)W[(int64_t)(wave * 64 + b * 16 + lr) * ld + kp];
+    __syncthreads();  // every wave has read its last A fragments of act
 #pragma unroll
     for (int a = 0; a < RT; ++a)
 #pragma unroll
       for (int b = 0; b < 4; ++b)
 #pragma unroll
         for (int e = 0; e < 4; ++e)
-          L.act[a * 16 + 4 * (lane >> 4) + e][wave * 64 + b * 16 + lr] = (__bf16)gelu(acc[a][b][e] + bias[b]);
-    if (l < 2) store_slice(L.w[0], pre);
+          act[a * 16 + 4 * (lane >> 4) + e][wave * 64 + b * 16 + lr] = (__bf16)gelu(acc[a][b][e] + bias[b]);
     __syncthreads();
+  };
+
+  // layer 0: one K step
+#pragma unroll
+  for (int a = 0; a < RT; ++a) {
+    const bf16x8 af = *(const bf16x8 *)&act[a * 16 + lr][lk];
+#pragma unroll
+    for (int b = 0; b < 4; ++b)
+      acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af, b0[b], f32x4{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
+  }
+  epilogue(wb, kLd[0], kKpad[0]);
+
+  // hidden layers 1, 2: 8 K steps of 32, B fragments prefetched PF steps ahead
+  for (int l = 1; l < 3; ++l) {
+    const __bf16 *W = l == 1 ? wb + off1 : wb + off2;
+    const __bf16 *wrow[4];
+#pragma unroll
+    for (int b = 0; b < 4; ++b) wrow[b] = W + (int64_t)(wave * 64 + b * 16 + lr) * kHld + lk;
+    bf16x8 bq[PF + 1][4];
+#pragma unroll
+    for (int q = 0; q < PF; ++q)
+#pragma unroll
+      for (int b = 0; b < 4; ++b) bq[q][b] = *(const bf16x8 *)(wrow[b] + 32 * q);
+#pragma unroll
+    for (int a = 0; a < RT; ++a)
+#pragma unroll
+      for (int b = 0; b < 4; ++b) acc[a][b] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int ks = 0; ks < 8; ++ks) {
+      if (ks + PF < 8)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) bq[(ks + PF) % (PF + 1)][b] = *(const bf16x8 *)(wrow[b] + 32 * (ks + PF));
+#pragma unroll
+      for (int a = 0; a < RT; ++a) {
+        const bf16x8 af = *(const bf16x8 *)&act[a * 16 + lr][32 * ks + lk];
+#pragma unroll
+        for (int b = 0; b < 4; ++b)
+          acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af, bq[ks % (PF + 1)][b], acc[a][b], 0, 0, 0);
+      }
+    }
+    epilogue(W, kHld, 256);
   }
 
-  // output layer: 7 columns (one 16-wide tile, rows 7..15 of W3 are zero)
+  // output layer: 8 columns (rows 8..15 of W3 are zero)
   {
     const __bf16 *W3 = wb + off3;
-    constexpr int RW = RT / 4;  // row tiles per wave
+    constexpr int RW = RT / 4;
     f32x4 o[RW];
 #pragma unroll
     for (int a = 0; a < RW; ++a) o[a] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
     for (int ks = 0; ks < 8; ++ks) {
       const bf16x8 bfr = *(const bf16x8 *)&W3[(int64_t)lr * kLd[3] + 32 * ks + lk];
 #pragma unroll
       for (int a = 0; a < RW; ++a) {
-        const bf16x8 af = *(const bf16x8 *)&L.act[(wave * RW + a) * 16 + lr][32 * ks + lk];
+        const bf16x8 af = *(const bf16x8 *)&act[(wave * RW + a) * 16 + lr][32 * ks + lk];
         o[a] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af, bfr, o[a], 0, 0, 0);
       }
     }
-    const float b3 = lr < kSurOut ? (float)W3[(int64_t)lr * kLd[3] + kKpad[3]] : 0.f;
+    if (lr < kSurOut) {
+      const float b3 = (float)W3[(int64_t)lr * kLd[3] + kKpad[3]];
 #pragma unroll
-    for (int a = 0; a < RW; ++a)
+      for (int a = 0; a < RW; ++a)
 #pragma unroll
-      for (int e = 0; e < 4; ++e)
-        if (lr < 8) L.out[(wave * RW + a) * 16 + 4 * (lane >> 4) + e][lr] = o[a][e] + b3;
-  }
-  __syncthreads();
-
-  for (int t0 = 0; t0 < R; t0 += 256) {
-    const int t = t0 + tid;
-    const int64_t c = r0 + t;
-    const bool active = t < R && c < count;
-    if (raw_out) {
-      if (active)
-#pragma unroll
-        for (int j = 0; j < kSurOut; ++j) raw_out[c * kSurOut + j] = L.out[t][j];
-      continue;
+        for (int e = 0; e < 4; ++e) {
+          const int64_t c = r0 + (wave * RW + a) * 16 + 4 * (lane >> 4) + e;
+          if (c < count) out[c * kSurOut + lr] = o[a][e] + b3;
+        }
     }
-    bool ok = false;
-    if (active && !nm.force) ok = finish_cell(m, nm, n, c, rho_a, T_a, Y_a, L.out[t], dY, dT);
-    append_fallback(active && !ok, c, fb_count, fb_idx);
   }
 }
 
@@ -549,7 +449,8 @@ struct SurrogateState {
   __bf16 *iX = nullptr, *iH[2] = {};
   float *iOut = nullptr;
   int *fb_idx = nullptr, *fb_count = nullptr;
-  int kernel = 1;  // inference: 1 fused (128 rows), 2 fused (64 rows), 0 layer-by-layer GEMMs
+  float *fb_out = nullptr;  // network outputs [fcap x 8] of the fused path
+  int kernel = 1;  // inference: 1 fused network (128 cells per workgroup), 2 (64), 0 layer-by-layer GEMMs
   hipEvent_t ev[3] = {};
   double last_ms[2] = {0, 0};
   int64_t last_fallback = 0;
@@ -577,7 +478,7 @@ struct SurrogateState {
     free_train();
     free_infer();
     for (void *p : {(void *)w, (void *)g, (void *)m1, (void *)m2, (void *)wb, (void *)aug, (void *)loss,
-                    (void *)fb_idx, (void *)fb_count})
+                    (void *)fb_idx, (void *)fb_count, (void *)fb_out})
       (void)hipFree(p);
     for (auto &e : ev)
       if (e) (void)hipEventDestroy(e);
@@ -632,8 +533,11 @@ int ensure_infer_ws(SurrogateState *s, int64_t chunk, int64_t n, hipStream_t st,
   }
   if (n > s->fcap) {
     (void)hipFree(s->fb_idx);
+    (void)hipFree(s->fb_out);
     s->fb_idx = nullptr;
+    s->fb_out = nullptr;
     SCHK(dalloc(&s->fb_idx, n));
+    SCHK(dalloc(&s->fb_out, n * kSurOut));
     s->fcap = n;
   }
   return 0;
@@ -724,7 +628,7 @@ void finish_norm(SurrogateState *s) {
   s->norm.cons_tol = s->gate.cons_tol;
   s->norm.dy_max = s->gate.dy_max;
   s->norm.dT_max = s->gate.dT_max;
-  s->norm.src_max = s->gate.src_max;
+  s->norm.sig_min = s->gate.sig_min;
   s->norm.dt = s->dt_train;
 }
 
@@ -846,11 +750,11 @@ int surrogate_get_norm(const SurrogateState *s, double *o) {
   }
   for (int j = 0; j < kSurOut; ++j) {
     o[44 + j] = s->norm.c_mean[j];
-    o[51 + j] = s->norm.c_std[j];
+    o[52 + j] = s->norm.c_std[j];
   }
-  o[58] = s->dt_train;
-  for (int k = 0; k < kNS * kSurNull; ++k) o[59 + k] = s->norm.basis[k];
-  for (int j = 0; j < kSurOut; ++j) o[113 + j] = s->norm.c_scale[j];
+  o[60] = s->dt_train;
+  for (int k = 0; k < kNS * kSurNull; ++k) o[61 + k] = s->norm.basis[k];
+  for (int j = 0; j < kSurOut; ++j) o[115 + j] = s->norm.c_scale[j];
   return 0;
 }
 
@@ -965,15 +869,19 @@ int surrogate_train(SurrogateState *s, const DMech *m, int64_t n, const double *
   SCHK(dalloc(&dY, n * kNS));
   SCHK(dalloc(&dTt, n));
   SCHK(launch_chem_batch(m, n, rho, T, Y, dt, dY, dTt, st));
+  double *sig = nullptr;
+  SCHK(dalloc(&sig, n));
+  SCHK(launch_epi_sigma(m, n, rho, T, Y, dt, sig, st));
   // 2. raw features / targets and their statistics
   float *F = nullptr, *Tt = nullptr;
   SCHK(dalloc(&F, n * kRawLd));
   SCHK(dalloc(&Tt, n * kRawLd));
   hipLaunchKernelGGL(k_sur_raw, dim3(nb256(n)), dim3(256), 0, st, m, n, rho, T, Y, (const double *)dY,
-                     (const double *)dTt, s->norm, F, Tt);
+                     (const double *)dTt, (const double *)sig, s->norm, F, Tt);
   SCHK(hipGetLastError());
   SCHK(hipFree(dY));
   SCHK(hipFree(dTt));
+  SCHK(hipFree(sig));
   // asinh scale of each target: the median |c_j| of an evenly strided sample
   {
     const int64_t ns = std::min<int64_t>(n, 65536), stride = n / ns;
@@ -1081,16 +989,15 @@ int surrogate_train(SurrogateState *s, const DMech *m, int64_t n, const double *
   return 0;
 }
 
-static int launch_fused(SurrogateState *s, const DMech *m, const SurNorm &nm, int64_t n, const double *rho,
-                        const double *T, const double *Y, double *dY, double *dT, float *raw, hipStream_t st,
-                        std::string &err) {
-  if (s->kernel == 2) {
-    hipLaunchKernelGGL(k_sur_fused<64>, dim3((unsigned)((n + 63) / 64)), dim3(256), 0, st, m, nm, n, n, rho, T, Y,
-                       (const __bf16 *)s->wb, s->woff[1], s->woff[2], s->woff[3], dY, dT, s->fb_count, s->fb_idx, raw);
-  } else {
-    hipLaunchKernelGGL(k_sur_fused<128>, dim3((unsigned)((n + 127) / 128)), dim3(256), 0, st, m, nm, n, n, rho, T, Y,
-                       (const __bf16 *)s->wb, s->woff[1], s->woff[2], s->woff[3], dY, dT, s->fb_count, s->fb_idx, raw);
-  }
+// fused network of n cells into out [n x 8]
+static int launch_net(SurrogateState *s, const DMech *m, int64_t n, const double *rho, const double *T,
+                      const double *Y, float *out, hipStream_t st, std::string &err) {
+  if (s->kernel == 2)
+    hipLaunchKernelGGL((k_sur_net<64, 2>), dim3((unsigned)((n + 63) / 64)), dim3(256), 0, st, m, s->norm, n, n, rho, T, Y,
+                       (const __bf16 *)s->wb, s->woff[1], s->woff[2], s->woff[3], out);
+  else
+    hipLaunchKernelGGL((k_sur_net<128, 1>), dim3((unsigned)((n + 127) / 128)), dim3(256), 0, st, m, s->norm, n, n, rho,
+                       T, Y, (const __bf16 *)s->wb, s->woff[1], s->woff[2], s->woff[3], out);
   SCHK(hipGetLastError());
   return 0;
 }
@@ -1108,7 +1015,7 @@ int surrogate_network_outputs(SurrogateState *s, const DMech *m, int64_t n, cons
   float *raw = nullptr;
   SCHK(dalloc(&raw, n * kSurOut));
   if (s->kernel != 0) {
-    if (launch_fused(s, m, s->norm, n, rho, T, Y, nullptr, nullptr, raw, st, err)) return 1;
+    if (launch_net(s, m, n, rho, T, Y, raw, st, err)) return 1;
   } else {
     const int64_t chunk = std::min<int64_t>(n, kInferChunk);
     if (ensure_infer_ws(s, (chunk + 7) / 8 * 8, 0, st, err)) return 1;
@@ -1153,9 +1060,10 @@ int surrogate_infer(SurrogateState *s, const DMech *m, int64_t n, const double *
   SCHK(hipEventRecord(s->ev[0], st));
   SCHK(hipMemsetAsync(s->fb_count, 0, sizeof(int), st));
   if (s->kernel != 0) {
-    SurNorm nm = s->norm;
-    nm.force = force;
-    if (launch_fused(s, m, nm, n, rho, T, Y, dY, dT, nullptr, st, err)) return 1;
+    if (!force && launch_net(s, m, n, rho, T, Y, s->fb_out, st, err)) return 1;
+    hipLaunchKernelGGL(k_sur_finish, dim3(nb256(n)), dim3(256), 0, st, m, s->norm, n, (int64_t)0, n, rho, T, Y,
+                       (const float *)s->fb_out, kSurOut, force, dY, dT, s->fb_count, s->fb_idx);
+    SCHK(hipGetLastError());
   }
   for (int64_t c0 = 0; c0 < n && s->kernel == 0; c0 += chunk) {
     const int64_t cnt = std::min(chunk, n - c0);
@@ -1167,7 +1075,7 @@ int surrogate_infer(SurrogateState *s, const DMech *m, int64_t n, const double *
       if (forward(s, cnt, s->iX, H, nullptr, s->iOut, true, st, err)) return 1;
     }
     hipLaunchKernelGGL(k_sur_finish, dim3(nb256(cnt)), dim3(256), 0, st, m, s->norm, n, c0, cnt, rho, T, Y,
-                       (const float *)s->iOut, force, dY, dT, s->fb_count, s->fb_idx);
+                       (const float *)s->iOut, kYld, force, dY, dT, s->fb_count, s->fb_idx);
     SCHK(hipGetLastError());
   }
   SCHK(hipEventRecord(s->ev[1], st));
@@ -1193,7 +1101,7 @@ int surrogate_infer(SurrogateState *s, const DMech *m, int64_t n, const double *
 
 // ---------------------------------------------------------------- file IO
 namespace {
-const char kMagic[8] = {'O', 'S', 'T', 'C', 'S', 'U', 'R', '2'};
+const char kMagic[8] = {'O', 'S', 'T', 'C', 'S', 'U', 'R', '3'};
 }
 
 int surrogate_save(SurrogateState *s, const std::string &path, hipStream_t st, std::string &err) {

@@ -554,6 +554,65 @@ __global__ void __launch_bounds__(128) k_chem_batch(const DMech *__restrict__ m,
   dT[c] = Tn - T;
 }
 
+// Smallest singular value of the point-implicit matrix I - dt J of each cell
+// (J the exact-point-implicit Jacobian of the chemical source at the cell's
+// state), by one-sided Jacobi rotations of its columns in FP64: the
+// conditioning of the point-implicit step, used as a training target.
+template <int NS>
+__global__ void __launch_bounds__(128) k_epi_sigma(const DMech *__restrict__ m, int64_t n,
+                                                   const double *__restrict__ rho_a, const double *__restrict__ T_a,
+                                                   const double *__restrict__ Y_a, double dt,
+                                                   double *__restrict__ sig) {
+  const int64_t c = (int64_t)blockIdx.x * 128 + threadIdx.x;
+  if (c >= n) return;
+  double Y[NS], om[NS], A[NS * NS];
+#pragma unroll
+  for (int l = 0; l < NS; ++l) Y[l] = Y_a[(int64_t)l * n + c];
+  d_chem_source<NS>(m, rho_a[c], T_a[c], Y, om, A);
+#pragma unroll
+  for (int a = 0; a < NS; ++a)
+#pragma unroll
+    for (int b = 0; b < NS; ++b) A[a * NS + b] = (a == b ? 1.0 : 0.0) - dt * A[a * NS + b];
+  for (int sweep = 0; sweep < 12; ++sweep) {
+    double off = 0;
+#pragma unroll
+    for (int p = 0; p < NS - 1; ++p)
+#pragma unroll
+      for (int q = p + 1; q < NS; ++q) {
+        double al = 0, be = 0, ga = 0;
+#pragma unroll
+        for (int k = 0; k < NS; ++k) {
+          al += A[k * NS + p] * A[k * NS + p];
+          be += A[k * NS + q] * A[k * NS + q];
+          ga += A[k * NS + p] * A[k * NS + q];
+        }
+        const double rel = fabs(ga) / sqrt(al * be + 1e-300);
+        off = fmax(off, rel);
+        if (rel > 1e-15) {
+          const double zeta = (be - al) / (2 * ga);
+          const double t = (zeta >= 0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1 + zeta * zeta));
+          const double cs = 1 / sqrt(1 + t * t), sn = cs * t;
+#pragma unroll
+          for (int k = 0; k < NS; ++k) {
+            const double x = A[k * NS + p], y = A[k * NS + q];
+            A[k * NS + p] = cs * x - sn * y;
+            A[k * NS + q] = sn * x + cs * y;
+          }
+        }
+      }
+    if (off < 1e-14) break;
+  }
+  double mn = 1e300;
+#pragma unroll
+  for (int p = 0; p < NS; ++p) {
+    double s2 = 0;
+#pragma unroll
+    for (int k = 0; k < NS; ++k) s2 += A[k * NS + p] * A[k * NS + p];
+    mn = fmin(mn, s2);
+  }
+  sig[c] = sqrt(mn);
+}
+
 inline unsigned nblk(int64_t n, int tb) { return (unsigned)((n + tb - 1) / tb); }
 
 }  // namespace
@@ -673,6 +732,13 @@ hipError_t launch_chem_batch(const DMech *m, int64_t n, const double *rho, const
   if (n <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_chem_batch<9>, dim3(nblk(n, 128)), dim3(128), 0, s, m, n, n, (const int *)nullptr, rho, T, Y,
                      dt, dY, dT);
+  return hipGetLastError();
+}
+
+hipError_t launch_epi_sigma(const DMech *m, int64_t n, const double *rho, const double *T, const double *Y, double dt,
+                            double *sig, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_epi_sigma<9>, dim3(nblk(n, 128)), dim3(128), 0, s, m, n, rho, T, Y, dt, sig);
   return hipGetLastError();
 }
 

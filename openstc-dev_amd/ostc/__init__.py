@@ -31,7 +31,7 @@ class TrainCfg(C.Structure):
 class Gate(C.Structure):
     """ostc_surrogate_gate"""
     _fields_ = [("cons_tol", C.c_double), ("dy_max", C.c_double), ("dT_max", C.c_double), ("margin", C.c_double),
-                ("src_max", C.c_double)]
+                ("sig_min", C.c_double)]
 
 
 def lib():
@@ -304,7 +304,7 @@ class Solver:
 
     def surrogate_forward(self, x):
         x = np.ascontiguousarray(x, dtype=np.float32)
-        y = np.empty((x.shape[0], 7), dtype=np.float32)
+        y = np.empty((x.shape[0], 8), dtype=np.float32)
         _chk(lib().ostc_surrogate_forward(self.h, x.shape[0], _fp(x), _fp(y)))
         return y
 
@@ -334,10 +334,10 @@ class Solver:
         return {k: getattr(g, k) for k, _ in Gate._fields_}
 
     def surrogate_norm(self):
-        o = np.zeros(120)
+        o = np.zeros(123)
         _chk(lib().ostc_surrogate_norm(self.h, _dp(o)))
-        return dict(x_mean=o[0:11], x_std=o[11:22], x_min=o[22:33], x_max=o[33:44], u_mean=o[44:51],
-                    u_std=o[51:58], dt=o[58], basis=o[59:113].reshape(9, 6), c_scale=o[113:120])
+        return dict(x_mean=o[0:11], x_std=o[11:22], x_min=o[22:33], x_max=o[33:44], u_mean=o[44:52],
+                    u_std=o[52:60], dt=o[60], basis=o[61:115].reshape(9, 6), c_scale=o[115:123])
 
     def surrogate_save(self, path):
         _chk(lib().ostc_surrogate_save(self.h, path.encode()))
@@ -350,7 +350,7 @@ class Solver:
         _chk(lib().ostc_surrogate_set_kernel(self.h, kernel))
 
     def surrogate_network_outputs(self, n, rho, T, Y):
-        out = np.empty((n, 7), dtype=np.float32)
+        out = np.empty((n, 8), dtype=np.float32)
         _chk(lib().ostc_surrogate_network_outputs(self.h, n, rho.ptr, T.ptr, Y.ptr, _fp(out)))
         return out
 

@@ -89,7 +89,7 @@ def _setup_common(extra):
     return base
 
 
-def c1_case(nx=200, ny=100, seed=1001, steps=100):
+def c1_case(nx=200, ny=100, seed=1001, steps=100, inviscid_scheme=3):
     """2-D inviscid air, 200x100 Cartesian (spacing 1 mm), AUSM+ with
     MUSCL-minmod, SSP-RK3 at CFL 0.5.  Mach 2 supersonic inflow (101325 Pa,
     300 K) at i=0, supersonic outflow at i=max, symmetry at j=0 and j=max.
@@ -104,8 +104,8 @@ def c1_case(nx=200, ny=100, seed=1001, steps=100):
     z = np.zeros_like(x)
     inflow = air_inflow(2.0, 101325.0, 300.0)
     setup = _setup_common({
-        "int dimension": 2, "real cfl": 0.5, "int inviscid_scheme": 3, "int reconstruction": 2, "int limiter": 0,
-        "int viscous_order": 0, "int total_step": steps,
+        "int dimension": 2, "real cfl": 0.5, "int inviscid_scheme": inviscid_scheme, "int reconstruction": 2,
+        "int limiter": 0, "int viscous_order": 0, "int total_step": steps,
         "array string boundary_conditions": ["freestream", "outflow", "symmetry"],
         "string reference_state": "freestream", "string default_init": "freestream",
         "struct freestream": [("string", "type", "inflow"), ("int", "label", 5), ("real", "mach", 2.0),
@@ -120,8 +120,8 @@ def c1_case(nx=200, ny=100, seed=1001, steps=100):
         dict(type=3, raw=(0, nx - 1, 0, 0, 0, 0), label=3),
         dict(type=3, raw=(0, nx - 1, ny - 1, ny - 1, 0, 0), label=3),
     ]
-    scheme = dict(mix=0, inviscid_scheme=3, reconstruction=2, limiter=0, viscous_order=0, cfl=0.5,
-                  fixed_time_step=0, output_screen=1)
+    scheme = dict(mix=0, inviscid_scheme=inviscid_scheme, reconstruction=2, limiter=0, viscous_order=0, cfl=0.5,
+                  fixed_time_step=0, output_screen=1, entropy_fix_factor=0.125)
     return dict(name="c1", mx=nx, my=ny, mz=1, ngg=2, x=x, y=y, z=z, setup=setup, bcs=bcs, scheme=scheme,
                 inflow=inflow, seed=seed, perturb="density", steps=steps)
 
@@ -181,7 +181,7 @@ def c2_case(nx=256, ny=128, nz=128, seed=1002, steps=20, mach=2.5, t_inf=300.0, 
                 steps=steps)
 
 
-def c3_case(nx=512, ny=256, seed=1003, steps=20, dx=1e-4, delta=5e-4, viscous_order=2):
+def c3_case(nx=512, ny=256, seed=1003, steps=20, dx=1e-4, delta=5e-4, viscous_order=2, inviscid_scheme=4):
     """2-D reacting H2/air supersonic mixing layer with the 9-species,
     19-reaction H2-O2 mechanism (input_files/chemistry): upper stream air
     (Y_O2 = 0.233, Y_N2 = 0.767) at 1500 K and Mach 2, lower stream
@@ -203,7 +203,8 @@ def c3_case(nx=512, ny=256, seed=1003, steps=20, dx=1e-4, delta=5e-4, viscous_or
     common = [("real", "pressure", 101325.0), ("real", "u", 1.0), ("real", "v", 0.0), ("real", "w", 0.0)]
     setup = _setup_common({
         "int dimension": 2, "int species": 1, "int reaction": 1, "int chemSrcMethod": 0, "real cfl": 0.5,
-        "int inviscid_scheme": 4, "int reconstruction": 2, "int limiter": 0, "int viscous_order": viscous_order,
+        "int inviscid_scheme": inviscid_scheme, "int reconstruction": 2, "int limiter": 0,
+        "int viscous_order": viscous_order,
         "int total_step": steps,
         "array string boundary_conditions": ["air_stream", "fuel_stream", "outflow", "symmetry"],
         "string reference_state": "air_stream", "string default_init": "air_stream",
@@ -222,8 +223,8 @@ def c3_case(nx=512, ny=256, seed=1003, steps=20, dx=1e-4, delta=5e-4, viscous_or
         dict(type=3, raw=(0, nx - 1, 0, 0, 0, 0), label=3),
         dict(type=3, raw=(0, nx - 1, ny - 1, ny - 1, 0, 0), label=3),
     ]
-    scheme = dict(mix=1, reaction=1, chemSrcMethod=0, inviscid_scheme=4, reconstruction=2, limiter=0,
-                  viscous_order=viscous_order, cfl=0.5, fixed_time_step=0, output_screen=1)
+    scheme = dict(mix=1, reaction=1, chemSrcMethod=0, inviscid_scheme=inviscid_scheme, reconstruction=2, limiter=0,
+                  viscous_order=viscous_order, cfl=0.5, fixed_time_step=0, output_screen=1, entropy_fix_factor=0.125)
     return dict(name="c3", mx=nx, my=ny, mz=1, ngg=2, x=x, y=y, z=z, setup=setup, bcs=bcs, scheme=scheme,
                 seed=seed, steps=steps, dx=dx, delta=delta, y0=0.5 * (ny - 1) * dx)
 

@@ -863,10 +863,130 @@ static void flux_lf(const orc_zone *zz, const double *pl, const double *pr, cons
   f[4] = ((pl[nv] + pl[4]) * Ukl + (pr[nv] + pr[4]) * Ukr - sr * (pr[nv] - pl[nv])) * hj;
 }
 
+/* Roe.cu:60-87 compute_entropy_fix_delta at one point: eps (|U| + |V| (+|W|)
+ * + a * mean |grad xi|), 2-D without the third direction. */
+static double efix_delta(const orc_zone *zz, int i, int j, int k) {
+  const int64_t p = at(zz->Lg, i, j, k);
+  const double *m = &zz->metric[p * 9];
+  const double u = BV(1, i, j, k), v = BV(2, i, j, k), w = BV(3, i, j, k);
+  const double U = fabs(u * m[0] + v * m[1] + w * m[2]);
+  const double V = fabs(u * m[3] + v * m[4] + w * m[5]);
+  const double W = fabs(u * m[6] + v * m[7] + w * m[8]);
+  const double kx = sqrt(m[0] * m[0] + m[1] * m[1] + m[2] * m[2]);
+  const double ky = sqrt(m[3] * m[3] + m[4] * m[4] + m[5] * m[5]);
+  const double kz = sqrt(m[6] * m[6] + m[7] * m[7] + m[8] * m[8]);
+  const double a = zz->acoustic[p];
+  if (zz->mz == 1) return zz->cfg.entropy_fix_factor * (U + V + a * 0.5 * (kx + ky));
+  return zz->cfg.entropy_fix_factor * (U + V + W + a * (kx + ky + kz) / 3.0);
+}
+
+/* Roe.cu:186-375: the half sum of the left and right fluxes, each with its
+ * own J * metric, minus 1/2 |A| dQ from the characteristic decomposition at
+ * the Roe average (species rows included; the mixture's gamma and c from the
+ * Roe-averaged temperature), with Harten's entropy fix on the face average
+ * of the per-point delta.  parity pinned by: consistency F(q, q), the Roe
+ * property on a uniform grid (supersonic face flux = F(q_L)). */
+static void flux_roe(const orc_zone *zz, const double *pl, const double *pr, const double *met, const double *jac,
+                     const double *efd, int ip, double *f) {
+  const int nv = zz->nv, ns = zz->ns;
+  for (int l = 0; l < nv; ++l) f[l] = 0;
+  for (int sd = 0; sd < 2; ++sd) {
+    const double *q = sd ? pr : pl;
+    const int pt = ip + sd;
+    const double Jx = jac[pt] * met[pt * 3], Jy = jac[pt] * met[pt * 3 + 1], Jz = jac[pt] * met[pt * 3 + 2];
+    const double Uk = q[1] * Jx + q[2] * Jy + q[3] * Jz;
+    const double co = Uk * q[0];
+    f[0] += 0.5 * co;
+    f[1] += 0.5 * (co * q[1] + q[4] * Jx);
+    f[2] += 0.5 * (co * q[2] + q[4] * Jy);
+    f[3] += 0.5 * (co * q[3] + q[4] * Jz);
+    f[4] += 0.5 * Uk * (q[4] + q[nv]);
+    for (int l = 5; l < nv; ++l) f[l] += 0.5 * co * q[l];
+  }
+  const double dl = sqrt(pl[0]), dr = sqrt(pr[0]), id = 1.0 / (dl + dr);
+  const double u = (dl * pl[1] + dr * pr[1]) * id, v = (dl * pl[2] + dr * pr[2]) * id,
+               w = (dl * pl[3] + dr * pr[3]) * id;
+  const double ek = 0.5 * (u * u + v * v + w * w);
+  const double h = (dl * (pl[nv] + pl[4]) / pl[0] + dr * (pr[nv] + pr[4]) / pr[0]) * id;
+  double gamma = GAMMA_AIR, c = sqrt((gamma - 1) * (h - ek)), mw = MW_AIR;
+  double svm[ORC_MAXS + 4] = {0}, hi[ORC_MAXS] = {0};
+  for (int l = 0; l < nv - 5; ++l) svm[l] = (dl * pl[l + 5] + dr * pr[l + 5]) * id;
+  if (zz->cfg.mix) {
+    const orc_mech *m = &zz->mech;
+    double mwi = 0;
+    for (int l = 0; l < ns; ++l) mwi += svm[l] / m->mw[l];
+    const double t = (dl * (pl[4] / pl[0]) + dr * (pr[4] / pr[0])) * id / (R_U * mwi);
+    double cpi[ORC_MAXS];
+    orc_enthalpy_and_cp(t, m, hi, cpi);
+    double cp = 0, cv = 0;
+    for (int l = 0; l < ns; ++l) {
+      cp += cpi[l] * svm[l];
+      cv += svm[l] * (cpi[l] - R_U / m->mw[l]);
+    }
+    gamma = cp / cv;
+    c = sqrt(gamma * R_U * mwi * t);
+    mw = 1.0 / mwi;
+  }
+  double kx = 0.5 * (met[ip * 3] + met[(ip + 1) * 3]);
+  double ky = 0.5 * (met[ip * 3 + 1] + met[(ip + 1) * 3 + 1]);
+  double kz = 0.5 * (met[ip * 3 + 2] + met[(ip + 1) * 3 + 2]);
+  const double gk = sqrt(kx * kx + ky * ky + kz * kz);
+  double Uk = kx * u + ky * v + kz * w;
+  double lam[3] = {Uk - gk * c, Uk, Uk + gk * c};
+  const double d = 0.5 * (efd[ip] + efd[ip + 1]);
+  for (int a = 0; a < 3; ++a) {
+    lam[a] = fabs(lam[a]);
+    if (lam[a] < d) lam[a] = 0.5 * (lam[a] * lam[a] / d + d);
+  }
+  kx /= gk;
+  ky /= gk;
+  kz /= gk;
+  Uk /= gk;
+  const double ja = 0.5 * (jac[ip] + jac[ip + 1]);
+  double dq[5 + ORC_MAXS + 4] = {0};
+  dq[0] = ja * (pr[0] - pl[0]);
+  for (int l = 1; l < nv; ++l) dq[l] = ja * (pr[0] * pr[l] - pl[0] * pl[l]);
+  dq[4] = ja * (pr[nv] - pl[nv]);
+  double c1 = (gamma - 1) * (ek * dq[0] - u * dq[1] - v * dq[2] - w * dq[3] + dq[4]) / (c * c);
+  double c2 = (kx * dq[1] + ky * dq[2] + kz * dq[3] - Uk * dq[0]) / c;
+  for (int l = 0; l < ns; ++l) c1 += (mw / zz->mech.mw[l] - hi[l] * (gamma - 1) / (c * c)) * dq[l + 5];
+  double c3 = dq[0] - c1;
+  double L[5 + ORC_MAXS + 4] = {0};
+  L[0] = 0.5 * (c1 - c2);
+  L[1] = kx * c3 - ((kz * v - ky * w) * dq[0] - kz * dq[2] + ky * dq[3]) / c;
+  L[2] = ky * c3 - ((kx * w - kz * u) * dq[0] - kx * dq[3] + kz * dq[1]) / c;
+  L[3] = kz * c3 - ((ky * u - kx * v) * dq[0] - ky * dq[1] + kx * dq[2]) / c;
+  L[4] = 0.5 * (c1 + c2);
+  for (int l = 0; l < nv - 5; ++l) L[l + 5] = dq[l + 5] - svm[l] * dq[0];
+  double b[5 + ORC_MAXS + 4];
+  b[0] = -lam[0] * L[0];
+  for (int l = 1; l < nv; ++l) b[l] = -lam[1] * L[l];
+  b[4] = -lam[2] * L[4];
+  const double c0 = kx * b[1] + ky * b[2] + kz * b[3];
+  c1 = c0 + b[0] + b[4];
+  c2 = c * (b[4] - b[0]);
+  c3 = 0;
+  for (int l = 0; l < ns; ++l) c3 += (hi[l] - mw / zz->mech.mw[l] * c * c / (gamma - 1)) * b[l + 5];
+  f[0] += 0.5 * c1;
+  f[1] += 0.5 * (u * c1 + kx * c2 - c * (kz * b[2] - ky * b[3]));
+  f[2] += 0.5 * (v * c1 + ky * c2 - c * (kx * b[3] - kz * b[1]));
+  f[3] += 0.5 * (w * c1 + kz * c2 - c * (ky * b[1] - kx * b[2]));
+  f[4] += 0.5 * (h * c1 + Uk * c2 - c * c * c0 / (gamma - 1) +
+                 c * ((kz * v - ky * w) * b[1] + (kx * w - kz * u) * b[2] + (ky * u - kx * v) * b[3]) + c3);
+  for (int l = 0; l < nv - 5; ++l) f[5 + l] += 0.5 * (b[l + 5] + svm[l] * c1);
+}
+
 static void riemann(const orc_zone *zz, const double *pl, const double *pr, const double *met, const double *jac,
-                    int ip, double *f) {
+                    const double *efd, int ip, double *f) {
   switch (zz->cfg.inviscid_scheme) {
     case 1: flux_lf(zz, pl, pr, met, jac, ip, f); break;
+    case 2: /* with AWENO (inviscid_type 1) the reference runs AUSM+ (InviscidScheme.cu:290-303) */
+      if (zz->inviscid_type == 2) {
+        flux_roe(zz, pl, pr, met, jac, efd, ip, f);
+        break;
+      }
+      flux_ausm(zz, pl, pr, met, jac, ip, f);
+      break;
     case 4: flux_hllc(zz, pl, pr, met, jac, ip, f); break;
     default: flux_ausm(zz, pl, pr, met, jac, ip, f); break;
   }
@@ -1039,6 +1159,7 @@ static void inviscid_dir(orc_zone *zz, int dir) {
   double *cvb = malloc(sizeof(double) * npt * 7 + 8);
   double *met = malloc(sizeof(double) * npt * 3 + 8);
   double *jac = malloc(sizeof(double) * npt + 8);
+  double *efd = malloc(sizeof(double) * npt + 8);
   double *fc = malloc(sizeof(double) * (n + 1) * nv + 8);
   double *fn = malloc(sizeof(double) * npt * 5 + 8);
   for (int b = 0; b < nb; ++b)
@@ -1055,6 +1176,7 @@ static void inviscid_dir(orc_zone *zz, int dir) {
         cvb[q * 7 + 6] = BV(5, i, j, k);
         for (int c = 0; c < 3; ++c) met[q * 3 + c] = zz->metric[p * 9 + dir * 3 + c];
         jac[q] = zz->jac[p];
+        efd[q] = (zz->inviscid_type == 2 && s >= -1 && s <= n) ? efix_delta(zz, i, j, k) : 0.0;
       }
       /* face fluxes for faces s+1/2, s = -1 .. n-1 */
       for (int s = -1; s < n; ++s) {
@@ -1064,7 +1186,7 @@ static void inviscid_dir(orc_zone *zz, int dir) {
           aweno_air(cvb, met, q, pl, pr);
         else
           reconstruct(zz, pv, q, pl, pr);
-        riemann(zz, pl, pr, met, jac, q, &fc[(s + 1) * nv]);
+        riemann(zz, pl, pr, met, jac, efd, q, &fc[(s + 1) * nv]);
       }
       for (int s = 0; s < n; ++s) {
         int i, j, k;
@@ -1105,6 +1227,7 @@ static void inviscid_dir(orc_zone *zz, int dir) {
   free(cvb);
   free(met);
   free(jac);
+  free(efd);
   free(fc);
   free(fn);
 }
@@ -1429,7 +1552,24 @@ void orc_face_flux_air(int scheme, const double *pl5, const double *pr5, const d
   }
   pl[5] = 0.5 * (pl[1] * pl[1] + pl[2] * pl[2] + pl[3] * pl[3]) * pl[0] + pl[4] / (GAMMA_AIR - 1);
   pr[5] = 0.5 * (pr[1] * pr[1] + pr[2] * pr[2] + pr[3] * pr[3]) * pr[0] + pr[4] / (GAMMA_AIR - 1);
-  riemann(&zz, pl, pr, met6, jac2, 0, f5);
+  riemann(&zz, pl, pr, met6, jac2, NULL, 0, f5);
+}
+
+/* Roe face flux of air (scheme 2) with the entropy-fix deltas of the two points */
+void orc_face_flux_roe_air(const double *pl5, const double *pr5, const double *met6, const double *jac2,
+                           const double *delta2, double *f5) {
+  orc_zone zz;
+  memset(&zz, 0, sizeof(zz));
+  zz.nv = 5;
+  zz.cfg.inviscid_scheme = 2;
+  double pl[8], pr[8];
+  for (int l = 0; l < 5; ++l) {
+    pl[l] = pl5[l];
+    pr[l] = pr5[l];
+  }
+  pl[5] = 0.5 * (pl[1] * pl[1] + pl[2] * pl[2] + pl[3] * pl[3]) * pl[0] + pl[4] / (GAMMA_AIR - 1);
+  pr[5] = 0.5 * (pr[1] * pr[1] + pr[2] * pr[2] + pr[3] * pr[3]) * pr[0] + pr[4] / (GAMMA_AIR - 1);
+  flux_roe(&zz, pl, pr, met6, jac2, delta2, 0, f5);
 }
 
 /* WENO5 of scalar data v[0..5] at points i-2..i+3: v-(i+1/2), v+(i+1/2) */

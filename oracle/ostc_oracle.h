@@ -134,6 +134,8 @@ void orc_grid_metrics(int mx, int my, int mz, int ngg, const double *x, const do
 /* point-level test hooks (see orc_flow.c) */
 void orc_face_flux_air(int scheme, const double *pl5, const double *pr5, const double *met6, const double *jac2,
                        double *f5);
+void orc_face_flux_roe_air(const double *pl5, const double *pr5, const double *met6, const double *jac2,
+                           const double *delta2, double *f5);
 void orc_weno5_scalar(const double *v6, double *vm, double *vp);
 void orc_aweno_matrices(const double *cl7, const double *cr7, const double *met6, double *L25, double *R25);
 void orc_ssprk3_coeffs(double *a, double *b, double *c);

@@ -118,3 +118,20 @@ def test_c2_small_mixed_precision_error(tmp_path):
         l2 = np.linalg.norm(d) / np.linalg.norm(bv_o[comps])
         print(f"mixed {name}: L2 {l2:.2e} Linf {linf:.2e}", flush=True)
         assert linf < 1e-3
+
+
+def test_c1_roe_steps(tmp_path):
+    """Roe flux with Harten's entropy fix (inviscid_scheme 2, MUSCL): RHS after
+    one evaluation and the state after 20 steps."""
+    case = cases.c1_case(80, 40, inviscid_scheme=2)
+    s = _solver(case, tmp_path)
+    z = oracle_zone(case)
+    s.rhs()
+    z.rhs()
+    errs = grouped(s.download("dq"), z.field("dq"), 5, CV_GROUPS)
+    print("c1 roe rhs err", {k: "%.2e" % v for k, v in errs.items()}, flush=True)
+    assert max(errs.values()) < 1e-12
+    dt_p, _ = s.step(20)
+    dt_o, _ = z.run(20)
+    assert relerr(dt_p, dt_o) < 1e-10
+    _compare_state(s, z, case, 1e-10, "c1-roe-20")

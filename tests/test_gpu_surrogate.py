@@ -79,7 +79,7 @@ def test_gradient_matches_oracle(solver, B):
     p = mo.pack(layers).astype(np.float32)
     solver.surrogate_set_params(p)
     x = rng.standard_normal((B, 11)).astype(np.float32)
-    t = rng.standard_normal((B, 7)).astype(np.float32)
+    t = rng.standard_normal((B, 8)).astype(np.float32)
     g, loss = solver.surrogate_gradient(x, t)
     lb, gb = mo.loss_and_grad(p.astype(float), x.astype(float), t.astype(float), bf=True)
     lf, gf = mo.loss_and_grad(p.astype(float), x.astype(float), t.astype(float), bf=False)

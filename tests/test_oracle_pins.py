@@ -331,6 +331,89 @@ def test_supersonic_upwinding(scheme):
                                      k, J), _physical_flux(ql * np.array([1, -1, -1, -1, 1]), k, J), rtol=1e-12)
 
 
+def _roe(ql, qr, k, J, delta=(0.0, 0.0)):
+    f = np.zeros(5)
+    met = np.array(list(k) + list(k), dtype=np.float64)
+    jac = np.array([J, J], dtype=np.float64)
+    d = np.asarray(delta, dtype=np.float64)
+    po.lib().orc_face_flux_roe_air(po._p(np.asarray(ql, float)), po._p(np.asarray(qr, float)), po._p(met),
+                                   po._p(jac), po._p(d), po._p(f))
+    return f
+
+
+def _cons(q):
+    rho, u, v, w, p = q
+    return np.array([rho, rho * u, rho * v, rho * w, 0.5 * rho * (u * u + v * v + w * w) + p / 0.4])
+
+
+def _prim(Q):
+    rho = Q[0]
+    u, v, w = Q[1:4] / rho
+    return np.array([rho, u, v, w, 0.4 * (Q[4] - 0.5 * rho * (u * u + v * v + w * w))])
+
+
+def test_roe_consistency():
+    q = np.array([1.2, 150.0, -40.0, 25.0, 9.0e4])
+    k = np.array([0.8, 0.3, -0.2])
+    J = 2.5e-3
+    ref = _physical_flux(q, k, J)
+    np.testing.assert_allclose(_roe(q, q, k, J, (30.0, 40.0)), ref, rtol=1e-12, atol=1e-12 * np.abs(ref).max())
+
+
+def test_roe_upwinds_supersonic_and_contact():
+    k = np.array([1.0, 0.0, 0.0])
+    ql = np.array([1.0, 900.0, 10.0, 0.0, 1.0e5])
+    qr = np.array([1.3, 850.0, -5.0, 2.0, 1.2e5])
+    np.testing.assert_allclose(_roe(ql, qr, k, 1.0), _physical_flux(ql, k, 1.0), rtol=1e-11)
+    # a contact (equal u and p) moving right: exact upwinding
+    ql = np.array([1.0, 80.0, 0.0, 0.0, 1.0e5])
+    qr = np.array([0.4, 80.0, 0.0, 0.0, 1.0e5])
+    np.testing.assert_allclose(_roe(ql, qr, k, 1.0), _physical_flux(ql, k, 1.0), rtol=1e-11)
+
+
+def test_roe_dissipation_matches_eigen_decomposition():
+    """1/2 (F_L + F_R) - 1/2 R |Lambda| R^-1 (Q_R - Q_L) with the flux
+    Jacobian of the Roe-averaged state taken by finite differences and
+    decomposed by numpy (entropy fix inactive)."""
+    rng = np.random.default_rng(4)
+    for _ in range(6):
+        ql = np.array([1.0, 50.0, 20.0, -10.0, 1.0e5]) * (1 + 0.2 * rng.standard_normal(5))
+        qr = np.array([1.1, 30.0, -10.0, 5.0, 1.1e5]) * (1 + 0.2 * rng.standard_normal(5))
+        k = rng.standard_normal(3)
+        rl, rr = np.sqrt(ql[0]), np.sqrt(qr[0])
+        Hl = (_cons(ql)[4] + ql[4]) / ql[0]
+        Hr = (_cons(qr)[4] + qr[4]) / qr[0]
+        uh = (rl * ql[1:4] + rr * qr[1:4]) / (rl + rr)
+        Hh = (rl * Hl + rr * Hr) / (rl + rr)
+        rh = rl * rr
+        ph = rh * 0.4 / 1.4 * (Hh - 0.5 * uh @ uh)
+        Qh = _cons(np.array([rh, *uh, ph]))
+        F = lambda Q: _physical_flux(_prim(Q), k, 1.0)
+        A = np.zeros((5, 5))
+        for c in range(5):
+            h = 1e-6 * max(abs(Qh[c]), 1.0)
+            e = np.zeros(5)
+            e[c] = h
+            A[:, c] = (F(Qh + e) - F(Qh - e)) / (2 * h)
+        lam, R = np.linalg.eig(A)
+        absA = (R * np.abs(lam)) @ np.linalg.inv(R)
+        ref = 0.5 * (F(_cons(ql)) + F(_cons(qr))) - 0.5 * np.real(absA) @ (_cons(qr) - _cons(ql))
+        np.testing.assert_allclose(_roe(ql, qr, k, 1.0), ref, rtol=1e-6, atol=1e-6 * np.abs(ref).max())
+
+
+def test_roe_entropy_fix():
+    """Harten's fix: |lambda| < delta is replaced by (lambda^2/delta + delta)/2,
+    so a stationary contact gets dissipation delta/2 instead of 0."""
+    k = np.array([1.0, 0.0, 0.0])
+    ql = np.array([1.0, 0.0, 0.0, 0.0, 1.0e5])
+    qr = np.array([0.5, 0.0, 0.0, 0.0, 1.0e5])
+    f0 = _roe(ql, qr, k, 1.0)
+    f1 = _roe(ql, qr, k, 1.0, (10.0, 30.0))   # face delta 20
+    # mass flux: only the entropy wave carries the density jump; -1/2 * (delta/2) * d rho
+    assert f0[0] == pytest.approx(0.0, abs=1e-9)
+    assert f1[0] == pytest.approx(-0.5 * (20.0 / 2) * (qr[0] - ql[0]), rel=1e-12)
+
+
 def _weno(v):
     vm, vp = C.c_double(), C.c_double()
     v = np.ascontiguousarray(v, dtype=np.float64)
