@@ -131,6 +131,19 @@ int ostc_update_properties(ostc_solver *s, int blk);
  * dt_hist (3*nsteps) and res_hist (5*nsteps: 4 residuals, err_max) may be NULL.
  * Replaces: RK3_bv (RK.cuh:19-195) without IO/monitors. */
 int ostc_step_rk3(ostc_solver *s, int nsteps, double *dt_hist, double *res_hist);
+/* The implicit treatment of the current dq of one block with the local dt
+ * and spectral radii of the last ostc_global_dt: implicit_method 0 applies
+ * the point-implicit chemistry (chemSrcMethod 1 EPI, 2 DA) in place;
+ * implicit_method 1 runs DPLUR with DPLUR_inner_step sweeps and returns dq/dt.
+ * Replaces: implicit_treatment (ImplicitTreatmentHPP.cuh:7-52), DPLUR
+ * (DPLUR.cuh:259-314), EPI / DA (FiniteRateChem.cu:234-256, 420-432). */
+int ostc_implicit(ostc_solver *s, int blk);
+/* `nsteps` steady iterations with local time steps: RHS, local dt, implicit
+ * treatment, update_bv, boundary conditions (no cv), properties, residual
+ * (res_hist 5*nsteps, NaN where not computed).
+ * Replaces: steady_simulation (SteadySim.cuh:15-135) without IO, update_bv
+ * (FieldOperation.cuh:221-296). */
+int ostc_step_steady(ostc_solver *s, int nsteps, double *res_hist);
 /* Physical time and step counter. */
 int ostc_time(ostc_solver *s, double *t, int *step);
 /* 1 = flux arithmetic in FP32 (state, thermo, Newton T and accumulation stay FP64). */

@@ -300,8 +300,9 @@ FieldDesc field_desc(ostc_solver *s, Block &b, int f) {
     case OSTC_BV_LAST: return {z.bv_last, s0 * 4, 0, 4};
     case OSTC_JAC: return {const_cast<double *>(z.jac), sg, z.g, 1};
     case OSTC_METRIC: return {const_cast<double *>(z.metric), sg * 9, z.g, 9};
-    case OSTC_CHEM_JAC: return {z.chem_jac, s->P.chemSrcMethod == 1 ? s0 * s->n_spec * s->n_spec : 0, 0,
-                                s->n_spec * s->n_spec};
+    case OSTC_CHEM_JAC: return {z.chem_jac, s->P.chemSrcMethod == 1 ? s0 * s->n_spec * s->n_spec
+                                            : (s->P.chemSrcMethod == 2 ? s0 * s->n_spec : 0), 0,
+                                s->P.chemSrcMethod == 2 ? s->n_spec : s->n_spec * s->n_spec};
   }
   return {nullptr, -1, 0, 0};
 }
@@ -657,7 +658,9 @@ int ostc_finalize(ostc_solver *s) {
         alloc(&z.gam, ns ? L.sz : 1) || alloc(&z.cp, ns ? L.sz : 1) || alloc(&jac, L.sz) ||
         alloc(&met, L.sz * 9) || alloc(&z.dq, L0.sz * nv) || alloc(&z.dt_local, L0.sz) ||
         alloc(&z.bv_last, L0.sz * 4) ||
-        alloc(&z.chem_jac, s->P.chemSrcMethod == 1 ? L0.sz * ns * ns : 1))
+        alloc(&z.chem_jac, s->P.chemSrcMethod == 1 ? L0.sz * ns * ns : (s->P.chemSrcMethod == 2 ? L0.sz * ns : 1)) ||
+        alloc(&z.inv_sr, L0.sz * 3) ||
+        (s->prm.get_int("implicit_method", 0) == 1 && (alloc(&z.dq0, L0.sz * nv) || alloc(&z.dqk, L0.sz * nv))))
       return 1;
     z.jac = jac;
     z.metric = met;
@@ -935,6 +938,86 @@ int ostc_step_rk3(ostc_solver *s, int nsteps, double *dt_hist, double *res_hist)
   }
   s->step += nsteps;
   if (dt_hist) std::copy(hdt.begin(), hdt.end(), dt_hist);
+  if (res_hist) std::copy(hres.begin(), hres.end(), res_hist);
+  s->drain_profile();
+  return 0;
+}
+
+static int implicit_block(ostc_solver *s, Block &b) {
+  const int method = s->prm.get_int("implicit_method", 0);
+  ostc_solver::Timer t(s, "implicit");
+  if (method == 1) {
+    std::vector<BCDesc> walls;
+    for (auto &bc : b.bcs)
+      if (bc.type == BC_WALL) walls.push_back(bc);
+    HIPCHK(launch_dplur(b.z, s->P, s->d_mech, s->prm.get_int("DPLUR_inner_step", 3), walls, s->stream));
+  } else {
+    HIPCHK(launch_chem_implicit(b.z, s->P, s->stream));
+  }
+  return 0;
+}
+
+int ostc_implicit(ostc_solver *s, int blk) {
+  if (check_blk(s, blk)) return 1;
+  HIPCHK(hipSetDevice(s->device));
+  if (implicit_block(s, s->blocks[blk])) return 1;
+  HIPCHK(hipStreamSynchronize(s->stream));
+  return 0;
+}
+
+int ostc_step_steady(ostc_solver *s, int nsteps, double *res_hist) {
+  if (check_blk(s, 0)) return 1;
+  if (nsteps < 0) return fail("negative step count");
+  if (nsteps == 0) return 0;
+  HIPCHK(hipSetDevice(s->device));
+  if (s->blocks.size() != 1) return fail("the steady driver runs single-block cases");
+  Block &b = s->blocks[0];
+  double *d_res = nullptr;
+  HIPCHK(hipMalloc(&d_res, sizeof(double) * 5 * nsteps));
+  s->w.res_hist = d_res;
+  const int os = std::max(1, s->prm.get_int("output_screen", 1));
+  auto body = [&]() -> int {
+    for (int it = 0; it < nsteps; ++it) {
+      const int step = s->step + 1 + it;
+      if (step % os == 0) HIPCHK(launch_store_last(b.z, s->stream));
+      if (rhs_block(s, b)) return 1;
+      {
+        ostc_solver::Timer t(s, "local_dt");
+        HIPCHK(launch_local_dt(b.z, s->P, s->w, s->w.dt, s->stream));
+      }
+      if (implicit_block(s, b)) return 1;
+      {
+        ostc_solver::Timer t(s, "update_bv");
+        HIPCHK(launch_update_bv(b.z, s->P, s->d_mech, s->stream));
+      }
+      if (run_bcs(s, b, false)) return 1;
+      if (s->P.dim == 2) HIPCHK(launch_eliminate_k(b.z, s->P, false, s->stream));
+      if (properties_block(s, b)) return 1;
+      if (step % os == 0 || step == 1) {
+        ostc_solver::Timer t(s, "residual");
+        HIPCHK(launch_residual(b.z, s->w, it, step == 1 ? 1 : 0, s->stream));
+      }
+    }
+    return 0;
+  };
+  int rc = body();
+  std::vector<double> hres(5 * nsteps, NAN);
+  if (!rc) {
+    hipError_t e = hipMemcpyAsync(hres.data(), d_res, sizeof(double) * 5 * nsteps, hipMemcpyDeviceToHost, s->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+    if (e != hipSuccess) rc = fail(std::string("step_steady: ") + hipGetErrorString(e));
+  }
+  (void)hipFree(d_res);
+  s->w.res_hist = nullptr;
+  if (rc) return rc;
+  for (int it = 0; it < nsteps; ++it) {
+    const int step = s->step + 1 + it;
+    if (!(step % os == 0 || step == 1))
+      for (int l = 0; l < 5; ++l) hres[it * 5 + l] = NAN;
+    else if (std::isnan(hres[it * 5 + 4]))
+      return fail("NaN residual at step " + std::to_string(step));
+  }
+  s->step += nsteps;
   if (res_hist) std::copy(hres.begin(), hres.end(), res_hist);
   s->drain_profile();
   return 0;

@@ -61,9 +61,10 @@ __device__ inline void d_rate(const DMech *__restrict__ m, int r, double t, cons
 // FiniteRateChem.cu:7-63 + 151-217: omega [kg/m3/s] and, when jac != null,
 // the exact-point-implicit Jacobian d omega_m / d(rho Y_n).  Rates, progress
 // and source are fused per reaction so no per-reaction array is kept.
+// omega_d (may be null): the destruction rates [kg/m3/s] of chemical_source.
 template <int NS>
 __device__ inline void d_chem_source(const DMech *__restrict__ m, double rho, double t, const double *Y, double *omega,
-                                     double *jac) {
+                                     double *jac, double *omega_d = nullptr) {
   double c[NS], g[NS];
 #pragma unroll
   for (int l = 0; l < NS; ++l) c[l] = rho * Y[l] / m->mw[l] * 1e-3;
@@ -108,6 +109,9 @@ __device__ inline void d_chem_source(const DMech *__restrict__ m, double rho, do
   }
 #pragma unroll
   for (int l = 0; l < NS; ++l) omega[l] = creation[l] * (m->mw[l] * 1e+3) - destr[l] * (m->mw[l] * 1e+3);
+  if (omega_d)
+#pragma unroll
+    for (int l = 0; l < NS; ++l) omega_d[l] = destr[l] * (m->mw[l] * 1e+3);
   if (jac) {
 #pragma unroll
     for (int n = 0; n < NS; ++n) {

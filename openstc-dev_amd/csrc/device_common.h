@@ -49,6 +49,8 @@ struct ZoneView {
   double *vel, *acoustic, *mach, *mul, *cond, *rhoD, *gam, *cp;
   const double *jac, *metric;
   double *dq, *dt_local, *bv_last, *chem_jac;
+  double *inv_sr;          // steady path: inviscid spectral radii, 3 x ghost-free (null otherwise)
+  double *dq0, *dqk;       // DPLUR work arrays, n_var x ghost-free (null otherwise)
 
   __host__ __device__ int64_t at(int i, int j, int k) const {
     return (int64_t)(k + g) * d1 + (int64_t)(j + g) * d2 + (i + g);

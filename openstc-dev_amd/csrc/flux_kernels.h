@@ -3,6 +3,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <vector>
+
 #include "device_common.h"
 
 namespace ostc {
@@ -47,6 +49,14 @@ hipError_t launch_residual(const ZoneView &z, const StepScratch &w, int slot, in
 hipError_t launch_copy_dt(const double *src, double *dst, hipStream_t s);
 hipError_t launch_chem_batch(const DMech *m, int64_t n, const double *rho, const double *T, const double *Y,
                              double dt, double *dY, double *dT, hipStream_t s);
+// ---- steady path (implicit.hip)
+// point-implicit chemistry on dq with the local dt (EPI or DA per chemSrcMethod)
+hipError_t launch_chem_implicit(const ZoneView &z, const DParams &P, hipStream_t s);
+// DPLUR: dq0, inner sweeps (wall dq zeroed before each), dq back to dq/dt
+hipError_t launch_dplur(const ZoneView &z, const DParams &P, const DMech *m, int inner_steps,
+                        const std::vector<BCDesc> &walls, hipStream_t s);
+// local-dt update of bv / sv from dq (cv untouched)
+hipError_t launch_update_bv(const ZoneView &z, const DParams &P, const DMech *m, hipStream_t s);
 // smallest singular value of I - dt J per cell (conditioning of the point-implicit step)
 hipError_t launch_epi_sigma(const DMech *m, int64_t n, const double *rho, const double *T, const double *Y, double dt,
                             double *sig, hipStream_t s);

@@ -1,0 +1,327 @@
+// Steady-path kernels: the implicit treatment of the RHS (point-implicit
+// chemistry, DPLUR) and the local-time-step update of the primitives.
+//   ImplicitTreatmentHPP.cuh:7-52, FiniteRateChem.cu:234-256 / 420-432,
+//   DPLUR.cuh:14-314, DPLUR.cu:5-52, FieldOperation.cuh:221-296.
+// One thread per interior cell; dq, dq0, dqk, dt_local, inv_sr and chem_jac
+// use the ghost-free component-major layout.
+#include <hip/hip_runtime.h>
+
+#include "device_chem.h"
+#include "flux_kernels.h"
+
+namespace ostc {
+
+namespace {
+
+constexpr int kTB = 256;
+
+__device__ inline bool cell_index(const ZoneView &z, int &i, int &j, int &k) {
+  const int64_t t = (int64_t)blockIdx.x * kTB + threadIdx.x;
+  const int64_t n = (int64_t)z.mx * z.my * z.mz;
+  if (t >= n) return false;
+  i = (int)(t % z.mx);
+  const int64_t r = t / z.mx;
+  j = (int)(r % z.my);
+  k = (int)(r / z.my);
+  return true;
+}
+
+// (diag I - dt J) x = b for the species rows, J the stored EPI Jacobian
+template <int NS>
+__device__ inline void epi_solve(const ZoneView &z, int64_t q, double diag, double dt, double *b) {
+  double A[NS * NS];
+#pragma unroll
+  for (int a = 0; a < NS; ++a)
+#pragma unroll
+    for (int c = 0; c < NS; ++c)
+      A[a * NS + c] = (a == c ? diag : 0.0) - dt * z.chem_jac[(int64_t)(a * NS + c) * z.sz0 + q];
+  d_lu_solve<NS>(A, b);
+}
+
+// explicit steady step with a point-implicit chemical source:
+// EPI (I - dt J) x = dq_species, or DA dq_l /= 1 - dt J_ll
+template <int NS>
+__global__ void __launch_bounds__(kTB) k_chem_implicit(ZoneView z, int method) {
+  int i, j, k;
+  if (!cell_index(z, i, j, k)) return;
+  const int64_t q = z.at0(i, j, k);
+  const double dt = z.dt_local[q];
+  if (method == 1) {
+    double b[NS];
+#pragma unroll
+    for (int l = 0; l < NS; ++l) b[l] = z.dq[(int64_t)(5 + l) * z.sz0 + q];
+    epi_solve<NS>(z, q, 1.0, dt, b);
+#pragma unroll
+    for (int l = 0; l < NS; ++l) z.dq[(int64_t)(5 + l) * z.sz0 + q] = b[l];
+  } else {
+#pragma unroll
+    for (int l = 0; l < NS; ++l) z.dq[(int64_t)(5 + l) * z.sz0 + q] /= 1 - dt * z.chem_jac[(int64_t)l * z.sz0 + q];
+  }
+}
+
+// DPLUR.cuh:14-78 compute_DQ_0, then dq0 = dq
+template <int NS>
+__global__ void __launch_bounds__(kTB) k_dplur_dq0(ZoneView z, int fr, int chem) {
+  constexpr int NV = 5 + NS;
+  int i, j, k;
+  if (!cell_index(z, i, j, k)) return;
+  const int64_t q = z.at0(i, j, k);
+  const double dt = z.dt_local[q];
+  const double diag = 1 + dt * (z.inv_sr[q] + z.inv_sr[z.sz0 + q] + z.inv_sr[2 * z.sz0 + q]);
+  double d[NV];
+#pragma unroll
+  for (int l = 0; l < NV; ++l) d[l] = z.dq[(int64_t)l * z.sz0 + q] * dt;
+#pragma unroll
+  for (int l = 0; l < 5; ++l) d[l] /= diag;
+  if constexpr (NS > 0) {
+    if (fr && chem == 1) {
+      epi_solve<NS>(z, q, diag, dt, d + 5);
+    } else if (fr && chem == 2) {
+#pragma unroll
+      for (int l = 0; l < NS; ++l) d[5 + l] /= diag - dt * z.chem_jac[(int64_t)l * z.sz0 + q];
+    } else {
+#pragma unroll
+      for (int l = 0; l < NS; ++l) d[5 + l] /= diag;
+    }
+  }
+#pragma unroll
+  for (int l = 0; l < NV; ++l) {
+    z.dq[(int64_t)l * z.sz0 + q] = d[l];
+    z.dq0[(int64_t)l * z.sz0 + q] = d[l];
+  }
+}
+
+// DPLUR.cuh:80-137: (A +- lambda I) dq at point (i, j, k) along dir
+template <int NS>
+__device__ inline void jac_times_dq(const ZoneView &z, const DMech *__restrict__ m, int i, int j, int k, int dir,
+                                    double lam, double *out) {
+  constexpr int NV = 5 + NS;
+  const int64_t p = z.at(i, j, k), q = z.at0(i, j, k);
+  const double mx = z.metric[(dir * 3 + 0) * z.sz + p], my = z.metric[(dir * 3 + 1) * z.sz + p],
+               mz = z.metric[(dir * 3 + 2) * z.sz + p];
+  const double u = z.bv[z.sz + p], v = z.bv[2 * z.sz + p], w = z.bv[3 * z.sz + p];
+  const double U = mx * u + my * v + mz * w;
+  const double l1 = U + lam;
+  const double e = 0.5 * z.vel[p] * z.vel[p];
+  double d[NV];
+#pragma unroll
+  for (int l = 0; l < NV; ++l) d[l] = z.dq[(int64_t)l * z.sz0 + q];
+  double gamma = kGam, b3 = 0, b4 = 0, h = 0;
+  if constexpr (NS == 0) {
+    h = gamma / (gamma - 1) * z.bv[4 * z.sz + p] / z.bv[p] + e;
+  } else {
+    double hs[NS];
+    const double t = z.bv[5 * z.sz + p];
+    d_enthalpy<NS>(t, m, hs);
+    gamma = z.gam[p];
+#pragma unroll
+    for (int l = 0; l < NS; ++l) {
+      b3 += kRuD * t / m->mw[l] * d[5 + l];
+      b4 += hs[l] * d[5 + l];
+      h += z.sv[(int64_t)l * z.sz + p] * hs[l];
+    }
+    b3 *= gamma;
+    b4 *= gamma - 1;
+    h += e;
+  }
+  const double b1 = mx * d[1] + my * d[2] + mz * d[3] - U * d[0];
+  const double b2 = (gamma - 1) * (e * d[0] - u * d[1] - v * d[2] - w * d[3] + d[4]);
+  out[0] = b1 + l1 * d[0];
+  out[1] = u * b1 + mx * b2 + l1 * d[1] + mx * (b3 - b4);
+  out[2] = v * b1 + my * b2 + l1 * d[2] + my * (b3 - b4);
+  out[3] = w * b1 + mz * b2 + l1 * d[3] + mz * (b3 - b4);
+  out[4] = h * b1 + U * b2 + l1 * d[4] + U * (b3 - b4);
+#pragma unroll
+  for (int l = 0; l < NS; ++l) out[5 + l] = l1 * d[5 + l] + z.sv[(int64_t)l * z.sz + p] * b1;
+}
+
+// DPLUR.cuh:139-257: dqk from the neighbours' current dq (interior only)
+template <int NS>
+__global__ void __launch_bounds__(kTB) k_dplur_inner(ZoneView z, const DMech *__restrict__ m, int fr, int chem) {
+  constexpr int NV = 5 + NS;
+  int i, j, k;
+  if (!cell_index(z, i, j, k)) return;
+  double tot[NV], jd[NV];
+#pragma unroll
+  for (int l = 0; l < NV; ++l) tot[l] = 0;
+  const int c[3] = {i, j, k}, ext[3] = {z.mx, z.my, z.mz};
+#pragma unroll
+  for (int dd = 0; dd < 3; ++dd) {
+    if (c[dd] > 0) {
+      const int ni = i - (dd == 0), nj = j - (dd == 1), nk = k - (dd == 2);
+      jac_times_dq<NS>(z, m, ni, nj, nk, dd, z.inv_sr[(int64_t)dd * z.sz0 + z.at0(ni, nj, nk)], jd);
+#pragma unroll
+      for (int l = 0; l < NV; ++l) tot[l] += 0.5 * jd[l];
+    }
+  }
+#pragma unroll
+  for (int dd = 0; dd < 3; ++dd) {
+    if (c[dd] != ext[dd] - 1) {
+      const int ni = i + (dd == 0), nj = j + (dd == 1), nk = k + (dd == 2);
+      jac_times_dq<NS>(z, m, ni, nj, nk, dd, -z.inv_sr[(int64_t)dd * z.sz0 + z.at0(ni, nj, nk)], jd);
+#pragma unroll
+      for (int l = 0; l < NV; ++l) tot[l] -= 0.5 * jd[l];
+    }
+  }
+  const int64_t q = z.at0(i, j, k);
+  const double dt = z.dt_local[q];
+  const double diag = 1 + dt * (z.inv_sr[q] + z.inv_sr[z.sz0 + q] + z.inv_sr[2 * z.sz0 + q]);
+#pragma unroll
+  for (int l = 0; l < 5; ++l) z.dqk[(int64_t)l * z.sz0 + q] = z.dq0[(int64_t)l * z.sz0 + q] + dt * tot[l] / diag;
+  if constexpr (NS > 0) {
+    if (fr && chem == 1) {
+      double b[NS];
+#pragma unroll
+      for (int l = 0; l < NS; ++l) b[l] = tot[5 + l] * dt;
+      epi_solve<NS>(z, q, diag, dt, b);
+#pragma unroll
+      for (int l = 0; l < NS; ++l) z.dqk[(int64_t)(5 + l) * z.sz0 + q] = z.dq0[(int64_t)(5 + l) * z.sz0 + q] + b[l];
+    } else if (fr && chem == 2) {
+#pragma unroll
+      for (int l = 0; l < NS; ++l)
+        z.dqk[(int64_t)(5 + l) * z.sz0 + q] =
+            z.dq0[(int64_t)(5 + l) * z.sz0 + q] + dt * tot[5 + l] / (diag - dt * z.chem_jac[(int64_t)l * z.sz0 + q]);
+    } else {
+#pragma unroll
+      for (int l = 0; l < NS; ++l)
+        z.dqk[(int64_t)(5 + l) * z.sz0 + q] = z.dq0[(int64_t)(5 + l) * z.sz0 + q] + dt * tot[5 + l] / diag;
+    }
+  }
+}
+
+// DPLUR.cu:20-52 set_dq_to_0 over one wall range (interior cells of the wall plane)
+__global__ void k_zero_dq_range(ZoneView z, int nv, int i0, int i1, int j0, int j1, int k0, int k1) {
+  const int64_t t = (int64_t)blockIdx.x * kTB + threadIdx.x;
+  const int ni = i1 - i0 + 1, nj = j1 - j0 + 1, nk = k1 - k0 + 1;
+  if (t >= (int64_t)ni * nj * nk) return;
+  const int i = i0 + (int)(t % ni), j = j0 + (int)((t / ni) % nj), k = k0 + (int)(t / ((int64_t)ni * nj));
+  const int64_t q = z.at0(i, j, k);
+  for (int l = 0; l < nv; ++l) z.dq[(int64_t)l * z.sz0 + q] = 0;
+}
+
+__global__ void k_copy(const double *__restrict__ src, double *__restrict__ dst, int64_t n) {
+  const int64_t t = (int64_t)blockIdx.x * kTB + threadIdx.x;
+  if (t < n) dst[t] = src[t];
+}
+
+// DPLUR.cu:5-17 convert_dq_back_to_dqDt
+__global__ void k_dq_back(ZoneView z, int nv) {
+  int i, j, k;
+  if (!cell_index(z, i, j, k)) return;
+  const int64_t q = z.at0(i, j, k);
+  const double dt = z.dt_local[q];
+  for (int l = 0; l < nv; ++l) z.dq[(int64_t)l * z.sz0 + q] /= dt;
+}
+
+// FieldOperation.cuh:221-296 update_bv: rho, momentum, total energy and
+// scalars advanced by dq dt_local / J from the old primitives; then T and p
+// (Newton from the energy for mixtures).  cv is not touched.
+template <int NS>
+__global__ void __launch_bounds__(kTB) k_update_bv(ZoneView z, DParams P, const DMech *__restrict__ m) {
+  int i, j, k;
+  if (!cell_index(z, i, j, k)) return;
+  const int64_t p = z.at(i, j, k), q = z.at0(i, j, k);
+  const int64_t S = z.sz, S0 = z.sz0;
+  const double rho_n = z.bv[p];
+  const double u0 = z.bv[S + p], v0 = z.bv[2 * S + p], w0 = z.bv[3 * S + p];
+  double En = 0.5 * (u0 * u0 + v0 * v0 + w0 * w0);
+  double Y[NS > 0 ? NS : 1];
+  if constexpr (NS > 0) {
+    double h[NS];
+    d_enthalpy<NS>(z.bv[5 * S + p], m, h);
+#pragma unroll
+    for (int l = 0; l < NS; ++l) {
+      Y[l] = z.sv[(int64_t)l * S + p];
+      En += h[l] * Y[l];
+    }
+    En = En * rho_n - z.bv[4 * S + p];
+  } else {
+    En = En * rho_n + z.bv[4 * S + p] / (kGam - 1);
+  }
+  const double dtj = z.dt_local[q] / z.jac[p];
+  const double rho = rho_n + z.dq[q] * dtj;
+  z.bv[p] = rho;
+  const double ri = 1.0 / rho;
+  const double u = ri * (rho_n * u0 + z.dq[S0 + q] * dtj);
+  const double v = ri * (rho_n * v0 + z.dq[2 * S0 + q] * dtj);
+  const double w = z.mz == 1 ? 0.0 : ri * (rho_n * w0 + z.dq[3 * S0 + q] * dtj);
+  z.bv[S + p] = u;
+  z.bv[2 * S + p] = v;
+  z.bv[3 * S + p] = w;
+  const double E = En + z.dq[4 * S0 + q] * dtj;
+  for (int l = 0; l < P.nsc; ++l) {
+    const double y = ri * (rho_n * z.sv[(int64_t)l * S + p] + z.dq[(int64_t)(5 + l) * S0 + q] * dtj);
+    z.sv[(int64_t)l * S + p] = y;
+    if constexpr (NS > 0)
+      if (l < NS) Y[l] = y;
+  }
+  const double V2 = u * u + v * v + w * w;
+  if constexpr (NS > 0) {
+    double pn;
+    const double t = d_temperature<NS>(m, rho, E, u, v, w, Y, z.bv[5 * S + p], pn);
+    z.bv[5 * S + p] = t;
+    z.bv[4 * S + p] = pn;
+  } else {
+    const double pn = (kGam - 1) * (E - 0.5 * rho * V2);
+    z.bv[4 * S + p] = pn;
+    z.bv[5 * S + p] = pn * kMwAirD * ri / kRuD;
+  }
+  z.vel[p] = sqrt(V2);
+}
+
+inline unsigned nb(int64_t n) { return (unsigned)((n + kTB - 1) / kTB); }
+
+}  // namespace
+
+hipError_t launch_chem_implicit(const ZoneView &z, const DParams &P, hipStream_t s) {
+  const int64_t n = (int64_t)z.mx * z.my * z.mz;
+  if (!(P.mix && P.reaction == 1) || (P.chemSrcMethod != 1 && P.chemSrcMethod != 2)) return hipSuccess;
+  if (P.ns != 9) return hipErrorNotSupported;
+  hipLaunchKernelGGL(k_chem_implicit<9>, dim3(nb(n)), dim3(kTB), 0, s, z, P.chemSrcMethod);
+  return hipGetLastError();
+}
+
+hipError_t launch_dplur(const ZoneView &z, const DParams &P, const DMech *m, int inner_steps,
+                        const std::vector<BCDesc> &walls, hipStream_t s) {
+  const int64_t n = (int64_t)z.mx * z.my * z.mz;
+  const int nv = P.nv;
+  const int fr = P.mix && P.reaction == 1;
+  if (!z.inv_sr || !z.dq0 || !z.dqk) return hipErrorInvalidValue;
+  if (P.ns == 0) {
+    hipLaunchKernelGGL(k_dplur_dq0<0>, dim3(nb(n)), dim3(kTB), 0, s, z, fr, P.chemSrcMethod);
+  } else if (P.ns == 9) {
+    hipLaunchKernelGGL(k_dplur_dq0<9>, dim3(nb(n)), dim3(kTB), 0, s, z, fr, P.chemSrcMethod);
+  } else {
+    return hipErrorNotSupported;
+  }
+  for (int it = 0; it < inner_steps; ++it) {
+    for (const BCDesc &b : walls) {
+      const int64_t cnt = (int64_t)(b.re[0] - b.rs[0] + 1) * (b.re[1] - b.rs[1] + 1) * (b.re[2] - b.rs[2] + 1);
+      if (cnt <= 0) continue;
+      if (b.rs[0] < 0 || b.rs[1] < 0 || b.rs[2] < 0 || b.re[0] >= z.mx || b.re[1] >= z.my || b.re[2] >= z.mz)
+        return hipErrorInvalidValue;
+      hipLaunchKernelGGL(k_zero_dq_range, dim3(nb(cnt)), dim3(kTB), 0, s, z, nv, b.rs[0], b.re[0], b.rs[1], b.re[1],
+                         b.rs[2], b.re[2]);
+    }
+    if (P.ns == 0)
+      hipLaunchKernelGGL(k_dplur_inner<0>, dim3(nb(n)), dim3(kTB), 0, s, z, m, fr, P.chemSrcMethod);
+    else
+      hipLaunchKernelGGL(k_dplur_inner<9>, dim3(nb(n)), dim3(kTB), 0, s, z, m, fr, P.chemSrcMethod);
+    hipLaunchKernelGGL(k_copy, dim3(nb(z.sz0 * nv)), dim3(kTB), 0, s, (const double *)z.dqk, z.dq, z.sz0 * nv);
+  }
+  hipLaunchKernelGGL(k_dq_back, dim3(nb(n)), dim3(kTB), 0, s, z, nv);
+  return hipGetLastError();
+}
+
+hipError_t launch_update_bv(const ZoneView &z, const DParams &P, const DMech *m, hipStream_t s) {
+  const int64_t n = (int64_t)z.mx * z.my * z.mz;
+  if (P.ns == 0)
+    hipLaunchKernelGGL(k_update_bv<0>, dim3(nb(n)), dim3(kTB), 0, s, z, P, m);
+  else if (P.ns == 9)
+    hipLaunchKernelGGL(k_update_bv<9>, dim3(nb(n)), dim3(kTB), 0, s, z, P, m);
+  else
+    return hipErrorNotSupported;
+  return hipGetLastError();
+}
+
+}  // namespace ostc

@@ -53,6 +53,12 @@ __global__ void __launch_bounds__(kTB) k_source(ZoneView z, DParams P, const DMe
     d_chem_source<NS>(m, rho, T, Y, om, jac);
 #pragma unroll
     for (int c = 0; c < NS * NS; ++c) z.chem_jac[(int64_t)c * z.sz0 + q] = jac[c];
+  } else if (P.chemSrcMethod == 2) {
+    // FiniteRateChem.cu:219-232 diagonal approximation: -omega_d / (rho Y)
+    double od[NS];
+    d_chem_source<NS>(m, rho, T, Y, om, nullptr, od);
+#pragma unroll
+    for (int l = 0; l < NS; ++l) z.chem_jac[(int64_t)l * z.sz0 + q] = Y[l] > 1e-25 ? -od[l] / (Y[l] * rho) : 0.0;
   } else {
     d_chem_source<NS>(m, rho, T, Y, om, nullptr);
   }
@@ -91,6 +97,12 @@ __global__ void __launch_bounds__(kTB) k_local_dt(ZoneView z, DParams P, double 
     const double r0 = fabs(U) + a * gxi, r1 = fabs(V) + a * geta;
     const double r2 = P.dim == 3 ? fabs(W) + a * gzeta : 0.0;
     const double sri = r0 + r1 + r2;
+    if (z.inv_sr) {
+      const int64_t q = z.at0(i, j, k);
+      z.inv_sr[q] = r0;
+      z.inv_sr[z.sz0 + q] = r1;
+      z.inv_sr[2 * z.sz0 + q] = r2;
+    }
     const double gamma = MIX ? z.gam[p] : kGam;
     const double c1 = fmax(gamma, 4.0 / 3.0) / z.bv[p];
     const double c2 = z.mul[p] / P.Pr;

@@ -68,6 +68,8 @@ def lib():
             "ostc_apply_bc": ([vp, C.c_int, C.c_int], C.c_int),
             "ostc_update_properties": ([vp, C.c_int], C.c_int),
             "ostc_step_rk3": ([vp, C.c_int, dp, dp], C.c_int),
+            "ostc_step_steady": ([vp, C.c_int, dp], C.c_int),
+            "ostc_implicit": ([vp, C.c_int], C.c_int),
             "ostc_time": ([vp, dp, ip], C.c_int),
             "ostc_set_flux_precision": ([vp, C.c_int], C.c_int),
             "ostc_chem_batch": ([vp, C.c_int64, vp, vp, vp, C.c_double, vp, vp, C.c_int,
@@ -264,6 +266,14 @@ class Solver:
         res = np.zeros(5 * nsteps)
         _chk(lib().ostc_step_rk3(self.h, nsteps, _dp(dt), _dp(res)))
         return dt.reshape(nsteps, 3), res.reshape(nsteps, 5)
+
+    def step_steady(self, nsteps):
+        res = np.zeros((nsteps, 5))
+        _chk(lib().ostc_step_steady(self.h, nsteps, _dp(res)))
+        return res
+
+    def implicit(self, blk=0):
+        _chk(lib().ostc_implicit(self.h, blk))
 
     def time(self):
         t, s = C.c_double(), C.c_int()

@@ -1537,6 +1537,252 @@ void orc_zone_run_rk3(orc_zone *zz, int nsteps, double *dt_hist, double *res_his
  * them against closed forms (consistency, supersonic upwinding, exactness of
  * WENO5 on low-order data, L R = I, the RK3 stability polynomial). */
 
+/* ----------------------------------------------------------- steady path */
+
+/* FiniteRateChem.cu:306-362 solve_chem_system / 234-256 EPI, EPI_for_dq0:
+ * (diag I - dt J) x = dq_species in place on the cell's dq */
+static void epi_in_place(orc_zone *zz, int i, int j, int k, double diag, double dt) {
+  const int ns = zz->ns;
+  const int64_t p = at(zz->L0, i, j, k);
+  double lhs[ORC_MAXS * ORC_MAXS], rhs[ORC_MAXS];
+  for (int a = 0; a < ns; ++a)
+    for (int b = 0; b < ns; ++b)
+      lhs[a * ns + b] = (a == b ? diag : 0.0) - dt * zz->chem_jac[(a * ns + b) * zz->L0.sz + p];
+  for (int l = 0; l < ns; ++l) rhs[l] = DQ(5 + l, i, j, k);
+  orc_lu_solve(lhs, rhs, ns);
+  for (int l = 0; l < ns; ++l) DQ(5 + l, i, j, k) = rhs[l];
+}
+
+static int is_fr(const orc_zone *zz) { return zz->cfg.mix && zz->cfg.reaction == 1; }
+
+/* DPLUR.cuh:14-78 compute_DQ_0 */
+static void dplur_dq0(orc_zone *zz, double *dq0) {
+  const int nv = zz->nv, ns = zz->ns;
+  for (int k = 0; k < zz->mz; ++k)
+    for (int j = 0; j < zz->my; ++j)
+      for (int i = 0; i < zz->mx; ++i) {
+        const int64_t p = at(zz->L0, i, j, k);
+        const double dt = zz->dt_local[p];
+        for (int l = 0; l < nv; ++l) DQ(l, i, j, k) *= dt;
+        const double *sr = &zz->inv_sr[p * 3];
+        const double diag = 1 + dt * (sr[0] + sr[1] + sr[2]);
+        for (int l = 0; l < 5; ++l) DQ(l, i, j, k) /= diag;
+        if (is_fr(zz) && zz->cfg.chemSrcMethod == 1) {
+          epi_in_place(zz, i, j, k, diag, dt);
+        } else if (is_fr(zz) && zz->cfg.chemSrcMethod == 2) {
+          for (int l = 0; l < ns; ++l) DQ(5 + l, i, j, k) /= diag - dt * zz->chem_jac[l * zz->L0.sz + p];
+        } else {
+          for (int l = 0; l < ns; ++l) DQ(5 + l, i, j, k) /= diag;
+        }
+      }
+  memcpy(dq0, zz->dq, sizeof(double) * zz->L0.sz * nv);
+}
+
+/* DPLUR.cuh:80-137 compute_jacobian_times_dq at point (i,j,k), direction dir,
+ * with the signed spectral radius lam: (A +- lam I) dq of the point */
+static void jac_times_dq(const orc_zone *zz, int i, int j, int k, int dir, double lam, double *out) {
+  const int ns = zz->ns;
+  const int64_t pg = at(zz->Lg, i, j, k);
+  const double *m = &zz->metric[pg * 9 + dir * 3];
+  const double u = BV(1, i, j, k), v = BV(2, i, j, k), w = BV(3, i, j, k);
+  const double U = m[0] * u + m[1] * v + m[2] * w;
+  const double l1 = U + lam;
+  const double e = 0.5 * zz->vel[pg] * zz->vel[pg];
+  double gamma = GAMMA_AIR, b3 = 0, b4 = 0, h = 0;
+  double dq[5 + ORC_MAXS];
+  for (int l = 0; l < zz->nv; ++l) dq[l] = DQ(l, i, j, k);
+  if (!zz->cfg.mix) {
+    h = gamma / (gamma - 1) * BV(4, i, j, k) / BV(0, i, j, k) + e;
+  } else {
+    double hs[ORC_MAXS];
+    const double t = BV(5, i, j, k);
+    orc_enthalpy(t, &zz->mech, hs);
+    gamma = zz->gam[pg];
+    for (int l = 0; l < ns; ++l) {
+      b3 += R_U * t / zz->mech.mw[l] * dq[5 + l];
+      b4 += hs[l] * dq[5 + l];
+      h += SV(l, i, j, k) * hs[l];
+    }
+    b3 *= gamma;
+    b4 *= gamma - 1;
+    h += e;
+  }
+  const double b1 = m[0] * dq[1] + m[1] * dq[2] + m[2] * dq[3] - U * dq[0];
+  const double b2 = (gamma - 1) * (e * dq[0] - u * dq[1] - v * dq[2] - w * dq[3] + dq[4]);
+  out[0] = b1 + l1 * dq[0];
+  out[1] = u * b1 + m[0] * b2 + l1 * dq[1] + m[0] * (b3 - b4);
+  out[2] = v * b1 + m[1] * b2 + l1 * dq[2] + m[1] * (b3 - b4);
+  out[3] = w * b1 + m[2] * b2 + l1 * dq[3] + m[2] * (b3 - b4);
+  out[4] = h * b1 + U * b2 + l1 * dq[4] + U * (b3 - b4);
+  for (int l = 0; l < zz->nsc; ++l) out[5 + l] = l1 * dq[5 + l] + SV(l, i, j, k) * b1;
+}
+
+/* DPLUR.cu:20-52 set_wall_dq_to_0 */
+static void wall_dq_zero(orc_zone *zz) {
+  for (int n = 0; n < zz->nbc; ++n) {
+    const int b = zz->order[n];
+    if (zz->cfg.bc[b].type != 2) continue;
+    for (int k = zz->rs[b][2]; k <= zz->re[b][2]; ++k)
+      for (int j = zz->rs[b][1]; j <= zz->re[b][1]; ++j)
+        for (int i = zz->rs[b][0]; i <= zz->re[b][0]; ++i)
+          for (int l = 0; l < zz->nv; ++l) DQ(l, i, j, k) = 0;
+  }
+}
+
+/* DPLUR.cuh:139-257 DPLUR_inner_iteration: dqk from the neighbours' current
+ * dq (interior neighbours only, no halo exchange), then dq = dqk */
+static void dplur_inner(orc_zone *zz, const double *dq0, double *dqk) {
+  const int nv = zz->nv, ns = zz->ns;
+  const int ext[3] = {zz->mx, zz->my, zz->mz};
+  for (int k = 0; k < zz->mz; ++k)
+    for (int j = 0; j < zz->my; ++j)
+      for (int i = 0; i < zz->mx; ++i) {
+        double tot[5 + ORC_MAXS] = {0}, jd[5 + ORC_MAXS];
+        const int c[3] = {i, j, k};
+        for (int d = 0; d < 3; ++d) {
+          if (c[d] > 0) {
+            int n3[3] = {i, j, k};
+            n3[d] -= 1;
+            jac_times_dq(zz, n3[0], n3[1], n3[2], d, zz->inv_sr[at(zz->L0, n3[0], n3[1], n3[2]) * 3 + d], jd);
+            for (int l = 0; l < nv; ++l) tot[l] += 0.5 * jd[l];
+          }
+        }
+        for (int d = 0; d < 3; ++d) {
+          if (c[d] != ext[d] - 1) {
+            int n3[3] = {i, j, k};
+            n3[d] += 1;
+            jac_times_dq(zz, n3[0], n3[1], n3[2], d, -zz->inv_sr[at(zz->L0, n3[0], n3[1], n3[2]) * 3 + d], jd);
+            for (int l = 0; l < nv; ++l) tot[l] -= 0.5 * jd[l];
+          }
+        }
+        const int64_t p = at(zz->L0, i, j, k);
+        const double dt = zz->dt_local[p];
+        const double *sr = &zz->inv_sr[p * 3];
+        const double diag = 1 + dt * (sr[0] + sr[1] + sr[2]);
+        for (int l = 0; l < 5; ++l) dqk[l * zz->L0.sz + p] = dq0[l * zz->L0.sz + p] + dt * tot[l] / diag;
+        if (is_fr(zz) && zz->cfg.chemSrcMethod == 1) {
+          double lhs[ORC_MAXS * ORC_MAXS], rhs[ORC_MAXS];
+          for (int a = 0; a < ns; ++a)
+            for (int b = 0; b < ns; ++b)
+              lhs[a * ns + b] = (a == b ? diag : 0.0) - dt * zz->chem_jac[(a * ns + b) * zz->L0.sz + p];
+          for (int l = 0; l < ns; ++l) rhs[l] = tot[5 + l] * dt;
+          orc_lu_solve(lhs, rhs, ns);
+          for (int l = 0; l < ns; ++l) dqk[(5 + l) * zz->L0.sz + p] = dq0[(5 + l) * zz->L0.sz + p] + rhs[l];
+        } else if (is_fr(zz) && zz->cfg.chemSrcMethod == 2) {
+          for (int l = 0; l < ns; ++l)
+            dqk[(5 + l) * zz->L0.sz + p] =
+                dq0[(5 + l) * zz->L0.sz + p] + dt * tot[5 + l] / (diag - dt * zz->chem_jac[l * zz->L0.sz + p]);
+        } else {
+          for (int l = 0; l < ns; ++l) dqk[(5 + l) * zz->L0.sz + p] = dq0[(5 + l) * zz->L0.sz + p] + dt * tot[5 + l] / diag;
+        }
+      }
+  memcpy(zz->dq, dqk, sizeof(double) * zz->L0.sz * nv);
+}
+
+/* ImplicitTreatmentHPP.cuh:7-52 (SST implicit terms not included) */
+void orc_zone_implicit(orc_zone *zz) {
+  const int ns = zz->ns;
+  if (zz->cfg.implicit_method == 1) {
+    const int64_t n = zz->L0.sz * zz->nv;
+    double *dq0 = malloc(sizeof(double) * n + 8), *dqk = malloc(sizeof(double) * n + 8);
+    dplur_dq0(zz, dq0);
+    for (int it = 0; it < zz->cfg.dplur_inner_step; ++it) {
+      wall_dq_zero(zz);
+      dplur_inner(zz, dq0, dqk);
+    }
+    /* DPLUR.cu:5-17 convert_dq_back_to_dqDt */
+    for (int k = 0; k < zz->mz; ++k)
+      for (int j = 0; j < zz->my; ++j)
+        for (int i = 0; i < zz->mx; ++i)
+          for (int l = 0; l < zz->nv; ++l) DQ(l, i, j, k) /= zz->dt_local[at(zz->L0, i, j, k)];
+    free(dq0);
+    free(dqk);
+    return;
+  }
+  if (!is_fr(zz)) return;
+  for (int k = 0; k < zz->mz; ++k)
+    for (int j = 0; j < zz->my; ++j)
+      for (int i = 0; i < zz->mx; ++i) {
+        const int64_t p = at(zz->L0, i, j, k);
+        const double dt = zz->dt_local[p];
+        if (zz->cfg.chemSrcMethod == 1) epi_in_place(zz, i, j, k, 1.0, dt);
+        if (zz->cfg.chemSrcMethod == 2) /* FiniteRateChem.cu:420-432 DA */
+          for (int l = 0; l < ns; ++l) DQ(5 + l, i, j, k) /= 1 - dt * zz->chem_jac[l * zz->L0.sz + p];
+      }
+}
+
+/* FieldOperation.cuh:221-296 update_bv: local-dt update of the primitives */
+static void update_bv(orc_zone *zz) {
+  for (int k = 0; k < zz->mz; ++k)
+    for (int j = 0; j < zz->my; ++j)
+      for (int i = 0; i < zz->mx; ++i) {
+        const int64_t pg = at(zz->Lg, i, j, k);
+        const double rho_n = BV(0, i, j, k);
+        /* compute_total_energy_1_point from the old primitives */
+        const double u0 = BV(1, i, j, k), v0 = BV(2, i, j, k), w0 = BV(3, i, j, k);
+        double E_n = 0.5 * (u0 * u0 + v0 * v0 + w0 * w0);
+        if (zz->cfg.mix) {
+          double h[ORC_MAXS];
+          orc_enthalpy(BV(5, i, j, k), &zz->mech, h);
+          for (int l = 0; l < zz->ns; ++l) E_n += h[l] * SV(l, i, j, k);
+          E_n = E_n * rho_n - BV(4, i, j, k);
+        } else {
+          E_n = E_n * rho_n + BV(4, i, j, k) / (GAMMA_AIR - 1);
+        }
+        const double dtj = zz->dt_local[at(zz->L0, i, j, k)] / zz->jac[pg];
+        BV(0, i, j, k) += DQ(0, i, j, k) * dtj;
+        const double ri = 1.0 / BV(0, i, j, k);
+        BV(1, i, j, k) = ri * (rho_n * BV(1, i, j, k) + DQ(1, i, j, k) * dtj);
+        BV(2, i, j, k) = ri * (rho_n * BV(2, i, j, k) + DQ(2, i, j, k) * dtj);
+        if (zz->mz == 1)
+          BV(3, i, j, k) = 0;
+        else
+          BV(3, i, j, k) = ri * (rho_n * BV(3, i, j, k) + DQ(3, i, j, k) * dtj);
+        const double E = E_n + DQ(4, i, j, k) * dtj;
+        for (int l = 0; l < zz->nsc; ++l) SV(l, i, j, k) = ri * (rho_n * SV(l, i, j, k) + DQ(5 + l, i, j, k) * dtj);
+        const double V2 = BV(1, i, j, k) * BV(1, i, j, k) + BV(2, i, j, k) * BV(2, i, j, k) +
+                          BV(3, i, j, k) * BV(3, i, j, k);
+        if (zz->cfg.mix) {
+          double Y[ORC_MAXS], p;
+          for (int l = 0; l < zz->ns; ++l) Y[l] = SV(l, i, j, k);
+          const double v3[3] = {BV(1, i, j, k), BV(2, i, j, k), BV(3, i, j, k)};
+          const double t = orc_temperature_newton(&zz->mech, BV(0, i, j, k), E, v3, Y, BV(5, i, j, k), &p, NULL);
+          BV(5, i, j, k) = t;
+          BV(4, i, j, k) = p;
+        } else {
+          BV(4, i, j, k) = (GAMMA_AIR - 1) * (E - 0.5 * BV(0, i, j, k) * V2);
+          BV(5, i, j, k) = BV(4, i, j, k) * MW_AIR * ri / R_U;
+        }
+        zz->vel[pg] = sqrt(V2);
+      }
+}
+
+/* SteadySim.cuh:52-124 (single block, no data communication) */
+void orc_zone_run_steady(orc_zone *zz, int nsteps, double *res_hist) {
+  const int os = zz->cfg.output_screen > 0 ? zz->cfg.output_screen : 1;
+  if (zz->step == 0) store_last_step(zz);
+  for (int it = 0; it < nsteps; ++it) {
+    const int step = ++zz->step;
+    if (step % os == 0) store_last_step(zz);
+    orc_zone_rhs(zz);
+    (void)orc_zone_global_dt(zz);
+    orc_zone_implicit(zz);
+    update_bv(zz);
+    apply_bcs(zz, 0);
+    if (zz->dim == 2) eliminate_k_gradient(zz, 0);
+    update_properties(zz);
+    if (step % os == 0 || step == 1) {
+      const double err = residual(zz, step);
+      if (res_hist) {
+        for (int l = 0; l < 4; ++l) res_hist[it * 5 + l] = zz->res[l];
+        res_hist[it * 5 + 4] = err;
+      }
+    } else if (res_hist) {
+      for (int l = 0; l < 5; ++l) res_hist[it * 5 + l] = NAN;
+    }
+  }
+}
+
 /* air face flux: pl/pr = (rho,u,v,w,p) on each side, met6 = metric row of
  * the sweep direction at the left and right points, jac2 their Jacobians */
 void orc_face_flux_air(int scheme, const double *pl5, const double *pr5, const double *met6, const double *jac2,

@@ -101,6 +101,8 @@ typedef struct {
   int fixed_time_step;
   int output_screen;
   int gradPInDiffusionFlux;
+  int implicit_method;        /* steady path: 0 explicit (+ EPI/DA chemistry), 1 DPLUR */
+  int dplur_inner_step;
   double cfl, Pr, Sc, Prt, Sct, dt, entropy_fix_factor;
   int n_bc;
   orc_bc bc[ORC_MAXBC];
@@ -126,6 +128,13 @@ double orc_zone_global_dt(orc_zone *zz);
 /* nsteps full RK3 steps (RK.cuh:89-193); res_hist gets 4 residuals + err_max per step (may be NULL) */
 void orc_zone_run_rk3(orc_zone *zz, int nsteps, double *dt_hist, double *res_hist);
 void orc_zone_state(orc_zone *zz, double *t_phys, int *step);
+
+/* nsteps steady iterations with local time steps (SteadySim.cuh:52-124):
+ * res_hist gets 4 residuals + err_max per step (NaN where not computed) */
+void orc_zone_run_steady(orc_zone *zz, int nsteps, double *res_hist);
+/* the implicit treatment alone on the current dq (ImplicitTreatmentHPP.cuh:7-52);
+ * dt_local / spectral radii must be current */
+void orc_zone_implicit(orc_zone *zz);
 
 /* grid metrics on their own (Mesh.cpp:259-521, 790-944): x,y,z interior in, outputs ghosted */
 void orc_grid_metrics(int mx, int my, int mz, int ngg, const double *x, const double *y, const double *z,

@@ -57,7 +57,7 @@ class Cfg(C.Structure):
     _fields_ = [("mix", C.c_int), ("reaction", C.c_int), ("chemSrcMethod", C.c_int), ("inviscid_scheme", C.c_int),
                 ("reconstruction", C.c_int), ("limiter", C.c_int), ("viscous_order", C.c_int),
                 ("fixed_time_step", C.c_int), ("output_screen", C.c_int), ("gradPInDiffusionFlux", C.c_int),
-                ("cfl", C.c_double), ("Pr", C.c_double), ("Sc", C.c_double), ("Prt", C.c_double),
+                ("implicit_method", C.c_int), ("dplur_inner_step", C.c_int), ("cfl", C.c_double), ("Pr", C.c_double), ("Sc", C.c_double), ("Prt", C.c_double),
                 ("Sct", C.c_double), ("dt", C.c_double), ("entropy_fix_factor", C.c_double),
                 ("n_bc", C.c_int), ("bc", BC * MAXBC)]
 
@@ -99,6 +99,8 @@ def lib():
         L.orc_zone_global_dt.argtypes = [C.c_void_p]
         L.orc_zone_global_dt.restype = C.c_double
         L.orc_zone_run_rk3.argtypes = [C.c_void_p, C.c_int, dp, dp]
+        L.orc_zone_run_steady.argtypes = [C.c_void_p, C.c_int, dp]
+        L.orc_zone_implicit.argtypes = [C.c_void_p]
         L.orc_zone_state.argtypes = [C.c_void_p, dp, C.POINTER(C.c_int)]
         L.orc_grid_metrics.argtypes = [C.c_int] * 4 + [dp] * 7
         _lib = L
@@ -298,6 +300,8 @@ def make_cfg(case):
     c.fixed_time_step = case.get("fixed_time_step", 0)
     c.output_screen = case.get("output_screen", 1)
     c.gradPInDiffusionFlux = case.get("gradPInDiffusionFlux", 0)
+    c.implicit_method = case.get("implicit_method", 0)
+    c.dplur_inner_step = case.get("DPLUR_inner_step", 3)
     c.cfl = case.get("cfl", 0.5)
     c.Pr = case.get("prandtl_number", 0.72)
     c.Sc = case.get("schmidt_number", 0.9)
@@ -346,6 +350,11 @@ class Zone:
 
     def initialize(self):
         lib().orc_zone_initialize(self.h)
+
+    def run_steady(self, n):
+        res = np.zeros((n, 5))
+        lib().orc_zone_run_steady(self.h, n, _p(res))
+        return res
 
     def rhs(self):
         lib().orc_zone_rhs(self.h)
