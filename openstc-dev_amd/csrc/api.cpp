@@ -225,6 +225,107 @@ int make_inflow(ostc_solver *s, const std::string &name, Inflow &in) {
   return 0;
 }
 
+// BoundCond.cpp:184-297 FarField::FarField: (mach, pressure, temperature),
+// (density, velocity, pressure) or (mach, temperature, Reynolds number);
+// also a, gamma and the entropy p / rho^gamma of the far state.
+int make_farfield(ostc_solver *s, const std::string &name, BCDesc &d) {
+  auto it = s->prm.st.find(name);
+  if (it == s->prm.st.end()) return fail("setup struct '" + name + "' not found");
+  const Struct &info = it->second;
+  auto get_r = [&](const char *k, double &v) {
+    auto f = info.find(k);
+    if (f == info.end()) return false;
+    if (std::holds_alternative<double>(f->second))
+      v = std::get<double>(f->second);
+    else if (std::holds_alternative<int>(f->second))
+      v = std::get<int>(f->second);
+    return true;
+  };
+  double mach = -1, pressure = -1, temperature = -1, velocity = 0, density = -1, u = 1, v = 0, w = 0, re = -1;
+  get_r("mach", mach);
+  get_r("pressure", pressure);
+  get_r("temperature", temperature);
+  get_r("velocity", velocity);
+  get_r("density", density);
+  get_r("u", u);
+  get_r("v", v);
+  get_r("w", w);
+  get_r("reynolds_number", re);
+  const int ns = s->n_spec;
+  std::vector<double> sv(std::max(s->n_scalar, 0), 0.0);
+  double mw = kMwAir;
+  if (ns > 0) {
+    for (int l = 0; l < ns; ++l) get_r(s->mech.names[l].c_str(), sv[l]);
+    double acc = 0;
+    for (int l = 0; l < ns; ++l) acc += sv[l] / s->mech.mw[l];
+    mw = 1 / acc;
+  }
+  if (temperature < 0) temperature = pressure * mw / (density * kRu);
+  double gamma = kGammaAir;
+  if (ns > 0) {
+    double cp = 0, cv = 0;
+    for (int l = 0; l < ns; ++l) {
+      const double tt = temperature < s->mech.t_low[l] ? s->mech.t_low[l] : temperature;
+      const double *c = (temperature < s->mech.t_low[l] || tt < s->mech.t_mid[l]) ? &s->mech.lo[l * 7]
+                                                                                  : &s->mech.hi[l * 7];
+      const double cpi = (c[0] + c[1] * tt + c[2] * tt * tt + c[3] * tt * tt * tt + c[4] * tt * tt * tt * tt) * kRu /
+                         s->mech.mw[l];
+      cp += sv[l] * cpi;
+      cv += sv[l] * (cpi - kRu / s->mech.mw[l]);
+    }
+    gamma = cp / cv;
+  }
+  const double a = std::sqrt(gamma * kRu / mw * temperature);
+  if (mach < 0)
+    mach = velocity / a;
+  else
+    velocity = mach * a;
+  u *= velocity;
+  v *= velocity;
+  w *= velocity;
+  if (pressure < 0) {
+    if (ns > 0) return fail("farfield '" + name + "': the Reynolds-number form is supported for air only");
+    const double mu = 1.716e-5 * std::pow(temperature / 273, 1.5) * (273 + 111) / (temperature + 111);
+    density = mu * re / velocity;
+    pressure = density * temperature * kRu / mw;
+  }
+  if (density < 0) density = pressure * mw / (kRu * temperature);
+  d.rho = density;
+  d.u = u;
+  d.v = v;
+  d.w = w;
+  d.p = pressure;
+  d.t = temperature;
+  d.vel = velocity;
+  for (size_t l = 0; l < sv.size() && l < (size_t)kMS + 4; ++l) d.sv[l] = sv[l];
+  d.a_inf = a;
+  d.gamma_inf = gamma;
+  d.entropy = pressure / std::pow(density, gamma);
+  d.mw_inf = mw;
+  return 0;
+}
+
+// p_inf and T_inf of the reference state (the inflow or farfield struct it names)
+int reference_pt(ostc_solver *s, double &p_inf, double &t_inf) {
+  const std::string ref = s->prm.get_string("reference_state", "");
+  auto it = s->prm.st.find(ref);
+  if (it == s->prm.st.end()) return fail("reference_state struct '" + ref + "' not found");
+  auto tf = it->second.find("type");
+  const std::string type = tf == it->second.end() ? "inflow" : std::get<std::string>(tf->second);
+  if (type == "farfield") {
+    BCDesc d{};
+    if (make_farfield(s, ref, d)) return 1;
+    p_inf = d.p;
+    t_inf = d.t;
+  } else {
+    Inflow in;
+    if (make_inflow(s, ref, in)) return 1;
+    p_inf = in.p;
+    t_inf = in.t;
+  }
+  return 0;
+}
+
 int upload_mech(ostc_solver *s) {
   DMech h{};
   const Mechanism &m = s->mech;
@@ -624,8 +725,8 @@ int ostc_finalize(ostc_solver *s) {
     }
   }
   for (auto &[t, l] : reg)
-    if (t == BC_FARFIELD || t == BC_SUBSONIC || t == BC_BACKPRESSURE)
-      return fail("boundary type " + std::to_string(t) + " is not supported by this build");
+    if ((t == BC_SUBSONIC || t == BC_BACKPRESSURE) && s->n_spec > 0)
+      return fail("subsonic inflow and back pressure boundaries are defined for air only (BoundCond.cpp:300-364)");
   // allocate zones
   const int g = s->ngg;
   int64_t max_cells = 0;
@@ -705,6 +806,52 @@ int ostc_finalize(ostc_solver *s) {
             d.t = in->t;
             d.vel = in->vel;
             for (size_t l = 0; l < in->sv.size() && l < (size_t)kMS + 4; ++l) d.sv[l] = in->sv[l];
+          } else if (t == BC_FARFIELD || t == BC_SUBSONIC || t == BC_BACKPRESSURE) {
+            std::string nm;
+            for (auto &[n2, v2] : s->prm.st) {
+              auto lf = v2.find("label");
+              if (lf != v2.end() && std::holds_alternative<int>(lf->second) && std::get<int>(lf->second) == rl) nm = n2;
+            }
+            if (nm.empty()) return fail("no setup struct for boundary label " + std::to_string(rl));
+            const Struct &info = s->prm.st.at(nm);
+            auto get_r = [&](const char *k, double &v) {
+              auto f = info.find(k);
+              if (f == info.end()) return false;
+              if (std::holds_alternative<double>(f->second))
+                v = std::get<double>(f->second);
+              else if (std::holds_alternative<int>(f->second))
+                v = std::get<int>(f->second);
+              return true;
+            };
+            if (t == BC_FARFIELD) {
+              if (make_farfield(s, nm, d)) return 1;
+            } else {
+              double p_inf = 0, t_inf = 0;
+              if (reference_pt(s, p_inf, t_inf)) return 1;
+              if (t == BC_SUBSONIC) {
+                // BoundCond.cpp:300-343
+                double ptr = 1, ttr = 1;
+                if (!get_r("pt_pRef_ratio", ptr) || !get_r("Tt_TRef_ratio", ttr))
+                  return fail("subsonic inflow '" + nm + "' needs pt_pRef_ratio and Tt_TRef_ratio");
+                d.u = 1;
+                d.v = 0;
+                d.w = 0;
+                get_r("u", d.u);
+                get_r("v", d.v);
+                get_r("w", d.w);
+                d.p_total = ptr * p_inf;
+                d.t_total = ttr * t_inf;
+              } else {
+                // BoundCond.cpp:345-364
+                double pb = -1, ratio = 1;
+                get_r("pressure", pb);
+                if (pb < 0) {
+                  get_r("p_pRef_ratio", ratio);
+                  pb = ratio * p_inf;
+                }
+                d.p_back = pb;
+              }
+            }
           } else if (t == BC_WALL) {
             d.t_wall = 300;
             d.isothermal = 1;

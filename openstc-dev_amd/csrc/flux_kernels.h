@@ -16,6 +16,8 @@ struct BCDesc {
   double sv[kMS + 4];
   double t_wall;
   int isothermal;
+  double a_inf, gamma_inf, entropy, mw_inf;  // farfield state (rho..vel, sv above)
+  double p_total, t_total, p_back;           // subsonic inflow (u, v, w: direction), back pressure
 };
 
 // per-step scratch kept on the device so a run needs no host round trip

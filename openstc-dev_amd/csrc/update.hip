@@ -309,6 +309,172 @@ __global__ void __launch_bounds__(kTB) k_bc(ZoneView z, DParams P, const DMech *
       if (g == 0) z.vel[pg] = b.vel;
       if (WITH_CV) cv_point<NS>(z, m, pg);
     }
+  } else if (b.type == 4) {
+    // farfield (BoundCond.cuh:213-424); never refreshes ghost cv
+    double nx = z.metric[(b.face * 3 + 0) * S + p], ny = z.metric[(b.face * 3 + 1) * S + p],
+           nz = z.metric[(b.face * 3 + 2) * S + p];
+    const double sc = b.dir / sqrt(nx * nx + ny * ny + nz * nz);
+    nx *= sc;
+    ny *= sc;
+    nz *= sc;
+    const double ub = bv[S + p], vb = bv[2 * S + p], wb = bv[3 * S + p];
+    const double uf = nx * ub + ny * vb + nz * wb;
+    double gb = kGam, mw = kMwAirD;
+    double svb[NS > 0 ? NS : 1];
+    if constexpr (NS > 0) {
+      double mi = 0;
+#pragma unroll
+      for (int l = 0; l < NS; ++l) {
+        svb[l] = sv[l * S + p];
+        mi += svb[l] / m->mw[l];
+      }
+      gb = z.gam[p];
+      mw = 1 / mi;
+    }
+    const double pb = bv[4 * S + p], rb = bv[p];
+    const double ab = sqrt(gb * pb / rb);
+    const double mb = uf / ab;
+    if (mb >= 1) {
+      for (int g = 1; g <= z.g; ++g) {
+        const int64_t pg = p + g * sd;
+#pragma unroll
+        for (int l = 0; l < 6; ++l) bv[l * S + pg] = bv[l * S + p];
+        for (int l = 0; l < P.nsc; ++l) sv[l * S + pg] = sv[l * S + p];
+      }
+      return;
+    }
+    double st[6];
+    bool far_sv = false;
+    if (mb <= -1) {
+      st[0] = b.rho;
+      st[1] = b.u;
+      st[2] = b.v;
+      st[3] = b.w;
+      st[4] = b.p;
+      st[5] = b.t;
+      far_sv = true;
+    } else {
+      const double rp = uf + 2 * ab / (gb - 1);
+      const double uinf = b.u * nx + b.v * ny + b.w * nz;
+      const double rn = uinf - 2 * b.a_inf / (b.gamma_inf - 1);
+      const double Un = 0.5 * (rp + rn);
+      double rho, pr, t, u, v, w;
+      if constexpr (NS == 0) {
+        const double cb = 0.25 * (kGam - 1) * (rp - rn);
+        double sb;
+        if (mb <= 0) {
+          sb = b.entropy;
+          u = b.u + (Un - uinf) * nx;
+          v = b.v + (Un - uinf) * ny;
+          w = b.w + (Un - uinf) * nz;
+          far_sv = true;
+        } else {
+          sb = pb / pow(rb, kGam);
+          u = ub + (Un - uf) * nx;
+          v = vb + (Un - uf) * ny;
+          w = wb + (Un - uf) * nz;
+        }
+        rho = pow(cb * cb / (kGam * sb), 1 / (kGam - 1));
+        pr = rho * cb * cb / kGam;
+        t = pr * kMwAirD / (rho * kRuD);
+      } else {
+        if (mb < 0) {
+          u = b.u + (Un - uinf) * nx;
+          v = b.v + (Un - uinf) * ny;
+          w = b.w + (Un - uinf) * nz;
+#pragma unroll
+          for (int l = 0; l < NS; ++l) svb[l] = b.sv[l];
+          far_sv = true;
+          mw = b.mw_inf;
+        } else {
+          u = ub + (Un - uf) * nx;
+          v = vb + (Un - uf) * ny;
+          w = wb + (Un - uf) * nz;
+        }
+        double gamma = kGam, err = 1;
+        while (err > 1e-4) {
+          const double glast = gamma;
+          const double cb = 0.25 * (gamma - 1) * (rp - rn);
+          const double sb = mb <= 0 ? b.entropy : pb / pow(rb, gamma);
+          rho = pow(cb * cb / (gamma * sb), 1 / (gamma - 1));
+          pr = rho * cb * cb / gamma;
+          t = pr * mw / (rho * kRuD);
+          double cpi[NS], cpt = 0;
+          d_cp<NS>(t, m, cpi);
+#pragma unroll
+          for (int l = 0; l < NS; ++l) cpt += cpi[l] * svb[l];
+          gamma = cpt / (cpt - kRuD / mw);
+          err = fabs(1 - gamma / glast);
+        }
+      }
+      st[0] = rho;
+      st[1] = u;
+      st[2] = v;
+      st[3] = w;
+      st[4] = pr;
+      st[5] = t;
+    }
+    for (int g = 0; g <= z.g; ++g) {
+      const int64_t pg = p + g * sd;
+#pragma unroll
+      for (int l = 0; l < 6; ++l) bv[l * S + pg] = st[l];
+      for (int l = 0; l < P.nsc; ++l) {
+        double y = far_sv ? b.sv[l] : sv[l * S + p];
+        if constexpr (NS > 0)
+          if (!far_sv && l < NS) y = svb[l];
+        sv[l * S + pg] = y;
+      }
+    }
+    z.vel[p] = sqrt(st[1] * st[1] + st[2] * st[2] + st[3] * st[3]);
+  } else if (b.type == 7) {
+    // subsonic inflow, air (BoundCond.cuh:554-636): ghosts only
+    double nx = z.metric[(b.face * 3 + 0) * S + p], ny = z.metric[(b.face * 3 + 1) * S + p],
+           nz = z.metric[(b.face * 3 + 2) * S + p];
+    const double sc = b.dir / sqrt(nx * nx + ny * ny + nz * nz);
+    nx *= sc;
+    ny *= sc;
+    nz *= sc;
+    const double ub = bv[S + p], vb = bv[2 * S + p], wb = bv[3 * S + p];
+    const double uf = nx * ub + ny * vb + nz * wb;
+    const double g1 = kGam - 1;
+    const double a = sqrt(kGam * bv[4 * S + p] / bv[p]);
+    const double rn = fabs(uf) - 2 * a / g1;
+    const double hti = bv[4 * S + p] / bv[p] * kGam / g1 + 0.5 * (ub * ub + vb * vb + wb * wb);
+    const double qa = 1 + 2.0 / g1, qb = 2 * rn, qc = g1 * (0.5 * rn * rn - hti);
+    const double dl = qb * qb - 4 * qa * qc;
+    double an = a;
+    if (dl >= 0) {
+      const double ap = (-qb + sqrt(dl)) / (2 * qa), am = (-qb - sqrt(dl)) / (2 * qa);
+      an = ap > am ? ap : am;
+    }
+    const double un = rn + 2 * an / g1;
+    const double M = un / an;
+    const double pr = b.p_total * pow(1 + 0.5 * g1 * M * M, -kGam / g1);
+    const double t = b.t_total * pow(pr / b.p_total, g1 / kGam);
+    const double rho = pr * kMwAirD / (t * kRuD);
+    for (int g = 1; g <= z.g; ++g) {
+      const int64_t pg = p + g * sd;
+      bv[pg] = rho;
+      bv[S + pg] = un * b.u;
+      bv[2 * S + pg] = un * b.v;
+      bv[3 * S + pg] = un * b.w;
+      bv[4 * S + pg] = pr;
+      bv[5 * S + pg] = t;
+      const double ubar = bv[S + pg] * nx + bv[2 * S + pg] * ny + bv[3 * S + pg] * nz;
+      for (int l = 0; l < P.nsc; ++l) sv[l * S + pg] = ubar > 0 ? sv[l * S + p] : b.sv[l];
+      if (WITH_CV) cv_point<NS>(z, m, pg);
+    }
+  } else if (b.type == 9) {
+    // back pressure (BoundCond.cuh:638-670)
+    for (int g = 1; g <= z.g; ++g) {
+      const int64_t pg = p + g * sd;
+#pragma unroll
+      for (int l = 0; l < 4; ++l) bv[l * S + pg] = bv[l * S + p];
+      bv[4 * S + pg] = b.p_back;
+      bv[5 * S + pg] = bv[5 * S + p];
+      for (int l = 0; l < P.nsc; ++l) sv[l * S + pg] = sv[l * S + p];
+      if (WITH_CV) cv_point<NS>(z, m, pg);
+    }
   } else if (b.type == 2) {
     // no-slip wall (BoundCond.cuh:424-552)
     const int64_t pi = p - sd;

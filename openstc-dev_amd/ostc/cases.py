@@ -262,6 +262,52 @@ def c3_fields(case, states, mw):
     return bv, sv
 
 
+def subsonic_case(nx=60, ny=30, seed=1006, mach=0.5, p=101325.0, t=300.0, pt_ratio=None, tt_ratio=None,
+                  back_ratio=1.0):
+    """2-D subsonic air channel (spacing 1 mm), AUSM+ with MUSCL-minmod,
+    SSP-RK3 at CFL 0.5: subsonic inflow at i=0 (total pressure and
+    temperature as ratios of the far state, default the isentropic totals of
+    the Mach number), back pressure at i=max, farfield (Riemann invariants)
+    at j=0 and j=max; the far state (Mach `mach`, p, T) is the reference and
+    initial state, with the seeded 1 % density perturbation."""
+    dx = 1e-3
+    X, Y = np.meshgrid(np.arange(nx) * dx, np.arange(ny) * dx, indexing="xy")
+    x, y = X[None].copy(), Y[None].copy()
+    z = np.zeros_like(x)
+    if pt_ratio is None:
+        pt_ratio = (1 + 0.5 * (GAMMA_AIR - 1) * mach * mach) ** (GAMMA_AIR / (GAMMA_AIR - 1))
+    if tt_ratio is None:
+        tt_ratio = 1 + 0.5 * (GAMMA_AIR - 1) * mach * mach
+    inflow = air_inflow(mach, p, t)
+    setup = _setup_common({
+        "int dimension": 2, "real cfl": 0.5, "int inviscid_scheme": 3, "int reconstruction": 2, "int limiter": 0,
+        "int viscous_order": 0, "int total_step": 10,
+        "array string boundary_conditions": ["farfield", "inlet", "outlet"],
+        "string reference_state": "farfield", "string default_init": "farfield",
+        "struct farfield": [("string", "type", "farfield"), ("int", "label", 4), ("real", "mach", mach),
+                            ("real", "pressure", p), ("real", "temperature", t), ("real", "u", 1.0),
+                            ("real", "v", 0.0), ("real", "w", 0.0)],
+        "struct inlet": [("string", "type", "subsonic_inflow"), ("int", "label", 7),
+                         ("real", "pt_pRef_ratio", pt_ratio), ("real", "Tt_TRef_ratio", tt_ratio),
+                         ("real", "u", 1.0), ("real", "v", 0.0), ("real", "w", 0.0)],
+        "struct outlet": [("string", "type", "back_pressure"), ("int", "label", 9),
+                          ("real", "p_pRef_ratio", back_ratio)],
+    })
+    a = np.sqrt(GAMMA_AIR * R_U / MW_AIR * t)
+    far = dict(inflow, a_inf=a, gamma_inf=GAMMA_AIR, entropy=p / inflow["rho"] ** GAMMA_AIR, mw_inf=MW_AIR, sv=[])
+    bcs = [
+        dict(type=7, raw=(0, 0, 0, ny - 1, 0, 0), label=7,
+             state=dict(u=1.0, v=0.0, w=0.0, p_total=pt_ratio * p, t_total=tt_ratio * t, sv=[])),
+        dict(type=9, raw=(nx - 1, nx - 1, 0, ny - 1, 0, 0), label=9, state=dict(p_back=back_ratio * p, sv=[])),
+        dict(type=4, raw=(0, nx - 1, 0, 0, 0, 0), label=4, state=far),
+        dict(type=4, raw=(0, nx - 1, ny - 1, ny - 1, 0, 0), label=4, state=far),
+    ]
+    scheme = dict(mix=0, inviscid_scheme=3, reconstruction=2, limiter=0, viscous_order=0, cfl=0.5,
+                  fixed_time_step=0, output_screen=1)
+    return dict(name="subsonic", mx=nx, my=ny, mz=1, ngg=2, x=x, y=y, z=z, setup=setup, bcs=bcs, scheme=scheme,
+                inflow=inflow, seed=seed, perturb="density", steps=10)
+
+
 def initial_fields(case):
     """Uniform inflow everywhere (ghosts included), then the seeded interior
     density perturbation at fixed pressure.  Returns bv (6, ...) and sv

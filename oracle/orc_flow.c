@@ -294,11 +294,11 @@ orc_zone *orc_zone_create(int mx, int my, int mz, int ngg, const orc_cfg *cfg, c
       zz->re[b][d] = e[d];
     }
   }
-  /* application order: BoundCond.cuh:673-822 (outflow, inflow, symmetry, wall;
-   * file order within one type) */
-  const int prio[4] = {6, 5, 3, 2};
+  /* application order: BoundCond.cuh:673-822 (farfield, outflow, back pressure,
+   * inflow, subsonic inflow, symmetry, wall; file order within one type) */
+  const int prio[7] = {4, 6, 9, 5, 7, 3, 2};
   int n = 0;
-  for (int p = 0; p < 4; ++p)
+  for (int p = 0; p < 7; ++p)
     for (int b = 0; b < cfg->n_bc; ++b)
       if (cfg->bc[b].type == prio[p]) zz->order[n++] = b;
   zz->nbc = n;
@@ -467,6 +467,175 @@ static void bc_inflow(orc_zone *zz, const orc_bc *B, int b, int i, int j, int k,
 
 /* BoundCond.cuh:424-552 apply_wall (no-slip; isothermal ghost T = 2Tw - Ti,
  * clipped at 0.1 Tw; rho from p and T) */
+/* outward unit normal of a boundary face at (i,j,k): metric row of the face
+ * direction, scaled by the face direction */
+static void face_normal(const orc_zone *zz, int b, int i, int j, int k, double *n) {
+  const int64_t p = at(zz->Lg, i, j, k);
+  const double *m = &zz->metric[p * 9 + zz->bface[b] * 3];
+  const double s = zz->bdir[b] / sqrt(m[0] * m[0] + m[1] * m[1] + m[2] * m[2]);
+  n[0] = m[0] * s;
+  n[1] = m[1] * s;
+  n[2] = m[2] * s;
+}
+
+static void set_point(orc_zone *zz, int i, int j, int k, const double *b6, const double *svv) {
+  for (int l = 0; l < 6; ++l) BV(l, i, j, k) = b6[l];
+  for (int l = 0; l < zz->nsc; ++l) SV(l, i, j, k) = svv[l];
+}
+
+/* BoundCond.cuh:213-424 apply_farfield: supersonic in/outflow by the normal
+ * Mach number of the boundary point, otherwise Riemann invariants with the
+ * entropy of the far state (inflow) or of the boundary point (outflow);
+ * mixtures iterate gamma(T) to 1e-4.  Ghost cv is never refreshed. */
+static void bc_farfield(orc_zone *zz, const orc_bc *B, int b, int i, int j, int k) {
+  int dir[3] = {0, 0, 0};
+  dir[zz->bface[b]] = zz->bdir[b];
+  double n[3];
+  face_normal(zz, b, i, j, k, n);
+  const double ub = BV(1, i, j, k), vb = BV(2, i, j, k), wb = BV(3, i, j, k);
+  const double uf = n[0] * ub + n[1] * vb + n[2] * wb;
+  const int ns = zz->ns, nsc = zz->nsc;
+  double gb = GAMMA_AIR, mw = MW_AIR, svb[ORC_MAXS + 4];
+  if (zz->cfg.mix) {
+    for (int l = 0; l < nsc; ++l) svb[l] = SV(l, i, j, k);
+    gb = S1(gam, i, j, k);
+    double mi = 0;
+    for (int l = 0; l < ns; ++l) mi += svb[l] / zz->mech.mw[l];
+    mw = 1.0 / mi;
+  }
+  const double pb = BV(4, i, j, k), rb = BV(0, i, j, k);
+  const double ab = sqrt(gb * pb / rb);
+  const double mb = uf / ab;
+  double st[6];
+  if (mb <= -1) {
+    const double far[6] = {B->rho, B->u, B->v, B->w, B->p, B->t};
+    set_point(zz, i, j, k, far, B->sv);
+    S1(vel, i, j, k) = sqrt(B->u * B->u + B->v * B->v + B->w * B->w);
+    for (int g = 1; g <= zz->g; ++g) set_point(zz, i + g * dir[0], j + g * dir[1], k + g * dir[2], far, B->sv);
+    return;
+  }
+  if (mb >= 1) {
+    for (int g = 1; g <= zz->g; ++g) {
+      const int gi = i + g * dir[0], gj = j + g * dir[1], gk = k + g * dir[2];
+      for (int l = 0; l < 6; ++l) BV(l, gi, gj, gk) = BV(l, i, j, k);
+      for (int l = 0; l < nsc; ++l) SV(l, gi, gj, gk) = SV(l, i, j, k);
+    }
+    return;
+  }
+  const double rp = uf + 2 * ab / (gb - 1);
+  const double uinf = B->u * n[0] + B->v * n[1] + B->w * n[2];
+  const double rn = uinf - 2 * B->a_inf / (B->gamma_inf - 1);
+  const double Un = 0.5 * (rp + rn);
+  double sb, rho, p, t, u, v, w;
+  if (!zz->cfg.mix) {
+    const double cb = 0.25 * (GAMMA_AIR - 1) * (rp - rn);
+    if (mb <= 0) {
+      sb = B->entropy;
+      u = B->u + (Un - uinf) * n[0];
+      v = B->v + (Un - uinf) * n[1];
+      w = B->w + (Un - uinf) * n[2];
+      for (int l = 0; l < nsc; ++l) svb[l] = B->sv[l];
+    } else {
+      sb = pb / pow(rb, GAMMA_AIR);
+      u = ub + (Un - uf) * n[0];
+      v = vb + (Un - uf) * n[1];
+      w = wb + (Un - uf) * n[2];
+    }
+    rho = pow(cb * cb / (GAMMA_AIR * sb), 1 / (GAMMA_AIR - 1));
+    p = rho * cb * cb / GAMMA_AIR;
+    t = p * MW_AIR / (rho * R_U);
+  } else {
+    if (mb < 0) {
+      u = B->u + (Un - uinf) * n[0];
+      v = B->v + (Un - uinf) * n[1];
+      w = B->w + (Un - uinf) * n[2];
+      for (int l = 0; l < nsc; ++l) svb[l] = B->sv[l];
+      mw = B->mw_inf;
+    } else {
+      u = ub + (Un - uf) * n[0];
+      v = vb + (Un - uf) * n[1];
+      w = wb + (Un - uf) * n[2];
+    }
+    double gamma = GAMMA_AIR, err = 1, glast;
+    while (err > 1e-4) {
+      glast = gamma;
+      const double cb = 0.25 * (gamma - 1) * (rp - rn);
+      sb = mb <= 0 ? B->entropy : pb / pow(rb, gamma);
+      rho = pow(cb * cb / (gamma * sb), 1 / (gamma - 1));
+      p = rho * cb * cb / gamma;
+      t = p * mw / (rho * R_U);
+      double cpi[ORC_MAXS], cpt = 0;
+      orc_cp(t, &zz->mech, cpi);
+      for (int l = 0; l < ns; ++l) cpt += cpi[l] * svb[l];
+      gamma = cpt / (cpt - R_U / mw);
+      err = fabs(1 - gamma / glast);
+    }
+  }
+  st[0] = rho;
+  st[1] = u;
+  st[2] = v;
+  st[3] = w;
+  st[4] = p;
+  st[5] = t;
+  set_point(zz, i, j, k, st, svb);
+  S1(vel, i, j, k) = sqrt(u * u + v * v + w * w);
+  for (int g = 1; g <= zz->g; ++g) set_point(zz, i + g * dir[0], j + g * dir[1], k + g * dir[2], st, svb);
+}
+
+/* BoundCond.cuh:554-636 apply_subsonic_inflow (air): the outgoing Riemann
+ * invariant and total enthalpy of the boundary point fix the speed of sound
+ * (larger root), total pressure / temperature the state; ghosts only. */
+static void bc_subsonic(orc_zone *zz, const orc_bc *B, int b, int i, int j, int k, int with_cv) {
+  int dir[3] = {0, 0, 0};
+  dir[zz->bface[b]] = zz->bdir[b];
+  double n[3];
+  face_normal(zz, b, i, j, k, n);
+  const double ub = BV(1, i, j, k), vb = BV(2, i, j, k), wb = BV(3, i, j, k);
+  const double uf = n[0] * ub + n[1] * vb + n[2] * wb;
+  const double g1 = GAMMA_AIR - 1;
+  const double a = sqrt(GAMMA_AIR * BV(4, i, j, k) / BV(0, i, j, k));
+  const double rn = fabs(uf) - 2 * a / g1;
+  const double hti = BV(4, i, j, k) / BV(0, i, j, k) * GAMMA_AIR / g1 + 0.5 * (ub * ub + vb * vb + wb * wb);
+  const double qa = 1 + 2.0 / g1, qb = 2 * rn, qc = g1 * (0.5 * rn * rn - hti);
+  const double dl = qb * qb - 4 * qa * qc;
+  double an = a;
+  if (dl >= 0) {
+    const double ap = (-qb + sqrt(dl)) / (2 * qa), am = (-qb - sqrt(dl)) / (2 * qa);
+    an = ap > am ? ap : am;
+  }
+  const double un = rn + 2 * an / g1;
+  const double M = un / an;
+  const double p = B->p_total * pow(1 + 0.5 * g1 * M * M, -GAMMA_AIR / g1);
+  const double t = B->t_total * pow(p / B->p_total, g1 / GAMMA_AIR);
+  const double rho = p * MW_AIR / (t * R_U);
+  for (int g = 1; g <= zz->g; ++g) {
+    const int gi = i + g * dir[0], gj = j + g * dir[1], gk = k + g * dir[2];
+    BV(0, gi, gj, gk) = rho;
+    BV(1, gi, gj, gk) = un * B->u;
+    BV(2, gi, gj, gk) = un * B->v;
+    BV(3, gi, gj, gk) = un * B->w;
+    BV(4, gi, gj, gk) = p;
+    BV(5, gi, gj, gk) = t;
+    const double ubar = BV(1, gi, gj, gk) * n[0] + BV(2, gi, gj, gk) * n[1] + BV(3, gi, gj, gk) * n[2];
+    for (int l = 0; l < zz->nsc; ++l) SV(l, gi, gj, gk) = ubar > 0 ? SV(l, i, j, k) : B->sv[l];
+    if (with_cv) cv_from_bv_point(zz, gi, gj, gk);
+  }
+}
+
+/* BoundCond.cuh:638-670 apply_back_pressure: extrapolation with p fixed */
+static void bc_back_pressure(orc_zone *zz, const orc_bc *B, int b, int i, int j, int k, int with_cv) {
+  int dir[3] = {0, 0, 0};
+  dir[zz->bface[b]] = zz->bdir[b];
+  for (int g = 1; g <= zz->g; ++g) {
+    const int gi = i + g * dir[0], gj = j + g * dir[1], gk = k + g * dir[2];
+    for (int l = 0; l < 4; ++l) BV(l, gi, gj, gk) = BV(l, i, j, k);
+    BV(4, gi, gj, gk) = B->p_back;
+    BV(5, gi, gj, gk) = BV(5, i, j, k);
+    for (int l = 0; l < zz->nsc; ++l) SV(l, gi, gj, gk) = SV(l, i, j, k);
+    if (with_cv) cv_from_bv_point(zz, gi, gj, gk);
+  }
+}
+
 static void bc_wall(orc_zone *zz, const orc_bc *B, int b, int i, int j, int k, int with_cv) {
   int dir[3] = {0, 0, 0};
   dir[zz->bface[b]] = zz->bdir[b];
@@ -535,6 +704,9 @@ static void apply_bcs(orc_zone *zz, int with_cv) {
             case 5: bc_inflow(zz, B, b, i, j, k, with_cv); break;
             case 6: bc_outflow(zz, b, i, j, k, with_cv); break;
             case 2: bc_wall(zz, B, b, i, j, k, with_cv); break;
+            case 4: bc_farfield(zz, B, b, i, j, k); break;
+            case 7: bc_subsonic(zz, B, b, i, j, k, with_cv); break;
+            case 9: bc_back_pressure(zz, B, b, i, j, k, with_cv); break;
             default: break;
           }
         }

@@ -80,7 +80,8 @@ void orc_chem_batch(const orc_mech *m, int64_t n, const double *rho, const doubl
 
 /* ---- flow solver ---- */
 typedef struct {
-  int type;                   /* 2 wall, 3 symmetry, 5 inflow, 6 outflow */
+  int type;                   /* 2 wall, 3 symmetry, 4 farfield, 5 inflow, 6 outflow, 7 subsonic inflow,
+                                 9 back pressure */
   int raw[6];                 /* i1 i2 j1 j2 k1 k2 as in boundary file (0-based) */
   /* inflow state */
   double rho, u, v, w, p, t, vel;
@@ -88,6 +89,10 @@ typedef struct {
   /* wall */
   double t_wall;
   int isothermal;
+  /* farfield: a, gamma, entropy p / rho^gamma, mw of the far state (rho..vel, sv above) */
+  double a_inf, gamma_inf, entropy, mw_inf;
+  /* subsonic inflow: total pressure and temperature (u, v, w above: direction); back pressure */
+  double p_total, t_total, p_back;
 } orc_bc;
 
 typedef struct {

@@ -50,7 +50,9 @@ class BC(C.Structure):
     _fields_ = [("type", C.c_int), ("raw", C.c_int * 6),
                 ("rho", C.c_double), ("u", C.c_double), ("v", C.c_double), ("w", C.c_double), ("p", C.c_double),
                 ("t", C.c_double), ("vel", C.c_double), ("sv", C.c_double * (MAXS + 4)),
-                ("t_wall", C.c_double), ("isothermal", C.c_int)]
+                ("t_wall", C.c_double), ("isothermal", C.c_int),
+                ("a_inf", C.c_double), ("gamma_inf", C.c_double), ("entropy", C.c_double), ("mw_inf", C.c_double),
+                ("p_total", C.c_double), ("t_total", C.c_double), ("p_back", C.c_double)]
 
 
 class Cfg(C.Structure):
@@ -316,8 +318,12 @@ def make_cfg(case):
         B.raw[:] = list(b["raw"])
         st = b.get("state")
         if st is not None:
-            B.rho, B.u, B.v, B.w, B.p, B.t, B.vel = (st[k] for k in ("rho", "u", "v", "w", "p", "t", "vel"))
-            B.sv[:len(st["sv"])] = list(st["sv"])
+            for key in ("rho", "u", "v", "w", "p", "t", "vel", "a_inf", "gamma_inf", "entropy", "mw_inf", "p_total",
+                        "t_total", "p_back"):
+                if key in st:
+                    setattr(B, key, st[key])
+            if "sv" in st:
+                B.sv[:len(st["sv"])] = list(st["sv"])
         B.t_wall = b.get("t_wall", 300.0)
         B.isothermal = 1 if b.get("isothermal", True) else 0
     return c

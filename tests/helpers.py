@@ -13,6 +13,8 @@ def oracle_zone(case, mech=None, bv=None, sv=None):
         if b["type"] == 5:
             st = b.get("state", case.get("inflow"))
             d["state"] = dict(st, sv=list(st.get("sv", [])))
+        elif "state" in b:
+            d["state"] = dict(b["state"], sv=list(b["state"].get("sv", [])))
         if b["type"] == 2:
             d["t_wall"] = b.get("t_wall", 300.0)
             d["isothermal"] = b.get("isothermal", True)

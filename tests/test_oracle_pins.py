@@ -500,3 +500,31 @@ def test_cartesian_metrics_closed_form():
     np.testing.assert_allclose(m[:, 4], 1 / dy, rtol=1e-12)
     np.testing.assert_allclose(m[:, 8], 1 / dz, rtol=1e-12)
     np.testing.assert_allclose(m[:, [1, 2, 3, 5, 6, 7]], 0, atol=1e-9)
+
+
+# ----------------------------------------------------------- boundaries
+
+def test_farfield_and_subsonic_inflow_keep_a_uniform_far_state():
+    """With the interior at the far state, the Riemann-invariant farfield and
+    the total-pressure subsonic inflow reproduce that state in their ghost
+    layers (the invariants and totals are consistent), and the back-pressure
+    ghosts carry the prescribed pressure."""
+    import helpers
+    from ostc import cases
+    case = cases.subsonic_case(20, 12, back_ratio=0.97)
+    case["perturb"] = None
+    z = helpers.oracle_zone(case)
+    g = case["ngg"]
+    bv = z.field("bv").reshape(6, 1 + 2 * g, 12 + 2 * g, 20 + 2 * g)
+    far = case["inflow"]
+    ref = [far["rho"], far["u"], far["v"], far["w"], far["p"], far["t"]]
+    k = g
+    # farfield ghosts (j < 0 and j >= my) and subsonic-inflow ghosts (i < 0)
+    for l in range(6):
+        scale = max(abs(ref[l]), 1.0)
+        assert np.max(np.abs(bv[l, k, :g, g:-g] - ref[l])) < 1e-12 * scale
+        assert np.max(np.abs(bv[l, k, -g:, g:-g] - ref[l])) < 1e-12 * scale
+        assert np.max(np.abs(bv[l, k, g:-g, :g] - ref[l])) < 1e-10 * scale
+    # back-pressure ghosts (i >= mx): pressure prescribed, the rest extrapolated
+    assert np.allclose(bv[4, k, g:-g, -g:], 0.97 * far["p"], rtol=1e-14)
+    assert np.allclose(bv[0, k, g:-g, -g:], far["rho"], rtol=1e-14)
