@@ -121,6 +121,8 @@ def main():
     ap.add_argument("--chem-cells", type=int, default=1 << 22)
     ap.add_argument("--train-pairs", type=int, default=1 << 24)
     ap.add_argument("--epochs", type=int, default=16)
+    ap.add_argument("--calibrate-copy", type=int, default=0,
+                    help="bytes of one device-to-device copy run at the end (HBM counter calibration)")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build runs on one GPU; multi-GPU sharding is described in DESIGN.md "
@@ -215,12 +217,13 @@ def main():
                     v.free()
                 log("bench: surrogate inference")
                 sc.chem_batch(n, dr, dT_, dY_, 1e-7, oY, oT, mode=1)
-                net_ms, fb_ms, nfs = [], [], []
+                net_ms, gate_ms, fb_ms, nfs = [], [], [], []
                 t8 = time.perf_counter()
                 for _ in range(reps):
                     nf = sc.chem_batch(n, dr, dT_, dY_, 1e-7, oY, oT, mode=1)
                     st = sc.surrogate_last_stats()
                     net_ms.append(st["net_ms"])
+                    gate_ms.append(st["gate_ms"])
                     fb_ms.append(st["fallback_ms"])
                     nfs.append(nf)
                 sc.sync()
@@ -232,17 +235,29 @@ def main():
                 achieved = SURROGATE_FLOP_PER_CELL * n / (net * 1e-3) / 1e12
                 sur = dict(cells=n, cells_per_s=n * reps / (t9 - t8),
                            speedup_vs_fp64=(n * reps / (t9 - t8)) / chem["cells_per_s"],
-                           fallback_fraction=nfs[-1] / n, network_and_gate_ms=net, fallback_ms=float(np.mean(fb_ms)),
+                           fallback_fraction=nfs[-1] / n, network_ms=net, gate_ms=float(np.mean(gate_ms)),
+                           fallback_ms=float(np.mean(fb_ms)),
                            major_species_linf=float(err[[0, 2, 7, 8]].max()), all_species_linf=float(err.max()),
                            T_rel_linf=float(np.max(np.abs(Tn1 - Tn0) / Tn0)),
                            peak_T_rel_err=float(abs(Tn1.max() - Tn0.max()) / Tn0.max()),
                            train_pairs=a.train_pairs, epochs=a.epochs, train_s=t7 - t6, final_loss=float(hist[-1]),
                            gate=sc.surrogate_gate(),
-                           roofline={"bound": "mfma", "kernel": "k_sur_fused<128>", "achieved": achieved,
+                           roofline={"bound": "mfma", "kernel": "k_sur_net<128, 1>", "achieved": achieved,
                                      "peak": BF16_MFMA_PEAK_TFLOPS, "unit": "TFLOP/s",
                                      "frac": achieved / BF16_MFMA_PEAK_TFLOPS,
                                      "flop_per_cell": SURROGATE_FLOP_PER_CELL})
             sc.close()
+
+        if a.calibrate_copy > 0:
+            # one device-to-device copy of known size, for scaling the
+            # FETCH_SIZE / WRITE_SIZE counters of a profiled run
+            cc = cases.c1_case(20, 12)
+            root = os.path.join(d, "copy")
+            cases.write_case_dir(cc, root)
+            sk = ostc.Solver(root, 0)
+            src, dst = ostc.DeviceArray(sk, a.calibrate_copy), ostc.DeviceArray(sk, a.calibrate_copy)
+            ostc.lib().ostc_memcpy_d2d(sk.h, dst.ptr, src.ptr, a.calibrate_copy)
+            sk.close()
 
     ms_per_step = fp64_s / a.steps * 1e3
     value = cells * a.steps / fp64_s / 1e6

@@ -212,7 +212,8 @@ int ostc_surrogate_set_kernel(ostc_solver *s, int kernel);
 /* Standardised network outputs out[n x 8] (host) of n device states, without the gate. */
 int ostc_surrogate_network_outputs(ostc_solver *s, int64_t n, const double *rho, const double *T, const double *Y,
                                    float *host_out);
-/* Last surrogate chem batch: out[3] = fallback cells, network+gate ms, fallback ms. */
+/* Last surrogate chem batch: out[4] = fallback cells, network ms (features + MLP),
+ * gate ms, FP64 fallback ms (HIP events on the solver stream). */
 int ostc_surrogate_last_stats(ostc_solver *s, double *out);
 
 /* Device memory helpers for callers without their own allocator. */
@@ -220,6 +221,7 @@ int ostc_dev_alloc(ostc_solver *s, int64_t bytes, void **dptr);
 int ostc_dev_free(ostc_solver *s, void *dptr);
 int ostc_memcpy_h2d(ostc_solver *s, void *dst, const void *src, int64_t bytes);
 int ostc_memcpy_d2h(ostc_solver *s, void *dst, const void *src, int64_t bytes);
+int ostc_memcpy_d2d(ostc_solver *s, void *dst, const void *src, int64_t bytes);
 int ostc_sync(ostc_solver *s);
 
 /* Kernel timing with HIP events on the solver stream (0 off, 1 on) and a

@@ -1385,6 +1385,13 @@ int ostc_memcpy_d2h(ostc_solver *s, void *dst, const void *src, int64_t bytes) {
   return 0;
 }
 
+int ostc_memcpy_d2d(ostc_solver *s, void *dst, const void *src, int64_t bytes) {
+  if (!s) return fail("null solver");
+  HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, s->stream));
+  HIPCHK(hipStreamSynchronize(s->stream));
+  return 0;
+}
+
 int ostc_sync(ostc_solver *s) {
   if (!s) return fail("null solver");
   HIPCHK(hipStreamSynchronize(s->stream));

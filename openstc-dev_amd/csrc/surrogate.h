@@ -87,8 +87,8 @@ void surrogate_set_kernel(SurrogateState *s, int kernel);
 // raw (standardised) network outputs [n x 8] of device states, copied to the host
 int surrogate_network_outputs(SurrogateState *s, const DMech *m, int64_t n, const double *rho, const double *T,
                               const double *Y, float *host_out, hipStream_t st, std::string &err);
-// fallback count, network+gate ms and fallback ms of the last inference
-int surrogate_last_stats(const SurrogateState *s, double *out3);
+// fallback count, network ms (features + MLP), gate ms and fallback ms of the last inference
+int surrogate_last_stats(const SurrogateState *s, double *out4);
 int surrogate_infer(SurrogateState *s, const DMech *m, int64_t n, const double *rho, const double *T,
                     const double *Y, double dt, double *dY, double *dT, int64_t *n_fallback, hipStream_t stream,
                     std::string &err);

@@ -451,8 +451,8 @@ struct SurrogateState {
   int *fb_idx = nullptr, *fb_count = nullptr;
   float *fb_out = nullptr;  // network outputs [fcap x 8] of the fused path
   int kernel = 1;  // inference: 1 fused network (128 cells per workgroup), 2 (64), 0 layer-by-layer GEMMs
-  hipEvent_t ev[3] = {};
-  double last_ms[2] = {0, 0};
+  hipEvent_t ev[4] = {};
+  double last_ms[3] = {0, 0, 0};  // network, gate, fallback
   int64_t last_fallback = 0;
 
   void free_train() {
@@ -1061,6 +1061,7 @@ int surrogate_infer(SurrogateState *s, const DMech *m, int64_t n, const double *
   SCHK(hipMemsetAsync(s->fb_count, 0, sizeof(int), st));
   if (s->kernel != 0) {
     if (!force && launch_net(s, m, n, rho, T, Y, s->fb_out, st, err)) return 1;
+    SCHK(hipEventRecord(s->ev[3], st));
     hipLaunchKernelGGL(k_sur_finish, dim3(nb256(n)), dim3(256), 0, st, m, s->norm, n, (int64_t)0, n, rho, T, Y,
                        (const float *)s->fb_out, kSurOut, force, dY, dT, s->fb_count, s->fb_idx);
     SCHK(hipGetLastError());
@@ -1089,11 +1090,17 @@ int surrogate_infer(SurrogateState *s, const DMech *m, int64_t n, const double *
   SCHK(launch_chem_batch_subset(m, n, nf, s->fb_idx, rho, T, Y, dt, dY, dT, st));
   SCHK(hipEventRecord(s->ev[2], st));
   SCHK(hipEventSynchronize(s->ev[2]));
-  float a = 0, b = 0;
-  SCHK(hipEventElapsedTime(&a, s->ev[0], s->ev[1]));
+  float a = 0, b = 0, c = 0;
+  if (s->kernel != 0) {
+    SCHK(hipEventElapsedTime(&a, s->ev[0], s->ev[3]));
+    SCHK(hipEventElapsedTime(&c, s->ev[3], s->ev[1]));
+  } else {
+    SCHK(hipEventElapsedTime(&a, s->ev[0], s->ev[1]));
+  }
   SCHK(hipEventElapsedTime(&b, s->ev[1], s->ev[2]));
   s->last_ms[0] = a;
-  s->last_ms[1] = b;
+  s->last_ms[1] = c;
+  s->last_ms[2] = b;
   s->last_fallback = nf;
   if (n_fallback) *n_fallback = nf;
   return 0;
@@ -1191,10 +1198,11 @@ int surrogate_load(SurrogateState *s, const std::string &path, hipStream_t st, s
   return 0;
 }
 
-int surrogate_last_stats(const SurrogateState *s, double *out3) {
-  out3[0] = (double)s->last_fallback;
-  out3[1] = s->last_ms[0];
-  out3[2] = s->last_ms[1];
+int surrogate_last_stats(const SurrogateState *s, double *out4) {
+  out4[0] = (double)s->last_fallback;
+  out4[1] = s->last_ms[0];
+  out4[2] = s->last_ms[1];
+  out4[3] = s->last_ms[2];
   return 0;
 }
 

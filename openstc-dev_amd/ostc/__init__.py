@@ -78,6 +78,7 @@ def lib():
             "ostc_dev_free": ([vp, vp], C.c_int),
             "ostc_memcpy_h2d": ([vp, vp, vp, C.c_int64], C.c_int),
             "ostc_memcpy_d2h": ([vp, vp, vp, C.c_int64], C.c_int),
+            "ostc_memcpy_d2d": ([vp, vp, vp, C.c_int64], C.c_int),
             "ostc_sync": ([vp], C.c_int),
             "ostc_profile": ([vp, C.c_int], C.c_int),
             "ostc_profile_report": ([vp, C.c_char_p, C.c_int], C.c_int),
@@ -365,9 +366,9 @@ class Solver:
         return out
 
     def surrogate_last_stats(self):
-        o = np.zeros(3)
+        o = np.zeros(4)
         _chk(lib().ostc_surrogate_last_stats(self.h, _dp(o)))
-        return dict(n_fallback=int(o[0]), net_ms=o[1], fallback_ms=o[2])
+        return dict(n_fallback=int(o[0]), net_ms=o[1], gate_ms=o[2], fallback_ms=o[3])
 
     def profile(self, on):
         _chk(lib().ostc_profile(self.h, 1 if on else 0))
