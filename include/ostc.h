@@ -45,7 +45,11 @@ enum ostc_field {
   OSTC_BV_LAST = 14, /* residual work array (4), no ghosts */
   OSTC_JAC = 15,     /* cell Jacobian J = det(dx/dxi), ghosted */
   OSTC_METRIC = 16,  /* 9 components d xi_r/d x_c, component-major, ghosted */
-  OSTC_CHEM_JAC = 17 /* n_spec^2 EPI Jacobian, no ghosts */
+  OSTC_CHEM_JAC = 17,/* n_spec^2 EPI Jacobian, no ghosts */
+  OSTC_MUT = 18,     /* SST turbulent viscosity, ghosted            Field.h:46 */
+  OSTC_TURB_COND = 19, /* SST turbulent conductivity mu_t cp / Prt, ghosted  Field.h:47 */
+  OSTC_WALL_DIST = 20, /* SST wall distance, ghosted                Field.h:23 */
+  OSTC_TURB_JAC = 21 /* SST source Jacobian diagonal of k, omega (2), no ghosts */
 };
 
 /* Version of this ABI (bumped on any signature change). */
@@ -103,7 +107,8 @@ int ostc_field_device_ptr(ostc_solver *s, int blk, int field, double **dptr);
  * Replaces: initialize_from_start (Initialize.cu:5-47) for groups_init = 1. */
 int ostc_initialize_uniform(ostc_solver *s, int blk);
 /* Reference inflow state of struct `name`: out[0..6] = rho,u,v,w,p,T,|V|,
- * out[7..] = scalars.  Replaces: Inflow::Inflow (BoundCond.cpp:7-122). */
+ * out[7..7+n_scalar) = scalars (species, then k and omega with SST), then
+ * mu_t.  Replaces: Inflow::Inflow (BoundCond.cpp:7-122). */
 int ostc_inflow_state(ostc_solver *s, const char *name, double *out, int n);
 /* Start-of-run sequence on every block: 2-D k-copy, boundary conditions,
  * |V|, properties, last-step store, cv from bv on all points.

@@ -42,6 +42,7 @@ struct Inflow {
   std::string name;
   int label = 5;
   double rho = 0, u = 0, v = 0, w = 0, p = 0, t = 0, vel = 0, mach = 0, mw = kMwAir;
+  double mut = 0;  // SST: turbulent viscosity of the state
   std::vector<double> sv;
 };
 
@@ -54,6 +55,7 @@ struct WallBC {
 struct Block {
   int mx = 0, my = 0, mz = 0;
   std::vector<double> jac, metric;  // host copies (ghosted, component-major metric)
+  std::vector<double> xyz;          // ghosted node coordinates (SST wall distance only)
   std::vector<BoundaryFace> faces;
   std::vector<BCDesc> bcs;          // in application order
   ZoneView z{};
@@ -75,7 +77,7 @@ struct ostc_solver {
   Mechanism mech;
   DMech *d_mech = nullptr;
   DParams P{};
-  int ngg = 2, n_var = 5, n_spec = 0, n_scalar = 0;
+  int ngg = 2, n_var = 5, n_spec = 0, n_scalar = 0, n_turb = 0;
   std::vector<Block> blocks;
   std::vector<Inflow> inflows;
   std::vector<WallBC> walls;
@@ -145,6 +147,59 @@ struct ostc_solver {
 };
 
 namespace {
+
+// Transport.cu:6-38 Sutherland / compute_viscosity (Wilke's rule) on the host
+double host_viscosity(const ostc_solver *s, double t, double mw, const double *Y) {
+  if (s->n_spec == 0) return 1.716e-5 * std::pow(t / 273, 1.5) * (273 + 111) / (t + 111);
+  const Mechanism &m = s->mech;
+  const int ns = m.n_spec;
+  std::vector<double> x(ns), vis(ns);
+  for (int i = 0; i < ns; ++i) {
+    x[i] = Y[i] * mw / m.mw[i];
+    const double tdl = t * m.LJ_potent_inv[i];
+    const double ci = 1.147 * std::pow(tdl, -0.145) + std::pow(tdl + 0.5, -2);
+    vis[i] = m.vis_coeff[i] * std::sqrt(t) / ci;
+  }
+  double mu = 0;
+  for (int i = 0; i < ns; ++i) {
+    double vt = 0;
+    for (int j = 0; j < ns; ++j) {
+      double phi = 1.0;
+      if (i != j) {
+        const double num = 1 + std::sqrt(vis[i] / vis[j]) * m.WjDivWi_to_One4th[i * ns + j];
+        phi = num * num * m.sqrt_WiDivWjPl1Mul8[i * ns + j];
+      }
+      vt += phi * x[j];
+    }
+    mu += vis[i] * x[i] / vt;
+  }
+  return mu;
+}
+
+// BoundCond.cpp:83-95 / 269-280: SST state of an inflow or farfield struct:
+// mu_t = ratio mu; with a turbulence intensity I, k = 1.5 (|V| I)^2 and
+// omega = rho k / mu_t (k, omega stay 0 without one)
+int turbulence_state(const ostc_solver *s, const Struct &info, const std::string &name, double viscosity,
+                     double velocity, double density, double &mut, double *sv) {
+  if (s->n_turb == 0) return 0;
+  auto get = [&](const char *k, double &v) {
+    auto f = info.find(k);
+    if (f == info.end()) return false;
+    if (std::holds_alternative<double>(f->second))
+      v = std::get<double>(f->second);
+    else if (std::holds_alternative<int>(f->second))
+      v = std::get<int>(f->second);
+    return true;
+  };
+  double ratio = 0, inten = 0;
+  if (!get("turb_viscosity_ratio", ratio)) return fail("'" + name + "' needs turb_viscosity_ratio for SST");
+  mut = ratio * viscosity;
+  if (get("turbulence_intensity", inten)) {
+    sv[s->n_spec] = 1.5 * velocity * velocity * inten * inten;
+    sv[s->n_spec + 1] = density * sv[s->n_spec] / mut;
+  }
+  return 0;
+}
 
 // BoundCond.cpp:7-122 Inflow::Inflow for the keys a struct may carry.
 int make_inflow(ostc_solver *s, const std::string &name, Inflow &in) {
@@ -222,6 +277,9 @@ int make_inflow(ostc_solver *s, const std::string &name, Inflow &in) {
   in.vel = velocity;
   in.mach = mach;
   in.mw = mw;
+  if (turbulence_state(s, info, name, host_viscosity(s, temperature, mw, in.sv.data()), velocity, density, in.mut,
+                       in.sv.data()))
+    return 1;
   return 0;
 }
 
@@ -290,6 +348,9 @@ int make_farfield(ostc_solver *s, const std::string &name, BCDesc &d) {
     pressure = density * temperature * kRu / mw;
   }
   if (density < 0) density = pressure * mw / (kRu * temperature);
+  if (turbulence_state(s, info, name, host_viscosity(s, temperature, mw, sv.data()), velocity, density, d.mut,
+                       sv.data()))
+    return 1;
   d.rho = density;
   d.u = u;
   d.v = v;
@@ -305,8 +366,9 @@ int make_farfield(ostc_solver *s, const std::string &name, BCDesc &d) {
   return 0;
 }
 
-// p_inf and T_inf of the reference state (the inflow or farfield struct it names)
-int reference_pt(ostc_solver *s, double &p_inf, double &t_inf) {
+// p_inf, T_inf, rho_inf and v_inf of the reference state (the inflow or
+// farfield struct it names)
+int reference_pt(ostc_solver *s, double &p_inf, double &t_inf, double *rho_inf = nullptr, double *v_inf = nullptr) {
   const std::string ref = s->prm.get_string("reference_state", "");
   auto it = s->prm.st.find(ref);
   if (it == s->prm.st.end()) return fail("reference_state struct '" + ref + "' not found");
@@ -317,11 +379,15 @@ int reference_pt(ostc_solver *s, double &p_inf, double &t_inf) {
     if (make_farfield(s, ref, d)) return 1;
     p_inf = d.p;
     t_inf = d.t;
+    if (rho_inf) *rho_inf = d.rho;
+    if (v_inf) *v_inf = d.vel;
   } else {
     Inflow in;
     if (make_inflow(s, ref, in)) return 1;
     p_inf = in.p;
     t_inf = in.t;
+    if (rho_inf) *rho_inf = in.rho;
+    if (v_inf) *v_inf = in.vel;
   }
   return 0;
 }
@@ -404,8 +470,61 @@ FieldDesc field_desc(ostc_solver *s, Block &b, int f) {
     case OSTC_CHEM_JAC: return {z.chem_jac, s->P.chemSrcMethod == 1 ? s0 * s->n_spec * s->n_spec
                                             : (s->P.chemSrcMethod == 2 ? s0 * s->n_spec : 0), 0,
                                 s->P.chemSrcMethod == 2 ? s->n_spec : s->n_spec * s->n_spec};
+    case OSTC_MUT: return {z.mut, z.mut ? sg : 0, z.g, 1};
+    case OSTC_TURB_COND: return {z.tcond, z.tcond ? sg : 0, z.g, 1};
+    case OSTC_WALL_DIST: return {z.wdist, z.wdist ? sg : 0, z.g, 1};
+    case OSTC_TURB_JAC: return {z.turb_jac, z.turb_jac ? s0 * 2 : 0, 0, 2};
   }
   return {nullptr, -1, 0, 0};
+}
+
+// WallDistance.cuh:9-81: the nodes of every wall range (registered and
+// trimmed) of every block are the wall points; each block's ghosted points
+// get their distance on the device.
+int compute_wall_distance(ostc_solver *s) {
+  std::vector<double> wx, wy, wz;
+  for (auto &b : s->blocks) {
+    const Layout L(b.mx, b.my, b.mz, s->ngg);
+    for (auto &f : b.faces) {
+      if (f.type != BC_WALL) continue;
+      for (int k = f.rs[2]; k <= f.re[2]; ++k)
+        for (int j = f.rs[1]; j <= f.re[1]; ++j)
+          for (int i = f.rs[0]; i <= f.re[0]; ++i) {
+            const int64_t p = L.at(i, j, k);
+            wx.push_back(b.xyz[p]);
+            wy.push_back(b.xyz[L.sz + p]);
+            wz.push_back(b.xyz[2 * L.sz + p]);
+          }
+    }
+  }
+  const int nw = (int)wx.size();
+  std::vector<double> wall(3 * (size_t)std::max(nw, 1));
+  std::copy(wx.begin(), wx.end(), wall.begin());
+  std::copy(wy.begin(), wy.end(), wall.begin() + nw);
+  std::copy(wz.begin(), wz.end(), wall.begin() + 2 * nw);
+  double *d_wall = nullptr, *d_xyz = nullptr;
+  HIPCHK(hipMalloc(&d_wall, wall.size() * sizeof(double)));
+  HIPCHK(hipMemcpyAsync(d_wall, wall.data(), wall.size() * sizeof(double), hipMemcpyHostToDevice, s->stream));
+  int rc = 0;
+  for (auto &b : s->blocks) {
+    if (b.xyz.size() != (size_t)b.z.sz * 3) {
+      rc = fail("node coordinates of a block are missing");
+      break;
+    }
+    hipError_t e = hipMalloc(&d_xyz, b.xyz.size() * sizeof(double));
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(d_xyz, b.xyz.data(), b.xyz.size() * sizeof(double), hipMemcpyHostToDevice, s->stream);
+    if (e == hipSuccess) e = launch_wall_distance(b.z, d_xyz, d_wall, nw, s->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+    (void)hipFree(d_xyz);
+    d_xyz = nullptr;
+    if (e != hipSuccess) {
+      rc = fail(std::string("wall distance: ") + hipGetErrorString(e));
+      break;
+    }
+  }
+  (void)hipFree(d_wall);
+  return rc;
 }
 
 int check_blk(ostc_solver *s, int blk) {
@@ -448,11 +567,19 @@ int run_bcs(ostc_solver *s, Block &b, bool with_cv) {
   return 0;
 }
 
+// SourceTerm.cuh:11-37 then the fluxes (SteadySim.cuh:73-78, RK.cuh:109-116):
+// the SST kernel (mu_t, lambda_t, k / omega sources) initialises dq, the
+// chemical source adds to it, the sweeps accumulate.
 int rhs_block(ostc_solver *s, Block &b) {
   bool init = true;
+  if (s->n_turb) {
+    ostc_solver::Timer t(s, "sst");
+    HIPCHK(launch_sst(b.z, s->P, s->stream));
+    init = false;
+  }
   if (s->P.mix && s->P.reaction == 1) {
     ostc_solver::Timer t(s, "source");
-    HIPCHK(launch_source(b.z, s->P, s->d_mech, s->stream));
+    HIPCHK(launch_source(b.z, s->P, s->d_mech, !init, s->stream));
     init = false;
   }
   static const char *names[3] = {"sweep_i", "sweep_j", "sweep_k"};
@@ -493,7 +620,13 @@ int ostc_init(const char *case_dir, int device, ostc_solver **out) {
   const int species = p.get_int("species", 0), reaction = p.get_int("reaction", 0);
   if (species != 0 && species != 1) return fail("species = " + std::to_string(species) + " is not supported");
   if (reaction == 2) return fail("flamelet model (reaction = 2) is not supported");
-  if (p.get_bool("turbulence", false)) return fail("turbulent runs are not supported by this build");
+  int n_turb = 0;
+  if (p.get_bool("turbulence", false)) {
+    // main.cu:17-90: RANS (turbulence_method 1) with k-omega SST (RANS_model 2)
+    if (p.get_int("turbulence_method", 1) != 1 || p.get_int("RANS_model", 2) != 2)
+      return fail("only RANS with the k-omega SST model (turbulence_method 1, RANS_model 2) is supported");
+    n_turb = 2;
+  }
   if (species == 1) {
     const std::string in = s->case_dir + "/input_files";
     auto e = read_mechanism(in, p.get_string("mechanism_file", "chemistry/combustion_mechanism.inp"),
@@ -502,19 +635,25 @@ int ostc_init(const char *case_dir, int device, ostc_solver **out) {
     if (!e.empty()) return fail(e);
     s->n_spec = s->mech.n_spec;
   }
-  // Field.cu:4-60: variables per cell
-  s->n_scalar = s->n_spec;
-  s->n_var = 5 + s->n_spec;
+  // Field.cu:4-60: variables per cell (species, then k and omega)
+  s->n_turb = n_turb;
+  s->n_scalar = s->n_spec + n_turb;
+  s->n_var = 5 + s->n_scalar;
   s->ngg = p.get_int("ngg", 2);
   s->prm.i["n_spec"] = s->n_spec;
   s->prm.i["n_reac"] = s->mech.n_reac;
   s->prm.i["n_var"] = s->n_var;
   s->prm.i["n_scalar"] = s->n_scalar;
+  s->prm.i["n_turb"] = n_turb;
+  s->prm.i["i_turb_cv"] = 5 + s->n_spec;
   DParams &P = s->P;
   P.g = s->ngg;
   P.nv = s->n_var;
   P.ns = s->n_spec;
   P.nsc = s->n_scalar;
+  P.nt = n_turb;
+  P.turb_implicit = p.get_int("turb_implicit", 1);
+  P.compress = p.get_int("compressibility_correction", 0);
   P.mix = species ? 1 : 0;
   P.reaction = reaction;
   P.chemSrcMethod = p.get_int("chemSrcMethod", 0);
@@ -531,6 +670,9 @@ int ostc_init(const char *case_dir, int device, ostc_solver **out) {
   P.Prt = p.get_real("turbulent_prandtl_number", 0.9);
   P.Sct = p.get_real("turbulent_schmidt_number", 0.9);
   if (P.inviscid_type == 1 && P.mix) return fail("AWENO for multi-species mixtures has no reference definition");
+  if (P.inviscid_type == 1 && n_turb)
+    return fail("the AWENO characteristic reconstruction does not carry k and omega (AWENO.cu:53-54)");
+  if (n_turb && !P.viscous) return fail("SST needs the viscous flux (viscous_order 2)");
   if (P.mix && P.ns != 9) return fail("this build is compiled for 9-species mechanisms");
   if (device < 0) {  // host-only: setup files, tables and grid metrics, no device work
     *out = s.release();
@@ -618,7 +760,8 @@ int ostc_add_block(ostc_solver *s, int mx, int my, int mz, const double *x, cons
   b.my = my;
   b.mz = mz;
   std::string err;
-  if (!compute_metrics(mx, my, mz, s->ngg, x, y, z, b.jac, b.metric, err)) return fail(err);
+  if (!compute_metrics(mx, my, mz, s->ngg, x, y, z, b.jac, b.metric, err, s->n_turb ? &b.xyz : nullptr))
+    return fail(err);
   s->blocks.push_back(std::move(b));
   if (blk) *blk = (int)s->blocks.size() - 1;
   return 0;
@@ -763,6 +906,11 @@ int ostc_finalize(ostc_solver *s) {
         alloc(&z.inv_sr, L0.sz * 3) ||
         (s->prm.get_int("implicit_method", 0) == 1 && (alloc(&z.dq0, L0.sz * nv) || alloc(&z.dqk, L0.sz * nv))))
       return 1;
+    // SST: mu_t and lambda_t (ghost lambda_t stays zero: only the source
+    // kernel writes it, on the interior), wall distance, source diagonal
+    if (s->n_turb && (alloc(&z.mut, L.sz) || alloc(&z.tcond, L.sz) || alloc(&z.wdist, L.sz) ||
+                      (s->P.turb_implicit == 1 && alloc(&z.turb_jac, L0.sz * 2))))
+      return 1;
     z.jac = jac;
     z.metric = met;
     HIPCHK(hipMemcpyAsync(jac, b.jac.data(), L.sz * sizeof(double), hipMemcpyHostToDevice, s->stream));
@@ -805,6 +953,7 @@ int ostc_finalize(ostc_solver *s) {
             d.p = in->p;
             d.t = in->t;
             d.vel = in->vel;
+            d.mut = in->mut;
             for (size_t l = 0; l < in->sv.size() && l < (size_t)kMS + 4; ++l) d.sv[l] = in->sv[l];
           } else if (t == BC_FARFIELD || t == BC_SUBSONIC || t == BC_BACKPRESSURE) {
             std::string nm;
@@ -826,8 +975,8 @@ int ostc_finalize(ostc_solver *s) {
             if (t == BC_FARFIELD) {
               if (make_farfield(s, nm, d)) return 1;
             } else {
-              double p_inf = 0, t_inf = 0;
-              if (reference_pt(s, p_inf, t_inf)) return 1;
+              double p_inf = 0, t_inf = 0, rho_inf = 0, v_inf = 0;
+              if (reference_pt(s, p_inf, t_inf, &rho_inf, &v_inf)) return 1;
               if (t == BC_SUBSONIC) {
                 // BoundCond.cpp:300-343
                 double ptr = 1, ttr = 1;
@@ -841,6 +990,11 @@ int ostc_finalize(ostc_solver *s) {
                 get_r("w", d.w);
                 d.p_total = ptr * p_inf;
                 d.t_total = ttr * t_inf;
+                // BoundCond.cpp:327-341: SST state from Sutherland(T_ref), v_inf, rho_inf
+                if (s->n_turb) {
+                  const double mu = 1.716e-5 * std::pow(t_inf / 273, 1.5) * (273 + 111) / (t_inf + 111);
+                  if (turbulence_state(s, info, nm, mu, v_inf, rho_inf, d.mut, d.sv)) return 1;
+                }
               } else {
                 // BoundCond.cpp:345-364
                 double pb = -1, ratio = 1;
@@ -865,6 +1019,7 @@ int ostc_finalize(ostc_solver *s) {
         }
       }
   }
+  if (s->n_turb && compute_wall_distance(s)) return 1;
   // per-run scratch: partials, ticket counter, dt slot, residual scale
   s->w.max_blocks = (int)((max_cells + 255) / 256) + 1;
   HIPCHK(hipMalloc(&s->w.partial, sizeof(double) * 4 * s->w.max_blocks));
@@ -932,6 +1087,7 @@ int ostc_inflow_state(ostc_solver *s, const char *name, double *out, int n) {
   const double v[7] = {in.rho, in.u, in.v, in.w, in.p, in.t, in.vel};
   for (int i = 0; i < n && i < 7; ++i) out[i] = v[i];
   for (int i = 7; i < n && i - 7 < (int)in.sv.size(); ++i) out[i] = in.sv[i - 7];
+  if (n > 7 + (int)in.sv.size()) out[7 + in.sv.size()] = in.mut;
   return 0;
 }
 
@@ -957,6 +1113,7 @@ int ostc_initialize(ostc_solver *s) {
     if (s->P.dim == 2) HIPCHK(launch_eliminate_k(b.z, s->P, false, s->stream));
     if (run_bcs(s, b, false)) return 1;
     HIPCHK(launch_velocity(b.z, s->stream));
+    if (s->n_turb) HIPCHK(launch_init_mut(b.z, s->P, s->d_mech, s->stream));
     if (properties_block(s, b)) return 1;
     HIPCHK(launch_store_last(b.z, s->stream));
     HIPCHK(launch_cv_from_bv(b.z, s->P, s->d_mech, s->stream));
@@ -1100,6 +1257,7 @@ static int implicit_block(ostc_solver *s, Block &b) {
     HIPCHK(launch_dplur(b.z, s->P, s->d_mech, s->prm.get_int("DPLUR_inner_step", 3), walls, s->stream));
   } else {
     HIPCHK(launch_chem_implicit(b.z, s->P, s->stream));
+    if (s->n_turb && s->P.turb_implicit == 1) HIPCHK(launch_turb_implicit(b.z, s->P, s->stream));
   }
   return 0;
 }

@@ -183,6 +183,37 @@ __device__ inline void d_lu_solve(double *A, double *b) {
   }
 }
 
+// Transport.cu:113-140 compute_viscosity: Wilke's rule alone, at the mean
+// molecular weight mw_total of composition Y
+template <int NS>
+__device__ inline double d_viscosity(const DMech *__restrict__ m, double t, double mw_total, const double *Y) {
+  double X[NS], vis[NS];
+  const double st = sqrt(t);
+#pragma unroll
+  for (int l = 0; l < NS; ++l) {
+    X[l] = Y[l] * mw_total / m->mw[l];
+    const double tdl = t * m->LJ_potent_inv[l];
+    const double ci = 1.147 * pow(tdl, -0.145) + pow(tdl + 0.5, -2.0);
+    vis[l] = m->vis_coeff[l] * st / ci;
+  }
+  double mu = 0;
+#pragma unroll
+  for (int a = 0; a < NS; ++a) {
+    double vt = 0;
+#pragma unroll
+    for (int b = 0; b < NS; ++b) {
+      double pf = 1.0;
+      if (a != b) {
+        const double num = 1 + sqrt(vis[a] / vis[b]) * m->WjDivWi_to_One4th[a * NS + b];
+        pf = num * num * m->sqrt_WiDivWjPl1Mul8[a * NS + b];
+      }
+      vt += pf * X[b];
+    }
+    mu += vis[a] * X[a] / vt;
+  }
+  return mu;
+}
+
 // Transport.cu:40-110 compute_transport_property
 template <int NS>
 __device__ inline void d_transport(const DMech *__restrict__ m, double t, double p, double rho, const double *Y,

@@ -30,11 +30,12 @@ struct DMech {
 // Scheme switches and constants (the scalar part of DParameter).
 struct DParams {
   int mx, my, mz, g, dim;
-  int nv, ns, nsc;
+  int nv, ns, nsc, nt;  // n_var, species, scalars (species + turbulence), turbulence scalars (0 or 2)
   int mix, reaction, chemSrcMethod;
   int inviscid_scheme, reconstruction, limiter, viscous, inviscid_type, gradP;
   double cfl, Pr, Sc, Prt, Sct;
   double efix;  // Roe entropy-fix factor
+  int turb_implicit, compress;  // SST: implicit source diagonal; compressibility correction 0-3
 };
 
 // Raw device pointers of one block (DZone, Field.h:13-71).  bv/sv/cv/qn and
@@ -51,6 +52,9 @@ struct ZoneView {
   double *dq, *dt_local, *bv_last, *chem_jac;
   double *inv_sr;          // steady path: inviscid spectral radii, 3 x ghost-free (null otherwise)
   double *dq0, *dqk;       // DPLUR work arrays, n_var x ghost-free (null otherwise)
+  double *mut, *tcond;     // SST: turbulent viscosity and conductivity (ghosted; null when laminar)
+  double *wdist;           // SST: wall distance (ghosted)
+  double *turb_jac;        // SST: source Jacobian diagonal of k and omega, 2 x ghost-free
 
   __host__ __device__ int64_t at(int i, int j, int k) const {
     return (int64_t)(k + g) * d1 + (int64_t)(j + g) * d2 + (i + g);

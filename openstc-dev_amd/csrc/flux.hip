@@ -24,6 +24,7 @@
 // that range is kept.
 #include "device_common.h"
 #include "flux_kernels.h"
+#include "sst_device.h"
 
 namespace ostc {
 
@@ -39,10 +40,10 @@ __device__ inline R minmod(R a, R b) {
 // Reconstruction.cuh:20-72, Reconstruction.cu:5-46 on primitives
 // (rho,u,v,w,p,Y...) of the 4 points p0-st .. p0+2st.  Appends rhoE at [nv]
 // and gamma at [nv+1].
-template <int NS, typename R>
+template <int NS, int NT, typename R>
 __device__ inline void recon_pv(const ZoneView &z, const DParams &P, const DMech *__restrict__ m, int64_t p0,
                                 int64_t st, R *pl, R *pr) {
-  constexpr int NV = 5 + NS;
+  constexpr int NV = 5 + NS + NT;
   const R kappa = R(1.0 / 3.0);
 #pragma unroll
   for (int l = 0; l < NV; ++l) {
@@ -254,9 +255,9 @@ __device__ inline void aweno_air(const R (*cvb)[6], const R *mk0, const R *mk1, 
 // ------------------------------------------------------ Riemann solvers
 // AusmP.cu:6-94.  pl/pr: (rho,u,v,w,p,Y..., rhoE, gamma); mk0/mk1 metric
 // rows and j0/j1 Jacobians of the two adjacent points.
-template <int NS, typename R>
+template <int NS, int NT, typename R>
 __device__ inline void ausm_plus(const R *pl, const R *pr, const R *mk0, const R *mk1, R j0, R j1, R *f) {
-  constexpr int NV = 5 + NS;
+  constexpr int NV = 5 + NS + NT;
   const R k1 = R(0.5) * (j0 * mk0[0] + j1 * mk1[0]);
   const R k2 = R(0.5) * (j0 * mk0[1] + j1 * mk1[1]);
   const R k3 = R(0.5) * (j0 * mk0[2] + j1 * mk1[2]);
@@ -301,10 +302,10 @@ __device__ inline void ausm_plus(const R *pl, const R *pr, const R *mk0, const R
 }
 
 // HLLC.cu:10-143
-template <int NS, typename R>
+template <int NS, int NT, typename R>
 __device__ inline void hllc(const DMech *__restrict__ m, const R *pl, const R *pr, const R *mk0, const R *mk1, R j0,
                             R j1, R *f) {
-  constexpr int NV = 5 + NS;
+  constexpr int NV = 5 + NS + NT;
   const R sl0 = sqrt(pl[0]), sr0 = sqrt(pr[0]);
   const R rlc = sl0 / (sl0 + sr0), rrc = sr0 / (sl0 + sr0);
   const R ut = pl[1] * rlc + pr[1] * rrc, vt = pl[2] * rlc + pr[2] * rrc, wt = pl[3] * rlc + pr[3] * rrc;
@@ -407,10 +408,10 @@ __device__ inline R efix_point(const ZoneView &z, const DParams &P, int64_t p) {
 // Roe average (species rows included; the mixture's gamma and c from the
 // Roe-averaged temperature, FP64), Harten's entropy fix with the face
 // average of the point deltas d0, d1.
-template <int NS, typename R>
+template <int NS, int NT, typename R>
 __device__ inline void roe(const DMech *__restrict__ m, const R *pl, const R *pr, const R *mk0, const R *mk1, R j0,
                            R j1, R d0, R d1, R *f) {
-  constexpr int NV = 5 + NS;
+  constexpr int NV = 5 + NS + NT;
 #pragma unroll
   for (int l = 0; l < NV; ++l) f[l] = R(0);
 #pragma unroll
@@ -434,16 +435,15 @@ __device__ inline void roe(const DMech *__restrict__ m, const R *pl, const R *pr
   const R ek = R(0.5) * (u * u + v * v + w * w);
   const R h = (dl * (pl[NV] + pl[4]) / pl[0] + dr * (pr[NV] + pr[4]) / pr[0]) * id;
   R gamma = R(kGam), c, mw = R(kMwAirD);
-  R svm[NS > 0 ? NS : 1], hi[NS > 0 ? NS : 1];
+  R svm[NV > 5 ? NV - 5 : 1], hi[NS > 0 ? NS : 1];
+#pragma unroll
+  for (int l = 0; l < NV - 5; ++l) svm[l] = (dl * pl[l + 5] + dr * pr[l + 5]) * id;
   if constexpr (NS == 0) {
     c = sqrt((gamma - R(1)) * (h - ek));
   } else {
     double mwi = 0;
 #pragma unroll
-    for (int l = 0; l < NS; ++l) {
-      svm[l] = (dl * pl[l + 5] + dr * pr[l + 5]) * id;
-      mwi += (double)svm[l] / m->mw[l];
-    }
+    for (int l = 0; l < NS; ++l) mwi += (double)svm[l] / m->mw[l];
     const double t = (double)((dl * (pl[4] / pl[0]) + dr * (pr[4] / pr[0])) * id) / (kRuD * mwi);
     double hh[NS], cc[NS], cp = 0, cv = 0;
     d_enthalpy_cp<NS>(t, m, hh, cc);
@@ -490,7 +490,7 @@ __device__ inline void roe(const DMech *__restrict__ m, const R *pl, const R *pr
   b[3] = -lam[1] * (kz * c3 - ((ky * u - kx * v) * dq[0] - ky * dq[1] + kx * dq[2]) / c);
   b[4] = -lam[2] * (R(0.5) * (c1 + c2));
 #pragma unroll
-  for (int l = 0; l < NS; ++l) b[l + 5] = -lam[1] * (dq[l + 5] - svm[l] * dq[0]);
+  for (int l = 0; l < NV - 5; ++l) b[l + 5] = -lam[1] * (dq[l + 5] - svm[l] * dq[0]);
   const R c0 = kx * b[1] + ky * b[2] + kz * b[3];
   c1 = c0 + b[0] + b[4];
   c2 = c * (b[4] - b[0]);
@@ -504,13 +504,13 @@ __device__ inline void roe(const DMech *__restrict__ m, const R *pl, const R *pr
   f[4] += R(0.5) * (h * c1 + Uk * c2 - c * c * c0 / g1 +
                     c * ((kz * v - ky * w) * b[1] + (kx * w - kz * u) * b[2] + (ky * u - kx * v) * b[3]) + c3);
 #pragma unroll
-  for (int l = 0; l < NS; ++l) f[5 + l] += R(0.5) * (b[l + 5] + svm[l] * c1);
+  for (int l = 0; l < NV - 5; ++l) f[5 + l] += R(0.5) * (b[l + 5] + svm[l] * c1);
 }
 
 // LaxFriedrichs.cu:14-49 (air only; the reference's generic version is empty)
-template <int NS, typename R>
+template <int NS, int NT, typename R>
 __device__ inline void lax_friedrichs(const R *pl, const R *pr, const R *mk0, const R *mk1, R j0, R j1, R *f) {
-  constexpr int NV = 5 + NS;
+  constexpr int NV = 5 + NS + NT;
   const R kx = R(0.5) * (mk0[0] + mk1[0]), ky = R(0.5) * (mk0[1] + mk1[1]), kz = R(0.5) * (mk0[2] + mk1[2]);
   const R gK = sqrt(kx * kx + ky * ky + kz * kz);
   const R Ukl = pl[1] * kx + pl[2] * ky + pl[3] * kz, Ukr = pr[1] * kx + pr[2] * ky + pr[3] * kz;
@@ -532,7 +532,7 @@ __device__ inline void lax_friedrichs(const R *pl, const R *pr, const R *mk0, co
 // 4-point cross averages), chain rule with face-averaged metrics, Stokes
 // stress, Fourier flux, mixture-averaged diffusion with the correction
 // velocity; the flux is projected on the J-weighted face metric.
-template <int NS, typename R>
+template <int NS, int NT, typename R>
 __device__ inline void viscous_face(const ZoneView &z, const DParams &P, const DMech *__restrict__ m, int dir,
                                     int64_t p0, int64_t st, R *fv) {
   const int64_t p1 = p0 + st;
@@ -567,11 +567,23 @@ __device__ inline void viscous_face(const ZoneView &z, const DParams &P, const D
   chain(g, wx);
   grad(z.bv + 5 * z.sz, g);
   chain(g, tx);
-  const R vis = R(0.5) * ((R)z.mul[p0] + (R)z.mul[p1]);
-  const R txx = vis * (R(4) * ux[0] - R(2) * vx[1] - R(2) * wx[2]) / R(3.0);
-  const R tyy = vis * (R(4) * vx[1] - R(2) * ux[0] - R(2) * wx[2]) / R(3.0);
-  const R tzz = vis * (R(4) * wx[2] - R(2) * ux[0] - R(2) * vx[1]) / R(3.0);
+  const R mul = R(0.5) * ((R)z.mul[p0] + (R)z.mul[p1]);
+  R mut = R(0);
+  if constexpr (NT > 0) mut = R(0.5) * ((R)z.mut[p0] + (R)z.mut[p1]);
+  const R vis = mul + mut;
+  R txx = vis * (R(4) * ux[0] - R(2) * vx[1] - R(2) * wx[2]) / R(3.0);
+  R tyy = vis * (R(4) * vx[1] - R(2) * ux[0] - R(2) * wx[2]) / R(3.0);
+  R tzz = vis * (R(4) * wx[2] - R(2) * ux[0] - R(2) * vx[1]) / R(3.0);
   const R txy = vis * (ux[1] + vx[0]), txz = vis * (ux[2] + wx[0]), tyz = vis * (vx[2] + wx[1]);
+  if constexpr (NT > 0) {
+    // -2/3 rho k on the normal stresses, faded out within 1e-3 of a wall
+    const double *ks = z.sv + (int64_t)NS * z.sz;
+    const R dyd = fmin(R(0.5) * ((R)z.wdist[p0] + (R)z.wdist[p1]) / R(1e-3), R(1));
+    const R rk = R(-2.0 / 3 * 0.5) * ((R)z.bv[p0] * (R)ks[p0] + (R)z.bv[p1] * (R)ks[p1]) * dyd;
+    txx += rk;
+    tyy += rk;
+    tzz += rk;
+  }
   const R J0 = (R)z.jac[p0], J1 = (R)z.jac[p1];
   const int rb = dir * 3;
   const R kx = R(0.5) * ((R)z.metric[(rb + 0) * z.sz + p0] * J0 + (R)z.metric[(rb + 0) * z.sz + p1] * J1);
@@ -584,7 +596,8 @@ __device__ inline void viscous_face(const ZoneView &z, const DParams &P, const D
   const R um = R(0.5) * ((R)z.bv[1 * z.sz + p0] + (R)z.bv[1 * z.sz + p1]);
   const R vm = R(0.5) * ((R)z.bv[2 * z.sz + p0] + (R)z.bv[2 * z.sz + p1]);
   const R wm = R(0.5) * ((R)z.bv[3 * z.sz + p0] + (R)z.bv[3 * z.sz + p1]);
-  const R cond = R(0.5) * ((R)z.cond[p0] + (R)z.cond[p1]);
+  R cond = R(0.5) * ((R)z.cond[p0] + (R)z.cond[p1]);
+  if constexpr (NT > 0) cond += R(0.5) * ((R)z.tcond[p0] + (R)z.tcond[p1]);
   fv[4] = um * fv[1] + vm * fv[2] + wm * fv[3] + cond * (kx * tx[0] + ky * tx[1] + kz * tx[2]);
   if constexpr (NS > 0) {
     R dif[NS], yk[NS], ddf[NS];
@@ -594,6 +607,7 @@ __device__ inline void viscous_face(const ZoneView &z, const DParams &P, const D
       const double *ys = z.sv + (int64_t)l * z.sz;
       yk[l] = R(0.5) * ((R)ys[p0] + (R)ys[p1]);
       dif[l] = R(0.5) * ((R)z.rhoD[(int64_t)l * z.sz + p0] + (R)z.rhoD[(int64_t)l * z.sz + p1]);
+      if constexpr (NT > 0) dif[l] += mut / (R)P.Sct;
       R gy[3], yx[3];
       grad(ys, gy);
       chain(gy, yx);
@@ -627,14 +641,42 @@ __device__ inline void viscous_face(const ZoneView &z, const DParams &P, const D
       fv[4] += (R)h[l] * df;
     }
   }
+  if constexpr (NT > 0) {
+    // SST diffusion of k and omega with sigma blended by F1 at the face
+    const double *ks = z.sv + (int64_t)NS * z.sz, *ws = z.sv + (int64_t)(NS + 1) * z.sz;
+    R gk[3], kxv[3], go[3], oxv[3];
+    grad(ks, gk);
+    chain(gk, kxv);
+    grad(ws, go);
+    chain(go, oxv);
+    const R wd = R(0.5) * ((R)z.wdist[p0] + (R)z.wdist[p1]);
+    R f1 = R(1);
+    if (wd > R(1e-25)) {
+      const R km = R(0.5) * ((R)ks[p0] + (R)ks[p1]);
+      const R om = R(0.5) * ((R)ws[p0] + (R)ws[p1]);
+      const R p1v = sqrt(km) / (R(0.09) * om * wd);
+      const R rhom = R(0.5) * ((R)z.bv[p0] + (R)z.bv[p1]);
+      const R d2 = wd * wd;
+      const R p2v = R(500) * mul / (rhom * d2 * om);
+      const R cd = fmax(R(1e-20), R(2) * rhom * R(sst::kSigmaOmega2) / om *
+                                      (kxv[0] * oxv[0] + kxv[1] * oxv[1] + kxv[2] * oxv[2]));
+      const R p3v = R(4) * rhom * R(sst::kSigmaOmega2) * km / (cd * d2);
+      const R a1 = fmin(fmax(p1v, p2v), p3v);
+      f1 = tanh(a1 * a1 * a1 * a1);
+    }
+    const R sk = R(sst::kSigmaK2) + R(sst::kSigmaK1 - sst::kSigmaK2) * f1;
+    const R sw = R(sst::kSigmaOmega2) + R(sst::kSigmaOmega1 - sst::kSigmaOmega2) * f1;
+    fv[5 + NS] = (mul + mut * sk) * (kx * kxv[0] + ky * kxv[1] + kz * kxv[2]);
+    fv[6 + NS] = (mul + mut * sw) * (kx * oxv[0] + ky * oxv[1] + kz * oxv[2]);
+  }
 }
 
 // ------------------------------------------------------ face assembly
 // F(face) = inviscid - viscous; G(face) = CDS flux-form term (AWENO only).
-template <int NS, int SCHEME, typename R>
+template <int NS, int NT, int SCHEME, typename R>
 __device__ inline void face_flux(const ZoneView &z, const DParams &P, const DMech *__restrict__ m, int dir, int64_t p0,
                                  int64_t st, bool want_g, R *F, R *G) {
-  constexpr int NV = 5 + NS;
+  constexpr int NV = 5 + NS + NT;
   const int rb = dir * 3;
   R mk0[3], mk1[3];
 #pragma unroll
@@ -678,33 +720,33 @@ __device__ inline void face_flux(const ZoneView &z, const DParams &P, const DMec
       }
     }
   } else {
-    recon_pv<NS, R>(z, P, m, p0, st, pl, pr);
+    recon_pv<NS, NT, R>(z, P, m, p0, st, pl, pr);
   }
   // inviscid_scheme 2 with AWENO (inviscid_type 1) runs AUSM+, as the
   // reference's AWENO switch has no Roe case (InviscidScheme.cu:290-303)
   switch (P.inviscid_scheme) {
-    case 1: lax_friedrichs<NS, R>(pl, pr, mk0, mk1, j0, j1, F); break;
-    case 4: hllc<NS, R>(m, pl, pr, mk0, mk1, j0, j1, F); break;
+    case 1: lax_friedrichs<NS, NT, R>(pl, pr, mk0, mk1, j0, j1, F); break;
+    case 4: hllc<NS, NT, R>(m, pl, pr, mk0, mk1, j0, j1, F); break;
     case 2:
       if constexpr (SCHEME == 0) {
-        roe<NS, R>(m, pl, pr, mk0, mk1, j0, j1, efix_point<R>(z, P, p0), efix_point<R>(z, P, p0 + st), F);
+        roe<NS, NT, R>(m, pl, pr, mk0, mk1, j0, j1, efix_point<R>(z, P, p0), efix_point<R>(z, P, p0 + st), F);
         break;
       }
       [[fallthrough]];
-    default: ausm_plus<NS, R>(pl, pr, mk0, mk1, j0, j1, F); break;
+    default: ausm_plus<NS, NT, R>(pl, pr, mk0, mk1, j0, j1, F); break;
   }
   if (P.viscous) {
     R fv[NV];
-    viscous_face<NS, R>(z, P, m, dir, p0, st, fv);
+    viscous_face<NS, NT, R>(z, P, m, dir, p0, st, fv);
 #pragma unroll
     for (int l = 0; l < NV; ++l) F[l] -= fv[l];
   }
 }
 
-template <int NS, typename R>
+template <int NS, int NT, typename R>
 __device__ inline void add_dq(const ZoneView &z, int64_t q0, bool init, const R *Fc, const R *Fp, bool cds, const R *Gc,
                               const R *Gp) {
-  constexpr int NV = 5 + NS;
+  constexpr int NV = 5 + NS + NT;
 #pragma unroll
   for (int l = 0; l < NV; ++l) {
     double d = (double)Fc[l] - (double)Fp[l];
@@ -715,9 +757,9 @@ __device__ inline void add_dq(const ZoneView &z, int64_t q0, bool init, const R 
 }
 
 // i-direction sweep: lane l of a wavefront owns face (i0 + l - 1) + 1/2.
-template <int NS, int SCHEME, typename R>
+template <int NS, int NT, int SCHEME, typename R>
 __global__ void __launch_bounds__(256) sweep_i(ZoneView z, DParams P, const DMech *__restrict__ m, int init) {
-  constexpr int NV = 5 + NS;
+  constexpr int NV = 5 + NS + NT;
   const int lane = threadIdx.x;
   const int i = blockIdx.x * 63 + lane - 1;
   const int j = blockIdx.y * blockDim.y + threadIdx.y;
@@ -730,13 +772,13 @@ __global__ void __launch_bounds__(256) sweep_i(ZoneView z, DParams P, const DMec
   for (int l = 0; l < NV; ++l) F[l] = R(0);
 #pragma unroll
   for (int l = 0; l < 5; ++l) G[l] = R(0);
-  if (live) face_flux<NS, SCHEME, R>(z, P, m, 0, z.at(i, j, k), 1, i <= ncds - 1, F, G);
+  if (live) face_flux<NS, NT, SCHEME, R>(z, P, m, 0, z.at(i, j, k), 1, i <= ncds - 1, F, G);
   R Fp[NV], Gp[5];
 #pragma unroll
   for (int l = 0; l < NV; ++l) Fp[l] = __shfl_up(F[l], 1, 64);
 #pragma unroll
   for (int l = 0; l < 5; ++l) Gp[l] = __shfl_up(G[l], 1, 64);
-  if (live && lane > 0) add_dq<NS, R>(z, z.at0(i, j, k), init != 0, F, Fp, i < ncds, G, Gp);
+  if (live && lane > 0) add_dq<NS, NT, R>(z, z.at0(i, j, k), init != 0, F, Fp, i < ncds, G, Gp);
 }
 
 // j- and k-direction sweeps: lanes along i (coalesced), kRows wavefronts
@@ -745,16 +787,17 @@ __global__ void __launch_bounds__(256) sweep_i(ZoneView z, DParams P, const DMec
 // previous row through LDS (one redundant face per kRows - 1 cells).
 template <int NS>
 struct JkRows {
-  // 8 rows (512 threads, <= 256 VGPRs) for air; 4 for mixtures, whose face
-  // arithmetic needs more registers than a 512-thread block allows
+  // 8 rows (512 threads, <= 256 VGPRs) for laminar air; 4 with transported
+  // scalars, whose face arithmetic needs more registers than a 512-thread
+  // block allows
   static constexpr int value = NS > 0 ? 4 : 8;
 };
 
-template <int DIR, int NS, int SCHEME, typename R>
-__global__ void __launch_bounds__(64 * JkRows<NS>::value) sweep_jk(ZoneView z, DParams P,
-                                                                   const DMech *__restrict__ m, int init) {
-  constexpr int NV = 5 + NS;
-  constexpr int kRows = JkRows<NS>::value;
+template <int DIR, int NS, int NT, int SCHEME, typename R>
+__global__ void __launch_bounds__(64 * JkRows<NS + NT>::value) sweep_jk(ZoneView z, DParams P,
+                                                                        const DMech *__restrict__ m, int init) {
+  constexpr int NV = 5 + NS + NT;
+  constexpr int kRows = JkRows<NS + NT>::value;
   __shared__ R sF[kRows][NV][64];
   __shared__ R sG[kRows][5][64];
   const int lane = threadIdx.x, r = threadIdx.y;
@@ -772,7 +815,7 @@ __global__ void __launch_bounds__(64 * JkRows<NS>::value) sweep_jk(ZoneView z, D
   for (int l = 0; l < 5; ++l) G[l] = R(0);
   if (live) {
     const int64_t p0 = DIR == 1 ? z.at(i, s, o) : z.at(i, o, s);
-    face_flux<NS, SCHEME, R>(z, P, m, DIR, p0, st, s <= ncds - 1, F, G);
+    face_flux<NS, NT, SCHEME, R>(z, P, m, DIR, p0, st, s <= ncds - 1, F, G);
   }
 #pragma unroll
   for (int l = 0; l < NV; ++l) sF[r][l][lane] = F[l];
@@ -786,25 +829,25 @@ __global__ void __launch_bounds__(64 * JkRows<NS>::value) sweep_jk(ZoneView z, D
 #pragma unroll
     for (int l = 0; l < 5; ++l) Gp[l] = sG[r - 1][l][lane];
     const int64_t q = DIR == 1 ? z.at0(i, s, o) : z.at0(i, o, s);
-    add_dq<NS, R>(z, q, init != 0, F, Fp, s < ncds, G, Gp);
+    add_dq<NS, NT, R>(z, q, init != 0, F, Fp, s < ncds, G, Gp);
   }
 }
 
 }  // namespace
 
-template <int NS, int SCHEME, typename R>
+template <int NS, int NT, int SCHEME, typename R>
 hipError_t launch_sweep_t(const ZoneView &z, const DParams &P, const DMech *m, int dir, bool init, hipStream_t s) {
   if (dir == 0) {
     const dim3 gr((z.mx + 62) / 63, (z.my + 3) / 4, z.mz);
-    hipLaunchKernelGGL((sweep_i<NS, SCHEME, R>), gr, dim3(64, 4, 1), 0, s, z, P, m, init ? 1 : 0);
+    hipLaunchKernelGGL((sweep_i<NS, NT, SCHEME, R>), gr, dim3(64, 4, 1), 0, s, z, P, m, init ? 1 : 0);
   } else if (dir == 1) {
-    constexpr int kRows = JkRows<NS>::value;
+    constexpr int kRows = JkRows<NS + NT>::value;
     const dim3 gr((z.mx + 63) / 64, z.mz, (z.my + kRows - 2) / (kRows - 1));
-    hipLaunchKernelGGL((sweep_jk<1, NS, SCHEME, R>), gr, dim3(64, kRows, 1), 0, s, z, P, m, init ? 1 : 0);
+    hipLaunchKernelGGL((sweep_jk<1, NS, NT, SCHEME, R>), gr, dim3(64, kRows, 1), 0, s, z, P, m, init ? 1 : 0);
   } else {
-    constexpr int kRows = JkRows<NS>::value;
+    constexpr int kRows = JkRows<NS + NT>::value;
     const dim3 gr((z.mx + 63) / 64, z.my, (z.mz + kRows - 2) / (kRows - 1));
-    hipLaunchKernelGGL((sweep_jk<2, NS, SCHEME, R>), gr, dim3(64, kRows, 1), 0, s, z, P, m, init ? 1 : 0);
+    hipLaunchKernelGGL((sweep_jk<2, NS, NT, SCHEME, R>), gr, dim3(64, kRows, 1), 0, s, z, P, m, init ? 1 : 0);
   }
   return hipGetLastError();
 }
@@ -812,16 +855,28 @@ hipError_t launch_sweep_t(const ZoneView &z, const DParams &P, const DMech *m, i
 hipError_t launch_flux_sweep(const ZoneView &z, const DParams &P, const DMech *m, int dir, bool init, bool fp32,
                              hipStream_t s) {
   const bool aweno = P.inviscid_type == 1;
-  if (P.ns == 0) {
-    if (aweno)
-      return fp32 ? launch_sweep_t<0, 1, float>(z, P, m, dir, init, s)
-                  : launch_sweep_t<0, 1, double>(z, P, m, dir, init, s);
-    return fp32 ? launch_sweep_t<0, 0, float>(z, P, m, dir, init, s)
-                : launch_sweep_t<0, 0, double>(z, P, m, dir, init, s);
+  if (P.nt == 0) {
+    if (P.ns == 0) {
+      if (aweno)
+        return fp32 ? launch_sweep_t<0, 0, 1, float>(z, P, m, dir, init, s)
+                    : launch_sweep_t<0, 0, 1, double>(z, P, m, dir, init, s);
+      return fp32 ? launch_sweep_t<0, 0, 0, float>(z, P, m, dir, init, s)
+                  : launch_sweep_t<0, 0, 0, double>(z, P, m, dir, init, s);
+    }
+    if (P.ns == 9 && !aweno)
+      return fp32 ? launch_sweep_t<9, 0, 0, float>(z, P, m, dir, init, s)
+                  : launch_sweep_t<9, 0, 0, double>(z, P, m, dir, init, s);
+    return hipErrorNotSupported;
   }
-  if (P.ns == 9 && !aweno)
-    return fp32 ? launch_sweep_t<9, 0, float>(z, P, m, dir, init, s)
-                : launch_sweep_t<9, 0, double>(z, P, m, dir, init, s);
+  // SST: k and omega are transported after the species; the characteristic
+  // AWENO reconstruction does not carry them (AWENO.cu:53-54)
+  if (P.nt != 2 || aweno) return hipErrorNotSupported;
+  if (P.ns == 0)
+    return fp32 ? launch_sweep_t<0, 2, 0, float>(z, P, m, dir, init, s)
+                : launch_sweep_t<0, 2, 0, double>(z, P, m, dir, init, s);
+  if (P.ns == 9)
+    return fp32 ? launch_sweep_t<9, 2, 0, float>(z, P, m, dir, init, s)
+                : launch_sweep_t<9, 2, 0, double>(z, P, m, dir, init, s);
   return hipErrorNotSupported;
 }
 

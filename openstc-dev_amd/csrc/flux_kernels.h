@@ -18,6 +18,7 @@ struct BCDesc {
   int isothermal;
   double a_inf, gamma_inf, entropy, mw_inf;  // farfield state (rho..vel, sv above)
   double p_total, t_total, p_back;           // subsonic inflow (u, v, w: direction), back pressure
+  double mut;                                // turbulent viscosity of the inflow / farfield state
 };
 
 // per-step scratch kept on the device so a run needs no host round trip
@@ -34,7 +35,8 @@ struct StepScratch {
 // one direction (0 = i, 1 = j, 2 = k); init: the sweep writes dq instead of accumulating
 hipError_t launch_flux_sweep(const ZoneView &z, const DParams &P, const DMech *m, int dir, bool init, bool fp32,
                              hipStream_t s);
-hipError_t launch_source(const ZoneView &z, const DParams &P, const DMech *m, hipStream_t s);
+// finite-rate source; accumulate: add to dq (already initialised by launch_sst)
+hipError_t launch_source(const ZoneView &z, const DParams &P, const DMech *m, bool accumulate, hipStream_t s);
 hipError_t launch_local_dt(const ZoneView &z, const DParams &P, const StepScratch &w, double *dt_out,
                            hipStream_t s);
 hipError_t launch_rk_update(const ZoneView &z, const DParams &P, const DMech *m, const double *dt, int rk,
@@ -59,6 +61,16 @@ hipError_t launch_dplur(const ZoneView &z, const DParams &P, const DMech *m, int
                         const std::vector<BCDesc> &walls, hipStream_t s);
 // local-dt update of bv / sv from dq (cv untouched)
 hipError_t launch_update_bv(const ZoneView &z, const DParams &P, const DMech *m, hipStream_t s);
+// ---- k-omega SST (sst.hip)
+// wall distance of every ghosted point: xyz the ghosted node coordinates
+// (3 x sz, component-major), wall the nw wall points (3 x nw, component-major)
+hipError_t launch_wall_distance(const ZoneView &z, const double *xyz, const double *wall, int nw, hipStream_t s);
+// mu_t on [-1, m] from the current state (initialisation)
+hipError_t launch_init_mut(const ZoneView &z, const DParams &P, const DMech *m, hipStream_t s);
+// mu_t, lambda_t, the source-Jacobian diagonal, and dq = (0, ..., 0, S_k, S_omega) on the interior
+hipError_t launch_sst(const ZoneView &z, const DParams &P, hipStream_t s);
+// explicit steady step: dq_turb /= 1 - dt J_turb
+hipError_t launch_turb_implicit(const ZoneView &z, const DParams &P, hipStream_t s);
 // smallest singular value of I - dt J per cell (conditioning of the point-implicit step)
 hipError_t launch_epi_sigma(const DMech *m, int64_t n, const double *rho, const double *T, const double *Y, double dt,
                             double *sig, hipStream_t s);

@@ -59,10 +59,10 @@ __global__ void __launch_bounds__(kTB) k_chem_implicit(ZoneView z, int method) {
   }
 }
 
-// DPLUR.cuh:14-78 compute_DQ_0, then dq0 = dq
-template <int NS>
-__global__ void __launch_bounds__(kTB) k_dplur_dq0(ZoneView z, int fr, int chem) {
-  constexpr int NV = 5 + NS;
+// DPLUR.cuh:14-85 compute_DQ_0 (SST rows: SST.cu:249-258), then dq0 = dq
+template <int NS, int NT>
+__global__ void __launch_bounds__(kTB) k_dplur_dq0(ZoneView z, int fr, int chem, int timp) {
+  constexpr int NV = 5 + NS + NT;
   int i, j, k;
   if (!cell_index(z, i, j, k)) return;
   const int64_t q = z.at0(i, j, k);
@@ -85,6 +85,8 @@ __global__ void __launch_bounds__(kTB) k_dplur_dq0(ZoneView z, int fr, int chem)
     }
   }
 #pragma unroll
+  for (int l = 0; l < NT; ++l) d[5 + NS + l] /= timp ? diag - dt * z.turb_jac[(int64_t)l * z.sz0 + q] : diag;
+#pragma unroll
   for (int l = 0; l < NV; ++l) {
     z.dq[(int64_t)l * z.sz0 + q] = d[l];
     z.dq0[(int64_t)l * z.sz0 + q] = d[l];
@@ -92,10 +94,10 @@ __global__ void __launch_bounds__(kTB) k_dplur_dq0(ZoneView z, int fr, int chem)
 }
 
 // DPLUR.cuh:80-137: (A +- lambda I) dq at point (i, j, k) along dir
-template <int NS>
+template <int NS, int NT>
 __device__ inline void jac_times_dq(const ZoneView &z, const DMech *__restrict__ m, int i, int j, int k, int dir,
                                     double lam, double *out) {
-  constexpr int NV = 5 + NS;
+  constexpr int NV = 5 + NS + NT;
   const int64_t p = z.at(i, j, k), q = z.at0(i, j, k);
   const double mx = z.metric[(dir * 3 + 0) * z.sz + p], my = z.metric[(dir * 3 + 1) * z.sz + p],
                mz = z.metric[(dir * 3 + 2) * z.sz + p];
@@ -132,13 +134,14 @@ __device__ inline void jac_times_dq(const ZoneView &z, const DMech *__restrict__
   out[3] = w * b1 + mz * b2 + l1 * d[3] + mz * (b3 - b4);
   out[4] = h * b1 + U * b2 + l1 * d[4] + U * (b3 - b4);
 #pragma unroll
-  for (int l = 0; l < NS; ++l) out[5 + l] = l1 * d[5 + l] + z.sv[(int64_t)l * z.sz + p] * b1;
+  for (int l = 0; l < NS + NT; ++l) out[5 + l] = l1 * d[5 + l] + z.sv[(int64_t)l * z.sz + p] * b1;
 }
 
 // DPLUR.cuh:139-257: dqk from the neighbours' current dq (interior only)
-template <int NS>
-__global__ void __launch_bounds__(kTB) k_dplur_inner(ZoneView z, const DMech *__restrict__ m, int fr, int chem) {
-  constexpr int NV = 5 + NS;
+template <int NS, int NT>
+__global__ void __launch_bounds__(kTB) k_dplur_inner(ZoneView z, const DMech *__restrict__ m, int fr, int chem,
+                                                     int timp) {
+  constexpr int NV = 5 + NS + NT;
   int i, j, k;
   if (!cell_index(z, i, j, k)) return;
   double tot[NV], jd[NV];
@@ -149,7 +152,7 @@ __global__ void __launch_bounds__(kTB) k_dplur_inner(ZoneView z, const DMech *__
   for (int dd = 0; dd < 3; ++dd) {
     if (c[dd] > 0) {
       const int ni = i - (dd == 0), nj = j - (dd == 1), nk = k - (dd == 2);
-      jac_times_dq<NS>(z, m, ni, nj, nk, dd, z.inv_sr[(int64_t)dd * z.sz0 + z.at0(ni, nj, nk)], jd);
+      jac_times_dq<NS, NT>(z, m, ni, nj, nk, dd, z.inv_sr[(int64_t)dd * z.sz0 + z.at0(ni, nj, nk)], jd);
 #pragma unroll
       for (int l = 0; l < NV; ++l) tot[l] += 0.5 * jd[l];
     }
@@ -158,7 +161,7 @@ __global__ void __launch_bounds__(kTB) k_dplur_inner(ZoneView z, const DMech *__
   for (int dd = 0; dd < 3; ++dd) {
     if (c[dd] != ext[dd] - 1) {
       const int ni = i + (dd == 0), nj = j + (dd == 1), nk = k + (dd == 2);
-      jac_times_dq<NS>(z, m, ni, nj, nk, dd, -z.inv_sr[(int64_t)dd * z.sz0 + z.at0(ni, nj, nk)], jd);
+      jac_times_dq<NS, NT>(z, m, ni, nj, nk, dd, -z.inv_sr[(int64_t)dd * z.sz0 + z.at0(ni, nj, nk)], jd);
 #pragma unroll
       for (int l = 0; l < NV; ++l) tot[l] -= 0.5 * jd[l];
     }
@@ -186,6 +189,13 @@ __global__ void __launch_bounds__(kTB) k_dplur_inner(ZoneView z, const DMech *__
       for (int l = 0; l < NS; ++l)
         z.dqk[(int64_t)(5 + l) * z.sz0 + q] = z.dq0[(int64_t)(5 + l) * z.sz0 + q] + dt * tot[5 + l] / diag;
     }
+  }
+  // SST.cu:260-273 implicit_treat_for_dqk
+#pragma unroll
+  for (int l = 0; l < NT; ++l) {
+    const int c = 5 + NS + l;
+    const double dg = timp ? diag - dt * z.turb_jac[(int64_t)l * z.sz0 + q] : diag;
+    z.dqk[(int64_t)c * z.sz0 + q] = z.dq0[(int64_t)c * z.sz0 + q] + dt * tot[c] / dg;
   }
 }
 
@@ -287,12 +297,20 @@ hipError_t launch_dplur(const ZoneView &z, const DParams &P, const DMech *m, int
   const int nv = P.nv;
   const int fr = P.mix && P.reaction == 1;
   if (!z.inv_sr || !z.dq0 || !z.dqk) return hipErrorInvalidValue;
-  if (P.ns == 0) {
-    hipLaunchKernelGGL(k_dplur_dq0<0>, dim3(nb(n)), dim3(kTB), 0, s, z, fr, P.chemSrcMethod);
-  } else if (P.ns == 9) {
-    hipLaunchKernelGGL(k_dplur_dq0<9>, dim3(nb(n)), dim3(kTB), 0, s, z, fr, P.chemSrcMethod);
+  if ((P.ns != 0 && P.ns != 9) || (P.nt != 0 && P.nt != 2)) return hipErrorNotSupported;
+  const int timp = P.nt && P.turb_implicit == 1;
+  if (timp && !z.turb_jac) return hipErrorInvalidValue;
+  const dim3 g(nb(n)), bk(kTB);
+  if (P.nt == 0) {
+    if (P.ns == 0)
+      hipLaunchKernelGGL((k_dplur_dq0<0, 0>), g, bk, 0, s, z, fr, P.chemSrcMethod, 0);
+    else
+      hipLaunchKernelGGL((k_dplur_dq0<9, 0>), g, bk, 0, s, z, fr, P.chemSrcMethod, 0);
   } else {
-    return hipErrorNotSupported;
+    if (P.ns == 0)
+      hipLaunchKernelGGL((k_dplur_dq0<0, 2>), g, bk, 0, s, z, fr, P.chemSrcMethod, timp);
+    else
+      hipLaunchKernelGGL((k_dplur_dq0<9, 2>), g, bk, 0, s, z, fr, P.chemSrcMethod, timp);
   }
   for (int it = 0; it < inner_steps; ++it) {
     for (const BCDesc &b : walls) {
@@ -303,10 +321,17 @@ hipError_t launch_dplur(const ZoneView &z, const DParams &P, const DMech *m, int
       hipLaunchKernelGGL(k_zero_dq_range, dim3(nb(cnt)), dim3(kTB), 0, s, z, nv, b.rs[0], b.re[0], b.rs[1], b.re[1],
                          b.rs[2], b.re[2]);
     }
-    if (P.ns == 0)
-      hipLaunchKernelGGL(k_dplur_inner<0>, dim3(nb(n)), dim3(kTB), 0, s, z, m, fr, P.chemSrcMethod);
-    else
-      hipLaunchKernelGGL(k_dplur_inner<9>, dim3(nb(n)), dim3(kTB), 0, s, z, m, fr, P.chemSrcMethod);
+    if (P.nt == 0) {
+      if (P.ns == 0)
+        hipLaunchKernelGGL((k_dplur_inner<0, 0>), g, bk, 0, s, z, m, fr, P.chemSrcMethod, 0);
+      else
+        hipLaunchKernelGGL((k_dplur_inner<9, 0>), g, bk, 0, s, z, m, fr, P.chemSrcMethod, 0);
+    } else {
+      if (P.ns == 0)
+        hipLaunchKernelGGL((k_dplur_inner<0, 2>), g, bk, 0, s, z, m, fr, P.chemSrcMethod, timp);
+      else
+        hipLaunchKernelGGL((k_dplur_inner<9, 2>), g, bk, 0, s, z, m, fr, P.chemSrcMethod, timp);
+    }
     hipLaunchKernelGGL(k_copy, dim3(nb(z.sz0 * nv)), dim3(kTB), 0, s, (const double *)z.dqk, z.dq, z.sz0 * nv);
   }
   hipLaunchKernelGGL(k_dq_back, dim3(nb(n)), dim3(kTB), 0, s, z, nv);

@@ -100,7 +100,8 @@ bool point_metric(const Layout &Lc, const double *X, const double *Y, const doub
 }  // namespace
 
 bool compute_metrics(int mx, int my, int mz, int ngg, const double *x, const double *y, const double *z,
-                     std::vector<double> &jac, std::vector<double> &metric, std::string &err) {
+                     std::vector<double> &jac, std::vector<double> &metric, std::string &err,
+                     std::vector<double> *xyz) {
   const Layout Lc(mx, my, mz, ngg + 1), L(mx, my, mz, ngg);
   std::vector<double> X(Lc.sz, 0), Y(Lc.sz, 0), Z(Lc.sz, 0);
   for (int k = 0; k < mz; ++k)
@@ -156,6 +157,18 @@ bool compute_metrics(int mx, int my, int mz, int ngg, const double *x, const dou
   for (int i = mx - 1; i < mx + ngg; ++i)
     for (int j = -ngg; j < my + ngg; ++j)
       for (int k = -ngg; k < mz + ngg; ++k) put(i, j, k, -1, 0, 0);
+  if (xyz) {
+    // node coordinates on the ngg-ghosted layout, component-major
+    xyz->assign((size_t)L.sz * 3, 0);
+    for (int k = -ngg; k < mz + ngg; ++k)
+      for (int j = -ngg; j < my + ngg; ++j)
+        for (int i = -ngg; i < mx + ngg; ++i) {
+          const int64_t p = L.at(i, j, k), q = Lc.at(i, j, k);
+          (*xyz)[p] = X[q];
+          (*xyz)[(size_t)L.sz + p] = Y[q];
+          (*xyz)[(size_t)2 * L.sz + p] = Z[q];
+        }
+  }
   jac.assign(L.sz, 0);
   metric.assign((size_t)L.sz * 9, 0);
   for (int64_t p = 0; p < L.sz; ++p) {

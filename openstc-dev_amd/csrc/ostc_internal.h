@@ -87,10 +87,12 @@ struct Layout {
 // Ghost nodes (Mesh.cpp:790-944) and Jacobian/metrics (Mesh.cpp:259-521).
 // x,y,z interior node coordinates with i fastest.  Outputs: jac (Lg) and
 // metric as 9 component-major arrays (metric[(r*3+c)*Lg.sz + p] =
-// d xi_r / d x_c), both on the ngg-ghosted layout.  Returns false on a
-// non-positive Jacobian in the interior.
+// d xi_r / d x_c), both on the ngg-ghosted layout; xyz (optional) gets the
+// ghosted node coordinates.  Returns false on a non-positive Jacobian in the
+// interior.
 bool compute_metrics(int mx, int my, int mz, int ngg, const double *x, const double *y, const double *z,
-                     std::vector<double> &jac, std::vector<double> &metric, std::string &err);
+                     std::vector<double> &jac, std::vector<double> &metric, std::string &err,
+                     std::vector<double> *xyz = nullptr);
 
 // ------------------------------------------------------------ boundaries
 enum BcType { BC_WALL = 2, BC_SYMMETRY = 3, BC_FARFIELD = 4, BC_INFLOW = 5, BC_OUTFLOW = 6, BC_SUBSONIC = 7,

@@ -5,6 +5,7 @@
 #include "device_chem.h"
 #include "device_common.h"
 #include "flux_kernels.h"
+#include "sst_device.h"
 
 #include <algorithm>
 
@@ -36,8 +37,10 @@ __device__ inline double wave_sum(double v) {
 // ------------------------------------------------------------- source
 // SourceTerm.cuh:11-37 (finite-rate part): dq = 0 for the flow components,
 // dq(5+l) = J omega_l, and the chemistry Jacobian (EPI) or its diagonal (DA).
+// With accumulate (the SST kernel already initialised dq) the species rows
+// are added to and the flow rows left alone.
 template <int NS>
-__global__ void __launch_bounds__(kTB) k_source(ZoneView z, DParams P, const DMech *__restrict__ m) {
+__global__ void __launch_bounds__(kTB) k_source(ZoneView z, DParams P, const DMech *__restrict__ m, int accumulate) {
   const int64_t t = (int64_t)blockIdx.x * kTB + threadIdx.x;
   const int64_t n = (int64_t)z.mx * z.my * z.mz;
   if (t >= n) return;
@@ -61,6 +64,11 @@ __global__ void __launch_bounds__(kTB) k_source(ZoneView z, DParams P, const DMe
     for (int l = 0; l < NS; ++l) z.chem_jac[(int64_t)l * z.sz0 + q] = Y[l] > 1e-25 ? -od[l] / (Y[l] * rho) : 0.0;
   } else {
     d_chem_source<NS>(m, rho, T, Y, om, nullptr);
+  }
+  if (accumulate) {
+#pragma unroll
+    for (int l = 0; l < NS; ++l) z.dq[(int64_t)(5 + l) * z.sz0 + q] += J * om[l];
+    return;
   }
 #pragma unroll
   for (int l = 0; l < 5; ++l) z.dq[(int64_t)l * z.sz0 + q] = 0.0;
@@ -105,7 +113,8 @@ __global__ void __launch_bounds__(kTB) k_local_dt(ZoneView z, DParams P, double 
     }
     const double gamma = MIX ? z.gam[p] : kGam;
     const double c1 = fmax(gamma, 4.0 / 3.0) / z.bv[p];
-    const double c2 = z.mul[p] / P.Pr;
+    double c2 = z.mul[p] / P.Pr;
+    if (z.mut) c2 += z.mut[p] / P.Prt;
     double srv = gxi * gxi + geta * geta;
     if (P.dim == 3) srv += gzeta * gzeta;
     srv *= c1 * c2;
@@ -166,6 +175,13 @@ __global__ void __launch_bounds__(kTB) k_rk_update(ZoneView z, DParams P, const 
 #pragma unroll
   for (int l = 0; l < NV; ++l) z.cv[(int64_t)l * z.sz + p] = cv[l];
   const double ri = 1.0 / cv[0];
+  // turbulence scalars (rho k, rho omega) after the species
+  for (int l = NV; l < P.nv; ++l) {
+    const int64_t o = (int64_t)l * z.sz + p;
+    const double c = A * z.qn[o] + B * z.cv[o] + C * dtj * z.dq[(int64_t)l * z.sz0 + q];
+    z.cv[o] = c;
+    z.sv[(int64_t)(l - 5) * z.sz + p] = c * ri;
+  }
   const double u = cv[1] * ri, v = cv[2] * ri, w = cv[3] * ri;
   z.bv[p] = cv[0];
   z.bv[z.sz + p] = u;
@@ -193,8 +209,9 @@ __global__ void __launch_bounds__(kTB) k_rk_update(ZoneView z, DParams P, const 
 
 // FieldOperation.cuh:18-110 compute_total_energy / compute_cv_from_bv_1_point
 template <int NS>
-__device__ inline void cv_point(const ZoneView &z, const DMech *__restrict__ m, int64_t p) {
+__device__ inline void cv_point(const ZoneView &z, const DMech *__restrict__ m, int64_t p, int nt) {
   const double rho = z.bv[p], u = z.bv[z.sz + p], v = z.bv[2 * z.sz + p], w = z.bv[3 * z.sz + p];
+  for (int l = NS; l < NS + nt; ++l) z.cv[(int64_t)(5 + l) * z.sz + p] = rho * z.sv[(int64_t)l * z.sz + p];
   z.cv[p] = rho;
   z.cv[z.sz + p] = rho * u;
   z.cv[2 * z.sz + p] = rho * v;
@@ -221,11 +238,11 @@ __device__ inline void cv_point(const ZoneView &z, const DMech *__restrict__ m, 
 }
 
 template <int NS>
-__global__ void __launch_bounds__(kTB) k_cv_from_bv(ZoneView z, const DMech *__restrict__ m) {
+__global__ void __launch_bounds__(kTB) k_cv_from_bv(ZoneView z, const DMech *__restrict__ m, int nt) {
   const int X = z.mx + 2 * z.g, Y = z.my + 2 * z.g, Z = z.mz + 2 * z.g;
   const int64_t t = (int64_t)blockIdx.x * kTB + threadIdx.x;
   if (t >= (int64_t)X * Y * Z) return;
-  cv_point<NS>(z, m, t);
+  cv_point<NS>(z, m, t, nt);
 }
 
 __global__ void __launch_bounds__(kTB) k_velocity(ZoneView z) {
@@ -285,6 +302,7 @@ __global__ void __launch_bounds__(kTB) k_bc(ZoneView z, DParams P, const DMech *
       bv[4 * S + pg] = bv[4 * S + pm];
       bv[5 * S + pg] = bv[5 * S + pm];
       for (int l = 0; l < P.nsc; ++l) sv[l * S + pg] = sv[l * S + pm];
+      if (z.mut) z.mut[pg] = z.mut[pm];
     }
   } else if (b.type == 6) {
     // outflow (BoundCond.cuh:120-148)
@@ -293,7 +311,8 @@ __global__ void __launch_bounds__(kTB) k_bc(ZoneView z, DParams P, const DMech *
 #pragma unroll
       for (int l = 0; l < 6; ++l) bv[l * S + pg] = bv[l * S + p];
       for (int l = 0; l < P.nsc; ++l) sv[l * S + pg] = sv[l * S + p];
-      if (WITH_CV) cv_point<NS>(z, m, pg);
+      if (z.mut) z.mut[pg] = z.mut[p];
+      if (WITH_CV) cv_point<NS>(z, m, pg, P.nt);
     }
   } else if (b.type == 5) {
     // inflow (BoundCond.cuh:150-212)
@@ -306,8 +325,9 @@ __global__ void __launch_bounds__(kTB) k_bc(ZoneView z, DParams P, const DMech *
       bv[4 * S + pg] = b.p;
       bv[5 * S + pg] = b.t;
       for (int l = 0; l < P.nsc; ++l) sv[l * S + pg] = b.sv[l];
+      if (z.mut) z.mut[pg] = b.mut;
       if (g == 0) z.vel[pg] = b.vel;
-      if (WITH_CV) cv_point<NS>(z, m, pg);
+      if (WITH_CV) cv_point<NS>(z, m, pg, P.nt);
     }
   } else if (b.type == 4) {
     // farfield (BoundCond.cuh:213-424); never refreshes ghost cv
@@ -340,6 +360,7 @@ __global__ void __launch_bounds__(kTB) k_bc(ZoneView z, DParams P, const DMech *
 #pragma unroll
         for (int l = 0; l < 6; ++l) bv[l * S + pg] = bv[l * S + p];
         for (int l = 0; l < P.nsc; ++l) sv[l * S + pg] = sv[l * S + p];
+        if (z.mut) z.mut[pg] = z.mut[p];
       }
       return;
     }
@@ -414,6 +435,7 @@ __global__ void __launch_bounds__(kTB) k_bc(ZoneView z, DParams P, const DMech *
       st[4] = pr;
       st[5] = t;
     }
+    const double mutb = z.mut ? (far_sv ? b.mut : z.mut[p]) : 0.0;
     for (int g = 0; g <= z.g; ++g) {
       const int64_t pg = p + g * sd;
 #pragma unroll
@@ -424,6 +446,7 @@ __global__ void __launch_bounds__(kTB) k_bc(ZoneView z, DParams P, const DMech *
           if (!far_sv && l < NS) y = svb[l];
         sv[l * S + pg] = y;
       }
+      if (z.mut) z.mut[pg] = mutb;
     }
     z.vel[p] = sqrt(st[1] * st[1] + st[2] * st[2] + st[3] * st[3]);
   } else if (b.type == 7) {
@@ -462,7 +485,8 @@ __global__ void __launch_bounds__(kTB) k_bc(ZoneView z, DParams P, const DMech *
       bv[5 * S + pg] = t;
       const double ubar = bv[S + pg] * nx + bv[2 * S + pg] * ny + bv[3 * S + pg] * nz;
       for (int l = 0; l < P.nsc; ++l) sv[l * S + pg] = ubar > 0 ? sv[l * S + p] : b.sv[l];
-      if (WITH_CV) cv_point<NS>(z, m, pg);
+      if (z.mut) z.mut[pg] = z.mut[p];
+      if (WITH_CV) cv_point<NS>(z, m, pg, P.nt);
     }
   } else if (b.type == 9) {
     // back pressure (BoundCond.cuh:638-670)
@@ -473,7 +497,8 @@ __global__ void __launch_bounds__(kTB) k_bc(ZoneView z, DParams P, const DMech *
       bv[4 * S + pg] = b.p_back;
       bv[5 * S + pg] = bv[5 * S + p];
       for (int l = 0; l < P.nsc; ++l) sv[l * S + pg] = sv[l * S + p];
-      if (WITH_CV) cv_point<NS>(z, m, pg);
+      if (z.mut) z.mut[pg] = z.mut[p];
+      if (WITH_CV) cv_point<NS>(z, m, pg, P.nt);
     }
   } else if (b.type == 2) {
     // no-slip wall (BoundCond.cuh:424-552)
@@ -491,14 +516,29 @@ __global__ void __launch_bounds__(kTB) k_bc(ZoneView z, DParams P, const DMech *
       }
       mw = 1 / s;
     }
-    bv[p] = pw * mw / (tw * kRuD);
+    const double rw = pw * mw / (tw * kRuD);
+    bv[p] = rw;
     bv[S + p] = 0;
     bv[2 * S + p] = 0;
     bv[3 * S + p] = 0;
     bv[4 * S + p] = pw;
     bv[5 * S + p] = tw;
     z.vel[p] = 0;
-    if (WITH_CV) cv_point<NS>(z, m, p);
+    if (P.nt) {
+      // SST wall values: k = 0, omega = 60 mu_w / (rho_w beta1 d^2) with the
+      // wall distance of the first interior point
+      double mu_w = d_sutherland(tw);
+      if constexpr (NS > 0) {
+        double Yw[NS];
+#pragma unroll
+        for (int l = 0; l < NS; ++l) Yw[l] = sv[l * S + p];
+        mu_w = d_viscosity<NS>(m, tw, mw, Yw);
+      }
+      const double dy = z.wdist[pi];
+      sv[NS * S + p] = 0;
+      sv[(NS + 1) * S + p] = dy > 1e-25 ? 60 * mu_w / (rw * sst::kBeta1 * dy * dy) : sv[(NS + 1) * S + pi];
+    }
+    if (WITH_CV) cv_point<NS>(z, m, p, P.nt);
     for (int g = 1; g <= z.g; ++g) {
       const int64_t pa = p - g * sd, pg = p + g * sd;
       const double ui = bv[S + pa], vi = bv[2 * S + pa], wi = bv[3 * S + pa], pii = bv[4 * S + pa],
@@ -524,7 +564,12 @@ __global__ void __launch_bounds__(kTB) k_bc(ZoneView z, DParams P, const DMech *
       bv[3 * S + pg] = -wi;
       bv[4 * S + pg] = pii;
       bv[5 * S + pg] = tg;
-      if (WITH_CV) cv_point<NS>(z, m, pg);
+      if (P.nt) {
+        sv[NS * S + pg] = 0;
+        sv[(NS + 1) * S + pg] = sv[(NS + 1) * S + p];
+        z.mut[pg] = 0;
+      }
+      if (WITH_CV) cv_point<NS>(z, m, pg, P.nt);
     }
   }
 }
@@ -795,10 +840,10 @@ inline unsigned nblk(int64_t n, int tb) { return (unsigned)((n + tb - 1) / tb); 
 
 }  // namespace
 
-hipError_t launch_source(const ZoneView &z, const DParams &P, const DMech *m, hipStream_t s) {
+hipError_t launch_source(const ZoneView &z, const DParams &P, const DMech *m, bool accumulate, hipStream_t s) {
   const int64_t n = (int64_t)z.mx * z.my * z.mz;
   if (P.ns != 9) return hipErrorNotSupported;
-  hipLaunchKernelGGL(k_source<9>, dim3(nblk(n, kTB)), dim3(kTB), 0, s, z, P, m);
+  hipLaunchKernelGGL(k_source<9>, dim3(nblk(n, kTB)), dim3(kTB), 0, s, z, P, m, accumulate ? 1 : 0);
   return hipGetLastError();
 }
 
@@ -876,9 +921,9 @@ hipError_t launch_velocity(const ZoneView &z, hipStream_t s) {
 hipError_t launch_cv_from_bv(const ZoneView &z, const DParams &P, const DMech *m, hipStream_t s) {
   const int64_t n = (int64_t)(z.mx + 2 * z.g) * (z.my + 2 * z.g) * (z.mz + 2 * z.g);
   if (P.ns == 0)
-    hipLaunchKernelGGL(k_cv_from_bv<0>, dim3(nblk(n, kTB)), dim3(kTB), 0, s, z, m);
+    hipLaunchKernelGGL(k_cv_from_bv<0>, dim3(nblk(n, kTB)), dim3(kTB), 0, s, z, m, P.nt);
   else if (P.ns == 9)
-    hipLaunchKernelGGL(k_cv_from_bv<9>, dim3(nblk(n, kTB)), dim3(kTB), 0, s, z, m);
+    hipLaunchKernelGGL(k_cv_from_bv<9>, dim3(nblk(n, kTB)), dim3(kTB), 0, s, z, m, P.nt);
   else
     return hipErrorNotSupported;
   return hipGetLastError();

@@ -12,7 +12,8 @@ PKG = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 LIB_PATH = os.path.join(PKG, "libostc.so")
 
 FIELDS = dict(bv=0, sv=1, cv=2, dq=3, vel=4, acoustic=5, mach=6, mul=7, cond=8, rhoD=9, gamma=10, cp=11,
-              dt_local=12, qn=13, bv_last=14, jac=15, metric=16, chem_jac=17)
+              dt_local=12, qn=13, bv_last=14, jac=15, metric=16, chem_jac=17, mut=18, turb_cond=19,
+              wall_distance=20, turb_jac=21)
 
 
 class OstcError(RuntimeError):

@@ -308,6 +308,204 @@ def subsonic_case(nx=60, ny=30, seed=1006, mach=0.5, p=101325.0, t=300.0, pt_rat
                 inflow=inflow, seed=seed, perturb="density", steps=10)
 
 
+def sst_state(st, mu, ratio, intensity):
+    """BoundCond.cpp:83-95: k = 1.5 (|V| I)^2, mu_t = ratio mu,
+    omega = rho k / mu_t, appended to the scalars of state `st`."""
+    mut = ratio * mu
+    k = 1.5 * st["vel"] ** 2 * intensity ** 2
+    return dict(st, sv=list(st.get("sv", [])) + [k, st["rho"] * k / mut], mut=mut)
+
+
+def stretched(n, length, first):
+    """n nodes on [0, length] growing geometrically from spacing `first`."""
+    lo, hi = 1.0, 2.0
+    for _ in range(200):
+        r = 0.5 * (lo + hi)
+        total = first * (r ** (n - 1) - 1) / (r - 1)
+        lo, hi = (r, hi) if total < length else (lo, r)
+    r = 0.5 * (lo + hi)
+    y = np.concatenate([[0.0], np.cumsum(first * r ** np.arange(n - 1))])
+    return y * (length / y[-1])
+
+
+def two_wall(n, length, first):
+    """n nodes on [0, length] clustered symmetrically towards both ends,
+    y = L/2 (1 + tanh(b (2s - 1)) / tanh b) on uniform s, with b chosen so
+    that the first spacing equals `first`."""
+    s_ = np.linspace(0.0, 1.0, n)
+    lo, hi = 1e-6, 20.0
+    for _ in range(200):
+        b = 0.5 * (lo + hi)
+        y1 = 0.5 * length * (1 + np.tanh(b * (2 * s_[1] - 1)) / np.tanh(b))
+        lo, hi = (b, hi) if y1 > first else (lo, b)
+    b = 0.5 * (lo + hi)
+    y = 0.5 * length * (1 + np.tanh(b * (2 * s_ - 1)) / np.tanh(b))
+    y[0], y[-1] = 0.0, length
+    return y
+
+
+def sst_plate_case(nx=64, ny=48, seed=1007, mach=2.0, p=20000.0, t=300.0, t_wall=300.0, lx=0.1, ly=0.02,
+                   first=2e-5, le=8, intensity=0.01, ratio=10.0, compressibility=3, inviscid_scheme=3):
+    """2-D turbulent (k-omega SST) supersonic flat plate in air: Mach 2 inflow
+    at i=0 carrying k, omega from a turbulence intensity and viscosity ratio,
+    outflow at i=max and j=max, symmetry on j=0 ahead of the leading edge
+    (node `le`) and an isothermal no-slip wall behind it; wall-normal
+    geometric stretching from spacing `first`.  AUSM+ with MUSCL-minmod,
+    second-order viscous fluxes with mu + mu_t, SSP-RK3 (or the steady path
+    when the caller switches it) at CFL 0.5; Zeman compressibility
+    correction by default."""
+    xi = np.linspace(0.0, lx, nx)
+    yj = stretched(ny, ly, first)
+    X, Y = np.meshgrid(xi, yj, indexing="xy")
+    x, y = X[None].copy(), Y[None].copy()
+    z = np.zeros_like(x)
+    inflow = sst_state(air_inflow(mach, p, t), sutherland(t), ratio, intensity)
+    setup = _setup_common({
+        "int dimension": 2, "real cfl": 0.5, "int inviscid_scheme": inviscid_scheme, "int reconstruction": 2,
+        "int limiter": 0, "int viscous_order": 2, "int total_step": 10, "bool turbulence": 1,
+        "int turbulence_method": 1, "int RANS_model": 2, "int turb_implicit": 1,
+        "int compressibility_correction": compressibility,
+        "array string boundary_conditions": ["freestream", "outflow", "symmetry", "wall"],
+        "string reference_state": "freestream", "string default_init": "freestream",
+        "struct freestream": [("string", "type", "inflow"), ("int", "label", 5), ("real", "mach", mach),
+                              ("real", "pressure", p), ("real", "temperature", t), ("real", "u", 1.0),
+                              ("real", "v", 0.0), ("real", "w", 0.0), ("real", "turbulence_intensity", intensity),
+                              ("real", "turb_viscosity_ratio", ratio)],
+        "struct outflow": [("string", "type", "outflow"), ("int", "label", 6)],
+        "struct symmetry": [("string", "type", "symmetry"), ("int", "label", 3)],
+        "struct wall": [("string", "type", "wall"), ("int", "label", 2), ("string", "thermal_type", "isothermal"),
+                        ("real", "temperature", t_wall)],
+    })
+    bcs = [
+        dict(type=5, raw=(0, 0, 0, ny - 1, 0, 0), label=5),
+        dict(type=6, raw=(nx - 1, nx - 1, 0, ny - 1, 0, 0), label=6),
+        dict(type=6, raw=(0, nx - 1, ny - 1, ny - 1, 0, 0), label=6),
+        dict(type=3, raw=(0, le - 1, 0, 0, 0, 0), label=3),
+        dict(type=2, raw=(le, nx - 1, 0, 0, 0, 0), label=2, t_wall=t_wall, isothermal=True),
+    ]
+    scheme = dict(mix=0, inviscid_scheme=inviscid_scheme, reconstruction=2, limiter=0, viscous_order=2, cfl=0.5,
+                  fixed_time_step=0, output_screen=1, turbulence=1, turb_implicit=1,
+                  compressibility_correction=compressibility)
+    return dict(name="sst_plate", mx=nx, my=ny, mz=1, ngg=2, x=x, y=y, z=z, setup=setup, bcs=bcs, scheme=scheme,
+                inflow=inflow, seed=seed, perturb="density", steps=10, le=le, delta=0.2 * ly)
+
+
+def sst_plate_fields(case):
+    """Uniform inflow, a tanh boundary-layer profile u = U tanh(y / delta)
+    over the plate (nodes i >= le) with T from the recovery-free
+    Crocco-Busemann relation to the wall temperature, then the seeded 1 %
+    density perturbation at fixed pressure; k and omega uniform."""
+    bv, sv = initial_fields(dict(case, perturb=None))
+    mx, my, mz, g = case["mx"], case["my"], case["mz"], case["ngg"]
+    st = case["inflow"]
+    yy = np.zeros(ghost_shape(mx, my, mz, g))
+    yy[:, g:g + my, g:g + mx] = case["y"]
+    f = np.tanh(np.maximum(yy, 0.0) / case["delta"])
+    plate = np.zeros_like(yy, dtype=bool)
+    plate[:, :, g + case["le"]:] = True
+    tw = case["bcs"][-1]["t_wall"]
+    u = np.where(plate, st["u"] * f, st["u"])
+    t = np.where(plate, tw + (st["t"] - tw) * f, st["t"])
+    bv[1] = u
+    bv[5] = t
+    rng = SplitMix64(case["seed"])
+    noise = 1.0 + 0.01 * rng.normal(mx * my * mz).reshape(mz, my, mx)
+    bv[0] = st["p"] * MW_AIR / (R_U * bv[5])
+    sl = (slice(g, g + mz), slice(g, g + my), slice(g, g + mx))
+    bv[0][sl] = bv[0][sl] * noise
+    bv[5][sl] = bv[4][sl] * MW_AIR / (bv[0][sl] * R_U)
+    return bv, sv
+
+
+def c5_ports(nx, nz, lx, lz, x_port=0.25, size=2e-3, count=4):
+    """Node ranges (i0, i1, k0, k1) of `count` square ports of side `size`
+    centred at x = x_port * lx, spread evenly across z."""
+    di = max(1, int(round(size / (lx / (nx - 1)))))
+    dk = max(1, int(round(size / (lz / (nz - 1)))))
+    ic = int(round(x_port * (nx - 1)))
+    ports = []
+    for c in range(count):
+        kc = int(round((c + 0.5) / count * (nz - 1)))
+        ports.append((ic - di // 2, ic - di // 2 + di - 1, kc - dk // 2, kc - dk // 2 + dk - 1))
+    return ports
+
+
+def c5_case(nx=384, ny=192, nz=128, seed=1005, lx=0.2, ly=0.05, lz=0.04, first=5e-6, p=101325.0, t_air=1200.0,
+            mach=2.0, p_jet=202650.0, t_jet=300.0, t_wall=300.0, intensity=0.01, ratio=10.0, inner=3,
+            chem=1, compressibility=3):
+    """3-D reacting RANS (k-omega SST) supersonic H2/air channel with wall
+    injection on one block: walls at j=0 and j=max with geometric stretching
+    from spacing `first` towards both (y+ ~ 1), vitiated-air inflow at i=0
+    (Y_O2 = 0.25, Y_H2O = 0.12, Y_N2 = 0.63; 1200 K, Mach 2), outflow at
+    i=max, symmetry at k=0 and k=max, and sonic H2 jets at 300 K through
+    four 2 mm square ports in the lower wall (inflow patches, normal
+    injection).  9 species plus k and omega; HLLC with MUSCL-minmod (the
+    AWENO interpolation carries neither mixtures nor k, omega), viscous
+    fluxes with mu + mu_t, steady pseudo-time with DPLUR (3 inner sweeps)
+    and the exact point-implicit chemistry, implicit SST source diagonal."""
+    xi = np.linspace(0.0, lx, nx)
+    yj = two_wall(ny, ly, first)
+    zk = np.linspace(0.0, lz, nz)
+    Z, Y, X = np.meshgrid(zk, yj, xi, indexing="ij")
+    x, y, z = X.copy(), Y.copy(), Z.copy()
+    common = [("real", "u", 1.0), ("real", "v", 0.0), ("real", "w", 0.0),
+              ("real", "turbulence_intensity", intensity), ("real", "turb_viscosity_ratio", ratio)]
+    setup = _setup_common({
+        "int dimension": 3, "int species": 1, "int reaction": 1, "int chemSrcMethod": chem, "real cfl": 0.5,
+        "int inviscid_scheme": 4, "int reconstruction": 2, "int limiter": 0, "int viscous_order": 2,
+        "int total_step": 10, "bool steady": 1, "int implicit_method": 1, "int DPLUR_inner_step": inner,
+        "bool turbulence": 1, "int turbulence_method": 1, "int RANS_model": 2, "int turb_implicit": 1,
+        "int compressibility_correction": compressibility,
+        "array string boundary_conditions": ["vitiated_air", "h2_jet", "outflow", "symmetry", "wall"],
+        "string reference_state": "vitiated_air", "string default_init": "vitiated_air",
+        "struct vitiated_air": [("string", "type", "inflow"), ("int", "label", 5), ("real", "mach", mach),
+                                ("real", "pressure", p), ("real", "temperature", t_air), ("real", "O2", 0.25),
+                                ("real", "H2O", 0.12), ("real", "N2", 0.63)] + common,
+        "struct h2_jet": [("string", "type", "inflow"), ("int", "label", 52), ("real", "mach", 1.0),
+                          ("real", "pressure", p_jet), ("real", "temperature", t_jet), ("real", "H2", 1.0),
+                          ("real", "u", 0.0), ("real", "v", 1.0), ("real", "w", 0.0),
+                          ("real", "turbulence_intensity", intensity), ("real", "turb_viscosity_ratio", ratio)],
+        "struct outflow": [("string", "type", "outflow"), ("int", "label", 6)],
+        "struct symmetry": [("string", "type", "symmetry"), ("int", "label", 3)],
+        "struct wall": [("string", "type", "wall"), ("int", "label", 2), ("string", "thermal_type", "isothermal"),
+                        ("real", "temperature", t_wall)],
+    })
+    ports = c5_ports(nx, nz, lx, lz)
+    # tensor partition of the lower wall face into wall and port rectangles
+    ib = sorted({0, nx} | {a for p_ in ports for a in (p_[0], p_[1] + 1)})
+    kb = sorted({0, nz} | {a for p_ in ports for a in (p_[2], p_[3] + 1)})
+    bcs = [
+        dict(type=5, raw=(0, 0, 0, ny - 1, 0, nz - 1), label=5, stream="vitiated_air"),
+        dict(type=6, raw=(nx - 1, nx - 1, 0, ny - 1, 0, nz - 1), label=6),
+        dict(type=3, raw=(0, nx - 1, 0, ny - 1, 0, 0), label=3),
+        dict(type=3, raw=(0, nx - 1, 0, ny - 1, nz - 1, nz - 1), label=3),
+        dict(type=2, raw=(0, nx - 1, ny - 1, ny - 1, 0, nz - 1), label=2, t_wall=t_wall, isothermal=True),
+    ]
+    for a in range(len(ib) - 1):
+        for c in range(len(kb) - 1):
+            i0, i1, k0, k1 = ib[a], ib[a + 1] - 1, kb[c], kb[c + 1] - 1
+            jet = any(p_[0] <= i0 and i1 <= p_[1] and p_[2] <= k0 and k1 <= p_[3] for p_ in ports)
+            if jet:
+                bcs.append(dict(type=5, raw=(i0, i1, 0, 0, k0, k1), label=52, stream="h2_jet"))
+            else:
+                bcs.append(dict(type=2, raw=(i0, i1, 0, 0, k0, k1), label=2, t_wall=t_wall, isothermal=True))
+    scheme = dict(mix=1, reaction=1, chemSrcMethod=chem, inviscid_scheme=4, reconstruction=2, limiter=0,
+                  viscous_order=2, cfl=0.5, fixed_time_step=0, output_screen=1, implicit_method=1,
+                  DPLUR_inner_step=inner, turbulence=1, turb_implicit=1, compressibility_correction=compressibility)
+    return dict(name="c5", mx=nx, my=ny, mz=nz, ngg=2, x=x, y=y, z=z, setup=setup, bcs=bcs, scheme=scheme,
+                seed=seed, steps=10, ports=ports)
+
+
+def c5_fields(case, air):
+    """Initial bv (6, ...) and sv (11, ...) of the channel: the vitiated-air
+    inflow state everywhere (ghosts included) with the seeded 1 % density
+    perturbation at fixed pressure in the interior.  `air` is the inflow
+    state dict (rho, u, v, w, p, t, sv incl. k, omega, mw)."""
+    mx, my, mz, g = case["mx"], case["my"], case["mz"], case["ngg"]
+    bv, sv = initial_fields(dict(case, inflow=air, perturb="density"))
+    return bv, sv
+
+
 def initial_fields(case):
     """Uniform inflow everywhere (ghosts included), then the seeded interior
     density perturbation at fixed pressure.  Returns bv (6, ...) and sv

@@ -33,13 +33,15 @@ static inline int64_t at(lay L, int i, int j, int k) {
 }
 
 struct orc_zone {
-  int mx, my, mz, g, nv, ns, nsc, dim;
+  int mx, my, mz, g, nv, ns, nt, nsc, dim;
   orc_cfg cfg;
   orc_mech mech;
   lay Lc, Lg, L0;
   double *x, *y, *z, *jac, *metric;
   double *bv, *sv, *cv, *qn, *vel, *acoustic, *mach, *mul, *cond, *rhoD, *gam, *cp;
   double *dq, *dt_local, *bv_last, *chem_jac, *inv_sr, *visc_sr;
+  /* SST: turbulent viscosity and conductivity, wall distance (ghosted), source Jacobian diagonal (2 x L0) */
+  double *mut, *tcond, *wdist, *turb_jac;
   /* registered boundaries (range after Boundary::register_boundary + wall trim) */
   int nbc;
   int bface[ORC_MAXBC], bdir[ORC_MAXBC], rs[ORC_MAXBC][3], re[ORC_MAXBC][3];
@@ -210,6 +212,47 @@ void orc_grid_metrics(int mx, int my, int mz, int ngg, const double *x, const do
 
 static void *cal(int64_t n) { return calloc((size_t)(n > 0 ? n : 1), sizeof(double)); }
 
+/* WallDistance.cuh:9-81 + Driver.cu:100-121: the wall points are the nodes
+ * of every wall boundary's registered (trimmed) range; every ghosted point
+ * gets the distance to the nearest of them by brute force (squared distances
+ * minimised from 1e6, square root at the end). */
+static void wall_distance(orc_zone *zz) {
+  int64_t nw = 0;
+  for (int b = 0; b < zz->cfg.n_bc; ++b)
+    if (zz->cfg.bc[b].type == 2)
+      nw += (int64_t)(zz->re[b][0] - zz->rs[b][0] + 1) * (zz->re[b][1] - zz->rs[b][1] + 1) *
+            (zz->re[b][2] - zz->rs[b][2] + 1);
+  double *wp = malloc(sizeof(double) * 3 * (nw > 0 ? nw : 1));
+  int64_t c = 0;
+  for (int b = 0; b < zz->cfg.n_bc; ++b) {
+    if (zz->cfg.bc[b].type != 2) continue;
+    for (int k = zz->rs[b][2]; k <= zz->re[b][2]; ++k)
+      for (int j = zz->rs[b][1]; j <= zz->re[b][1]; ++j)
+        for (int i = zz->rs[b][0]; i <= zz->re[b][0]; ++i) {
+          const int64_t p = at(zz->Lc, i, j, k);
+          wp[3 * c] = zz->x[p];
+          wp[3 * c + 1] = zz->y[p];
+          wp[3 * c + 2] = zz->z[p];
+          ++c;
+        }
+  }
+  const int g = zz->g;
+  for (int k = -g; k < zz->mz + g; ++k)
+    for (int j = -g; j < zz->my + g; ++j)
+      for (int i = -g; i < zz->mx + g; ++i) {
+        const int64_t p = at(zz->Lc, i, j, k);
+        const double x = zz->x[p], y = zz->y[p], z = zz->z[p];
+        double d2 = 1e+6;
+        for (int64_t l = 0; l < nw; ++l) {
+          const double d = (x - wp[3 * l]) * (x - wp[3 * l]) + (y - wp[3 * l + 1]) * (y - wp[3 * l + 1]) +
+                           (z - wp[3 * l + 2]) * (z - wp[3 * l + 2]);
+          if (d2 > d) d2 = d;
+        }
+        zz->wdist[at(zz->Lg, i, j, k)] = sqrt(d2);
+      }
+  free(wp);
+}
+
 orc_zone *orc_zone_create(int mx, int my, int mz, int ngg, const orc_cfg *cfg, const orc_mech *mech, const double *x,
                           const double *y, const double *z) {
   orc_zone *zz = (orc_zone *)calloc(1, sizeof(orc_zone));
@@ -221,7 +264,8 @@ orc_zone *orc_zone_create(int mx, int my, int mz, int ngg, const orc_cfg *cfg, c
   zz->cfg = *cfg;
   if (mech) zz->mech = *mech;
   zz->ns = cfg->mix ? zz->mech.n_spec : 0;
-  zz->nsc = zz->ns;
+  zz->nt = cfg->turbulence ? 2 : 0;
+  zz->nsc = zz->ns + zz->nt;
   zz->nv = 5 + zz->nsc;
   zz->Lc = mklay(mx, my, mz, ngg + 1);
   zz->Lg = mklay(mx, my, mz, ngg);
@@ -250,6 +294,10 @@ orc_zone *orc_zone_create(int mx, int my, int mz, int ngg, const orc_cfg *cfg, c
   zz->chem_jac = cal(s0 * (zz->ns ? zz->ns * zz->ns : 1));
   zz->inv_sr = cal(s0 * 3);
   zz->visc_sr = cal(s0);
+  zz->mut = cal(sg);
+  zz->tcond = cal(sg);
+  zz->wdist = cal(sg);
+  zz->turb_jac = cal(s0 * 2);
   orc_grid_metrics(mx, my, mz, ngg, x, y, z, zz->x, zz->y, zz->z, zz->jac, zz->metric);
   /* inviscid_type: Parameter.cpp:165-219 deduce_known_info */
   zz->inviscid_type = 0;
@@ -302,6 +350,7 @@ orc_zone *orc_zone_create(int mx, int my, int mz, int ngg, const orc_cfg *cfg, c
     for (int b = 0; b < cfg->n_bc; ++b)
       if (cfg->bc[b].type == prio[p]) zz->order[n++] = b;
   zz->nbc = n;
+  if (zz->nt) wall_distance(zz);
   return zz;
 }
 
@@ -309,7 +358,7 @@ void orc_zone_destroy(orc_zone *zz) {
   if (!zz) return;
   double *ptrs[] = {zz->x, zz->y, zz->z, zz->jac, zz->metric, zz->bv, zz->sv, zz->cv, zz->qn, zz->vel, zz->acoustic,
                     zz->mach, zz->mul, zz->cond, zz->rhoD, zz->gam, zz->cp, zz->dq, zz->dt_local, zz->bv_last,
-                    zz->chem_jac, zz->inv_sr, zz->visc_sr};
+                    zz->chem_jac, zz->inv_sr, zz->visc_sr, zz->mut, zz->tcond, zz->wdist, zz->turb_jac};
   for (size_t i = 0; i < sizeof(ptrs) / sizeof(ptrs[0]); ++i) free(ptrs[i]);
   free(zz);
 }
@@ -338,6 +387,10 @@ double *orc_zone_field(orc_zone *zz, int id, int64_t *count) {
     case ORC_QN: *count = sg * zz->nv; return zz->qn;
     case ORC_BV_LAST: *count = s0 * 4; return zz->bv_last;
     case ORC_CHEM_JAC: *count = s0 * zz->ns * zz->ns; return zz->chem_jac;
+    case ORC_MUT: *count = sg; return zz->mut;
+    case ORC_TURB_COND: *count = sg; return zz->tcond;
+    case ORC_WALL_DIST: *count = sg; return zz->wdist;
+    case ORC_TURB_JAC: *count = s0 * 2; return zz->turb_jac;
   }
   *count = 0;
   return NULL;
@@ -424,6 +477,7 @@ static void bc_symmetry(orc_zone *zz, int b, int i, int j, int k) {
     BV(4, gi, gj, gk) = BV(4, ii, ij, ik);
     BV(5, gi, gj, gk) = BV(5, ii, ij, ik);
     for (int l = 0; l < zz->nsc; ++l) SV(l, gi, gj, gk) = SV(l, ii, ij, ik);
+    if (zz->nt) S1(mut, gi, gj, gk) = S1(mut, ii, ij, ik);
   }
 }
 
@@ -435,6 +489,7 @@ static void bc_outflow(orc_zone *zz, int b, int i, int j, int k, int with_cv) {
     const int gi = i + g * dir[0], gj = j + g * dir[1], gk = k + g * dir[2];
     for (int l = 0; l < 6; ++l) BV(l, gi, gj, gk) = BV(l, i, j, k);
     for (int l = 0; l < zz->nsc; ++l) SV(l, gi, gj, gk) = SV(l, i, j, k);
+    if (zz->nt) S1(mut, gi, gj, gk) = S1(mut, i, j, k);
     if (with_cv) cv_from_bv_point(zz, gi, gj, gk);
   }
 }
@@ -450,6 +505,7 @@ static void bc_inflow(orc_zone *zz, const orc_bc *B, int b, int i, int j, int k,
   BV(4, i, j, k) = B->p;
   BV(5, i, j, k) = B->t;
   for (int l = 0; l < zz->nsc; ++l) SV(l, i, j, k) = B->sv[l];
+  if (zz->nt) S1(mut, i, j, k) = B->mut;
   S1(vel, i, j, k) = B->vel;
   if (with_cv) cv_from_bv_point(zz, i, j, k);
   for (int g = 1; g <= zz->g; ++g) {
@@ -461,6 +517,7 @@ static void bc_inflow(orc_zone *zz, const orc_bc *B, int b, int i, int j, int k,
     BV(4, gi, gj, gk) = B->p;
     BV(5, gi, gj, gk) = B->t;
     for (int l = 0; l < zz->nsc; ++l) SV(l, gi, gj, gk) = B->sv[l];
+    if (zz->nt) S1(mut, gi, gj, gk) = B->mut;
     if (with_cv) cv_from_bv_point(zz, gi, gj, gk);
   }
 }
@@ -478,9 +535,10 @@ static void face_normal(const orc_zone *zz, int b, int i, int j, int k, double *
   n[2] = m[2] * s;
 }
 
-static void set_point(orc_zone *zz, int i, int j, int k, const double *b6, const double *svv) {
+static void set_point(orc_zone *zz, int i, int j, int k, const double *b6, const double *svv, double mut) {
   for (int l = 0; l < 6; ++l) BV(l, i, j, k) = b6[l];
   for (int l = 0; l < zz->nsc; ++l) SV(l, i, j, k) = svv[l];
+  if (zz->nt) S1(mut, i, j, k) = mut;
 }
 
 /* BoundCond.cuh:213-424 apply_farfield: supersonic in/outflow by the normal
@@ -509,9 +567,10 @@ static void bc_farfield(orc_zone *zz, const orc_bc *B, int b, int i, int j, int 
   double st[6];
   if (mb <= -1) {
     const double far[6] = {B->rho, B->u, B->v, B->w, B->p, B->t};
-    set_point(zz, i, j, k, far, B->sv);
+    set_point(zz, i, j, k, far, B->sv, B->mut);
     S1(vel, i, j, k) = sqrt(B->u * B->u + B->v * B->v + B->w * B->w);
-    for (int g = 1; g <= zz->g; ++g) set_point(zz, i + g * dir[0], j + g * dir[1], k + g * dir[2], far, B->sv);
+    for (int g = 1; g <= zz->g; ++g)
+      set_point(zz, i + g * dir[0], j + g * dir[1], k + g * dir[2], far, B->sv, B->mut);
     return;
   }
   if (mb >= 1) {
@@ -519,6 +578,7 @@ static void bc_farfield(orc_zone *zz, const orc_bc *B, int b, int i, int j, int 
       const int gi = i + g * dir[0], gj = j + g * dir[1], gk = k + g * dir[2];
       for (int l = 0; l < 6; ++l) BV(l, gi, gj, gk) = BV(l, i, j, k);
       for (int l = 0; l < nsc; ++l) SV(l, gi, gj, gk) = SV(l, i, j, k);
+      if (zz->nt) S1(mut, gi, gj, gk) = S1(mut, i, j, k);
     }
     return;
   }
@@ -527,6 +587,7 @@ static void bc_farfield(orc_zone *zz, const orc_bc *B, int b, int i, int j, int 
   const double rn = uinf - 2 * B->a_inf / (B->gamma_inf - 1);
   const double Un = 0.5 * (rp + rn);
   double sb, rho, p, t, u, v, w;
+  double mut = zz->nt ? S1(mut, i, j, k) : 0.0;
   if (!zz->cfg.mix) {
     const double cb = 0.25 * (GAMMA_AIR - 1) * (rp - rn);
     if (mb <= 0) {
@@ -535,6 +596,7 @@ static void bc_farfield(orc_zone *zz, const orc_bc *B, int b, int i, int j, int 
       v = B->v + (Un - uinf) * n[1];
       w = B->w + (Un - uinf) * n[2];
       for (int l = 0; l < nsc; ++l) svb[l] = B->sv[l];
+      mut = B->mut;
     } else {
       sb = pb / pow(rb, GAMMA_AIR);
       u = ub + (Un - uf) * n[0];
@@ -551,6 +613,7 @@ static void bc_farfield(orc_zone *zz, const orc_bc *B, int b, int i, int j, int 
       w = B->w + (Un - uinf) * n[2];
       for (int l = 0; l < nsc; ++l) svb[l] = B->sv[l];
       mw = B->mw_inf;
+      mut = B->mut;
     } else {
       u = ub + (Un - uf) * n[0];
       v = vb + (Un - uf) * n[1];
@@ -577,9 +640,9 @@ static void bc_farfield(orc_zone *zz, const orc_bc *B, int b, int i, int j, int 
   st[3] = w;
   st[4] = p;
   st[5] = t;
-  set_point(zz, i, j, k, st, svb);
+  set_point(zz, i, j, k, st, svb, mut);
   S1(vel, i, j, k) = sqrt(u * u + v * v + w * w);
-  for (int g = 1; g <= zz->g; ++g) set_point(zz, i + g * dir[0], j + g * dir[1], k + g * dir[2], st, svb);
+  for (int g = 1; g <= zz->g; ++g) set_point(zz, i + g * dir[0], j + g * dir[1], k + g * dir[2], st, svb, mut);
 }
 
 /* BoundCond.cuh:554-636 apply_subsonic_inflow (air): the outgoing Riemann
@@ -618,6 +681,7 @@ static void bc_subsonic(orc_zone *zz, const orc_bc *B, int b, int i, int j, int 
     BV(5, gi, gj, gk) = t;
     const double ubar = BV(1, gi, gj, gk) * n[0] + BV(2, gi, gj, gk) * n[1] + BV(3, gi, gj, gk) * n[2];
     for (int l = 0; l < zz->nsc; ++l) SV(l, gi, gj, gk) = ubar > 0 ? SV(l, i, j, k) : B->sv[l];
+    if (zz->nt) S1(mut, gi, gj, gk) = S1(mut, i, j, k);
     if (with_cv) cv_from_bv_point(zz, gi, gj, gk);
   }
 }
@@ -632,8 +696,50 @@ static void bc_back_pressure(orc_zone *zz, const orc_bc *B, int b, int i, int j,
     BV(4, gi, gj, gk) = B->p_back;
     BV(5, gi, gj, gk) = BV(5, i, j, k);
     for (int l = 0; l < zz->nsc; ++l) SV(l, gi, gj, gk) = SV(l, i, j, k);
+    if (zz->nt) S1(mut, gi, gj, gk) = S1(mut, i, j, k);
     if (with_cv) cv_from_bv_point(zz, gi, gj, gk);
   }
+}
+
+/* SST.cuh:13-34 model constants (Menter's SST: inner set 1, k-epsilon set 2) */
+#define SST_BETA_STAR 0.09
+#define SST_SQRT_BETA_STAR 0.3
+#define SST_KAPPA 0.41
+#define SST_SIGMA_K1 0.85
+#define SST_SIGMA_OMEGA1 0.5
+#define SST_BETA1 0.0750
+#define SST_A1 0.31
+#define SST_SIGMA_K2 1.0
+#define SST_SIGMA_OMEGA2 0.856
+#define SST_BETA2 0.0828
+static const double SST_GAMMA1 = SST_BETA1 / SST_BETA_STAR - SST_SIGMA_OMEGA1 * SST_KAPPA * SST_KAPPA / SST_SQRT_BETA_STAR;
+static const double SST_GAMMA2 = SST_BETA2 / SST_BETA_STAR - SST_SIGMA_OMEGA2 * SST_KAPPA * SST_KAPPA / SST_SQRT_BETA_STAR;
+
+/* Transport.cu:11-38 / 113-140 compute_viscosity: Wilke's mixing rule over
+ * the species viscosities of the collision-integral fit */
+static double wilke_viscosity(const orc_mech *m, double t, double mw_total, const double *Y) {
+  const int ns = m->n_spec;
+  double x[ORC_MAXS], vis[ORC_MAXS];
+  for (int i = 0; i < ns; ++i) {
+    x[i] = Y[i] * mw_total / m->mw[i];
+    const double tdl = t * m->LJ_potent_inv[i];
+    const double ci = 1.147 * pow(tdl, -0.145) + pow(tdl + 0.5, -2);
+    vis[i] = m->vis_coeff[i] * sqrt(t) / ci;
+  }
+  double mu = 0;
+  for (int i = 0; i < ns; ++i) {
+    double vt = 0;
+    for (int j = 0; j < ns; ++j) {
+      double phi = 1.0;
+      if (i != j) {
+        const double num = 1 + sqrt(vis[i] / vis[j]) * m->WjDivWi_to_One4th[i * ns + j];
+        phi = num * num * m->sqrt_WiDivWjPl1Mul8[i * ns + j];
+      }
+      vt += phi * x[j];
+    }
+    mu += vis[i] * x[i] / vt;
+  }
+  return mu;
 }
 
 static void bc_wall(orc_zone *zz, const orc_bc *B, int b, int i, int j, int k, int with_cv) {
@@ -660,6 +766,19 @@ static void bc_wall(orc_zone *zz, const orc_bc *B, int b, int i, int j, int k, i
   BV(4, i, j, k) = p;
   BV(5, i, j, k) = t_wall;
   S1(vel, i, j, k) = 0;
+  if (zz->nt) {
+    /* SST wall values: k = 0, omega = 60 mu_w / (rho_w beta1 d^2) with the
+     * wall distance of the first interior point */
+    double Yw[ORC_MAXS];
+    for (int l = 0; l < zz->ns; ++l) Yw[l] = SV(l, i, j, k);
+    const double mu_wall = zz->cfg.mix ? wilke_viscosity(&zz->mech, t_wall, mw, Yw) : orc_sutherland(t_wall);
+    const double dy = S1(wdist, i0, j0, k0);
+    SV(zz->ns, i, j, k) = 0;
+    if (dy > 1e-25)
+      SV(zz->ns + 1, i, j, k) = 60 * mu_wall / (rho_wall * SST_BETA1 * dy * dy);
+    else
+      SV(zz->ns + 1, i, j, k) = SV(zz->ns + 1, i0, j0, k0);
+  }
   if (with_cv) cv_from_bv_point(zz, i, j, k);
   for (int g = 1; g <= zz->g; ++g) {
     const int ai = i - g * dir[0], aj = j - g * dir[1], ak = k - g * dir[2];
@@ -686,6 +805,11 @@ static void bc_wall(orc_zone *zz, const orc_bc *B, int b, int i, int j, int k, i
     BV(3, gi, gj, gk) = -wi;
     BV(4, gi, gj, gk) = pi;
     BV(5, gi, gj, gk) = tg;
+    if (zz->nt) {
+      SV(zz->ns, gi, gj, gk) = 0;
+      SV(zz->ns + 1, gi, gj, gk) = SV(zz->ns + 1, i, j, k);
+      S1(mut, gi, gj, gk) = 0;
+    }
     if (with_cv) cv_from_bv_point(zz, gi, gj, gk);
   }
 }
@@ -770,6 +894,8 @@ static void update_properties(orc_zone *zz) {
       }
 }
 
+static void initialize_mut(orc_zone *zz);
+
 /* Driver.cu:40-98 initialize_computation followed by RK.cuh:38-48
  * (store_last_step, compute_cv_from_bv on every point). */
 void orc_zone_initialize(orc_zone *zz) {
@@ -781,6 +907,7 @@ void orc_zone_initialize(orc_zone *zz) {
       for (int i = -g; i < zz->mx + g; ++i)
         S1(vel, i, j, k) = sqrt(BV(1, i, j, k) * BV(1, i, j, k) + BV(2, i, j, k) * BV(2, i, j, k) +
                                 BV(3, i, j, k) * BV(3, i, j, k));
+  if (zz->nt) initialize_mut(zz);
   update_properties(zz);
   for (int k = 0; k < zz->mz; ++k)
     for (int j = 0; j < zz->my; ++j)
@@ -1445,11 +1572,20 @@ static void viscous_face(orc_zone *zz, int dir, int i, int j, int k, double *fv)
   const double v_x = GX(2, 0), v_y = GX(2, 1), v_z = GX(2, 2);
   const double w_x = GX(3, 0), w_y = GX(3, 1), w_z = GX(3, 2);
   const double mul = 0.5 * (S1(mul, i, j, k) + S1(mul, i1, j1, k1));
-  const double vis = mul;
-  const double txx = vis * (4 * u_x - 2 * v_y - 2 * w_z) / 3.0;
-  const double tyy = vis * (4 * v_y - 2 * u_x - 2 * w_z) / 3.0;
-  const double tzz = vis * (4 * w_z - 2 * u_x - 2 * v_y) / 3.0;
+  const double mut = zz->nt ? 0.5 * (S1(mut, i, j, k) + S1(mut, i1, j1, k1)) : 0.0;
+  const double vis = mul + mut;
+  double txx = vis * (4 * u_x - 2 * v_y - 2 * w_z) / 3.0;
+  double tyy = vis * (4 * v_y - 2 * u_x - 2 * w_z) / 3.0;
+  double tzz = vis * (4 * w_z - 2 * u_x - 2 * v_y) / 3.0;
   const double txy = vis * (u_y + v_x), txz = vis * (u_z + w_x), tyz = vis * (v_z + w_y);
+  if (zz->nt) {
+    /* SST: -2/3 rho k on the normal stresses, faded out within 1e-3 of a wall */
+    const double dyd = fmin(0.5 * (S1(wdist, i, j, k) + S1(wdist, i1, j1, k1)) / 1e-3, 1.0);
+    const double rk = -2.0 / 3 * 0.5 * (BV(0, i, j, k) * SV(ns, i, j, k) + BV(0, i1, j1, k1) * SV(ns, i1, j1, k1)) * dyd;
+    txx += rk;
+    tyy += rk;
+    tzz += rk;
+  }
   const double J0 = zz->jac[at(zz->Lg, i, j, k)], J1 = zz->jac[at(zz->Lg, i1, j1, k1)];
   const double kx = 0.5 * (m[dir * 3 + 0] * J0 + m1[dir * 3 + 0] * J1);
   const double ky = 0.5 * (m[dir * 3 + 1] * J0 + m1[dir * 3 + 1] * J1);
@@ -1462,7 +1598,8 @@ static void viscous_face(orc_zone *zz, int dir, int i, int j, int k, double *fv)
   const double vm = 0.5 * (BV(2, i, j, k) + BV(2, i1, j1, k1));
   const double wm = 0.5 * (BV(3, i, j, k) + BV(3, i1, j1, k1));
   const double t_x = GX(5, 0), t_y = GX(5, 1), t_z = GX(5, 2);
-  const double cond = 0.5 * (S1(cond, i, j, k) + S1(cond, i1, j1, k1));
+  double cond = 0.5 * (S1(cond, i, j, k) + S1(cond, i1, j1, k1));
+  if (zz->nt) cond += 0.5 * (S1(tcond, i, j, k) + S1(tcond, i1, j1, k1));
   fv[4] = um * fv[1] + vm * fv[2] + wm * fv[3] + cond * (kx * t_x + ky * t_y + kz * t_z);
   if (zz->cfg.mix) {
     const orc_mech *me = &zz->mech;
@@ -1471,7 +1608,8 @@ static void viscous_face(orc_zone *zz, int dir, int i, int j, int k, double *fv)
     for (int l = 0; l < ns; ++l) {
       yk[l] = 0.5 * (SV(l, i, j, k) + SV(l, i1, j1, k1));
       diffusivity[l] = 0.5 * (zz->rhoD[l * zz->Lg.sz + at(zz->Lg, i, j, k)] +
-                              zz->rhoD[l * zz->Lg.sz + at(zz->Lg, i1, j1, k1)]);
+                              zz->rhoD[l * zz->Lg.sz + at(zz->Lg, i1, j1, k1)]) +
+                       mut / zz->cfg.Sct;
       const double y_x = GX(6 + l, 0), y_y = GX(6 + l, 1), y_z = GX(6 + l, 2);
       const double gy = kx * y_x + ky * y_y + kz * y_z;
       ddf[l] = gy;
@@ -1498,6 +1636,29 @@ static void viscous_face(orc_zone *zz, int dir, int i, int j, int k, double *fv)
       fv[5 + l] = dflux;
       fv[4] += h[l] * dflux;
     }
+  }
+  if (zz->nt) {
+    /* SST diffusion of k and omega, sigma blended with F1 at the face */
+    const double k_x = GX(6 + ns, 0), k_y = GX(6 + ns, 1), k_z = GX(6 + ns, 2);
+    const double o_x = GX(7 + ns, 0), o_y = GX(7 + ns, 1), o_z = GX(7 + ns, 2);
+    const double wd = 0.5 * (S1(wdist, i, j, k) + S1(wdist, i1, j1, k1));
+    double f1 = 1;
+    if (wd > 1e-25) {
+      const double km = 0.5 * (SV(ns, i, j, k) + SV(ns, i1, j1, k1));
+      const double om = 0.5 * (SV(ns + 1, i, j, k) + SV(ns + 1, i1, j1, k1));
+      const double p1 = sqrt(km) / (0.09 * om * wd);
+      const double rhom = 0.5 * (BV(0, i, j, k) + BV(0, i1, j1, k1));
+      const double d2 = wd * wd;
+      const double p2 = 500 * mul / (rhom * d2 * om);
+      const double cdkw = fmax(1e-20, 2 * rhom * SST_SIGMA_OMEGA2 / om * (k_x * o_x + k_y * o_y + k_z * o_z));
+      const double p3 = 4 * rhom * SST_SIGMA_OMEGA2 * km / (cdkw * d2);
+      const double a1 = fmin(fmax(p1, p2), p3);
+      f1 = tanh(a1 * a1 * a1 * a1);
+    }
+    const double sk = SST_SIGMA_K2 + (SST_SIGMA_K1 - SST_SIGMA_K2) * f1;
+    const double sw = SST_SIGMA_OMEGA2 + (SST_SIGMA_OMEGA1 - SST_SIGMA_OMEGA2) * f1;
+    fv[5 + ns] = (mul + mut * sk) * (kx * k_x + ky * k_y + kz * k_z);
+    fv[6 + ns] = (mul + mut * sw) * (kx * o_x + ky * o_y + kz * o_z);
   }
   (void)nv;
 #undef GX
@@ -1527,8 +1688,163 @@ static void viscous_dir(orc_zone *zz, int dir) {
   free(fv);
 }
 
-/* SourceTerm.cuh:11-37 compute_source (finite-rate chemistry part) */
+/* ------------------------------------------------------------- SST */
+
+/* central point gradient 0.5 (xi_c dF/di + eta_c dF/dj + zeta_c dF/dk) of
+ * bv component l (l < 6) or sv component l - 6, as written in SST.cu:52-157 */
+static void point_grad(const orc_zone *zz, int l, int i, int j, int k, double *g) {
+  const double *m = &zz->metric[at(zz->Lg, i, j, k) * 9];
+#define F_(a, b, c) (l < 6 ? BV(l, a, b, c) : SV(l - 6, a, b, c))
+  const double di = F_(i + 1, j, k) - F_(i - 1, j, k);
+  const double dj = F_(i, j + 1, k) - F_(i, j - 1, k);
+  const double dk = F_(i, j, k + 1) - F_(i, j, k - 1);
+#undef F_
+  for (int c = 0; c < 3; ++c) g[c] = 0.5 * (m[c] * di + m[3 + c] * dj + m[6 + c] * dk);
+}
+
+/* SST.cu:44-97 compute_mut: mu_t = a1 rho k / max(a1 omega, |vorticity| F2) */
+static double sst_mut(const orc_zone *zz, int i, int j, int k, double mul) {
+  double gu[3], gv[3], gw[3];
+  point_grad(zz, 1, i, j, k, gu);
+  point_grad(zz, 2, i, j, k, gv);
+  point_grad(zz, 3, i, j, k, gw);
+  const double rho = BV(0, i, j, k), tke = SV(zz->ns, i, j, k), omega = SV(zz->ns + 1, i, j, k);
+  const double vort = sqrt((gv[0] - gu[1]) * (gv[0] - gu[1]) + (gw[0] - gu[2]) * (gw[0] - gu[2]) +
+                           (gw[1] - gv[2]) * (gw[1] - gv[2]));
+  double f2 = 1;
+  const double dy = S1(wdist, i, j, k);
+  if (dy > 1e-25) {
+    const double p1 = 2 * sqrt(tke) / (0.09 * omega * dy);
+    const double p2 = 500 * mul / (rho * dy * dy * omega);
+    const double a2 = fmax(p1, p2);
+    f2 = tanh(a2 * a2);
+  }
+  const double den = fmax(SST_A1 * omega, vort * f2);
+  return den > 1e-25 ? SST_A1 * rho * tke / den : 0.0;
+}
+
+/* FieldOperation.cuh:175-195 initialize_mut on [-1, m], with the laminar
+ * viscosity from T (Sutherland, or the mixture rule at the point's Y) */
+static void initialize_mut(orc_zone *zz) {
+  for (int k = -1; k <= zz->mz; ++k)
+    for (int j = -1; j <= zz->my; ++j)
+      for (int i = -1; i <= zz->mx; ++i) {
+        const double t = BV(5, i, j, k);
+        double mul = orc_sutherland(t);
+        if (zz->cfg.mix) {
+          double Y[ORC_MAXS], mw = 0;
+          for (int l = 0; l < zz->ns; ++l) {
+            Y[l] = SV(l, i, j, k);
+            mw += Y[l] / zz->mech.mw[l];
+          }
+          mul = wilke_viscosity(&zz->mech, t, 1 / mw, Y);
+        }
+        S1(mut, i, j, k) = sst_mut(zz, i, j, k, mul);
+      }
+}
+
+/* SST.cu:24-42 compressibility corrections beta* xi* F(Mt) */
+double orc_sst_wilcox(double mt) {
+  const double mt0 = 0.25;
+  return mt - mt0 > 0 ? SST_BETA_STAR * 2 * (mt * mt - mt0 * mt0) : 0.0;
+}
+
+double orc_sst_zeman(double mt, double gamma_p1) {
+  const double mt0 = 0.25 * sqrt(2.0 / gamma_p1);
+  const double dmt = mt - mt0;
+  if (dmt <= 0) return 0.0;
+  const double lambda2 = 0.66 * 0.66;
+  return SST_BETA_STAR * 0.75 * (1 - exp(-0.5 * gamma_p1 * dmt * dmt / lambda2));
+}
+
+void orc_sst_constants(double *gamma1, double *gamma2) {
+  *gamma1 = SST_GAMMA1;
+  *gamma2 = SST_GAMMA2;
+}
+
+/* SST.cu:99-247 compute_source_and_mut + SourceTerm.cuh:19-28: F1/F2 blend,
+ * mu_t, the k and omega sources (production limited to 10x, replaced by
+ * mu_t |vorticity|^2), compressibility correction, the source Jacobian
+ * diagonal and lambda_t = mu_t cp / Prt */
+static void sst_source_point(orc_zone *zz, int i, int j, int k) {
+  const int ns = zz->ns;
+  double gu[3], gv[3], gw[3], gk[3], go[3];
+  point_grad(zz, 1, i, j, k, gu);
+  point_grad(zz, 2, i, j, k, gv);
+  point_grad(zz, 3, i, j, k, gw);
+  point_grad(zz, 6 + ns, i, j, k, gk);
+  point_grad(zz, 7 + ns, i, j, k, go);
+  const double u_x = gu[0], u_y = gu[1], u_z = gu[2], v_x = gv[0], v_y = gv[1], v_z = gv[2];
+  const double w_x = gw[0], w_y = gw[1], w_z = gw[2];
+  const double rho = BV(0, i, j, k);
+  const double omega = SV(ns + 1, i, j, k), tke = SV(ns, i, j, k);
+  const double inter = 2 * rho * SST_SIGMA_OMEGA2 / omega * (gk[0] * go[0] + gk[1] * go[1] + gk[2] * go[2]);
+  const double rhoK = rho * tke;
+  const double vort = sqrt((v_x - u_y) * (v_x - u_y) + (w_x - u_z) * (w_x - u_z) + (w_y - v_z) * (w_y - v_z));
+  double f1 = 1, f2 = 1;
+  const double dy = S1(wdist, i, j, k);
+  if (dy > 1e-25) {
+    const double p1 = sqrt(tke) / (0.09 * omega * dy);
+    const double d2 = dy * dy;
+    const double p2 = 500 * S1(mul, i, j, k) / (rho * d2 * omega);
+    const double a2 = fmax(2 * p1, p2);
+    f2 = tanh(a2 * a2);
+    const double cdkw = fmax(1e-20, inter);
+    const double p3 = 4 * rho * SST_SIGMA_OMEGA2 * tke / (cdkw * d2);
+    const double a1 = fmin(fmax(p1, p2), p3);
+    f1 = tanh(a1 * a1 * a1 * a1);
+  }
+  double mut = 0;
+  const double den = fmax(SST_A1 * omega, vort * f2);
+  if (den > 1e-25) mut = SST_A1 * rhoK / den;
+  S1(mut, i, j, k) = mut;
+  double beta = SST_BETA2 + (SST_BETA1 - SST_BETA2) * f1;
+  double beta_star = SST_BETA_STAR;
+  const int corr = zz->cfg.compressibility_correction;
+  if (corr) {
+    const double gam = zz->cfg.mix ? S1(gam, i, j, k) : GAMMA_AIR;
+    const double mt = sqrt(2 * tke / (gam * BV(4, i, j, k) / rho));
+    double c;
+    if (corr == 1)
+      c = orc_sst_wilcox(mt);
+    else if (corr == 2)
+      c = SST_BETA_STAR * mt * mt;
+    else
+      c = orc_sst_zeman(mt, gam + 1.0);
+    beta_star += c;
+    beta -= c;
+  }
+  if (mut > 1e-25) {
+    const double divU = u_x + v_y + w_z;
+    double prod_k = mut * (2 * (u_x * u_x + v_y * v_y + w_z * w_z) - 2.0 / 3 * divU * divU + (u_y + v_x) * (u_y + v_x) +
+                           (u_z + w_x) * (u_z + w_x) + (v_z + w_y) * (v_z + w_y)) -
+                    2.0 / 3 * rhoK * divU;
+    const double prod_k2 = mut * vort * vort;
+    if (prod_k > 10 * prod_k2) prod_k = prod_k2;
+    const double diss_k = beta_star * rhoK * omega;
+    const double J = zz->jac[at(zz->Lg, i, j, k)];
+    DQ(5 + ns, i, j, k) += J * (prod_k - diss_k);
+    const double gamma = SST_GAMMA2 + (SST_GAMMA1 - SST_GAMMA2) * f1;
+    const double prod_w = gamma * rho / mut * prod_k + (1 - f1) * inter;
+    const double diss_w = beta * rho * omega * omega;
+    DQ(6 + ns, i, j, k) += J * (prod_w - diss_w);
+  }
+  if (zz->cfg.turb_implicit == 1) {
+    const int64_t p = at(zz->L0, i, j, k);
+    zz->turb_jac[p] = -2 * beta_star * omega;
+    zz->turb_jac[zz->L0.sz + p] = -2 * beta * omega;
+  }
+  const double cp = zz->cfg.mix ? S1(cp, i, j, k) : GAMMA_AIR * R_U / MW_AIR / (GAMMA_AIR - 1);
+  S1(tcond, i, j, k) = mut * cp / zz->cfg.Prt;
+}
+
+/* SourceTerm.cuh:11-37 compute_source: SST (mu_t and the k, omega sources),
+ * then the finite-rate chemistry */
 static void source(orc_zone *zz) {
+  if (zz->nt)
+    for (int k = 0; k < zz->mz; ++k)
+      for (int j = 0; j < zz->my; ++j)
+        for (int i = 0; i < zz->mx; ++i) sst_source_point(zz, i, j, k);
   if (!(zz->cfg.mix && zz->cfg.reaction == 1)) return;
   const int ns = zz->ns;
   for (int k = 0; k < zz->mz; ++k)
@@ -1584,7 +1900,8 @@ double orc_zone_global_dt(orc_zone *zz) {
         double gamma = GAMMA_AIR;
         if (zz->cfg.mix) gamma = S1(gam, i, j, k);
         const double c1 = fmax(gamma, 4.0 / 3.0) / BV(0, i, j, k);
-        const double c2 = S1(mul, i, j, k) / zz->cfg.Pr;
+        double c2 = S1(mul, i, j, k) / zz->cfg.Pr;
+        if (zz->nt) c2 += S1(mut, i, j, k) / zz->cfg.Prt;
         double srv = gxi * gxi + geta * geta;
         if (zz->dim == 3) srv += gzeta * gzeta;
         srv *= c1 * c2;
@@ -1746,6 +2063,9 @@ static void dplur_dq0(orc_zone *zz, double *dq0) {
         } else {
           for (int l = 0; l < ns; ++l) DQ(5 + l, i, j, k) /= diag;
         }
+        /* SST.cu:249-258 implicit_treat_for_dq0 */
+        for (int l = 0; l < zz->nt; ++l)
+          DQ(5 + ns + l, i, j, k) /= zz->cfg.turb_implicit == 1 ? diag - dt * zz->turb_jac[l * zz->L0.sz + p] : diag;
       }
   memcpy(dq0, zz->dq, sizeof(double) * zz->L0.sz * nv);
 }
@@ -1761,7 +2081,7 @@ static void jac_times_dq(const orc_zone *zz, int i, int j, int k, int dir, doubl
   const double l1 = U + lam;
   const double e = 0.5 * zz->vel[pg] * zz->vel[pg];
   double gamma = GAMMA_AIR, b3 = 0, b4 = 0, h = 0;
-  double dq[5 + ORC_MAXS];
+  double dq[5 + ORC_MAXS + 4];
   for (int l = 0; l < zz->nv; ++l) dq[l] = DQ(l, i, j, k);
   if (!zz->cfg.mix) {
     h = gamma / (gamma - 1) * BV(4, i, j, k) / BV(0, i, j, k) + e;
@@ -1809,7 +2129,7 @@ static void dplur_inner(orc_zone *zz, const double *dq0, double *dqk) {
   for (int k = 0; k < zz->mz; ++k)
     for (int j = 0; j < zz->my; ++j)
       for (int i = 0; i < zz->mx; ++i) {
-        double tot[5 + ORC_MAXS] = {0}, jd[5 + ORC_MAXS];
+        double tot[5 + ORC_MAXS + 4] = {0}, jd[5 + ORC_MAXS + 4];
         const int c[3] = {i, j, k};
         for (int d = 0; d < 3; ++d) {
           if (c[d] > 0) {
@@ -1847,11 +2167,17 @@ static void dplur_inner(orc_zone *zz, const double *dq0, double *dqk) {
         } else {
           for (int l = 0; l < ns; ++l) dqk[(5 + l) * zz->L0.sz + p] = dq0[(5 + l) * zz->L0.sz + p] + dt * tot[5 + l] / diag;
         }
+        /* SST.cu:260-273 implicit_treat_for_dqk */
+        for (int l = 0; l < zz->nt; ++l) {
+          const int c = 5 + ns + l;
+          const double dg = zz->cfg.turb_implicit == 1 ? diag - dt * zz->turb_jac[l * zz->L0.sz + p] : diag;
+          dqk[c * zz->L0.sz + p] = dq0[c * zz->L0.sz + p] + dt * tot[c] / dg;
+        }
       }
   memcpy(zz->dq, dqk, sizeof(double) * zz->L0.sz * nv);
 }
 
-/* ImplicitTreatmentHPP.cuh:7-52 (SST implicit terms not included) */
+/* ImplicitTreatmentHPP.cuh:7-52 */
 void orc_zone_implicit(orc_zone *zz) {
   const int ns = zz->ns;
   if (zz->cfg.implicit_method == 1) {
@@ -1871,15 +2197,17 @@ void orc_zone_implicit(orc_zone *zz) {
     free(dqk);
     return;
   }
-  if (!is_fr(zz)) return;
   for (int k = 0; k < zz->mz; ++k)
     for (int j = 0; j < zz->my; ++j)
       for (int i = 0; i < zz->mx; ++i) {
         const int64_t p = at(zz->L0, i, j, k);
         const double dt = zz->dt_local[p];
-        if (zz->cfg.chemSrcMethod == 1) epi_in_place(zz, i, j, k, 1.0, dt);
-        if (zz->cfg.chemSrcMethod == 2) /* FiniteRateChem.cu:420-432 DA */
+        if (is_fr(zz) && zz->cfg.chemSrcMethod == 1) epi_in_place(zz, i, j, k, 1.0, dt);
+        if (is_fr(zz) && zz->cfg.chemSrcMethod == 2) /* FiniteRateChem.cu:420-432 DA */
           for (int l = 0; l < ns; ++l) DQ(5 + l, i, j, k) /= 1 - dt * zz->chem_jac[l * zz->L0.sz + p];
+        /* SST.cu:8-22 implicit_treat_for_SST */
+        if (zz->nt && zz->cfg.turb_implicit == 1)
+          for (int l = 0; l < 2; ++l) DQ(5 + ns + l, i, j, k) /= 1 - dt * zz->turb_jac[l * zz->L0.sz + p];
       }
 }
 

@@ -93,6 +93,8 @@ typedef struct {
   double a_inf, gamma_inf, entropy, mw_inf;
   /* subsonic inflow: total pressure and temperature (u, v, w above: direction); back pressure */
   double p_total, t_total, p_back;
+  /* turbulent viscosity of the inflow / farfield state (k, omega are in sv after the species) */
+  double mut;
 } orc_bc;
 
 typedef struct {
@@ -108,6 +110,9 @@ typedef struct {
   int gradPInDiffusionFlux;
   int implicit_method;        /* steady path: 0 explicit (+ EPI/DA chemistry), 1 DPLUR */
   int dplur_inner_step;
+  int turbulence;             /* 0 laminar, 1 k-omega SST (two scalars k, omega after the species) */
+  int turb_implicit;          /* SST source Jacobian diagonal in the implicit treatment */
+  int compressibility_correction; /* 0 none, 1 Wilcox, 2 Sarkar, 3 Zeman */
   double cfl, Pr, Sc, Prt, Sct, dt, entropy_fix_factor;
   int n_bc;
   orc_bc bc[ORC_MAXBC];
@@ -121,7 +126,8 @@ void orc_zone_destroy(orc_zone *zz);
 /* field ids for orc_zone_field */
 enum {
   ORC_X = 0, ORC_Y, ORC_Z, ORC_JAC, ORC_METRIC, ORC_BV, ORC_SV, ORC_CV, ORC_DQ, ORC_VEL, ORC_ACOUSTIC,
-  ORC_MACH, ORC_MUL, ORC_COND, ORC_RHOD, ORC_GAMMA, ORC_CP, ORC_DT_LOCAL, ORC_QN, ORC_BV_LAST, ORC_CHEM_JAC
+  ORC_MACH, ORC_MUL, ORC_COND, ORC_RHOD, ORC_GAMMA, ORC_CP, ORC_DT_LOCAL, ORC_QN, ORC_BV_LAST, ORC_CHEM_JAC,
+  ORC_MUT, ORC_TURB_COND, ORC_WALL_DIST, ORC_TURB_JAC
 };
 double *orc_zone_field(orc_zone *zz, int id, int64_t *count);
 /* initialise after bv/sv were filled: 2-D k-copy, BCs, |V|, properties, cv from bv (Driver.cu:40-98, RK.cuh:38-48) */
@@ -153,6 +159,11 @@ void orc_face_flux_roe_air(const double *pl5, const double *pr5, const double *m
 void orc_weno5_scalar(const double *v6, double *vm, double *vp);
 void orc_aweno_matrices(const double *cl7, const double *cr7, const double *met6, double *L25, double *R25);
 void orc_ssprk3_coeffs(double *a, double *b, double *c);
+/* SST point hooks: compressibility corrections (SST.cu:24-42) and the
+ * constants gamma1, gamma2 derived from the model constants (SST.cuh:13-34) */
+double orc_sst_wilcox(double mt);
+double orc_sst_zeman(double mt, double gamma_p1);
+void orc_sst_constants(double *gamma1, double *gamma2);
 
 #ifdef __cplusplus
 }

@@ -52,14 +52,15 @@ class BC(C.Structure):
                 ("t", C.c_double), ("vel", C.c_double), ("sv", C.c_double * (MAXS + 4)),
                 ("t_wall", C.c_double), ("isothermal", C.c_int),
                 ("a_inf", C.c_double), ("gamma_inf", C.c_double), ("entropy", C.c_double), ("mw_inf", C.c_double),
-                ("p_total", C.c_double), ("t_total", C.c_double), ("p_back", C.c_double)]
+                ("p_total", C.c_double), ("t_total", C.c_double), ("p_back", C.c_double), ("mut", C.c_double)]
 
 
 class Cfg(C.Structure):
     _fields_ = [("mix", C.c_int), ("reaction", C.c_int), ("chemSrcMethod", C.c_int), ("inviscid_scheme", C.c_int),
                 ("reconstruction", C.c_int), ("limiter", C.c_int), ("viscous_order", C.c_int),
                 ("fixed_time_step", C.c_int), ("output_screen", C.c_int), ("gradPInDiffusionFlux", C.c_int),
-                ("implicit_method", C.c_int), ("dplur_inner_step", C.c_int), ("cfl", C.c_double), ("Pr", C.c_double), ("Sc", C.c_double), ("Prt", C.c_double),
+                ("implicit_method", C.c_int), ("dplur_inner_step", C.c_int), ("turbulence", C.c_int),
+                ("turb_implicit", C.c_int), ("compressibility_correction", C.c_int), ("cfl", C.c_double), ("Pr", C.c_double), ("Sc", C.c_double), ("Prt", C.c_double),
                 ("Sct", C.c_double), ("dt", C.c_double), ("entropy_fix_factor", C.c_double),
                 ("n_bc", C.c_int), ("bc", BC * MAXBC)]
 
@@ -105,6 +106,11 @@ def lib():
         L.orc_zone_implicit.argtypes = [C.c_void_p]
         L.orc_zone_state.argtypes = [C.c_void_p, dp, C.POINTER(C.c_int)]
         L.orc_grid_metrics.argtypes = [C.c_int] * 4 + [dp] * 7
+        L.orc_sst_wilcox.argtypes = [C.c_double]
+        L.orc_sst_wilcox.restype = C.c_double
+        L.orc_sst_zeman.argtypes = [C.c_double, C.c_double]
+        L.orc_sst_zeman.restype = C.c_double
+        L.orc_sst_constants.argtypes = [dp, dp]
         _lib = L
     return _lib
 
@@ -286,7 +292,8 @@ def inflow_state(mach, pressure, temperature, direction=(1.0, 0.0, 0.0), Y=None,
 # ------------------------------------------------------------- flow zone
 
 FIELDS = dict(x=0, y=1, z=2, jac=3, metric=4, bv=5, sv=6, cv=7, dq=8, vel=9, acoustic=10, mach=11, mul=12, cond=13,
-              rhoD=14, gamma=15, cp=16, dt_local=17, qn=18, bv_last=19, chem_jac=20)
+              rhoD=14, gamma=15, cp=16, dt_local=17, qn=18, bv_last=19, chem_jac=20, mut=21, turb_cond=22,
+              wall_distance=23, turb_jac=24)
 
 
 def make_cfg(case):
@@ -304,6 +311,9 @@ def make_cfg(case):
     c.gradPInDiffusionFlux = case.get("gradPInDiffusionFlux", 0)
     c.implicit_method = case.get("implicit_method", 0)
     c.dplur_inner_step = case.get("DPLUR_inner_step", 3)
+    c.turbulence = case.get("turbulence", 0)
+    c.turb_implicit = case.get("turb_implicit", 1)
+    c.compressibility_correction = case.get("compressibility_correction", 0)
     c.cfl = case.get("cfl", 0.5)
     c.Pr = case.get("prandtl_number", 0.72)
     c.Sc = case.get("schmidt_number", 0.9)
@@ -319,7 +329,7 @@ def make_cfg(case):
         st = b.get("state")
         if st is not None:
             for key in ("rho", "u", "v", "w", "p", "t", "vel", "a_inf", "gamma_inf", "entropy", "mw_inf", "p_total",
-                        "t_total", "p_back"):
+                        "t_total", "p_back", "mut"):
                 if key in st:
                     setattr(B, key, st[key])
             if "sv" in st:
