@@ -99,6 +99,53 @@ def c3_solver(d, nx, ny):
     return s
 
 
+def c5_solver(d, nx, ny, nz):
+    """Reacting RANS channel with wall injection and its seeded initial field."""
+    import ostc
+    from ostc import cases
+    case = cases.c5_case(nx, ny, nz)
+    s = ostc.solver_for_case(case, os.path.join(d, "c5"), initial=False)
+    o = s.inflow_state("vitiated_air", 7 + 11 + 1)
+    mw = s.table("mw", 9)
+    air = dict(rho=o[0], u=o[1], v=o[2], w=o[3], p=o[4], t=o[5], vel=o[6], sv=list(o[7:18]), mut=o[18],
+               mw=1.0 / sum(o[7 + l] / mw[l] for l in range(9)))
+    bv, sv = cases.c5_fields(case, air)
+    s.upload("bv", bv)
+    s.upload("sv", sv)
+    s.initialize()
+    return case, s, air
+
+
+def cpu_baseline_c5(nx=96, ny=48, nz=32, iters=2):
+    """Serial FP64 oracle on a reduced grid of the channel, same scheme,
+    boundaries and port layout."""
+    import helpers
+    import pyoracle
+    from ostc import cases
+    import ostc
+    with tempfile.TemporaryDirectory() as d:
+        case = cases.c5_case(nx, ny, nz)
+        cases.write_case_dir(case, d)
+        h = ostc.Solver(d, -1)
+        m = pyoracle.make_mech()
+        states = {}
+        for name in ("vitiated_air", "h2_jet"):
+            o = h.inflow_state(name, 19)
+            states[name] = dict(rho=o[0], u=o[1], v=o[2], w=o[3], p=o[4], t=o[5], vel=o[6], sv=list(o[7:18]),
+                                mut=o[18], mw=1.0 / sum(o[7 + l] / m.mw[l] for l in range(9)))
+        h.close()
+    bv, sv = cases.c5_fields(case, states["vitiated_air"])
+    for b in case["bcs"]:
+        if b["type"] == 5:
+            b["state"] = states[b["stream"]]
+    z = helpers.oracle_zone(case, mech=m, bv=bv, sv=sv)
+    t0 = time.perf_counter()
+    z.run_steady(iters)
+    t1 = time.perf_counter()
+    cells = nx * ny * nz
+    return cells * iters / (t1 - t0) / 1e6, f"{nx}x{ny}x{nz} grid, {iters} pseudo-iterations", t1 - t0
+
+
 def time_steps(s, steps, warmup):
     s.step(warmup)
     s.sync()
@@ -118,6 +165,8 @@ def main():
     ap.add_argument("--nz", type=int, default=128)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-c3", action="store_true")
+    ap.add_argument("--no-c5", action="store_true")
+    ap.add_argument("--c5-iters", type=int, default=5)
     ap.add_argument("--chem-cells", type=int, default=1 << 22)
     ap.add_argument("--train-pairs", type=int, default=1 << 24)
     ap.add_argument("--epochs", type=int, default=16)
@@ -169,6 +218,39 @@ def main():
                               "viscous, explicit finite-rate source, SSP-RK3",
                   "fp64_mcell_per_s": c3_cells * a.steps / t64 / 1e6, "fp64_ms_per_step": t64 / a.steps * 1e3,
                   "mixed_mcell_per_s": c3_cells * a.steps / t32 / 1e6, "mixed_ms_per_step": t32 / a.steps * 1e3}
+
+        c5 = None
+        if not a.no_c5:
+            log("bench: reacting RANS channel 384x192x128 (steady DPLUR, SST, EPI)")
+            case5, s5, _ = c5_solver(d, 384, 192, 128)
+            c5_cells = 384 * 192 * 128
+            s5.step_steady(2)
+            s5.sync()
+            s5.profile(True)
+            t0 = time.perf_counter()
+            res5 = s5.step_steady(a.c5_iters)
+            s5.sync()
+            t1 = time.perf_counter()
+            p5 = s5.profile_report()
+            s5.profile(False)
+            s5.set_flux_fp32(True)
+            s5.step_steady(1)
+            s5.sync()
+            t2 = time.perf_counter()
+            s5.step_steady(a.c5_iters)
+            s5.sync()
+            t3 = time.perf_counter()
+            s5.close()
+            c5 = {"workload": "3-D reacting RANS (k-omega SST) H2/air channel 384x192x128 with 4 sonic H2 wall "
+                              "jets, 9 species + k, omega, HLLC + MUSCL, viscous, steady DPLUR (3 inner sweeps), "
+                              "exact point-implicit chemistry, implicit SST source",
+                  "cells": c5_cells, "iters": a.c5_iters,
+                  "fp64_mcell_iter_per_s": c5_cells * a.c5_iters / (t1 - t0) / 1e6,
+                  "fp64_ms_per_iter": (t1 - t0) / a.c5_iters * 1e3,
+                  "mixed_mcell_iter_per_s": c5_cells * a.c5_iters / (t3 - t2) / 1e6,
+                  "mixed_ms_per_iter": (t3 - t2) / a.c5_iters * 1e3,
+                  "residual_finite": bool(np.isfinite(res5[:, 4]).all()),
+                  "kernel_avg_ms": {k: v[1] / v[0] for k, v in p5.items()}}
 
         # chemistry batch: FP64 point-implicit, then the surrogate
         chem = sur = None
@@ -280,6 +362,10 @@ def main():
             cv_, csample, csecs = cpu_baseline_chem()
             cpu["chem_cells_per_s"] = cv_
             cpu["chem_sample"] = f"{csample} ({csecs:.1f} s)"
+        if c5 is not None:
+            v5, s5s, t5s = cpu_baseline_c5()
+            cpu["c5_mcell_iter_per_s"] = v5
+            cpu["c5_sample"] = f"FP64 serial oracle, {s5s} of the channel ({t5s:.1f} s)"
 
     out = {
         "metric": "Mcell-updates/s per RK3 step; chem cells/s surrogate vs point-implicit",
@@ -304,6 +390,7 @@ def main():
         "mixed_fp32_flux": {"value": cells * a.steps / mixed_s / 1e6, "unit": "Mcell-updates/s",
                             "ms_per_step": mixed_s / a.steps * 1e3},
         "reacting_mixing_layer": c3,
+        "reacting_rans_channel": c5,
         "chem_point_implicit_fp64": chem,
         "chem_surrogate": sur,
         "kernel_avg_ms": stage_ms,
