@@ -163,6 +163,16 @@ int ostc_set_flux_precision(ostc_solver *s, int fp32);
  * (FiniteRateChem.cu:7-63, 234-256). */
 int ostc_chem_batch(ostc_solver *s, int64_t n, const double *rho, const double *T, const double *Y, double dt,
                     double *dY, double *dT, int mode, int64_t *n_fallback);
+/* Operator-split reacting steps.  ostc_set_chem_split(s, 1) leaves the
+ * finite-rate source out of the right-hand side (the flow step runs
+ * non-reacting); ostc_chem_step then advances every interior cell of a block
+ * by one point-implicit chemistry step dt at its density and total energy
+ * (mode 0 FP64 integrator, mode 1 surrogate + fallback, as ostc_chem_batch),
+ * and re-applies the boundary conditions and properties.  No counterpart in
+ * the reference, whose RK3 adds the explicit source in every stage
+ * (RK.cuh:107-161); this is the split form of "RK3 + point-implicit". */
+int ostc_set_chem_split(ostc_solver *s, int on);
+int ostc_chem_step(ostc_solver *s, int blk, double dt, int mode, int64_t *n_fallback);
 
 /* ---- MLP chemistry surrogate (no counterpart in the reference) ----
  * A 11 -> 256 -> 256 -> 256 -> 8 GELU network, BF16 MFMA with FP32

@@ -91,11 +91,15 @@ struct ostc_solver {
   std::map<std::string, std::pair<long, double>> stats;
   std::vector<void *> user_mem;
   SurrogateState *sur = nullptr;
+  bool chem_split = false;          // finite-rate source left out of the RHS (operator-split chemistry)
+  double *chem_buf = nullptr;       // gather / result buffers of ostc_chem_step (21 doubles per cell)
+  int64_t chem_cap = 0;
 
   ~ostc_solver() {
     for (auto &b : blocks)
       for (void *p : b.mem) (void)hipFree(p);
     for (void *p : user_mem) (void)hipFree(p);
+    if (chem_buf) (void)hipFree(chem_buf);
     if (d_mech) (void)hipFree(d_mech);
     (void)hipFree(w.partial);
     (void)hipFree(w.counter);
@@ -577,7 +581,7 @@ int rhs_block(ostc_solver *s, Block &b) {
     HIPCHK(launch_sst(b.z, s->P, s->stream));
     init = false;
   }
-  if (s->P.mix && s->P.reaction == 1) {
+  if (s->P.mix && s->P.reaction == 1 && !s->chem_split) {
     ostc_solver::Timer t(s, "source");
     HIPCHK(launch_source(b.z, s->P, s->d_mech, !init, s->stream));
     init = false;
@@ -1363,6 +1367,51 @@ int ostc_chem_batch(ostc_solver *s, int64_t n, const double *rho, const double *
   }
   HIPCHK(hipStreamSynchronize(s->stream));
   s->drain_profile();
+  return 0;
+}
+
+int ostc_set_chem_split(ostc_solver *s, int on) {
+  if (!s) return fail("null solver");
+  if (on && !(s->P.mix && s->P.reaction == 1)) return fail("operator-split chemistry needs a reacting mixture");
+  s->chem_split = on != 0;
+  return 0;
+}
+
+int ostc_chem_step(ostc_solver *s, int blk, double dt, int mode, int64_t *n_fallback) {
+  if (check_blk(s, blk)) return 1;
+  if (s->n_spec != 9 || s->mech.n_reac <= 0) return fail("the chemistry step needs the 9-species reacting mechanism");
+  if (!(dt > 0)) return fail("dt must be positive");
+  if (mode != 0 && mode != 1) return fail("mode must be 0 (point-implicit) or 1 (surrogate)");
+  if (mode == 1 && (!s->sur || !surrogate_ready(s->sur))) return fail("no trained surrogate: train or load one first");
+  HIPCHK(hipSetDevice(s->device));
+  Block &b = s->blocks[blk];
+  const int64_t n = (int64_t)b.mx * b.my * b.mz;
+  if (n > s->chem_cap) {
+    if (s->chem_buf) HIPCHK(hipFree(s->chem_buf));
+    s->chem_buf = nullptr;
+    s->chem_cap = 0;
+    HIPCHK(hipMalloc(&s->chem_buf, sizeof(double) * 21 * n));
+    s->chem_cap = n;
+  }
+  double *rho = s->chem_buf, *T = rho + n, *Y = T + n, *dY = Y + 9 * n, *dT = dY + 9 * n;
+  int64_t nf = 0;
+  {
+    ostc_solver::Timer t(s, "chem_step");
+    HIPCHK(launch_chem_gather(b.z, s->P, rho, T, Y, s->stream));
+    if (mode == 0) {
+      HIPCHK(launch_chem_batch(s->d_mech, n, rho, T, Y, dt, dY, dT, s->stream));
+    } else {
+      std::string err;
+      if (surrogate_infer(s->sur, s->d_mech, n, rho, T, Y, dt, dY, dT, &nf, s->stream, err)) return fail(err);
+    }
+    HIPCHK(launch_chem_scatter(b.z, s->P, s->d_mech, dY, dT, s->stream));
+  }
+  if (run_bcs(s, b, true)) return 1;
+  if (s->P.dim == 2) HIPCHK(launch_eliminate_k(b.z, s->P, true, s->stream));
+  if (properties_block(s, b)) return 1;
+  HIPCHK(hipStreamSynchronize(s->stream));
+  s->drain_profile();
+  if (n_fallback) *n_fallback = nf;
   return 0;
 }
 

@@ -61,6 +61,12 @@ hipError_t launch_dplur(const ZoneView &z, const DParams &P, const DMech *m, int
                         const std::vector<BCDesc> &walls, hipStream_t s);
 // local-dt update of bv / sv from dq (cv untouched)
 hipError_t launch_update_bv(const ZoneView &z, const DParams &P, const DMech *m, hipStream_t s);
+// operator-split chemistry on a zone's interior: gather (rho, T, Y SoA with
+// stride mx*my*mz) and scatter (Y += dY, T += dT at fixed rho and rho E, p
+// from the equation of state, rho Y rows of cv)
+hipError_t launch_chem_gather(const ZoneView &z, const DParams &P, double *rho, double *T, double *Y, hipStream_t s);
+hipError_t launch_chem_scatter(const ZoneView &z, const DParams &P, const DMech *m, const double *dY, const double *dT,
+                               hipStream_t s);
 // ---- k-omega SST (sst.hip)
 // wall distance of every ghosted point: xyz the ghosted node coordinates
 // (3 x sz, component-major), wall the nw wall points (3 x nw, component-major)

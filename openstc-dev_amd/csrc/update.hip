@@ -836,9 +836,67 @@ __global__ void __launch_bounds__(128) k_epi_sigma(const DMech *__restrict__ m, 
   sig[c] = sqrt(mn);
 }
 
+// Operator-split chemistry on a zone: the interior state is gathered into
+// batch arrays (rho, T, Y SoA with stride n, cells i fastest), integrated by
+// the batch kernels (or the surrogate), and scattered back: Y += dY,
+// T += dT at the unchanged density and total energy, p from the equation of
+// state, the rho Y rows of cv.
+template <int NS>
+__global__ void __launch_bounds__(kTB) k_chem_gather(ZoneView z, double *__restrict__ rho, double *__restrict__ T,
+                                                     double *__restrict__ Y) {
+  const int64_t t = (int64_t)blockIdx.x * kTB + threadIdx.x;
+  const int64_t n = (int64_t)z.mx * z.my * z.mz;
+  if (t >= n) return;
+  int i, j, k;
+  cell_of(z, t, i, j, k);
+  const int64_t p = z.at(i, j, k);
+  rho[t] = z.bv[p];
+  T[t] = z.bv[5 * z.sz + p];
+#pragma unroll
+  for (int l = 0; l < NS; ++l) Y[(int64_t)l * n + t] = z.sv[(int64_t)l * z.sz + p];
+}
+
+template <int NS>
+__global__ void __launch_bounds__(kTB) k_chem_scatter(ZoneView z, const DMech *__restrict__ m,
+                                                      const double *__restrict__ dY, const double *__restrict__ dT) {
+  const int64_t t = (int64_t)blockIdx.x * kTB + threadIdx.x;
+  const int64_t n = (int64_t)z.mx * z.my * z.mz;
+  if (t >= n) return;
+  int i, j, k;
+  cell_of(z, t, i, j, k);
+  const int64_t p = z.at(i, j, k);
+  const double rho = z.bv[p];
+  double mwi = 0;
+#pragma unroll
+  for (int l = 0; l < NS; ++l) {
+    const double y = z.sv[(int64_t)l * z.sz + p] + dY[(int64_t)l * n + t];
+    z.sv[(int64_t)l * z.sz + p] = y;
+    z.cv[(int64_t)(5 + l) * z.sz + p] = rho * y;
+    mwi += y / m->mw[l];
+  }
+  const double T = z.bv[5 * z.sz + p] + dT[t];
+  z.bv[5 * z.sz + p] = T;
+  z.bv[4 * z.sz + p] = rho * T * kRuD * mwi;
+}
+
 inline unsigned nblk(int64_t n, int tb) { return (unsigned)((n + tb - 1) / tb); }
 
 }  // namespace
+
+hipError_t launch_chem_gather(const ZoneView &z, const DParams &P, double *rho, double *T, double *Y, hipStream_t s) {
+  const int64_t n = (int64_t)z.mx * z.my * z.mz;
+  if (P.ns != 9) return hipErrorNotSupported;
+  hipLaunchKernelGGL(k_chem_gather<9>, dim3(nblk(n, kTB)), dim3(kTB), 0, s, z, rho, T, Y);
+  return hipGetLastError();
+}
+
+hipError_t launch_chem_scatter(const ZoneView &z, const DParams &P, const DMech *m, const double *dY, const double *dT,
+                               hipStream_t s) {
+  const int64_t n = (int64_t)z.mx * z.my * z.mz;
+  if (P.ns != 9) return hipErrorNotSupported;
+  hipLaunchKernelGGL(k_chem_scatter<9>, dim3(nblk(n, kTB)), dim3(kTB), 0, s, z, m, dY, dT);
+  return hipGetLastError();
+}
 
 hipError_t launch_source(const ZoneView &z, const DParams &P, const DMech *m, bool accumulate, hipStream_t s) {
   const int64_t n = (int64_t)z.mx * z.my * z.mz;

@@ -75,6 +75,8 @@ def lib():
             "ostc_set_flux_precision": ([vp, C.c_int], C.c_int),
             "ostc_chem_batch": ([vp, C.c_int64, vp, vp, vp, C.c_double, vp, vp, C.c_int,
                                  C.POINTER(C.c_int64)], C.c_int),
+            "ostc_set_chem_split": ([vp, C.c_int], C.c_int),
+            "ostc_chem_step": ([vp, C.c_int, C.c_double, C.c_int, C.POINTER(C.c_int64)], C.c_int),
             "ostc_dev_alloc": ([vp, C.c_int64, C.POINTER(vp)], C.c_int),
             "ostc_dev_free": ([vp, vp], C.c_int),
             "ostc_memcpy_h2d": ([vp, vp, vp, C.c_int64], C.c_int),
@@ -284,6 +286,15 @@ class Solver:
 
     def set_flux_fp32(self, on):
         _chk(lib().ostc_set_flux_precision(self.h, 1 if on else 0))
+
+    def set_chem_split(self, on):
+        _chk(lib().ostc_set_chem_split(self.h, 1 if on else 0))
+
+    def chem_step(self, dt, mode=0, blk=0):
+        """One operator-split chemistry step on a block; returns the fallback count."""
+        nf = C.c_int64()
+        _chk(lib().ostc_chem_step(self.h, blk, dt, mode, C.byref(nf)))
+        return nf.value
 
     def chem_batch(self, n, rho, T, Y, dt, dY, dT, mode=0):
         """rho/T/Y/dY/dT: DeviceArray (Y, dY component-major n_spec x n)."""

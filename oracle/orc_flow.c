@@ -1845,7 +1845,7 @@ static void source(orc_zone *zz) {
     for (int k = 0; k < zz->mz; ++k)
       for (int j = 0; j < zz->my; ++j)
         for (int i = 0; i < zz->mx; ++i) sst_source_point(zz, i, j, k);
-  if (!(zz->cfg.mix && zz->cfg.reaction == 1)) return;
+  if (!(zz->cfg.mix && zz->cfg.reaction == 1) || zz->cfg.split_chem) return;
   const int ns = zz->ns;
   for (int k = 0; k < zz->mz; ++k)
     for (int j = 0; j < zz->my; ++j)
@@ -2255,6 +2255,40 @@ static void update_bv(orc_zone *zz) {
         }
         zz->vel[pg] = sqrt(V2);
       }
+}
+
+void orc_zone_chem_step(orc_zone *zz, double dt) {
+  const int ns = zz->ns;
+  const int64_t n = (int64_t)zz->mx * zz->my * zz->mz;
+  double *rho = malloc(sizeof(double) * n * 21 + 8);
+  double *T = rho + n, *Y = T + n, *dY = Y + ns * n, *dT = dY + ns * n;
+  int64_t c = 0;
+  for (int k = 0; k < zz->mz; ++k)
+    for (int j = 0; j < zz->my; ++j)
+      for (int i = 0; i < zz->mx; ++i, ++c) {
+        rho[c] = BV(0, i, j, k);
+        T[c] = BV(5, i, j, k);
+        for (int l = 0; l < ns; ++l) Y[l * n + c] = SV(l, i, j, k);
+      }
+  orc_chem_batch(&zz->mech, n, rho, T, Y, dt, dY, dT);
+  c = 0;
+  for (int k = 0; k < zz->mz; ++k)
+    for (int j = 0; j < zz->my; ++j)
+      for (int i = 0; i < zz->mx; ++i, ++c) {
+        double mwi = 0;
+        for (int l = 0; l < ns; ++l) {
+          const double y = SV(l, i, j, k) + dY[l * n + c];
+          SV(l, i, j, k) = y;
+          CV(5 + l, i, j, k) = BV(0, i, j, k) * y;
+          mwi += y / zz->mech.mw[l];
+        }
+        BV(5, i, j, k) += dT[c];
+        BV(4, i, j, k) = BV(0, i, j, k) * BV(5, i, j, k) * R_U * mwi;
+      }
+  free(rho);
+  apply_bcs(zz, 1);
+  if (zz->dim == 2) eliminate_k_gradient(zz, 1);
+  update_properties(zz);
 }
 
 /* SteadySim.cuh:52-124 (single block, no data communication) */

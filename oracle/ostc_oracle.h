@@ -113,6 +113,7 @@ typedef struct {
   int turbulence;             /* 0 laminar, 1 k-omega SST (two scalars k, omega after the species) */
   int turb_implicit;          /* SST source Jacobian diagonal in the implicit treatment */
   int compressibility_correction; /* 0 none, 1 Wilcox, 2 Sarkar, 3 Zeman */
+  int split_chem;             /* finite-rate source left out of the RHS (operator-split chemistry) */
   double cfl, Pr, Sc, Prt, Sct, dt, entropy_fix_factor;
   int n_bc;
   orc_bc bc[ORC_MAXBC];
@@ -146,6 +147,10 @@ void orc_zone_run_steady(orc_zone *zz, int nsteps, double *res_hist);
 /* the implicit treatment alone on the current dq (ImplicitTreatmentHPP.cuh:7-52);
  * dt_local / spectral radii must be current */
 void orc_zone_implicit(orc_zone *zz);
+/* operator-split chemistry: one point-implicit step dt of every interior cell
+ * at its density and total energy (orc_chem_batch on the gathered state),
+ * then boundary conditions with cv, the 2-D k-copy and the properties */
+void orc_zone_chem_step(orc_zone *zz, double dt);
 
 /* grid metrics on their own (Mesh.cpp:259-521, 790-944): x,y,z interior in, outputs ghosted */
 void orc_grid_metrics(int mx, int my, int mz, int ngg, const double *x, const double *y, const double *z,

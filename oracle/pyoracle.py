@@ -60,7 +60,8 @@ class Cfg(C.Structure):
                 ("reconstruction", C.c_int), ("limiter", C.c_int), ("viscous_order", C.c_int),
                 ("fixed_time_step", C.c_int), ("output_screen", C.c_int), ("gradPInDiffusionFlux", C.c_int),
                 ("implicit_method", C.c_int), ("dplur_inner_step", C.c_int), ("turbulence", C.c_int),
-                ("turb_implicit", C.c_int), ("compressibility_correction", C.c_int), ("cfl", C.c_double), ("Pr", C.c_double), ("Sc", C.c_double), ("Prt", C.c_double),
+                ("turb_implicit", C.c_int), ("compressibility_correction", C.c_int), ("split_chem", C.c_int),
+                ("cfl", C.c_double), ("Pr", C.c_double), ("Sc", C.c_double), ("Prt", C.c_double),
                 ("Sct", C.c_double), ("dt", C.c_double), ("entropy_fix_factor", C.c_double),
                 ("n_bc", C.c_int), ("bc", BC * MAXBC)]
 
@@ -104,6 +105,7 @@ def lib():
         L.orc_zone_run_rk3.argtypes = [C.c_void_p, C.c_int, dp, dp]
         L.orc_zone_run_steady.argtypes = [C.c_void_p, C.c_int, dp]
         L.orc_zone_implicit.argtypes = [C.c_void_p]
+        L.orc_zone_chem_step.argtypes = [C.c_void_p, C.c_double]
         L.orc_zone_state.argtypes = [C.c_void_p, dp, C.POINTER(C.c_int)]
         L.orc_grid_metrics.argtypes = [C.c_int] * 4 + [dp] * 7
         L.orc_sst_wilcox.argtypes = [C.c_double]
@@ -314,6 +316,7 @@ def make_cfg(case):
     c.turbulence = case.get("turbulence", 0)
     c.turb_implicit = case.get("turb_implicit", 1)
     c.compressibility_correction = case.get("compressibility_correction", 0)
+    c.split_chem = case.get("split_chem", 0)
     c.cfl = case.get("cfl", 0.5)
     c.Pr = case.get("prandtl_number", 0.72)
     c.Sc = case.get("schmidt_number", 0.9)
@@ -374,6 +377,9 @@ class Zone:
 
     def rhs(self):
         lib().orc_zone_rhs(self.h)
+
+    def chem_step(self, dt):
+        lib().orc_zone_chem_step(self.h, dt)
 
     def global_dt(self):
         return lib().orc_zone_global_dt(self.h)
