@@ -82,12 +82,17 @@ def cpu_baseline_chem(n=1 << 20):
     return n / (t1 - t0), f"{n} cells of the seeded batch", t1 - t0
 
 
-def c3_solver(d, nx, ny):
-    """Reacting mixing-layer solver with its seeded initial field."""
+def c3_solver(d, nx, ny, fixed_dt=None):
+    """Reacting mixing-layer solver with its seeded initial field (CFL dt,
+    or a fixed dt)."""
     import ostc
     from ostc import cases
     case = cases.c3_case(nx, ny)
-    s = ostc.solver_for_case(case, os.path.join(d, "c3"), initial=False)
+    if fixed_dt is not None:
+        case["setup"]["bool fixed_time_step"] = 1
+        case["setup"]["real dt"] = fixed_dt
+        case["name"] = "c3_fixed"
+    s = ostc.solver_for_case(case, os.path.join(d, case["name"]), initial=False)
     states = {}
     for name in ("air_stream", "fuel_stream"):
         o = s.inflow_state(name, 16)
@@ -166,6 +171,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-c3", action="store_true")
     ap.add_argument("--no-c5", action="store_true")
+    ap.add_argument("--no-split", action="store_true")
+    ap.add_argument("--split-dt", type=float, default=1e-8)
+    ap.add_argument("--split-pairs", type=int, default=1 << 22)
     ap.add_argument("--c5-iters", type=int, default=5)
     ap.add_argument("--chem-cells", type=int, default=1 << 22)
     ap.add_argument("--train-pairs", type=int, default=1 << 24)
@@ -218,6 +226,72 @@ def main():
                               "viscous, explicit finite-rate source, SSP-RK3",
                   "fp64_mcell_per_s": c3_cells * a.steps / t64 / 1e6, "fp64_ms_per_step": t64 / a.steps * 1e3,
                   "mixed_mcell_per_s": c3_cells * a.steps / t32 / 1e6, "mixed_ms_per_step": t32 / a.steps * 1e3}
+
+        split = None
+        if not a.no_split and a.split_pairs > 0:
+            # operator-split reacting step on the mixing layer at a fixed dt:
+            # FP64 flow + FP64 point-implicit chemistry, against FP32-flux flow
+            # + surrogate chemistry (network trained untimed on mixing-layer
+            # states at the same dt); both runs start from the same state
+            log("bench: split reacting step 512x256, fixed dt")
+            s4 = c3_solver(d, 512, 256, fixed_dt=a.split_dt)
+            n4 = 512 * 256
+            s4.set_chem_split(True)
+            bv0, sv0 = s4.download("bv").copy(), s4.download("sv").copy()
+            mw9 = s4.table("mw", 9)
+
+            def run(mode, nsteps):
+                for _ in range(nsteps):
+                    s4.step(1)
+                    s4.chem_step(a.split_dt, mode=mode)
+                s4.sync()
+
+            run(0, a.warmup)
+            t0 = time.perf_counter()
+            run(0, a.steps)
+            t1 = time.perf_counter()
+            bv_ref, sv_ref = s4.download("bv").reshape(6, -1), s4.download("sv").reshape(9, -1)
+            log(f"bench: surrogate training for the split step on {a.split_pairs} mixing-layer states (untimed)")
+            r2, t2, p2, y2 = cases.mixing_layer_states(a.split_pairs, mw=mw9)
+            tr = [ostc.DeviceArray.from_host(s4, v) for v in (r2, t2, y2)]
+            s4.surrogate_init(13)
+            hist4 = s4.surrogate_train(a.split_pairs, tr[0], tr[1], tr[2], a.split_dt, epochs=8, batch=8192,
+                                       seed=2003)
+            for v in tr:
+                v.free()
+            s4.upload("bv", bv0)
+            s4.upload("sv", sv0)
+            s4.initialize()
+            s4.set_flux_fp32(True)
+            nfb = []
+
+            def run_sur(nsteps):
+                for _ in range(nsteps):
+                    s4.step(1)
+                    nfb.append(s4.chem_step(a.split_dt, mode=1))
+                s4.sync()
+
+            run_sur(a.warmup)
+            t2_ = time.perf_counter()
+            run_sur(a.steps)
+            t3_ = time.perf_counter()
+            bv_s, sv_s = s4.download("bv").reshape(6, -1), s4.download("sv").reshape(9, -1)
+            s4.close()
+            Tr, Ts = bv_ref[5], bv_s[5]
+            split = {"workload": "2-D reacting H2/air mixing layer 512x256, operator-split step: SSP-RK3 flow step "
+                                 "(HLLC + MUSCL, viscous) + one point-implicit chemistry step, fixed dt",
+                     "dt": a.split_dt,
+                     "fp64_plain_ms_per_step": (t1 - t0) / a.steps * 1e3,
+                     "mixed_surrogate_ms_per_step": (t3_ - t2_) / a.steps * 1e3,
+                     "speedup": (t1 - t0) / (t3_ - t2_),
+                     "fp64_plain_mcell_per_s": n4 * a.steps / (t1 - t0) / 1e6,
+                     "mixed_surrogate_mcell_per_s": n4 * a.steps / (t3_ - t2_) / 1e6,
+                     "fallback_fraction": float(np.mean(nfb)) / n4,
+                     "steps_compared": a.warmup + a.steps,
+                     "peak_T_rel_err": float(abs(Ts.max() - Tr.max()) / Tr.max()),
+                     "T_rel_linf": float(np.max(np.abs(Ts - Tr) / Tr)),
+                     "major_species_linf": float(np.abs(sv_s[[0, 2, 7, 8]] - sv_ref[[0, 2, 7, 8]]).max()),
+                     "train_final_loss": float(hist4[-1])}
 
         c5 = None
         if not a.no_c5:
@@ -390,6 +464,7 @@ def main():
         "mixed_fp32_flux": {"value": cells * a.steps / mixed_s / 1e6, "unit": "Mcell-updates/s",
                             "ms_per_step": mixed_s / a.steps * 1e3},
         "reacting_mixing_layer": c3,
+        "reacting_split_step": split,
         "reacting_rans_channel": c5,
         "chem_point_implicit_fp64": chem,
         "chem_surrogate": sur,

@@ -1115,6 +1115,10 @@ int ostc_initialize(ostc_solver *s) {
   HIPCHK(hipSetDevice(s->device));
   for (auto &b : s->blocks) {
     if (s->P.dim == 2) HIPCHK(launch_eliminate_k(b.z, s->P, false, s->stream));
+    // the mixture farfield reads gamma of its boundary point; the reference
+    // applies the first BCs before any properties exist (Driver.cu:53-92,
+    // reading unset memory), here they are computed from the initial field
+    if (s->P.mix && properties_block(s, b)) return 1;
     if (run_bcs(s, b, false)) return 1;
     HIPCHK(launch_velocity(b.z, s->stream));
     if (s->n_turb) HIPCHK(launch_init_mut(b.z, s->P, s->d_mech, s->stream));

@@ -563,6 +563,34 @@ def chem_batch_inputs(n, seed=1004, species=("H2", "H", "O2", "O", "OH", "HO2", 
     return rho, t, p, Y
 
 
+def mixing_layer_states(n, seed=2003, species=("H2", "H", "O2", "O", "OH", "HO2", "H2O2", "H2O", "N2"), mw=None):
+    """Chemistry training states spanning the reacting mixing layer: mixture
+    fraction Z ~ U(0, 1) between the air stream (Y_O2 = 0.233, Y_N2 = 0.767)
+    and the fuel stream (Y_H2 = 0.3, Y_N2 = 0.7), the frozen mixing
+    temperature 1500 (1 - Z) + 300 Z plus U(-50, 600) K (floored at 250 K),
+    p ~ logU(0.5, 2) atm, and H, O, OH, HO2, H2O2, H2O at 10^U(-16, -4)
+    mass fraction (so that streams without radicals lie inside the range);
+    renormalised, rho from the equation of state.  Returns (rho, T, p,
+    Y[ns, n])."""
+    rng = SplitMix64(seed)
+    z = rng.uniform(n)
+    t = 1500.0 * (1 - z) + 300.0 * z - 50.0 + 650.0 * rng.uniform(n)
+    t = np.maximum(t, 250.0)
+    p = 101325.0 * np.exp(np.log(0.5) + (np.log(2.0) - np.log(0.5)) * rng.uniform(n))
+    idx = {s: i for i, s in enumerate(species)}
+    Y = np.zeros((len(species), n))
+    Y[idx["O2"]] = 0.233 * (1 - z)
+    Y[idx["N2"]] = 0.767 * (1 - z) + 0.7 * z
+    Y[idx["H2"]] = 0.3 * z
+    for s in ("H", "O", "OH", "HO2", "H2O2", "H2O"):
+        Y[idx[s]] = 10.0 ** (-16.0 + 12.0 * rng.uniform(n))
+    Y /= Y.sum(axis=0, keepdims=True)
+    if mw is None:
+        mw = np.array([2.016, 1.008, 31.998, 15.999, 17.007, 33.006, 34.014, 18.015, 28.014])
+    rho = p / (R_U * t * (Y / mw[:, None]).sum(axis=0))
+    return rho, t, p, Y
+
+
 def _fmt(v):
     if isinstance(v, (float, np.floating)):
         return repr(float(v))

@@ -901,6 +901,10 @@ static void initialize_mut(orc_zone *zz);
 void orc_zone_initialize(orc_zone *zz) {
   const int g = zz->g;
   if (zz->dim == 2) eliminate_k_gradient(zz, 0);
+  /* the mixture farfield reads gamma of its boundary point; the reference
+   * applies the first BCs before any properties exist (Driver.cu:53-92,
+   * reading unset memory), here they are computed from the initial field */
+  if (zz->cfg.mix) update_properties(zz);
   apply_bcs(zz, 0);
   for (int k = -g; k < zz->mz + g; ++k)
     for (int j = -g; j < zz->my + g; ++j)
