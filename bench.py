@@ -54,6 +54,25 @@ def chem_flop_per_cell(labels, ns=9):
     return 154 + 143 + sum(per.get(int(l), 325) for l in labels) + 279 + 162 + 650 + 27 + 542 + 10
 
 
+# kernel behind each timer name of the air case (AWENO-5 + CDS, FP64)
+SWEEP_KERNELS = {"sweep_i": "sweep_i<0, 0, 1, double>", "sweep_j": "sweep_jk<1, 0, 0, 1, double>",
+                 "sweep_k": "sweep_jk<2, 0, 0, 1, double>"}
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "air_256x128x128_hbm_traffic.md")
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed counter summary
+    (FETCH_SIZE + WRITE_SIZE, copy-calibrated), or None."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch " + kernel + " = "):
+                    return float(line.rsplit("=", 1)[1])
+    except OSError:
+        pass
+    return None
+
+
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
@@ -424,6 +443,8 @@ def main():
     avg_ms = total_ms / calls
     algo_bytes = SWEEP_JK_BYTES_PER_CELL * cells
     achieved = algo_bytes / (avg_ms * 1e-3) / 1e9
+    kname = SWEEP_KERNELS.get(dom, dom)
+    traffic = measured_traffic(kname) if (a.nx, a.ny, a.nz) == (256, 128, 128) else None
     stage_ms = {k: v[1] / v[0] for k, v in prof.items()}
 
     cpu = None
@@ -457,8 +478,8 @@ def main():
         "config": {"workload": f"3-D laminar NS air, curvilinear {a.nx}x{a.ny}x{a.nz}, AWENO-5+CDS, AUSM+ "
                                f"(Roe+WENO5 request), 2nd-order viscous, SSP-RK3 CFL 0.5, isothermal wall",
                    "cells": cells, "parallelism": "single GPU"},
-        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+        "roofline": {"bound": "hbm", "kernel": kname, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "algorithmic_bytes_per_launch": algo_bytes, "avg_launch_ms": avg_ms},
         "cpu_baseline": cpu,
         "mixed_fp32_flux": {"value": cells * a.steps / mixed_s / 1e6, "unit": "Mcell-updates/s",

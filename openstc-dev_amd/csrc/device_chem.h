@@ -129,12 +129,21 @@ __device__ inline void d_lu_solve(double *A, double *b) {
   int piv[N];
 #pragma unroll
   for (int n = 0; n < N; ++n) {
+    // the running maximum is kept in a register (comparing against
+    // A[ik][n] would index A dynamically and push it to scratch memory)
     int ik = n;
+    double best = 0;
 #pragma unroll
     for (int r = n; r < N; ++r) {
 #pragma unroll
       for (int t = 0; t < n; ++t) A[r * N + n] -= A[r * N + t] * A[t * N + n];
-      if (fabs(A[r * N + n]) > fabs(A[ik * N + n])) ik = r;
+      const double v = fabs(A[r * N + n]);
+      if (r == n) {
+        best = v;
+      } else if (v > best) {
+        best = v;
+        ik = r;
+      }
     }
     piv[n] = ik;
     if (ik != n) {
