@@ -638,6 +638,9 @@ int ostc_init(const char *case_dir, int device, ostc_solver **out) {
                             p.get_string("transport_file", "chemistry/tran.dat"), reaction == 1, s->mech);
     if (!e.empty()) return fail(e);
     s->n_spec = s->mech.n_spec;
+    // the point-implicit kernels park the rates of progress of at most 24
+    // reactions per cell in LDS
+    if (s->mech.n_reac > 24) return fail("mechanisms with more than 24 reactions are not supported by this build");
   }
   // Field.cu:4-60: variables per cell (species, then k and omega)
   s->n_turb = n_turb;

@@ -11,6 +11,24 @@ __host__ __device__ constexpr double kPatmD() { return 101325.0; }
 
 __device__ inline double d_arrhenius(double t, double A, double b, double Ea) { return A * pow(t, b) * exp(-Ea / t); }
 
+// x^n for the small integer exponents of stoichiometric coefficients and
+// reaction orders, as products (x^0 = 1, x^1 = x and x^2 = x*x exactly as a
+// correctly rounded pow gives them; x^3 and negative orders within one or
+// two roundings); any other exponent goes to pow.  The exponent is uniform
+// across a wavefront (it belongs to the reaction), so the switch is a
+// scalar branch.
+__device__ inline double d_ipow(double x, int n) {
+  switch (n) {
+    case 0: return 1.0;
+    case 1: return x;
+    case 2: return x * x;
+    case 3: return x * x * x;
+    case -1: return 1.0 / x;
+    case -2: return 1.0 / (x * x);
+    default: return pow(x, (double)n);
+  }
+}
+
 // FiniteRateChem.cu:65-149 for reaction r: kf (Arrhenius + duplicate, third
 // body, Lindemann, Troe) and kb = kf / Kc (or the REV set, 0 if irreversible).
 template <int NS>
@@ -53,7 +71,7 @@ __device__ inline void d_rate(const DMech *__restrict__ m, int r, double t, cons
     double dg = 0;
 #pragma unroll
     for (int l = 0; l < NS; ++l) dg += g[l] * (m->stoi_b[r * NS + l] - m->stoi_f[r * NS + l]);
-    const double kc = pow(temp_t, (double)m->order[r]) * exp(-dg);
+    const double kc = d_ipow(temp_t, m->order[r]) * exp(-dg);
     kb = k / kc;
   }
 }
@@ -87,8 +105,8 @@ __device__ inline void d_chem_source(const DMech *__restrict__ m, double rho, do
     double q1 = 1.0, q2 = 1.0;
 #pragma unroll
     for (int l = 0; l < NS; ++l) {
-      q1 *= pow(c[l], (double)m->stoi_f[r * NS + l]);
-      q2 *= pow(c[l], (double)m->stoi_b[r * NS + l]);
+      q1 *= d_ipow(c[l], m->stoi_f[r * NS + l]);
+      q2 *= d_ipow(c[l], m->stoi_b[r * NS + l]);
     }
     q1 *= kf;
     q2 = m->label[r] != 0 ? q2 * kb : 0.0;
@@ -122,6 +140,80 @@ __device__ inline void d_chem_source(const DMech *__restrict__ m, double rho, do
   }
 }
 
+// The same source and Jacobian in two passes over the reactions, for kernels
+// that solve with I - dt J: the first computes omega and parks the rates of
+// progress q1, q2 of reaction r at q[2r * stride], q[(2r + 1) * stride]
+// (LDS, one column per thread), the second accumulates the Jacobian from
+// them.  Keeping the 81 Jacobian sums out of the first pass (where the
+// concentrations, free energies and creation / destruction sums are live)
+// keeps the kernel in registers.  Sums run over r in the same order as in
+// d_chem_source, so the results are the same.
+template <int NS>
+__device__ inline void d_chem_rates(const DMech *__restrict__ m, double rho, double t, const double *Y, double *omega,
+                                    double *q, int stride) {
+  double c[NS], g[NS];
+#pragma unroll
+  for (int l = 0; l < NS; ++l) c[l] = rho * Y[l] / m->mw[l] * 1e-3;
+  d_gibbs_rt<NS>(t, m, g);
+  const double temp_t = (kPatmD() / kRuD * 1e-3) / t;
+  const int nr = m->n_reac;
+  double creation[NS], destr[NS];
+#pragma unroll
+  for (int l = 0; l < NS; ++l) {
+    creation[l] = 0;
+    destr[l] = 0;
+  }
+  for (int r = 0; r < nr; ++r) {
+    double kf, kb;
+    d_rate<NS>(m, r, t, c, g, temp_t, kf, kb);
+    double q1 = 1.0, q2 = 1.0;
+#pragma unroll
+    for (int l = 0; l < NS; ++l) {
+      q1 *= d_ipow(c[l], m->stoi_f[r * NS + l]);
+      q2 *= d_ipow(c[l], m->stoi_b[r * NS + l]);
+    }
+    q1 *= kf;
+    q2 = m->label[r] != 0 ? q2 * kb : 0.0;
+    q[2 * r * stride] = q1;
+    q[(2 * r + 1) * stride] = q2;
+#pragma unroll
+    for (int l = 0; l < NS; ++l) {
+      const int nf = m->stoi_f[r * NS + l], nb = m->stoi_b[r * NS + l];
+      creation[l] += q2 * nf + q1 * nb;
+      destr[l] += q1 * nf + q2 * nb;
+    }
+  }
+#pragma unroll
+  for (int l = 0; l < NS; ++l) omega[l] = creation[l] * (m->mw[l] * 1e+3) - destr[l] * (m->mw[l] * 1e+3);
+}
+
+// The EPI Jacobian J from the parked rates of progress; with EPI, A = I - dt J
+template <int NS, bool EPI = true>
+__device__ inline void d_epi_matrix(const DMech *__restrict__ m, double rho, const double *Y, double dt,
+                                    const double *q, int stride, double *A) {
+#pragma unroll
+  for (int a = 0; a < NS * NS; ++a) A[a] = 0;
+  const int nr = m->n_reac;
+  for (int r = 0; r < nr; ++r) {
+    const double q1 = q[2 * r * stride], q2 = q[(2 * r + 1) * stride];
+#pragma unroll
+    for (int n = 0; n < NS; ++n) {
+      const double dn = m->stoi_f[r * NS + n] * q1 - m->stoi_b[r * NS + n] * q2;
+#pragma unroll
+      for (int a = 0; a < NS; ++a) A[a * NS + n] += (m->stoi_b[r * NS + a] - m->stoi_f[r * NS + a]) * dn;
+    }
+  }
+#pragma unroll
+  for (int n = 0; n < NS; ++n) {
+    const bool on = Y[n] > 1e-25;
+#pragma unroll
+    for (int a = 0; a < NS; ++a) {
+      const double J = on ? m->mw[a] * (A[a * NS + n] / (rho * Y[n])) * 1e+3 : 0.0;
+      A[a * NS + n] = EPI ? (a == n ? 1.0 : 0.0) - dt * J : J;
+    }
+  }
+}
+
 // FiniteRateChem.cu:364-418 solve_chem_system: Crout-style LU with row
 // pivoting (first maximum wins), then substitution; rhs overwritten.
 template <int N>
@@ -146,18 +238,16 @@ __device__ inline void d_lu_solve(double *A, double *b) {
       }
     }
     piv[n] = ik;
-    if (ik != n) {
+    // row swap n <-> ik as selects on static indices (a branch around the
+    // swap keeps A out of registers)
+#pragma unroll
+    for (int r = n + 1; r < N; ++r) {
+      const bool sw = r == ik;
 #pragma unroll
       for (int t = 0; t < N; ++t) {
-        // row swap through selects keeps the indexing static
-#pragma unroll
-        for (int r = n + 1; r < N; ++r) {
-          if (r == ik) {
-            const double tmp = A[r * N + t];
-            A[r * N + t] = A[n * N + t];
-            A[n * N + t] = tmp;
-          }
-        }
+        const double x = A[r * N + t], y = A[n * N + t];
+        A[r * N + t] = sw ? y : x;
+        A[n * N + t] = sw ? x : y;
       }
     }
 #pragma unroll
@@ -169,14 +259,12 @@ __device__ inline void d_lu_solve(double *A, double *b) {
   }
 #pragma unroll
   for (int r = 0; r < N; ++r) {
-    const int t = piv[r];
 #pragma unroll
     for (int q = r + 1; q < N; ++q) {
-      if (q == t) {
-        const double tmp = b[q];
-        b[q] = b[r];
-        b[r] = tmp;
-      }
+      const bool sw = q == piv[r];
+      const double x = b[q], y = b[r];
+      b[q] = sw ? y : x;
+      b[r] = sw ? x : y;
     }
   }
 #pragma unroll
@@ -261,6 +349,16 @@ __device__ inline void d_transport(const DMech *__restrict__ m, double t, double
   mu = viscosity;
   lam = conductivity;
   const double t32p = t * st / p;
+  // binary diffusivities of the NS (NS - 1) / 2 pairs, each once
+  double dij[NS * (NS - 1) / 2 > 0 ? NS * (NS - 1) / 2 : 1];
+#pragma unroll
+  for (int a = 0; a < NS; ++a)
+#pragma unroll
+    for (int b = a + 1; b < NS; ++b) {
+      const double tr = t * m->kb_over_eps_jk[a * NS + b];
+      const double od = 1.0 / pow(tr, 0.145) + 1.0 / ((tr + 0.5) * (tr + 0.5));
+      dij[a * NS - a * (a + 1) / 2 + (b - a - 1)] = m->binary_diffusivity_coeff[a * NS + b] * t32p / od;
+    }
 #pragma unroll
   for (int l = 0; l < NS; ++l) {
     double num = 0, den = 0;
@@ -268,11 +366,8 @@ __device__ inline void d_transport(const DMech *__restrict__ m, double t, double
     for (int n = 0; n < NS; ++n) {
       if (l != n) {
         const int a = l < n ? l : n, b = l < n ? n : l;
-        const double tr = t * m->kb_over_eps_jk[a * NS + b];
-        const double od = 1.0 / pow(tr, 0.145) + 1.0 / ((tr + 0.5) * (tr + 0.5));
-        const double dij = m->binary_diffusivity_coeff[a * NS + b] * t32p / od;
         num += (X[n] + 1e-12) * m->mw[n];
-        den += (X[n] + 1e-12) / dij;
+        den += (X[n] + 1e-12) / dij[a * NS - a * (a + 1) / 2 + (b - a - 1)];
       }
     }
     rhoD[l] = rho * num / (den * mw_total);

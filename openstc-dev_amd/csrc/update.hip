@@ -38,10 +38,16 @@ __device__ inline double wave_sum(double v) {
 // SourceTerm.cuh:11-37 (finite-rate part): dq = 0 for the flow components,
 // dq(5+l) = J omega_l, and the chemistry Jacobian (EPI) or its diagonal (DA).
 // With accumulate (the SST kernel already initialised dq) the species rows
-// are added to and the flow rows left alone.
+// are added to and the flow rows left alone.  The EPI Jacobian is built in a
+// second pass over the rates of progress parked in LDS (one column per
+// thread, no barrier).
+constexpr int kSrcTB = 128;
+constexpr int kSrcMaxR = 24;
+
 template <int NS>
-__global__ void __launch_bounds__(kTB) k_source(ZoneView z, DParams P, const DMech *__restrict__ m, int accumulate) {
-  const int64_t t = (int64_t)blockIdx.x * kTB + threadIdx.x;
+__global__ void __launch_bounds__(kSrcTB) k_source(ZoneView z, DParams P, const DMech *__restrict__ m, int accumulate) {
+  __shared__ double sq[2 * kSrcMaxR][kSrcTB];
+  const int64_t t = (int64_t)blockIdx.x * kSrcTB + threadIdx.x;
   const int64_t n = (int64_t)z.mx * z.my * z.mz;
   if (t >= n) return;
   int i, j, k;
@@ -53,7 +59,8 @@ __global__ void __launch_bounds__(kTB) k_source(ZoneView z, DParams P, const DMe
   const double rho = z.bv[p], T = z.bv[5 * z.sz + p], J = z.jac[p];
   if (P.chemSrcMethod == 1) {
     double jac[NS * NS];
-    d_chem_source<NS>(m, rho, T, Y, om, jac);
+    d_chem_rates<NS>(m, rho, T, Y, om, &sq[0][threadIdx.x], kSrcTB);
+    d_epi_matrix<NS, false>(m, rho, Y, 0.0, &sq[0][threadIdx.x], kSrcTB, jac);
 #pragma unroll
     for (int c = 0; c < NS * NS; ++c) z.chem_jac[(int64_t)c * z.sz0 + q] = jac[c];
   } else if (P.chemSrcMethod == 2) {
@@ -737,12 +744,19 @@ __global__ void k_copy1(const double *src, double *dst) { *dst = *src; }
 // Newton temperature at the unchanged internal energy (the composition of
 // FiniteRateChem.cu:7-63/234-256 and FieldOperation.cuh:221-296 at V = 0).
 // With idx, thread t handles cell idx[t] for t < count; otherwise cell t < n.
+// The rates of progress are parked in LDS between the source pass and the
+// Jacobian pass (d_chem_rates / d_epi_matrix), one column per thread: no
+// thread reads another's column, so the kernel has no barrier.
+constexpr int kChemTB = 128;
+constexpr int kChemMaxR = 24;  // reactions the LDS park holds
+
 template <int NS>
-__global__ void __launch_bounds__(128) k_chem_batch(const DMech *__restrict__ m, int64_t n, int64_t count,
-                                                    const int *__restrict__ idx, const double *__restrict__ rho_a,
-                                                    const double *__restrict__ T_a, const double *__restrict__ Y_a,
-                                                    double dt, double *__restrict__ dY, double *__restrict__ dT) {
-  const int64_t t = (int64_t)blockIdx.x * 128 + threadIdx.x;
+__global__ void __launch_bounds__(kChemTB) k_chem_batch(const DMech *__restrict__ m, int64_t n, int64_t count,
+                                                        const int *__restrict__ idx, const double *__restrict__ rho_a,
+                                                        const double *__restrict__ T_a, const double *__restrict__ Y_a,
+                                                        double dt, double *__restrict__ dY, double *__restrict__ dT) {
+  __shared__ double sq[2 * kChemMaxR][kChemTB];
+  const int64_t t = (int64_t)blockIdx.x * kChemTB + threadIdx.x;
   if (t >= count) return;
   const int64_t c = idx ? (int64_t)idx[t] : t;
   const double rho = rho_a[c], T = T_a[c];
@@ -760,11 +774,8 @@ __global__ void __launch_bounds__(128) k_chem_batch(const DMech *__restrict__ m,
   for (int l = 0; l < NS; ++l) rhoE += h[l] * Y[l];
   rhoE *= rho;
   rhoE -= p_old;
-  d_chem_source<NS>(m, rho, T, Y, om, J);
-#pragma unroll
-  for (int a = 0; a < NS; ++a)
-#pragma unroll
-    for (int b = 0; b < NS; ++b) J[a * NS + b] = (a == b ? 1.0 : 0.0) - dt * J[a * NS + b];
+  d_chem_rates<NS>(m, rho, T, Y, om, &sq[0][threadIdx.x], kChemTB);
+  d_epi_matrix<NS>(m, rho, Y, dt, &sq[0][threadIdx.x], kChemTB, J);
   d_lu_solve<NS>(J, om);
   const double ri = 1.0 / rho;
   double Yn[NS];
@@ -901,7 +912,7 @@ hipError_t launch_chem_scatter(const ZoneView &z, const DParams &P, const DMech 
 hipError_t launch_source(const ZoneView &z, const DParams &P, const DMech *m, bool accumulate, hipStream_t s) {
   const int64_t n = (int64_t)z.mx * z.my * z.mz;
   if (P.ns != 9) return hipErrorNotSupported;
-  hipLaunchKernelGGL(k_source<9>, dim3(nblk(n, kTB)), dim3(kTB), 0, s, z, P, m, accumulate ? 1 : 0);
+  hipLaunchKernelGGL(k_source<9>, dim3(nblk(n, kSrcTB)), dim3(kSrcTB), 0, s, z, P, m, accumulate ? 1 : 0);
   return hipGetLastError();
 }
 
@@ -1011,8 +1022,8 @@ hipError_t launch_copy_dt(const double *src, double *dst, hipStream_t s) {
 hipError_t launch_chem_batch(const DMech *m, int64_t n, const double *rho, const double *T, const double *Y, double dt,
                              double *dY, double *dT, hipStream_t s) {
   if (n <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_chem_batch<9>, dim3(nblk(n, 128)), dim3(128), 0, s, m, n, n, (const int *)nullptr, rho, T, Y,
-                     dt, dY, dT);
+  hipLaunchKernelGGL(k_chem_batch<9>, dim3(nblk(n, kChemTB)), dim3(kChemTB), 0, s, m, n, n, (const int *)nullptr, rho,
+                     T, Y, dt, dY, dT);
   return hipGetLastError();
 }
 
@@ -1028,8 +1039,8 @@ hipError_t launch_chem_batch_subset(const DMech *m, int64_t n, int64_t count, co
                                     hipStream_t s) {
   if (count <= 0) return hipSuccess;
   if (count > n) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(k_chem_batch<9>, dim3(nblk(count, 128)), dim3(128), 0, s, m, n, count, idx, rho, T, Y, dt, dY,
-                     dT);
+  hipLaunchKernelGGL(k_chem_batch<9>, dim3(nblk(count, kChemTB)), dim3(kChemTB), 0, s, m, n, count, idx, rho, T, Y,
+                     dt, dY, dT);
   return hipGetLastError();
 }
 
