@@ -9,12 +9,13 @@
 // InviscidScheme.cu:10-46/160-205, AWENO.cu:213-264, SchemeSelector.cuh:49-157).
 //
 // Thread mapping (wave64):
-//  * i-sweep: a wavefront owns 64 consecutive faces (the 63 cells between
-//    them); the left face of a cell comes from the neighbouring lane through
-//    a cross-lane shuffle.  Loads along i are coalesced.
-//  * j- and k-sweeps: lanes run along i (coalesced); the 8 wavefronts of a
-//    workgroup own 8 consecutive faces along the sweep direction and pass
-//    face fluxes to the next row through LDS (7 cells per 8 faces).
+//  * i-sweep: a wavefront owns 64 consecutive faces of the row-after-row
+//    face sequence of a k-plane (63 new cells); the left face of a cell
+//    comes from the neighbouring lane through a cross-lane shuffle.  Loads
+//    along i are coalesced.
+//  * j- and k-sweeps: lanes run along i (coalesced); a workgroup marches
+//    along a line segment, its wavefronts computing consecutive faces and
+//    passing them to the next row through LDS.
 //
 // The G form of the CDS term: with node fluxes f, the reference's
 //   c1 (f[+3]-f[-3]) + c2 (f[+2]-f[-2]) + c3 (f[+1]-f[-1])
@@ -22,6 +23,9 @@
 //   G(i+1/2) = c1 (f[-2]+f[+3]) + (c1+c2) (f[-1]+f[+2]) + (c1+c2+c3) (f[0]+f[+1]).
 // The reference applies it to cells [0, n-2*ngg) only (AWENO.cu:234-258);
 // that range is kept.
+#include <algorithm>
+#include <climits>
+
 #include "device_common.h"
 #include "flux_kernels.h"
 #include "sst_device.h"
@@ -103,9 +107,29 @@ __device__ inline void recon_pv(const ZoneView &z, const DParams &P, const DMech
   }
 }
 
+// ------------------------------------------------------ reciprocals
+// 1/x for finite, normal x: the hardware estimate refined by two
+// Newton-Raphson steps r <- r + r (1 - x r), each squaring the relative
+// error, so the result is within an ulp of 1/x.  A quotient a * rcp(b)
+// then differs from a / b by rounding only; shared denominators are
+// inverted once.
+__device__ inline double rcp(double x) {
+  double r = __builtin_amdgcn_rcp(x);
+  double e = fma(-x, r, 1.0);
+  r = fma(r, e, r);
+  e = fma(-x, r, 1.0);
+  return fma(r, e, r);
+}
+__device__ inline float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+
 // ------------------------------------------------------ AWENO (air)
 // AWENO.cu:151-206 WENO5 on one characteristic row; cvb[s][0..5] =
 // (rho, rho u, rho v, rho w, rho E, p) at stencil points s = 0..5 (i-2..i+3).
+// The nonlinear weights a_k = c_k (1 + tau / (eps + b_k)^2) of each side
+// share one reciprocal of the product of their three denominators
+// (each denominator lies in [1e-12, 1e14] for characteristic variables in
+// density units, far from the double range), and the two normalisations
+// share one reciprocal of the product of the weight sums.
 template <typename R>
 __device__ inline void weno5_row(const R *L, const R (*cvb)[6], R &vm, R &vp) {
   R v[6];
@@ -118,32 +142,32 @@ __device__ inline void weno5_row(const R *L, const R (*cvb)[6], R &vm, R &vp) {
   }
   const R t12 = R(13.0 / 12), eps = R(1e-6);
   const R o16 = R(1.0 / 16), f8 = R(5.0 / 8), f16 = R(5.0 / 16);
-  R v0 = R(0.125) * (R(3) * v[0] - R(10) * v[1] + R(15) * v[2]);
-  R v1 = R(0.125) * (-v[1] + R(6) * v[2] + R(3) * v[3]);
-  R v2 = R(0.125) * (R(3) * v[2] + R(6) * v[3] - v[4]);
-  R b0 = t12 * (v[0] + v[2] - R(2) * v[1]) * (v[0] + v[2] - R(2) * v[1]) +
-         R(0.25) * (v[0] + R(3) * v[2] - R(4) * v[1]) * (v[0] + R(3) * v[2] - R(4) * v[1]);
-  R b1 = t12 * (v[1] + v[3] - R(2) * v[2]) * (v[1] + v[3] - R(2) * v[2]) + R(0.25) * (v[3] - v[1]) * (v[3] - v[1]);
-  R b2 = t12 * (v[2] + v[4] - R(2) * v[3]) * (v[2] + v[4] - R(2) * v[3]) +
-         R(0.25) * (R(3) * v[2] + v[4] - R(4) * v[3]) * (R(3) * v[2] + v[4] - R(4) * v[3]);
-  R tau = (b0 - b2) * (b0 - b2);
-  R a0 = o16 + o16 * tau / ((eps + b0) * (eps + b0));
-  R a1 = f8 + f8 * tau / ((eps + b1) * (eps + b1));
-  R a2 = f16 + f16 * tau / ((eps + b2) * (eps + b2));
-  vm = (a0 * v0 + a1 * v1 + a2 * v2) / (a0 + a1 + a2);
-  v0 = v1;
-  v1 = v2;
-  v2 = R(0.125) * (R(15) * v[3] - R(10) * v[4] + R(3) * v[5]);
-  b0 = t12 * (v[1] + v[3] - R(2) * v[2]) * (v[1] + v[3] - R(2) * v[2]) +
-       R(0.25) * (v[1] + R(3) * v[3] - R(4) * v[2]) * (v[1] + R(3) * v[3] - R(4) * v[2]);
-  b1 = t12 * (v[2] + v[4] - R(2) * v[3]) * (v[2] + v[4] - R(2) * v[3]) + R(0.25) * (v[4] - v[2]) * (v[4] - v[2]);
-  b2 = t12 * (v[3] + v[5] - R(2) * v[4]) * (v[3] + v[5] - R(2) * v[4]) +
-       R(0.25) * (R(3) * v[3] + v[5] - R(4) * v[4]) * (R(3) * v[3] + v[5] - R(4) * v[4]);
-  tau = (b0 - b2) * (b0 - b2);
-  a0 = f16 + f16 * tau / ((eps + b0) * (eps + b0));
-  a1 = f8 + f8 * tau / ((eps + b1) * (eps + b1));
-  a2 = o16 + o16 * tau / ((eps + b2) * (eps + b2));
-  vp = (a0 * v0 + a1 * v1 + a2 * v2) / (a0 + a1 + a2);
+  // left-biased candidates and smoothness indicators
+  const R u0 = R(0.125) * (R(3) * v[0] - R(10) * v[1] + R(15) * v[2]);
+  const R u1 = R(0.125) * (-v[1] + R(6) * v[2] + R(3) * v[3]);
+  const R u2 = R(0.125) * (R(3) * v[2] + R(6) * v[3] - v[4]);
+  const R bl0 = t12 * (v[0] + v[2] - R(2) * v[1]) * (v[0] + v[2] - R(2) * v[1]) +
+                R(0.25) * (v[0] + R(3) * v[2] - R(4) * v[1]) * (v[0] + R(3) * v[2] - R(4) * v[1]);
+  const R bl1 = t12 * (v[1] + v[3] - R(2) * v[2]) * (v[1] + v[3] - R(2) * v[2]) + R(0.25) * (v[3] - v[1]) * (v[3] - v[1]);
+  const R bl2 = t12 * (v[2] + v[4] - R(2) * v[3]) * (v[2] + v[4] - R(2) * v[3]) +
+                R(0.25) * (R(3) * v[2] + v[4] - R(4) * v[3]) * (R(3) * v[2] + v[4] - R(4) * v[3]);
+  // right-biased: candidates u1, u2, u3
+  const R u3 = R(0.125) * (R(15) * v[3] - R(10) * v[4] + R(3) * v[5]);
+  const R br0 = t12 * (v[1] + v[3] - R(2) * v[2]) * (v[1] + v[3] - R(2) * v[2]) +
+                R(0.25) * (v[1] + R(3) * v[3] - R(4) * v[2]) * (v[1] + R(3) * v[3] - R(4) * v[2]);
+  const R br1 = t12 * (v[2] + v[4] - R(2) * v[3]) * (v[2] + v[4] - R(2) * v[3]) + R(0.25) * (v[4] - v[2]) * (v[4] - v[2]);
+  const R br2 = t12 * (v[3] + v[5] - R(2) * v[4]) * (v[3] + v[5] - R(2) * v[4]) +
+                R(0.25) * (R(3) * v[3] + v[5] - R(4) * v[4]) * (R(3) * v[3] + v[5] - R(4) * v[4]);
+  const R taul = (bl0 - bl2) * (bl0 - bl2), taur = (br0 - br2) * (br0 - br2);
+  const R dl0 = (eps + bl0) * (eps + bl0), dl1 = (eps + bl1) * (eps + bl1), dl2 = (eps + bl2) * (eps + bl2);
+  const R dr0 = (eps + br0) * (eps + br0), dr1 = (eps + br1) * (eps + br1), dr2 = (eps + br2) * (eps + br2);
+  const R il = taul * rcp(dl0 * dl1 * dl2), ir = taur * rcp(dr0 * dr1 * dr2);
+  const R al0 = o16 + o16 * (il * (dl1 * dl2)), al1 = f8 + f8 * (il * (dl0 * dl2)), al2 = f16 + f16 * (il * (dl0 * dl1));
+  const R ar0 = f16 + f16 * (ir * (dr1 * dr2)), ar1 = f8 + f8 * (ir * (dr0 * dr2)), ar2 = o16 + o16 * (ir * (dr0 * dr1));
+  const R sl = al0 + al1 + al2, sr = ar0 + ar1 + ar2;
+  const R is = rcp(sl * sr);
+  vm = (al0 * u0 + al1 * u1 + al2 * u2) * (sr * is);
+  vp = (ar0 * u1 + ar1 * u2 + ar2 * u3) * (sl * is);
 }
 
 // AWENO.cu:17-149 AWENO_interpolation<Air>.  mk0/mk1: metric row of the
@@ -152,53 +176,57 @@ template <typename R>
 __device__ inline void aweno_air(const R (*cvb)[6], const R *mk0, const R *mk1, R *pl, R *pr) {
   const R *cl = cvb[2], *cr = cvb[3];
   const R sl = sqrt(cl[0]), sr = sqrt(cr[0]);
-  const R rlc = sl / (sl + sr), rrc = sr / (sl + sr);
-  const R um = rlc * cl[1] / cl[0] + rrc * cr[1] / cr[0];
-  const R vm = rlc * cl[2] / cl[0] + rrc * cr[2] / cr[0];
-  const R wm = rlc * cl[3] / cl[0] + rrc * cr[3] / cr[0];
+  const R isum = rcp(sl + sr), icl = rcp(cl[0]), icr = rcp(cr[0]);
+  const R rlc = sl * isum, rrc = sr * isum;
+  const R um = rlc * cl[1] * icl + rrc * cr[1] * icr;
+  const R vm = rlc * cl[2] * icl + rrc * cr[2] * icr;
+  const R wm = rlc * cl[3] * icl + rrc * cr[3] * icr;
   const R ekm = R(0.5) * (um * um + vm * vm + wm * wm);
   const R gm1 = R(kGam - 1);
-  const R hl = (cl[4] + cl[5]) / cl[0], hr = (cr[4] + cr[5]) / cr[0];
+  const R hl = (cl[4] + cl[5]) * icl, hr = (cr[4] + cr[5]) * icr;
   const R hm = rlc * hl + rrc * hr;
   const R cm2 = gm1 * (hm - ekm);
   const R cm = sqrt(cm2);
+  const R ic2 = rcp(cm2), ic = cm * ic2, hc2 = R(0.5) * ic2;
   R kx = R(0.5) * (mk0[0] + mk1[0]), ky = R(0.5) * (mk0[1] + mk1[1]), kz = R(0.5) * (mk0[2] + mk1[2]);
-  const R gK = sqrt(kx * kx + ky * ky + kz * kz);
-  kx /= gK;
-  ky /= gK;
-  kz /= gK;
+  const R igK = rcp(sqrt(kx * kx + ky * ky + kz * kz));
+  kx *= igK;
+  ky *= igK;
+  kz *= igK;
   const R Uk = kx * um + ky * vm + kz * wm;
   const R alpha = gm1 * ekm;
+  const R ac = R(1) - alpha * ic2, g2 = gm1 * ic2;
   R L[25];
-  L[0] = (alpha + Uk * cm) / cm2 * R(0.5);
-  L[1] = -(gm1 * um + kx * cm) / cm2 * R(0.5);
-  L[2] = -(gm1 * vm + ky * cm) / cm2 * R(0.5);
-  L[3] = -(gm1 * wm + kz * cm) / cm2 * R(0.5);
-  L[4] = gm1 / cm2 * R(0.5);
-  L[5] = kx * (R(1) - alpha / cm2) - (kz * vm - ky * wm) / cm;
-  L[6] = kx * gm1 * um / cm2;
-  L[7] = (kx * gm1 * vm + kz * cm) / cm2;
-  L[8] = (kx * gm1 * wm - ky * cm) / cm2;
-  L[9] = -kx * gm1 / cm2;
-  L[10] = ky * (R(1) - alpha / cm2) - (kx * wm - kz * um) / cm;
-  L[11] = (ky * gm1 * um - kz * cm) / cm2;
-  L[12] = ky * gm1 * vm / cm2;
-  L[13] = (ky * gm1 * wm + kx * cm) / cm2;
-  L[14] = -ky * gm1 / cm2;
-  L[15] = kz * (R(1) - alpha / cm2) - (ky * um - kx * vm) / cm;
-  L[16] = (kz * gm1 * um + ky * cm) / cm2;
-  L[17] = (kz * gm1 * vm - kx * cm) / cm2;
-  L[18] = kz * gm1 * wm / cm2;
-  L[19] = -kz * gm1 / cm2;
-  L[20] = (alpha - Uk * cm) / cm2 * R(0.5);
-  L[21] = -(gm1 * um - kx * cm) / cm2 * R(0.5);
-  L[22] = -(gm1 * vm - ky * cm) / cm2 * R(0.5);
-  L[23] = -(gm1 * wm - kz * cm) / cm2 * R(0.5);
-  L[24] = gm1 / cm2 * R(0.5);
+  L[0] = (alpha + Uk * cm) * hc2;
+  L[1] = -(gm1 * um + kx * cm) * hc2;
+  L[2] = -(gm1 * vm + ky * cm) * hc2;
+  L[3] = -(gm1 * wm + kz * cm) * hc2;
+  L[4] = gm1 * hc2;
+  L[5] = kx * ac - (kz * vm - ky * wm) * ic;
+  L[6] = kx * g2 * um;
+  L[7] = (kx * gm1 * vm + kz * cm) * ic2;
+  L[8] = (kx * gm1 * wm - ky * cm) * ic2;
+  L[9] = -kx * g2;
+  L[10] = ky * ac - (kx * wm - kz * um) * ic;
+  L[11] = (ky * gm1 * um - kz * cm) * ic2;
+  L[12] = ky * g2 * vm;
+  L[13] = (ky * gm1 * wm + kx * cm) * ic2;
+  L[14] = -ky * g2;
+  L[15] = kz * ac - (ky * um - kx * vm) * ic;
+  L[16] = (kz * gm1 * um + ky * cm) * ic2;
+  L[17] = (kz * gm1 * vm - kx * cm) * ic2;
+  L[18] = kz * g2 * wm;
+  L[19] = -kz * g2;
+  L[20] = (alpha - Uk * cm) * hc2;
+  L[21] = -(gm1 * um - kx * cm) * hc2;
+  L[22] = -(gm1 * vm - ky * cm) * hc2;
+  L[23] = -(gm1 * wm - kz * cm) * hc2;
+  L[24] = gm1 * hc2;
   R vmn[5], vpl[5];
 #pragma unroll
   for (int r = 0; r < 5; ++r) weno5_row(&L[r * 5], cvb, vmn[r], vpl[r]);
   // right eigenvectors (AWENO.cu:97-122), applied row by row
+  const R ag = alpha * R(1.0 / (kGam - 1));
   const R Rm[25] = {R(1),
                     kx,
                     ky,
@@ -220,9 +248,9 @@ __device__ inline void aweno_air(const R (*cvb)[6], const R *mk0, const R *mk1, 
                     kz * wm,
                     wm + kz * cm,
                     hm - Uk * cm,
-                    kx * alpha / gm1 + (kz * vm - ky * wm) * cm,
-                    ky * alpha / gm1 + (kx * wm - kz * um) * cm,
-                    kz * alpha / gm1 + (ky * um - kx * vm) * cm,
+                    kx * ag + (kz * vm - ky * wm) * cm,
+                    ky * ag + (kx * wm - kz * um) * cm,
+                    kz * ag + (ky * um - kx * vm) * cm,
                     hm + Uk * cm};
   R ql[5], qr[5];
 #pragma unroll
@@ -236,17 +264,18 @@ __device__ inline void aweno_air(const R (*cvb)[6], const R *mk0, const R *mk1, 
     ql[a] = sa;
     qr[a] = sb;
   }
+  const R iql = rcp(ql[0]), iqr = rcp(qr[0]);
   pl[0] = ql[0];
-  pl[1] = ql[1] / ql[0];
-  pl[2] = ql[2] / ql[0];
-  pl[3] = ql[3] / ql[0];
+  pl[1] = ql[1] * iql;
+  pl[2] = ql[2] * iql;
+  pl[3] = ql[3] * iql;
   pl[4] = gm1 * (ql[4] - R(0.5) * (pl[1] * pl[1] + pl[2] * pl[2] + pl[3] * pl[3]) * ql[0]);
   pl[5] = ql[4];
   pl[6] = R(kGam);
   pr[0] = qr[0];
-  pr[1] = qr[1] / qr[0];
-  pr[2] = qr[2] / qr[0];
-  pr[3] = qr[3] / qr[0];
+  pr[1] = qr[1] * iqr;
+  pr[2] = qr[2] * iqr;
+  pr[3] = qr[3] * iqr;
   pr[4] = gm1 * (qr[4] - R(0.5) * (pr[1] * pr[1] + pr[2] * pr[2] + pr[3] * pr[3]) * qr[0]);
   pr[5] = qr[4];
   pr[6] = R(kGam);
@@ -262,10 +291,12 @@ __device__ inline void ausm_plus(const R *pl, const R *pr, const R *mk0, const R
   const R k2 = R(0.5) * (j0 * mk0[1] + j1 * mk1[1]);
   const R k3 = R(0.5) * (j0 * mk0[2] + j1 * mk1[2]);
   const R gk = sqrt(k1 * k1 + k2 * k2 + k3 * k3);
-  const R ul = (k1 * pl[1] + k2 * pl[2] + k3 * pl[3]) / gk;
-  const R ur = (k1 * pr[1] + k2 * pr[2] + k3 * pr[3]) / gk;
-  const R c = R(0.5) * (sqrt(pl[NV + 1] * pl[4] / pl[0]) + sqrt(pr[NV + 1] * pr[4] / pr[0]));
-  const R ml = ul / c, mr = ur / c;
+  const R igk = rcp(gk);
+  const R ul = (k1 * pl[1] + k2 * pl[2] + k3 * pl[3]) * igk;
+  const R ur = (k1 * pr[1] + k2 * pr[2] + k3 * pr[3]) * igk;
+  const R c = R(0.5) * (sqrt(pl[NV + 1] * pl[4] * rcp(pl[0])) + sqrt(pr[NV + 1] * pr[4] * rcp(pr[0])));
+  const R ic = rcp(c);
+  const R ml = ul * ic, mr = ur * ic;
   const R alpha = R(3 / 16.0);
   R mlp, mrn, plp, prn;
   if (fabs(ml) > R(1)) {
@@ -291,14 +322,21 @@ __device__ inline void ausm_plus(const R *pl, const R *pr, const R *mk0, const R
   const R mpos = R(0.5) * (mh + fabs(mh)), mneg = R(0.5) * (mh - fabs(mh));
   const R mass = c * (pl[0] * mpos + pr[0] * mneg);
   const R coeff = mass * gk;
-  const R *s = mass >= R(0) ? pl : pr;
+  // upwind state, selected element by element (a selected pointer would
+  // force both state arrays out of registers)
+  const bool up = mass >= R(0);
   f[0] = coeff;
-  f[1] = coeff * s[1] + pc * k1;
-  f[2] = coeff * s[2] + pc * k2;
-  f[3] = coeff * s[3] + pc * k3;
-  f[4] = coeff * (s[NV] + s[4]) / s[0];
+  f[1] = coeff * (up ? pl[1] : pr[1]) + pc * k1;
+  f[2] = coeff * (up ? pl[2] : pr[2]) + pc * k2;
+  f[3] = coeff * (up ? pl[3] : pr[3]) + pc * k3;
+  f[4] = coeff * ((up ? pl[NV] : pr[NV]) + (up ? pl[4] : pr[4])) / (up ? pl[0] : pr[0]);
+  if (up) {
 #pragma unroll
-  for (int l = 5; l < NV; ++l) f[l] = coeff * s[l];
+    for (int l = 5; l < NV; ++l) f[l] = coeff * pl[l];
+  } else {
+#pragma unroll
+    for (int l = 5; l < NV; ++l) f[l] = coeff * pr[l];
+  }
 }
 
 // HLLC.cu:10-143
@@ -307,9 +345,10 @@ __device__ inline void hllc(const DMech *__restrict__ m, const R *pl, const R *p
                             R j1, R *f) {
   constexpr int NV = 5 + NS + NT;
   const R sl0 = sqrt(pl[0]), sr0 = sqrt(pr[0]);
-  const R rlc = sl0 / (sl0 + sr0), rrc = sr0 / (sl0 + sr0);
+  const R isum = rcp(sl0 + sr0), ipl = rcp(pl[0]), ipr = rcp(pr[0]);
+  const R rlc = sl0 * isum, rrc = sr0 * isum;
   const R ut = pl[1] * rlc + pr[1] * rrc, vt = pl[2] * rlc + pr[2] * rrc, wt = pl[3] * rlc + pr[3] * rrc;
-  const R hl = (pl[NV] + pl[4]) / pl[0], hr = (pr[NV] + pr[4]) / pr[0];
+  const R hl = (pl[NV] + pl[4]) * ipl, hr = (pr[NV] + pr[4]) * ipr;
   const R ht = hl * rlc + hr * rrc;
   const R ekt = R(0.5) * (ut * ut + vt * vt + wt * wt);
   R ct;
@@ -323,7 +362,7 @@ __device__ inline void hllc(const DMech *__restrict__ m, const R *pl, const R *p
       svm[l] = (double)(rlc * pl[l + 5] + rrc * pr[l + 5]);
       mwi += svm[l] / m->mw[l];
     }
-    const double t = (double)(rlc * (pl[4] / pl[0]) + rrc * (pr[4] / pr[0])) / (kRuD * mwi);
+    const double t = (double)(rlc * (pl[4] * ipl) + rrc * (pr[4] * ipr)) / (kRuD * mwi);
     double h[NS], c[NS], cpp = 0, cvv = 0;
     d_enthalpy_cp<NS>(t, m, h, c);
 #pragma unroll
@@ -335,11 +374,13 @@ __device__ inline void hllc(const DMech *__restrict__ m, const R *pl, const R *p
   }
   const R kx = R(0.5) * (mk0[0] + mk1[0]), ky = R(0.5) * (mk0[1] + mk1[1]), kz = R(0.5) * (mk0[2] + mk1[2]);
   const R gK = sqrt(kx * kx + ky * ky + kz * kz);
-  const R Utb = (kx * ut + ky * vt + kz * wt) / gK;
+  const R igK = rcp(gK);
+  const R Utb = (kx * ut + ky * vt + kz * wt) * igK;
   const R ja = R(0.5) * (j0 + j1);
   const R Ul = kx * pl[1] + ky * pl[2] + kz * pl[3];
-  const R cl = sqrt(pl[NV + 1] * pl[4] / pl[0]);
-  const R sl = fmin(Ul / gK - cl, Utb - ct);
+  const R Ulg = Ul * igK;
+  const R cl = sqrt(pl[NV + 1] * pl[4] * ipl);
+  const R sl = fmin(Ulg - cl, Utb - ct);
   if (sl >= R(0)) {
     const R rU = pl[0] * Ul;
     f[0] = ja * rU;
@@ -352,8 +393,9 @@ __device__ inline void hllc(const DMech *__restrict__ m, const R *pl, const R *p
     return;
   }
   const R Ur = kx * pr[1] + ky * pr[2] + kz * pr[3];
-  const R cr = sqrt(pr[NV + 1] * pr[4] / pr[0]);
-  const R sr = fmax(Ur / gK + cr, Utb + ct);
+  const R Urg = Ur * igK;
+  const R cr = sqrt(pr[NV + 1] * pr[4] * ipr);
+  const R sr = fmax(Urg + cr, Utb + ct);
   if (sr < R(0)) {
     const R rU = pr[0] * Ur;
     f[0] = ja * rU;
@@ -365,23 +407,30 @@ __device__ inline void hllc(const DMech *__restrict__ m, const R *pl, const R *p
     for (int l = 5; l < NV; ++l) f[l] = ja * rU * pr[l];
     return;
   }
-  const R sm = ((pr[0] * Ur * (sr - Ur / gK) - pl[0] * Ul * (sl - Ul / gK)) / gK + pl[4] - pr[4]) /
-               (pr[0] * (sr - Ur / gK) - pl[0] * (sl - Ul / gK));
-  const R pm = pl[0] * (sl - Ul / gK) * (sm - Ul / gK) + pl[4];
+  const R sm = ((pr[0] * Ur * (sr - Urg) - pl[0] * Ul * (sl - Ulg)) * igK + pl[4] - pr[4]) /
+               (pr[0] * (sr - Urg) - pl[0] * (sl - Ulg));
+  const R pm = pl[0] * (sl - Ulg) * (sm - Ulg) + pl[4];
+  // star state on the side of the contact, selected element by element
   const bool left = sm >= R(0);
-  const R *q = left ? pl : pr;
   const R s = left ? sl : sr;
   const R Uq = left ? Ul : Ur;
-  const R pC = R(1) / (s - sm);
-  const R Q = ja * pC * sm * (s * gK - Uq) * q[0];
-  const R dP = (s * pm - sm * q[4]) * pC * ja;
+  const R iq = left ? ipl : ipr;
+  const R q0 = left ? pl[0] : pr[0], q4 = left ? pl[4] : pr[4];
+  const R pC = rcp(s - sm);
+  const R Q = ja * pC * sm * (s * gK - Uq) * q0;
+  const R dP = (s * pm - sm * q4) * pC * ja;
   f[0] = Q;
-  f[1] = Q * q[1] + dP * kx;
-  f[2] = Q * q[2] + dP * ky;
-  f[3] = Q * q[3] + dP * kz;
-  f[4] = Q * q[NV] / q[0] + pC * ja * (s * pm * sm * gK - sm * q[4] * Uq);
+  f[1] = Q * (left ? pl[1] : pr[1]) + dP * kx;
+  f[2] = Q * (left ? pl[2] : pr[2]) + dP * ky;
+  f[3] = Q * (left ? pl[3] : pr[3]) + dP * kz;
+  f[4] = Q * (left ? pl[NV] : pr[NV]) * iq + pC * ja * (s * pm * sm * gK - sm * q4 * Uq);
+  if (left) {
 #pragma unroll
-  for (int l = 5; l < NV; ++l) f[l] = Q * q[l];
+    for (int l = 5; l < NV; ++l) f[l] = Q * pl[l];
+  } else {
+#pragma unroll
+    for (int l = 5; l < NV; ++l) f[l] = Q * pr[l];
+  }
 }
 
 // Roe.cu:60-87: entropy-fix delta of one point, eps (|U| + |V| (+|W|) +
@@ -414,11 +463,9 @@ __device__ inline void roe(const DMech *__restrict__ m, const R *pl, const R *pr
   constexpr int NV = 5 + NS + NT;
 #pragma unroll
   for (int l = 0; l < NV; ++l) f[l] = R(0);
-#pragma unroll
-  for (int sd = 0; sd < 2; ++sd) {
-    const R *q = sd ? pr : pl;
-    const R *mk = sd ? mk1 : mk0;
-    const R J = sd ? j1 : j0;
+  // half of each side's physical flux (called once per side so that no
+  // pointer to either state array is selected at run time)
+  auto half_flux = [&](const R *q, const R *mk, const R J) {
     const R Jx = J * mk[0], Jy = J * mk[1], Jz = J * mk[2];
     const R Uk = q[1] * Jx + q[2] * Jy + q[3] * Jz;
     const R co = Uk * q[0];
@@ -429,7 +476,9 @@ __device__ inline void roe(const DMech *__restrict__ m, const R *pl, const R *pr
     f[4] += R(0.5) * Uk * (q[4] + q[NV]);
 #pragma unroll
     for (int l = 5; l < NV; ++l) f[l] += R(0.5) * co * q[l];
-  }
+  };
+  half_flux(pl, mk0, j0);
+  half_flux(pr, mk1, j1);
   const R dl = sqrt(pl[0]), dr = sqrt(pr[0]), id = R(1) / (dl + dr);
   const R u = (dl * pl[1] + dr * pr[1]) * id, v = (dl * pl[2] + dr * pr[2]) * id, w = (dl * pl[3] + dr * pr[3]) * id;
   const R ek = R(0.5) * (u * u + v * v + w * w);
@@ -571,9 +620,10 @@ __device__ inline void viscous_face(const ZoneView &z, const DParams &P, const D
   R mut = R(0);
   if constexpr (NT > 0) mut = R(0.5) * ((R)z.mut[p0] + (R)z.mut[p1]);
   const R vis = mul + mut;
-  R txx = vis * (R(4) * ux[0] - R(2) * vx[1] - R(2) * wx[2]) / R(3.0);
-  R tyy = vis * (R(4) * vx[1] - R(2) * ux[0] - R(2) * wx[2]) / R(3.0);
-  R tzz = vis * (R(4) * wx[2] - R(2) * ux[0] - R(2) * vx[1]) / R(3.0);
+  const R v3 = vis * R(1.0 / 3.0);
+  R txx = v3 * (R(4) * ux[0] - R(2) * vx[1] - R(2) * wx[2]);
+  R tyy = v3 * (R(4) * vx[1] - R(2) * ux[0] - R(2) * wx[2]);
+  R tzz = v3 * (R(4) * wx[2] - R(2) * ux[0] - R(2) * vx[1]);
   const R txy = vis * (ux[1] + vx[0]), txz = vis * (ux[2] + wx[0]), tyz = vis * (vx[2] + wx[1]);
   if constexpr (NT > 0) {
     // -2/3 rho k on the normal stresses, faded out within 1e-3 of a wall
@@ -672,7 +722,61 @@ __device__ inline void viscous_face(const ZoneView &z, const DParams &P, const D
 }
 
 // ------------------------------------------------------ face assembly
-// F(face) = inviscid - viscous; G(face) = CDS flux-form term (AWENO only).
+// AWENO.cu:233-242: node flux of the CDS correction at point q (metric row
+// rb of the sweep direction), J (rho U, rho U u + kx p, .., U (rho E + p)).
+template <typename R>
+__device__ inline void node_flux(const ZoneView &z, int rb, int64_t q, R *fn) {
+  const R kx = (R)z.metric[(rb + 0) * z.sz + q], ky = (R)z.metric[(rb + 1) * z.sz + q],
+          kz = (R)z.metric[(rb + 2) * z.sz + q];
+  const R J = (R)z.jac[q];
+  const R u = (R)z.bv[1 * z.sz + q], v = (R)z.bv[2 * z.sz + q], w = (R)z.bv[3 * z.sz + q];
+  const R Uk = kx * u + ky * v + kz * w;
+  const R rU = (R)z.bv[q] * Uk;
+  const R p = (R)z.bv[4 * z.sz + q];
+  fn[0] = rU * J;
+  fn[1] = J * (rU * u + kx * p);
+  fn[2] = J * (rU * v + ky * p);
+  fn[3] = J * (rU * w + kz * p);
+  fn[4] = J * Uk * ((R)z.cv[4 * z.sz + q] + p);
+}
+
+// CDS weights of the flux form G(i+1/2) = sum_s w_s f(i-2+s), s = 0..5
+template <typename R>
+__device__ inline R cds_weight(int s) {
+  const R c1 = R(19.0 / 3840), c2 = R(-13.0 / 320), c3 = R(17.0 / 256);
+  return (s == 0 || s == 5) ? c1 : (s == 1 || s == 4) ? c1 + c2 : c1 + c2 + c3;
+}
+
+// Riemann solver on the reconstructed states plus the viscous flux:
+// F(face) = inviscid - viscous.
+template <int NS, int NT, int SCHEME, typename R>
+__device__ inline void riemann_viscous(const ZoneView &z, const DParams &P, const DMech *__restrict__ m, int dir,
+                                       int64_t p0, int64_t st, const R *pl, const R *pr, const R *mk0, const R *mk1,
+                                       R j0, R j1, R *F) {
+  constexpr int NV = 5 + NS + NT;
+  // inviscid_scheme 2 with AWENO (inviscid_type 1) runs AUSM+, as the
+  // reference's AWENO switch has no Roe case (InviscidScheme.cu:290-303)
+  switch (P.inviscid_scheme) {
+    case 1: lax_friedrichs<NS, NT, R>(pl, pr, mk0, mk1, j0, j1, F); break;
+    case 4: hllc<NS, NT, R>(m, pl, pr, mk0, mk1, j0, j1, F); break;
+    case 2:
+      if constexpr (SCHEME == 0) {
+        roe<NS, NT, R>(m, pl, pr, mk0, mk1, j0, j1, efix_point<R>(z, P, p0), efix_point<R>(z, P, p0 + st), F);
+        break;
+      }
+      [[fallthrough]];
+    default: ausm_plus<NS, NT, R>(pl, pr, mk0, mk1, j0, j1, F); break;
+  }
+  if (P.viscous) {
+    R fv[NV];
+    viscous_face<NS, NT, R>(z, P, m, dir, p0, st, fv);
+#pragma unroll
+    for (int l = 0; l < NV; ++l) F[l] -= fv[l];
+  }
+}
+
+// F(face) = inviscid - viscous; G(face) = CDS flux-form term (AWENO only),
+// every stencil value read from memory.
 template <int NS, int NT, int SCHEME, typename R>
 __device__ inline void face_flux(const ZoneView &z, const DParams &P, const DMech *__restrict__ m, int dir, int64_t p0,
                                  int64_t st, bool want_g, R *F, R *G) {
@@ -697,50 +801,20 @@ __device__ inline void face_flux(const ZoneView &z, const DParams &P, const DMec
     }
     aweno_air(cvb, mk0, mk1, pl, pr);
     if (want_g) {
-      // node fluxes (AWENO.cu:233-242) at the 6 stencil points
-      const R c1 = R(19.0 / 3840), c2 = R(-13.0 / 320), c3 = R(17.0 / 256);
-      const R wgt[6] = {c1, c1 + c2, c1 + c2 + c3, c1 + c2 + c3, c1 + c2, c1};
 #pragma unroll
       for (int l = 0; l < 5; ++l) G[l] = R(0);
 #pragma unroll
       for (int s = 0; s < 6; ++s) {
-        const int64_t q = p0 + (s - 2) * st;
-        const R kx = (R)z.metric[(rb + 0) * z.sz + q], ky = (R)z.metric[(rb + 1) * z.sz + q],
-                kz = (R)z.metric[(rb + 2) * z.sz + q];
-        const R J = (R)z.jac[q];
-        const R u = (R)z.bv[1 * z.sz + q], v = (R)z.bv[2 * z.sz + q], w = (R)z.bv[3 * z.sz + q];
-        const R Uk = kx * u + ky * v + kz * w;
-        const R rU = (R)z.bv[q] * Uk;
-        const R p = (R)z.bv[4 * z.sz + q];
-        G[0] += wgt[s] * (rU * J);
-        G[1] += wgt[s] * (J * (rU * u + kx * p));
-        G[2] += wgt[s] * (J * (rU * v + ky * p));
-        G[3] += wgt[s] * (J * (rU * w + kz * p));
-        G[4] += wgt[s] * (J * Uk * ((R)z.cv[4 * z.sz + q] + p));
+        R fn[5];
+        node_flux<R>(z, rb, p0 + (s - 2) * st, fn);
+#pragma unroll
+        for (int l = 0; l < 5; ++l) G[l] += cds_weight<R>(s) * fn[l];
       }
     }
   } else {
     recon_pv<NS, NT, R>(z, P, m, p0, st, pl, pr);
   }
-  // inviscid_scheme 2 with AWENO (inviscid_type 1) runs AUSM+, as the
-  // reference's AWENO switch has no Roe case (InviscidScheme.cu:290-303)
-  switch (P.inviscid_scheme) {
-    case 1: lax_friedrichs<NS, NT, R>(pl, pr, mk0, mk1, j0, j1, F); break;
-    case 4: hllc<NS, NT, R>(m, pl, pr, mk0, mk1, j0, j1, F); break;
-    case 2:
-      if constexpr (SCHEME == 0) {
-        roe<NS, NT, R>(m, pl, pr, mk0, mk1, j0, j1, efix_point<R>(z, P, p0), efix_point<R>(z, P, p0 + st), F);
-        break;
-      }
-      [[fallthrough]];
-    default: ausm_plus<NS, NT, R>(pl, pr, mk0, mk1, j0, j1, F); break;
-  }
-  if (P.viscous) {
-    R fv[NV];
-    viscous_face<NS, NT, R>(z, P, m, dir, p0, st, fv);
-#pragma unroll
-    for (int l = 0; l < NV; ++l) F[l] -= fv[l];
-  }
+  riemann_viscous<NS, NT, SCHEME, R>(z, P, m, dir, p0, st, pl, pr, mk0, mk1, j0, j1, F);
 }
 
 template <int NS, int NT, typename R>
@@ -756,16 +830,33 @@ __device__ inline void add_dq(const ZoneView &z, int64_t q0, bool init, const R 
   }
 }
 
-// i-direction sweep: lane l of a wavefront owns face (i0 + l - 1) + 1/2.
+// Workgroups are handed to the 8 XCDs round-robin by linear id; this maps
+// the hardware id to a logical id so that each XCD receives one contiguous
+// range of logical blocks (neighbouring lines and planes then share that
+// XCD's L2).  A bijection on [0, nb) for any nb.
+__device__ inline int xcd_logical_block(int hw, int nb) {
+  constexpr int kXcd = 8;
+  const int q = nb / kXcd, rem = nb % kXcd;
+  const int x = hw % kXcd, t = hw / kXcd;
+  return x * q + (x < rem ? x : rem) + t;
+}
+
+// i-direction sweep.  The faces of one k-plane are numbered row after row,
+// n + 1 per row (face f + 1/2 for f = -1 .. n-1), and a wavefront takes 63
+// new faces of that sequence plus the one before them, so no lane idles at
+// the end of a row; lane l's cell takes its left face from lane l - 1
+// through a cross-lane shuffle.
 template <int NS, int NT, int SCHEME, typename R>
-__global__ void __launch_bounds__(256) sweep_i(ZoneView z, DParams P, const DMech *__restrict__ m, int init) {
+__global__ void __launch_bounds__(256) sweep_i(ZoneView z, DParams P, const DMech *__restrict__ m, int init, int bpp) {
   constexpr int NV = 5 + NS + NT;
   const int lane = threadIdx.x;
-  const int i = blockIdx.x * 63 + lane - 1;
-  const int j = blockIdx.y * blockDim.y + threadIdx.y;
-  const int k = blockIdx.z;
   const int n = z.mx;
-  const bool live = j < z.my && i < n;
+  const int L = xcd_logical_block(blockIdx.x, gridDim.x);
+  const int k = L / bpp;
+  const unsigned fo = (unsigned)((L - k * bpp) * 4 + threadIdx.y) * 63u + lane;
+  const int j = (int)(fo / (unsigned)(n + 1));
+  const int i = (int)(fo - (unsigned)j * (n + 1)) - 1;
+  const bool live = j < z.my;
   const int ncds = SCHEME == 1 ? n - 2 * z.g : 0;
   R F[NV], G[5];
 #pragma unroll
@@ -778,14 +869,18 @@ __global__ void __launch_bounds__(256) sweep_i(ZoneView z, DParams P, const DMec
   for (int l = 0; l < NV; ++l) Fp[l] = __shfl_up(F[l], 1, 64);
 #pragma unroll
   for (int l = 0; l < 5; ++l) Gp[l] = __shfl_up(G[l], 1, 64);
-  if (live && lane > 0) add_dq<NS, NT, R>(z, z.at0(i, j, k), init != 0, F, Fp, i < ncds, G, Gp);
+  if (live && lane > 0 && i >= 0) add_dq<NS, NT, R>(z, z.at0(i, j, k), init != 0, F, Fp, i < ncds, G, Gp);
 }
 
-// j- and k-direction sweeps: lanes along i (coalesced), kRows wavefronts
-// per workgroup own kRows consecutive faces along the sweep direction; each
-// thread computes one face and the left face of its cell comes from the
-// previous row through LDS (one redundant face per kRows - 1 cells).
-template <int NS>
+// j- and k-direction sweeps: lanes along i (coalesced); a workgroup of kRows
+// wavefronts marches along one line segment [c0, c1) of the sweep direction
+// in batches of kRows consecutive faces, one face per thread.  A cell's left
+// face is the previous row's, read through LDS; the first row of a batch
+// reads the last face of the batch before, which that batch's last row
+// leaves in one of two carry slots (alternating, so that one barrier
+// between the flux stores and the cell updates suffices).  Each face is
+// computed once (plus one face per segment).
+template <int NS, typename R>
 struct JkRows {
   // 8 rows (512 threads, <= 256 VGPRs) for laminar air; 4 with transported
   // scalars, whose face arithmetic needs more registers than a 512-thread
@@ -793,61 +888,239 @@ struct JkRows {
   static constexpr int value = NS > 0 ? 4 : 8;
 };
 
-template <int DIR, int NS, int NT, int SCHEME, typename R>
-__global__ void __launch_bounds__(64 * JkRows<NS + NT>::value) sweep_jk(ZoneView z, DParams P,
-                                                                        const DMech *__restrict__ m, int init) {
-  constexpr int NV = 5 + NS + NT;
-  constexpr int kRows = JkRows<NS + NT>::value;
-  __shared__ R sF[kRows][NV][64];
-  __shared__ R sG[kRows][5][64];
-  const int lane = threadIdx.x, r = threadIdx.y;
-  const int i = blockIdx.x * 64 + lane;
-  const int o = blockIdx.y;  // k for DIR 1, j for DIR 2
+// line-segment decomposition shared by the j/k sweeps
+struct JkLine {
+  int i, o, c0, c1, nbatch;
+  int64_t st;
+};
+
+template <int DIR, int kRows>
+__device__ inline JkLine jk_line(const ZoneView &z, int gx, int nseg) {
+  JkLine L;
+  const int b = xcd_logical_block(blockIdx.x, gridDim.x);
+  const int bx = b % gx, rest = b / gx;
+  const int seg = rest % nseg;
+  L.o = rest / nseg;  // k for DIR 1, j for DIR 2
+  L.i = bx * 64 + threadIdx.x;
   const int n = DIR == 1 ? z.my : z.mz;
-  const int s = blockIdx.z * (kRows - 1) + r - 1;  // this thread's face is s + 1/2
-  const bool live = i < z.mx && s < n;
-  const int64_t st = DIR == 1 ? z.d2 : z.d1;
+  L.c0 = (int)((int64_t)seg * n / nseg);
+  L.c1 = (int)((int64_t)(seg + 1) * n / nseg);
+  L.nbatch = (L.c1 - L.c0 + kRows) / kRows;  // faces c0 - 1 .. c1 - 1
+  L.st = DIR == 1 ? z.d2 : z.d1;
+  return L;
+}
+
+template <int DIR, int NS, int NT, int SCHEME, typename R>
+__global__ void __launch_bounds__((64 * JkRows<NS + NT, R>::value)) sweep_jk(ZoneView z, DParams P,
+                                                                           const DMech *__restrict__ m, int init,
+                                                                           int gx, int nseg) {
+  constexpr int NV = 5 + NS + NT;
+  constexpr int NG = SCHEME == 1 ? 5 : 1;  // the CDS flux G exists for AWENO only
+  constexpr int kRows = JkRows<NS + NT, R>::value;
+  __shared__ R sF[kRows + 2][NV][64];  // slots 0, 1: carry; 2 + r: row r
+  __shared__ R sG[kRows + 2][NG][64];
+  const int lane = threadIdx.x, r = threadIdx.y;
+  const JkLine L = jk_line<DIR, kRows>(z, gx, nseg);
+  const int n = DIR == 1 ? z.my : z.mz;
   const int ncds = SCHEME == 1 ? n - 2 * z.g : 0;
-  R F[NV], G[5];
+  for (int b = 0; b < L.nbatch; ++b) {
+    const int f = L.c0 - 1 + b * kRows + r;  // this thread's face is f + 1/2
+    const bool live = L.i < z.mx && f < L.c1;
+    R F[NV], G[NG];
 #pragma unroll
-  for (int l = 0; l < NV; ++l) F[l] = R(0);
+    for (int l = 0; l < NV; ++l) F[l] = R(0);
 #pragma unroll
-  for (int l = 0; l < 5; ++l) G[l] = R(0);
-  if (live) {
-    const int64_t p0 = DIR == 1 ? z.at(i, s, o) : z.at(i, o, s);
-    face_flux<NS, NT, SCHEME, R>(z, P, m, DIR, p0, st, s <= ncds - 1, F, G);
+    for (int l = 0; l < NG; ++l) G[l] = R(0);
+    if (live) {
+      const int64_t p0 = DIR == 1 ? z.at(L.i, f, L.o) : z.at(L.i, L.o, f);
+      face_flux<NS, NT, SCHEME, R>(z, P, m, DIR, p0, L.st, f <= ncds - 1, F, G);
+    }
+#pragma unroll
+    for (int l = 0; l < NV; ++l) sF[r + 2][l][lane] = F[l];
+#pragma unroll
+    for (int l = 0; l < NG; ++l) sG[r + 2][l][lane] = G[l];
+    if (r == kRows - 1) {
+      const int cs = (b + 1) & 1;
+#pragma unroll
+      for (int l = 0; l < NV; ++l) sF[cs][l][lane] = F[l];
+#pragma unroll
+      for (int l = 0; l < NG; ++l) sG[cs][l][lane] = G[l];
+    }
+    __syncthreads();
+    if (live && f >= L.c0) {
+      const int ls = r > 0 ? r + 1 : (b & 1);
+      R Fp[NV], Gp[NG];
+#pragma unroll
+      for (int l = 0; l < NV; ++l) Fp[l] = sF[ls][l][lane];
+#pragma unroll
+      for (int l = 0; l < NG; ++l) Gp[l] = sG[ls][l][lane];
+      const int64_t q = DIR == 1 ? z.at0(L.i, f, L.o) : z.at0(L.i, L.o, f);
+      add_dq<NS, NT, R>(z, q, init != 0, F, Fp, f < ncds, G, Gp);
+    }
+    __syncthreads();  // the next batch's flux stores overwrite the slots read here
   }
+}
+
+// AWENO j/k sweep (air): the rows of the line pass through an LDS ring of
+// kRows + 5 rows, each row staged once by one wavefront -- its conservative
+// variables and pressure (the characteristic stencil) and its CDS node flux
+// -- so a face reads its six stencil rows from LDS instead of six wavefronts
+// loading every row from memory.  Face fluxes pass to the next row as in
+// sweep_jk.
+template <int DIR, typename R>
+__global__ void __launch_bounds__(512) sweep_jk_aweno(ZoneView z, DParams P, const DMech *__restrict__ m, int init,
+                                                      int gx, int nseg) {
+  constexpr int kRows = 8, kRing = kRows + 5;
+  __shared__ R sQ[kRing][6][64];     // cv 0..4, p
+  __shared__ R sN[kRing][5][64];     // node flux
+  __shared__ R sF[kRows + 2][5][64];  // slots 0, 1: carry; 2 + r: row r
+  __shared__ R sG[kRows + 2][5][64];
+  const int lane = threadIdx.x, r = threadIdx.y;
+  const JkLine L = jk_line<DIR, kRows>(z, gx, nseg);
+  const int n = DIR == 1 ? z.my : z.mz;
+  const int ncds = n - 2 * z.g;
+  const int rb = DIR * 3;
+  const bool col = L.i < z.mx;
+  // row q of the line (q >= -3) into ring slot (q + 3) % kRing
+  auto stage = [&](int q) {
+    const int64_t p = DIR == 1 ? z.at(L.i, q, L.o) : z.at(L.i, L.o, q);
+    const int sl = (q + 3) % kRing;
 #pragma unroll
-  for (int l = 0; l < NV; ++l) sF[r][l][lane] = F[l];
+    for (int c = 0; c < 5; ++c) sQ[sl][c][lane] = (R)z.cv[(int64_t)c * z.sz + p];
+    sQ[sl][5][lane] = (R)z.bv[4 * z.sz + p];
+    R fn[5];
+    node_flux<R>(z, rb, p, fn);
 #pragma unroll
-  for (int l = 0; l < 5; ++l) sG[r][l][lane] = G[l];
-  __syncthreads();
-  if (live && r > 0) {
-    R Fp[NV], Gp[5];
+    for (int c = 0; c < 5; ++c) sN[sl][c][lane] = fn[c];
+  };
+  // the five rows before the first batch's leading rows: c0 - 3 .. c0 + 1
+  if (col && r < 5) stage(L.c0 - 3 + r);
+  for (int b = 0; b < L.nbatch; ++b) {
+    const int f = L.c0 - 1 + b * kRows + r;  // this thread's face is f + 1/2
+    const bool live = col && f < L.c1;
+    if (live) stage(f + 3);
+    __syncthreads();
+    R F[5], G[5];
 #pragma unroll
-    for (int l = 0; l < NV; ++l) Fp[l] = sF[r - 1][l][lane];
+    for (int l = 0; l < 5; ++l) F[l] = G[l] = R(0);
+    if (live) {
+      const int64_t p0 = DIR == 1 ? z.at(L.i, f, L.o) : z.at(L.i, L.o, f);
+      R mk0[3], mk1[3];
 #pragma unroll
-    for (int l = 0; l < 5; ++l) Gp[l] = sG[r - 1][l][lane];
-    const int64_t q = DIR == 1 ? z.at0(i, s, o) : z.at0(i, o, s);
-    add_dq<NS, NT, R>(z, q, init != 0, F, Fp, s < ncds, G, Gp);
+      for (int c = 0; c < 3; ++c) {
+        mk0[c] = (R)z.metric[(rb + c) * z.sz + p0];
+        mk1[c] = (R)z.metric[(rb + c) * z.sz + p0 + L.st];
+      }
+      const R j0 = (R)z.jac[p0], j1 = (R)z.jac[p0 + L.st];
+      R cvb[6][6];
+#pragma unroll
+      for (int s = 0; s < 6; ++s) {
+        const int sl = (f + 1 + s) % kRing;  // row f - 2 + s
+#pragma unroll
+        for (int c = 0; c < 6; ++c) cvb[s][c] = sQ[sl][c][lane];
+#pragma unroll
+        for (int c = 0; c < 5; ++c) G[c] += cds_weight<R>(s) * sN[sl][c][lane];
+      }
+      R pl[7], pr[7];
+      aweno_air(cvb, mk0, mk1, pl, pr);
+      riemann_viscous<0, 0, 1, R>(z, P, m, DIR, p0, L.st, pl, pr, mk0, mk1, j0, j1, F);
+    }
+#pragma unroll
+    for (int l = 0; l < 5; ++l) {
+      sF[r + 2][l][lane] = F[l];
+      sG[r + 2][l][lane] = G[l];
+    }
+    if (r == kRows - 1) {
+      const int cs = (b + 1) & 1;
+#pragma unroll
+      for (int l = 0; l < 5; ++l) {
+        sF[cs][l][lane] = F[l];
+        sG[cs][l][lane] = G[l];
+      }
+    }
+    __syncthreads();
+    if (live && f >= L.c0) {
+      const int ls = r > 0 ? r + 1 : (b & 1);
+      R Fp[5], Gp[5];
+#pragma unroll
+      for (int l = 0; l < 5; ++l) {
+        Fp[l] = sF[ls][l][lane];
+        Gp[l] = sG[ls][l][lane];
+      }
+      const int64_t q = DIR == 1 ? z.at0(L.i, f, L.o) : z.at0(L.i, L.o, f);
+      add_dq<0, 0, R>(z, q, init != 0, F, Fp, f < ncds, G, Gp);
+    }
+    // no barrier here: the next batch's staging and flux stores come after
+    // its first barrier, which every thread reaches only once done here
+    // (ring rows overwritten next were last read before this batch's
+    // second barrier)
   }
 }
 
 }  // namespace
 
+// Segments per line of the j/k sweeps.  With B workgroups of the kernel
+// resident per CU (its occupancy), the sweep takes about
+// ceil(workgroups / (B CUs)) rounds of (faces per segment / rows) batches;
+// the fewest segments that minimise that product are taken.
+static int sweep_segments(const void *kernel, int threads, int cross, int n, int rows) {
+  static int ncu_of[64];
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) dev = 0;
+  if (!ncu_of[dev]) {
+    int v = 0;
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || v <= 0) v = 256;
+    ncu_of[dev] = v;
+  }
+  int per_cu = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, threads, 0) != hipSuccess || per_cu <= 0)
+    per_cu = 1;
+  const int64_t slots = (int64_t)ncu_of[dev] * per_cu;
+  int best = 1;
+  int64_t best_cost = INT64_MAX;
+  for (int ns = 1; ns <= std::min(n, 4096); ++ns) {
+    const int64_t rounds = ((int64_t)cross * ns + slots - 1) / slots;
+    const int len = (n + ns - 1) / ns;
+    const int64_t cost = rounds * ((len + rows) / rows);
+    if (cost < best_cost) {
+      best_cost = cost;
+      best = ns;
+    }
+  }
+  return best;
+}
+
 template <int NS, int NT, int SCHEME, typename R>
 hipError_t launch_sweep_t(const ZoneView &z, const DParams &P, const DMech *m, int dir, bool init, hipStream_t s) {
   if (dir == 0) {
-    const dim3 gr((z.mx + 62) / 63, (z.my + 3) / 4, z.mz);
-    hipLaunchKernelGGL((sweep_i<NS, NT, SCHEME, R>), gr, dim3(64, 4, 1), 0, s, z, P, m, init ? 1 : 0);
-  } else if (dir == 1) {
-    constexpr int kRows = JkRows<NS + NT>::value;
-    const dim3 gr((z.mx + 63) / 64, z.mz, (z.my + kRows - 2) / (kRows - 1));
-    hipLaunchKernelGGL((sweep_jk<1, NS, NT, SCHEME, R>), gr, dim3(64, kRows, 1), 0, s, z, P, m, init ? 1 : 0);
+    const int64_t waves = ((int64_t)z.my * (z.mx + 1) + 62) / 63;
+    const int bpp = (int)((waves + 3) / 4);
+    const dim3 gr((unsigned)((int64_t)bpp * z.mz));
+    hipLaunchKernelGGL((sweep_i<NS, NT, SCHEME, R>), gr, dim3(64, 4, 1), 0, s, z, P, m, init ? 1 : 0, bpp);
+    return hipGetLastError();
+  }
+  constexpr bool kStaged = SCHEME == 1 && NS == 0 && NT == 0;
+  constexpr int kRows = kStaged ? 8 : JkRows<NS + NT, R>::value;
+  const int gx = (z.mx + 63) / 64, gy = dir == 1 ? z.mz : z.my, n = dir == 1 ? z.my : z.mz;
+  const int cross = gx * gy;
+  const void *fn;
+  if constexpr (kStaged)
+    fn = dir == 1 ? (const void *)sweep_jk_aweno<1, R> : (const void *)sweep_jk_aweno<2, R>;
+  else
+    fn = dir == 1 ? (const void *)sweep_jk<1, NS, NT, SCHEME, R> : (const void *)sweep_jk<2, NS, NT, SCHEME, R>;
+  const int nseg = sweep_segments(fn, 64 * kRows, cross, n, kRows);
+  const dim3 gr((unsigned)((int64_t)cross * nseg));
+  const int in = init ? 1 : 0;
+  if constexpr (kStaged) {
+    if (dir == 1)
+      hipLaunchKernelGGL((sweep_jk_aweno<1, R>), gr, dim3(64, kRows, 1), 0, s, z, P, m, in, gx, nseg);
+    else
+      hipLaunchKernelGGL((sweep_jk_aweno<2, R>), gr, dim3(64, kRows, 1), 0, s, z, P, m, in, gx, nseg);
   } else {
-    constexpr int kRows = JkRows<NS + NT>::value;
-    const dim3 gr((z.mx + 63) / 64, z.my, (z.mz + kRows - 2) / (kRows - 1));
-    hipLaunchKernelGGL((sweep_jk<2, NS, NT, SCHEME, R>), gr, dim3(64, kRows, 1), 0, s, z, P, m, init ? 1 : 0);
+    if (dir == 1)
+      hipLaunchKernelGGL((sweep_jk<1, NS, NT, SCHEME, R>), gr, dim3(64, kRows, 1), 0, s, z, P, m, in, gx, nseg);
+    else
+      hipLaunchKernelGGL((sweep_jk<2, NS, NT, SCHEME, R>), gr, dim3(64, kRows, 1), 0, s, z, P, m, in, gx, nseg);
   }
   return hipGetLastError();
 }
