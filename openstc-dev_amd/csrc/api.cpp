@@ -887,6 +887,10 @@ int ostc_finalize(ostc_solver *s) {
     z.mz = b.mz;
     z.g = g;
     const Layout L(b.mx, b.my, b.mz, g), L0(b.mx, b.my, b.mz, 0);
+    // kernels decompose point indices in 32-bit arithmetic
+    if (L.sz >= ((int64_t)1 << 32))
+      return fail("block " + std::to_string(&b - &s->blocks[0]) +
+                  " has 2^32 or more ghosted points; split it into smaller blocks");
     z.d2 = L.d2;
     z.d1 = L.d1;
     z.sz = L.sz;
@@ -911,7 +915,8 @@ int ostc_finalize(ostc_solver *s) {
         alloc(&z.bv_last, L0.sz * 4) ||
         alloc(&z.chem_jac, s->P.chemSrcMethod == 1 ? L0.sz * ns * ns : (s->P.chemSrcMethod == 2 ? L0.sz * ns : 1)) ||
         alloc(&z.inv_sr, L0.sz * 3) ||
-        (s->prm.get_int("implicit_method", 0) == 1 && (alloc(&z.dq0, L0.sz * nv) || alloc(&z.dqk, L0.sz * nv))))
+        (s->prm.get_int("implicit_method", 0) == 1 &&
+         (alloc(&z.dq0, L0.sz * nv) || alloc(&z.dqk, L0.sz * nv) || (ns && alloc(&z.dplur_h, L0.sz * (ns + 1))))))
       return 1;
     // SST: mu_t and lambda_t (ghost lambda_t stays zero: only the source
     // kernel writes it, on the interior), wall distance, source diagonal

@@ -38,6 +38,17 @@ struct DParams {
   int turb_implicit, compress;  // SST: implicit source diagonal; compressibility correction 0-3
 };
 
+// (a, b, c) of linear index t over an a-fastest na x nb x nc box, in 32-bit
+// unsigned arithmetic: block point counts stay below 2^32 (checked when the
+// blocks are built), and 64-bit division is a long instruction sequence
+__device__ inline void unflatten3(int64_t t, int na, int nb, int &a, int &b, int &c) {
+  const uint32_t u = (uint32_t)t, ua = (uint32_t)na, ub = (uint32_t)nb;
+  const uint32_t r = u / ua;
+  a = (int)(u - r * ua);
+  c = (int)(r / ub);
+  b = (int)(r - (uint32_t)c * ub);
+}
+
 // Raw device pointers of one block (DZone, Field.h:13-71).  bv/sv/cv/qn and
 // the property arrays use the ngg-ghosted layout; dq/dt_local/bv_last use
 // the ghost-free layout.  metric is 9 component-major arrays
@@ -52,6 +63,8 @@ struct ZoneView {
   double *dq, *dt_local, *bv_last, *chem_jac;
   double *inv_sr;          // steady path: inviscid spectral radii, 3 x ghost-free (null otherwise)
   double *dq0, *dqk;       // DPLUR work arrays, n_var x ghost-free (null otherwise)
+  double *dplur_h;         // DPLUR, mixtures: species enthalpies and mixture total enthalpy of each
+                           // interior point, (n_spec + 1) x ghost-free (null otherwise)
   double *mut, *tcond;     // SST: turbulent viscosity and conductivity (ghosted; null when laminar)
   double *wdist;           // SST: wall distance (ghosted)
   double *turb_jac;        // SST: source Jacobian diagonal of k and omega, 2 x ghost-free

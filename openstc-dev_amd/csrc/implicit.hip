@@ -6,6 +6,8 @@
 // use the ghost-free component-major layout.
 #include <hip/hip_runtime.h>
 
+#include <utility>
+
 #include "device_chem.h"
 #include "flux_kernels.h"
 
@@ -19,10 +21,7 @@ __device__ inline bool cell_index(const ZoneView &z, int &i, int &j, int &k) {
   const int64_t t = (int64_t)blockIdx.x * kTB + threadIdx.x;
   const int64_t n = (int64_t)z.mx * z.my * z.mz;
   if (t >= n) return false;
-  i = (int)(t % z.mx);
-  const int64_t r = t / z.mx;
-  j = (int)(r % z.my);
-  k = (int)(r / z.my);
+  unflatten3(t, z.mx, z.my, i, j, k);
   return true;
 }
 
@@ -112,19 +111,18 @@ __device__ inline void jac_times_dq(const ZoneView &z, const DMech *__restrict__
   if constexpr (NS == 0) {
     h = gamma / (gamma - 1) * z.bv[4 * z.sz + p] / z.bv[p] + e;
   } else {
-    double hs[NS];
+    // species enthalpies and h = sum Y h + e of this point, fixed during the
+    // inner sweeps (k_dplur_enthalpy)
     const double t = z.bv[5 * z.sz + p];
-    d_enthalpy<NS>(t, m, hs);
     gamma = z.gam[p];
 #pragma unroll
     for (int l = 0; l < NS; ++l) {
       b3 += kRuD * t / m->mw[l] * d[5 + l];
-      b4 += hs[l] * d[5 + l];
-      h += z.sv[(int64_t)l * z.sz + p] * hs[l];
+      b4 += z.dplur_h[(int64_t)l * z.sz0 + q] * d[5 + l];
     }
+    h = z.dplur_h[(int64_t)NS * z.sz0 + q];
     b3 *= gamma;
     b4 *= gamma - 1;
-    h += e;
   }
   const double b1 = mx * d[1] + my * d[2] + mz * d[3] - U * d[0];
   const double b2 = (gamma - 1) * (e * d[0] - u * d[1] - v * d[2] - w * d[3] + d[4]);
@@ -135,6 +133,25 @@ __device__ inline void jac_times_dq(const ZoneView &z, const DMech *__restrict__
   out[4] = h * b1 + U * b2 + l1 * d[4] + U * (b3 - b4);
 #pragma unroll
   for (int l = 0; l < NS + NT; ++l) out[5 + l] = l1 * d[5 + l] + z.sv[(int64_t)l * z.sz + p] * b1;
+}
+
+// the per-point part of DPLUR.cuh:80-137 for mixtures: species enthalpies
+// at the point's temperature and h = sum_l Y_l h_l + |u|^2 / 2, summed in the
+// order the Jacobian product uses
+template <int NS>
+__global__ void __launch_bounds__(kTB) k_dplur_enthalpy(ZoneView z, const DMech *__restrict__ m) {
+  int i, j, k;
+  if (!cell_index(z, i, j, k)) return;
+  const int64_t p = z.at(i, j, k), q = z.at0(i, j, k);
+  double hs[NS];
+  d_enthalpy<NS>(z.bv[5 * z.sz + p], m, hs);
+  double h = 0;
+#pragma unroll
+  for (int l = 0; l < NS; ++l) {
+    z.dplur_h[(int64_t)l * z.sz0 + q] = hs[l];
+    h += z.sv[(int64_t)l * z.sz + p] * hs[l];
+  }
+  z.dplur_h[(int64_t)NS * z.sz0 + q] = h + 0.5 * z.vel[p] * z.vel[p];
 }
 
 // DPLUR.cuh:139-257: dqk from the neighbours' current dq (interior only)
@@ -204,7 +221,11 @@ __global__ void k_zero_dq_range(ZoneView z, int nv, int i0, int i1, int j0, int 
   const int64_t t = (int64_t)blockIdx.x * kTB + threadIdx.x;
   const int ni = i1 - i0 + 1, nj = j1 - j0 + 1, nk = k1 - k0 + 1;
   if (t >= (int64_t)ni * nj * nk) return;
-  const int i = i0 + (int)(t % ni), j = j0 + (int)((t / ni) % nj), k = k0 + (int)(t / ((int64_t)ni * nj));
+  int i, j, k;
+  unflatten3(t, ni, nj, i, j, k);
+  i += i0;
+  j += j0;
+  k += k0;
   const int64_t q = z.at0(i, j, k);
   for (int l = 0; l < nv; ++l) z.dq[(int64_t)l * z.sz0 + q] = 0;
 }
@@ -312,28 +333,38 @@ hipError_t launch_dplur(const ZoneView &z, const DParams &P, const DMech *m, int
     else
       hipLaunchKernelGGL((k_dplur_dq0<9, 2>), g, bk, 0, s, z, fr, P.chemSrcMethod, timp);
   }
+  if (P.ns == 9) {
+    if (!z.dplur_h) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_dplur_enthalpy<9>, g, bk, 0, s, z, m);
+  }
+  // The inner sweeps alternate between dq and dqk as input and output (the
+  // reference copies dqk back into dq after each sweep); one copy at the end
+  // when the count is odd leaves the result in dq.
   for (int it = 0; it < inner_steps; ++it) {
+    ZoneView zi = z;
+    if (it & 1) std::swap(zi.dq, zi.dqk);
     for (const BCDesc &b : walls) {
       const int64_t cnt = (int64_t)(b.re[0] - b.rs[0] + 1) * (b.re[1] - b.rs[1] + 1) * (b.re[2] - b.rs[2] + 1);
       if (cnt <= 0) continue;
       if (b.rs[0] < 0 || b.rs[1] < 0 || b.rs[2] < 0 || b.re[0] >= z.mx || b.re[1] >= z.my || b.re[2] >= z.mz)
         return hipErrorInvalidValue;
-      hipLaunchKernelGGL(k_zero_dq_range, dim3(nb(cnt)), dim3(kTB), 0, s, z, nv, b.rs[0], b.re[0], b.rs[1], b.re[1],
+      hipLaunchKernelGGL(k_zero_dq_range, dim3(nb(cnt)), dim3(kTB), 0, s, zi, nv, b.rs[0], b.re[0], b.rs[1], b.re[1],
                          b.rs[2], b.re[2]);
     }
     if (P.nt == 0) {
       if (P.ns == 0)
-        hipLaunchKernelGGL((k_dplur_inner<0, 0>), g, bk, 0, s, z, m, fr, P.chemSrcMethod, 0);
+        hipLaunchKernelGGL((k_dplur_inner<0, 0>), g, bk, 0, s, zi, m, fr, P.chemSrcMethod, 0);
       else
-        hipLaunchKernelGGL((k_dplur_inner<9, 0>), g, bk, 0, s, z, m, fr, P.chemSrcMethod, 0);
+        hipLaunchKernelGGL((k_dplur_inner<9, 0>), g, bk, 0, s, zi, m, fr, P.chemSrcMethod, 0);
     } else {
       if (P.ns == 0)
-        hipLaunchKernelGGL((k_dplur_inner<0, 2>), g, bk, 0, s, z, m, fr, P.chemSrcMethod, timp);
+        hipLaunchKernelGGL((k_dplur_inner<0, 2>), g, bk, 0, s, zi, m, fr, P.chemSrcMethod, timp);
       else
-        hipLaunchKernelGGL((k_dplur_inner<9, 2>), g, bk, 0, s, z, m, fr, P.chemSrcMethod, timp);
+        hipLaunchKernelGGL((k_dplur_inner<9, 2>), g, bk, 0, s, zi, m, fr, P.chemSrcMethod, timp);
     }
-    hipLaunchKernelGGL(k_copy, dim3(nb(z.sz0 * nv)), dim3(kTB), 0, s, (const double *)z.dqk, z.dq, z.sz0 * nv);
   }
+  if (inner_steps & 1)
+    hipLaunchKernelGGL(k_copy, dim3(nb(z.sz0 * nv)), dim3(kTB), 0, s, (const double *)z.dqk, z.dq, z.sz0 * nv);
   hipLaunchKernelGGL(k_dq_back, dim3(nb(n)), dim3(kTB), 0, s, z, nv);
   return hipGetLastError();
 }

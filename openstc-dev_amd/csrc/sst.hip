@@ -87,7 +87,11 @@ __global__ void __launch_bounds__(kTB) k_init_mut(ZoneView z, const DMech *__res
   const int X = z.mx + 2, Y = z.my + 2, Z = z.mz + 2;
   const int64_t t = (int64_t)blockIdx.x * kTB + threadIdx.x;
   if (t >= (int64_t)X * Y * Z) return;
-  const int i = (int)(t % X) - 1, j = (int)((t / X) % Y) - 1, k = (int)(t / ((int64_t)X * Y)) - 1;
+  int i, j, k;
+  unflatten3(t, X, Y, i, j, k);
+  --i;
+  --j;
+  --k;
   const int64_t p = z.at(i, j, k);
   const double T = z.bv[5 * z.sz + p];
   double mul = d_sutherland(T);
@@ -128,7 +132,8 @@ __global__ void __launch_bounds__(kTB) k_sst(ZoneView z, DParams P) {
   const int64_t t = (int64_t)blockIdx.x * kTB + threadIdx.x;
   const int64_t n = (int64_t)z.mx * z.my * z.mz;
   if (t >= n) return;
-  const int i = (int)(t % z.mx), j = (int)((t / z.mx) % z.my), k = (int)(t / ((int64_t)z.mx * z.my));
+  int i, j, k;
+  unflatten3(t, z.mx, z.my, i, j, k);
   const int64_t p = z.at(i, j, k), q = z.at0(i, j, k);
   const double *ks = z.sv + (int64_t)NS * z.sz, *ws = z.sv + (int64_t)(NS + 1) * z.sz;
   double mm[9], gu[3], gv[3], gw[3], gk[3], go[3];

@@ -16,10 +16,7 @@ namespace {
 constexpr int kTB = 256;
 
 __device__ inline void cell_of(const ZoneView &z, int64_t t, int &i, int &j, int &k) {
-  i = (int)(t % z.mx);
-  const int64_t r = t / z.mx;
-  j = (int)(r % z.my);
-  k = (int)(r / z.my);
+  unflatten3(t, z.mx, z.my, i, j, k);
 }
 
 __device__ inline double wave_min(double v) {
@@ -268,7 +265,11 @@ __global__ void __launch_bounds__(kTB) k_bc(ZoneView z, DParams P, const DMech *
   const int nx = b.re[0] - b.rs[0] + 1, ny = b.re[1] - b.rs[1] + 1, nz = b.re[2] - b.rs[2] + 1;
   const int64_t t = (int64_t)blockIdx.x * kTB + threadIdx.x;
   if (t >= (int64_t)nx * ny * nz) return;
-  const int i = b.rs[0] + (int)(t % nx), j = b.rs[1] + (int)((t / nx) % ny), k = b.rs[2] + (int)(t / ((int64_t)nx * ny));
+  int i, j, k;
+  unflatten3(t, nx, ny, i, j, k);
+  i += b.rs[0];
+  j += b.rs[1];
+  k += b.rs[2];
   int d[3] = {0, 0, 0};
   d[b.face] = b.dir;
   const int64_t so[3] = {1, z.d2, z.d1};
@@ -587,7 +588,8 @@ __global__ void __launch_bounds__(kTB) k_eliminate_k(ZoneView z, DParams P) {
   const int X = z.mx + 2 * z.g, Y = z.my + 2 * z.g;
   const int64_t t = (int64_t)blockIdx.x * kTB + threadIdx.x;
   if (t >= (int64_t)X * Y) return;
-  const int i = (int)(t % X) - z.g, j = (int)(t / X) - z.g;
+  const uint32_t jj = (uint32_t)t / (uint32_t)X;
+  const int i = (int)((uint32_t)t - jj * (uint32_t)X) - z.g, j = (int)jj - z.g;
   const int64_t p0 = z.at(i, j, 0);
   for (int k = 1; k <= z.g; ++k) {
     const int64_t pu = p0 + k * z.d1, pd = p0 - k * z.d1;
@@ -617,7 +619,11 @@ __global__ void __launch_bounds__(kTB) k_properties(ZoneView z, DParams P, const
   const int X = z.mx + 2, Y = z.my + 2, Z = z.mz + 2;
   const int64_t t = (int64_t)blockIdx.x * kTB + threadIdx.x;
   if (t >= (int64_t)X * Y * Z) return;
-  const int i = (int)(t % X) - 1, j = (int)((t / X) % Y) - 1, k = (int)(t / ((int64_t)X * Y)) - 1;
+  int i, j, k;
+  unflatten3(t, X, Y, i, j, k);
+  --i;
+  --j;
+  --k;
   const int64_t p = z.at(i, j, k);
   const double T = z.bv[5 * z.sz + p];
   if constexpr (NS > 0) {
