@@ -11,14 +11,16 @@ typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8;
 typedef __attribute__((__vector_size__(4 * sizeof(float)))) float f32x4;
 
 // GELU, tanh form: x (1 + tanh u) / 2 = x / (1 + exp(-2u)),
-// u = sqrt(2/pi) (x + 0.044715 x^3); and its derivative
+// u = sqrt(2/pi) (x + 0.044715 x^3); and its derivative.  The logistic
+// factor uses the hardware reciprocal (1 ulp; exp(-2u) = inf gives 0), as
+// the value is rounded to BF16 next; training and inference share it.
 __device__ inline float gelu(float x) {
   const float u2 = 1.5957691216057308f * (x + 0.044715f * x * x * x);
-  return x / (1.0f + __expf(-u2));
+  return x * __builtin_amdgcn_rcpf(1.0f + __expf(-u2));
 }
 __device__ inline float gelu_grad(float x) {
   const float u2 = 1.5957691216057308f * (x + 0.044715f * x * x * x);
-  const float s = 1.0f / (1.0f + __expf(-u2));
+  const float s = __builtin_amdgcn_rcpf(1.0f + __expf(-u2));
   return s + x * s * (1.0f - s) * 1.5957691216057308f * (1.0f + 0.134145f * x * x);
 }
 

@@ -329,19 +329,28 @@ __global__ void __launch_bounds__(256, 2) k_sur_net(const DMech *__restrict__ m,
   }
   __syncthreads();
 
+  // Hidden-layer tiles are computed transposed, C[neuron][cell] = W x act^T
+  // (the weight fragment as the A operand, the activation fragment as B:
+  // the same registers as act x W^T), so that a lane holds four consecutive
+  // neurons of one cell and the epilogue stores them with one 8-byte write.
   f32x4 acc[RT][4];
+  typedef __attribute__((__vector_size__(4 * sizeof(__bf16)))) __bf16 bf16x4;
   auto epilogue = [&](const __bf16 *W, int ld, int kp) {
-    float bias[4];
+    float bias[4][4];
 #pragma unroll
-    for (int b = 0; b < 4; ++b) bias[b] = (float)W[(int64_t)(wave * 64 + b * 16 + lr) * ld + kp];
-    __syncthreads();  // every wave has read its last A fragments of act
+    for (int b = 0; b < 4; ++b)
+#pragma unroll
+      for (int e = 0; e < 4; ++e) bias[b][e] = (float)W[(int64_t)(wave * 64 + b * 16 + 4 * (lane >> 4) + e) * ld + kp];
+    __syncthreads();  // every wave has read its last fragments of act
 #pragma unroll
     for (int a = 0; a < RT; ++a)
 #pragma unroll
-      for (int b = 0; b < 4; ++b)
+      for (int b = 0; b < 4; ++b) {
+        bf16x4 v;
 #pragma unroll
-        for (int e = 0; e < 4; ++e)
-          act[a * 16 + 4 * (lane >> 4) + e][wave * 64 + b * 16 + lr] = (__bf16)gelu(acc[a][b][e] + bias[b]);
+        for (int e = 0; e < 4; ++e) v[e] = (__bf16)gelu(acc[a][b][e] + bias[b][e]);
+        *(bf16x4 *)&act[a * 16 + lr][wave * 64 + b * 16 + 4 * (lane >> 4)] = v;
+      }
     __syncthreads();
   };
 
@@ -351,7 +360,7 @@ __global__ void __launch_bounds__(256, 2) k_sur_net(const DMech *__restrict__ m,
     const bf16x8 af = *(const bf16x8 *)&act[a * 16 + lr][lk];
 #pragma unroll
     for (int b = 0; b < 4; ++b)
-      acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af, b0[b], f32x4{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
+      acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(b0[b], af, f32x4{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
   }
   epilogue(wb, kLd[0], kKpad[0]);
 
@@ -380,7 +389,7 @@ __global__ void __launch_bounds__(256, 2) k_sur_net(const DMech *__restrict__ m,
         const bf16x8 af = *(const bf16x8 *)&act[a * 16 + lr][32 * ks + lk];
 #pragma unroll
         for (int b = 0; b < 4; ++b)
-          acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af, bq[ks % (PF + 1)][b], acc[a][b], 0, 0, 0);
+          acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bq[ks % (PF + 1)][b], af, acc[a][b], 0, 0, 0);
       }
     }
     epilogue(W, kHld, 256);

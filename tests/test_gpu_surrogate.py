@@ -149,6 +149,28 @@ def test_surrogate_accuracy_and_conservation(solver, trained, mech):
         v.free()
 
 
+def test_fused_network_matches_layer_path(solver, trained):
+    """The fused inference kernels (128 and 64 cells per workgroup, hidden
+    tiles computed transposed) against one GEMM launch per layer on the same
+    trained weights: BF16 operands and FP32 accumulation in both, so the
+    standardised outputs agree to BF16 rounding (ragged count: not a
+    multiple of either workgroup size)."""
+    n = 5000
+    d, _ = _batch(solver, n, seed=606)
+    outs = {}
+    for kernel in (0, 1, 2):
+        solver.surrogate_set_kernel(kernel)
+        outs[kernel] = solver.surrogate_network_outputs(n, d["rho"], d["T"], d["Y"])
+    solver.surrogate_set_kernel(1)
+    for kernel in (1, 2):
+        e = relerr(outs[kernel], outs[0])
+        print("fused kernel %d vs layer path: rel. error %.2e" % (kernel, e), flush=True)
+        assert np.all(np.isfinite(outs[kernel]))
+        assert e < 1e-2
+    for v in d.values():
+        v.free()
+
+
 def test_gate_sends_out_of_range_and_wrong_dt_to_fallback(solver, trained):
     n = 4096
     d, host = _batch(solver, n, seed=77, T=3000.0)  # outside the 900-2400 K training range
