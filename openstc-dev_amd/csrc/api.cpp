@@ -411,6 +411,7 @@ int upload_mech(ostc_solver *s) {
       h.lo[i * 7 + c] = m.lo[i * 7 + c];
     }
     h.LJ_potent_inv[i] = m.LJ_potent_inv[i];
+    h.log_LJ_potent_inv[i] = m.LJ_potent_inv[i] > 0 ? std::log(m.LJ_potent_inv[i]) : 0.0;
     h.vis_coeff[i] = m.vis_coeff[i];
   }
   const int ns = m.n_spec, nr = m.n_reac;
@@ -419,6 +420,7 @@ int upload_mech(ostc_solver *s) {
     h.sqrt_WiDivWjPl1Mul8[i] = m.sqrt_WiDivWjPl1Mul8[i];
     h.binary_diffusivity_coeff[i] = m.binary_diffusivity_coeff[i];
     h.kb_over_eps_jk[i] = m.kb_over_eps_jk[i];
+    h.log_kb_over_eps_jk[i] = m.kb_over_eps_jk[i] > 0 ? std::log(m.kb_over_eps_jk[i]) : 0.0;
   }
   for (int r = 0; r < nr; ++r) {
     h.label[r] = m.label[r];
@@ -594,9 +596,11 @@ int rhs_block(ostc_solver *s, Block &b) {
   return 0;
 }
 
-int properties_block(ostc_solver *s, Block &b) {
+// k_planes_copied: in 2-D, k_eliminate_k ran right before (the k-ghost
+// planes hold copies of k = 0)
+int properties_block(ostc_solver *s, Block &b, bool k_planes_copied) {
   ostc_solver::Timer t(s, "properties");
-  HIPCHK(launch_properties(b.z, s->P, s->d_mech, s->stream));
+  HIPCHK(launch_properties(b.z, s->P, s->d_mech, k_planes_copied, s->stream));
   return 0;
 }
 
@@ -1126,11 +1130,11 @@ int ostc_initialize(ostc_solver *s) {
     // the mixture farfield reads gamma of its boundary point; the reference
     // applies the first BCs before any properties exist (Driver.cu:53-92,
     // reading unset memory), here they are computed from the initial field
-    if (s->P.mix && properties_block(s, b)) return 1;
+    if (s->P.mix && properties_block(s, b, true)) return 1;
     if (run_bcs(s, b, false)) return 1;
     HIPCHK(launch_velocity(b.z, s->stream));
     if (s->n_turb) HIPCHK(launch_init_mut(b.z, s->P, s->d_mech, s->stream));
-    if (properties_block(s, b)) return 1;
+    if (properties_block(s, b, false)) return 1;
     HIPCHK(launch_store_last(b.z, s->stream));
     HIPCHK(launch_cv_from_bv(b.z, s->P, s->d_mech, s->stream));
   }
@@ -1184,7 +1188,7 @@ int ostc_update_properties(ostc_solver *s, int blk) {
   if (check_blk(s, blk)) return 1;
   Block &b = s->blocks[blk];
   if (s->P.dim == 2) HIPCHK(launch_eliminate_k(b.z, s->P, true, s->stream));
-  if (properties_block(s, b)) return 1;
+  if (properties_block(s, b, true)) return 1;
   HIPCHK(hipStreamSynchronize(s->stream));
   return 0;
 }
@@ -1225,7 +1229,7 @@ int ostc_step_rk3(ostc_solver *s, int nsteps, double *dt_hist, double *res_hist)
         }
         if (run_bcs(s, b, true)) return 1;
         if (s->P.dim == 2) HIPCHK(launch_eliminate_k(b.z, s->P, true, s->stream));
-        if (properties_block(s, b)) return 1;
+        if (properties_block(s, b, true)) return 1;
       }
       if (step % os == 0 || step == 1) {
         ostc_solver::Timer t(s, "residual");
@@ -1313,7 +1317,7 @@ int ostc_step_steady(ostc_solver *s, int nsteps, double *res_hist) {
       }
       if (run_bcs(s, b, false)) return 1;
       if (s->P.dim == 2) HIPCHK(launch_eliminate_k(b.z, s->P, false, s->stream));
-      if (properties_block(s, b)) return 1;
+      if (properties_block(s, b, true)) return 1;
       if (step % os == 0 || step == 1) {
         ostc_solver::Timer t(s, "residual");
         HIPCHK(launch_residual(b.z, s->w, it, step == 1 ? 1 : 0, s->stream));
@@ -1420,7 +1424,7 @@ int ostc_chem_step(ostc_solver *s, int blk, double dt, int mode, int64_t *n_fall
   }
   if (run_bcs(s, b, true)) return 1;
   if (s->P.dim == 2) HIPCHK(launch_eliminate_k(b.z, s->P, true, s->stream));
-  if (properties_block(s, b)) return 1;
+  if (properties_block(s, b, true)) return 1;
   HIPCHK(hipStreamSynchronize(s->stream));
   s->drain_profile();
   if (n_fallback) *n_fallback = nf;

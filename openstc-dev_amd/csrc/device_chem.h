@@ -9,7 +9,11 @@ namespace ostc {
 
 __host__ __device__ constexpr double kPatmD() { return 101325.0; }
 
-__device__ inline double d_arrhenius(double t, double A, double b, double Ea) { return A * pow(t, b) * exp(-Ea / t); }
+// A T^b exp(-Ea / T) as A exp(b ln T - Ea / T): one exponential per rate,
+// with ln T and 1 / T (lt, it) computed once per cell
+__device__ inline double d_arrhenius(double lt, double it, double A, double b, double Ea) {
+  return A * exp(b * lt - Ea * it);
+}
 
 // x^n for the small integer exponents of stoichiometric coefficients and
 // reaction orders, as products (x^0 = 1, x^1 = x and x^2 = x*x exactly as a
@@ -32,11 +36,11 @@ __device__ inline double d_ipow(double x, int n) {
 // FiniteRateChem.cu:65-149 for reaction r: kf (Arrhenius + duplicate, third
 // body, Lindemann, Troe) and kb = kf / Kc (or the REV set, 0 if irreversible).
 template <int NS>
-__device__ inline void d_rate(const DMech *__restrict__ m, int r, double t, const double *c, const double *g,
-                              double temp_t, double &kf, double &kb) {
+__device__ inline void d_rate(const DMech *__restrict__ m, int r, double t, double lt, double it, const double *c,
+                              const double *g, double temp_t, double &kf, double &kb) {
   const int type = m->label[r];
-  double k = d_arrhenius(t, m->A[r], m->b[r], m->Ea[r]);
-  if (type == 3) k += d_arrhenius(t, m->A2[r], m->b2[r], m->Ea2[r]);
+  double k = d_arrhenius(lt, it, m->A[r], m->b[r], m->Ea[r]);
+  if (type == 3) k += d_arrhenius(lt, it, m->A2[r], m->b2[r], m->Ea2[r]);
   if (type > 3) {
     double cc = 0;
 #pragma unroll
@@ -44,7 +48,7 @@ __device__ inline void d_rate(const DMech *__restrict__ m, int r, double t, cons
     if (type == 4) {
       k *= cc;
     } else {
-      const double klow = d_arrhenius(t, m->A2[r], m->b2[r], m->Ea2[r]);
+      const double klow = d_arrhenius(lt, it, m->A2[r], m->b2[r], m->Ea2[r]);
       const double khigh = k;
       const double pr = klow * cc / khigh;
       double F = 1.0;
@@ -57,7 +61,7 @@ __device__ inline void d_rate(const DMech *__restrict__ m, int r, double t, cons
         const double lpr = log10(pr);
         const double tempo = (lpr + c2) / (nn - 0.14 * (lpr + c2));
         const double pp = lfc / (1.0 + tempo * tempo);
-        F = pow(10.0, pp);
+        F = exp(2.302585092994046 * pp);  // 10^pp
       }
       k = khigh * pr / (1.0 + pr) * F;
     }
@@ -66,7 +70,7 @@ __device__ inline void d_rate(const DMech *__restrict__ m, int r, double t, cons
   if (type == 0) {
     kb = 0;
   } else if (type == 2) {
-    kb = d_arrhenius(t, m->A2[r], m->b2[r], m->Ea2[r]);
+    kb = d_arrhenius(lt, it, m->A2[r], m->b2[r], m->Ea2[r]);
   } else {
     double dg = 0;
 #pragma unroll
@@ -88,6 +92,7 @@ __device__ inline void d_chem_source(const DMech *__restrict__ m, double rho, do
   for (int l = 0; l < NS; ++l) c[l] = rho * Y[l] / m->mw[l] * 1e-3;
   d_gibbs_rt<NS>(t, m, g);
   const double temp_t = (kPatmD() / kRuD * 1e-3) / t;
+  const double lt = log(t), it = 1.0 / t;
   const int nr = m->n_reac;
   double creation[NS], destr[NS];
 #pragma unroll
@@ -101,7 +106,7 @@ __device__ inline void d_chem_source(const DMech *__restrict__ m, double rho, do
   }
   for (int r = 0; r < nr; ++r) {
     double kf, kb;
-    d_rate<NS>(m, r, t, c, g, temp_t, kf, kb);
+    d_rate<NS>(m, r, t, lt, it, c, g, temp_t, kf, kb);
     double q1 = 1.0, q2 = 1.0;
 #pragma unroll
     for (int l = 0; l < NS; ++l) {
@@ -156,6 +161,7 @@ __device__ inline void d_chem_rates(const DMech *__restrict__ m, double rho, dou
   for (int l = 0; l < NS; ++l) c[l] = rho * Y[l] / m->mw[l] * 1e-3;
   d_gibbs_rt<NS>(t, m, g);
   const double temp_t = (kPatmD() / kRuD * 1e-3) / t;
+  const double lt = log(t), it = 1.0 / t;
   const int nr = m->n_reac;
   double creation[NS], destr[NS];
 #pragma unroll
@@ -165,7 +171,7 @@ __device__ inline void d_chem_rates(const DMech *__restrict__ m, double rho, dou
   }
   for (int r = 0; r < nr; ++r) {
     double kf, kb;
-    d_rate<NS>(m, r, t, c, g, temp_t, kf, kb);
+    d_rate<NS>(m, r, t, lt, it, c, g, temp_t, kf, kb);
     double q1 = 1.0, q2 = 1.0;
 #pragma unroll
     for (int l = 0; l < NS; ++l) {
@@ -280,19 +286,32 @@ __device__ inline void d_lu_solve(double *A, double *b) {
   }
 }
 
+// Transport.cu:6-38 species viscosities of Chapman-Enskog form,
+// vis = vis_coeff sqrt(T) / (1.147 T*^-0.145 + (T* + 0.5)^-2), T* = T / (eps/kb),
+// with T*^-0.145 = exp(-0.145 (log T + log(kb/eps))); also sqrt(vis) and
+// 1 / sqrt(vis), so that Wilke's sqrt(vis_a / vis_b) is a product
+template <int NS>
+__device__ inline void species_viscosities(const DMech *__restrict__ m, double t, double *vis, double *sv,
+                                           double *isv) {
+  const double st = sqrt(t), lt = log(t);
+#pragma unroll
+  for (int l = 0; l < NS; ++l) {
+    const double x5 = t * m->LJ_potent_inv[l] + 0.5;
+    const double ci = 1.147 * exp(-0.145 * (lt + m->log_LJ_potent_inv[l])) + 1.0 / (x5 * x5);
+    vis[l] = m->vis_coeff[l] * st / ci;
+    sv[l] = sqrt(vis[l]);
+    isv[l] = 1.0 / sv[l];
+  }
+}
+
 // Transport.cu:113-140 compute_viscosity: Wilke's rule alone, at the mean
 // molecular weight mw_total of composition Y
 template <int NS>
 __device__ inline double d_viscosity(const DMech *__restrict__ m, double t, double mw_total, const double *Y) {
-  double X[NS], vis[NS];
-  const double st = sqrt(t);
+  double X[NS], vis[NS], sv[NS], isv[NS];
+  species_viscosities<NS>(m, t, vis, sv, isv);
 #pragma unroll
-  for (int l = 0; l < NS; ++l) {
-    X[l] = Y[l] * mw_total / m->mw[l];
-    const double tdl = t * m->LJ_potent_inv[l];
-    const double ci = 1.147 * pow(tdl, -0.145) + pow(tdl + 0.5, -2.0);
-    vis[l] = m->vis_coeff[l] * st / ci;
-  }
+  for (int l = 0; l < NS; ++l) X[l] = Y[l] * mw_total / m->mw[l];
   double mu = 0;
 #pragma unroll
   for (int a = 0; a < NS; ++a) {
@@ -301,7 +320,7 @@ __device__ inline double d_viscosity(const DMech *__restrict__ m, double t, doub
     for (int b = 0; b < NS; ++b) {
       double pf = 1.0;
       if (a != b) {
-        const double num = 1 + sqrt(vis[a] / vis[b]) * m->WjDivWi_to_One4th[a * NS + b];
+        const double num = 1 + sv[a] * isv[b] * m->WjDivWi_to_One4th[a * NS + b];
         pf = num * num * m->sqrt_WiDivWjPl1Mul8[a * NS + b];
       }
       vt += pf * X[b];
@@ -319,15 +338,11 @@ __device__ inline void d_transport(const DMech *__restrict__ m, double t, double
 #pragma unroll
   for (int l = 0; l < NS; ++l) mwi += Y[l] / m->mw[l];
   const double mw_total = 1 / mwi;
-  double X[NS], vis[NS];
+  double X[NS], vis[NS], sv[NS], isv[NS];
   const double st = sqrt(t);
+  species_viscosities<NS>(m, t, vis, sv, isv);
 #pragma unroll
-  for (int l = 0; l < NS; ++l) {
-    X[l] = Y[l] * mw_total / m->mw[l];
-    const double tdl = t * m->LJ_potent_inv[l];
-    const double ci = 1.147 * pow(tdl, -0.145) + pow(tdl + 0.5, -2.0);
-    vis[l] = m->vis_coeff[l] * st / ci;
-  }
+  for (int l = 0; l < NS; ++l) X[l] = Y[l] * mw_total / m->mw[l];
   double viscosity = 0, conductivity = 0;
 #pragma unroll
   for (int a = 0; a < NS; ++a) {
@@ -336,7 +351,7 @@ __device__ inline void d_transport(const DMech *__restrict__ m, double t, double
     for (int b = 0; b < NS; ++b) {
       double pf = 1.0;
       if (a != b) {
-        const double num = 1 + sqrt(vis[a] / vis[b]) * m->WjDivWi_to_One4th[a * NS + b];
+        const double num = 1 + sv[a] * isv[b] * m->WjDivWi_to_One4th[a * NS + b];
         pf = num * num * m->sqrt_WiDivWjPl1Mul8[a * NS + b];
       }
       vt += pf * X[b];
@@ -348,16 +363,17 @@ __device__ inline void d_transport(const DMech *__restrict__ m, double t, double
   }
   mu = viscosity;
   lam = conductivity;
-  const double t32p = t * st / p;
-  // binary diffusivities of the NS (NS - 1) / 2 pairs, each once
-  double dij[NS * (NS - 1) / 2 > 0 ? NS * (NS - 1) / 2 : 1];
+  const double t32p = t * st / p, lt = log(t);
+  // reciprocal binary diffusivities 1 / D_ab of the NS (NS - 1) / 2 pairs,
+  // each once: T*^-0.145 = exp(-0.145 (log T + log k_ab))
+  double idij[NS * (NS - 1) / 2 > 0 ? NS * (NS - 1) / 2 : 1];
 #pragma unroll
   for (int a = 0; a < NS; ++a)
 #pragma unroll
     for (int b = a + 1; b < NS; ++b) {
       const double tr = t * m->kb_over_eps_jk[a * NS + b];
-      const double od = 1.0 / pow(tr, 0.145) + 1.0 / ((tr + 0.5) * (tr + 0.5));
-      dij[a * NS - a * (a + 1) / 2 + (b - a - 1)] = m->binary_diffusivity_coeff[a * NS + b] * t32p / od;
+      const double od = exp(-0.145 * (lt + m->log_kb_over_eps_jk[a * NS + b])) + 1.0 / ((tr + 0.5) * (tr + 0.5));
+      idij[a * NS - a * (a + 1) / 2 + (b - a - 1)] = od / (m->binary_diffusivity_coeff[a * NS + b] * t32p);
     }
 #pragma unroll
   for (int l = 0; l < NS; ++l) {
@@ -367,7 +383,7 @@ __device__ inline void d_transport(const DMech *__restrict__ m, double t, double
       if (l != n) {
         const int a = l < n ? l : n, b = l < n ? n : l;
         num += (X[n] + 1e-12) * m->mw[n];
-        den += (X[n] + 1e-12) / dij[a * NS - a * (a + 1) / 2 + (b - a - 1)];
+        den += (X[n] + 1e-12) * idij[a * NS - a * (a + 1) / 2 + (b - a - 1)];
       }
     }
     rhoD[l] = rho * num / (den * mw_total);

@@ -25,6 +25,9 @@ struct DMech {
   double A[kMR], b[kMR], Ea[kMR], A2[kMR], b2[kMR], Ea2[kMR];
   double third_body_coeff[kMR * kMS];
   double troe_alpha[kMR], troe_t3[kMR], troe_t1[kMR], troe_t2[kMR];
+  // logarithms of LJ_potent_inv and kb_over_eps_jk: the collision-integral
+  // powers T*^-0.145 become exp(-0.145 (log T + log k))
+  double log_LJ_potent_inv[kMS], log_kb_over_eps_jk[kMS * kMS];
 };
 
 // Scheme switches and constants (the scalar part of DParameter).
@@ -162,7 +165,12 @@ __device__ inline void d_gibbs_rt(double t, const DMech *__restrict__ m, double 
 }
 
 // Transport.cu:6-9
-__device__ inline double d_sutherland(double t) { return 1.716e-5 * pow(t / 273, 1.5) * (273 + 111) / (t + 111); }
+// (T / 273)^1.5 as x sqrt(x): the same power within an ulp, without the
+// log / exp of a general pow
+__device__ inline double d_sutherland(double t) {
+  const double x = t / 273;
+  return 1.716e-5 * (x * sqrt(x)) * (273 + 111) / (t + 111);
+}
 
 // FieldOperation.cu:3-39: Newton on e(T) from the warm start; tolerance
 // |1 - T1/T| <= 1e-3, at most 1000 iterations.

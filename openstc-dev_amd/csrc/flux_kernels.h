@@ -44,7 +44,10 @@ hipError_t launch_rk_update(const ZoneView &z, const DParams &P, const DMech *m,
 hipError_t launch_bc(const ZoneView &z, const DParams &P, const DMech *m, const BCDesc &bc, bool with_cv,
                      hipStream_t s);
 hipError_t launch_eliminate_k(const ZoneView &z, const DParams &P, bool with_cv, hipStream_t s);
-hipError_t launch_properties(const ZoneView &z, const DParams &P, const DMech *m, hipStream_t s);
+// k_planes_copied: a 2-D block whose k-ghost planes k_eliminate_k has just
+// made copies of k = 0 (then the k = 0 values are computed once)
+hipError_t launch_properties(const ZoneView &z, const DParams &P, const DMech *m, bool k_planes_copied,
+                             hipStream_t s);
 hipError_t launch_velocity(const ZoneView &z, hipStream_t s);
 hipError_t launch_cv_from_bv(const ZoneView &z, const DParams &P, const DMech *m, hipStream_t s);
 hipError_t launch_store_last(const ZoneView &z, hipStream_t s);

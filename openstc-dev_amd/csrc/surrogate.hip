@@ -41,6 +41,9 @@ struct SurNorm {
   double cons[4 * kNS];          // element mass-fraction rows, then the mass row
   int n_cons;
   double cons_tol, dy_max, dT_max, sig_min, dt;
+  // the input bounds on log(Y + 1e-12) and log p exponentiated, so the gate
+  // tests Y + 1e-12 and p directly (log is monotone: the same test)
+  double y_lo[kNS], y_hi[kNS], p_lo, p_hi;
 };
 
 // (log(Y + 1e-12), T, log p) of one cell; p from the equation of state as
@@ -193,11 +196,22 @@ __device__ inline bool finish_cell(const DMech *__restrict__ m, const SurNorm &n
                                    double *__restrict__ dT) {
   bool ok = true;
   const double rho = rho_a[c], T = T_a[c];
-  double Y[kNS], x[kSurIn];
+  double Y[kNS];
   load_cell(n, c, Y_a, Y);
-  raw_features(m, rho, T, Y, x);
+  {
+    // the training-range test of the raw features (raw_features), on
+    // Y + 1e-12, T and p against the exponentiated bounds
+    double mwi = 0;
 #pragma unroll
-  for (int i = 0; i < kSurIn; ++i) ok = ok && x[i] >= nm.x_lo[i] && x[i] <= nm.x_hi[i];
+    for (int l = 0; l < kNS; ++l) mwi += Y[l] / m->mw[l];
+    const double p = rho * T * (kRuD / (1 / mwi));
+#pragma unroll
+    for (int l = 0; l < kNS; ++l) {
+      const double y = fmax(Y[l], 0.0) + kYFloor;
+      ok = ok && y >= nm.y_lo[l] && y <= nm.y_hi[l];
+    }
+    ok = ok && T >= nm.x_lo[9] && T <= nm.x_hi[9] && p >= nm.p_lo && p <= nm.p_hi;
+  }
   double cj[kSurOut];
 #pragma unroll
   for (int j = 0; j < kSurOut; ++j) cj[j] = nm.c_scale[j] * sinh(nm.c_mean[j] + nm.c_std[j] * (double)o[j]);
@@ -634,6 +648,12 @@ void finish_norm(SurrogateState *s) {
     s->norm.x_lo[i] = s->x_min[i] - s->gate.margin * r - 1e-6;
     s->norm.x_hi[i] = s->x_max[i] + s->gate.margin * r + 1e-6;
   }
+  for (int l = 0; l < kNS; ++l) {
+    s->norm.y_lo[l] = std::exp(s->norm.x_lo[l]);
+    s->norm.y_hi[l] = std::exp(s->norm.x_hi[l]);
+  }
+  s->norm.p_lo = std::exp(s->norm.x_lo[10]);
+  s->norm.p_hi = std::exp(s->norm.x_hi[10]);
   s->norm.cons_tol = s->gate.cons_tol;
   s->norm.dy_max = s->gate.dy_max;
   s->norm.dT_max = s->gate.dT_max;

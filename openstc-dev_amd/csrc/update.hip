@@ -613,23 +613,28 @@ __global__ void __launch_bounds__(kTB) k_eliminate_k(ZoneView z, DParams P) {
   }
 }
 
-// FieldOperation.cuh:112-146 update_physical_properties on [-1, m]
-template <int NS>
+// FieldOperation.cuh:112-146 update_physical_properties on [-1, m].  PLANE:
+// a 2-D block, whose k-ghost planes the preceding k_eliminate_k made copies
+// of k = 0 (bv and every scalar): the k = 0 values are computed once and
+// stored on k = -1, 0, 1 (bit-identical to computing each plane), the Mach
+// number from each plane's own |u|.
+template <int NS, bool PLANE>
 __global__ void __launch_bounds__(kTB) k_properties(ZoneView z, DParams P, const DMech *__restrict__ m) {
-  const int X = z.mx + 2, Y = z.my + 2, Z = z.mz + 2;
+  const int X = z.mx + 2, Y = z.my + 2, Z = PLANE ? 1 : z.mz + 2;
   const int64_t t = (int64_t)blockIdx.x * kTB + threadIdx.x;
   if (t >= (int64_t)X * Y * Z) return;
   int i, j, k;
   unflatten3(t, X, Y, i, j, k);
   --i;
   --j;
-  --k;
+  k = PLANE ? 0 : k - 1;
   const int64_t p = z.at(i, j, k);
   const double T = z.bv[5 * z.sz + p];
+  double a, mu, lam, cpt = 0, gam = 0, rd[NS > 0 ? NS : 1];
   if constexpr (NS > 0) {
     double Yk[NS], cp[NS];
     d_cp<NS>(T, m, cp);
-    double mwi = 0, cpt = 0, cvt = 0;
+    double mwi = 0, cvt = 0;
 #pragma unroll
     for (int l = 0; l < NS; ++l) {
       Yk[l] = z.sv[(int64_t)l * z.sz + p];
@@ -638,25 +643,29 @@ __global__ void __launch_bounds__(kTB) k_properties(ZoneView z, DParams P, const
       cvt += Yk[l] * (cp[l] - kRuD / m->mw[l]);
     }
     const double mw = 1 / mwi;
-    const double gam = cpt / cvt;
-    z.cp[p] = cpt;
-    z.gam[p] = gam;
-    z.acoustic[p] = sqrt(gam * kRuD * T / mw);
-    double mu, lam, rd[NS];
+    gam = cpt / cvt;
+    a = sqrt(gam * kRuD * T / mw);
     d_transport<NS>(m, T, z.bv[4 * z.sz + p], z.bv[p], Yk, cp, mu, lam, rd);
-    z.mul[p] = mu;
-    z.cond[p] = lam;
-#pragma unroll
-    for (int l = 0; l < NS; ++l) z.rhoD[(int64_t)l * z.sz + p] = rd[l];
   } else {
     const double c_temp = kGam * kRuD / kMwAirD;
     const double cp = kGam * kRuD / kMwAirD / (kGam - 1);
-    z.acoustic[p] = sqrt(c_temp * T);
-    const double mu = d_sutherland(T);
-    z.mul[p] = mu;
-    z.cond[p] = mu * cp / P.Pr;
+    a = sqrt(c_temp * T);
+    mu = d_sutherland(T);
+    lam = mu * cp / P.Pr;
   }
-  z.mach[p] = z.vel[p] / z.acoustic[p];
+  for (int kk = PLANE ? -1 : 0; kk <= (PLANE ? 1 : 0); ++kk) {
+    const int64_t q = p + kk * z.d1;
+    if constexpr (NS > 0) {
+      z.cp[q] = cpt;
+      z.gam[q] = gam;
+#pragma unroll
+      for (int l = 0; l < NS; ++l) z.rhoD[(int64_t)l * z.sz + q] = rd[l];
+    }
+    z.acoustic[q] = a;
+    z.mul[q] = mu;
+    z.cond[q] = lam;
+    z.mach[q] = z.vel[q] / a;
+  }
 }
 
 // TimeAdvanceFunc.cu:6-17 store_last_step
@@ -976,14 +985,24 @@ hipError_t launch_eliminate_k(const ZoneView &z, const DParams &P, bool with_cv,
   return hipGetLastError();
 }
 
-hipError_t launch_properties(const ZoneView &z, const DParams &P, const DMech *m, hipStream_t s) {
-  const int64_t n = (int64_t)(z.mx + 2) * (z.my + 2) * (z.mz + 2);
-  if (P.ns == 0)
-    hipLaunchKernelGGL(k_properties<0>, dim3(nblk(n, kTB)), dim3(kTB), 0, s, z, P, m);
-  else if (P.ns == 9)
-    hipLaunchKernelGGL(k_properties<9>, dim3(nblk(n, kTB)), dim3(kTB), 0, s, z, P, m);
-  else
+hipError_t launch_properties(const ZoneView &z, const DParams &P, const DMech *m, bool k_planes_copied,
+                             hipStream_t s) {
+  const bool plane = k_planes_copied && P.dim == 2 && z.mz == 1;
+  const int64_t n = (int64_t)(z.mx + 2) * (z.my + 2) * (plane ? 1 : z.mz + 2);
+  const dim3 g(nblk(n, kTB)), b(kTB);
+  if (P.ns == 0) {
+    if (plane)
+      hipLaunchKernelGGL((k_properties<0, true>), g, b, 0, s, z, P, m);
+    else
+      hipLaunchKernelGGL((k_properties<0, false>), g, b, 0, s, z, P, m);
+  } else if (P.ns == 9) {
+    if (plane)
+      hipLaunchKernelGGL((k_properties<9, true>), g, b, 0, s, z, P, m);
+    else
+      hipLaunchKernelGGL((k_properties<9, false>), g, b, 0, s, z, P, m);
+  } else {
     return hipErrorNotSupported;
+  }
   return hipGetLastError();
 }
 
