@@ -103,6 +103,26 @@ int ostc_download(ostc_solver *s, int blk, int field, double *host);
 /* Device address of a field (for callers that keep their data in HBM). */
 int ostc_field_device_ptr(ostc_solver *s, int blk, int field, double **dptr);
 
+/* A block's state in caller-owned device memory, DZone layout (Field.h:13-71):
+ * ghosted arrays of (mx+2ngg)(my+2ngg)(mz+2ngg) points, component-major,
+ * i fastest; dq without ghosts (mx*my*mz per component). */
+typedef struct ostc_zone_view {
+  int mx, my, mz, ngg, n_var, n_scalar;
+  double *bv; /* 6 components: rho, u, v, w, p, T (required) */
+  double *sv; /* n_scalar components: Y_1..Y_ns, k, omega (required when n_scalar > 0) */
+  double *cv; /* n_var components: rho, rho u, rho v, rho w, rho E, rho Y.., rho k, rho omega (required) */
+  double *qn; /* n_var components: cv at t^n (NULL: the library keeps its own) */
+  double *dq; /* n_var components, no ghosts (NULL: the library keeps its own) */
+} ostc_zone_view;
+/* Binds caller device arrays as block blk's state after ostc_finalize: the
+ * kernels then read and write them in place.  The sizes must match the
+ * block, the setup's ngg, n_var and n_scalar.  The library never frees
+ * caller memory (it frees its own copy of each bound field here); the
+ * arrays must stay valid until ostc_destroy or a new binding.
+ * Replaces: the DZone device pointers a reference block carries
+ * (Field.h:13-71, Field.cu:182-189), which ostc_bind_zone hands over. */
+int ostc_bind_zone(ostc_solver *s, int blk, const ostc_zone_view *view);
+
 /* Uniform initial field from the setup files' default_init inflow.
  * Replaces: initialize_from_start (Initialize.cu:5-47) for groups_init = 1. */
 int ostc_initialize_uniform(ostc_solver *s, int blk);
@@ -119,6 +139,10 @@ int ostc_initialize(ostc_solver *s);
  * Replaces: compute_source (SourceTerm.cuh:11-37), compute_inviscid_flux
  * (SchemeSelector.cuh:30-47), compute_viscous_flux (SchemeSelector.cuh:49-69). */
 int ostc_rhs(ostc_solver *s, int blk);
+/* Local CFL dt of every interior cell of one block (the dt_local field and,
+ * for the steady path, the spectral radii); stream-ordered, no host sync.
+ * Replaces: local_time_step (TimeAdvanceFunc.cuh:50-100). */
+int ostc_local_dt(ostc_solver *s, int blk);
 /* Local CFL dt on every block and the global minimum, returned to the host.
  * Replaces: local_time_step (TimeAdvanceFunc.cuh:50-100) + global_time_step
  * (TimeAdvanceFunc.cu:155-180). */
@@ -128,6 +152,15 @@ int ostc_rk3_stage(ostc_solver *s, int blk, double dt, int rk);
 /* Boundary conditions in the reference priority order.
  * Replaces: DBoundCond::apply_boundary_conditions (BoundCond.cuh:673-822). */
 int ostc_apply_bc(ostc_solver *s, int blk, int with_cv);
+/* Stores rho, |V|, p, T of one block as the reference for the next residual.
+ * Replaces: store_last_step (TimeAdvanceFunc.cu:6-17). */
+int ostc_store_last(ostc_solver *s, int blk);
+/* Residual of a single-block case against the last ostc_store_last:
+ * res[l] = sqrt(sum (q_l - q_l,last)^2 / N) for q = (rho, |V|, p, T), divided
+ * by the values of the first call with step == 1 (floor 1e-20);
+ * err_max = their maximum.  Synchronises; a NaN residual is an error.
+ * Replaces: compute_residual (Residual.cuh:57-161, Residual.cu:49-89). */
+int ostc_residual(ostc_solver *s, int step, double res[4], double *err_max);
 /* Physical properties on [-1, m] (+ 2-D k-copy first when with_k_copy).
  * Replaces: update_physical_properties (FieldOperation.cuh:112-146). */
 int ostc_update_properties(ostc_solver *s, int blk);

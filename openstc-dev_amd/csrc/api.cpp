@@ -107,6 +107,7 @@ struct ostc_solver {
     (void)hipFree(w.dt_hist);
     (void)hipFree(w.res_hist);
     (void)hipFree(w.res_scale);
+    (void)hipFree(w.res_one);
     for (auto &p : pending) {
       (void)hipEventDestroy(p.a);
       (void)hipEventDestroy(p.b);
@@ -1043,6 +1044,7 @@ int ostc_finalize(ostc_solver *s) {
   HIPCHK(hipMemsetAsync(s->w.counter, 0, sizeof(unsigned), s->stream));
   HIPCHK(hipMalloc(&s->w.dt, sizeof(double)));
   HIPCHK(hipMalloc(&s->w.res_scale, sizeof(double) * 4));
+  HIPCHK(hipMalloc(&s->w.res_one, sizeof(double) * 5));
   HIPCHK(hipStreamSynchronize(s->stream));
   s->finalized = true;
   return 0;
@@ -1085,6 +1087,65 @@ int ostc_download(ostc_solver *s, int blk, int field, double *host) {
   if (d.count == 0) return 0;
   HIPCHK(hipMemcpyAsync(host, d.ptr, d.count * sizeof(double), hipMemcpyDeviceToHost, s->stream));
   HIPCHK(hipStreamSynchronize(s->stream));
+  return 0;
+}
+
+int ostc_bind_zone(ostc_solver *s, int blk, const ostc_zone_view *v) {
+  if (check_blk(s, blk)) return 1;
+  if (!v) return fail("bind_zone: null view");
+  Block &b = s->blocks[blk];
+  if (v->mx != b.mx || v->my != b.my || v->mz != b.mz)
+    return fail("bind_zone: view extent " + std::to_string(v->mx) + "x" + std::to_string(v->my) + "x" +
+                std::to_string(v->mz) + " does not match block " + std::to_string(blk));
+  if (v->ngg != s->ngg || v->n_var != s->n_var || v->n_scalar != s->n_scalar)
+    return fail("bind_zone: ngg / n_var / n_scalar differ from the setup (" + std::to_string(s->ngg) + ", " +
+                std::to_string(s->n_var) + ", " + std::to_string(s->n_scalar) + ")");
+  if (!v->bv || !v->cv || (s->n_scalar > 0 && !v->sv)) return fail("bind_zone: bv, cv (and sv) are required");
+  HIPCHK(hipSetDevice(s->device));
+  HIPCHK(hipStreamSynchronize(s->stream));
+  // the library's own array of a bound field is freed (never a caller's)
+  auto rebind = [&](double *&slot, double *caller) -> int {
+    if (!caller || caller == slot) return 0;
+    auto it = std::find(b.mem.begin(), b.mem.end(), (void *)slot);
+    if (it != b.mem.end()) {
+      HIPCHK(hipFree(*it));
+      b.mem.erase(it);
+    }
+    slot = caller;
+    return 0;
+  };
+  if (rebind(b.z.bv, v->bv) || rebind(b.z.cv, v->cv) || (s->n_scalar > 0 && rebind(b.z.sv, v->sv)) ||
+      rebind(b.z.qn, v->qn) || rebind(b.z.dq, v->dq))
+    return 1;
+  return 0;
+}
+
+int ostc_local_dt(ostc_solver *s, int blk) {
+  if (check_blk(s, blk)) return 1;
+  HIPCHK(hipSetDevice(s->device));
+  HIPCHK(launch_local_dt(s->blocks[blk].z, s->P, s->w, s->w.dt, s->stream));
+  return 0;
+}
+
+int ostc_store_last(ostc_solver *s, int blk) {
+  if (check_blk(s, blk)) return 1;
+  HIPCHK(hipSetDevice(s->device));
+  HIPCHK(launch_store_last(s->blocks[blk].z, s->stream));
+  return 0;
+}
+
+int ostc_residual(ostc_solver *s, int step, double res[4], double *err_max) {
+  if (check_blk(s, 0)) return 1;
+  if (s->blocks.size() != 1) return fail("ostc_residual runs single-block cases");
+  HIPCHK(hipSetDevice(s->device));
+  HIPCHK(launch_residual(s->blocks[0].z, s->w, 0, step == 1 ? 1 : 0, s->stream, s->w.res_one));
+  double h[5];
+  HIPCHK(hipMemcpyAsync(h, s->w.res_one, sizeof(h), hipMemcpyDeviceToHost, s->stream));
+  HIPCHK(hipStreamSynchronize(s->stream));
+  if (std::isnan(h[4])) return fail("NaN residual at step " + std::to_string(step));
+  if (res)
+    for (int l = 0; l < 4; ++l) res[l] = h[l];
+  if (err_max) *err_max = h[4];
   return 0;
 }
 

@@ -29,6 +29,7 @@ struct StepScratch {
   double *dt_hist;         // 3 per step
   double *res_hist;        // 5 per step (4 residuals + err_max)
   double *res_scale;       // 4
+  double *res_one;         // 5: the residual of one ostc_residual call
   int max_blocks;
 };
 
@@ -51,8 +52,10 @@ hipError_t launch_properties(const ZoneView &z, const DParams &P, const DMech *m
 hipError_t launch_velocity(const ZoneView &z, hipStream_t s);
 hipError_t launch_cv_from_bv(const ZoneView &z, const DParams &P, const DMech *m, hipStream_t s);
 hipError_t launch_store_last(const ZoneView &z, hipStream_t s);
-// residual of the current step into res_hist[slot*5 ..]; first != 0 sets the scale
-hipError_t launch_residual(const ZoneView &z, const StepScratch &w, int slot, int first, hipStream_t s);
+// residual of the current step into res_hist[slot*5 ..] (or into out when
+// given); first != 0 sets the scale
+hipError_t launch_residual(const ZoneView &z, const StepScratch &w, int slot, int first, hipStream_t s,
+                           double *out = nullptr);
 hipError_t launch_copy_dt(const double *src, double *dst, hipStream_t s);
 hipError_t launch_chem_batch(const DMech *m, int64_t n, const double *rho, const double *T, const double *Y,
                              double dt, double *dY, double *dT, hipStream_t s);

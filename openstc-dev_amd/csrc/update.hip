@@ -1029,13 +1029,14 @@ hipError_t launch_store_last(const ZoneView &z, hipStream_t s) {
   return hipGetLastError();
 }
 
-hipError_t launch_residual(const ZoneView &z, const StepScratch &w, int slot, int first, hipStream_t s) {
+hipError_t launch_residual(const ZoneView &z, const StepScratch &w, int slot, int first, hipStream_t s,
+                           double *out) {
   const int64_t n = (int64_t)z.mx * z.my * z.mz;
   const int nb = (int)std::min<int64_t>(kRedBlocks, nblk(n, kTB));
   if (nb > w.max_blocks) return hipErrorInvalidValue;
   hipLaunchKernelGGL(k_residual, dim3(nb), dim3(kTB), 0, s, z, w.partial);
   hipLaunchKernelGGL(k_residual_final, dim3(1), dim3(kTB), 0, s, (const double *)w.partial, nb, n, w.res_scale,
-                     w.res_hist + (size_t)slot * 5, first);
+                     out ? out : w.res_hist + (size_t)slot * 5, first);
   return hipGetLastError();
 }
 

@@ -63,6 +63,10 @@ def lib():
             "ostc_initialize_uniform": ([vp, C.c_int], C.c_int),
             "ostc_inflow_state": ([vp, C.c_char_p, dp, C.c_int], C.c_int),
             "ostc_initialize": ([vp], C.c_int),
+            "ostc_bind_zone": ([vp, C.c_int, C.POINTER(ZoneView)], C.c_int),
+            "ostc_local_dt": ([vp, C.c_int], C.c_int),
+            "ostc_store_last": ([vp, C.c_int], C.c_int),
+            "ostc_residual": ([vp, C.c_int, dp, dp], C.c_int),
             "ostc_rhs": ([vp, C.c_int], C.c_int),
             "ostc_global_dt": ([vp, dp], C.c_int),
             "ostc_rk3_stage": ([vp, C.c_int, C.c_double, C.c_int], C.c_int),
@@ -124,6 +128,13 @@ def _dp(a):
 
 def _fp(a):
     return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+class ZoneView(C.Structure):
+    """ostc_zone_view: a block's state in caller-owned device memory."""
+    _fields_ = [("mx", C.c_int), ("my", C.c_int), ("mz", C.c_int), ("ngg", C.c_int), ("n_var", C.c_int),
+                ("n_scalar", C.c_int), ("bv", C.c_void_p), ("sv", C.c_void_p), ("cv", C.c_void_p),
+                ("qn", C.c_void_p), ("dq", C.c_void_p)]
 
 
 class DeviceArray:
@@ -247,6 +258,24 @@ class Solver:
 
     def initialize(self):
         _chk(lib().ostc_initialize(self.h))
+
+    def bind_zone(self, mx, my, mz, ngg, n_var, n_scalar, bv, cv, sv=None, qn=None, dq=None, blk=0):
+        """Binds DeviceArray buffers (DZone layout) as block blk's state."""
+        v = ZoneView(mx, my, mz, ngg, n_var, n_scalar, bv.ptr, sv.ptr if sv else None, cv.ptr,
+                     qn.ptr if qn else None, dq.ptr if dq else None)
+        _chk(lib().ostc_bind_zone(self.h, blk, C.byref(v)))
+
+    def local_dt(self, blk=0):
+        _chk(lib().ostc_local_dt(self.h, blk))
+
+    def store_last(self, blk=0):
+        _chk(lib().ostc_store_last(self.h, blk))
+
+    def residual(self, step):
+        res = np.zeros(4)
+        e = C.c_double()
+        _chk(lib().ostc_residual(self.h, step, _dp(res), C.byref(e)))
+        return res, e.value
 
     def rhs(self, blk=0):
         _chk(lib().ostc_rhs(self.h, blk))
