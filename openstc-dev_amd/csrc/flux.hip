@@ -967,10 +967,10 @@ __global__ void __launch_bounds__((64 * JkRows<NS + NT, R>::value)) sweep_jk(Zon
 // -- so a face reads its six stencil rows from LDS instead of six wavefronts
 // loading every row from memory.  Face fluxes pass to the next row as in
 // sweep_jk.
-template <int DIR, typename R>
-__global__ void __launch_bounds__(512) sweep_jk_aweno(ZoneView z, DParams P, const DMech *__restrict__ m, int init,
-                                                      int gx, int nseg) {
-  constexpr int kRows = 8, kRing = kRows + 5;
+template <int DIR, int kRows, typename R>
+__global__ void __launch_bounds__(64 * kRows) sweep_jk_aweno(ZoneView z, DParams P, const DMech *__restrict__ m,
+                                                             int init, int gx, int nseg) {
+  constexpr int kRing = kRows + 5;
   __shared__ R sQ[kRing][6][64];     // cv 0..4, p
   __shared__ R sN[kRing][5][64];     // node flux
   __shared__ R sF[kRows + 2][5][64];  // slots 0, 1: carry; 2 + r: row r
@@ -994,7 +994,9 @@ __global__ void __launch_bounds__(512) sweep_jk_aweno(ZoneView z, DParams P, con
     for (int c = 0; c < 5; ++c) sN[sl][c][lane] = fn[c];
   };
   // the five rows before the first batch's leading rows: c0 - 3 .. c0 + 1
-  if (col && r < 5) stage(L.c0 - 3 + r);
+  // (strided over the wavefronts, so any kRows covers them)
+  if (col)
+    for (int q = r; q < 5; q += kRows) stage(L.c0 - 3 + q);
   for (int b = 0; b < L.nbatch; ++b) {
     const int f = L.c0 - 1 + b * kRows + r;  // this thread's face is f + 1/2
     const bool live = col && f < L.c1;
@@ -1059,6 +1061,12 @@ __global__ void __launch_bounds__(512) sweep_jk_aweno(ZoneView z, DParams P, con
 
 }  // namespace
 
+// wavefronts (face rows) per workgroup of the staged AWENO j/k sweep: 4
+// rows take 81 KB of LDS in FP64, so two independent workgroups share a CU
+// and one computes while the other waits at a barrier (8 rows, one
+// 124 KB workgroup per CU, ran the sweeps 1.3-1.4x slower)
+constexpr int kAwenoRows = 4;
+
 // Segments per line of the j/k sweeps.  With B workgroups of the kernel
 // resident per CU (its occupancy), the sweep takes about
 // ceil(workgroups / (B CUs)) rounds of (faces per segment / rows) batches;
@@ -1100,12 +1108,12 @@ hipError_t launch_sweep_t(const ZoneView &z, const DParams &P, const DMech *m, i
     return hipGetLastError();
   }
   constexpr bool kStaged = SCHEME == 1 && NS == 0 && NT == 0;
-  constexpr int kRows = kStaged ? 8 : JkRows<NS + NT, R>::value;
+  constexpr int kRows = kStaged ? kAwenoRows : JkRows<NS + NT, R>::value;
   const int gx = (z.mx + 63) / 64, gy = dir == 1 ? z.mz : z.my, n = dir == 1 ? z.my : z.mz;
   const int cross = gx * gy;
   const void *fn;
   if constexpr (kStaged)
-    fn = dir == 1 ? (const void *)sweep_jk_aweno<1, R> : (const void *)sweep_jk_aweno<2, R>;
+    fn = dir == 1 ? (const void *)sweep_jk_aweno<1, kRows, R> : (const void *)sweep_jk_aweno<2, kRows, R>;
   else
     fn = dir == 1 ? (const void *)sweep_jk<1, NS, NT, SCHEME, R> : (const void *)sweep_jk<2, NS, NT, SCHEME, R>;
   const int nseg = sweep_segments(fn, 64 * kRows, cross, n, kRows);
@@ -1113,9 +1121,9 @@ hipError_t launch_sweep_t(const ZoneView &z, const DParams &P, const DMech *m, i
   const int in = init ? 1 : 0;
   if constexpr (kStaged) {
     if (dir == 1)
-      hipLaunchKernelGGL((sweep_jk_aweno<1, R>), gr, dim3(64, kRows, 1), 0, s, z, P, m, in, gx, nseg);
+      hipLaunchKernelGGL((sweep_jk_aweno<1, kRows, R>), gr, dim3(64, kRows, 1), 0, s, z, P, m, in, gx, nseg);
     else
-      hipLaunchKernelGGL((sweep_jk_aweno<2, R>), gr, dim3(64, kRows, 1), 0, s, z, P, m, in, gx, nseg);
+      hipLaunchKernelGGL((sweep_jk_aweno<2, kRows, R>), gr, dim3(64, kRows, 1), 0, s, z, P, m, in, gx, nseg);
   } else {
     if (dir == 1)
       hipLaunchKernelGGL((sweep_jk<1, NS, NT, SCHEME, R>), gr, dim3(64, kRows, 1), 0, s, z, P, m, in, gx, nseg);

@@ -214,7 +214,13 @@ __device__ inline bool finish_cell(const DMech *__restrict__ m, const SurNorm &n
   }
   double cj[kSurOut];
 #pragma unroll
-  for (int j = 0; j < kSurOut; ++j) cj[j] = nm.c_scale[j] * sinh(nm.c_mean[j] + nm.c_std[j] * (double)o[j]);
+  for (int j = 0; j < kSurOut; ++j) {
+    // sinh u = (e^u - e^-u) / 2 from one exp: near u = 0 the difference
+    // loses relative digits but its absolute error stays ~1e-16 c_scale,
+    // far below the network's own error
+    const double e = exp(nm.c_mean[j] + nm.c_std[j] * (double)o[j]);
+    cj[j] = nm.c_scale[j] * (0.5 * (e - 1.0 / e));
+  }
   double dy[kNS];
   double amax = 0;
 #pragma unroll
