@@ -55,8 +55,8 @@ def chem_flop_per_cell(labels, ns=9):
 
 
 # kernel behind each timer name of the air case (AWENO-5 + CDS, FP64)
-SWEEP_KERNELS = {"sweep_i": "sweep_i<0, 0, 1, double>", "sweep_j": "sweep_jk_aweno<1, double>",
-                 "sweep_k": "sweep_jk_aweno<2, double>"}
+SWEEP_KERNELS = {"sweep_i": "sweep_i<0, 0, 1, double>", "sweep_j": "sweep_jk_aweno<1, 4, double>",
+                 "sweep_k": "sweep_jk_aweno<2, 4, double>"}
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "air_256x128x128_hbm_traffic.md")
 
 
