@@ -872,6 +872,83 @@ __global__ void __launch_bounds__(256) sweep_i(ZoneView z, DParams P, const DMec
   if (live && lane > 0 && i >= 0) add_dq<NS, NT, R>(z, z.at0(i, j, k), init != 0, F, Fp, i < ncds, G, Gp);
 }
 
+// AWENO i-sweep (air) with the stencil staged in LDS.  Points of a k-plane
+// are numbered row after row, n + 6 per row (i = -3 .. n+2); face f + 1/2
+// of row j (number F = j (n+1) + f + 1 in the face sequence) has its six
+// stencil points at P .. P + 5, P = F + 5 j.  A workgroup's 253 consecutive
+// faces therefore need one contiguous run of points (the rows join without
+// a gap), which its threads stage once -- conservative variables, pressure
+// and the CDS node flux of each point -- before every face reads its stencil
+// and node fluxes from LDS.  Used when rows have at least 63 faces (the run
+// then fits kISlots points).
+constexpr int kISlots = 288;
+
+template <typename R>
+__global__ void __launch_bounds__(256) sweep_i_aweno(ZoneView z, DParams P, const DMech *__restrict__ m, int init,
+                                                     int bpp) {
+  __shared__ R sQ[6][kISlots];  // cv 0..4, p
+  __shared__ R sN[5][kISlots];  // node flux
+  const int lane = threadIdx.x, tid = threadIdx.y * 64 + lane;
+  const int n = z.mx;
+  const unsigned rowf = (unsigned)(n + 1), rowp = (unsigned)(n + 6);
+  const int L = xcd_logical_block(blockIdx.x, gridDim.x);
+  const int k = L / bpp, b = L - k * bpp;
+  const unsigned nf = (unsigned)z.my * rowf;  // faces of the plane
+  const unsigned fo = (unsigned)(b * 4 + threadIdx.y) * 63u + lane;
+  const int j = (int)(fo / rowf);
+  const int i = (int)(fo - (unsigned)j * rowf) - 1;
+  const bool live = fo < nf;
+  // the workgroup's faces [fo0, fo1] and their points [P0, P0 + S)
+  const unsigned fo0 = (unsigned)b * 252u, fo1 = min(fo0 + 252u, nf - 1u);
+  const unsigned P0 = fo0 + 5u * (fo0 / rowf);
+  const unsigned S = fo1 + 5u * (fo1 / rowf) + 6u - P0;
+  for (unsigned q = tid; q < S; q += 256) {
+    const unsigned Pq = P0 + q, jq = Pq / rowp;
+    const int64_t p = z.at((int)(Pq - jq * rowp) - 3, (int)jq, k);
+#pragma unroll
+    for (int c = 0; c < 5; ++c) sQ[c][q] = (R)z.cv[(int64_t)c * z.sz + p];
+    sQ[5][q] = (R)z.bv[4 * z.sz + p];
+    R fn[5];
+    node_flux<R>(z, 0, p, fn);
+#pragma unroll
+    for (int c = 0; c < 5; ++c) sN[c][q] = fn[c];
+  }
+  __syncthreads();
+  const int ncds = n - 2 * z.g;
+  R F[5], G[5];
+#pragma unroll
+  for (int l = 0; l < 5; ++l) F[l] = G[l] = R(0);
+  if (live) {
+    const unsigned t0 = fo + 5u * (unsigned)j - P0;  // slot of the face's first stencil point
+    const int64_t p0 = z.at(i, j, k);
+    R mk0[3], mk1[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      mk0[c] = (R)z.metric[c * z.sz + p0];
+      mk1[c] = (R)z.metric[c * z.sz + p0 + 1];
+    }
+    const R j0 = (R)z.jac[p0], j1 = (R)z.jac[p0 + 1];
+    R cvb[6][6];
+#pragma unroll
+    for (int s = 0; s < 6; ++s) {
+#pragma unroll
+      for (int c = 0; c < 6; ++c) cvb[s][c] = sQ[c][t0 + s];
+#pragma unroll
+      for (int c = 0; c < 5; ++c) G[c] += cds_weight<R>(s) * sN[c][t0 + s];
+    }
+    R pl[7], pr[7];
+    aweno_air(cvb, mk0, mk1, pl, pr);
+    riemann_viscous<0, 0, 1, R>(z, P, m, 0, p0, 1, pl, pr, mk0, mk1, j0, j1, F);
+  }
+  R Fp[5], Gp[5];
+#pragma unroll
+  for (int l = 0; l < 5; ++l) {
+    Fp[l] = __shfl_up(F[l], 1, 64);
+    Gp[l] = __shfl_up(G[l], 1, 64);
+  }
+  if (live && lane > 0 && i >= 0) add_dq<0, 0, R>(z, z.at0(i, j, k), init != 0, F, Fp, i < ncds, G, Gp);
+}
+
 // j- and k-direction sweeps: lanes along i (coalesced); a workgroup of kRows
 // wavefronts marches along one line segment [c0, c1) of the sweep direction
 // in batches of kRows consecutive faces, one face per thread.  A cell's left
@@ -1071,7 +1148,7 @@ constexpr int kAwenoRows = 4;
 // resident per CU (its occupancy), the sweep takes about
 // ceil(workgroups / (B CUs)) rounds of (faces per segment / rows) batches;
 // the fewest segments that minimise that product are taken.
-static int sweep_segments(const void *kernel, int threads, int cross, int n, int rows) {
+static int sweep_segments(int per_cu, int cross, int n, int rows) {
   static int ncu_of[64];
   int dev = 0;
   if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) dev = 0;
@@ -1080,10 +1157,7 @@ static int sweep_segments(const void *kernel, int threads, int cross, int n, int
     if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || v <= 0) v = 256;
     ncu_of[dev] = v;
   }
-  int per_cu = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, threads, 0) != hipSuccess || per_cu <= 0)
-    per_cu = 1;
-  const int64_t slots = (int64_t)ncu_of[dev] * per_cu;
+  const int64_t slots = (int64_t)ncu_of[dev] * std::max(per_cu, 1);
   int best = 1;
   int64_t best_cost = INT64_MAX;
   for (int ns = 1; ns <= std::min(n, 4096); ++ns) {
@@ -1104,6 +1178,14 @@ hipError_t launch_sweep_t(const ZoneView &z, const DParams &P, const DMech *m, i
     const int64_t waves = ((int64_t)z.my * (z.mx + 1) + 62) / 63;
     const int bpp = (int)((waves + 3) / 4);
     const dim3 gr((unsigned)((int64_t)bpp * z.mz));
+    // the staged AWENO i-sweep needs rows of at least 63 faces (its point
+    // run then fits kISlots: 253 faces + 6 + 5 per row boundary, at most 5)
+    if constexpr (SCHEME == 1 && NS == 0 && NT == 0) {
+      if (z.mx + 1 >= 64) {
+        hipLaunchKernelGGL((sweep_i_aweno<R>), gr, dim3(64, 4, 1), 0, s, z, P, m, init ? 1 : 0, bpp);
+        return hipGetLastError();
+      }
+    }
     hipLaunchKernelGGL((sweep_i<NS, NT, SCHEME, R>), gr, dim3(64, 4, 1), 0, s, z, P, m, init ? 1 : 0, bpp);
     return hipGetLastError();
   }
@@ -1111,12 +1193,21 @@ hipError_t launch_sweep_t(const ZoneView &z, const DParams &P, const DMech *m, i
   constexpr int kRows = kStaged ? kAwenoRows : JkRows<NS + NT, R>::value;
   const int gx = (z.mx + 63) / 64, gy = dir == 1 ? z.mz : z.my, n = dir == 1 ? z.my : z.mz;
   const int cross = gx * gy;
-  const void *fn;
-  if constexpr (kStaged)
-    fn = dir == 1 ? (const void *)sweep_jk_aweno<1, kRows, R> : (const void *)sweep_jk_aweno<2, kRows, R>;
-  else
-    fn = dir == 1 ? (const void *)sweep_jk<1, NS, NT, SCHEME, R> : (const void *)sweep_jk<2, NS, NT, SCHEME, R>;
-  const int nseg = sweep_segments(fn, 64 * kRows, cross, n, kRows);
+  // workgroups resident per CU (registers and LDS of the kernel), queried
+  // once per kernel
+  auto occupancy = [](const void *fn) {
+    int v = 0;
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(&v, fn, 64 * kRows, 0) == hipSuccess && v > 0 ? v : 1;
+  };
+  auto kernel = [](int d) -> const void * {
+    if constexpr (kStaged)
+      return d == 1 ? (const void *)sweep_jk_aweno<1, kRows, R> : (const void *)sweep_jk_aweno<2, kRows, R>;
+    else
+      return d == 1 ? (const void *)sweep_jk<1, NS, NT, SCHEME, R> : (const void *)sweep_jk<2, NS, NT, SCHEME, R>;
+  };
+  static const int per_cu_j = occupancy(kernel(1));
+  static const int per_cu_k = occupancy(kernel(2));
+  const int nseg = sweep_segments(dir == 1 ? per_cu_j : per_cu_k, cross, n, kRows);
   const dim3 gr((unsigned)((int64_t)cross * nseg));
   const int in = init ? 1 : 0;
   if constexpr (kStaged) {
