@@ -14,13 +14,15 @@ typedef __attribute__((__vector_size__(4 * sizeof(float)))) float f32x4;
 // u = sqrt(2/pi) (x + 0.044715 x^3); and its derivative.  The logistic
 // factor uses the hardware reciprocal (1 ulp; exp(-2u) = inf gives 0), as
 // the value is rounded to BF16 next; training and inference share it.
-__device__ inline float gelu(float x) {
-  const float u2 = 1.5957691216057308f * (x + 0.044715f * x * x * x);
-  return x * __builtin_amdgcn_rcpf(1.0f + __expf(-u2));
+// exp(-2u) = 2^(-x (a + b x^2)), a = 2 sqrt(2/pi) / ln 2, b = 0.044715 a: one
+// multiply-add and the hardware base-2 exponential
+constexpr float kGeluA = 2.3022082f, kGeluB = 0.10294324f;
+__device__ inline float gelu_logistic(float x) {
+  return __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(-x * __builtin_fmaf(kGeluB, x * x, kGeluA)));
 }
+__device__ inline float gelu(float x) { return x * gelu_logistic(x); }
 __device__ inline float gelu_grad(float x) {
-  const float u2 = 1.5957691216057308f * (x + 0.044715f * x * x * x);
-  const float s = __builtin_amdgcn_rcpf(1.0f + __expf(-u2));
+  const float s = gelu_logistic(x);
   return s + x * s * (1.0f - s) * 1.5957691216057308f * (1.0f + 0.134145f * x * x);
 }
 
