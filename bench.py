@@ -55,7 +55,7 @@ def chem_flop_per_cell(labels, ns=9):
 
 
 # kernel behind each timer name of the air case (AWENO-5 + CDS, FP64)
-SWEEP_KERNELS = {"sweep_i": "sweep_i<0, 0, 1, double>", "sweep_j": "sweep_jk_aweno<1, 4, double>",
+SWEEP_KERNELS = {"sweep_i": "sweep_i_aweno<double>", "sweep_j": "sweep_jk_aweno<1, 4, double>",
                  "sweep_k": "sweep_jk_aweno<2, 4, double>"}
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "air_256x128x128_hbm_traffic.md")
 
@@ -417,7 +417,7 @@ def main():
                            peak_T_rel_err=float(abs(Tn1.max() - Tn0.max()) / Tn0.max()),
                            train_pairs=a.train_pairs, epochs=a.epochs, train_s=t7 - t6, final_loss=float(hist[-1]),
                            gate=sc.surrogate_gate(),
-                           roofline={"bound": "mfma", "kernel": "k_sur_net<128, 1>", "achieved": achieved,
+                           roofline={"bound": "mfma", "kernel": "k_sur_net_pipe<1>", "achieved": achieved,
                                      "peak": BF16_MFMA_PEAK_TFLOPS, "unit": "TFLOP/s",
                                      "frac": achieved / BF16_MFMA_PEAK_TFLOPS,
                                      "flop_per_cell": SURROGATE_FLOP_PER_CELL})

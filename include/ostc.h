@@ -254,8 +254,10 @@ int ostc_surrogate_get_gate(ostc_solver *s, ostc_surrogate_gate *g);
 int ostc_surrogate_norm(ostc_solver *s, double *out);
 int ostc_surrogate_save(ostc_solver *s, const char *path);
 int ostc_surrogate_load(ostc_solver *s, const char *path);
-/* Inference kernel of ostc_chem_batch mode 1: 1 = fused network + gate, 128 cells per
- * workgroup (default); 2 = fused, 64 cells; 0 = one GEMM launch per layer. */
+/* Inference kernel of ostc_chem_batch mode 1: 3 = fused network + gate, 128 cells per
+ * workgroup, the two halves pipelined (matrix products of one half beside the GELU of
+ * the other; default); 1 = the same without the pipeline (identical results);
+ * 2 = fused, 64 cells; 0 = one GEMM launch per layer. */
 int ostc_surrogate_set_kernel(ostc_solver *s, int kernel);
 /* Standardised network outputs out[n x 8] (host) of n device states, without the gate. */
 int ostc_surrogate_network_outputs(ostc_solver *s, int64_t n, const double *rho, const double *T, const double *Y,

@@ -1619,7 +1619,8 @@ int ostc_surrogate_load(ostc_solver *s, const char *path) {
 
 int ostc_surrogate_set_kernel(ostc_solver *s, int kernel) {
   if (need_sur(s)) return 1;
-  if (kernel < 0 || kernel > 2) return fail("kernel must be 0 (layer GEMMs), 1 (fused, 128 cells) or 2 (fused, 64)");
+  if (kernel < 0 || kernel > 3)
+    return fail("kernel must be 0 (layer GEMMs), 1 (fused, 128 cells), 2 (fused, 64) or 3 (fused, 128, pipelined)");
   surrogate_set_kernel(s->sur, kernel);
   return 0;
 }

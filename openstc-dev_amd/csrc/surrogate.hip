@@ -444,6 +444,167 @@ __global__ void __launch_bounds__(256, 2) k_sur_net(const DMech *__restrict__ m,
   }
 }
 
+// k_sur_net with R = 128 and the hidden and output layers pipelined over
+// the two halves (64 cells) of the workgroup: while the MFMAs of one half
+// (or of the next layer's first half) run, the same wave applies the
+// bias + GELU epilogue to the other half's tiles, two tiles per K step, so
+// the matrix cores and the VALU (the GELU's exp and reciprocal) overlap.
+// Cells are independent, so a half needs only its own rows of the previous
+// layer; the halves alternate in the single activation buffer, with one
+// barrier between phases:
+//   M1(0) | M1(1)+E1(0) | M2(0)+E1(1) | M2(1)+E2(0) | O(0)+E2(1) | O(1)
+// (Ml(h): layer-l products of half h, El(h): its epilogue, O: output layer).
+// Every product and epilogue is the one of k_sur_net, in the same order.
+template <int PF>
+__global__ void __launch_bounds__(256, 2) k_sur_net_pipe(const DMech *__restrict__ m, SurNorm nm, int64_t n,
+                                                         int64_t count, const double *__restrict__ rho_a,
+                                                         const double *__restrict__ T_a,
+                                                         const double *__restrict__ Y_a,
+                                                         const __bf16 *__restrict__ wb, int64_t off1, int64_t off2,
+                                                         int64_t off3, float *__restrict__ out) {
+  constexpr int R = 128, RT = 8;
+  __shared__ __bf16 act[R][kHld];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int lr = lane & 15, lk = 8 * (lane >> 4);
+  const int64_t r0 = (int64_t)blockIdx.x * R;
+  const bf16x8 zero8 = {};
+  typedef __attribute__((__vector_size__(4 * sizeof(__bf16)))) __bf16 bf16x4;
+
+  bf16x8 b0[4];
+#pragma unroll
+  for (int b = 0; b < 4; ++b)
+    b0[b] = lk < kLd[0] ? *(const bf16x8 *)(wb + (int64_t)(wave * 64 + b * 16 + lr) * kLd[0] + lk) : zero8;
+  for (int t = tid; t < R; t += 256) {
+    const int64_t c = r0 + t;
+    double x[kSurIn];
+    if (c < count) {
+      double Y[kNS];
+      load_cell(n, c, Y_a, Y);
+      raw_features(m, rho_a[c], T_a[c], Y, x);
+#pragma unroll
+      for (int i = 0; i < kSurIn; ++i) x[i] = (x[i] - nm.x_mean[i]) * nm.x_istd[i];
+    } else {
+#pragma unroll
+      for (int i = 0; i < kSurIn; ++i) x[i] = 0;
+    }
+#pragma unroll
+    for (int i = 0; i < 32; ++i) act[t][i] = (__bf16)(i < kSurIn ? (float)x[i] : 0.f);
+  }
+  __syncthreads();
+
+  f32x4 acc[RT][4];
+  auto load_bias = [&](const __bf16 *W, int ld, int kp, float (&bias)[4][4]) {
+#pragma unroll
+    for (int b = 0; b < 4; ++b)
+#pragma unroll
+      for (int e = 0; e < 4; ++e) bias[b][e] = (float)W[(int64_t)(wave * 64 + b * 16 + 4 * (lane >> 4) + e) * ld + kp];
+  };
+  // epilogue of tile (a, b): bias, GELU, BF16, one 8-byte LDS store
+  auto epi_tile = [&](int a, int b, const float (&bias)[4][4]) {
+    bf16x4 v;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) v[e] = (__bf16)gelu(acc[a][b][e] + bias[b][e]);
+    *(bf16x4 *)&act[a * 16 + lr][wave * 64 + b * 16 + 4 * (lane >> 4)] = v;
+  };
+
+  // layer 0 (one K step) and its epilogue, not pipelined
+#pragma unroll
+  for (int a = 0; a < RT; ++a) {
+    const bf16x8 af = *(const bf16x8 *)&act[a * 16 + lr][lk];
+#pragma unroll
+    for (int b = 0; b < 4; ++b)
+      acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(b0[b], af, f32x4{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
+  }
+  {
+    float bias0[4][4];
+    load_bias(wb, kLd[0], kKpad[0], bias0);
+    __syncthreads();
+#pragma unroll
+    for (int a = 0; a < RT; ++a)
+#pragma unroll
+      for (int b = 0; b < 4; ++b) epi_tile(a, b, bias0);
+    __syncthreads();
+  }
+
+  // layer products of half h (row tiles 4h .. 4h+3), 8 K steps with the
+  // weight fragments prefetched PF steps ahead; with eh >= 0 the 16
+  // epilogue tiles of half eh ride along, two per K step
+  auto mma_half = [&](const __bf16 *W, int h, int eh, const float (&ebias)[4][4]) {
+    const __bf16 *wrow[4];
+#pragma unroll
+    for (int b = 0; b < 4; ++b) wrow[b] = W + (int64_t)(wave * 64 + b * 16 + lr) * kHld + lk;
+    bf16x8 bq[PF + 1][4];
+#pragma unroll
+    for (int q = 0; q < PF; ++q)
+#pragma unroll
+      for (int b = 0; b < 4; ++b) bq[q][b] = *(const bf16x8 *)(wrow[b] + 32 * q);
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+      for (int b = 0; b < 4; ++b) acc[4 * h + a][b] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int ks = 0; ks < 8; ++ks) {
+      if (ks + PF < 8)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) bq[(ks + PF) % (PF + 1)][b] = *(const bf16x8 *)(wrow[b] + 32 * (ks + PF));
+#pragma unroll
+      for (int a = 0; a < 4; ++a) {
+        const bf16x8 af = *(const bf16x8 *)&act[(4 * h + a) * 16 + lr][32 * ks + lk];
+#pragma unroll
+        for (int b = 0; b < 4; ++b)
+          acc[4 * h + a][b] =
+              __builtin_amdgcn_mfma_f32_16x16x32_bf16(bq[ks % (PF + 1)][b], af, acc[4 * h + a][b], 0, 0, 0);
+      }
+      if (eh >= 0) {
+        epi_tile(4 * eh + (2 * ks) / 4, (2 * ks) % 4, ebias);
+        epi_tile(4 * eh + (2 * ks + 1) / 4, (2 * ks + 1) % 4, ebias);
+      }
+    }
+  };
+  // output layer of half h: wave w takes row tile 4h + w (8 outputs, one
+  // 16-wide tile; rows 8..15 of W3 are zero), with the epilogue tiles of
+  // half eh riding along
+  const __bf16 *W3 = wb + off3;
+  auto out_half = [&](int h, int eh, const float (&ebias)[4][4]) {
+    const int a = 4 * h + wave;
+    f32x4 o = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int ks = 0; ks < 8; ++ks) {
+      const bf16x8 bfr = *(const bf16x8 *)&W3[(int64_t)lr * kLd[3] + 32 * ks + lk];
+      const bf16x8 af = *(const bf16x8 *)&act[a * 16 + lr][32 * ks + lk];
+      o = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af, bfr, o, 0, 0, 0);
+      if (eh >= 0) {
+        epi_tile(4 * eh + (2 * ks) / 4, (2 * ks) % 4, ebias);
+        epi_tile(4 * eh + (2 * ks + 1) / 4, (2 * ks + 1) % 4, ebias);
+      }
+    }
+    if (lr < kSurOut) {
+      const float b3 = (float)W3[(int64_t)lr * kLd[3] + kKpad[3]];
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int64_t c = r0 + a * 16 + 4 * (lane >> 4) + e;
+        if (c < count) out[c * kSurOut + lr] = o[e] + b3;
+      }
+    }
+  };
+
+  const __bf16 *W1 = wb + off1, *W2 = wb + off2;
+  float b1[4][4], b2[4][4];
+  load_bias(W1, kHld, 256, b1);
+  load_bias(W2, kHld, 256, b2);
+  mma_half(W1, 0, -1, b1);
+  __syncthreads();  // half 0 of the layer-1 input is read
+  mma_half(W1, 1, 0, b1);
+  __syncthreads();  // layer-1 output of half 0 written, half-1 input read
+  mma_half(W2, 0, 1, b1);
+  __syncthreads();
+  mma_half(W2, 1, 0, b2);
+  __syncthreads();
+  out_half(0, 1, b2);
+  __syncthreads();
+  out_half(1, -1, b2);
+}
+
 inline unsigned nb256(int64_t n) { return (unsigned)((n + 255) / 256); }
 
 uint64_t splitmix(uint64_t &s) {
@@ -479,7 +640,8 @@ struct SurrogateState {
   float *iOut = nullptr;
   int *fb_idx = nullptr, *fb_count = nullptr;
   float *fb_out = nullptr;  // network outputs [fcap x 8] of the fused path
-  int kernel = 1;  // inference: 1 fused network (128 cells per workgroup), 2 (64), 0 layer-by-layer GEMMs
+  int kernel = 3;  // inference: 3 fused network, 128 cells per workgroup, halves pipelined (default);
+                   // 1 the same unpipelined; 2 fused, 64 cells; 0 layer-by-layer GEMMs
   hipEvent_t ev[4] = {};
   double last_ms[3] = {0, 0, 0};  // network, gate, fallback
   int64_t last_fallback = 0;
@@ -1030,6 +1192,9 @@ static int launch_net(SurrogateState *s, const DMech *m, int64_t n, const double
   if (s->kernel == 2)
     hipLaunchKernelGGL((k_sur_net<64, 2>), dim3((unsigned)((n + 63) / 64)), dim3(256), 0, st, m, s->norm, n, n, rho, T, Y,
                        (const __bf16 *)s->wb, s->woff[1], s->woff[2], s->woff[3], out);
+  else if (s->kernel == 3)
+    hipLaunchKernelGGL((k_sur_net_pipe<1>), dim3((unsigned)((n + 127) / 128)), dim3(256), 0, st, m, s->norm, n, n,
+                       rho, T, Y, (const __bf16 *)s->wb, s->woff[1], s->woff[2], s->woff[3], out);
   else
     hipLaunchKernelGGL((k_sur_net<128, 1>), dim3((unsigned)((n + 127) / 128)), dim3(256), 0, st, m, s->norm, n, n, rho,
                        T, Y, (const __bf16 *)s->wb, s->woff[1], s->woff[2], s->woff[3], out);

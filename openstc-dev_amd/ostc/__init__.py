@@ -398,7 +398,8 @@ class Solver:
         _chk(lib().ostc_surrogate_load(self.h, path.encode()))
 
     def surrogate_set_kernel(self, kernel):
-        """1 fused (128 cells per workgroup), 2 fused (64), 0 one GEMM launch per layer."""
+        """3 fused (128 cells per workgroup, pipelined halves; default), 1 the same
+        unpipelined, 2 fused (64), 0 one GEMM launch per layer."""
         _chk(lib().ostc_surrogate_set_kernel(self.h, kernel))
 
     def surrogate_network_outputs(self, n, rho, T, Y):

@@ -158,11 +158,12 @@ def test_fused_network_matches_layer_path(solver, trained):
     n = 5000
     d, _ = _batch(solver, n, seed=606)
     outs = {}
-    for kernel in (0, 1, 2):
+    for kernel in (0, 1, 2, 3):
         solver.surrogate_set_kernel(kernel)
         outs[kernel] = solver.surrogate_network_outputs(n, d["rho"], d["T"], d["Y"])
-    solver.surrogate_set_kernel(1)
-    for kernel in (1, 2):
+    solver.surrogate_set_kernel(3)
+    assert np.array_equal(outs[3], outs[1])  # the pipelined kernel reorders nothing
+    for kernel in (1, 2, 3):
         e = relerr(outs[kernel], outs[0])
         print("fused kernel %d vs layer path: rel. error %.2e" % (kernel, e), flush=True)
         assert np.all(np.isfinite(outs[kernel]))
