@@ -258,15 +258,34 @@ __global__ void __launch_bounds__(kTB) k_velocity(ZoneView z) {
 }
 
 // ------------------------------------------------------------- BCs
-// BoundCond.cuh:52-552: one thread per face point of a registered range,
-// writing the face point (where the reference does) and its ngg ghosts.
+// BoundCond.cuh:52-552: the face point of a registered range (where the
+// reference writes it) and its ngg ghosts.  A ghost layer depends only on
+// interior points, the untouched face point or constants, so each of the
+// ngg + 1 layers of a point gets its own thread (layer 0: the face point).
+// Farfield, subsonic inflow and the SST wall derive their ghosts from the
+// face point's new values and keep one thread per point (layer 0 walks all).
+// Lanes run along the face's fastest index: the layer for i-faces (ghosts
+// adjacent in memory), the point for j- and k-faces.
 template <int NS, bool WITH_CV>
 __global__ void __launch_bounds__(kTB) k_bc(ZoneView z, DParams P, const DMech *__restrict__ m, BCDesc b) {
   const int nx = b.re[0] - b.rs[0] + 1, ny = b.re[1] - b.rs[1] + 1, nz = b.re[2] - b.rs[2] + 1;
-  const int64_t t = (int64_t)blockIdx.x * kTB + threadIdx.x;
-  if (t >= (int64_t)nx * ny * nz) return;
+  const uint32_t npt = (uint32_t)nx * ny * nz, nl = z.g + 1;
+  const uint32_t t = blockIdx.x * kTB + threadIdx.x;
+  if (t >= npt * nl) return;
+  uint32_t pt, gl;
+  if (b.face == 0) {
+    pt = t / nl;
+    gl = t - pt * nl;
+  } else {
+    gl = t / npt;
+    pt = t - gl * npt;
+  }
+  const bool par = !(b.type == 4 || b.type == 7 || (b.type == 2 && P.nt));
+  if (!par && gl != 0) return;
+  const bool face = !par || gl == 0;                          // this thread writes the face point
+  const int glo = par ? (int)gl : 1, ghi = par ? (int)gl : z.g;  // ghost layers of this thread
   int i, j, k;
-  unflatten3(t, nx, ny, i, j, k);
+  unflatten3(pt, nx, ny, i, j, k);
   i += b.rs[0];
   j += b.rs[1];
   k += b.rs[2];
@@ -286,22 +305,24 @@ __global__ void __launch_bounds__(kTB) k_bc(ZoneView z, DParams P, const DMech *
     kx /= km;
     ky /= km;
     kz /= km;
-    const double u1 = bv[S + pi], v1 = bv[2 * S + pi], w1 = bv[3 * S + pi];
-    double uk = kx * u1 + ky * v1 + kz * w1;
-    const double ut = u1 - kx * uk, vt = v1 - ky * uk, wt = w1 - kz * uk;
-    bv[S + p] = ut;
-    bv[2 * S + p] = vt;
-    bv[3 * S + p] = wt;
-    z.vel[p] = sqrt(ut * ut + vt * vt + wt * wt);
-    bv[p] = bv[pi];
-    bv[4 * S + p] = bv[4 * S + pi];
-    bv[5 * S + p] = bv[5 * S + pi];
-    for (int l = 0; l < P.nsc; ++l) sv[l * S + p] = sv[l * S + pi];
-    for (int g = 1; g <= z.g; ++g) {
+    if (face) {
+      const double u1 = bv[S + pi], v1 = bv[2 * S + pi], w1 = bv[3 * S + pi];
+      const double uk = kx * u1 + ky * v1 + kz * w1;
+      const double ut = u1 - kx * uk, vt = v1 - ky * uk, wt = w1 - kz * uk;
+      bv[S + p] = ut;
+      bv[2 * S + p] = vt;
+      bv[3 * S + p] = wt;
+      z.vel[p] = sqrt(ut * ut + vt * vt + wt * wt);
+      bv[p] = bv[pi];
+      bv[4 * S + p] = bv[4 * S + pi];
+      bv[5 * S + p] = bv[5 * S + pi];
+      for (int l = 0; l < P.nsc; ++l) sv[l * S + p] = sv[l * S + pi];
+    }
+    for (int g = max(glo, 1); g <= ghi; ++g) {
       const int64_t pg = p + g * sd, pm = p - g * sd;
       bv[pg] = bv[pm];
       const double u = bv[S + pm], v = bv[2 * S + pm], w = bv[3 * S + pm];
-      uk = kx * u + ky * v + kz * w;
+      const double uk = kx * u + ky * v + kz * w;
       const double ug = u - 2 * uk * kx, vg = v - 2 * uk * ky, wg = w - 2 * uk * kz;
       bv[S + pg] = ug;
       bv[2 * S + pg] = vg;
@@ -314,7 +335,7 @@ __global__ void __launch_bounds__(kTB) k_bc(ZoneView z, DParams P, const DMech *
     }
   } else if (b.type == 6) {
     // outflow (BoundCond.cuh:120-148)
-    for (int g = 1; g <= z.g; ++g) {
+    for (int g = max(glo, 1); g <= ghi; ++g) {
       const int64_t pg = p + g * sd;
 #pragma unroll
       for (int l = 0; l < 6; ++l) bv[l * S + pg] = bv[l * S + p];
@@ -324,7 +345,7 @@ __global__ void __launch_bounds__(kTB) k_bc(ZoneView z, DParams P, const DMech *
     }
   } else if (b.type == 5) {
     // inflow (BoundCond.cuh:150-212)
-    for (int g = 0; g <= z.g; ++g) {
+    for (int g = par ? glo : 0; g <= ghi; ++g) {
       const int64_t pg = p + g * sd;
       bv[pg] = b.rho;
       bv[S + pg] = b.u;
@@ -498,7 +519,7 @@ __global__ void __launch_bounds__(kTB) k_bc(ZoneView z, DParams P, const DMech *
     }
   } else if (b.type == 9) {
     // back pressure (BoundCond.cuh:638-670)
-    for (int g = 1; g <= z.g; ++g) {
+    for (int g = max(glo, 1); g <= ghi; ++g) {
       const int64_t pg = p + g * sd;
 #pragma unroll
       for (int l = 0; l < 4; ++l) bv[l * S + pg] = bv[l * S + p];
@@ -512,42 +533,44 @@ __global__ void __launch_bounds__(kTB) k_bc(ZoneView z, DParams P, const DMech *
     // no-slip wall (BoundCond.cuh:424-552)
     const int64_t pi = p - sd;
     double tw = b.isothermal ? b.t_wall : bv[5 * S + pi];
-    const double pw = bv[4 * S + pi];
     double mw = kMwAirD;
-    if constexpr (NS > 0) {
-      double s = 0;
-#pragma unroll
-      for (int l = 0; l < NS; ++l) {
-        const double y = sv[l * S + pi];
-        sv[l * S + p] = y;
-        s += y / m->mw[l];
-      }
-      mw = 1 / s;
-    }
-    const double rw = pw * mw / (tw * kRuD);
-    bv[p] = rw;
-    bv[S + p] = 0;
-    bv[2 * S + p] = 0;
-    bv[3 * S + p] = 0;
-    bv[4 * S + p] = pw;
-    bv[5 * S + p] = tw;
-    z.vel[p] = 0;
-    if (P.nt) {
-      // SST wall values: k = 0, omega = 60 mu_w / (rho_w beta1 d^2) with the
-      // wall distance of the first interior point
-      double mu_w = d_sutherland(tw);
+    if (face) {
+      const double pw = bv[4 * S + pi];
       if constexpr (NS > 0) {
-        double Yw[NS];
+        double s = 0;
 #pragma unroll
-        for (int l = 0; l < NS; ++l) Yw[l] = sv[l * S + p];
-        mu_w = d_viscosity<NS>(m, tw, mw, Yw);
+        for (int l = 0; l < NS; ++l) {
+          const double y = sv[l * S + pi];
+          sv[l * S + p] = y;
+          s += y / m->mw[l];
+        }
+        mw = 1 / s;
       }
-      const double dy = z.wdist[pi];
-      sv[NS * S + p] = 0;
-      sv[(NS + 1) * S + p] = dy > 1e-25 ? 60 * mu_w / (rw * sst::kBeta1 * dy * dy) : sv[(NS + 1) * S + pi];
+      const double rw = pw * mw / (tw * kRuD);
+      bv[p] = rw;
+      bv[S + p] = 0;
+      bv[2 * S + p] = 0;
+      bv[3 * S + p] = 0;
+      bv[4 * S + p] = pw;
+      bv[5 * S + p] = tw;
+      z.vel[p] = 0;
+      if (P.nt) {
+        // SST wall values: k = 0, omega = 60 mu_w / (rho_w beta1 d^2) with the
+        // wall distance of the first interior point
+        double mu_w = d_sutherland(tw);
+        if constexpr (NS > 0) {
+          double Yw[NS];
+#pragma unroll
+          for (int l = 0; l < NS; ++l) Yw[l] = sv[l * S + p];
+          mu_w = d_viscosity<NS>(m, tw, mw, Yw);
+        }
+        const double dy = z.wdist[pi];
+        sv[NS * S + p] = 0;
+        sv[(NS + 1) * S + p] = dy > 1e-25 ? 60 * mu_w / (rw * sst::kBeta1 * dy * dy) : sv[(NS + 1) * S + pi];
+      }
+      if (WITH_CV) cv_point<NS>(z, m, p, P.nt);
     }
-    if (WITH_CV) cv_point<NS>(z, m, p, P.nt);
-    for (int g = 1; g <= z.g; ++g) {
+    for (int g = max(glo, 1); g <= ghi; ++g) {
       const int64_t pa = p - g * sd, pg = p + g * sd;
       const double ui = bv[S + pa], vi = bv[2 * S + pa], wi = bv[3 * S + pa], pii = bv[4 * S + pa],
                    ti = bv[5 * S + pa];
@@ -959,7 +982,8 @@ hipError_t launch_bc(const ZoneView &z, const DParams &P, const DMech *m, const 
                      hipStream_t s) {
   const int64_t n = (int64_t)(bc.re[0] - bc.rs[0] + 1) * (bc.re[1] - bc.rs[1] + 1) * (bc.re[2] - bc.rs[2] + 1);
   if (n <= 0) return hipSuccess;
-  const dim3 g(nblk(n, kTB)), b(kTB);
+  if (n * (z.g + 1) >= (int64_t)1 << 32) return hipErrorInvalidValue;
+  const dim3 g(nblk(n * (z.g + 1), kTB)), b(kTB);
   if (P.ns == 0) {
     if (with_cv)
       hipLaunchKernelGGL((k_bc<0, true>), g, b, 0, s, z, P, m, bc);
