@@ -17,7 +17,12 @@
  *
  * Parity pins (how each part is checked without trusting this file) are
  * listed in tests/test_oracle_*.py; routines no independent check reaches
- * are marked "parity unpinned" in their comment.
+ * are marked "parity unpinned" in their comment.  Terms that no check reaches
+ * inside otherwise pinned routines: the mixture heat-conduction and
+ * species-diffusion terms of the viscous flux (the shear-stress and u . tau
+ * work terms are pinned by plane Couette shear), and in the SST source the
+ * cross-diffusion term, F1 near a wall and the production limiter's active
+ * branch (decay, production and the far-field constants are pinned).
  */
 #ifndef OSTC_ORACLE_H
 #define OSTC_ORACLE_H

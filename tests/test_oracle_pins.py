@@ -14,7 +14,10 @@ own data fix, independent of the oracle's code:
   rules, Sutherland at its reference point;
 * fluxes: consistency F(U,U) = physical flux, supersonic upwinding, WENO5
   exact on quadratics and 5th-order convergent, L R = I of the AWENO
-  eigenvectors, SSP-RK3 stability polynomial, uniform flow preserved.
+  eigenvectors, SSP-RK3 stability polynomial, uniform flow preserved;
+* viscous flux: plane Couette shear, whose interior right-hand side is fixed
+  in closed form (viscous dissipation mu a^2 heats the gas, nothing else
+  changes), for air and the 9-species mixture.
 """
 import ctypes as C
 
@@ -528,3 +531,62 @@ def test_farfield_and_subsonic_inflow_keep_a_uniform_far_state():
     # back-pressure ghosts (i >= mx): pressure prescribed, the rest extrapolated
     assert np.allclose(bv[4, k, g:-g, -g:], 0.97 * far["p"], rtol=1e-14)
     assert np.allclose(bv[0, k, g:-g, -g:], far["rho"], rtol=1e-14)
+
+
+@pytest.mark.parametrize("variant", ["air_aweno", "mixture_muscl_hllc"])
+def test_couette_shear_dissipation(variant, mech):
+    """Plane Couette shear u = a (y - y0), v = w = 0 at uniform rho, p, T (and
+    composition) on a Cartesian grid.  Exact solution of the flux balance in
+    the interior: no mass, momentum or species change (the inviscid face
+    fluxes are (0, 0, p, 0, 0) on every j-face, the viscous stress tau_xy =
+    mu a is uniform, no heat or species diffusion), and the viscous
+    dissipation alone heats the gas: d(rho E)/dt = mu a^2, so the energy row of
+    the right-hand side equals mu a^2 times the cell volume.  Pins the signs,
+    indices and the u . tau term of the viscous flux (air and mixture
+    transport), independent of the oracle's formulas."""
+    import helpers
+    d = 0.01 if variant == "air_aweno" else 1e-4
+    if variant == "air_aweno":
+        case = cases.c2_case(20, 14, 12, perturb=False)
+        case["x"], case["y"], case["z"] = np.meshgrid(np.arange(12) * d, np.arange(14) * d, np.arange(20) * d,
+                                                      indexing="ij")[::-1]
+        ns, m = 0, None
+        Y = np.zeros(0)
+        mw = cases.MW_AIR
+        T, p = 300.0, 2.0e4
+    else:
+        case = cases.c3_case(24, 16, dx=d)
+        case["scheme"]["reaction"] = 0
+        ns, m = 9, mech
+        Y = np.zeros(ns)
+        Y[2], Y[8] = 0.233, 0.767   # O2, N2
+        mw = 1.0 / sum(Y[l] / mech.mw[l] for l in range(ns))
+        T, p = 300.0, 101325.0
+    for b in case["bcs"]:
+        b["type"] = 6
+    mx, my, mz, g = case["mx"], case["my"], case["mz"], case["ngg"]
+    shp = cases.ghost_shape(mx, my, mz, g)
+    rho = p * mw / (R_U * T)
+    a = 2.0e3 if variant == "air_aweno" else 2.0e5       # 40 m/s across the zone
+    yj = (np.arange(shp[1]) - g) * d
+    bv = np.empty((6,) + shp)
+    bv[0], bv[2], bv[3], bv[4], bv[5] = rho, 0.0, 0.0, p, T
+    bv[1] = a * (yj - 0.5 * (my - 1) * d)[None, :, None]
+    sv = np.empty((ns,) + shp)
+    for l in range(ns):
+        sv[l] = Y[l]
+    z = helpers.oracle_zone(case, mech=m, bv=bv, sv=sv)
+    z.rhs()
+    nv = 5 + ns
+    dq = z.field("dq").reshape(nv, mz, my, mx)
+    mu = z.field("mul").reshape(shp)
+    jac = z.field("jac").reshape(shp)
+    inner = (slice(4, mz - 4) if mz > 1 else slice(0, 1), slice(4, my - 4), slice(4, mx - 4))
+    ginner = tuple(slice(s.start + g, s.stop + g) for s in inner)
+    mu_c, vol = mu[ginner], jac[ginner]
+    assert np.ptp(mu_c) <= 1e-15 * mu_c.max()
+    heat = mu_c * a * a * vol
+    assert np.allclose(dq[4][inner], heat, rtol=1e-9, atol=0), (dq[4][inner] / heat).ravel()[:4]
+    tol = 1e-9 * heat.max()
+    for l in (0, 1, 2, 3) + tuple(range(5, nv)):
+        assert np.abs(dq[l][inner]).max() < tol, (l, np.abs(dq[l][inner]).max(), tol)

@@ -10,6 +10,8 @@ the mathematics fixes, independent of the oracle's code:
   mu_t = rho k / omega, dq_k = -beta* rho k omega J, dq_omega = -beta2 rho
   omega^2 J (F1 -> 0 far from any wall), and the flow rows stay zero;
 * the wall value omega_w = 60 mu_w / (rho_w beta1 d1^2) and k_w = 0;
+* plane Couette shear: Menter's production mu_t |vorticity|^2, the k-epsilon
+  constants far from walls and the (mu + mu_t) a^2 dissipation in closed form;
 * the laminar limit: with k = 0 the flow rows of the turbulent right-hand
   side equal the laminar ones bit for bit (mu_t = 0, lambda_t = 0, no
   -2/3 rho k), and the k row vanishes.
@@ -136,3 +138,49 @@ def test_laminar_limit():
     # mu_t vanishes wherever it was computed ([-1, m] at initialisation, the interior by the source)
     assert not zt.field("mut").reshape(_shape(case))[g, g - 1:g + my + 1, g - 1:g + mx + 1].any()
     np.testing.assert_array_equal(dqt[5], 0.0)
+
+
+def test_couette_shear_production():
+    """Plane Couette shear u = a (y - y0) with uniform rho, p, T, k, omega on
+    a uniform Cartesian grid and no wall (F1, F2 -> 0, so the k-epsilon set
+    gamma2, beta2 applies and mu_t = rho k / omega).  |vorticity| = a, so
+    Menter's production is P_k = mu_t a^2 (below its 10 beta* rho k omega
+    limit here) and, with grad k = 0 (no diffusion, no cross-diffusion):
+    dq_k = (mu_t a^2 - beta* rho k omega) J and dq_omega = (gamma2 rho a^2 -
+    beta2 rho omega^2) J, gamma2 = beta2 / beta* - sigma_w2 kappa^2 /
+    sqrt(beta*) (Menter 1994, set 2).  The total stress (mu + mu_t) a is
+    uniform, so the momentum rows vanish, and the dissipation heats the gas:
+    dq_E = (mu + mu_t) a^2 J.  Pins the production, the blended constants, the
+    eddy-viscosity stress and the u . tau work of the SST path."""
+    case = cases.sst_plate_case(nx=20, ny=16, compressibility=3)
+    for b in case["bcs"]:
+        b["type"] = 6
+    mx, my, g, d = case["mx"], case["my"], case["ngg"], 0.01
+    X, Y = np.meshgrid(np.arange(mx) * d, np.arange(my) * d, indexing="xy")
+    case["x"], case["y"], case["z"] = X[None].copy(), Y[None].copy(), np.zeros((1, my, mx))
+    shp = _shape(case)
+    T, p, a, k, w = 300.0, 2.0e4, 2.0e3, 1.0, 1.0e4
+    rho = p * MW / (R_U * T)
+    bv = np.empty((6,) + shp)
+    bv[0], bv[2], bv[3], bv[4], bv[5] = rho, 0.0, 0.0, p, T
+    bv[1] = a * ((np.arange(shp[1]) - g) * d - 0.5 * (my - 1) * d)[None, :, None]
+    sv = np.empty((2,) + shp)
+    sv[0], sv[1] = k, w
+    z = _zone(case, bv, sv)
+    z.rhs()
+    inner = (slice(4, my - 4), slice(4, mx - 4))
+    gin = (g,) + tuple(slice(s.start + g, s.stop + g) for s in inner)
+    J = z.field("jac").reshape(shp)[gin]
+    mut = z.field("mut").reshape(shp)[gin]
+    mu = cases.sutherland(T)
+    dq = z.field("dq").reshape(7, my, mx)[(slice(None),) + inner]
+    np.testing.assert_allclose(mut, rho * k / w, rtol=1e-12)
+    beta_s, beta2, sigma_w2, kappa = 0.09, 0.0828, 0.856, 0.41
+    gamma2 = beta2 / beta_s - sigma_w2 * kappa**2 / np.sqrt(beta_s)
+    mut0 = rho * k / w
+    assert mut0 * a * a < 10 * beta_s * rho * k * w        # production limiter inactive
+    np.testing.assert_allclose(dq[5], (mut0 * a * a - beta_s * rho * k * w) * J, rtol=1e-9)
+    np.testing.assert_allclose(dq[6], (gamma2 * rho * a * a - beta2 * rho * w * w) * J, rtol=1e-9)
+    heat = (mu + mut0) * a * a * J
+    np.testing.assert_allclose(dq[4], heat, rtol=1e-9)
+    assert np.abs(dq[:4]).max() < 1e-9 * heat.max()
