@@ -669,6 +669,7 @@ int ostc_init(const char *case_dir, int device, ostc_solver **out) {
   P.mix = species ? 1 : 0;
   P.reaction = reaction;
   P.chemSrcMethod = p.get_int("chemSrcMethod", 0);
+  P.nr = s->mech.n_reac;
   P.inviscid_scheme = p.get_int("inviscid_scheme", 3);
   P.reconstruction = p.get_int("reconstruction", 2);
   P.limiter = p.get_int("limiter", 0);
@@ -1430,7 +1431,7 @@ int ostc_chem_batch(ostc_solver *s, int64_t n, const double *rho, const double *
   HIPCHK(hipSetDevice(s->device));
   if (mode == 0) {
     ostc_solver::Timer t(s, "chem_batch_fp64");
-    HIPCHK(launch_chem_batch(s->d_mech, n, rho, T, Y, dt, dY, dT, s->stream));
+    HIPCHK(launch_chem_batch(s->d_mech, s->mech.n_reac, n, rho, T, Y, dt, dY, dT, s->stream));
     if (n_fallback) *n_fallback = 0;
   } else if (mode == 1) {
     if (!s->sur || !surrogate_ready(s->sur)) return fail("no trained surrogate: train or load one first");
@@ -1476,7 +1477,7 @@ int ostc_chem_step(ostc_solver *s, int blk, double dt, int mode, int64_t *n_fall
     ostc_solver::Timer t(s, "chem_step");
     HIPCHK(launch_chem_gather(b.z, s->P, rho, T, Y, s->stream));
     if (mode == 0) {
-      HIPCHK(launch_chem_batch(s->d_mech, n, rho, T, Y, dt, dY, dT, s->stream));
+      HIPCHK(launch_chem_batch(s->d_mech, s->mech.n_reac, n, rho, T, Y, dt, dY, dT, s->stream));
     } else {
       std::string err;
       if (surrogate_infer(s->sur, s->d_mech, n, rho, T, Y, dt, dY, dT, &nf, s->stream, err)) return fail(err);

@@ -39,6 +39,7 @@ struct DParams {
   double cfl, Pr, Sc, Prt, Sct;
   double efix;  // Roe entropy-fix factor
   int turb_implicit, compress;  // SST: implicit source diagonal; compressibility correction 0-3
+  int nr;                       // reactions of the mechanism (0 without chemistry)
 };
 
 // (a, b, c) of linear index t over an a-fastest na x nb x nc box, in 32-bit

@@ -57,7 +57,7 @@ hipError_t launch_store_last(const ZoneView &z, hipStream_t s);
 hipError_t launch_residual(const ZoneView &z, const StepScratch &w, int slot, int first, hipStream_t s,
                            double *out = nullptr);
 hipError_t launch_copy_dt(const double *src, double *dst, hipStream_t s);
-hipError_t launch_chem_batch(const DMech *m, int64_t n, const double *rho, const double *T, const double *Y,
+hipError_t launch_chem_batch(const DMech *m, int nr, int64_t n, const double *rho, const double *T, const double *Y,
                              double dt, double *dY, double *dT, hipStream_t s);
 // ---- steady path (implicit.hip)
 // point-implicit chemistry on dq with the local dt (EPI or DA per chemSrcMethod)
@@ -87,7 +87,7 @@ hipError_t launch_turb_implicit(const ZoneView &z, const DParams &P, hipStream_t
 hipError_t launch_epi_sigma(const DMech *m, int64_t n, const double *rho, const double *T, const double *Y, double dt,
                             double *sig, hipStream_t s);
 // the same for the cells idx[0..count) of arrays of stride n (idx values < n)
-hipError_t launch_chem_batch_subset(const DMech *m, int64_t n, int64_t count, const int *idx, const double *rho,
+hipError_t launch_chem_batch_subset(const DMech *m, int nr, int64_t n, int64_t count, const int *idx, const double *rho,
                                     const double *T, const double *Y, double dt, double *dY, double *dT,
                                     hipStream_t s);
 

@@ -624,6 +624,7 @@ struct SurrogateState {
   SurrogateGate gate{};
   double x_min[kSurIn]{}, x_max[kSurIn]{};
   bool ready = false;
+  int n_reac = 0;  // reactions of the mechanism (sizes the integrator's LDS park)
   double dt_train = 0;
   int64_t woff[kSurLayers]{}, nw = 0;
   float *w = nullptr, *g = nullptr, *m1 = nullptr, *m2 = nullptr;
@@ -839,6 +840,7 @@ int surrogate_create(const Mechanism &mech, int device, SurrogateState **out, st
   }
   auto s = std::make_unique<SurrogateState>();
   s->device = device;
+  s->n_reac = mech.n_reac;
   SCHK(hipSetDevice(device));
   // conservation rows: mass fraction of each element per unit dY_k, then mass
   const int ne = (int)mech.elements.size();
@@ -1065,7 +1067,7 @@ int surrogate_train(SurrogateState *s, const DMech *m, int64_t n, const double *
   double *dY = nullptr, *dTt = nullptr;
   SCHK(dalloc(&dY, n * kNS));
   SCHK(dalloc(&dTt, n));
-  SCHK(launch_chem_batch(m, n, rho, T, Y, dt, dY, dTt, st));
+  SCHK(launch_chem_batch(m, s->n_reac, n, rho, T, Y, dt, dY, dTt, st));
   double *sig = nullptr;
   SCHK(dalloc(&sig, n));
   SCHK(launch_epi_sigma(m, n, rho, T, Y, dt, sig, st));
@@ -1287,7 +1289,7 @@ int surrogate_infer(SurrogateState *s, const DMech *m, int64_t n, const double *
     err = "fallback count out of range";
     return 1;
   }
-  SCHK(launch_chem_batch_subset(m, n, nf, s->fb_idx, rho, T, Y, dt, dY, dT, st));
+  SCHK(launch_chem_batch_subset(m, s->n_reac, n, nf, s->fb_idx, rho, T, Y, dt, dY, dT, st));
   SCHK(hipEventRecord(s->ev[2], st));
   SCHK(hipEventSynchronize(s->ev[2]));
   float a = 0, b = 0, c = 0;

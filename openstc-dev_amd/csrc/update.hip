@@ -37,13 +37,15 @@ __device__ inline double wave_sum(double v) {
 // With accumulate (the SST kernel already initialised dq) the species rows
 // are added to and the flow rows left alone.  The EPI Jacobian is built in a
 // second pass over the rates of progress parked in LDS (one column per
-// thread, no barrier).
+// thread, no barrier).  The park is dynamic LDS of 2 n_reac rows of kSrcTB
+// doubles, launched only with the EPI Jacobian: its size sets how many
+// workgroups share a CU (4 of them for 19 reactions, 2 waves per SIMD).
 constexpr int kSrcTB = 128;
-constexpr int kSrcMaxR = 24;
+constexpr int kParkMaxReac = 24;  // reactions per cell the LDS park holds
 
 template <int NS>
 __global__ void __launch_bounds__(kSrcTB) k_source(ZoneView z, DParams P, const DMech *__restrict__ m, int accumulate) {
-  __shared__ double sq[2 * kSrcMaxR][kSrcTB];
+  extern __shared__ double sq_park[];
   const int64_t t = (int64_t)blockIdx.x * kSrcTB + threadIdx.x;
   const int64_t n = (int64_t)z.mx * z.my * z.mz;
   if (t >= n) return;
@@ -56,8 +58,8 @@ __global__ void __launch_bounds__(kSrcTB) k_source(ZoneView z, DParams P, const 
   const double rho = z.bv[p], T = z.bv[5 * z.sz + p], J = z.jac[p];
   if (P.chemSrcMethod == 1) {
     double jac[NS * NS];
-    d_chem_rates<NS>(m, rho, T, Y, om, &sq[0][threadIdx.x], kSrcTB);
-    d_epi_matrix<NS, false>(m, rho, Y, 0.0, &sq[0][threadIdx.x], kSrcTB, jac);
+    d_chem_rates<NS>(m, rho, T, Y, om, sq_park + threadIdx.x, kSrcTB);
+    d_epi_matrix<NS, false>(m, rho, Y, 0.0, sq_park + threadIdx.x, kSrcTB, jac);
 #pragma unroll
     for (int c = 0; c < NS * NS; ++c) z.chem_jac[(int64_t)c * z.sz0 + q] = jac[c];
   } else if (P.chemSrcMethod == 2) {
@@ -784,16 +786,16 @@ __global__ void k_copy1(const double *src, double *dst) { *dst = *src; }
 // With idx, thread t handles cell idx[t] for t < count; otherwise cell t < n.
 // The rates of progress are parked in LDS between the source pass and the
 // Jacobian pass (d_chem_rates / d_epi_matrix), one column per thread: no
-// thread reads another's column, so the kernel has no barrier.
+// thread reads another's column, so the kernel has no barrier.  The park is
+// dynamic LDS of 2 n_reac rows of kChemTB doubles.
 constexpr int kChemTB = 128;
-constexpr int kChemMaxR = 24;  // reactions the LDS park holds
 
 template <int NS>
 __global__ void __launch_bounds__(kChemTB) k_chem_batch(const DMech *__restrict__ m, int64_t n, int64_t count,
                                                         const int *__restrict__ idx, const double *__restrict__ rho_a,
                                                         const double *__restrict__ T_a, const double *__restrict__ Y_a,
                                                         double dt, double *__restrict__ dY, double *__restrict__ dT) {
-  __shared__ double sq[2 * kChemMaxR][kChemTB];
+  extern __shared__ double sq_park[];
   const int64_t t = (int64_t)blockIdx.x * kChemTB + threadIdx.x;
   if (t >= count) return;
   const int64_t c = idx ? (int64_t)idx[t] : t;
@@ -812,8 +814,8 @@ __global__ void __launch_bounds__(kChemTB) k_chem_batch(const DMech *__restrict_
   for (int l = 0; l < NS; ++l) rhoE += h[l] * Y[l];
   rhoE *= rho;
   rhoE -= p_old;
-  d_chem_rates<NS>(m, rho, T, Y, om, &sq[0][threadIdx.x], kChemTB);
-  d_epi_matrix<NS>(m, rho, Y, dt, &sq[0][threadIdx.x], kChemTB, J);
+  d_chem_rates<NS>(m, rho, T, Y, om, sq_park + threadIdx.x, kChemTB);
+  d_epi_matrix<NS>(m, rho, Y, dt, sq_park + threadIdx.x, kChemTB, J);
   d_lu_solve<NS>(J, om);
   const double ri = 1.0 / rho;
   double Yn[NS];
@@ -950,7 +952,9 @@ hipError_t launch_chem_scatter(const ZoneView &z, const DParams &P, const DMech 
 hipError_t launch_source(const ZoneView &z, const DParams &P, const DMech *m, bool accumulate, hipStream_t s) {
   const int64_t n = (int64_t)z.mx * z.my * z.mz;
   if (P.ns != 9) return hipErrorNotSupported;
-  hipLaunchKernelGGL(k_source<9>, dim3(nblk(n, kSrcTB)), dim3(kSrcTB), 0, s, z, P, m, accumulate ? 1 : 0);
+  if (P.chemSrcMethod == 1 && (P.nr <= 0 || P.nr > kParkMaxReac)) return hipErrorInvalidValue;
+  const size_t park = P.chemSrcMethod == 1 ? (size_t)2 * P.nr * kSrcTB * sizeof(double) : 0;
+  hipLaunchKernelGGL(k_source<9>, dim3(nblk(n, kSrcTB)), dim3(kSrcTB), park, s, z, P, m, accumulate ? 1 : 0);
   return hipGetLastError();
 }
 
@@ -1069,10 +1073,12 @@ hipError_t launch_copy_dt(const double *src, double *dst, hipStream_t s) {
   return hipGetLastError();
 }
 
-hipError_t launch_chem_batch(const DMech *m, int64_t n, const double *rho, const double *T, const double *Y, double dt,
-                             double *dY, double *dT, hipStream_t s) {
+hipError_t launch_chem_batch(const DMech *m, int nr, int64_t n, const double *rho, const double *T, const double *Y,
+                             double dt, double *dY, double *dT, hipStream_t s) {
   if (n <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_chem_batch<9>, dim3(nblk(n, kChemTB)), dim3(kChemTB), 0, s, m, n, n, (const int *)nullptr, rho,
+  if (nr <= 0 || nr > kParkMaxReac) return hipErrorInvalidValue;
+  const size_t park = (size_t)2 * nr * kChemTB * sizeof(double);
+  hipLaunchKernelGGL(k_chem_batch<9>, dim3(nblk(n, kChemTB)), dim3(kChemTB), park, s, m, n, n, (const int *)nullptr, rho,
                      T, Y, dt, dY, dT);
   return hipGetLastError();
 }
@@ -1084,12 +1090,13 @@ hipError_t launch_epi_sigma(const DMech *m, int64_t n, const double *rho, const 
   return hipGetLastError();
 }
 
-hipError_t launch_chem_batch_subset(const DMech *m, int64_t n, int64_t count, const int *idx, const double *rho,
-                                    const double *T, const double *Y, double dt, double *dY, double *dT,
-                                    hipStream_t s) {
+hipError_t launch_chem_batch_subset(const DMech *m, int nr, int64_t n, int64_t count, const int *idx,
+                                    const double *rho, const double *T, const double *Y, double dt, double *dY,
+                                    double *dT, hipStream_t s) {
   if (count <= 0) return hipSuccess;
-  if (count > n) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(k_chem_batch<9>, dim3(nblk(count, kChemTB)), dim3(kChemTB), 0, s, m, n, count, idx, rho, T, Y,
+  if (count > n || nr <= 0 || nr > kParkMaxReac) return hipErrorInvalidValue;
+  const size_t park = (size_t)2 * nr * kChemTB * sizeof(double);
+  hipLaunchKernelGGL(k_chem_batch<9>, dim3(nblk(count, kChemTB)), dim3(kChemTB), park, s, m, n, count, idx, rho, T, Y,
                      dt, dY, dT);
   return hipGetLastError();
 }
