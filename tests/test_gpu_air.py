@@ -101,6 +101,37 @@ def test_c2_small_rhs_and_steps(tmp_path):
     assert relerr(dt_p, dt_o) < 1e-9
 
 
+@pytest.mark.parametrize("shape", [(63, 16, 10), (130, 20, 12)])
+def test_c2_staged_i_sweep(tmp_path, shape):
+    """Rows of 64 and more faces (mx + 1 >= 64) take the i-sweep that stages
+    one contiguous run of points per workgroup in LDS (rows joined in the
+    numbering P = F + 5 j): RHS, one step and 10 steps against the oracle,
+    and the FP32-flux variant of the same kernel within the mixed bound."""
+    case = cases.c2_case(*shape)
+    s = _solver(case, tmp_path)
+    z = oracle_zone(case)
+    s.rhs()
+    z.rhs()
+    errs = grouped(s.download("dq"), z.field("dq"), 5, CV_GROUPS)
+    print("c2 staged", shape, "rhs err", {k: "%.2e" % v for k, v in errs.items()}, flush=True)
+    assert max(errs.values()) < 1e-12
+    s.step(1)
+    z.run(1)
+    _compare_state(s, z, case, 1e-12, "c2-staged-1step")
+    dt_p, _ = s.step(9)
+    dt_o, _ = z.run(9)
+    _compare_state(s, z, case, 1e-9, "c2-staged-10step")
+    assert relerr(dt_p, dt_o) < 1e-9
+    f = _solver(case, tmp_path / "fp32")
+    f.set_flux_fp32(True)
+    f.step(10)
+    bv_p, bv_o = f.download("bv").reshape(6, -1), z.field("bv").reshape(6, -1)
+    for name, comps in BV_GROUPS.items():
+        linf = np.max(np.abs(bv_p[comps] - bv_o[comps])) / np.max(np.abs(bv_o[comps]))
+        print(f"c2 staged fp32 {name}: Linf {linf:.2e}", flush=True)
+        assert linf < 1e-3
+
+
 def test_c2_small_mixed_precision_error(tmp_path):
     """FP32 flux arithmetic against the FP64 oracle: per-field L2 / Linf
     relative errors after 10 steps are reported and bounded."""
