@@ -450,7 +450,7 @@ def main():
     cpu = None
     if not a.no_cpu:
         log("bench: CPU baseline (serial oracle)")
-        v, sample, secs = cpu_baseline_flow(lambda: cases.c2_case(128, 64, 64), steps=3)
+        v, sample, secs = cpu_baseline_flow(lambda: cases.c2_case(128, 64, 64), steps=7)
         cpu = {"value": v, "unit": "Mcell-updates/s", "cores": 1, "kind": "port",
                "sample": f"FP64 serial oracle, {sample} of the same scheme/boundaries ({secs:.1f} s)"}
         if chem is not None:
