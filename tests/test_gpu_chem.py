@@ -42,3 +42,23 @@ def test_chem_batch_parity(chem_solver, mech):
     assert eY < 1e-11
     assert eT < 1e-11
     assert np.all(np.isfinite(gY)) and np.all(np.isfinite(gT))
+
+
+@pytest.mark.parametrize("n", [1, 129, 1000])
+def test_chem_batch_ragged_sizes(chem_solver, mech, n):
+    """Batch sizes that leave a partial workgroup (the kernel runs 128 cells
+    per workgroup): every cell matches the oracle; n = 0 is a no-op."""
+    s = chem_solver
+    rho, T, p, Y = cases.chem_batch_inputs(n, seed=2000 + n)
+    dt = 1e-7
+    d = {k: ostc.DeviceArray.from_host(s, v) for k, v in dict(rho=rho, T=T, Y=Y).items()}
+    dY = ostc.DeviceArray(s, Y.nbytes)
+    dT = ostc.DeviceArray(s, T.nbytes)
+    assert s.chem_batch(n, d["rho"], d["T"], d["Y"], dt, dY, dT, mode=0) == 0
+    gY, gT = dY.to_host(Y.shape), dT.to_host(T.shape)
+    oY, oT = po.chem_batch(mech, rho, T, Y, dt)
+    print("chem batch n=%d dY err %.2e dT err %.2e" % (n, relerr(gY, oY), relerr(gT, oT)), flush=True)
+    assert relerr(gY, oY) < 1e-11 and relerr(gT, oT) < 1e-11
+    # an empty batch leaves the outputs alone
+    assert s.chem_batch(0, d["rho"], d["T"], d["Y"], dt, dY, dT, mode=0) == 0
+    assert np.array_equal(dT.to_host(T.shape), gT)
