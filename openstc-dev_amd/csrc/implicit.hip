@@ -230,18 +230,15 @@ __global__ void k_zero_dq_range(ZoneView z, int nv, int i0, int i1, int j0, int 
   for (int l = 0; l < nv; ++l) z.dq[(int64_t)l * z.sz0 + q] = 0;
 }
 
-__global__ void k_copy(const double *__restrict__ src, double *__restrict__ dst, int64_t n) {
-  const int64_t t = (int64_t)blockIdx.x * kTB + threadIdx.x;
-  if (t < n) dst[t] = src[t];
-}
-
 // DPLUR.cu:5-17 convert_dq_back_to_dqDt
-__global__ void k_dq_back(ZoneView z, int nv) {
+// src: the array holding the last inner sweep's result (dq or dqk), so an
+// odd sweep count needs no copy back into dq first.
+__global__ void k_dq_back(ZoneView z, const double *__restrict__ src, int nv) {
   int i, j, k;
   if (!cell_index(z, i, j, k)) return;
   const int64_t q = z.at0(i, j, k);
   const double dt = z.dt_local[q];
-  for (int l = 0; l < nv; ++l) z.dq[(int64_t)l * z.sz0 + q] /= dt;
+  for (int l = 0; l < nv; ++l) z.dq[(int64_t)l * z.sz0 + q] = src[(int64_t)l * z.sz0 + q] / dt;
 }
 
 // FieldOperation.cuh:221-296 update_bv: rho, momentum, total energy and
@@ -363,9 +360,8 @@ hipError_t launch_dplur(const ZoneView &z, const DParams &P, const DMech *m, int
         hipLaunchKernelGGL((k_dplur_inner<9, 2>), g, bk, 0, s, zi, m, fr, P.chemSrcMethod, timp);
     }
   }
-  if (inner_steps & 1)
-    hipLaunchKernelGGL(k_copy, dim3(nb(z.sz0 * nv)), dim3(kTB), 0, s, (const double *)z.dqk, z.dq, z.sz0 * nv);
-  hipLaunchKernelGGL(k_dq_back, dim3(nb(n)), dim3(kTB), 0, s, z, nv);
+  hipLaunchKernelGGL(k_dq_back, dim3(nb(n)), dim3(kTB), 0, s, z, (const double *)((inner_steps & 1) ? z.dqk : z.dq),
+                     nv);
   return hipGetLastError();
 }
 
