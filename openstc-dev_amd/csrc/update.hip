@@ -264,8 +264,10 @@ __global__ void __launch_bounds__(kTB) k_velocity(ZoneView z) {
 // reference writes it) and its ngg ghosts.  A ghost layer depends only on
 // interior points, the untouched face point or constants, so each of the
 // ngg + 1 layers of a point gets its own thread (layer 0: the face point).
-// Farfield, subsonic inflow and the SST wall derive their ghosts from the
-// face point's new values and keep one thread per point (layer 0 walks all).
+// Farfield and subsonic inflow derive their ghosts from the face point's new
+// values and keep one thread per point (layer 0 walks all); the SST wall's
+// ghost omega is the face value, which each layer's thread recomputes from
+// the first interior point exactly as the face thread does.
 // Lanes run along the face's fastest index: the layer for i-faces (ghosts
 // adjacent in memory), the point for j- and k-faces.
 template <int NS, bool WITH_CV>
@@ -282,7 +284,7 @@ __global__ void __launch_bounds__(kTB) k_bc(ZoneView z, DParams P, const DMech *
     gl = t / npt;
     pt = t - gl * npt;
   }
-  const bool par = !(b.type == 4 || b.type == 7 || (b.type == 2 && P.nt));
+  const bool par = !(b.type == 4 || b.type == 7);
   if (!par && gl != 0) return;
   const bool face = !par || gl == 0;                          // this thread writes the face point
   const int glo = par ? (int)gl : 1, ghi = par ? (int)gl : z.g;  // ghost layers of this thread
@@ -534,21 +536,34 @@ __global__ void __launch_bounds__(kTB) k_bc(ZoneView z, DParams P, const DMech *
   } else if (b.type == 2) {
     // no-slip wall (BoundCond.cuh:424-552)
     const int64_t pi = p - sd;
-    double tw = b.isothermal ? b.t_wall : bv[5 * S + pi];
+    const double tw = b.isothermal ? b.t_wall : bv[5 * S + pi];
+    const double pw = bv[4 * S + pi];
     double mw = kMwAirD;
-    if (face) {
-      const double pw = bv[4 * S + pi];
-      if constexpr (NS > 0) {
-        double s = 0;
+    double Yw[NS > 0 ? NS : 1];
+    if constexpr (NS > 0) {
+      double s = 0;
 #pragma unroll
-        for (int l = 0; l < NS; ++l) {
-          const double y = sv[l * S + pi];
-          sv[l * S + p] = y;
-          s += y / m->mw[l];
-        }
-        mw = 1 / s;
+      for (int l = 0; l < NS; ++l) {
+        Yw[l] = sv[l * S + pi];
+        s += Yw[l] / m->mw[l];
       }
-      const double rw = pw * mw / (tw * kRuD);
+      mw = 1 / s;
+    }
+    const double rw = pw * mw / (tw * kRuD);
+    // SST wall values: k = 0, omega = 60 mu_w / (rho_w beta1 d^2) with the
+    // wall distance of the first interior point
+    double om_w = 0;
+    if (P.nt) {
+      double mu_w = d_sutherland(tw);
+      if constexpr (NS > 0) mu_w = d_viscosity<NS>(m, tw, mw, Yw);
+      const double dy = z.wdist[pi];
+      om_w = dy > 1e-25 ? 60 * mu_w / (rw * sst::kBeta1 * dy * dy) : sv[(NS + 1) * S + pi];
+    }
+    if (face) {
+      if constexpr (NS > 0) {
+#pragma unroll
+        for (int l = 0; l < NS; ++l) sv[l * S + p] = Yw[l];
+      }
       bv[p] = rw;
       bv[S + p] = 0;
       bv[2 * S + p] = 0;
@@ -557,18 +572,8 @@ __global__ void __launch_bounds__(kTB) k_bc(ZoneView z, DParams P, const DMech *
       bv[5 * S + p] = tw;
       z.vel[p] = 0;
       if (P.nt) {
-        // SST wall values: k = 0, omega = 60 mu_w / (rho_w beta1 d^2) with the
-        // wall distance of the first interior point
-        double mu_w = d_sutherland(tw);
-        if constexpr (NS > 0) {
-          double Yw[NS];
-#pragma unroll
-          for (int l = 0; l < NS; ++l) Yw[l] = sv[l * S + p];
-          mu_w = d_viscosity<NS>(m, tw, mw, Yw);
-        }
-        const double dy = z.wdist[pi];
         sv[NS * S + p] = 0;
-        sv[(NS + 1) * S + p] = dy > 1e-25 ? 60 * mu_w / (rw * sst::kBeta1 * dy * dy) : sv[(NS + 1) * S + pi];
+        sv[(NS + 1) * S + p] = om_w;
       }
       if (WITH_CV) cv_point<NS>(z, m, p, P.nt);
     }
@@ -599,7 +604,7 @@ __global__ void __launch_bounds__(kTB) k_bc(ZoneView z, DParams P, const DMech *
       bv[5 * S + pg] = tg;
       if (P.nt) {
         sv[NS * S + pg] = 0;
-        sv[(NS + 1) * S + pg] = sv[(NS + 1) * S + p];
+        sv[(NS + 1) * S + pg] = om_w;
         z.mut[pg] = 0;
       }
       if (WITH_CV) cv_point<NS>(z, m, pg, P.nt);
