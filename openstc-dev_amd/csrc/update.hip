@@ -991,7 +991,7 @@ hipError_t launch_bc(const ZoneView &z, const DParams &P, const DMech *m, const 
                      hipStream_t s) {
   const int64_t n = (int64_t)(bc.re[0] - bc.rs[0] + 1) * (bc.re[1] - bc.rs[1] + 1) * (bc.re[2] - bc.rs[2] + 1);
   if (n <= 0) return hipSuccess;
-  if (n * (z.g + 1) >= (int64_t)1 << 32) return hipErrorInvalidValue;
+  if (n * (z.g + 1) > ((int64_t)1 << 32) - kTB) return hipErrorInvalidValue;  // 32-bit thread index
   const dim3 g(nblk(n * (z.g + 1), kTB)), b(kTB);
   if (P.ns == 0) {
     if (with_cv)
