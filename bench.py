@@ -170,6 +170,70 @@ def cpu_baseline_c5(nx=96, ny=48, nz=32, iters=2):
     return cells * iters / (t1 - t0) / 1e6, f"{nx}x{ny}x{nz} grid, {iters} pseudo-iterations", t1 - t0
 
 
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
+def cpu_baseline(a, d_chem=True, d_c5=True):
+    """The oracle built for this host (-O3 -march=native, no FMA contraction)
+    and timed pinned to one core on bounded samples of the workloads; the
+    chemistry batch is also timed on all the cores this process may use
+    (OpenMP build, in a child process)."""
+    import subprocess
+    import pyoracle
+    from ostc import cases
+    share = sorted(os.sched_getaffinity(0))
+    threads = min(len(share), int(os.environ.get("OMP_NUM_THREADS", len(share)) or len(share)))
+    info = {"cores": 1, "kind": "port", "cpu_model": cpu_model(), "cpus_available": len(share)}
+    with tempfile.TemporaryDirectory() as bd:
+        try:
+            serial = pyoracle.build_native(bd)
+            omp = pyoracle.build_native(bd, openmp=True)
+            pyoracle.use_library(serial)
+            info["build"] = "gcc -O3 -march=native -ffp-contract=off, FP64, serial"
+        except (OSError, subprocess.CalledProcessError) as e:
+            omp = None
+            info["build"] = f"prebuilt oracle (-O3, no -march): native build failed ({e})"
+        os.sched_setaffinity(0, {share[0]})
+        try:
+            log(f"bench: CPU baseline (oracle pinned to CPU {share[0]})")
+            v, sample, secs = cpu_baseline_flow(lambda: cases.c2_case(128, 64, 64), steps=7)
+            info.update(value=v, unit="Mcell-updates/s",
+                        sample=f"FP64 serial oracle, {sample} of the same scheme/boundaries ({secs:.1f} s)")
+            v1, s1, t1 = cpu_baseline_flow(lambda: cases.c1_case(200, 100), steps=100)
+            info["c1_mcell_per_s"] = v1
+            info["c1_sample"] = f"all 100 steps of the 200x100 case ({t1:.1f} s)"
+            if d_chem:
+                cv_, csample, csecs = cpu_baseline_chem()
+                info["chem_cells_per_s"] = cv_
+                info["chem_sample"] = f"{csample} ({csecs:.1f} s)"
+            if d_c5:
+                v5, s5s, t5s = cpu_baseline_c5()
+                info["c5_mcell_iter_per_s"] = v5
+                info["c5_sample"] = f"FP64 serial oracle, {s5s} of the channel ({t5s:.1f} s)"
+        finally:
+            os.sched_setaffinity(0, set(share))
+        if d_chem and omp is not None and threads > 1:
+            log(f"bench: CPU baseline, chemistry batch on {threads} threads")
+            env = dict(os.environ, OSTC_ORACLE_LIB=omp, OMP_NUM_THREADS=str(threads))
+            r = subprocess.run([sys.executable, os.path.abspath(__file__), "--cpu-chem-worker", str(1 << 22)],
+                               env=env, capture_output=True, text=True)
+            if r.returncode == 0:
+                w = json.loads(r.stdout.strip().splitlines()[-1])
+                info["chem_allcore"] = {"cells_per_s": w["cells_per_s"], "threads": threads,
+                                        "sample": f"{w['sample']} ({w['seconds']:.1f} s), OpenMP over cells"}
+            else:
+                info["chem_allcore"] = {"error": r.stderr.strip()[-300:]}
+    return info
+
+
 def time_steps(s, steps, warmup):
     s.step(warmup)
     s.sync()
@@ -188,6 +252,8 @@ def main():
     ap.add_argument("--ny", type=int, default=128)
     ap.add_argument("--nz", type=int, default=128)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-c1", action="store_true")
+    ap.add_argument("--cpu-chem-worker", type=int, default=0, help=argparse.SUPPRESS)
     ap.add_argument("--no-c3", action="store_true")
     ap.add_argument("--no-c5", action="store_true")
     ap.add_argument("--no-split", action="store_true")
@@ -200,6 +266,12 @@ def main():
     ap.add_argument("--calibrate-copy", type=int, default=0,
                     help="bytes of one device-to-device copy run at the end (HBM counter calibration)")
     a = ap.parse_args()
+    if a.cpu_chem_worker > 0:
+        # child process of the CPU baseline: the oracle named by
+        # OSTC_ORACLE_LIB on all the threads OMP_NUM_THREADS allows
+        v, sample, secs = cpu_baseline_chem(a.cpu_chem_worker)
+        print(json.dumps({"cells_per_s": v, "sample": sample, "seconds": secs}))
+        return
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build runs on one GPU; multi-GPU sharding is described in DESIGN.md "
               "and not launched here", file=sys.stderr)
@@ -217,20 +289,46 @@ def main():
         s.step(a.warmup)
         s.sync()
         log(f"bench: timing {a.steps} FP64 steps")
-        s.profile(True)
         t0 = time.perf_counter()
         dt_h, res_h = s.step(a.steps)
         s.sync()
         t1 = time.perf_counter()
+        fp64_s = t1 - t0
+        # per-kernel HIP-event times from a separate, profiled run (the
+        # events would otherwise sit inside the timed region above)
+        prof_steps = max(2, min(a.steps, 10))
+        log(f"bench: profiled run of {prof_steps} FP64 steps (per-kernel times)")
+        s.profile(True)
+        s.step(prof_steps)
         prof = s.profile_report()
         s.profile(False)
-        fp64_s = t1 - t0
         # FP32-flux variant on the same state path
         s.set_flux_fp32(True)
         log(f"bench: timing {a.steps} mixed-precision steps")
         mixed_s = time_steps(s, a.steps, a.warmup)
         finite = bool(np.isfinite(res_h).all())
         s.close()
+
+        c1 = None
+        if not a.no_c1:
+            # the reference's own 2-D case, all 100 steps from the seeded
+            # initial field (after an untimed warm-up run from the same field)
+            log("bench: 2-D air 200x100, 100 RK3 steps")
+            case1 = cases.c1_case(200, 100)
+            s1 = ostc.solver_for_case(case1, os.path.join(d, "c1"))
+            bv1, _ = cases.initial_fields(case1)
+            s1.step(5)
+            s1.upload("bv", bv1)
+            s1.initialize()
+            t0 = time.perf_counter()
+            _, res1 = s1.step(100)
+            s1.sync()
+            t1 = time.perf_counter()
+            s1.close()
+            c1 = {"workload": "2-D inviscid air 200x100 Cartesian, AUSM+ with MUSCL-minmod, SSP-RK3 CFL 0.5, "
+                              "Mach 2 inflow / outflow / symmetry, 100 steps from the seeded initial field",
+                  "mcell_per_s": 200 * 100 * 100 / (t1 - t0) / 1e6, "ms_per_step": (t1 - t0) / 100 * 1e3,
+                  "residual_finite": bool(np.isfinite(res1[:, 4]).all())}
 
         c3 = None
         if not a.no_c3:
@@ -449,18 +547,7 @@ def main():
 
     cpu = None
     if not a.no_cpu:
-        log("bench: CPU baseline (serial oracle)")
-        v, sample, secs = cpu_baseline_flow(lambda: cases.c2_case(128, 64, 64), steps=7)
-        cpu = {"value": v, "unit": "Mcell-updates/s", "cores": 1, "kind": "port",
-               "sample": f"FP64 serial oracle, {sample} of the same scheme/boundaries ({secs:.1f} s)"}
-        if chem is not None:
-            cv_, csample, csecs = cpu_baseline_chem()
-            cpu["chem_cells_per_s"] = cv_
-            cpu["chem_sample"] = f"{csample} ({csecs:.1f} s)"
-        if c5 is not None:
-            v5, s5s, t5s = cpu_baseline_c5()
-            cpu["c5_mcell_iter_per_s"] = v5
-            cpu["c5_sample"] = f"FP64 serial oracle, {s5s} of the channel ({t5s:.1f} s)"
+        cpu = cpu_baseline(a, d_chem=chem is not None, d_c5=c5 is not None)
 
     out = {
         "metric": "Mcell-updates/s per RK3 step; chem cells/s surrogate vs point-implicit",
@@ -482,6 +569,7 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "algorithmic_bytes_per_launch": algo_bytes, "avg_launch_ms": avg_ms},
         "cpu_baseline": cpu,
+        "c1_air_2d": c1,
         "mixed_fp32_flux": {"value": cells * a.steps / mixed_s / 1e6, "unit": "Mcell-updates/s",
                             "ms_per_step": mixed_s / a.steps * 1e3},
         "reacting_mixing_layer": c3,

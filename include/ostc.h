@@ -167,6 +167,10 @@ int ostc_update_properties(ostc_solver *s, int blk);
 /* `nsteps` full SSP-RK3 steps with the dt recomputed on the device every
  * stage and the residual every step, no host synchronisation inside.
  * dt_hist (3*nsteps) and res_hist (5*nsteps: 4 residuals, err_max) may be NULL.
+ * The device histories are kept between calls (reallocated only when a call
+ * asks for more steps than any before).  After a failed call (nonzero
+ * status) the block state is undefined: some stages may have run, and the
+ * step counter and physical time are not advanced.
  * Replaces: RK3_bv (RK.cuh:19-195) without IO/monitors. */
 int ostc_step_rk3(ostc_solver *s, int nsteps, double *dt_hist, double *res_hist);
 /* The implicit treatment of the current dq of one block with the local dt
@@ -178,7 +182,8 @@ int ostc_step_rk3(ostc_solver *s, int nsteps, double *dt_hist, double *res_hist)
 int ostc_implicit(ostc_solver *s, int blk);
 /* `nsteps` steady iterations with local time steps: RHS, local dt, implicit
  * treatment, update_bv, boundary conditions (no cv), properties, residual
- * (res_hist 5*nsteps, NaN where not computed).
+ * (res_hist 5*nsteps, NaN where not computed).  As for ostc_step_rk3, the
+ * block state is undefined after a failed call.
  * Replaces: steady_simulation (SteadySim.cuh:15-135) without IO, update_bv
  * (FieldOperation.cuh:221-296). */
 int ostc_step_steady(ostc_solver *s, int nsteps, double *res_hist);

@@ -605,10 +605,20 @@ int properties_block(ostc_solver *s, Block &b, bool k_planes_copied) {
   return 0;
 }
 
+// Device dt / residual histories for a run of nsteps steps: kept in the
+// solver's scratch and only reallocated when a longer run asks for more.
 int ensure_hist(ostc_solver *s, int nsteps) {
-  static thread_local int cap = 0;
-  (void)cap;
-  return nsteps < 0 ? fail("negative step count") : 0;
+  if (nsteps < 0) return fail("negative step count");
+  if (nsteps <= s->w.hist_cap) return 0;
+  HIPCHK(hipStreamSynchronize(s->stream));
+  (void)hipFree(s->w.dt_hist);
+  (void)hipFree(s->w.res_hist);
+  s->w.dt_hist = s->w.res_hist = nullptr;
+  s->w.hist_cap = 0;
+  HIPCHK(hipMalloc(&s->w.dt_hist, sizeof(double) * 3 * nsteps));
+  HIPCHK(hipMalloc(&s->w.res_hist, sizeof(double) * 5 * nsteps));
+  s->w.hist_cap = nsteps;
+  return 0;
 }
 
 }  // namespace
@@ -682,6 +692,12 @@ int ostc_init(const char *case_dir, int device, ostc_solver **out) {
   P.Sc = p.get_real("schmidt_number", 0.9);
   P.Prt = p.get_real("turbulent_prandtl_number", 0.9);
   P.Sct = p.get_real("turbulent_schmidt_number", 0.9);
+  // AWENO.cu:86: the characteristic interpolation reconstructs only for
+  // reconstruction 4 (AWENO-5); reconstruction 5 (WENO7) selects the same
+  // kernel in the reference but is not that scheme
+  if (P.inviscid_type == 1 && P.reconstruction != 4)
+    return fail("reconstruction " + std::to_string(P.reconstruction) +
+                " with the characteristic (AWENO) path is not supported: only reconstruction 4 (AWENO-5)");
   if (P.inviscid_type == 1 && P.mix) return fail("AWENO for multi-species mixtures has no reference definition");
   if (P.inviscid_type == 1 && n_turb)
     return fail("the AWENO characteristic reconstruction does not carry k and omega (AWENO.cu:53-54)");
@@ -1257,16 +1273,13 @@ int ostc_update_properties(ostc_solver *s, int blk) {
 
 int ostc_step_rk3(ostc_solver *s, int nsteps, double *dt_hist, double *res_hist) {
   if (check_blk(s, 0)) return 1;
-  if (ensure_hist(s, nsteps)) return 1;
+  if (nsteps < 0) return fail("negative step count");
   if (nsteps == 0) return 0;
   HIPCHK(hipSetDevice(s->device));
   if (s->blocks.size() != 1) return fail("the fused RK3 driver runs single-block cases");
+  if (ensure_hist(s, nsteps)) return 1;
   Block &b = s->blocks[0];
-  double *d_dt = nullptr, *d_res = nullptr;
-  HIPCHK(hipMalloc(&d_dt, sizeof(double) * 3 * nsteps));
-  HIPCHK(hipMalloc(&d_res, sizeof(double) * 5 * nsteps));
-  s->w.dt_hist = d_dt;
-  s->w.res_hist = d_res;
+  double *d_dt = s->w.dt_hist, *d_res = s->w.res_hist;
   const bool fixed = s->prm.get_bool("fixed_time_step", false);
   const double fixed_dt = s->prm.get_real("dt", 1e-5);
   const int os = std::max(1, s->prm.get_int("output_screen", 1));
@@ -1309,11 +1322,13 @@ int ostc_step_rk3(ostc_solver *s, int nsteps, double *dt_hist, double *res_hist)
     if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
     if (e != hipSuccess) rc = fail(std::string("step_rk3: ") + hipGetErrorString(e));
   }
-  (void)hipFree(d_dt);
-  (void)hipFree(d_res);
-  s->w.dt_hist = nullptr;
-  s->w.res_hist = nullptr;
-  if (rc) return rc;
+  if (rc) {
+    // the block state is partly advanced; timing events of the launches
+    // that were queued are released with the rest
+    s->drain_profile();
+    return rc;
+  }
+  s->drain_profile();
   for (int it = 0; it < nsteps; ++it) {
     const int step = s->step + 1 + it;
     s->t_phys += hdt[it * 3 + 2];
@@ -1325,7 +1340,6 @@ int ostc_step_rk3(ostc_solver *s, int nsteps, double *dt_hist, double *res_hist)
   s->step += nsteps;
   if (dt_hist) std::copy(hdt.begin(), hdt.end(), dt_hist);
   if (res_hist) std::copy(hres.begin(), hres.end(), res_hist);
-  s->drain_profile();
   return 0;
 }
 
@@ -1358,10 +1372,9 @@ int ostc_step_steady(ostc_solver *s, int nsteps, double *res_hist) {
   if (nsteps == 0) return 0;
   HIPCHK(hipSetDevice(s->device));
   if (s->blocks.size() != 1) return fail("the steady driver runs single-block cases");
+  if (ensure_hist(s, nsteps)) return 1;
   Block &b = s->blocks[0];
-  double *d_res = nullptr;
-  HIPCHK(hipMalloc(&d_res, sizeof(double) * 5 * nsteps));
-  s->w.res_hist = d_res;
+  double *d_res = s->w.res_hist;
   const int os = std::max(1, s->prm.get_int("output_screen", 1));
   auto body = [&]() -> int {
     for (int it = 0; it < nsteps; ++it) {
@@ -1394,9 +1407,11 @@ int ostc_step_steady(ostc_solver *s, int nsteps, double *res_hist) {
     if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
     if (e != hipSuccess) rc = fail(std::string("step_steady: ") + hipGetErrorString(e));
   }
-  (void)hipFree(d_res);
-  s->w.res_hist = nullptr;
-  if (rc) return rc;
+  if (rc) {
+    s->drain_profile();
+    return rc;
+  }
+  s->drain_profile();
   for (int it = 0; it < nsteps; ++it) {
     const int step = s->step + 1 + it;
     if (!(step % os == 0 || step == 1))
@@ -1406,7 +1421,6 @@ int ostc_step_steady(ostc_solver *s, int nsteps, double *res_hist) {
   }
   s->step += nsteps;
   if (res_hist) std::copy(hres.begin(), hres.end(), res_hist);
-  s->drain_profile();
   return 0;
 }
 

@@ -31,6 +31,7 @@ struct StepScratch {
   double *res_scale;       // 4
   double *res_one;         // 5: the residual of one ostc_residual call
   int max_blocks;
+  int hist_cap;            // steps dt_hist / res_hist hold (grows, never shrinks)
 };
 
 // one direction (0 = i, 1 = j, 2 = k); init: the sweep writes dq instead of accumulating

@@ -335,8 +335,9 @@ hipError_t launch_dplur(const ZoneView &z, const DParams &P, const DMech *m, int
     hipLaunchKernelGGL(k_dplur_enthalpy<9>, g, bk, 0, s, z, m);
   }
   // The inner sweeps alternate between dq and dqk as input and output (the
-  // reference copies dqk back into dq after each sweep); one copy at the end
-  // when the count is odd leaves the result in dq.
+  // reference copies dqk back into dq after each sweep); k_dq_back reads the
+  // last sweep's output array (dqk for an odd count) and writes dq / dt, so
+  // no copy is needed.
   for (int it = 0; it < inner_steps; ++it) {
     ZoneView zi = z;
     if (it & 1) std::swap(zi.dq, zi.dqk);

@@ -6,7 +6,7 @@
 set -euo pipefail
 HERE="$(cd "$(dirname "$0")" && pwd)"
 mkdir -p "$HERE/_build"
-gcc -std=c11 -O3 -ffp-contract=off -fPIC -shared -Wall -Wno-unused-parameter \
+gcc -std=c11 -O3 -ffp-contract=off -fPIC -shared -Wall -Wno-unused-parameter -Wno-unknown-pragmas \
     -o "$HERE/_build/liboracle.so.tmp" "$HERE/orc_chem.c" "$HERE/orc_flow.c" -lm
 mv -f "$HERE/_build/liboracle.so.tmp" "$HERE/_build/liboracle.so"
 bash "$HERE/build_ref.sh"

@@ -373,6 +373,8 @@ double orc_temperature_newton(const orc_mech *m, double rho, double rhoE, const 
 void orc_chem_batch(const orc_mech *m, int64_t n, const double *rho_a, const double *T_a, const double *Y_a,
                     double dt, double *dY, double *dT) {
   const int ns = m->n_spec;
+  /* cells are independent: an OpenMP build (-fopenmp) splits them over threads */
+#pragma omp parallel for schedule(static)
   for (int64_t i = 0; i < n; ++i) {
     const double rho = rho_a[i], t = T_a[i];
     double Y[ORC_MAXS], omega[ORC_MAXS], jac[ORC_MAXS * ORC_MAXS], lhs[ORC_MAXS * ORC_MAXS], h[ORC_MAXS];

@@ -28,6 +28,28 @@ def build():
     subprocess.run(["bash", os.path.join(HERE, "build.sh")], check=True)
 
 
+def build_native(out_dir, openmp=False):
+    """Compiles the oracle for the host it runs on (-O3 -march=native, no FMA
+    contraction; -fopenmp splits the independent cells of orc_chem_batch over
+    threads) into out_dir and returns the library path.  Used by the CPU
+    baseline, which times the oracle on the machine that also runs the GPU."""
+    os.makedirs(out_dir, exist_ok=True)
+    path = os.path.join(out_dir, "liboracle_native%s.so" % ("_omp" if openmp else ""))
+    cmd = ["gcc", "-std=c11", "-O3", "-march=native", "-ffp-contract=off", "-fPIC", "-shared", "-w"]
+    if openmp:
+        cmd.append("-fopenmp")
+    cmd += ["-o", path, os.path.join(HERE, "orc_chem.c"), os.path.join(HERE, "orc_flow.c"), "-lm"]
+    subprocess.run(cmd, check=True)
+    return path
+
+
+def use_library(path):
+    """Loads the oracle from `path` instead of oracle/_build (before first use)."""
+    global LIB, _lib
+    LIB = path
+    _lib = None
+
+
 class Mech(C.Structure):
     _fields_ = [("n_spec", C.c_int), ("n_reac", C.c_int),
                 ("mw", C.c_double * MAXS), ("t_low", C.c_double * MAXS), ("t_mid", C.c_double * MAXS),
@@ -67,6 +89,8 @@ class Cfg(C.Structure):
 
 
 _lib = None
+if os.environ.get("OSTC_ORACLE_LIB"):
+    LIB = os.environ["OSTC_ORACLE_LIB"]
 
 
 def lib():
