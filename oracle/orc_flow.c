@@ -2074,30 +2074,32 @@ static void dplur_dq0(orc_zone *zz, double *dq0) {
   memcpy(dq0, zz->dq, sizeof(double) * zz->L0.sz * nv);
 }
 
-/* DPLUR.cuh:80-137 compute_jacobian_times_dq at point (i,j,k), direction dir,
- * with the signed spectral radius lam: (A +- lam I) dq of the point */
-static void jac_times_dq(const orc_zone *zz, int i, int j, int k, int dir, double lam, double *out) {
-  const int ns = zz->ns;
-  const int64_t pg = at(zz->Lg, i, j, k);
-  const double *m = &zz->metric[pg * 9 + dir * 3];
-  const double u = BV(1, i, j, k), v = BV(2, i, j, k), w = BV(3, i, j, k);
+/* DPLUR.cuh:80-137 compute_jacobian_times_dq: (A +- lam I) dq at one point
+ * with primitive state (rho, u, v, w, p, T), |V|, the scalars sv (species
+ * then k, omega), the mixture gamma (mix only), the metric row m of
+ * direction dir and the signed spectral radius lam */
+static void jac_times_dq_point(int mix, int ns, int nsc, const orc_mech *mech, const double *b6, double vel,
+                               const double *sv, double gam, const double *m, double lam, const double *dqin,
+                               double *out) {
+  const int nv = 5 + nsc;
+  const double u = b6[1], v = b6[2], w = b6[3];
   const double U = m[0] * u + m[1] * v + m[2] * w;
   const double l1 = U + lam;
-  const double e = 0.5 * zz->vel[pg] * zz->vel[pg];
+  const double e = 0.5 * vel * vel;
   double gamma = GAMMA_AIR, b3 = 0, b4 = 0, h = 0;
   double dq[5 + ORC_MAXS + 4];
-  for (int l = 0; l < zz->nv; ++l) dq[l] = DQ(l, i, j, k);
-  if (!zz->cfg.mix) {
-    h = gamma / (gamma - 1) * BV(4, i, j, k) / BV(0, i, j, k) + e;
+  for (int l = 0; l < nv; ++l) dq[l] = dqin[l];
+  if (!mix) {
+    h = gamma / (gamma - 1) * b6[4] / b6[0] + e;
   } else {
     double hs[ORC_MAXS];
-    const double t = BV(5, i, j, k);
-    orc_enthalpy(t, &zz->mech, hs);
-    gamma = zz->gam[pg];
+    const double t = b6[5];
+    orc_enthalpy(t, mech, hs);
+    gamma = gam;
     for (int l = 0; l < ns; ++l) {
-      b3 += R_U * t / zz->mech.mw[l] * dq[5 + l];
+      b3 += R_U * t / mech->mw[l] * dq[5 + l];
       b4 += hs[l] * dq[5 + l];
-      h += SV(l, i, j, k) * hs[l];
+      h += sv[l] * hs[l];
     }
     b3 *= gamma;
     b4 *= gamma - 1;
@@ -2110,7 +2112,18 @@ static void jac_times_dq(const orc_zone *zz, int i, int j, int k, int dir, doubl
   out[2] = v * b1 + m[1] * b2 + l1 * dq[2] + m[1] * (b3 - b4);
   out[3] = w * b1 + m[2] * b2 + l1 * dq[3] + m[2] * (b3 - b4);
   out[4] = h * b1 + U * b2 + l1 * dq[4] + U * (b3 - b4);
-  for (int l = 0; l < zz->nsc; ++l) out[5 + l] = l1 * dq[5 + l] + SV(l, i, j, k) * b1;
+  for (int l = 0; l < nsc; ++l) out[5 + l] = l1 * dq[5 + l] + sv[l] * b1;
+}
+
+/* the same at zone point (i,j,k) with its current dq */
+static void jac_times_dq(const orc_zone *zz, int i, int j, int k, int dir, double lam, double *out) {
+  const int64_t pg = at(zz->Lg, i, j, k);
+  double b6[6], sv[ORC_MAXS + 4], dq[5 + ORC_MAXS + 4];
+  for (int l = 0; l < 6; ++l) b6[l] = BV(l, i, j, k);
+  for (int l = 0; l < zz->nsc; ++l) sv[l] = SV(l, i, j, k);
+  for (int l = 0; l < zz->nv; ++l) dq[l] = DQ(l, i, j, k);
+  jac_times_dq_point(zz->cfg.mix, zz->ns, zz->nsc, &zz->mech, b6, zz->vel[pg], sv, zz->cfg.mix ? zz->gam[pg] : 0.0,
+                     &zz->metric[pg * 9 + dir * 3], lam, dq, out);
 }
 
 /* DPLUR.cu:20-52 set_wall_dq_to_0 */
@@ -2376,6 +2389,73 @@ void orc_aweno_matrices(const double *cl7, const double *cr7, const double *met6
     for (int c = 0; c < 3; ++c) met[s * 3 + c] = s < 3 ? met6[c] : met6[3 + c];
   double pl[8], pr[8];
   aweno_air_m(cvb, met, 2, pl, pr, L25, R25);
+}
+
+/* Face flux of Riemann solver `scheme` (1 LF, 2 Roe, 3 AUSM+, 4 HLLC) for one
+ * state pair of air (mech NULL) or a mixture: pl/pr = rho, u, v, w, p, then
+ * the n_spec mass fractions (mixtures); met6 the metric rows of the two
+ * points, jac2 their Jacobians, delta2 their entropy-fix deltas (Roe; NULL
+ * = 0).  The states pass through the first-order reconstruction, which
+ * appends rho E (and gamma for mixtures) as the sweeps do. */
+void orc_face_flux(int scheme, const orc_mech *mech, const double *plx, const double *prx, const double *met6,
+                   const double *jac2, const double *delta2, double *f) {
+  orc_zone *zz = calloc(1, sizeof(orc_zone));
+  zz->ns = mech ? mech->n_spec : 0;
+  zz->nsc = zz->ns;
+  zz->nv = 5 + zz->ns;
+  zz->cfg.mix = mech != NULL;
+  if (mech) zz->mech = *mech;
+  zz->cfg.inviscid_scheme = scheme;
+  zz->cfg.reconstruction = 1;
+  zz->inviscid_type = scheme == 2 ? 2 : 0;
+  const int nv = zz->nv;
+  double pv[4 * (5 + ORC_MAXS)], met[12], jac[4], efd[4], pl[ORC_MAXS + 16], pr[ORC_MAXS + 16];
+  for (int q = 0; q < 4; ++q) {
+    const double *src = q < 2 ? plx : prx;
+    for (int l = 0; l < nv; ++l) pv[q * nv + l] = src[l];
+    for (int c = 0; c < 3; ++c) met[q * 3 + c] = met6[(q < 2 ? 0 : 3) + c];
+    jac[q] = jac2[q < 2 ? 0 : 1];
+    efd[q] = delta2 ? delta2[q < 2 ? 0 : 1] : 0.0;
+  }
+  reconstruct(zz, pv, 1, pl, pr);
+  riemann(zz, pl, pr, met, jac, efd, 1, f);
+  free(zz);
+}
+
+/* Reconstruction.cuh:20-72 of air at one face: pv4 holds the primitive
+ * states (rho, u, v, w, p) of points i-1, i, i+1, i+2; pl/pr receive the
+ * face states (and rho E at index 5). */
+void orc_reconstruct_air(int reconstruction, const double *pv4, double *pl, double *pr) {
+  orc_zone *zz = calloc(1, sizeof(orc_zone));
+  zz->nv = 5;
+  zz->cfg.reconstruction = reconstruction;
+  reconstruct(zz, pv4, 1, pl, pr);
+  free(zz);
+}
+
+/* DPLUR Jacobian product (A + lam I) dq at one state: b6 = rho, u, v, w, p,
+ * T; sv the n_scalar scalars (species first); gam the mixture gamma (mixtures);
+ * m the metric row of the direction. */
+void orc_jac_times_dq(const orc_mech *mech, int n_scalar, const double *b6, const double *sv, double gam,
+                      const double *m3, double lam, const double *dq, double *out) {
+  const double vel = sqrt(b6[1] * b6[1] + b6[2] * b6[2] + b6[3] * b6[3]);
+  jac_times_dq_point(mech != NULL, mech ? mech->n_spec : 0, n_scalar, mech, b6, vel, sv, gam, m3, lam, dq, out);
+}
+
+/* SourceTerm.cuh:11-37 alone: dq = the SST and chemistry source terms */
+void orc_zone_source(orc_zone *zz) {
+  memset(zz->dq, 0, sizeof(double) * zz->L0.sz * zz->nv);
+  source(zz);
+}
+
+/* TimeAdvanceFunc.cu:6-17 store_last_step on the zone's current state */
+void orc_zone_store_last(orc_zone *zz) { store_last_step(zz); }
+
+/* Residual.cuh:57-161 against the last store: res[0..3] scaled as at `step`
+ * (step 1 sets the scale), res[4] = their maximum */
+void orc_zone_residual(orc_zone *zz, int step, double *res5) {
+  res5[4] = residual(zz, step);
+  for (int l = 0; l < 4; ++l) res5[l] = zz->res[l];
 }
 
 /* the SSP-RK3 coefficients used by the stage update */

@@ -169,6 +169,14 @@ void orc_face_flux_roe_air(const double *pl5, const double *pr5, const double *m
 void orc_weno5_scalar(const double *v6, double *vm, double *vp);
 void orc_aweno_matrices(const double *cl7, const double *cr7, const double *met6, double *L25, double *R25);
 void orc_ssprk3_coeffs(double *a, double *b, double *c);
+void orc_face_flux(int scheme, const orc_mech *mech, const double *pl, const double *pr, const double *met6,
+                   const double *jac2, const double *delta2, double *f);
+void orc_jac_times_dq(const orc_mech *mech, int n_scalar, const double *b6, const double *sv, double gam,
+                      const double *m3, double lam, const double *dq, double *out);
+void orc_reconstruct_air(int reconstruction, const double *pv4, double *pl, double *pr);
+void orc_zone_store_last(orc_zone *zz);
+void orc_zone_source(orc_zone *zz);
+void orc_zone_residual(orc_zone *zz, int step, double *res5);
 /* SST point hooks: compressibility corrections (SST.cu:24-42) and the
  * constants gamma1, gamma2 derived from the model constants (SST.cuh:13-34) */
 double orc_sst_wilcox(double mt);

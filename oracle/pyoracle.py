@@ -137,12 +137,45 @@ def lib():
         L.orc_sst_zeman.argtypes = [C.c_double, C.c_double]
         L.orc_sst_zeman.restype = C.c_double
         L.orc_sst_constants.argtypes = [dp, dp]
+        L.orc_face_flux.argtypes = [C.c_int, C.c_void_p, dp, dp, dp, dp, C.c_void_p, dp]
+        L.orc_jac_times_dq.argtypes = [C.c_void_p, C.c_int, dp, dp, C.c_double, dp, C.c_double, dp, dp]
+        L.orc_reconstruct_air.argtypes = [C.c_int, dp, dp, dp]
+        L.orc_zone_store_last.argtypes = [C.c_void_p]
+        L.orc_zone_source.argtypes = [C.c_void_p]
+        L.orc_zone_residual.argtypes = [C.c_void_p, C.c_int, dp]
         _lib = L
     return _lib
 
 
 def _p(a):
     return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def face_flux(scheme, pl, pr, met6, jac2, m=None, delta2=None):
+    """Face flux of one Riemann solver (1 LF, 2 Roe, 3 AUSM+, 4 HLLC) for the
+    primitive states pl, pr = (rho, u, v, w, p[, Y...]) of air (m None) or a
+    mixture; met6 the two points' metric rows, jac2 their Jacobians."""
+    pl = np.ascontiguousarray(pl, dtype=np.float64)
+    pr = np.ascontiguousarray(pr, dtype=np.float64)
+    met6 = np.ascontiguousarray(met6, dtype=np.float64)
+    jac2 = np.ascontiguousarray(jac2, dtype=np.float64)
+    f = np.zeros(pl.size)
+    d = None if delta2 is None else np.ascontiguousarray(delta2, dtype=np.float64)
+    lib().orc_face_flux(scheme, None if m is None else C.cast(C.byref(m), C.c_void_p), _p(pl), _p(pr), _p(met6),
+                        _p(jac2), None if d is None else d.ctypes.data_as(C.c_void_p), _p(f))
+    return f
+
+
+def jac_times_dq(b6, sv, m3, lam, dq, m=None, gam=0.0):
+    """DPLUR's (A + lam I) dq at one state (b6 = rho, u, v, w, p, T)."""
+    b6 = np.ascontiguousarray(b6, dtype=np.float64)
+    sv = np.ascontiguousarray(sv if len(sv) else [0.0], dtype=np.float64)
+    m3 = np.ascontiguousarray(m3, dtype=np.float64)
+    dq = np.ascontiguousarray(dq, dtype=np.float64)
+    out = np.zeros(dq.size)
+    lib().orc_jac_times_dq(None if m is None else C.cast(C.byref(m), C.c_void_p), dq.size - 5, _p(b6), _p(sv), gam,
+                           _p(m3), lam, _p(dq), _p(out))
+    return out
 
 
 # ----------------------------------------------------------------- tables
@@ -407,6 +440,17 @@ class Zone:
 
     def global_dt(self):
         return lib().orc_zone_global_dt(self.h)
+
+    def source(self):
+        lib().orc_zone_source(self.h)
+
+    def store_last(self):
+        lib().orc_zone_store_last(self.h)
+
+    def residual(self, step):
+        r = np.zeros(5)
+        lib().orc_zone_residual(self.h, step, _p(r))
+        return r
 
     def run(self, nsteps):
         dt = np.zeros(nsteps * 3)
