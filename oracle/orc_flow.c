@@ -1297,12 +1297,20 @@ static void riemann(const orc_zone *zz, const double *pl, const double *pr, cons
 
 /* AWENO.cu:151-206 WENO5: characteristic projection of the 6-point stencil
  * and the alternative-WENO weights (linear 1/16, 5/8, 5/16; tau5 = (b0-b2)^2) */
+static void weno5_values(const double *v, double *vm, double *vp);
+
 static void weno5(const double *L, const double *cvb, int ip, int row, double *vm, double *vp) {
   double v[6] = {0, 0, 0, 0, 0, 0};
   for (int s = -2; s < 4; ++s) {
     const double *U = &cvb[(ip + s) * 7];
     for (int c = 0; c < 5; ++c) v[s + 2] += L[row * 5 + c] * U[c];
   }
+  weno5_values(v, vm, vp);
+}
+
+/* the WENO5 weights and candidate stencils on six characteristic values of
+ * points i-2 .. i+3: v-(i+1/2) and v+(i+1/2) */
+static void weno5_values(const double *v, double *vm, double *vp) {
   double v0 = 0.125 * (3 * v[0] - 10 * v[1] + 15 * v[2]);
   double v1 = 0.125 * (-v[1] + 6 * v[2] + 3 * v[3]);
   double v2 = 0.125 * (3 * v[2] + 6 * v[3] - v[4]);
@@ -1443,6 +1451,186 @@ static void aweno_air(const double *cvb, const double *met, int ip, double *pl, 
   aweno_air_m(cvb, met, ip, pl, pr, NULL, NULL);
 }
 
+/* ---- AWENO-5 for multi-species mixtures (and transported scalars).
+ * The reference's mixture interpolation is an empty stub (AWENO.cu:10-15),
+ * so this is the build's definition: the air algorithm (AWENO.cu:17-149) on
+ * the 5 + n_scalar conservative variables, with the characteristic fields of
+ * the frozen multi-species Euler Jacobian at the face's Roe average -- the
+ * eigen-system the mixture Roe flux uses (Roe.cu:186-327): acoustic waves
+ * from c1 = dp/c^2 (frozen thermodynamics: gamma and h_k at the Roe-averaged
+ * temperature, and the total enthalpy ek~ + sum Y~ h_k(T~) that makes L and R
+ * exact inverses), the entropy and shear combinations of air, and one wave
+ * Q_{5+l} - Y~_l Q_0 per scalar.  Each reconstructed side is turned into
+ * primitives by a Newton inversion of its internal energy to 1e-12, started
+ * at the Roe-averaged temperature; gamma of each side at its temperature. */
+typedef struct {
+  double u, v, w, h, ek, c, gm1, mw, nx, ny, nz, U, t;
+  double y[ORC_MAXS + 4], hi[ORC_MAXS];
+} mix_eig;
+
+/* Roe average of the two face points cl, cr = (Q[nv], p, T) and the unit
+ * normal of the face-averaged metric row k = (k0 + k1)/2. */
+static void mix_roe_state(const orc_zone *zz, const double *cl, const double *cr, const double *k0, const double *k1,
+                          mix_eig *E) {
+  const int nv = zz->nv, ns = zz->ns, nsc = nv - 5;
+  const double dl = sqrt(cl[0]), dr = sqrt(cr[0]), id = 1.0 / (dl + dr);
+  E->u = (dl * cl[1] / cl[0] + dr * cr[1] / cr[0]) * id;
+  E->v = (dl * cl[2] / cl[0] + dr * cr[2] / cr[0]) * id;
+  E->w = (dl * cl[3] / cl[0] + dr * cr[3] / cr[0]) * id;
+  E->ek = 0.5 * (E->u * E->u + E->v * E->v + E->w * E->w);
+  E->h = (dl * (cl[4] + cl[nv]) / cl[0] + dr * (cr[4] + cr[nv]) / cr[0]) * id;
+  for (int l = 0; l < nsc; ++l) E->y[l] = (dl * cl[5 + l] / cl[0] + dr * cr[5 + l] / cr[0]) * id;
+  double gamma = GAMMA_AIR;
+  E->mw = MW_AIR;
+  E->t = 0;
+  if (zz->cfg.mix) {
+    const orc_mech *m = &zz->mech;
+    double mwi = 0;
+    for (int l = 0; l < ns; ++l) mwi += E->y[l] / m->mw[l];
+    E->t = (dl * (cl[nv] / cl[0]) + dr * (cr[nv] / cr[0])) * id / (R_U * mwi);
+    double cpi[ORC_MAXS];
+    orc_enthalpy_and_cp(E->t, m, E->hi, cpi);
+    double cp = 0, cv = 0;
+    for (int l = 0; l < ns; ++l) {
+      cp += cpi[l] * E->y[l];
+      cv += E->y[l] * (cpi[l] - R_U / m->mw[l]);
+    }
+    gamma = cp / cv;
+    E->c = sqrt(gamma * R_U * mwi * E->t);
+    E->mw = 1.0 / mwi;
+    /* the total enthalpy consistent with T~: L R = I needs
+     * H~ - ek~ = sum Y~ h(T~), which the sqrt(rho) average of H does not
+     * satisfy when cp depends on T */
+    double hs = 0;
+    for (int l = 0; l < ns; ++l) hs += E->y[l] * E->hi[l];
+    E->h = E->ek + hs;
+  } else {
+    E->c = sqrt((gamma - 1) * (E->h - E->ek));
+  }
+  E->gm1 = gamma - 1;
+  double kx = 0.5 * (k0[0] + k1[0]), ky = 0.5 * (k0[1] + k1[1]), kz = 0.5 * (k0[2] + k1[2]);
+  const double gk = sqrt(kx * kx + ky * ky + kz * kz);
+  E->nx = kx / gk;
+  E->ny = ky / gk;
+  E->nz = kz / gk;
+  E->U = E->nx * E->u + E->ny * E->v + E->nz * E->w;
+}
+
+/* characteristic variables w = L Q of a conservative vector Q */
+static void mix_left(const orc_zone *zz, const mix_eig *E, const double *Q, double *wv) {
+  const int nv = zz->nv, ns = zz->ns;
+  const double c = E->c, c2 = c * c;
+  double c1 = E->gm1 * (E->ek * Q[0] - E->u * Q[1] - E->v * Q[2] - E->w * Q[3] + Q[4]) / c2;
+  for (int l = 0; l < ns; ++l) c1 += (E->mw / zz->mech.mw[l] - E->hi[l] * E->gm1 / c2) * Q[5 + l];
+  const double c2v = (E->nx * Q[1] + E->ny * Q[2] + E->nz * Q[3] - E->U * Q[0]) / c;
+  const double c3 = Q[0] - c1;
+  wv[0] = 0.5 * (c1 - c2v);
+  wv[1] = E->nx * c3 - ((E->nz * E->v - E->ny * E->w) * Q[0] - E->nz * Q[2] + E->ny * Q[3]) / c;
+  wv[2] = E->ny * c3 - ((E->nx * E->w - E->nz * E->u) * Q[0] - E->nx * Q[3] + E->nz * Q[1]) / c;
+  wv[3] = E->nz * c3 - ((E->ny * E->u - E->nx * E->v) * Q[0] - E->ny * Q[1] + E->nx * Q[2]) / c;
+  wv[4] = 0.5 * (c1 + c2v);
+  for (int l = 5; l < nv; ++l) wv[l] = Q[l] - E->y[l - 5] * Q[0];
+}
+
+/* conservative vector Q = R w of characteristic amplitudes w */
+static void mix_right(const orc_zone *zz, const mix_eig *E, const double *wv, double *Q) {
+  const int nv = zz->nv, ns = zz->ns;
+  const double c = E->c;
+  const double c0 = E->nx * wv[1] + E->ny * wv[2] + E->nz * wv[3];
+  const double C1 = c0 + wv[0] + wv[4], C2 = c * (wv[4] - wv[0]);
+  double C3 = 0;
+  for (int l = 0; l < ns; ++l) C3 += (E->hi[l] - E->mw / zz->mech.mw[l] * c * c / E->gm1) * wv[5 + l];
+  Q[0] = C1;
+  Q[1] = E->u * C1 + E->nx * C2 - c * (E->nz * wv[2] - E->ny * wv[3]);
+  Q[2] = E->v * C1 + E->ny * C2 - c * (E->nx * wv[3] - E->nz * wv[1]);
+  Q[3] = E->w * C1 + E->nz * C2 - c * (E->ny * wv[1] - E->nx * wv[2]);
+  Q[4] = E->h * C1 + E->U * C2 - c * c * c0 / E->gm1 +
+         c * ((E->nz * E->v - E->ny * E->w) * wv[1] + (E->nx * E->w - E->nz * E->u) * wv[2] +
+              (E->ny * E->u - E->nx * E->v) * wv[3]) +
+         C3;
+  for (int l = 5; l < nv; ++l) Q[l] = wv[l] + E->y[l - 5] * C1;
+}
+
+/* temperature of a mixture state from its internal energy per unit mass e:
+ * Newton on sum Y h(T) - R T sum Y/W = e, from t0, until |dT| <= 1e-12 T
+ * (at most 30 iterations) */
+static double mix_temperature(const orc_mech *m, const double *Y, double e, double t0) {
+  const int ns = m->n_spec;
+  double mwi = 0;
+  for (int l = 0; l < ns; ++l) mwi += Y[l] / m->mw[l];
+  double t = t0;
+  for (int it = 0; it < 30; ++it) {
+    double h[ORC_MAXS], cp[ORC_MAXS];
+    orc_enthalpy_and_cp(t, m, h, cp);
+    double f = 0, fp = 0;
+    for (int l = 0; l < ns; ++l) {
+      f += Y[l] * h[l];
+      fp += Y[l] * cp[l];
+    }
+    f -= R_U * t * mwi + e;
+    fp -= R_U * mwi;
+    const double dt = f / fp;
+    t -= dt;
+    if (fabs(dt) <= 1e-12 * t) break;
+  }
+  return t;
+}
+
+/* primitives (rho, u, v, w, p, scalars; rho E at [nv], gamma at [nv+1]) of
+ * a reconstructed conservative side Q */
+static void mix_side(const orc_zone *zz, const mix_eig *E, const double *Q, double *pv) {
+  const int nv = zz->nv, ns = zz->ns;
+  const double rho = Q[0], ri = 1.0 / rho;
+  pv[0] = rho;
+  pv[1] = Q[1] * ri;
+  pv[2] = Q[2] * ri;
+  pv[3] = Q[3] * ri;
+  for (int l = 5; l < nv; ++l) pv[l] = Q[l] * ri;
+  const double ek = 0.5 * (pv[1] * pv[1] + pv[2] * pv[2] + pv[3] * pv[3]);
+  pv[nv] = Q[4];
+  if (zz->cfg.mix) {
+    const orc_mech *m = &zz->mech;
+    double Y[ORC_MAXS], mwi = 0;
+    for (int l = 0; l < ns; ++l) {
+      Y[l] = pv[5 + l];
+      mwi += Y[l] / m->mw[l];
+    }
+    const double t = mix_temperature(m, Y, Q[4] * ri - ek, E->t);
+    pv[4] = rho * R_U * t * mwi;
+    double cpi[ORC_MAXS];
+    orc_cp(t, m, cpi);
+    double cp = 0, cv = 0;
+    for (int l = 0; l < ns; ++l) {
+      cp += cpi[l] * Y[l];
+      cv += Y[l] * (cpi[l] - R_U / m->mw[l]);
+    }
+    pv[nv + 1] = cp / cv;
+  } else {
+    pv[4] = (GAMMA_AIR - 1) * (Q[4] - ek * rho);
+    pv[nv + 1] = GAMMA_AIR;
+  }
+}
+
+/* AWENO interpolation of a mixture face: cm holds (Q[nv], p, T) per point
+ * (stride nv + 2), met the metric rows of the line, ip the face's left point */
+static void aweno_mix(const orc_zone *zz, const double *cm, const double *met, int ip, double *pl, double *pr) {
+  const int nv = zz->nv, st = nv + 2;
+  mix_eig E;
+  mix_roe_state(zz, &cm[ip * st], &cm[(ip + 1) * st], &met[ip * 3], &met[(ip + 1) * 3], &E);
+  double wv[6][5 + ORC_MAXS + 4];
+  for (int s = 0; s < 6; ++s) mix_left(zz, &E, &cm[(ip - 2 + s) * st], wv[s]);
+  double wm[5 + ORC_MAXS + 4], wp[5 + ORC_MAXS + 4];
+  for (int l = 0; l < nv; ++l) {
+    const double v[6] = {wv[0][l], wv[1][l], wv[2][l], wv[3][l], wv[4][l], wv[5][l]};
+    weno5_values(v, &wm[l], &wp[l]);
+  }
+  double ql[5 + ORC_MAXS + 4], qr[5 + ORC_MAXS + 4];
+  mix_right(zz, &E, wm, ql);
+  mix_right(zz, &E, wp, qr);
+  mix_side(zz, &E, ql, pl);
+  mix_side(zz, &E, qr, pr);
+}
+
 /* line gather helpers: position s on line (dir, a, b) -> (i,j,k) */
 static void line_ijk(int dir, int a, int b, int s, int *i, int *j, int *k) {
   if (dir == 0) { *i = s; *j = a; *k = b; }
@@ -1458,13 +1646,18 @@ static void inviscid_dir(orc_zone *zz, int dir) {
   const int oa = dir == 0 ? 1 : 0, ob = dir == 2 ? 1 : 2;
   const int na = mext[oa], nb = mext[ob];
   const int npt = n + 2 * g;
+  /* AWENO with species or transported scalars takes the mixture
+   * interpolation, whose CDS node flux carries every component */
+  const int mixaw = zz->inviscid_type == 1 && (zz->cfg.mix || zz->nt);
+  const int nfn = mixaw ? nv : 5, stm = nv + 2;
   double *pv = malloc(sizeof(double) * npt * (nv + 2) + 8);
   double *cvb = malloc(sizeof(double) * npt * 7 + 8);
+  double *cm = malloc(sizeof(double) * npt * stm + 8);
   double *met = malloc(sizeof(double) * npt * 3 + 8);
   double *jac = malloc(sizeof(double) * npt + 8);
   double *efd = malloc(sizeof(double) * npt + 8);
   double *fc = malloc(sizeof(double) * (n + 1) * nv + 8);
-  double *fn = malloc(sizeof(double) * npt * 5 + 8);
+  double *fn = malloc(sizeof(double) * npt * nfn + 8);
   for (int b = 0; b < nb; ++b)
     for (int a = 0; a < na; ++a) {
       for (int s = -g; s < n + g; ++s) {
@@ -1477,6 +1670,11 @@ static void inviscid_dir(orc_zone *zz, int dir) {
         for (int l = 0; l < 5; ++l) cvb[q * 7 + l] = CV(l, i, j, k);
         cvb[q * 7 + 5] = BV(4, i, j, k);
         cvb[q * 7 + 6] = BV(5, i, j, k);
+        if (mixaw) {
+          for (int l = 0; l < nv; ++l) cm[q * stm + l] = CV(l, i, j, k);
+          cm[q * stm + nv] = BV(4, i, j, k);
+          cm[q * stm + nv + 1] = BV(5, i, j, k);
+        }
         for (int c = 0; c < 3; ++c) met[q * 3 + c] = zz->metric[p * 9 + dir * 3 + c];
         jac[q] = zz->jac[p];
         efd[q] = (zz->inviscid_type == 2 && s >= -1 && s <= n) ? efix_delta(zz, i, j, k) : 0.0;
@@ -1485,7 +1683,9 @@ static void inviscid_dir(orc_zone *zz, int dir) {
       for (int s = -1; s < n; ++s) {
         const int q = s + g;
         double pl[ORC_MAXS + 16], pr[ORC_MAXS + 16];
-        if (zz->inviscid_type == 1)
+        if (mixaw)
+          aweno_mix(zz, cm, met, q, pl, pr);
+        else if (zz->inviscid_type == 1)
           aweno_air(cvb, met, q, pl, pr);
         else
           reconstruct(zz, pv, q, pl, pr);
@@ -1508,26 +1708,29 @@ static void inviscid_dir(orc_zone *zz, int dir) {
           const double Uk = mm[0] * u + mm[1] * v + mm[2] * w;
           const double rU = BV(0, i, j, k) * Uk;
           const double p = BV(4, i, j, k);
-          fn[q * 5 + 0] = rU * J;
-          fn[q * 5 + 1] = J * (rU * u + mm[0] * p);
-          fn[q * 5 + 2] = J * (rU * v + mm[1] * p);
-          fn[q * 5 + 3] = J * (rU * w + mm[2] * p);
-          fn[q * 5 + 4] = J * Uk * (CV(4, i, j, k) + p);
+          fn[q * nfn + 0] = rU * J;
+          fn[q * nfn + 1] = J * (rU * u + mm[0] * p);
+          fn[q * nfn + 2] = J * (rU * v + mm[1] * p);
+          fn[q * nfn + 3] = J * (rU * w + mm[2] * p);
+          fn[q * nfn + 4] = J * Uk * (CV(4, i, j, k) + p);
+          /* scalars (mixture build definition): J U rho Y */
+          for (int l = 5; l < nfn; ++l) fn[q * nfn + l] = J * Uk * CV(l, i, j, k);
         }
         const double c1 = 19.0 / 3840, c2 = -13.0 / 320, c3 = 17.0 / 256;
         for (int s = 0; s < n - 2 * g; ++s) {
           int i, j, k;
           line_ijk(dir, a, b, s, &i, &j, &k);
           const int q = s + g;
-          for (int l = 0; l < nv; ++l)
-            DQ(l, i, j, k) -= c1 * (fn[(q + 3) * 5 + l] - fn[(q - 3) * 5 + l]) +
-                              c2 * (fn[(q + 2) * 5 + l] - fn[(q - 2) * 5 + l]) +
-                              c3 * (fn[(q + 1) * 5 + l] - fn[(q - 1) * 5 + l]);
+          for (int l = 0; l < nfn; ++l)
+            DQ(l, i, j, k) -= c1 * (fn[(q + 3) * nfn + l] - fn[(q - 3) * nfn + l]) +
+                              c2 * (fn[(q + 2) * nfn + l] - fn[(q - 2) * nfn + l]) +
+                              c3 * (fn[(q + 1) * nfn + l] - fn[(q - 1) * nfn + l]);
         }
       }
     }
   free(pv);
   free(cvb);
+  free(cm);
   free(met);
   free(jac);
   free(efd);
@@ -2419,6 +2622,32 @@ void orc_face_flux(int scheme, const orc_mech *mech, const double *plx, const do
   }
   reconstruct(zz, pv, 1, pl, pr);
   riemann(zz, pl, pr, met, jac, efd, 1, f);
+  free(zz);
+}
+
+/* Characteristic matrices of the mixture AWENO at the face between two
+ * points c = (Q[5 + nsc], p, T) with metric rows k0, k1: L and R
+ * (row-major (5 + nsc)^2) from the projections of unit vectors.  mech NULL:
+ * air with nsc transported scalars. */
+void orc_aweno_mix_matrices(const orc_mech *mech, int nsc, const double *cl, const double *cr, const double *k0,
+                            const double *k1, double *L, double *R) {
+  orc_zone *zz = calloc(1, sizeof(orc_zone));
+  zz->ns = mech ? mech->n_spec : 0;
+  zz->nsc = nsc;
+  zz->nv = 5 + nsc;
+  zz->cfg.mix = mech != NULL;
+  if (mech) zz->mech = *mech;
+  const int nv = zz->nv;
+  mix_eig E;
+  mix_roe_state(zz, cl, cr, k0, k1, &E);
+  double e[5 + ORC_MAXS + 4], col[5 + ORC_MAXS + 4];
+  for (int c = 0; c < nv; ++c) {
+    for (int l = 0; l < nv; ++l) e[l] = l == c ? 1.0 : 0.0;
+    mix_left(zz, &E, e, col);
+    for (int r = 0; r < nv; ++r) L[r * nv + c] = col[r];
+    mix_right(zz, &E, e, col);
+    for (int r = 0; r < nv; ++r) R[r * nv + c] = col[r];
+  }
   free(zz);
 }
 

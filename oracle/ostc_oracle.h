@@ -174,6 +174,8 @@ void orc_face_flux(int scheme, const orc_mech *mech, const double *pl, const dou
 void orc_jac_times_dq(const orc_mech *mech, int n_scalar, const double *b6, const double *sv, double gam,
                       const double *m3, double lam, const double *dq, double *out);
 void orc_reconstruct_air(int reconstruction, const double *pv4, double *pl, double *pr);
+void orc_aweno_mix_matrices(const orc_mech *mech, int nsc, const double *cl, const double *cr, const double *k0,
+                            const double *k1, double *L, double *R);
 void orc_zone_store_last(orc_zone *zz);
 void orc_zone_source(orc_zone *zz);
 void orc_zone_residual(orc_zone *zz, int step, double *res5);

@@ -140,6 +140,7 @@ def lib():
         L.orc_face_flux.argtypes = [C.c_int, C.c_void_p, dp, dp, dp, dp, C.c_void_p, dp]
         L.orc_jac_times_dq.argtypes = [C.c_void_p, C.c_int, dp, dp, C.c_double, dp, C.c_double, dp, dp]
         L.orc_reconstruct_air.argtypes = [C.c_int, dp, dp, dp]
+        L.orc_aweno_mix_matrices.argtypes = [C.c_void_p, C.c_int, dp, dp, dp, dp, dp, dp]
         L.orc_zone_store_last.argtypes = [C.c_void_p]
         L.orc_zone_source.argtypes = [C.c_void_p]
         L.orc_zone_residual.argtypes = [C.c_void_p, C.c_int, dp]
