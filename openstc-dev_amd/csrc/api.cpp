@@ -698,9 +698,9 @@ int ostc_init(const char *case_dir, int device, ostc_solver **out) {
   if (P.inviscid_type == 1 && P.reconstruction != 4)
     return fail("reconstruction " + std::to_string(P.reconstruction) +
                 " with the characteristic (AWENO) path is not supported: only reconstruction 4 (AWENO-5)");
-  if (P.inviscid_type == 1 && P.mix) return fail("AWENO for multi-species mixtures has no reference definition");
-  if (P.inviscid_type == 1 && n_turb)
-    return fail("the AWENO characteristic reconstruction does not carry k and omega (AWENO.cu:53-54)");
+  // mixtures and k / omega take the multi-species characteristic
+  // interpolation (the reference's mixture AWENO is an empty stub,
+  // AWENO.cu:10-15; this build defines it in flux.hip)
   if (n_turb && !P.viscous) return fail("SST needs the viscous flux (viscous_order 2)");
   if (P.mix && P.ns != 9) return fail("this build is compiled for 9-species mechanisms");
   if (device < 0) {  // host-only: setup files, tables and grid metrics, no device work
@@ -1390,8 +1390,10 @@ int ostc_step_steady(ostc_solver *s, int nsteps, double *res_hist) {
         ostc_solver::Timer t(s, "update_bv");
         HIPCHK(launch_update_bv(b.z, s->P, s->d_mech, s->stream));
       }
-      if (run_bcs(s, b, false)) return 1;
-      if (s->P.dim == 2) HIPCHK(launch_eliminate_k(b.z, s->P, false, s->stream));
+      // ghost cv follows for the AWENO path, which reads it
+      const bool with_cv = s->P.inviscid_type == 1;
+      if (run_bcs(s, b, with_cv)) return 1;
+      if (s->P.dim == 2) HIPCHK(launch_eliminate_k(b.z, s->P, with_cv, s->stream));
       if (properties_block(s, b, true)) return 1;
       if (step % os == 0 || step == 1) {
         ostc_solver::Timer t(s, "residual");

@@ -42,23 +42,26 @@ __device__ inline R minmod(R a, R b) {
 
 // ------------------------------------------------------ reconstruction
 // Reconstruction.cuh:20-72, Reconstruction.cu:5-46 on primitives
-// (rho,u,v,w,p,Y...) of the 4 points p0-st .. p0+2st.  Appends rhoE at [nv]
-// and gamma at [nv+1].
+// (rho,u,v,w,p,Y...) of the 4 points p0-st .. p0+2st (recon 2 MUSCL, 3 NND2,
+// otherwise first order: the two adjacent points only).  Appends rhoE at
+// [nv] and gamma at [nv+1].
 template <int NS, int NT, typename R>
-__device__ inline void recon_pv(const ZoneView &z, const DParams &P, const DMech *__restrict__ m, int64_t p0,
-                                int64_t st, R *pl, R *pr) {
+__device__ inline void recon_pv(const ZoneView &z, int recon, const DMech *__restrict__ m, int64_t p0, int64_t st,
+                                R *pl, R *pr) {
   constexpr int NV = 5 + NS + NT;
   const R kappa = R(1.0 / 3.0);
 #pragma unroll
   for (int l = 0; l < NV; ++l) {
     const double *src = l < 5 ? z.bv + (int64_t)l * z.sz : z.sv + (int64_t)(l - 5) * z.sz;
-    const R q0 = (R)src[p0 - st], q1 = (R)src[p0], q2 = (R)src[p0 + st], q3 = (R)src[p0 + 2 * st];
-    if (P.reconstruction == 2) {
+    const R q1 = (R)src[p0], q2 = (R)src[p0 + st];
+    if (recon == 2) {
+      const R q0 = (R)src[p0 - st], q3 = (R)src[p0 + 2 * st];
       const R di = q1 - q0, di1 = q2 - q1, di2 = q3 - q2;
       const R dnl = minmod(di, di1), dpl = minmod(di1, di), dnr = minmod(di1, di2), dpr = minmod(di2, di1);
       pl[l] = q1 + R(0.25) * ((R(1) - kappa) * dnl + (R(1) + kappa) * dpl);
       pr[l] = q2 - R(0.25) * ((R(1) - kappa) * dpr + (R(1) + kappa) * dnr);
-    } else if (P.reconstruction == 3) {
+    } else if (recon == 3) {
+      const R q0 = (R)src[p0 - st], q3 = (R)src[p0 + 2 * st];
       const R di = q1 - q0, di1 = q2 - q1, di2 = q3 - q2;
       pl[l] = q1 + R(0.5) * minmod(di, di1);
       pr[l] = q2 - R(0.5) * minmod(di1, di2);
@@ -123,23 +126,15 @@ __device__ inline double rcp(double x) {
 __device__ inline float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
 
 // ------------------------------------------------------ AWENO (air)
-// AWENO.cu:151-206 WENO5 on one characteristic row; cvb[s][0..5] =
-// (rho, rho u, rho v, rho w, rho E, p) at stencil points s = 0..5 (i-2..i+3).
+// AWENO.cu:151-206 WENO5 on the six characteristic values v[s] of the
+// stencil points s = 0..5 (i-2..i+3): v-(i+1/2) and v+(i+1/2).
 // The nonlinear weights a_k = c_k (1 + tau / (eps + b_k)^2) of each side
 // share one reciprocal of the product of their three denominators
 // (each denominator lies in [1e-12, 1e14] for characteristic variables in
 // density units, far from the double range), and the two normalisations
 // share one reciprocal of the product of the weight sums.
 template <typename R>
-__device__ inline void weno5_row(const R *L, const R (*cvb)[6], R &vm, R &vp) {
-  R v[6];
-#pragma unroll
-  for (int s = 0; s < 6; ++s) {
-    R a = R(0);
-#pragma unroll
-    for (int c = 0; c < 5; ++c) a += L[c] * cvb[s][c];
-    v[s] = a;
-  }
+__device__ inline void weno5_vals(const R *v, R &vm, R &vp) {
   const R t12 = R(13.0 / 12), eps = R(1e-6);
   const R o16 = R(1.0 / 16), f8 = R(5.0 / 8), f16 = R(5.0 / 16);
   // left-biased candidates and smoothness indicators
@@ -168,6 +163,20 @@ __device__ inline void weno5_row(const R *L, const R (*cvb)[6], R &vm, R &vp) {
   const R is = rcp(sl * sr);
   vm = (al0 * u0 + al1 * u1 + al2 * u2) * (sr * is);
   vp = (ar0 * u1 + ar1 * u2 + ar2 * u3) * (sl * is);
+}
+
+// one characteristic row: v(s) = L . cvb[s][0..4], then WENO5
+template <typename R>
+__device__ inline void weno5_row(const R *L, const R (*cvb)[6], R &vm, R &vp) {
+  R v[6];
+#pragma unroll
+  for (int s = 0; s < 6; ++s) {
+    R a = R(0);
+#pragma unroll
+    for (int c = 0; c < 5; ++c) a += L[c] * cvb[s][c];
+    v[s] = a;
+  }
+  weno5_vals(v, vm, vp);
 }
 
 // AWENO.cu:17-149 AWENO_interpolation<Air>.  mk0/mk1: metric row of the
@@ -279,6 +288,259 @@ __device__ inline void aweno_air(const R (*cvb)[6], const R *mk0, const R *mk1, 
   pr[4] = gm1 * (qr[4] - R(0.5) * (pr[1] * pr[1] + pr[2] * pr[2] + pr[3] * pr[3]) * qr[0]);
   pr[5] = qr[4];
   pr[6] = R(kGam);
+}
+
+// ------------------------------------------------------ AWENO (mixtures, scalars)
+// The reference's mixture interpolation is an empty stub (AWENO.cu:10-15) and
+// its air interpolation carries no transported scalars (AWENO.cu:53-54).  The
+// build's definition, shared with the oracle: the air algorithm
+// (AWENO.cu:17-149) on the 5 + n_scalar conservative variables, projected on
+// the characteristic fields of the frozen multi-species Euler Jacobian at the
+// face's Roe average (the eigen-system of the mixture Roe flux, Roe.cu:186-327):
+//  * gamma and the species enthalpies h_l at the Roe temperature
+//    T~ = <p / rho> / (R sum Y~_l / W_l) (sqrt(rho)-weighted averages), the
+//    sound speed c~ = sqrt(gamma R T~ sum Y~/W), and the total enthalpy
+//    H~ = ek~ + sum Y~_l h_l(T~) (which makes L and R exact inverses);
+//  * acoustic waves from c1 = dp / c~^2, the entropy / shear combinations of
+//    air, and one wave Q_{5+l} - Y~_l Q_0 per scalar (species, k, omega);
+//  * each reconstructed side is turned into primitives by a Newton inversion
+//    of its internal energy to 1e-12 relative (FP64, at most 30 iterations,
+//    from T~), with gamma of the side at its own temperature.
+// Laminar air with k and omega (NS = 0) keeps gamma = 1.4 and the
+// sqrt(rho)-averaged H.
+template <int NS, int NT, typename R>
+struct MixEig {
+  static constexpr int NSC = NS + NT;
+  R u, v, w, h, ek, c, ic, c2, gm1, nx, ny, nz, U;
+  double t;
+  R y[NSC > 0 ? NSC : 1];
+  R a[NS > 0 ? NS : 1];  // left rows:  W~/W_l - h_l (gamma - 1) / c~^2
+  R b[NS > 0 ? NS : 1];  // right rows: h_l - W~/W_l c~^2 / (gamma - 1)
+};
+
+// Roe state of the face between cl and cr = (Q[NV], p); k0/k1 the metric
+// rows of the sweep direction at the two points
+template <int NS, int NT, typename R>
+__device__ inline void mix_roe_state(const DMech *__restrict__ m, const R *cl, const R *cr, const R *k0, const R *k1,
+                                     MixEig<NS, NT, R> &E) {
+  constexpr int NV = 5 + NS + NT;
+  const R dl = sqrt(cl[0]), dr = sqrt(cr[0]), id = rcp(dl + dr);
+  const R il = rcp(cl[0]), ir = rcp(cr[0]);
+  const R wl = dl * il, wr = dr * ir;  // sqrt(rho) / rho of each side
+  E.u = (wl * cl[1] + wr * cr[1]) * id;
+  E.v = (wl * cl[2] + wr * cr[2]) * id;
+  E.w = (wl * cl[3] + wr * cr[3]) * id;
+  E.ek = R(0.5) * (E.u * E.u + E.v * E.v + E.w * E.w);
+#pragma unroll
+  for (int l = 0; l < NS + NT; ++l) E.y[l] = (wl * cl[5 + l] + wr * cr[5 + l]) * id;
+  R gm1;
+  if constexpr (NS > 0) {
+    double mwi = 0;
+#pragma unroll
+    for (int l = 0; l < NS; ++l) mwi += (double)E.y[l] / m->mw[l];
+    const double t = (double)((dl * (cl[NV] * il) + dr * (cr[NV] * ir)) * id) / (kRuD * mwi);
+    double hi[NS], cpi[NS];
+    d_enthalpy_cp<NS>(t, m, hi, cpi);
+    double cp = 0, cv = 0, hs = 0;
+#pragma unroll
+    for (int l = 0; l < NS; ++l) {
+      cp += cpi[l] * (double)E.y[l];
+      cv += (double)E.y[l] * (cpi[l] - kRuD / m->mw[l]);
+      hs += (double)E.y[l] * hi[l];
+    }
+    const double gamma = cp / cv;
+    const double c = sqrt(gamma * kRuD * mwi * t);
+    const double mw = 1 / mwi;
+    E.t = t;
+    E.c = (R)c;
+    E.h = E.ek + (R)hs;
+    gm1 = (R)(gamma - 1);
+    const double c2 = c * c, g2 = (gamma - 1) / c2, cg = c2 / (gamma - 1);
+#pragma unroll
+    for (int l = 0; l < NS; ++l) {
+      E.a[l] = (R)(mw / m->mw[l] - hi[l] * g2);
+      E.b[l] = (R)(hi[l] - mw / m->mw[l] * cg);
+    }
+  } else {
+    E.t = 0;
+    E.h = (wl * (cl[4] + cl[NV]) + wr * (cr[4] + cr[NV])) * id;
+    gm1 = R(kGam - 1);
+    E.c = sqrt(gm1 * (E.h - E.ek));
+  }
+  E.gm1 = gm1;
+  E.c2 = E.c * E.c;
+  E.ic = rcp(E.c);
+  const R kx = R(0.5) * (k0[0] + k1[0]), ky = R(0.5) * (k0[1] + k1[1]), kz = R(0.5) * (k0[2] + k1[2]);
+  const R igk = rcp(sqrt(kx * kx + ky * ky + kz * kz));
+  E.nx = kx * igk;
+  E.ny = ky * igk;
+  E.nz = kz * igk;
+  E.U = E.nx * E.u + E.ny * E.v + E.nz * E.w;
+}
+
+// characteristic amplitudes w = L Q of a conservative vector Q
+template <int NS, int NT, typename R>
+__device__ inline void mix_left(const MixEig<NS, NT, R> &E, const R *Q, R *wv) {
+  constexpr int NV = 5 + NS + NT;
+  R c1 = E.gm1 * (E.ek * Q[0] - E.u * Q[1] - E.v * Q[2] - E.w * Q[3] + Q[4]) * rcp(E.c2);
+#pragma unroll
+  for (int l = 0; l < NS; ++l) c1 += E.a[l] * Q[5 + l];
+  const R c2v = (E.nx * Q[1] + E.ny * Q[2] + E.nz * Q[3] - E.U * Q[0]) * E.ic;
+  const R c3 = Q[0] - c1;
+  wv[0] = R(0.5) * (c1 - c2v);
+  wv[1] = E.nx * c3 - ((E.nz * E.v - E.ny * E.w) * Q[0] - E.nz * Q[2] + E.ny * Q[3]) * E.ic;
+  wv[2] = E.ny * c3 - ((E.nx * E.w - E.nz * E.u) * Q[0] - E.nx * Q[3] + E.nz * Q[1]) * E.ic;
+  wv[3] = E.nz * c3 - ((E.ny * E.u - E.nx * E.v) * Q[0] - E.ny * Q[1] + E.nx * Q[2]) * E.ic;
+  wv[4] = R(0.5) * (c1 + c2v);
+#pragma unroll
+  for (int l = 5; l < NV; ++l) wv[l] = Q[l] - E.y[l - 5] * Q[0];
+}
+
+// conservative vector Q = R w of characteristic amplitudes w
+template <int NS, int NT, typename R>
+__device__ inline void mix_right(const MixEig<NS, NT, R> &E, const R *wv, R *Q) {
+  constexpr int NV = 5 + NS + NT;
+  const R c = E.c;
+  const R c0 = E.nx * wv[1] + E.ny * wv[2] + E.nz * wv[3];
+  const R C1 = c0 + wv[0] + wv[4], C2 = c * (wv[4] - wv[0]);
+  R C3 = R(0);
+#pragma unroll
+  for (int l = 0; l < NS; ++l) C3 += E.b[l] * wv[5 + l];
+  Q[0] = C1;
+  Q[1] = E.u * C1 + E.nx * C2 - c * (E.nz * wv[2] - E.ny * wv[3]);
+  Q[2] = E.v * C1 + E.ny * C2 - c * (E.nx * wv[3] - E.nz * wv[1]);
+  Q[3] = E.w * C1 + E.nz * C2 - c * (E.ny * wv[1] - E.nx * wv[2]);
+  Q[4] = E.h * C1 + E.U * C2 - E.c2 * c0 * rcp(E.gm1) +
+         c * ((E.nz * E.v - E.ny * E.w) * wv[1] + (E.nx * E.w - E.nz * E.u) * wv[2] +
+              (E.ny * E.u - E.nx * E.v) * wv[3]) +
+         C3;
+#pragma unroll
+  for (int l = 5; l < NV; ++l) Q[l] = wv[l] + E.y[l - 5] * C1;
+}
+
+// Temperature of a mixture of mass fractions Y at internal energy e per unit
+// mass: Newton on sum Y h(T) - R T sum Y/W = e from t0, until |dT| <= 1e-12 T
+// (at most 30 iterations).  FP64 in every precision mode.
+template <int NS>
+__device__ inline double d_energy_temperature(const DMech *__restrict__ m, const double *Y, double e, double t0,
+                                              double &mwi) {
+  mwi = 0;
+#pragma unroll
+  for (int l = 0; l < NS; ++l) mwi += Y[l] / m->mw[l];
+  double t = t0;
+  for (int it = 0; it < 30; ++it) {
+    double h[NS], cp[NS];
+    d_enthalpy_cp<NS>(t, m, h, cp);
+    double f = 0, fp = 0;
+#pragma unroll
+    for (int l = 0; l < NS; ++l) {
+      f += Y[l] * h[l];
+      fp += Y[l] * cp[l];
+    }
+    f -= kRuD * t * mwi + e;
+    fp -= kRuD * mwi;
+    const double dt = f / fp;
+    t -= dt;
+    if (fabs(dt) <= 1e-12 * t) break;
+  }
+  return t;
+}
+
+// primitives (rho, u, v, w, p, scalars; rho E at [NV], gamma at [NV+1]) of a
+// reconstructed conservative side Q
+template <int NS, int NT, typename R>
+__device__ inline void mix_side(const DMech *__restrict__ m, const MixEig<NS, NT, R> &E, const R *Q, R *pv) {
+  constexpr int NV = 5 + NS + NT;
+  const R rho = Q[0], ri = rcp(rho);
+  pv[0] = rho;
+  pv[1] = Q[1] * ri;
+  pv[2] = Q[2] * ri;
+  pv[3] = Q[3] * ri;
+#pragma unroll
+  for (int l = 5; l < NV; ++l) pv[l] = Q[l] * ri;
+  const R ek = R(0.5) * (pv[1] * pv[1] + pv[2] * pv[2] + pv[3] * pv[3]);
+  pv[NV] = Q[4];
+  if constexpr (NS > 0) {
+    double Y[NS], mwi;
+#pragma unroll
+    for (int l = 0; l < NS; ++l) Y[l] = (double)pv[5 + l];
+    const double t = d_energy_temperature<NS>(m, Y, (double)(Q[4] * ri - ek), E.t, mwi);
+    pv[4] = (R)((double)rho * kRuD * t * mwi);
+    double cpi[NS];
+    d_cp<NS>(t, m, cpi);
+    double cp = 0, cv = 0;
+#pragma unroll
+    for (int l = 0; l < NS; ++l) {
+      cp += cpi[l] * Y[l];
+      cv += Y[l] * (cpi[l] - kRuD / m->mw[l]);
+    }
+    pv[NV + 1] = (R)(cp / cv);
+  } else {
+    pv[4] = R(kGam - 1) * (Q[4] - ek * rho);
+    pv[NV + 1] = R(kGam);
+  }
+}
+
+// A reconstructed side is admissible when its density and pressure are
+// positive, its gamma exceeds 1, everything is finite, and each mass
+// fraction lies in [-0.01, 1.01].
+template <int NS, int NT, typename R>
+__device__ inline bool mix_side_ok(const R *pv) {
+  constexpr int NV = 5 + NS + NT;
+  bool ok = pv[0] > R(0) && pv[4] > R(0) && pv[NV + 1] > R(1) && isfinite(pv[4]) && isfinite(pv[NV]) &&
+            isfinite(pv[NV + 1]);
+#pragma unroll
+  for (int l = 5; l < NV; ++l) {
+    ok = ok && isfinite(pv[l]);
+    if (l < 5 + NS) ok = ok && pv[l] >= R(-0.01) && pv[l] <= R(1.01);
+  }
+  return ok;
+}
+
+// AWENO interpolation of one face from the stencil points p0 + (s - 2) st,
+// s = 0..5, every value read from memory (conservative variables and p).
+// Where either reconstructed side is not admissible (WENO overshoot at a
+// jump no candidate stencil avoids, e.g. a jet port two cells wide), both
+// sides take the first-order values of the two adjacent points.
+template <int NS, int NT, typename R>
+__device__ inline void aweno_mix(const ZoneView &z, const DMech *__restrict__ m, int64_t p0, int64_t st,
+                                 const R *mk0, const R *mk1, R *pl, R *pr) {
+  constexpr int NV = 5 + NS + NT;
+  auto load = [&](int s, R *Q) {
+    const int64_t q = p0 + (s - 2) * st;
+#pragma unroll
+    for (int l = 0; l < NV; ++l) Q[l] = (R)z.cv[(int64_t)l * z.sz + q];
+    Q[NV] = (R)z.bv[4 * z.sz + q];
+  };
+  MixEig<NS, NT, R> E;
+  R wv[6][NV];
+  {
+    R cl[NV + 1], cr[NV + 1];
+    load(2, cl);
+    load(3, cr);
+    mix_roe_state<NS, NT, R>(m, cl, cr, mk0, mk1, E);
+    mix_left<NS, NT, R>(E, cl, wv[2]);
+    mix_left<NS, NT, R>(E, cr, wv[3]);
+  }
+#pragma unroll
+  for (int s = 0; s < 6; ++s) {
+    if (s == 2 || s == 3) continue;
+    R Q[NV + 1];
+    load(s, Q);
+    mix_left<NS, NT, R>(E, Q, wv[s]);
+  }
+  R wm[NV], wp[NV];
+#pragma unroll
+  for (int l = 0; l < NV; ++l) {
+    const R v[6] = {wv[0][l], wv[1][l], wv[2][l], wv[3][l], wv[4][l], wv[5][l]};
+    weno5_vals(v, wm[l], wp[l]);
+  }
+  R q[NV];
+  mix_right<NS, NT, R>(E, wm, q);
+  mix_side<NS, NT, R>(m, E, q, pl);
+  mix_right<NS, NT, R>(E, wp, q);
+  mix_side<NS, NT, R>(m, E, q, pr);
+  if (!(mix_side_ok<NS, NT, R>(pl) && mix_side_ok<NS, NT, R>(pr))) recon_pv<NS, NT, R>(z, 1, m, p0, st, pl, pr);
 }
 
 // ------------------------------------------------------ Riemann solvers
@@ -723,8 +985,10 @@ __device__ inline void viscous_face(const ZoneView &z, const DParams &P, const D
 
 // ------------------------------------------------------ face assembly
 // AWENO.cu:233-242: node flux of the CDS correction at point q (metric row
-// rb of the sweep direction), J (rho U, rho U u + kx p, .., U (rho E + p)).
-template <typename R>
+// rb of the sweep direction), J (rho U, rho U u + kx p, .., U (rho E + p)),
+// and (mixtures and transported scalars, the build's definition) J U rho Y_l
+// for the components 5 .. NG-1.
+template <typename R, int NG = 5>
 __device__ inline void node_flux(const ZoneView &z, int rb, int64_t q, R *fn) {
   const R kx = (R)z.metric[(rb + 0) * z.sz + q], ky = (R)z.metric[(rb + 1) * z.sz + q],
           kz = (R)z.metric[(rb + 2) * z.sz + q];
@@ -738,7 +1002,16 @@ __device__ inline void node_flux(const ZoneView &z, int rb, int64_t q, R *fn) {
   fn[2] = J * (rU * v + ky * p);
   fn[3] = J * (rU * w + kz * p);
   fn[4] = J * Uk * ((R)z.cv[4 * z.sz + q] + p);
+#pragma unroll
+  for (int l = 5; l < NG; ++l) fn[l] = J * Uk * (R)z.cv[(int64_t)l * z.sz + q];
 }
+
+// components of the CDS flux G: all conservative variables for the mixture /
+// scalar AWENO, the 5 flow rows for air, none (one unused slot) without AWENO
+template <int NS, int NT, int SCHEME>
+struct CdsWidth {
+  static constexpr int value = SCHEME == 1 ? 5 + NS + NT : 1;
+};
 
 // CDS weights of the flux form G(i+1/2) = sum_s w_s f(i-2+s), s = 0..5
 template <typename R>
@@ -781,6 +1054,7 @@ template <int NS, int NT, int SCHEME, typename R>
 __device__ inline void face_flux(const ZoneView &z, const DParams &P, const DMech *__restrict__ m, int dir, int64_t p0,
                                  int64_t st, bool want_g, R *F, R *G) {
   constexpr int NV = 5 + NS + NT;
+  constexpr int NG = CdsWidth<NS, NT, SCHEME>::value;
   const int rb = dir * 3;
   R mk0[3], mk1[3];
 #pragma unroll
@@ -791,40 +1065,44 @@ __device__ inline void face_flux(const ZoneView &z, const DParams &P, const DMec
   const R j0 = (R)z.jac[p0], j1 = (R)z.jac[p0 + st];
   R pl[NV + 2], pr[NV + 2];
   if constexpr (SCHEME == 1) {
-    R cvb[6][6];
-#pragma unroll
-    for (int s = 0; s < 6; ++s) {
-      const int64_t q = p0 + (s - 2) * st;
-#pragma unroll
-      for (int c = 0; c < 5; ++c) cvb[s][c] = (R)z.cv[(int64_t)c * z.sz + q];
-      cvb[s][5] = (R)z.bv[4 * z.sz + q];
-    }
-    aweno_air(cvb, mk0, mk1, pl, pr);
-    if (want_g) {
-#pragma unroll
-      for (int l = 0; l < 5; ++l) G[l] = R(0);
+    if constexpr (NS + NT > 0) {
+      aweno_mix<NS, NT, R>(z, m, p0, st, mk0, mk1, pl, pr);
+    } else {
+      R cvb[6][6];
 #pragma unroll
       for (int s = 0; s < 6; ++s) {
-        R fn[5];
-        node_flux<R>(z, rb, p0 + (s - 2) * st, fn);
+        const int64_t q = p0 + (s - 2) * st;
 #pragma unroll
-        for (int l = 0; l < 5; ++l) G[l] += cds_weight<R>(s) * fn[l];
+        for (int c = 0; c < 5; ++c) cvb[s][c] = (R)z.cv[(int64_t)c * z.sz + q];
+        cvb[s][5] = (R)z.bv[4 * z.sz + q];
+      }
+      aweno_air(cvb, mk0, mk1, pl, pr);
+    }
+    if (want_g) {
+#pragma unroll
+      for (int l = 0; l < NG; ++l) G[l] = R(0);
+#pragma unroll
+      for (int s = 0; s < 6; ++s) {
+        R fn[NG];
+        node_flux<R, NG>(z, rb, p0 + (s - 2) * st, fn);
+#pragma unroll
+        for (int l = 0; l < NG; ++l) G[l] += cds_weight<R>(s) * fn[l];
       }
     }
   } else {
-    recon_pv<NS, NT, R>(z, P, m, p0, st, pl, pr);
+    recon_pv<NS, NT, R>(z, P.reconstruction, m, p0, st, pl, pr);
   }
   riemann_viscous<NS, NT, SCHEME, R>(z, P, m, dir, p0, st, pl, pr, mk0, mk1, j0, j1, F);
 }
 
-template <int NS, int NT, typename R>
+template <int NS, int NT, typename R, int NG = 5>
 __device__ inline void add_dq(const ZoneView &z, int64_t q0, bool init, const R *Fc, const R *Fp, bool cds, const R *Gc,
                               const R *Gp) {
   constexpr int NV = 5 + NS + NT;
 #pragma unroll
   for (int l = 0; l < NV; ++l) {
     double d = (double)Fc[l] - (double)Fp[l];
-    if (l < 5 && cds) d += (double)Gc[l] - (double)Gp[l];
+    if (l < NG && cds) d += (double)Gc[l] - (double)Gp[l];
     double *dst = z.dq + (int64_t)l * z.sz0 + q0;
     *dst = init ? -d : *dst - d;
   }
@@ -858,18 +1136,19 @@ __global__ void __launch_bounds__(256) sweep_i(ZoneView z, DParams P, const DMec
   const int i = (int)(fo - (unsigned)j * (n + 1)) - 1;
   const bool live = j < z.my;
   const int ncds = SCHEME == 1 ? n - 2 * z.g : 0;
-  R F[NV], G[5];
+  constexpr int NG = CdsWidth<NS, NT, SCHEME>::value;
+  R F[NV], G[NG];
 #pragma unroll
   for (int l = 0; l < NV; ++l) F[l] = R(0);
 #pragma unroll
-  for (int l = 0; l < 5; ++l) G[l] = R(0);
+  for (int l = 0; l < NG; ++l) G[l] = R(0);
   if (live) face_flux<NS, NT, SCHEME, R>(z, P, m, 0, z.at(i, j, k), 1, i <= ncds - 1, F, G);
-  R Fp[NV], Gp[5];
+  R Fp[NV], Gp[NG];
 #pragma unroll
   for (int l = 0; l < NV; ++l) Fp[l] = __shfl_up(F[l], 1, 64);
 #pragma unroll
-  for (int l = 0; l < 5; ++l) Gp[l] = __shfl_up(G[l], 1, 64);
-  if (live && lane > 0 && i >= 0) add_dq<NS, NT, R>(z, z.at0(i, j, k), init != 0, F, Fp, i < ncds, G, Gp);
+  for (int l = 0; l < NG; ++l) Gp[l] = __shfl_up(G[l], 1, 64);
+  if (live && lane > 0 && i >= 0) add_dq<NS, NT, R, NG>(z, z.at0(i, j, k), init != 0, F, Fp, i < ncds, G, Gp);
 }
 
 // AWENO i-sweep (air) with the stencil staged in LDS.  Points of a k-plane
@@ -992,7 +1271,7 @@ __global__ void __launch_bounds__((64 * JkRows<NS + NT, R>::value)) sweep_jk(Zon
                                                                            const DMech *__restrict__ m, int init,
                                                                            int gx, int nseg) {
   constexpr int NV = 5 + NS + NT;
-  constexpr int NG = SCHEME == 1 ? 5 : 1;  // the CDS flux G exists for AWENO only
+  constexpr int NG = CdsWidth<NS, NT, SCHEME>::value;  // the CDS flux G exists for AWENO only
   constexpr int kRows = JkRows<NS + NT, R>::value;
   __shared__ R sF[kRows + 2][NV][64];  // slots 0, 1: carry; 2 + r: row r
   __shared__ R sG[kRows + 2][NG][64];
@@ -1032,7 +1311,7 @@ __global__ void __launch_bounds__((64 * JkRows<NS + NT, R>::value)) sweep_jk(Zon
 #pragma unroll
       for (int l = 0; l < NG; ++l) Gp[l] = sG[ls][l][lane];
       const int64_t q = DIR == 1 ? z.at0(L.i, f, L.o) : z.at0(L.i, L.o, f);
-      add_dq<NS, NT, R>(z, q, init != 0, F, Fp, f < ncds, G, Gp);
+      add_dq<NS, NT, R, NG>(z, q, init != 0, F, Fp, f < ncds, G, Gp);
     }
     __syncthreads();  // the next batch's flux stores overwrite the slots read here
   }
@@ -1235,20 +1514,32 @@ hipError_t launch_flux_sweep(const ZoneView &z, const DParams &P, const DMech *m
       return fp32 ? launch_sweep_t<0, 0, 0, float>(z, P, m, dir, init, s)
                   : launch_sweep_t<0, 0, 0, double>(z, P, m, dir, init, s);
     }
-    if (P.ns == 9 && !aweno)
+    if (P.ns == 9) {
+      if (aweno)
+        return fp32 ? launch_sweep_t<9, 0, 1, float>(z, P, m, dir, init, s)
+                    : launch_sweep_t<9, 0, 1, double>(z, P, m, dir, init, s);
       return fp32 ? launch_sweep_t<9, 0, 0, float>(z, P, m, dir, init, s)
                   : launch_sweep_t<9, 0, 0, double>(z, P, m, dir, init, s);
+    }
     return hipErrorNotSupported;
   }
-  // SST: k and omega are transported after the species; the characteristic
-  // AWENO reconstruction does not carry them (AWENO.cu:53-54)
-  if (P.nt != 2 || aweno) return hipErrorNotSupported;
-  if (P.ns == 0)
+  // SST: k and omega are transported after the species (the AWENO path
+  // projects them on their own characteristic waves)
+  if (P.nt != 2) return hipErrorNotSupported;
+  if (P.ns == 0) {
+    if (aweno)
+      return fp32 ? launch_sweep_t<0, 2, 1, float>(z, P, m, dir, init, s)
+                  : launch_sweep_t<0, 2, 1, double>(z, P, m, dir, init, s);
     return fp32 ? launch_sweep_t<0, 2, 0, float>(z, P, m, dir, init, s)
                 : launch_sweep_t<0, 2, 0, double>(z, P, m, dir, init, s);
-  if (P.ns == 9)
+  }
+  if (P.ns == 9) {
+    if (aweno)
+      return fp32 ? launch_sweep_t<9, 2, 1, float>(z, P, m, dir, init, s)
+                  : launch_sweep_t<9, 2, 1, double>(z, P, m, dir, init, s);
     return fp32 ? launch_sweep_t<9, 2, 0, float>(z, P, m, dir, init, s)
                 : launch_sweep_t<9, 2, 0, double>(z, P, m, dir, init, s);
+  }
   return hipErrorNotSupported;
 }
 

@@ -284,6 +284,17 @@ __global__ void __launch_bounds__(kTB) k_update_bv(ZoneView z, DParams P, const 
       if (l < NS) Y[l] = y;
   }
   const double V2 = u * u + v * v + w * w;
+  if (P.inviscid_type == 1) {
+    // the AWENO path reconstructs conservative variables: keep cv the
+    // updated conservative state (the reference's steady update leaves cv
+    // at its initial values, FieldOperation.cuh:221-296)
+    z.cv[p] = rho;
+    z.cv[S + p] = rho * u;
+    z.cv[2 * S + p] = rho * v;
+    z.cv[3 * S + p] = rho * w;
+    z.cv[4 * S + p] = E;
+    for (int l = 0; l < P.nsc; ++l) z.cv[(int64_t)(5 + l) * S + p] = rho * z.sv[(int64_t)l * S + p];
+  }
   if constexpr (NS > 0) {
     double pn;
     const double t = d_temperature<NS>(m, rho, E, u, v, w, Y, z.bv[5 * S + p], pn);

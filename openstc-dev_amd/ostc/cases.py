@@ -181,17 +181,19 @@ def c2_case(nx=256, ny=128, nz=128, seed=1002, steps=20, mach=2.5, t_inf=300.0, 
                 steps=steps)
 
 
-def c3_case(nx=512, ny=256, seed=1003, steps=20, dx=1e-4, delta=5e-4, viscous_order=2, inviscid_scheme=4):
+def c3_case(nx=512, ny=256, seed=1003, steps=20, dx=1e-4, delta=5e-4, viscous_order=2, inviscid_scheme=4,
+            reconstruction=4):
     """2-D reacting H2/air supersonic mixing layer with the 9-species,
     19-reaction H2-O2 mechanism (input_files/chemistry): upper stream air
     (Y_O2 = 0.233, Y_N2 = 0.767) at 1500 K and Mach 2, lower stream
     H2/N2 (0.3/0.7 by mass) at 300 K and Mach 1.2, both at 101325 Pa; a
     tanh profile of thickness `delta` across the centre line plus 1 % seeded
-    velocity noise (see c3_fields).  HLLC with MUSCL-minmod (the reference's
-    AWENO interpolation has no mixture form), second-order viscous fluxes
-    with mixture-averaged transport, explicit finite-rate source inside
-    SSP-RK3 at CFL 0.5.  Inflow i=0 (air above the centre line, fuel below,
-    labels 5 and 51), outflow i=max, symmetry j=0 and j=max."""
+    velocity noise (see c3_fields).  HLLC with AWENO-5 (reconstruction 4,
+    the multi-species characteristic interpolation, 3 ghost layers) or
+    MUSCL-minmod (reconstruction 2, 2 ghost layers), second-order viscous
+    fluxes with mixture-averaged transport, explicit finite-rate source
+    inside SSP-RK3 at CFL 0.5.  Inflow i=0 (air above the centre line, fuel
+    below, labels 5 and 51), outflow i=max, symmetry j=0 and j=max."""
     xi = np.arange(nx) * dx
     yj = np.arange(ny) * dx
     X, Y = np.meshgrid(xi, yj, indexing="xy")
@@ -203,7 +205,7 @@ def c3_case(nx=512, ny=256, seed=1003, steps=20, dx=1e-4, delta=5e-4, viscous_or
     common = [("real", "pressure", 101325.0), ("real", "u", 1.0), ("real", "v", 0.0), ("real", "w", 0.0)]
     setup = _setup_common({
         "int dimension": 2, "int species": 1, "int reaction": 1, "int chemSrcMethod": 0, "real cfl": 0.5,
-        "int inviscid_scheme": inviscid_scheme, "int reconstruction": 2, "int limiter": 0,
+        "int inviscid_scheme": inviscid_scheme, "int reconstruction": reconstruction, "int limiter": 0,
         "int viscous_order": viscous_order,
         "int total_step": steps,
         "array string boundary_conditions": ["air_stream", "fuel_stream", "outflow", "symmetry"],
@@ -223,9 +225,11 @@ def c3_case(nx=512, ny=256, seed=1003, steps=20, dx=1e-4, delta=5e-4, viscous_or
         dict(type=3, raw=(0, nx - 1, 0, 0, 0, 0), label=3),
         dict(type=3, raw=(0, nx - 1, ny - 1, ny - 1, 0, 0), label=3),
     ]
-    scheme = dict(mix=1, reaction=1, chemSrcMethod=0, inviscid_scheme=inviscid_scheme, reconstruction=2, limiter=0,
-                  viscous_order=viscous_order, cfl=0.5, fixed_time_step=0, output_screen=1, entropy_fix_factor=0.125)
-    return dict(name="c3", mx=nx, my=ny, mz=1, ngg=2, x=x, y=y, z=z, setup=setup, bcs=bcs, scheme=scheme,
+    scheme = dict(mix=1, reaction=1, chemSrcMethod=0, inviscid_scheme=inviscid_scheme, reconstruction=reconstruction,
+                  limiter=0, viscous_order=viscous_order, cfl=0.5, fixed_time_step=0, output_screen=1,
+                  entropy_fix_factor=0.125)
+    return dict(name="c3", mx=nx, my=ny, mz=1, ngg=3 if reconstruction == 4 else 2, x=x, y=y, z=z, setup=setup,
+                bcs=bcs, scheme=scheme,
                 seed=seed, steps=steps, dx=dx, delta=delta, y0=0.5 * (ny - 1) * dx)
 
 
@@ -345,12 +349,14 @@ def two_wall(n, length, first):
 
 
 def sst_plate_case(nx=64, ny=48, seed=1007, mach=2.0, p=20000.0, t=300.0, t_wall=300.0, lx=0.1, ly=0.02,
-                   first=2e-5, le=8, intensity=0.01, ratio=10.0, compressibility=3, inviscid_scheme=3):
+                   first=2e-5, le=8, intensity=0.01, ratio=10.0, compressibility=3, inviscid_scheme=3,
+                   reconstruction=2):
     """2-D turbulent (k-omega SST) supersonic flat plate in air: Mach 2 inflow
     at i=0 carrying k, omega from a turbulence intensity and viscosity ratio,
     outflow at i=max and j=max, symmetry on j=0 ahead of the leading edge
     (node `le`) and an isothermal no-slip wall behind it; wall-normal
-    geometric stretching from spacing `first`.  AUSM+ with MUSCL-minmod,
+    geometric stretching from spacing `first`.  AUSM+ with MUSCL-minmod
+    (reconstruction 2) or AWENO-5 carrying k and omega (reconstruction 4),
     second-order viscous fluxes with mu + mu_t, SSP-RK3 (or the steady path
     when the caller switches it) at CFL 0.5; Zeman compressibility
     correction by default."""
@@ -361,7 +367,8 @@ def sst_plate_case(nx=64, ny=48, seed=1007, mach=2.0, p=20000.0, t=300.0, t_wall
     z = np.zeros_like(x)
     inflow = sst_state(air_inflow(mach, p, t), sutherland(t), ratio, intensity)
     setup = _setup_common({
-        "int dimension": 2, "real cfl": 0.5, "int inviscid_scheme": inviscid_scheme, "int reconstruction": 2,
+        "int dimension": 2, "real cfl": 0.5, "int inviscid_scheme": inviscid_scheme,
+        "int reconstruction": reconstruction,
         "int limiter": 0, "int viscous_order": 2, "int total_step": 10, "bool turbulence": 1,
         "int turbulence_method": 1, "int RANS_model": 2, "int turb_implicit": 1,
         "int compressibility_correction": compressibility,
@@ -383,10 +390,11 @@ def sst_plate_case(nx=64, ny=48, seed=1007, mach=2.0, p=20000.0, t=300.0, t_wall
         dict(type=3, raw=(0, le - 1, 0, 0, 0, 0), label=3),
         dict(type=2, raw=(le, nx - 1, 0, 0, 0, 0), label=2, t_wall=t_wall, isothermal=True),
     ]
-    scheme = dict(mix=0, inviscid_scheme=inviscid_scheme, reconstruction=2, limiter=0, viscous_order=2, cfl=0.5,
-                  fixed_time_step=0, output_screen=1, turbulence=1, turb_implicit=1,
+    scheme = dict(mix=0, inviscid_scheme=inviscid_scheme, reconstruction=reconstruction, limiter=0, viscous_order=2,
+                  cfl=0.5, fixed_time_step=0, output_screen=1, turbulence=1, turb_implicit=1,
                   compressibility_correction=compressibility)
-    return dict(name="sst_plate", mx=nx, my=ny, mz=1, ngg=2, x=x, y=y, z=z, setup=setup, bcs=bcs, scheme=scheme,
+    return dict(name="sst_plate", mx=nx, my=ny, mz=1, ngg=3 if reconstruction == 4 else 2, x=x, y=y, z=z,
+                setup=setup, bcs=bcs, scheme=scheme,
                 inflow=inflow, seed=seed, perturb="density", steps=10, le=le, delta=0.2 * ly)
 
 
@@ -432,17 +440,19 @@ def c5_ports(nx, nz, lx, lz, x_port=0.25, size=2e-3, count=4):
 
 def c5_case(nx=384, ny=192, nz=128, seed=1005, lx=0.2, ly=0.05, lz=0.04, first=5e-6, p=101325.0, t_air=1200.0,
             mach=2.0, p_jet=202650.0, t_jet=300.0, t_wall=300.0, intensity=0.01, ratio=10.0, inner=3,
-            chem=1, compressibility=3):
+            chem=1, compressibility=3, reconstruction=4):
     """3-D reacting RANS (k-omega SST) supersonic H2/air channel with wall
     injection on one block: walls at j=0 and j=max with geometric stretching
     from spacing `first` towards both (y+ ~ 1), vitiated-air inflow at i=0
     (Y_O2 = 0.25, Y_H2O = 0.12, Y_N2 = 0.63; 1200 K, Mach 2), outflow at
     i=max, symmetry at k=0 and k=max, and sonic H2 jets at 300 K through
     four 2 mm square ports in the lower wall (inflow patches, normal
-    injection).  9 species plus k and omega; HLLC with MUSCL-minmod (the
-    AWENO interpolation carries neither mixtures nor k, omega), viscous
-    fluxes with mu + mu_t, steady pseudo-time with DPLUR (3 inner sweeps)
-    and the exact point-implicit chemistry, implicit SST source diagonal."""
+    injection).  9 species plus k and omega; HLLC with AWENO-5
+    (reconstruction 4: the multi-species characteristic interpolation
+    carrying k and omega, 3 ghost layers) or MUSCL-minmod (reconstruction
+    2), viscous fluxes with mu + mu_t, steady pseudo-time with DPLUR (3
+    inner sweeps) and the exact point-implicit chemistry, implicit SST
+    source diagonal."""
     xi = np.linspace(0.0, lx, nx)
     yj = two_wall(ny, ly, first)
     zk = np.linspace(0.0, lz, nz)
@@ -452,7 +462,7 @@ def c5_case(nx=384, ny=192, nz=128, seed=1005, lx=0.2, ly=0.05, lz=0.04, first=5
               ("real", "turbulence_intensity", intensity), ("real", "turb_viscosity_ratio", ratio)]
     setup = _setup_common({
         "int dimension": 3, "int species": 1, "int reaction": 1, "int chemSrcMethod": chem, "real cfl": 0.5,
-        "int inviscid_scheme": 4, "int reconstruction": 2, "int limiter": 0, "int viscous_order": 2,
+        "int inviscid_scheme": 4, "int reconstruction": reconstruction, "int limiter": 0, "int viscous_order": 2,
         "int total_step": 10, "bool steady": 1, "int implicit_method": 1, "int DPLUR_inner_step": inner,
         "bool turbulence": 1, "int turbulence_method": 1, "int RANS_model": 2, "int turb_implicit": 1,
         "int compressibility_correction": compressibility,
@@ -489,10 +499,11 @@ def c5_case(nx=384, ny=192, nz=128, seed=1005, lx=0.2, ly=0.05, lz=0.04, first=5
                 bcs.append(dict(type=5, raw=(i0, i1, 0, 0, k0, k1), label=52, stream="h2_jet"))
             else:
                 bcs.append(dict(type=2, raw=(i0, i1, 0, 0, k0, k1), label=2, t_wall=t_wall, isothermal=True))
-    scheme = dict(mix=1, reaction=1, chemSrcMethod=chem, inviscid_scheme=4, reconstruction=2, limiter=0,
+    scheme = dict(mix=1, reaction=1, chemSrcMethod=chem, inviscid_scheme=4, reconstruction=reconstruction, limiter=0,
                   viscous_order=2, cfl=0.5, fixed_time_step=0, output_screen=1, implicit_method=1,
                   DPLUR_inner_step=inner, turbulence=1, turb_implicit=1, compressibility_correction=compressibility)
-    return dict(name="c5", mx=nx, my=ny, mz=nz, ngg=2, x=x, y=y, z=z, setup=setup, bcs=bcs, scheme=scheme,
+    return dict(name="c5", mx=nx, my=ny, mz=nz, ngg=3 if reconstruction == 4 else 2, x=x, y=y, z=z, setup=setup,
+                bcs=bcs, scheme=scheme,
                 seed=seed, steps=10, ports=ports)
 
 

@@ -940,13 +940,13 @@ static double minmod(double a, double b) {
 /* Reconstruction.cuh:20-72 + Reconstruction.cu:5-46.  pv is a line buffer of
  * nv entries per point; ip is the left point of the face.  Appends E (and
  * gamma for mixtures) at pv_l[nv], pv_l[nv+1]. */
-static void reconstruct(const orc_zone *zz, const double *pv, int ip, double *pl, double *pr) {
+static void reconstruct_order(const orc_zone *zz, const double *pv, int ip, int recon, double *pl, double *pr) {
   const int nv = zz->nv;
   const double kappa = 1.0 / 3.0;
   for (int l = 0; l < nv; ++l) {
     const double q0 = pv[(ip - 1) * nv + l], q1 = pv[ip * nv + l], q2 = pv[(ip + 1) * nv + l],
                  q3 = pv[(ip + 2) * nv + l];
-    switch (zz->cfg.reconstruction) {
+    switch (recon) {
       case 2: {
         const double di = q1 - q0, di1 = q2 - q1, di2 = q3 - q2;
         const double dnl = minmod(di, di1), dpl = minmod(di1, di), dnr = minmod(di1, di2), dpr = minmod(di2, di1);
@@ -996,7 +996,13 @@ static void reconstruct(const orc_zone *zz, const double *pv, int ip, double *pl
     const double er = 0.5 * (pr[1] * pr[1] + pr[2] * pr[2] + pr[3] * pr[3]);
     pl[nv] = el * pl[0] + pl[4] / (GAMMA_AIR - 1);
     pr[nv] = er * pr[0] + pr[4] / (GAMMA_AIR - 1);
+    pl[nv + 1] = GAMMA_AIR;
+    pr[nv + 1] = GAMMA_AIR;
   }
+}
+
+static void reconstruct(const orc_zone *zz, const double *pv, int ip, double *pl, double *pr) {
+  reconstruct_order(zz, pv, ip, zz->cfg.reconstruction, pl, pr);
 }
 
 /* AusmP.cu:6-94 compute_ausmPlus_flux; met/jac are the line buffers, ip the left point. */
@@ -1611,9 +1617,29 @@ static void mix_side(const orc_zone *zz, const mix_eig *E, const double *Q, doub
   }
 }
 
+/* A reconstructed side is admissible when its density and pressure are
+ * positive, its gamma exceeds 1, everything is finite, and each mass
+ * fraction lies in [-0.01, 1.01]. */
+static int mix_side_ok(const orc_zone *zz, const double *pv) {
+  const int nv = zz->nv;
+  if (!(pv[0] > 0) || !(pv[4] > 0) || !(pv[nv + 1] > 1) || !isfinite(pv[4]) || !isfinite(pv[nv]) ||
+      !isfinite(pv[nv + 1]))
+    return 0;
+  for (int l = 5; l < nv; ++l) {
+    if (!isfinite(pv[l])) return 0;
+    if (l < 5 + zz->ns && (pv[l] < -0.01 || pv[l] > 1.01)) return 0;
+  }
+  return 1;
+}
+
 /* AWENO interpolation of a mixture face: cm holds (Q[nv], p, T) per point
- * (stride nv + 2), met the metric rows of the line, ip the face's left point */
-static void aweno_mix(const orc_zone *zz, const double *cm, const double *met, int ip, double *pl, double *pr) {
+ * (stride nv + 2), met the metric rows of the line, ip the face's left point.
+ * Where either reconstructed side is not admissible (WENO overshoot at a
+ * jump no candidate stencil avoids, e.g. a jet port two cells wide), both
+ * sides take the first-order values of the two adjacent points (pv holds
+ * their primitives, stride nv). */
+static void aweno_mix(const orc_zone *zz, const double *cm, const double *pv, const double *met, int ip, double *pl,
+                      double *pr) {
   const int nv = zz->nv, st = nv + 2;
   mix_eig E;
   mix_roe_state(zz, &cm[ip * st], &cm[(ip + 1) * st], &met[ip * 3], &met[(ip + 1) * 3], &E);
@@ -1629,6 +1655,7 @@ static void aweno_mix(const orc_zone *zz, const double *cm, const double *met, i
   mix_right(zz, &E, wp, qr);
   mix_side(zz, &E, ql, pl);
   mix_side(zz, &E, qr, pr);
+  if (!mix_side_ok(zz, pl) || !mix_side_ok(zz, pr)) reconstruct_order(zz, pv, ip, 1, pl, pr);
 }
 
 /* line gather helpers: position s on line (dir, a, b) -> (i,j,k) */
@@ -1684,7 +1711,7 @@ static void inviscid_dir(orc_zone *zz, int dir) {
         const int q = s + g;
         double pl[ORC_MAXS + 16], pr[ORC_MAXS + 16];
         if (mixaw)
-          aweno_mix(zz, cm, met, q, pl, pr);
+          aweno_mix(zz, cm, pv, met, q, pl, pr);
         else if (zz->inviscid_type == 1)
           aweno_air(cvb, met, q, pl, pr);
         else
@@ -2462,6 +2489,18 @@ static void update_bv(orc_zone *zz) {
         for (int l = 0; l < zz->nsc; ++l) SV(l, i, j, k) = ri * (rho_n * SV(l, i, j, k) + DQ(5 + l, i, j, k) * dtj);
         const double V2 = BV(1, i, j, k) * BV(1, i, j, k) + BV(2, i, j, k) * BV(2, i, j, k) +
                           BV(3, i, j, k) * BV(3, i, j, k);
+        if (zz->inviscid_type == 1) {
+          /* build definition: the AWENO path reconstructs cv, so the steady
+           * update keeps cv the updated conservative state (the reference
+           * leaves cv at its initial values, FieldOperation.cuh:221-296) */
+          const double rho = BV(0, i, j, k);
+          CV(0, i, j, k) = rho;
+          CV(1, i, j, k) = rho * BV(1, i, j, k);
+          CV(2, i, j, k) = rho * BV(2, i, j, k);
+          CV(3, i, j, k) = rho * BV(3, i, j, k);
+          CV(4, i, j, k) = E;
+          for (int l = 0; l < zz->nsc; ++l) CV(5 + l, i, j, k) = rho * SV(l, i, j, k);
+        }
         if (zz->cfg.mix) {
           double Y[ORC_MAXS], p;
           for (int l = 0; l < zz->ns; ++l) Y[l] = SV(l, i, j, k);
@@ -2522,8 +2561,9 @@ void orc_zone_run_steady(orc_zone *zz, int nsteps, double *res_hist) {
     (void)orc_zone_global_dt(zz);
     orc_zone_implicit(zz);
     update_bv(zz);
-    apply_bcs(zz, 0);
-    if (zz->dim == 2) eliminate_k_gradient(zz, 0);
+    /* ghost cv follows for the AWENO path, which reads it */
+    apply_bcs(zz, zz->inviscid_type == 1);
+    if (zz->dim == 2) eliminate_k_gradient(zz, zz->inviscid_type == 1);
     update_properties(zz);
     if (step % os == 0 || step == 1) {
       const double err = residual(zz, step);
@@ -2649,6 +2689,39 @@ void orc_aweno_mix_matrices(const orc_mech *mech, int nsc, const double *cl, con
     for (int r = 0; r < nv; ++r) R[r * nv + c] = col[r];
   }
   free(zz);
+}
+
+/* Mixture AWENO face states from six stencil points: c6 holds
+ * (Q[5 + nsc], p, T) of points i-2 .. i+3 (stride 7 + nsc), k0/k1 the metric
+ * rows of points i and i+1; pl/pr receive (rho, u, v, w, p, scalars, rho E,
+ * gamma).  Returns 1 when the sides failed the admissibility test and took
+ * the first-order values. */
+int orc_aweno_mix_sides(const orc_mech *mech, int nsc, const double *c6, const double *k0, const double *k1,
+                        double *pl, double *pr) {
+  orc_zone *zz = calloc(1, sizeof(orc_zone));
+  zz->ns = mech ? mech->n_spec : 0;
+  zz->nsc = nsc;
+  zz->nv = 5 + nsc;
+  zz->cfg.mix = mech != NULL;
+  if (mech) zz->mech = *mech;
+  const int nv = zz->nv, st = nv + 2;
+  double met[6 * 3], pv[6 * (5 + ORC_MAXS + 4)];
+  for (int q = 0; q < 6; ++q) {
+    const double *k = q < 3 ? k0 : k1;
+    for (int c = 0; c < 3; ++c) met[q * 3 + c] = k[c];
+    const double *Q = &c6[q * st];
+    pv[q * nv] = Q[0];
+    for (int l = 1; l < 4; ++l) pv[q * nv + l] = Q[l] / Q[0];
+    pv[q * nv + 4] = Q[nv];
+    for (int l = 5; l < nv; ++l) pv[q * nv + l] = Q[l] / Q[0];
+  }
+  aweno_mix(zz, c6, pv, met, 2, pl, pr);
+  double fl[5 + ORC_MAXS + 6], fr[5 + ORC_MAXS + 6];
+  reconstruct_order(zz, pv, 2, 1, fl, fr);
+  int same = 1;
+  for (int l = 0; l < nv + 2; ++l) same = same && fl[l] == pl[l] && fr[l] == pr[l];
+  free(zz);
+  return same;
 }
 
 /* Reconstruction.cuh:20-72 of air at one face: pv4 holds the primitive

@@ -176,6 +176,8 @@ void orc_jac_times_dq(const orc_mech *mech, int n_scalar, const double *b6, cons
 void orc_reconstruct_air(int reconstruction, const double *pv4, double *pl, double *pr);
 void orc_aweno_mix_matrices(const orc_mech *mech, int nsc, const double *cl, const double *cr, const double *k0,
                             const double *k1, double *L, double *R);
+int orc_aweno_mix_sides(const orc_mech *mech, int nsc, const double *c6, const double *k0, const double *k1,
+                        double *pl, double *pr);
 void orc_zone_store_last(orc_zone *zz);
 void orc_zone_source(orc_zone *zz);
 void orc_zone_residual(orc_zone *zz, int step, double *res5);

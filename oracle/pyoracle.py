@@ -141,6 +141,7 @@ def lib():
         L.orc_jac_times_dq.argtypes = [C.c_void_p, C.c_int, dp, dp, C.c_double, dp, C.c_double, dp, dp]
         L.orc_reconstruct_air.argtypes = [C.c_int, dp, dp, dp]
         L.orc_aweno_mix_matrices.argtypes = [C.c_void_p, C.c_int, dp, dp, dp, dp, dp, dp]
+        L.orc_aweno_mix_sides.argtypes = [C.c_void_p, C.c_int, dp, dp, dp, dp, dp]
         L.orc_zone_store_last.argtypes = [C.c_void_p]
         L.orc_zone_source.argtypes = [C.c_void_p]
         L.orc_zone_residual.argtypes = [C.c_void_p, C.c_int, dp]
@@ -423,7 +424,11 @@ class Zone:
         ptr = lib().orc_zone_field(self.h, FIELDS[name], C.byref(n))
         if n.value == 0:
             return np.zeros(0)
-        return np.ctypeslib.as_array(ptr, shape=(n.value,))
+        # the array keeps this zone alive (its memory is the zone's), so a
+        # view outliving the Python name of the zone never reads freed memory
+        buf = (C.c_double * n.value).from_address(C.cast(ptr, C.c_void_p).value)
+        buf._owner = self
+        return np.frombuffer(buf, dtype=np.float64)
 
     def initialize(self):
         lib().orc_zone_initialize(self.h)

@@ -34,7 +34,7 @@ def test_subsonic_channel(tmp_path):
 
 def _mixture_farfield(tmp_path, mech, sides, name):
     NS = 9
-    case = cases.c3_case(32, 16)
+    case = cases.c3_case(32, 16, reconstruction=2)
     Ya = np.zeros(NS)
     Ya[2], Ya[8] = 0.233, 0.767
     st = po.inflow_state(2.0, 101325.0, 1500.0, Y=Ya, m=mech)

@@ -18,8 +18,8 @@ CV14 = {"rho": [0], "mom": [1, 2, 3], "rhoE": [4], "rhoY": list(range(5, 14))}
 SPECIES = {"Y": list(range(NS))}
 
 
-def _setup(tmp_path, mech, nx=48, ny=24, fp32=False, scheme=4):
-    case = cases.c3_case(nx, ny, inviscid_scheme=scheme)
+def _setup(tmp_path, mech, nx=48, ny=24, fp32=False, scheme=4, recon=2):
+    case = cases.c3_case(nx, ny, inviscid_scheme=scheme, reconstruction=recon)
     s = ostc.solver_for_case(case, str(tmp_path / "c3"), initial=False)
     states = {}
     for name in ("air_stream", "fuel_stream"):

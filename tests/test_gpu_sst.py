@@ -109,8 +109,8 @@ def test_plate_fp32_flux(tmp_path):
     assert max(out.values()) < 1e-4, out
 
 
-def _c5(tmp_path, mech, nx=28, ny=20, nz=16):
-    case = cases.c5_case(nx=nx, ny=ny, nz=nz, lx=0.03, ly=0.01, lz=0.008, first=2e-5)
+def _c5(tmp_path, mech, nx=28, ny=20, nz=16, recon=2):
+    case = cases.c5_case(nx=nx, ny=ny, nz=nz, lx=0.03, ly=0.01, lz=0.008, first=2e-5, reconstruction=recon)
     s = ostc.solver_for_case(case, str(tmp_path / "c5"), initial=False)
     states = {}
     for name in ("vitiated_air", "h2_jet"):

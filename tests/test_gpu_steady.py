@@ -89,7 +89,7 @@ def test_c2_viscous_wall_dplur(tmp_path):
 
 @pytest.mark.parametrize("method,chem", [(1, 1), (0, 1), (0, 2)])
 def test_c3_reacting_steady(tmp_path, mech, method, chem):
-    case = _steady(cases.c3_case(32, 16), method, chem=chem)
+    case = _steady(cases.c3_case(32, 16, reconstruction=2), method, chem=chem)
     s, z = _mixture(case, tmp_path, mech)
     s.step_steady(1)
     z.run_steady(1)

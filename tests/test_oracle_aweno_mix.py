@@ -200,3 +200,57 @@ def test_scalar_awenos_flow_rows_reduce_to_air(tmp_path):
         out[nt] = z.field("dq").reshape(5 + nt, -1)[:5]
     scale = np.abs(out[0]).max(axis=1, keepdims=True)
     assert np.abs(out[2] - out[0]).max() < 1e-12 * scale.max(), np.abs(out[2] - out[0]).max() / scale.max()
+
+
+def _sides(mech, nsc, pts, k0, k1):
+    c6 = np.ascontiguousarray(np.concatenate(pts))
+    nv = 5 + nsc
+    pl, pr = np.zeros(nv + 2), np.zeros(nv + 2)
+    m = None if mech is None else po.C.cast(po.C.byref(mech), po.C.c_void_p)
+    fell_back = po.lib().orc_aweno_mix_sides(m, nsc, po._p(c6), po._p(np.ascontiguousarray(k0, dtype=float)),
+                                             po._p(np.ascontiguousarray(k1, dtype=float)), po._p(pl), po._p(pr))
+    return pl, pr, bool(fell_back)
+
+
+def test_admissibility_fallback_at_a_two_cell_jet(mech, tables):
+    """A jet port two cells wide in a wall row (air at 300 K, pure H2 at 300 K
+    and twice the pressure, sonic normal velocity): every candidate stencil of
+    the face between the two jet points crosses a jump, the WENO overshoot
+    drives the reconstructed density towards zero and the mass fractions far
+    outside [0, 1], and both sides take the first-order values -- the jet
+    state itself, with its own rho E and gamma."""
+    W = tables["mw"]
+    Ya = np.zeros(9)
+    Ya[2], Ya[7], Ya[8] = 0.25, 0.12, 0.63
+    Yj = np.zeros(9)
+    Yj[0] = 1.0
+    air = _cons(mech, W, 1.099, np.zeros(3), 300.0, Ya)
+    jet = _cons(mech, W, 0.1638, np.array([0.0, 1318.0, 0.0]), 300.0, Yj)
+    k = np.array([1.0, 0.0, 0.0])
+    pl, pr, fb = _sides(mech, 9, [air, air, jet, jet, air, air], k, k)
+    assert fb
+    p_jet = jet[14]
+    for side in (pl, pr):
+        np.testing.assert_allclose(side[:5], [0.1638, 0.0, 1318.0, 0.0, p_jet], rtol=1e-14)
+        np.testing.assert_allclose(side[5:14], Yj, atol=1e-15)
+        assert side[14] == pytest.approx(jet[4], rel=1e-12)       # rho E of the jet
+        assert side[15] == pytest.approx(1.405, abs=0.01)         # gamma of H2 at 300 K
+
+
+def test_admissible_smooth_faces_keep_the_weno_states(mech, tables):
+    """Smooth data: no fallback, and the sides agree with the midpoint value
+    of the smooth profile to the reconstruction's accuracy."""
+    W = tables["mw"]
+    xs = np.arange(-2, 4) * 0.02
+    pts = []
+    for x in xs:
+        Y = np.array([0.02 + 0.01 * x, 0.001, 0.22 - 0.02 * x, 0.002, 0.003, 0.0, 0.0, 0.1 + 0.05 * x, 0.0])
+        Y[8] = 1.0 - Y[:8].sum()
+        pts.append(_cons(mech, W, 0.4 + 0.1 * x, np.array([600.0 + 100 * x, 20.0, 0.0]), 1500.0 + 200 * x, Y))
+    k = np.array([1.0, 0.0, 0.0])
+    pl, pr, fb = _sides(mech, 9, pts, k, k)
+    assert not fb
+    mid = 0.5 * (pts[2] + pts[3])
+    assert pl[0] == pytest.approx(mid[0], rel=1e-3) and pr[0] == pytest.approx(mid[0], rel=1e-3)
+    assert pl[1] == pytest.approx(mid[1] / mid[0], rel=1e-3)
+    assert abs(pl[5:14].sum() - 1) < 1e-10 and abs(pr[5:14].sum() - 1) < 1e-10

@@ -555,7 +555,7 @@ def test_couette_shear_dissipation(variant, mech):
         mw = cases.MW_AIR
         T, p = 300.0, 2.0e4
     else:
-        case = cases.c3_case(24, 16, dx=d)
+        case = cases.c3_case(24, 16, dx=d, reconstruction=2)
         case["scheme"]["reaction"] = 0
         ns, m = 9, mech
         Y = np.zeros(ns)

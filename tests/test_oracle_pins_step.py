@@ -173,7 +173,7 @@ def test_cfl_time_step_closed_form_air(dim):
 
 def test_cfl_time_step_closed_form_mixture(mech, tables):
     d = 1e-4
-    case = cases.c3_case(12, 10, dx=d)
+    case = cases.c3_case(12, 10, dx=d, reconstruction=2)
     case["scheme"]["reaction"] = 0
     case["perturb"] = None
     for b in case["bcs"]:

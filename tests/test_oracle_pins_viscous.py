@@ -112,7 +112,7 @@ def _slab_case(variant, mech, n=24, ny=12):
         d = 1e-3
     else:
         d = 1e-4
-        case = cases.c3_case(n, ny, dx=d)
+        case = cases.c3_case(n, ny, dx=d, reconstruction=2)
         case["scheme"]["reaction"] = 0
         m = mech
     case["perturb"] = None
