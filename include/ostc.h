@@ -86,14 +86,43 @@ int ostc_add_block(ostc_solver *s, int mx, int my, int mz, const double *x, cons
 int ostc_add_boundary(ostc_solver *s, int blk, int i1, int i2, int j1, int j2, int k1, int k2, int label);
 /* Reads <case_dir>/input_files/boundary_condition/boundary   0.txt for all blocks. */
 int ostc_read_boundary_file(ostc_solver *s);
-/* Allocates the device zones and resolves boundary labels against the
- * setup-file structs.  Replaces: Field::setup_device_memory (Field.cu:133-251),
- * DBoundCond::initialize_bc_on_GPU (BoundCond.cu:167-384). */
+/* Reads <case_dir>/input_files/grid/grid   0.grd (block count, extents, then
+ * per block all x, all y, all z with i fastest), scales it by gridScale and
+ * adds every block.  Replaces: Mesh::read_grid + Mesh::scale
+ * (Mesh.cpp:599-686, 571-578, 776-788). */
+int ostc_read_grid(ostc_solver *s);
+/* One inner face of block blk: the source and target lines of the reference's
+ * inner-face file (7 integers each, 1-based; the target line ends with the
+ * target block, 1-based; in 3-D one tangential pair is negative to state the
+ * orientation).  2-D faces must join the same index direction on both sides.
+ * Replaces: InnerFace::InnerFace + register_boundary (Mesh.cpp:34-187). */
+int ostc_add_inner_face(ostc_solver *s, int blk, const int *src7, const int *tar7);
+/* Reads <case_dir>/input_files/boundary_condition/inner   0.txt.
+ * Replaces: Mesh::read_inner_interface (Mesh.cpp:710-742). */
+int ostc_read_inner_file(ostc_solver *s);
+/* Grid, boundary file and (when present) inner-face file of the case.
+ * Replaces: the reading part of Mesh::Mesh (Mesh.cpp:550-568). */
+int ostc_read_mesh(ostc_solver *s);
+/* Number of blocks, and the interior extents (mx, my, mz) of one. */
+int ostc_block_count(ostc_solver *s, int *n);
+int ostc_block_dims(ostc_solver *s, int blk, int *dims);
+/* Registered inner faces of a block: *count faces; out (31 ints per face, up
+ * to cap faces): rs[3] re[3] face dir ts[3] te[3] target_block target_face
+ * target_dir src_tar[3] loop[3] target_loop[3] n[3] 0 0 (0-based). */
+int ostc_inner_faces(ostc_solver *s, int blk, int *count, int *out, int cap);
+/* Allocates the device zones, builds the ghost nodes of every block (those
+ * beyond an inner face from the neighbour block), computes the Jacobian and
+ * metrics on the device and resolves boundary labels against the setup-file
+ * structs.  Replaces: Field::setup_device_memory (Field.cu:133-251),
+ * init_ghost_grid + init_inner_ghost_grid + compute_jac_metric
+ * (Mesh.cpp:790-994, 259-521), DBoundCond::initialize_bc_on_GPU
+ * (BoundCond.cu:167-384). */
 int ostc_finalize(ostc_solver *s);
 
-/* Host copies of a block's Jacobian (ghosted) and 9 metric components
- * (component-major), available before ostc_finalize and in host-only mode.
- * Either pointer may be NULL. */
+/* A block's Jacobian (ghosted) and 9 metric components (component-major):
+ * before ostc_finalize and in host-only mode the host computation of the
+ * block on its own (reflected ghosts); after ostc_finalize the device arrays
+ * in use.  Either pointer may be NULL. */
 int ostc_block_geometry(ostc_solver *s, int blk, double *jac, double *metric);
 
 /* Field shape: element count, ghost layers, components. */
@@ -152,11 +181,17 @@ int ostc_rk3_stage(ostc_solver *s, int blk, double dt, int rk);
 /* Boundary conditions in the reference priority order.
  * Replaces: DBoundCond::apply_boundary_conditions (BoundCond.cuh:673-822). */
 int ostc_apply_bc(ostc_solver *s, int blk, int with_cv);
+/* Inner-face exchange of every block in order: face layers averaged with
+ * the neighbour, ghost layers from the neighbour's interior, cv refreshed
+ * with with_cv.  Replaces: data_communication / inner_communication
+ * (DataCommunication.cuh:27-110). */
+int ostc_exchange(ostc_solver *s, int with_cv);
 /* Stores rho, |V|, p, T of one block as the reference for the next residual.
  * Replaces: store_last_step (TimeAdvanceFunc.cu:6-17). */
 int ostc_store_last(ostc_solver *s, int blk);
-/* Residual of a single-block case against the last ostc_store_last:
- * res[l] = sqrt(sum (q_l - q_l,last)^2 / N) for q = (rho, |V|, p, T), divided
+/* Residual over all blocks against the last ostc_store_last of each:
+ * res[l] = sqrt(sum (q_l - q_l,last)^2 / N) for q = (rho, |V|, p, T), N the
+ * points of all blocks, divided
  * by the values of the first call with step == 1 (floor 1e-20);
  * err_max = their maximum.  Synchronises; a NaN residual is an error.
  * Replaces: compute_residual (Residual.cuh:57-161, Residual.cu:49-89). */

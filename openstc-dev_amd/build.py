@@ -17,7 +17,7 @@ OBJ = os.path.join(HERE, "_build")
 OUT = os.path.join(HERE, "libostc.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
-SOURCES = ["setup_io.cpp", "chemkin_io.cpp", "mesh.cpp", "api.cpp", "flux.hip", "update.hip", "implicit.hip", "sst.hip", "mlp.hip", "surrogate.hip"]
+SOURCES = ["setup_io.cpp", "chemkin_io.cpp", "mesh.cpp", "api.cpp", "flux.hip", "update.hip", "implicit.hip", "sst.hip", "mlp.hip", "surrogate.hip", "grid.hip"]
 
 COMMON = ["-O3", "-std=c++17", "-fPIC", "--offload-arch=gfx950", "-Xclang", "-target-feature", "-Xclang",
           "-packed-fp32-ops", "-Wall", "-Wno-unused-function", "-Wno-unused-variable",

@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -54,8 +55,10 @@ struct WallBC {
 
 struct Block {
   int mx = 0, my = 0, mz = 0;
-  std::vector<double> jac, metric;  // host copies (ghosted, component-major metric)
+  std::vector<double> jac, metric;  // host copies (ghosted, component-major metric) of a lone block
   std::vector<double> xyz;          // ghosted node coordinates (SST wall distance only)
+  std::vector<double> x0, y0, z0;   // interior node coordinates, i fastest
+  std::vector<InnerFace> inner;     // inner faces to other blocks, in file order
   std::vector<BoundaryFace> faces;
   std::vector<BCDesc> bcs;          // in application order
   ZoneView z{};
@@ -108,6 +111,7 @@ struct ostc_solver {
     (void)hipFree(w.res_hist);
     (void)hipFree(w.res_scale);
     (void)hipFree(w.res_one);
+    (void)hipFree(w.blk_dt);
     for (auto &p : pending) {
       (void)hipEventDestroy(p.a);
       (void)hipEventDestroy(p.b);
@@ -568,6 +572,89 @@ void register_face(BoundaryFace &b, int ngg, int dim) {
   }
 }
 
+// Ghosted node coordinates ((ngg + 1) layers) of every block, with the
+// ghosts beyond inner faces taken from the neighbour (Mesh.cpp:790-944), and
+// a check that every inner face's ranges lie inside both blocks.
+int build_nodes(ostc_solver *s, std::vector<std::array<std::vector<double>, 3>> &nodes) {
+  const int g = s->ngg;
+  const size_t nb = s->blocks.size();
+  nodes.resize(nb);
+  std::vector<Layout> Lc(nb);
+  for (size_t bi = 0; bi < nb; ++bi) {
+    Block &b = s->blocks[bi];
+    Lc[bi] = Layout(b.mx, b.my, b.mz, g + 1);
+    const double *src[3] = {b.x0.data(), b.y0.data(), b.z0.data()};
+    for (int c = 0; c < 3; ++c) {
+      auto &A = nodes[bi][c];
+      A.assign(Lc[bi].sz, 0.0);
+      for (int k = 0; k < b.mz; ++k)
+        for (int j = 0; j < b.my; ++j)
+          for (int i = 0; i < b.mx; ++i) A[Lc[bi].at(i, j, k)] = src[c][((size_t)k * b.my + j) * b.mx + i];
+      ghost_node_faces(b.mx, b.my, b.mz, g, Lc[bi], A.data(), c == 2);
+    }
+  }
+  auto inside = [&](const Block &b, const int *ijk, int lo_pad, int hi_pad) {
+    const int m[3] = {b.mx, b.my, b.mz};
+    for (int d = 0; d < 3; ++d)
+      if (ijk[d] < -lo_pad || ijk[d] >= m[d] + hi_pad) return false;
+    return true;
+  };
+  for (size_t bi = 0; bi < nb; ++bi) {
+    const Block &b = s->blocks[bi];
+    for (size_t q = 0; q < b.inner.size(); ++q) {
+      const InnerFace &f = b.inner[q];
+      const Block &t = s->blocks[f.tblk];
+      // the exchange walks f.n points per direction from rs (this block) and
+      // ts (target); the node copy reaches one layer further out
+      for (int corner = 0; corner < 8; ++corner) {
+        int d[3], a[3], at[3];
+        for (int x = 0; x < 3; ++x) d[x] = ((corner >> x) & 1) ? f.loop[x] * (f.n[x] - 1) : 0;
+        for (int x = 0; x < 3; ++x) {
+          a[x] = f.rs[x] + d[x];
+          at[x] = f.ts[x] + f.tloop[x] * d[f.src_tar[x]];
+        }
+        if (!inside(b, a, g, g) || !inside(t, at, g, g))
+          return fail("inner face " + std::to_string(q) + " of block " + std::to_string(bi) +
+                      " does not fit the blocks it joins");
+        if (f.n[f.src_tar[f.tface]] != f.n[f.face] || b.mz != t.mz)
+          return fail("inner face " + std::to_string(q) + " of block " + std::to_string(bi) + " is inconsistent");
+      }
+      double *B[3] = {nodes[bi][0].data(), nodes[bi][1].data(), nodes[bi][2].data()};
+      const double *N[3] = {nodes[f.tblk][0].data(), nodes[f.tblk][1].data(), nodes[f.tblk][2].data()};
+      inner_ghost_nodes(f, Lc[bi], B, Lc[f.tblk], N);
+    }
+  }
+  for (size_t bi = 0; bi < nb; ++bi) {
+    Block &b = s->blocks[bi];
+    for (int c = 0; c < 3; ++c) ghost_node_edges(b.mx, b.my, b.mz, g, Lc[bi], nodes[bi][c].data());
+  }
+  return 0;
+}
+
+// inner-face exchange of every block in order (DataCommunication.cuh:27-56)
+int exchange_all(ostc_solver *s, bool with_cv) {
+  bool any = false;
+  for (auto &b : s->blocks) any = any || !b.inner.empty();
+  if (!any) return 0;
+  ostc_solver::Timer t(s, "inner_exchange");
+  for (auto &b : s->blocks)
+    for (const InnerFace &f : b.inner) {
+      InnerFaceDesc d;
+      for (int x = 0; x < 3; ++x) {
+        d.rs[x] = f.rs[x];
+        d.ts[x] = f.ts[x];
+        d.src_tar[x] = f.src_tar[x];
+        d.loop[x] = f.loop[x];
+        d.tloop[x] = f.tloop[x];
+        d.n[x] = f.n[x];
+      }
+      d.face = f.face;
+      d.face_at = f.dir > 0 ? f.rs[f.face] : f.re[f.face];
+      HIPCHK(launch_inner_exchange(b.z, s->blocks[f.tblk].z, s->P, s->d_mech, d, with_cv, s->stream));
+    }
+  return 0;
+}
+
 int run_bcs(ostc_solver *s, Block &b, bool with_cv) {
   ostc_solver::Timer t(s, "boundary");
   for (auto &bc : b.bcs) HIPCHK(launch_bc(b.z, s->P, s->d_mech, bc, with_cv, s->stream));
@@ -602,6 +689,25 @@ int rhs_block(ostc_solver *s, Block &b) {
 int properties_block(ostc_solver *s, Block &b, bool k_planes_copied) {
   ostc_solver::Timer t(s, "properties");
   HIPCHK(launch_properties(b.z, s->P, s->d_mech, k_planes_copied, s->stream));
+  return 0;
+}
+
+// CFL dt of the stage into w.dt: the minimum over every block
+// (TimeAdvanceFunc.cu:155-201), on the device
+int global_dt_device(ostc_solver *s) {
+  ostc_solver::Timer t(s, "local_dt");
+  const int nb = (int)s->blocks.size();
+  for (int bi = 0; bi < nb; ++bi)
+    HIPCHK(launch_local_dt(s->blocks[bi].z, s->P, s->w, nb == 1 ? s->w.dt : s->w.blk_dt + bi, s->stream));
+  if (nb > 1) HIPCHK(launch_min_of(s->w.blk_dt, nb, s->w.dt, s->stream));
+  return 0;
+}
+
+// residual over every block (Residual.cuh:57-139) into res_hist[slot] or out
+int residual_all(ostc_solver *s, int slot, int first, double *out) {
+  std::vector<const ZoneView *> zs;
+  for (auto &b : s->blocks) zs.push_back(&b.z);
+  HIPCHK(launch_residual_blocks(zs, s->w, slot, first, s->stream, out));
   return 0;
 }
 
@@ -791,8 +897,140 @@ int ostc_add_block(ostc_solver *s, int mx, int my, int mz, const double *x, cons
   std::string err;
   if (!compute_metrics(mx, my, mz, s->ngg, x, y, z, b.jac, b.metric, err, s->n_turb ? &b.xyz : nullptr))
     return fail(err);
+  const size_t n = (size_t)mx * my * mz;
+  b.x0.assign(x, x + n);
+  b.y0.assign(y, y + n);
+  b.z0.assign(z, z + n);
   s->blocks.push_back(std::move(b));
   if (blk) *blk = (int)s->blocks.size() - 1;
+  return 0;
+}
+
+int ostc_read_grid(ostc_solver *s) {
+  if (!s) return fail("null solver");
+  if (s->finalized) return fail("the grid must be read before ostc_finalize");
+  if (!s->blocks.empty()) return fail("blocks were already added");
+  const std::string path = s->case_dir + "/input_files/grid/grid   0.grd";
+  std::ifstream f(path);
+  if (!f) return fail("grid file not found: " + path);
+  // Mesh.cpp:599-686: block count, the three extents of every block, then
+  // per block all x (i fastest, then j, then k), all y, all z
+  std::string line;
+  if (!std::getline(f, line)) return fail("grid file is empty");
+  int nb = 0;
+  try {
+    nb = std::stoi(line);
+  } catch (...) {
+    return fail("grid file: bad block count");
+  }
+  if (nb <= 0) return fail("grid file: no blocks");
+  std::vector<int> dims(3 * (size_t)nb);
+  for (int b = 0; b < nb; ++b) {
+    if (!std::getline(f, line)) return fail("grid file ends in the block extents");
+    std::istringstream ls(line);
+    if (!(ls >> dims[3 * b] >> dims[3 * b + 1] >> dims[3 * b + 2])) return fail("grid file: bad block extents");
+  }
+  // Mesh.cpp:571-578: coordinates scaled to metres unless gridScale is 1
+  const double scale = s->prm.get_real("gridScale", 1.0);
+  const bool do_scale = std::abs(scale - 1.0) > 1e-15;
+  for (int b = 0; b < nb; ++b) {
+    const size_t n = (size_t)dims[3 * b] * dims[3 * b + 1] * dims[3 * b + 2];
+    std::vector<double> c[3];
+    for (int a = 0; a < 3; ++a) {
+      c[a].resize(n);
+      for (size_t q = 0; q < n; ++q)
+        if (!(f >> c[a][q])) return fail("grid file ends in the coordinates of block " + std::to_string(b));
+      if (do_scale)
+        for (double &v : c[a]) v *= scale;
+    }
+    if (ostc_add_block(s, dims[3 * b], dims[3 * b + 1], dims[3 * b + 2], c[0].data(), c[1].data(), c[2].data(),
+                       nullptr))
+      return 1;
+  }
+  return 0;
+}
+
+int ostc_add_inner_face(ostc_solver *s, int blk, const int *src7, const int *tar7) {
+  if (!s || !src7 || !tar7) return fail("null argument");
+  if (s->finalized) return fail("inner faces must be added before ostc_finalize");
+  if (blk < 0 || blk >= (int)s->blocks.size()) return fail("block index out of range");
+  InnerFace f;
+  std::string err;
+  const int dim = s->blocks[0].mz == 1 ? 2 : 3;
+  if (!register_inner_face(src7, tar7, s->ngg, dim, f, err)) return fail(err);
+  if (f.tblk < 0 || f.tblk >= (int)s->blocks.size()) return fail("inner face targets a block that does not exist");
+  s->blocks[blk].inner.push_back(f);
+  return 0;
+}
+
+int ostc_read_inner_file(ostc_solver *s) {
+  if (!s) return fail("null solver");
+  const std::string path = s->case_dir + "/input_files/boundary_condition/inner   0.txt";
+  std::ifstream f(path);
+  if (!f) return fail("inner-face file not found: " + path);
+  // Mesh.cpp:710-742: the face count of every block, a header line, then
+  // per face a source line and a target line of seven integers
+  std::string line;
+  if (!std::getline(f, line)) return fail("inner-face file is empty");
+  const int nb = (int)s->blocks.size();
+  std::vector<int> nface(nb);
+  for (int b = 0; b < nb; ++b)
+    if (!(f >> nface[b])) return fail("inner-face file: missing face count of block " + std::to_string(b));
+  std::getline(f, line);
+  std::getline(f, line);
+  for (int b = 0; b < nb; ++b)
+    for (int q = 0; q < nface[b]; ++q) {
+      int a[7] = {0, 0, 0, 0, 0, 0, 0}, t[7] = {0, 0, 0, 0, 0, 0, 0};
+      if (!std::getline(f, line)) return fail("inner-face file ends early");
+      std::istringstream l1(line);
+      for (int &v : a) l1 >> v;
+      if (!std::getline(f, line)) return fail("inner-face file ends early");
+      std::istringstream l2(line);
+      for (int &v : t) l2 >> v;
+      if (ostc_add_inner_face(s, b, a, t)) return 1;
+    }
+  return 0;
+}
+
+int ostc_read_mesh(ostc_solver *s) {
+  // Mesh.cpp:550-568: grid, boundary conditions, inner faces
+  if (ostc_read_grid(s)) return 1;
+  if (ostc_read_boundary_file(s)) return 1;
+  std::ifstream probe(s->case_dir + "/input_files/boundary_condition/inner   0.txt");
+  if (probe && ostc_read_inner_file(s)) return 1;
+  return 0;
+}
+
+int ostc_block_dims(ostc_solver *s, int blk, int *dims) {
+  if (!s || !dims) return fail("null argument");
+  if (blk < 0 || blk >= (int)s->blocks.size()) return fail("block index out of range");
+  dims[0] = s->blocks[blk].mx;
+  dims[1] = s->blocks[blk].my;
+  dims[2] = s->blocks[blk].mz;
+  return 0;
+}
+
+int ostc_block_count(ostc_solver *s, int *n) {
+  if (!s || !n) return fail("null argument");
+  *n = (int)s->blocks.size();
+  return 0;
+}
+
+int ostc_inner_faces(ostc_solver *s, int blk, int *count, int *out, int cap) {
+  if (!s) return fail("null solver");
+  if (blk < 0 || blk >= (int)s->blocks.size()) return fail("block index out of range");
+  const auto &v = s->blocks[blk].inner;
+  if (count) *count = (int)v.size();
+  if (out)
+    for (int q = 0; q < (int)v.size() && q < cap; ++q) {
+      const InnerFace &f = v[q];
+      int *o = out + 31 * q;
+      const int vals[31] = {f.rs[0], f.rs[1], f.rs[2], f.re[0], f.re[1], f.re[2], f.face, f.dir,
+                            f.ts[0], f.ts[1], f.ts[2], f.te[0], f.te[1], f.te[2], f.tblk, f.tface,
+                            f.tdir, f.src_tar[0], f.src_tar[1], f.src_tar[2], f.loop[0], f.loop[1], f.loop[2],
+                            f.tloop[0], f.tloop[1], f.tloop[2], f.n[0], f.n[1], f.n[2], 0, 0};
+      std::copy(vals, vals + 31, o);
+    }
   return 0;
 }
 
@@ -901,7 +1139,17 @@ int ostc_finalize(ostc_solver *s) {
       return fail("subsonic inflow and back pressure boundaries are defined for air only (BoundCond.cpp:300-364)");
   // allocate zones
   const int g = s->ngg;
-  int64_t max_cells = 0;
+  int64_t max_cells = 0, red_blocks = 0;
+  std::vector<std::array<std::vector<double>, 3>> nodes;
+  if (build_nodes(s, nodes)) return 1;
+  int *d_bad_interior = nullptr;
+  unsigned char *d_bad = nullptr;
+  double *d_nodes = nullptr;
+  auto free_geo = [&]() {
+    (void)hipFree(d_bad_interior);
+    (void)hipFree(d_bad);
+    (void)hipFree(d_nodes);
+  };
   for (auto &b : s->blocks) {
     ZoneView &z = b.z;
     z.mx = b.mx;
@@ -920,6 +1168,7 @@ int ostc_finalize(ostc_solver *s) {
     z.e1 = L0.d1;
     z.sz0 = L0.sz;
     max_cells = std::max<int64_t>(max_cells, L0.sz);
+    red_blocks += (L0.sz + 255) / 256 + 1;
     auto alloc = [&](double **p, int64_t n) -> int {
       n = std::max<int64_t>(n, 1);
       HIPCHK(hipMalloc((void **)p, n * sizeof(double)));
@@ -947,9 +1196,40 @@ int ostc_finalize(ostc_solver *s) {
       return 1;
     z.jac = jac;
     z.metric = met;
-    HIPCHK(hipMemcpyAsync(jac, b.jac.data(), L.sz * sizeof(double), hipMemcpyHostToDevice, s->stream));
-    HIPCHK(hipMemcpyAsync(met, b.metric.data(), L.sz * 9 * sizeof(double), hipMemcpyHostToDevice, s->stream));
-    HIPCHK(hipStreamSynchronize(s->stream));
+    {
+      // Jacobian and metrics on the device from the ghosted nodes
+      // (Mesh.cpp:259-521), bit-identical to the host computation
+      const size_t bi = &b - &s->blocks[0];
+      const Layout Lc(b.mx, b.my, b.mz, g + 1);
+      hipError_t e = hipMalloc(&d_nodes, sizeof(double) * 3 * Lc.sz);
+      if (e == hipSuccess) e = hipMalloc(&d_bad, L.sz);
+      if (e == hipSuccess) e = hipMalloc(&d_bad_interior, sizeof(int));
+      if (e == hipSuccess) e = hipMemsetAsync(d_bad_interior, 0, sizeof(int), s->stream);
+      for (int c = 0; c < 3 && e == hipSuccess; ++c)
+        e = hipMemcpyAsync(d_nodes + c * Lc.sz, nodes[bi][c].data(), sizeof(double) * Lc.sz, hipMemcpyHostToDevice,
+                           s->stream);
+      if (e == hipSuccess)
+        e = launch_metrics(b.mx, b.my, b.mz, g, d_nodes, d_nodes + Lc.sz, d_nodes + 2 * Lc.sz, jac, met, d_bad,
+                           d_bad_interior, s->stream);
+      int bad_interior = 0;
+      if (e == hipSuccess)
+        e = hipMemcpyAsync(&bad_interior, d_bad_interior, sizeof(int), hipMemcpyDeviceToHost, s->stream);
+      if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+      free_geo();
+      d_nodes = nullptr;
+      d_bad = nullptr;
+      d_bad_interior = nullptr;
+      if (e != hipSuccess) return fail(std::string("device metrics: ") + hipGetErrorString(e));
+      if (bad_interior) return fail("negative Jacobian in the interior of block " + std::to_string(bi));
+      if (s->n_turb) {
+        // ghosted node coordinates for the wall distance (ngg layers)
+        b.xyz.assign((size_t)L.sz * 3, 0);
+        for (int k = -g; k < b.mz + g; ++k)
+          for (int j = -g; j < b.my + g; ++j)
+            for (int i = -g; i < b.mx + g; ++i)
+              for (int c = 0; c < 3; ++c) b.xyz[(size_t)c * L.sz + L.at(i, j, k)] = nodes[bi][c][Lc.at(i, j, k)];
+      }
+    }
     // boundaries: register, trim walls (BoundCond.cu:369-384), order by priority
     for (auto &f : b.faces) {
       register_face(f, g, dim);
@@ -1055,8 +1335,9 @@ int ostc_finalize(ostc_solver *s) {
   }
   if (s->n_turb && compute_wall_distance(s)) return 1;
   // per-run scratch: partials, ticket counter, dt slot, residual scale
-  s->w.max_blocks = (int)((max_cells + 255) / 256) + 1;
+  s->w.max_blocks = (int)std::max<int64_t>((max_cells + 255) / 256 + 1, red_blocks);
   HIPCHK(hipMalloc(&s->w.partial, sizeof(double) * 4 * s->w.max_blocks));
+  HIPCHK(hipMalloc(&s->w.blk_dt, sizeof(double) * s->blocks.size()));
   HIPCHK(hipMalloc(&s->w.counter, sizeof(unsigned)));
   HIPCHK(hipMemsetAsync(s->w.counter, 0, sizeof(unsigned), s->stream));
   HIPCHK(hipMalloc(&s->w.dt, sizeof(double)));
@@ -1071,6 +1352,15 @@ int ostc_block_geometry(ostc_solver *s, int blk, double *jac, double *metric) {
   if (!s) return fail("null solver");
   if (blk < 0 || blk >= (int)s->blocks.size()) return fail("block index out of range");
   const Block &b = s->blocks[blk];
+  if (s->finalized) {
+    // the metrics in use: computed on the device with the inner faces' ghosts
+    HIPCHK(hipSetDevice(s->device));
+    if (jac) HIPCHK(hipMemcpyAsync(jac, b.z.jac, sizeof(double) * b.z.sz, hipMemcpyDeviceToHost, s->stream));
+    if (metric)
+      HIPCHK(hipMemcpyAsync(metric, b.z.metric, sizeof(double) * 9 * b.z.sz, hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    return 0;
+  }
   if (jac) std::copy(b.jac.begin(), b.jac.end(), jac);
   if (metric) std::copy(b.metric.begin(), b.metric.end(), metric);
   return 0;
@@ -1153,9 +1443,8 @@ int ostc_store_last(ostc_solver *s, int blk) {
 
 int ostc_residual(ostc_solver *s, int step, double res[4], double *err_max) {
   if (check_blk(s, 0)) return 1;
-  if (s->blocks.size() != 1) return fail("ostc_residual runs single-block cases");
   HIPCHK(hipSetDevice(s->device));
-  HIPCHK(launch_residual(s->blocks[0].z, s->w, 0, step == 1 ? 1 : 0, s->stream, s->w.res_one));
+  if (residual_all(s, 0, step == 1 ? 1 : 0, s->w.res_one)) return 1;
   double h[5];
   HIPCHK(hipMemcpyAsync(h, s->w.res_one, sizeof(h), hipMemcpyDeviceToHost, s->stream));
   HIPCHK(hipStreamSynchronize(s->stream));
@@ -1203,15 +1492,22 @@ int ostc_initialize_uniform(ostc_solver *s, int blk) {
 int ostc_initialize(ostc_solver *s) {
   if (check_blk(s, 0)) return 1;
   HIPCHK(hipSetDevice(s->device));
+  // Driver.cu:40-98 phase by phase over all blocks, then RK.cuh:38-48
+  if (s->P.dim == 2)
+    for (auto &b : s->blocks) HIPCHK(launch_eliminate_k(b.z, s->P, false, s->stream));
   for (auto &b : s->blocks) {
-    if (s->P.dim == 2) HIPCHK(launch_eliminate_k(b.z, s->P, false, s->stream));
     // the mixture farfield reads gamma of its boundary point; the reference
     // applies the first BCs before any properties exist (Driver.cu:53-92,
     // reading unset memory), here they are computed from the initial field
     if (s->P.mix && properties_block(s, b, true)) return 1;
     if (run_bcs(s, b, false)) return 1;
+  }
+  for (auto &b : s->blocks) {
     HIPCHK(launch_velocity(b.z, s->stream));
     if (s->n_turb) HIPCHK(launch_init_mut(b.z, s->P, s->d_mech, s->stream));
+  }
+  if (exchange_all(s, false)) return 1;
+  for (auto &b : s->blocks) {
     if (properties_block(s, b, false)) return 1;
     HIPCHK(launch_store_last(b.z, s->stream));
     HIPCHK(launch_cv_from_bv(b.z, s->P, s->d_mech, s->stream));
@@ -1262,6 +1558,14 @@ int ostc_apply_bc(ostc_solver *s, int blk, int with_cv) {
   return 0;
 }
 
+int ostc_exchange(ostc_solver *s, int with_cv) {
+  if (check_blk(s, 0)) return 1;
+  HIPCHK(hipSetDevice(s->device));
+  if (exchange_all(s, with_cv != 0)) return 1;
+  HIPCHK(hipStreamSynchronize(s->stream));
+  return 0;
+}
+
 int ostc_update_properties(ostc_solver *s, int blk) {
   if (check_blk(s, blk)) return 1;
   Block &b = s->blocks[blk];
@@ -1276,9 +1580,7 @@ int ostc_step_rk3(ostc_solver *s, int nsteps, double *dt_hist, double *res_hist)
   if (nsteps < 0) return fail("negative step count");
   if (nsteps == 0) return 0;
   HIPCHK(hipSetDevice(s->device));
-  if (s->blocks.size() != 1) return fail("the fused RK3 driver runs single-block cases");
   if (ensure_hist(s, nsteps)) return 1;
-  Block &b = s->blocks[0];
   double *d_dt = s->w.dt_hist, *d_res = s->w.res_hist;
   const bool fixed = s->prm.get_bool("fixed_time_step", false);
   const double fixed_dt = s->prm.get_real("dt", 1e-5);
@@ -1288,27 +1590,32 @@ int ostc_step_rk3(ostc_solver *s, int nsteps, double *dt_hist, double *res_hist)
     if (fixed) HIPCHK(hipMemcpyAsync(s->w.dt, &fixed_dt, sizeof(double), hipMemcpyHostToDevice, s->stream));
     for (int it = 0; it < nsteps; ++it) {
       const int step = s->step + 1 + it;
-      if (step % os == 0) HIPCHK(launch_store_last(b.z, s->stream));
-      HIPCHK(hipMemcpyAsync(b.z.qn, b.z.cv, b.z.sz * s->n_var * sizeof(double), hipMemcpyDeviceToDevice,
-                            s->stream));
+      for (auto &b : s->blocks) {
+        if (step % os == 0) HIPCHK(launch_store_last(b.z, s->stream));
+        HIPCHK(hipMemcpyAsync(b.z.qn, b.z.cv, b.z.sz * s->n_var * sizeof(double), hipMemcpyDeviceToDevice,
+                              s->stream));
+      }
       for (int rk = 0; rk < 3; ++rk) {
-        if (rhs_block(s, b)) return 1;
-        if (!fixed) {
-          ostc_solver::Timer t(s, "local_dt");
-          HIPCHK(launch_local_dt(b.z, s->P, s->w, s->w.dt, s->stream));
-        }
+        for (auto &b : s->blocks)
+          if (rhs_block(s, b)) return 1;
+        if (!fixed && global_dt_device(s)) return 1;
         HIPCHK(launch_copy_dt(s->w.dt, d_dt + it * 3 + rk, s->stream));
-        {
-          ostc_solver::Timer t(s, "rk_update");
-          HIPCHK(launch_rk_update(b.z, s->P, s->d_mech, s->w.dt, rk, s->stream));
+        for (auto &b : s->blocks) {
+          {
+            ostc_solver::Timer t(s, "rk_update");
+            HIPCHK(launch_rk_update(b.z, s->P, s->d_mech, s->w.dt, rk, s->stream));
+          }
+          if (run_bcs(s, b, true)) return 1;
         }
-        if (run_bcs(s, b, true)) return 1;
-        if (s->P.dim == 2) HIPCHK(launch_eliminate_k(b.z, s->P, true, s->stream));
-        if (properties_block(s, b, true)) return 1;
+        if (exchange_all(s, true)) return 1;
+        if (s->P.dim == 2)
+          for (auto &b : s->blocks) HIPCHK(launch_eliminate_k(b.z, s->P, true, s->stream));
+        for (auto &b : s->blocks)
+          if (properties_block(s, b, true)) return 1;
       }
       if (step % os == 0 || step == 1) {
         ostc_solver::Timer t(s, "residual");
-        HIPCHK(launch_residual(b.z, s->w, it, step == 1 ? 1 : 0, s->stream));
+        if (residual_all(s, it, step == 1 ? 1 : 0, nullptr)) return 1;
       }
     }
     return 0;
@@ -1371,33 +1678,36 @@ int ostc_step_steady(ostc_solver *s, int nsteps, double *res_hist) {
   if (nsteps < 0) return fail("negative step count");
   if (nsteps == 0) return 0;
   HIPCHK(hipSetDevice(s->device));
-  if (s->blocks.size() != 1) return fail("the steady driver runs single-block cases");
   if (ensure_hist(s, nsteps)) return 1;
-  Block &b = s->blocks[0];
   double *d_res = s->w.res_hist;
   const int os = std::max(1, s->prm.get_int("output_screen", 1));
   auto body = [&]() -> int {
+    // ghost cv follows for the AWENO path, which reads it
+    const bool with_cv = s->P.inviscid_type == 1;
     for (int it = 0; it < nsteps; ++it) {
       const int step = s->step + 1 + it;
-      if (step % os == 0) HIPCHK(launch_store_last(b.z, s->stream));
-      if (rhs_block(s, b)) return 1;
-      {
-        ostc_solver::Timer t(s, "local_dt");
-        HIPCHK(launch_local_dt(b.z, s->P, s->w, s->w.dt, s->stream));
+      for (auto &b : s->blocks) {
+        if (step % os == 0) HIPCHK(launch_store_last(b.z, s->stream));
+        if (rhs_block(s, b)) return 1;
+        {
+          ostc_solver::Timer t(s, "local_dt");
+          HIPCHK(launch_local_dt(b.z, s->P, s->w, s->w.dt, s->stream));
+        }
+        if (implicit_block(s, b)) return 1;
+        {
+          ostc_solver::Timer t(s, "update_bv");
+          HIPCHK(launch_update_bv(b.z, s->P, s->d_mech, s->stream));
+        }
+        if (run_bcs(s, b, with_cv)) return 1;
       }
-      if (implicit_block(s, b)) return 1;
-      {
-        ostc_solver::Timer t(s, "update_bv");
-        HIPCHK(launch_update_bv(b.z, s->P, s->d_mech, s->stream));
-      }
-      // ghost cv follows for the AWENO path, which reads it
-      const bool with_cv = s->P.inviscid_type == 1;
-      if (run_bcs(s, b, with_cv)) return 1;
-      if (s->P.dim == 2) HIPCHK(launch_eliminate_k(b.z, s->P, with_cv, s->stream));
-      if (properties_block(s, b, true)) return 1;
+      if (exchange_all(s, with_cv)) return 1;
+      if (s->P.dim == 2)
+        for (auto &b : s->blocks) HIPCHK(launch_eliminate_k(b.z, s->P, with_cv, s->stream));
+      for (auto &b : s->blocks)
+        if (properties_block(s, b, true)) return 1;
       if (step % os == 0 || step == 1) {
         ostc_solver::Timer t(s, "residual");
-        HIPCHK(launch_residual(b.z, s->w, it, step == 1 ? 1 : 0, s->stream));
+        if (residual_all(s, it, step == 1 ? 1 : 0, nullptr)) return 1;
       }
     }
     return 0;

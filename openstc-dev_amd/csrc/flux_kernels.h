@@ -30,8 +30,16 @@ struct StepScratch {
   double *res_hist;        // 5 per step (4 residuals + err_max)
   double *res_scale;       // 4
   double *res_one;         // 5: the residual of one ostc_residual call
+  double *blk_dt;          // one CFL minimum per block (multi-block cases)
   int max_blocks;
   int hist_cap;            // steps dt_hist / res_hist hold (grows, never shrinks)
+};
+
+// an inner face as the exchange kernel walks it (after InnerFace::register_boundary)
+struct InnerFaceDesc {
+  int rs[3], ts[3];            // start of this block's range and of the target's
+  int face, face_at;           // normal direction and the index of the face layer
+  int src_tar[3], loop[3], tloop[3], n[3];
 };
 
 // one direction (0 = i, 1 = j, 2 = k); init: the sweep writes dq instead of accumulating
@@ -57,7 +65,21 @@ hipError_t launch_store_last(const ZoneView &z, hipStream_t s);
 // given); first != 0 sets the scale
 hipError_t launch_residual(const ZoneView &z, const StepScratch &w, int slot, int first, hipStream_t s,
                            double *out = nullptr);
+// the same over several blocks (sums of all blocks, divided by all their points)
+hipError_t launch_residual_blocks(const std::vector<const ZoneView *> &zs, const StepScratch &w, int slot, int first,
+                                  hipStream_t s, double *out = nullptr);
 hipError_t launch_copy_dt(const double *src, double *dst, hipStream_t s);
+// *out = min(in[0..n)) (n >= 1)
+hipError_t launch_min_of(const double *in, int n, double *out, hipStream_t s);
+// inner-face exchange of block z with target tz (DataCommunication.cuh:58-110)
+hipError_t launch_inner_exchange(const ZoneView &z, const ZoneView &tz, const DParams &P, const DMech *m,
+                                 const InnerFaceDesc &f, bool with_cv, hipStream_t s);
+// J and metrics of a block from its (g + 1)-ghosted node coordinates, with the
+// reference's ghost fall-back; bad: one byte per metric point (scratch);
+// *interior_bad (device int, zeroed by the caller) set on a non-positive
+// interior Jacobian
+hipError_t launch_metrics(int mx, int my, int mz, int g, const double *X, const double *Y, const double *Z,
+                          double *jac, double *met, unsigned char *bad, int *interior_bad, hipStream_t s);
 hipError_t launch_chem_batch(const DMech *m, int nr, int64_t n, const double *rho, const double *T, const double *Y,
                              double dt, double *dY, double *dT, hipStream_t s);
 // ---- steady path (implicit.hip)

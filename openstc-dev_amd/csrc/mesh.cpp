@@ -3,6 +3,7 @@
 // k-ghosts at z0+k, Mesh.cpp:848) and the Jacobian / metrics by second-order
 // central differences (Mesh.cpp:259-521).  One-time work; the result is
 // uploaded as component-major arrays for coalesced device reads.
+#include <cstdlib>
 #include <cstring>
 
 #include "ostc_internal.h"
@@ -11,7 +12,11 @@ namespace ostc {
 
 namespace {
 
-void ghost_nodes(int mx, int my, int mz, int ngg, const Layout &L, double *A, bool is_z) {
+}  // namespace
+
+// Mesh.cpp:790-858: reflected ghost nodes on the six faces (ngg + 1 layers),
+// 2-D k-ghosts at z0 + k.
+void ghost_node_faces(int mx, int my, int mz, int ngg, const Layout &L, double *A, bool is_z) {
   auto a = [&](int i, int j, int k) -> double & { return A[L.at(i, j, k)]; };
   const int G = ngg + 1;
   for (int k = 0; k < mz; ++k)
@@ -37,6 +42,13 @@ void ghost_nodes(int mx, int my, int mz, int ngg, const Layout &L, double *A, bo
         for (int k = mz; k < mz + G; ++k) a(i, j, k) = a(i, j, mz - 1) + (is_z ? k - mz + 1 : 0);
       }
   }
+}
+
+// Mesh.cpp:861-937: parallelogram edges and corners from the face ghosts
+// (run after an inner face's ghosts were taken from the neighbour block).
+void ghost_node_edges(int mx, int my, int mz, int ngg, const Layout &L, double *A) {
+  auto a = [&](int i, int j, int k) -> double & { return A[L.at(i, j, k)]; };
+  const int G = ngg + 1;
   // edges and corners: x-y edges over the interior k range
   for (int i = 1; i <= G; ++i)
     for (int j = 1; j <= G; ++j)
@@ -67,6 +79,13 @@ void ghost_nodes(int mx, int my, int mz, int ngg, const Layout &L, double *A, bo
         a(i, j1, k2) = a(i, j1, mz - 1) + a(i, 0, k2) - a(i, 0, mz - 1);
         a(i, j2, k1) = a(i, j2, 0) + a(i, my - 1, k1) - a(i, my - 1, 0);
       }
+}
+
+namespace {
+
+void ghost_nodes(int mx, int my, int mz, int ngg, const Layout &L, double *A, bool is_z) {
+  ghost_node_faces(mx, my, mz, ngg, L, A, is_z);
+  ghost_node_edges(mx, my, mz, ngg, L, A);
 }
 
 struct Metric9 {
@@ -176,6 +195,129 @@ bool compute_metrics(int mx, int my, int mz, int ngg, const double *x, const dou
     for (int c = 0; c < 9; ++c) metric[(size_t)c * L.sz + p] = M[p].m[c];
   }
   return true;
+}
+
+// gxl_lib/Math.hpp:13-15 sgn of an integer (sgn(0) = +1)
+static int isgn(int a) { return a < 0 ? -1 : 1; }
+
+// Mesh.cpp:34-187 InnerFace constructor + register_boundary.
+bool register_inner_face(const int *s7, const int *t7, int ngg, int dim, InnerFace &f, std::string &err) {
+  f = InnerFace{};
+  for (int i = 0; i < 3; ++i) {
+    f.rs[i] = s7[2 * i];
+    f.re[i] = s7[2 * i + 1];
+    f.ts[i] = t7[2 * i];
+    f.te[i] = t7[2 * i + 1];
+  }
+  f.tblk = t7[6] - 1;
+  for (int i = 0; i < dim; ++i)
+    if (f.rs[i] == f.re[i]) {
+      f.face = i;
+      if (f.rs[i] == 1) f.dir = -1;
+      f.rs[i] -= 1;
+      f.re[i] -= 1;
+      break;
+    }
+  for (int i = 0; i < dim; ++i)
+    if (f.ts[i] == f.te[i]) {
+      f.tface = i;
+      if (f.ts[i] == 1) f.tdir = -1;
+      f.ts[i] -= 1;
+      f.te[i] -= 1;
+      break;
+    }
+  f.src_tar[f.tface] = -(f.dir * f.tdir) * (f.face + 1);
+  if (dim == 2) {
+    // Mesh.cpp:69-72 shifts the target's ranges by the source's tangential
+    // direction, which only describes faces with the same normal direction
+    if (f.face != f.tface) {
+      err = "2-D inner faces must join faces of the same index direction (i with i, j with j)";
+      return false;
+    }
+    const int nf = 1 - f.face, tnf = 1 - f.tface;
+    f.rs[nf] = std::abs(f.rs[nf]) - 1;
+    f.re[nf] = std::abs(f.re[nf]) - 1;
+    f.ts[nf] = std::abs(f.ts[nf]) - 1;
+    f.te[nf] = std::abs(f.te[nf]) - 1;
+    f.src_tar[tnf] = isgn((f.te[tnf] - f.ts[tnf]) * (f.re[nf] - f.rs[nf])) * (nf + 1);
+    f.src_tar[2] = 3;
+    f.rs[2] = f.re[2] = f.ts[2] = f.te[2] = 0;
+  } else {
+    // the negative pair marks the tangential direction whose correspondence
+    // the line states; the third direction follows from the other two
+    int nf = -1, tnf = -1;
+    for (int i = 0; i < dim; ++i)
+      if (f.rs[i] < 0) {
+        nf = i;
+        f.rs[i] = -f.rs[i] - 1;
+        f.re[i] = -f.re[i] - 1;
+        break;
+      }
+    for (int i = 0; i < dim; ++i)
+      if (f.ts[i] < 0) {
+        tnf = i;
+        f.ts[i] = -f.ts[i] - 1;
+        f.te[i] = -f.te[i] - 1;
+        break;
+      }
+    if (nf < 0 || tnf < 0 || nf == f.face || tnf == f.tface) {
+      err = "3-D inner face lines need one negative index pair (not the normal direction) on each side";
+      return false;
+    }
+    f.src_tar[tnf] = isgn((f.te[tnf] - f.ts[tnf]) * (f.re[nf] - f.rs[nf])) * (nf + 1);
+    nf = 3 - f.face - nf;
+    tnf = 3 - f.tface - tnf;
+    f.rs[nf] -= 1;
+    f.re[nf] -= 1;
+    f.ts[tnf] -= 1;
+    f.te[tnf] -= 1;
+    f.src_tar[tnf] = isgn((f.te[tnf] - f.ts[tnf]) * (f.re[nf] - f.rs[nf])) * (nf + 1);
+  }
+  // the face's own range becomes the face layer plus its ngg ghost layers
+  f.rs[f.face] = f.dir < 0 ? -ngg : f.re[f.face];
+  f.re[f.face] = f.dir < 0 ? 0 : f.re[f.face] + ngg;
+  const int sgn_tar = isgn(f.src_tar[f.tface]);
+  if (f.tdir > 0)
+    f.ts[f.tface] = sgn_tar > 0 ? f.te[f.tface] - ngg : f.te[f.tface];
+  else
+    f.ts[f.tface] = sgn_tar > 0 ? 0 : ngg;
+  for (int i = 0; i < 3; ++i) {
+    f.loop[i] = f.rs[i] <= f.re[i] ? 1 : -1;
+    f.tloop[i] = isgn(f.src_tar[i]);
+    f.src_tar[i] = std::abs(f.src_tar[i]) - 1;
+    f.n[i] = std::abs(f.rs[i] - f.re[i]) + 1;
+  }
+  return true;
+}
+
+// Mesh.cpp:946-994 init_inner_ghost_grid for one face of block B with
+// neighbour N (both on their (ngg + 1)-ghosted node layouts).
+void inner_ghost_nodes(const InnerFace &f, const Layout &LB, double *const *B, const Layout &LN,
+                       const double *const *N) {
+  int mn[3] = {f.rs[0], f.rs[1], f.rs[2]}, mxx[3] = {f.re[0], f.re[1], f.re[2]}, mt[3] = {f.ts[0], f.ts[1], f.ts[2]};
+  if (f.dir < 0) {
+    mn[f.face] -= 1;
+    mxx[f.face] -= 1;
+  } else {
+    mxx[f.face] += 1;
+    mn[f.face] += 1;
+  }
+  mt[f.tface] += f.tdir > 0 ? -1 : 1;
+  int d[3];
+  for (int i1 = mn[0]; f.loop[0] * (i1 - mxx[0]) != 1; i1 += f.loop[0]) {
+    d[0] = i1 - mn[0];
+    for (int j1 = mn[1]; f.loop[1] * (j1 - mxx[1]) != 1; j1 += f.loop[1]) {
+      d[1] = j1 - mn[1];
+      for (int k1 = mn[2]; f.loop[2] * (k1 - mxx[2]) != 1; k1 += f.loop[2]) {
+        d[2] = k1 - mn[2];
+        const int i2 = mt[0] + f.tloop[0] * d[f.src_tar[0]];
+        const int j2 = mt[1] + f.tloop[1] * d[f.src_tar[1]];
+        const int k2 = mt[2] + f.tloop[2] * d[f.src_tar[2]];
+        const int64_t p = LB.at(i1, j1, k1), q = LN.at(i2, j2, k2);
+        for (int c = 0; c < 3; ++c) B[c][p] = N[c][q];
+      }
+    }
+  }
 }
 
 }  // namespace ostc

@@ -94,6 +94,34 @@ bool compute_metrics(int mx, int my, int mz, int ngg, const double *x, const dou
                      std::vector<double> &jac, std::vector<double> &metric, std::string &err,
                      std::vector<double> *xyz = nullptr);
 
+// Ghost node steps of Mesh.cpp:790-944 on an (ngg + 1)-ghosted node layout:
+// reflected faces first, then (after the inner faces' ghost nodes were taken
+// from the neighbours) the edges and corners.
+void ghost_node_faces(int mx, int my, int mz, int ngg, const Layout &L, double *A, bool is_z);
+void ghost_node_edges(int mx, int my, int mz, int ngg, const Layout &L, double *A);
+
+// An inner face after Mesh.cpp:34-187 InnerFace::register_boundary, 0-based:
+// this block's range (the face layer and its ngg ghost layers, full
+// tangential extent), the target block's start indices, which of this
+// block's directions each target direction follows (src_tar) and the index
+// steps on both sides.
+struct InnerFace {
+  int rs[3] = {0, 0, 0}, re[3] = {0, 0, 0};
+  int face = 0, dir = 1;
+  int ts[3] = {0, 0, 0}, te[3] = {0, 0, 0};
+  int tblk = 0, tface = 0, tdir = 1;
+  int src_tar[3] = {0, 0, 0};
+  int loop[3] = {1, 1, 1}, tloop[3] = {1, 1, 1};
+  int n[3] = {0, 0, 0};
+};
+// s7 / t7: the two lines of an inner-face pair as in the reference's file
+// (1-based; t7[6] the target block, 1-based).
+bool register_inner_face(const int *s7, const int *t7, int ngg, int dim, InnerFace &f, std::string &err);
+// Mesh.cpp:946-994: ghost nodes beyond face f of block B from neighbour N
+// (B[3], N[3]: x, y, z arrays on the layouts LB, LN).
+void inner_ghost_nodes(const InnerFace &f, const Layout &LB, double *const *B, const Layout &LN,
+                       const double *const *N);
+
 // ------------------------------------------------------------ boundaries
 enum BcType { BC_WALL = 2, BC_SYMMETRY = 3, BC_FARFIELD = 4, BC_INFLOW = 5, BC_OUTFLOW = 6, BC_SUBSONIC = 7,
               BC_BACKPRESSURE = 9 };

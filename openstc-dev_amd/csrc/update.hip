@@ -259,6 +259,50 @@ __global__ void __launch_bounds__(kTB) k_velocity(ZoneView z) {
   z.vel[p] = sqrt(u * u + v * v + w * w);
 }
 
+// ------------------------------------------------------- inner faces
+// DataCommunication.cuh:58-110 inner_communication: one thread per point of
+// an inner face's range in this block; the face layer is averaged with the
+// target block's (both blocks written, |V| of both set), the ghost layers
+// take the target's interior values.  Threads touch disjoint points: the
+// ghost layers read target points off its face layer.
+template <int NS, bool WITH_CV>
+__global__ void __launch_bounds__(kTB) k_inner_exchange(ZoneView z, ZoneView tz, const DMech *__restrict__ m,
+                                                        InnerFaceDesc f, int nsc, int nt) {
+  const int64_t t = (int64_t)blockIdx.x * kTB + threadIdx.x;
+  const int64_t n01 = (int64_t)f.n[0] * f.n[1];
+  if (t >= n01 * f.n[2]) return;
+  const int nn[3] = {(int)(t % f.n[0]), (int)((t / f.n[0]) % f.n[1]), (int)(t / n01)};
+  int idx[3], it[3], d[3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    d[i] = f.loop[i] * nn[i];
+    idx[i] = f.rs[i] + d[i];
+  }
+#pragma unroll
+  for (int i = 0; i < 3; ++i) it[i] = f.ts[i] + f.tloop[i] * d[f.src_tar[i]];
+  const int64_t p = z.at(idx[0], idx[1], idx[2]), q = tz.at(it[0], it[1], it[2]);
+  if (idx[f.face] == f.face_at) {
+    for (int l = 0; l < 6; ++l) {
+      const double ave = 0.5 * (tz.bv[l * tz.sz + q] + z.bv[l * z.sz + p]);
+      z.bv[l * z.sz + p] = ave;
+      tz.bv[l * tz.sz + q] = ave;
+    }
+    for (int l = 0; l < nsc; ++l) {
+      const double ave = 0.5 * (tz.sv[l * tz.sz + q] + z.sv[l * z.sz + p]);
+      z.sv[l * z.sz + p] = ave;
+      tz.sv[l * tz.sz + q] = ave;
+    }
+    const double u = z.bv[z.sz + p], v = z.bv[2 * z.sz + p], w = z.bv[3 * z.sz + p];
+    const double vel = sqrt(u * u + v * v + w * w);
+    z.vel[p] = vel;
+    tz.vel[q] = vel;
+  } else {
+    for (int l = 0; l < 6; ++l) z.bv[l * z.sz + p] = tz.bv[l * tz.sz + q];
+    for (int l = 0; l < nsc; ++l) z.sv[l * z.sz + p] = tz.sv[l * tz.sz + q];
+  }
+  if (WITH_CV) cv_point<NS>(z, m, p, nt);
+}
+
 // ------------------------------------------------------------- BCs
 // BoundCond.cuh:52-552: the face point of a registered range (where the
 // reference writes it) and its ngg ghosts.  A ghost layer depends only on
@@ -715,7 +759,7 @@ __global__ void __launch_bounds__(kTB) k_store_last(ZoneView z) {
 // Residual.cuh:57-161 + TimeAdvanceFunc.cu:19-33: squared change of
 // (rho, |V|, p, T) summed per block (grid-stride), then one block finishes
 // the RMS, the step-1 scale (floor 1e-20) and err_max on the device.
-__global__ void __launch_bounds__(kTB) k_residual(ZoneView z, double *partial) {
+__global__ void __launch_bounds__(kTB) k_residual(ZoneView z, double *partial, int ld, int off) {
   const int64_t n = (int64_t)z.mx * z.my * z.mz;
   double s[4] = {0, 0, 0, 0};
   for (int64_t t = (int64_t)blockIdx.x * kTB + threadIdx.x; t < n; t += (int64_t)gridDim.x * kTB) {
@@ -743,7 +787,7 @@ __global__ void __launch_bounds__(kTB) k_residual(ZoneView z, double *partial) {
       double b = 0;
 #pragma unroll
       for (int w = 0; w < kTB / 64; ++w) b += sm[l][w];
-      partial[(size_t)l * gridDim.x + blockIdx.x] = b;
+      partial[(size_t)l * ld + off + blockIdx.x] = b;
     }
   }
 }
@@ -1067,9 +1111,58 @@ hipError_t launch_residual(const ZoneView &z, const StepScratch &w, int slot, in
   const int64_t n = (int64_t)z.mx * z.my * z.mz;
   const int nb = (int)std::min<int64_t>(kRedBlocks, nblk(n, kTB));
   if (nb > w.max_blocks) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(k_residual, dim3(nb), dim3(kTB), 0, s, z, w.partial);
+  hipLaunchKernelGGL(k_residual, dim3(nb), dim3(kTB), 0, s, z, w.partial, nb, 0);
   hipLaunchKernelGGL(k_residual_final, dim3(1), dim3(kTB), 0, s, (const double *)w.partial, nb, n, w.res_scale,
                      out ? out : w.res_hist + (size_t)slot * 5, first);
+  return hipGetLastError();
+}
+
+hipError_t launch_residual_blocks(const std::vector<const ZoneView *> &zs, const StepScratch &w, int slot, int first,
+                                  hipStream_t s, double *out) {
+  if (zs.size() == 1) return launch_residual(*zs[0], w, slot, first, s, out);
+  int ld = 0;
+  int64_t n = 0;
+  for (const ZoneView *z : zs) {
+    const int64_t c = (int64_t)z->mx * z->my * z->mz;
+    ld += (int)std::min<int64_t>(kRedBlocks, nblk(c, kTB));
+    n += c;
+  }
+  if (ld > w.max_blocks) return hipErrorInvalidValue;
+  int off = 0;
+  for (const ZoneView *z : zs) {
+    const int nb = (int)std::min<int64_t>(kRedBlocks, nblk((int64_t)z->mx * z->my * z->mz, kTB));
+    hipLaunchKernelGGL(k_residual, dim3(nb), dim3(kTB), 0, s, *z, w.partial, ld, off);
+    off += nb;
+  }
+  hipLaunchKernelGGL(k_residual_final, dim3(1), dim3(kTB), 0, s, (const double *)w.partial, ld, n, w.res_scale,
+                     out ? out : w.res_hist + (size_t)slot * 5, first);
+  return hipGetLastError();
+}
+
+hipError_t launch_inner_exchange(const ZoneView &z, const ZoneView &tz, const DParams &P, const DMech *m,
+                                 const InnerFaceDesc &f, bool with_cv, hipStream_t s) {
+  const int64_t n = (int64_t)f.n[0] * f.n[1] * f.n[2];
+  if (n <= 0) return hipSuccess;
+  if (n > ((int64_t)1 << 31)) return hipErrorInvalidValue;
+  const dim3 g(nblk(n, kTB)), b(kTB);
+  if (P.ns == 0) {
+    if (with_cv)
+      hipLaunchKernelGGL((k_inner_exchange<0, true>), g, b, 0, s, z, tz, m, f, P.nsc, P.nt);
+    else
+      hipLaunchKernelGGL((k_inner_exchange<0, false>), g, b, 0, s, z, tz, m, f, P.nsc, P.nt);
+  } else if (P.ns == 9) {
+    if (with_cv)
+      hipLaunchKernelGGL((k_inner_exchange<9, true>), g, b, 0, s, z, tz, m, f, P.nsc, P.nt);
+    else
+      hipLaunchKernelGGL((k_inner_exchange<9, false>), g, b, 0, s, z, tz, m, f, P.nsc, P.nt);
+  } else {
+    return hipErrorNotSupported;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_min_of(const double *in, int n, double *out, hipStream_t s) {
+  hipLaunchKernelGGL(k_min_final, dim3(1), dim3(kTB), 0, s, in, n, out);
   return hipGetLastError();
 }
 

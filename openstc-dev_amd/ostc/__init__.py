@@ -80,6 +80,14 @@ def lib():
             "ostc_chem_batch": ([vp, C.c_int64, vp, vp, vp, C.c_double, vp, vp, C.c_int,
                                  C.POINTER(C.c_int64)], C.c_int),
             "ostc_set_chem_split": ([vp, C.c_int], C.c_int),
+            "ostc_read_grid": ([vp], C.c_int),
+            "ostc_add_inner_face": ([vp, C.c_int, ip, ip], C.c_int),
+            "ostc_read_inner_file": ([vp], C.c_int),
+            "ostc_read_mesh": ([vp], C.c_int),
+            "ostc_inner_faces": ([vp, C.c_int, ip, ip, C.c_int], C.c_int),
+            "ostc_exchange": ([vp, C.c_int], C.c_int),
+            "ostc_block_count": ([vp, ip], C.c_int),
+            "ostc_block_dims": ([vp, C.c_int, ip], C.c_int),
             "ostc_chem_step": ([vp, C.c_int, C.c_double, C.c_int, C.POINTER(C.c_int64)], C.c_int),
             "ostc_dev_alloc": ([vp, C.c_int64, C.POINTER(vp)], C.c_int),
             "ostc_dev_free": ([vp, vp], C.c_int),
@@ -213,11 +221,47 @@ class Solver:
     def read_boundary_file(self):
         _chk(lib().ostc_read_boundary_file(self.h))
 
+    def read_grid(self):
+        _chk(lib().ostc_read_grid(self.h))
+
+    def read_inner_file(self):
+        _chk(lib().ostc_read_inner_file(self.h))
+
+    def read_mesh(self):
+        _chk(lib().ostc_read_mesh(self.h))
+
+    def add_inner_face(self, blk, src7, tar7):
+        a = np.ascontiguousarray(src7, dtype=np.int32)
+        b = np.ascontiguousarray(tar7, dtype=np.int32)
+        _chk(lib().ostc_add_inner_face(self.h, blk, a.ctypes.data_as(C.POINTER(C.c_int)),
+                                       b.ctypes.data_as(C.POINTER(C.c_int))))
+
+    def inner_faces(self, blk):
+        """Registered inner faces of a block: (n, 31) int array."""
+        n = C.c_int()
+        _chk(lib().ostc_inner_faces(self.h, blk, C.byref(n), None, 0))
+        out = np.zeros((max(n.value, 1), 31), dtype=np.int32)
+        _chk(lib().ostc_inner_faces(self.h, blk, C.byref(n), out.ctypes.data_as(C.POINTER(C.c_int)), n.value))
+        return out[:n.value]
+
+    def exchange(self, with_cv=False):
+        _chk(lib().ostc_exchange(self.h, 1 if with_cv else 0))
+
     def finalize(self):
         _chk(lib().ostc_finalize(self.h))
 
+    def block_count(self):
+        n = C.c_int()
+        _chk(lib().ostc_block_count(self.h, C.byref(n)))
+        return n.value
+
+    def block_dims(self, blk=0):
+        d = (C.c_int * 3)()
+        _chk(lib().ostc_block_dims(self.h, blk, d))
+        return tuple(d)
+
     def geometry(self, blk=0):
-        mx, my, mz = self.blocks[blk]
+        mx, my, mz = self.block_dims(blk)
         g = self.get_int("ngg")
         n = (mx + 2 * g) * (my + 2 * g) * (mz + 2 * g)
         jac, met = np.zeros(n), np.zeros(9 * n)
@@ -423,6 +467,25 @@ class Solver:
             name, calls, ms = line.split()
             out[name] = (int(calls), float(ms))
         return out
+
+
+def solver_for_mesh_case(mcase, root, device=0, fields=None):
+    """Build a solver for a multi-block case (see ostc.cases.split_case)
+    through the reference's grid, boundary and inner-face files; fields:
+    per-block (bv, sv) to upload before the initialisation (None: no
+    initialisation)."""
+    from . import cases as _cases
+    _cases.write_mesh_case_dir(mcase, root)
+    s = Solver(root, device)
+    s.read_mesh()
+    s.finalize()
+    if fields is not None:
+        for b, (bv, sv) in enumerate(fields):
+            s.upload("bv", bv, blk=b)
+            if sv.shape[0]:
+                s.upload("sv", sv, blk=b)
+        s.initialize()
+    return s
 
 
 def solver_for_case(case, root, device=0, initial=True):

@@ -642,3 +642,137 @@ def write_case_dir(case, root):
             r = b["raw"]
             f.write("".join(f"{v:>6d}" for v in r) + f"{b['label']:>6d}\n")
     return root
+
+
+# ------------------------------------------------------------ multi-block
+def _rot(a, rotate):
+    """Turn an array (..., k, j, i) by 180 degrees about k (i and j reversed)."""
+    return a[..., ::-1, ::-1].copy() if rotate else a
+
+
+def split_case(case, axis=0, at=None, rotate=False):
+    """The single-block `case` cut into two blocks joined by one inner face:
+    block 0 holds the index range [0, at] along `axis` (0 = i, 1 = j), block
+    1 the range [at, m - 1] (the face layer belongs to both).  rotate: block
+    1 is stored turned by 180 degrees about k (its i and j run backwards), so
+    the inner face pairs a high face with a high face and reverses the
+    tangential direction.  Boundary ranges are split with the blocks.
+    Returns a multi-block case: blocks (mx, my, mz, x, y, z, bcs, lo, hi),
+    inner (block, source line, target line) in the reference's 1-based
+    inner-face format, and the single-block case under "full"."""
+    if axis not in (0, 1):
+        raise ValueError("split along i or j")
+    m = (case["mx"], case["my"], case["mz"])
+    at = m[axis] // 2 if at is None else at
+    dim = 2 if case["mz"] == 1 else 3
+    blocks = []
+    for b, (lo, hi) in enumerate(((0, at), (at, m[axis] - 1))):
+        rot = rotate and b == 1
+        sl = [slice(None)] * 3
+        sl[2 - axis] = slice(lo, hi + 1)
+        x = _rot(case["x"][tuple(sl)], rot)
+        y = _rot(case["y"][tuple(sl)], rot)
+        z = _rot(case["z"][tuple(sl)], rot)
+        ext = list(m)
+        ext[axis] = hi - lo + 1
+        bcs = []
+        for bc in case["bcs"]:
+            r = list(bc["raw"])
+            a1, a2 = sorted((r[2 * axis], r[2 * axis + 1]))
+            a1, a2 = max(a1, lo), min(a2, hi)
+            if a1 > a2:
+                continue
+            # a face lying on the cut itself is not a boundary
+            if r[2 * axis] == r[2 * axis + 1] and lo < r[2 * axis] < hi:
+                continue
+            r[2 * axis], r[2 * axis + 1] = a1 - lo, a2 - lo
+            if rot:
+                for d in (0, 1):
+                    v1, v2 = ext[d] - 1 - r[2 * d], ext[d] - 1 - r[2 * d + 1]
+                    r[2 * d], r[2 * d + 1] = min(v1, v2), max(v1, v2)
+            bcs.append(dict(bc, raw=tuple(r)))
+        blocks.append(dict(mx=ext[0], my=ext[1], mz=ext[2], x=x, y=y, z=z, bcs=bcs, lo=lo, hi=hi, rotated=rot))
+    n1 = blocks[1]["mx"] if axis == 0 else blocks[1]["my"]
+    tang = 1 - axis
+
+    def line(b, face_idx, reverse, blk_id):
+        v = [0] * 7
+        v[2 * axis] = v[2 * axis + 1] = face_idx
+        t1, t2 = (m[tang], 1) if reverse else (1, m[tang])
+        if dim == 3:
+            t1, t2 = -t1, -t2
+        v[2 * tang], v[2 * tang + 1] = t1, t2
+        v[4], v[5] = 1, m[2]
+        v[6] = blk_id
+        return v
+
+    f0 = line(0, at + 1, False, 1)
+    f1 = line(1, n1 if rotate else 1, rotate, 2)
+    inner = [(0, f0, f1), (1, f1, f0)]
+    mc = dict(case, name=case["name"] + "_split", blocks=blocks, inner=inner, full=case, split_axis=axis)
+    for key in ("mx", "my", "mz", "x", "y", "z", "bcs"):
+        mc.pop(key, None)
+    return mc
+
+
+def split_fields(mcase, bv, sv):
+    """Per-block ghosted fields cut from the single-block ghosted fields."""
+    g = mcase["ngg"]
+    axis = mcase["split_axis"]
+    out = []
+    for blk in mcase["blocks"]:
+        sl = [slice(None)] * 4
+        sl[3 - axis] = slice(blk["lo"], blk["hi"] + 1 + 2 * g)
+        fb = _rot(bv[tuple(sl)], blk["rotated"])
+        fs = _rot(sv[tuple(sl)], blk["rotated"]) if sv.shape[0] else sv[tuple(sl)]
+        out.append((np.ascontiguousarray(fb), np.ascontiguousarray(fs)))
+    return out
+
+
+def block_interior_of_full(mcase, b, arr, ncomp):
+    """Block b's interior cut from a single-block ghosted array (ncomp, ...),
+    in block b's index order."""
+    full = mcase["full"]
+    g = mcase["ngg"]
+    a = np.asarray(arr).reshape((ncomp,) + ghost_shape(full["mx"], full["my"], full["mz"], g))
+    a = a[:, g:g + full["mz"], g:g + full["my"], g:g + full["mx"]]
+    blk = mcase["blocks"][b]
+    sl = [slice(None)] * 4
+    sl[3 - mcase["split_axis"]] = slice(blk["lo"], blk["hi"] + 1)
+    return _rot(a[tuple(sl)], blk["rotated"])
+
+
+def write_mesh_case_dir(mcase, root):
+    """Reference-format case directory of a multi-block case: setup files,
+    grid file, boundary file and inner-face file."""
+    first = dict(mcase["full"], bcs=mcase["blocks"][0]["bcs"])
+    write_case_dir(first, root)
+    bc_dir = os.path.join(root, "input_files", "boundary_condition")
+    with open(os.path.join(bc_dir, "boundary   0.txt"), "w") as f:
+        for blk in mcase["blocks"]:
+            f.write(f"{len(blk['bcs']):>12d}\n")
+            for b in blk["bcs"]:
+                f.write("".join(f"{v:>6d}" for v in b["raw"]) + f"{b['label']:>6d}\n")
+    nb = len(mcase["blocks"])
+    counts = [sum(1 for (fb, _, _) in mcase["inner"] if fb == b) for b in range(nb)]
+    with open(os.path.join(bc_dir, "inner   0.txt"), "w") as f:
+        f.write(f"{nb:>12d}\n")
+        f.write("".join(f"{c:>12d}" for c in counts) + "\n")
+        f.write(" imin  imax  jmin  jmax  kmin  kmax   id\n")
+        for b in range(nb):
+            for (fb, src, tar) in mcase["inner"]:
+                if fb == b:
+                    f.write("".join(f"{v:>9d}" for v in src) + "\n")
+                    f.write("".join(f"{v:>9d}" for v in tar) + "\n")
+    grid_dir = os.path.join(root, "input_files", "grid")
+    os.makedirs(grid_dir, exist_ok=True)
+    with open(os.path.join(grid_dir, "grid   0.grd"), "w") as f:
+        f.write(f"{nb:>12d}\n")
+        for blk in mcase["blocks"]:
+            f.write(f"{blk['mx']:>12d}{blk['my']:>12d}{blk['mz']:>12d}\n")
+        for blk in mcase["blocks"]:
+            for key in ("x", "y", "z"):
+                vals = np.asarray(blk[key], dtype=np.float64).ravel()
+                for q in range(0, vals.size, 4):
+                    f.write(" ".join(repr(float(v)) for v in vals[q:q + 4]) + "\n")
+    return root

@@ -56,9 +56,9 @@ struct orc_zone {
 
 /* ------------------------------------------------------------------ grid */
 
-/* Mesh.cpp:790-944 init_ghost_grid (single block, no inner/parallel faces):
- * reflected ghost nodes, 2-D k-ghosts at z0+k, parallelogram edges/corners. */
-static void ghost_grid(int mx, int my, int mz, int ngg, int dim, lay L, double *X, double *Y, double *Z) {
+/* Mesh.cpp:790-858 init_ghost_grid, first part: reflected ghost nodes on
+ * the six faces (ngg + 1 layers), 2-D k-ghosts at z0+k. */
+static void ghost_faces(int mx, int my, int mz, int ngg, int dim, lay L, double *X, double *Y, double *Z) {
   double *C[3] = {X, Y, Z};
   for (int c = 0; c < 3; ++c) {
     double *A = C[c];
@@ -86,6 +86,18 @@ static void ghost_grid(int mx, int my, int mz, int ngg, int dim, lay L, double *
           for (int k = mz; k < mz + ngg + 1; ++k) A_(i, j, k) = A_(i, j, mz - 1) + (c == 2 ? k - mz + 1 : 0);
         }
     }
+#undef A_
+  }
+}
+
+/* Mesh.cpp:861-937 init_ghost_grid, last part: parallelogram edges and
+ * corners from the face ghosts (after the inner faces' ghosts were taken
+ * from the neighbour block). */
+static void ghost_edges(int mx, int my, int mz, int ngg, lay L, double *X, double *Y, double *Z) {
+  double *C[3] = {X, Y, Z};
+  for (int c = 0; c < 3; ++c) {
+    double *A = C[c];
+#define A_(i, j, k) A[at(L, i, j, k)]
     for (int i = 1; i <= ngg + 1; ++i)
       for (int j = 1; j <= ngg + 1; ++j)
         for (int k = 0; k < mz; ++k) {
@@ -115,6 +127,12 @@ static void ghost_grid(int mx, int my, int mz, int ngg, int dim, lay L, double *
         }
 #undef A_
   }
+}
+
+/* Mesh.cpp:790-944 init_ghost_grid of a single block (no inner faces) */
+static void ghost_grid(int mx, int my, int mz, int ngg, int dim, lay L, double *X, double *Y, double *Z) {
+  ghost_faces(mx, my, mz, ngg, dim, L, X, Y, Z);
+  ghost_edges(mx, my, mz, ngg, L, X, Y, Z);
 }
 
 /* one point of Mesh.cpp:259-521: central differences of x,y,z; J and the
@@ -253,6 +271,10 @@ static void wall_distance(orc_zone *zz) {
   free(wp);
 }
 
+static void zone_register(orc_zone *zz);
+
+/* x == NULL leaves the grid, metrics and boundaries to the caller (the
+ * multi-block mesh sets them after the inner faces are known) */
 orc_zone *orc_zone_create(int mx, int my, int mz, int ngg, const orc_cfg *cfg, const orc_mech *mech, const double *x,
                           const double *y, const double *z) {
   orc_zone *zz = (orc_zone *)calloc(1, sizeof(orc_zone));
@@ -298,7 +320,17 @@ orc_zone *orc_zone_create(int mx, int my, int mz, int ngg, const orc_cfg *cfg, c
   zz->tcond = cal(sg);
   zz->wdist = cal(sg);
   zz->turb_jac = cal(s0 * 2);
-  orc_grid_metrics(mx, my, mz, ngg, x, y, z, zz->x, zz->y, zz->z, zz->jac, zz->metric);
+  if (x) {
+    orc_grid_metrics(mx, my, mz, ngg, x, y, z, zz->x, zz->y, zz->z, zz->jac, zz->metric);
+    zone_register(zz);
+  }
+  return zz;
+}
+
+/* boundaries and the SST wall distance of a zone whose grid is set */
+static void zone_register(orc_zone *zz) {
+  const orc_cfg *cfg = &zz->cfg;
+  const int ngg = zz->g;
   /* inviscid_type: Parameter.cpp:165-219 deduce_known_info */
   zz->inviscid_type = 0;
   if (cfg->inviscid_scheme == 2) zz->inviscid_type = 2;
@@ -351,7 +383,6 @@ orc_zone *orc_zone_create(int mx, int my, int mz, int ngg, const orc_cfg *cfg, c
       if (cfg->bc[b].type == prio[p]) zz->order[n++] = b;
   zz->nbc = n;
   if (zz->nt) wall_distance(zz);
-  return zz;
 }
 
 void orc_zone_destroy(orc_zone *zz) {
@@ -2197,9 +2228,7 @@ static void store_last_step(orc_zone *zz) {
 
 /* Residual.cuh:57-161 + TimeAdvanceFunc.cu:19-33: RMS of the change of
  * (rho, |V|, p, T), scaled by the first step's value (floor 1e-20). */
-static double residual(orc_zone *zz, int step) {
-  double s[4] = {0, 0, 0, 0};
-  const int64_t n = (int64_t)zz->mx * zz->my * zz->mz;
+static void residual_sums(orc_zone *zz, double *s) {
   for (int k = 0; k < zz->mz; ++k)
     for (int j = 0; j < zz->my; ++j)
       for (int i = 0; i < zz->mx; ++i) {
@@ -2211,17 +2240,27 @@ static double residual(orc_zone *zz, int step) {
           s[l] += d * d;
         }
       }
-  for (int l = 0; l < 4; ++l) zz->res[l] = sqrt(s[l] / (double)n);
+}
+
+/* Residual.cuh:113-139: RMS over all n points, scaled by the first step's */
+static double residual_finish(const double *s, int64_t n, int step, double *res, double *res_scale) {
+  for (int l = 0; l < 4; ++l) res[l] = sqrt(s[l] / (double)n);
   if (step == 1)
     for (int l = 0; l < 4; ++l) {
-      zz->res_scale[l] = zz->res[l];
-      if (zz->res_scale[l] < 1e-20) zz->res_scale[l] = 1e-20;
+      res_scale[l] = res[l];
+      if (res_scale[l] < 1e-20) res_scale[l] = 1e-20;
     }
-  for (int l = 0; l < 4; ++l) zz->res[l] /= zz->res_scale[l];
-  double err = zz->res[0];
+  for (int l = 0; l < 4; ++l) res[l] /= res_scale[l];
+  double err = res[0];
   for (int l = 1; l < 4; ++l)
-    if (fabs(zz->res[l]) > err) err = zz->res[l];
+    if (fabs(res[l]) > err) err = res[l];
   return err;
+}
+
+static double residual(orc_zone *zz, int step) {
+  double s[4] = {0, 0, 0, 0};
+  residual_sums(zz, s);
+  return residual_finish(s, (int64_t)zz->mx * zz->my * zz->mz, step, zz->res, zz->res_scale);
 }
 
 /* RK.cuh:89-193 RK3_bv loop (single block, no IO/monitors) */
@@ -2768,3 +2807,386 @@ void orc_ssprk3_coeffs(double *a, double *b, double *c) {
     c[r] = RK_C[r];
   }
 }
+
+/* ======================================================= multi-block mesh */
+
+/* gxl_lib/Math.hpp:13-15 sgn of an integer (sgn(0) = +1) */
+static int isgn(int a) { return a < 0 ? -1 : 1; }
+
+/* Mesh.cpp:34-187 InnerFace constructor + register_boundary: s7 = the
+ * source line (i1 i2 j1 j2 k1 k2 ...), t7 = the target line (... block id,
+ * 1-based), 1-based in the file.  In 3-D the negative pair marks the
+ * non-normal direction whose correspondence the line states.  Returns 0, or
+ * -1 for a 2-D pair of faces with different normal directions, whose target
+ * ranges the reference shifts by the source's direction (Mesh.cpp:69-72) and
+ * so does not describe. */
+int orc_inner_face_register(const int *s7, const int *t7, int ngg, int dim, orc_inner_face *f) {
+  memset(f, 0, sizeof(*f));
+  for (int i = 0; i < 3; ++i) {
+    f->rs[i] = s7[2 * i];
+    f->re[i] = s7[2 * i + 1];
+    f->ts[i] = t7[2 * i];
+    f->te[i] = t7[2 * i + 1];
+    f->loop[i] = f->tloop[i] = 1;
+  }
+  f->tblk = t7[6] - 1;
+  f->dir = f->tdir = 1;
+  for (int i = 0; i < dim; ++i)
+    if (f->rs[i] == f->re[i]) {
+      f->face = i;
+      if (f->rs[i] == 1) f->dir = -1;
+      f->rs[i] -= 1;
+      f->re[i] -= 1;
+      break;
+    }
+  for (int i = 0; i < dim; ++i)
+    if (f->ts[i] == f->te[i]) {
+      f->tface = i;
+      if (f->ts[i] == 1) f->tdir = -1;
+      f->ts[i] -= 1;
+      f->te[i] -= 1;
+      break;
+    }
+  f->src_tar[f->tface] = -(f->dir * f->tdir) * (f->face + 1);
+  if (dim == 2) {
+    if (f->face != f->tface) return -1;
+    const int nf = 1 - f->face, tnf = 1 - f->tface;
+    f->rs[nf] = abs(f->rs[nf]) - 1;
+    f->re[nf] = abs(f->re[nf]) - 1;
+    f->ts[nf] = abs(f->ts[nf]) - 1;
+    f->te[nf] = abs(f->te[nf]) - 1;
+    f->src_tar[tnf] = isgn((f->te[tnf] - f->ts[tnf]) * (f->re[nf] - f->rs[nf])) * (nf + 1);
+    f->src_tar[2] = 3;
+    f->rs[2] = f->re[2] = f->ts[2] = f->te[2] = 0;
+  } else {
+    int nf = 0, tnf = 0;
+    for (int i = 0; i < dim; ++i)
+      if (f->rs[i] < 0) {
+        nf = i;
+        f->rs[i] = -f->rs[i] - 1;
+        f->re[i] = -f->re[i] - 1;
+        break;
+      }
+    for (int i = 0; i < dim; ++i)
+      if (f->ts[i] < 0) {
+        tnf = i;
+        f->ts[i] = -f->ts[i] - 1;
+        f->te[i] = -f->te[i] - 1;
+        break;
+      }
+    f->src_tar[tnf] = isgn((f->te[tnf] - f->ts[tnf]) * (f->re[nf] - f->rs[nf])) * (nf + 1);
+    nf = 3 - f->face - nf;
+    tnf = 3 - f->tface - tnf;
+    f->rs[nf] -= 1;
+    f->re[nf] -= 1;
+    f->ts[tnf] -= 1;
+    f->te[tnf] -= 1;
+    f->src_tar[tnf] = isgn((f->te[tnf] - f->ts[tnf]) * (f->re[nf] - f->rs[nf])) * (nf + 1);
+  }
+  /* the face's own range becomes the face layer plus its ngg ghost layers */
+  f->rs[f->face] = f->dir < 0 ? -ngg : f->re[f->face];
+  f->re[f->face] = f->dir < 0 ? 0 : f->re[f->face] + ngg;
+  const int sgn_tar = isgn(f->src_tar[f->tface]);
+  if (f->tdir > 0)
+    f->ts[f->tface] = sgn_tar > 0 ? f->te[f->tface] - ngg : f->te[f->tface];
+  else
+    f->ts[f->tface] = sgn_tar > 0 ? 0 : ngg;
+  for (int i = 0; i < 3; ++i) {
+    f->loop[i] = f->rs[i] <= f->re[i] ? 1 : -1;
+    f->tloop[i] = isgn(f->src_tar[i]);
+    f->src_tar[i] = abs(f->src_tar[i]) - 1;
+  }
+  for (int i = 0; i < 3; ++i) f->n[i] = abs(f->rs[i] - f->re[i]) + 1;
+  return 0;
+}
+
+struct orc_mesh {
+  int nb, dim, step;
+  orc_zone **z;
+  int *nface, *face0;        /* faces of block b: face0[b] .. face0[b] + nface[b] - 1 */
+  orc_inner_face *faces;
+  double res[4], res_scale[4];
+  double t_phys;
+};
+
+/* Mesh.cpp:946-994 init_inner_ghost_grid: the ngg + 1 ghost node layers
+ * beyond an inner face are the neighbour block's nodes */
+static void inner_ghost_grid(orc_mesh *M) {
+  for (int b = 0; b < M->nb; ++b) {
+    orc_zone *B = M->z[b];
+    for (int q = 0; q < M->nface[b]; ++q) {
+      const orc_inner_face *f = &M->faces[M->face0[b] + q];
+      const orc_zone *N = M->z[f->tblk];
+      int mn[3] = {f->rs[0], f->rs[1], f->rs[2]}, mxx[3] = {f->re[0], f->re[1], f->re[2]},
+          mt[3] = {f->ts[0], f->ts[1], f->ts[2]};
+      if (f->dir < 0) {
+        mn[f->face] -= 1;
+        mxx[f->face] -= 1;
+      } else {
+        mxx[f->face] += 1;
+        mn[f->face] += 1;
+      }
+      mt[f->tface] += f->tdir > 0 ? -1 : 1;
+      int d[3];
+      for (int i1 = mn[0]; f->loop[0] * (i1 - mxx[0]) != 1; i1 += f->loop[0]) {
+        d[0] = i1 - mn[0];
+        for (int j1 = mn[1]; f->loop[1] * (j1 - mxx[1]) != 1; j1 += f->loop[1]) {
+          d[1] = j1 - mn[1];
+          for (int k1 = mn[2]; f->loop[2] * (k1 - mxx[2]) != 1; k1 += f->loop[2]) {
+            d[2] = k1 - mn[2];
+            const int i2 = mt[0] + f->tloop[0] * d[f->src_tar[0]];
+            const int j2 = mt[1] + f->tloop[1] * d[f->src_tar[1]];
+            const int k2 = mt[2] + f->tloop[2] * d[f->src_tar[2]];
+            const int64_t p = at(B->Lc, i1, j1, k1), pn = at(N->Lc, i2, j2, k2);
+            B->x[p] = N->x[pn];
+            B->y[p] = N->y[pn];
+            B->z[p] = N->z[pn];
+          }
+        }
+      }
+    }
+  }
+}
+
+/* DataCommunication.cuh:27-56, 58-110: for each block's inner faces in
+ * order, the face layer is averaged with the neighbour's (both written, |V|
+ * of both set) and the ghost layers take the neighbour's interior values;
+ * with_cv recomputes cv (and |V|) of this block's points. */
+static void inner_exchange(orc_mesh *M, int with_cv) {
+  for (int b = 0; b < M->nb; ++b) {
+    orc_zone *zz = M->z[b];
+    for (int q = 0; q < M->nface[b]; ++q) {
+      const orc_inner_face *f = &M->faces[M->face0[b] + q];
+      orc_zone *tz = M->z[f->tblk];
+      const int face_at = f->dir > 0 ? f->rs[f->face] : f->re[f->face];
+      for (int n0 = 0; n0 < f->n[0]; ++n0)
+        for (int n1 = 0; n1 < f->n[1]; ++n1)
+          for (int n2 = 0; n2 < f->n[2]; ++n2) {
+            const int nn[3] = {n0, n1, n2};
+            int idx[3], it[3], d[3];
+            for (int i = 0; i < 3; ++i) {
+              d[i] = f->loop[i] * nn[i];
+              idx[i] = f->rs[i] + d[i];
+            }
+            for (int i = 0; i < 3; ++i) it[i] = f->ts[i] + f->tloop[i] * d[f->src_tar[i]];
+            const int64_t p = at(zz->Lg, idx[0], idx[1], idx[2]), pt = at(tz->Lg, it[0], it[1], it[2]);
+            const int64_t sz = zz->Lg.sz, tsz = tz->Lg.sz;
+            if (idx[f->face] == face_at) {
+              for (int l = 0; l < 6; ++l) {
+                const double ave = 0.5 * (tz->bv[l * tsz + pt] + zz->bv[l * sz + p]);
+                zz->bv[l * sz + p] = ave;
+                tz->bv[l * tsz + pt] = ave;
+              }
+              for (int l = 0; l < zz->nsc; ++l) {
+                const double ave = 0.5 * (tz->sv[l * tsz + pt] + zz->sv[l * sz + p]);
+                zz->sv[l * sz + p] = ave;
+                tz->sv[l * tsz + pt] = ave;
+              }
+              const double u = zz->bv[sz + p], v = zz->bv[2 * sz + p], w = zz->bv[3 * sz + p];
+              const double vel = sqrt(u * u + v * v + w * w);
+              zz->vel[p] = vel;
+              tz->vel[pt] = vel;
+            } else {
+              for (int l = 0; l < 6; ++l) zz->bv[l * sz + p] = tz->bv[l * tsz + pt];
+              for (int l = 0; l < zz->nsc; ++l) zz->sv[l * sz + p] = tz->sv[l * tsz + pt];
+            }
+            if (with_cv) cv_from_bv_point(zz, idx[0], idx[1], idx[2]);
+          }
+    }
+  }
+}
+
+orc_mesh *orc_mesh_create(int nb, const int *dims, int ngg, const orc_cfg *cfgs, const orc_mech *mech,
+                          const double *const *xyz, const int *nface, const int *lines) {
+  orc_mesh *M = (orc_mesh *)calloc(1, sizeof(orc_mesh));
+  M->nb = nb;
+  M->z = (orc_zone **)calloc((size_t)nb, sizeof(orc_zone *));
+  M->nface = (int *)calloc((size_t)nb, sizeof(int));
+  M->face0 = (int *)calloc((size_t)nb, sizeof(int));
+  int nf = 0;
+  for (int b = 0; b < nb; ++b) {
+    M->nface[b] = nface[b];
+    M->face0[b] = nf;
+    nf += nface[b];
+  }
+  M->faces = (orc_inner_face *)calloc((size_t)(nf > 0 ? nf : 1), sizeof(orc_inner_face));
+  M->dim = dims[2] == 1 ? 2 : 3;
+  for (int q = 0; q < nf; ++q)
+    if (orc_inner_face_register(&lines[q * 14], &lines[q * 14 + 7], ngg, M->dim, &M->faces[q])) {
+      orc_mesh_destroy(M);
+      return NULL;
+    }
+  for (int b = 0; b < nb; ++b) {
+    const int mx = dims[3 * b], my = dims[3 * b + 1], mz = dims[3 * b + 2];
+    orc_zone *zz = orc_zone_create(mx, my, mz, ngg, &cfgs[b], mech, NULL, NULL, NULL);
+    M->z[b] = zz;
+    const double *x = xyz[3 * b], *y = xyz[3 * b + 1], *z = xyz[3 * b + 2];
+    for (int k = 0; k < mz; ++k)
+      for (int j = 0; j < my; ++j)
+        for (int i = 0; i < mx; ++i) {
+          const int64_t s = ((int64_t)k * my + j) * mx + i, p = at(zz->Lc, i, j, k);
+          zz->x[p] = x[s];
+          zz->y[p] = y[s];
+          zz->z[p] = z[s];
+        }
+    ghost_faces(mx, my, mz, ngg, M->dim, zz->Lc, zz->x, zz->y, zz->z);
+  }
+  inner_ghost_grid(M);
+  for (int b = 0; b < nb; ++b) {
+    orc_zone *zz = M->z[b];
+    ghost_edges(zz->mx, zz->my, zz->mz, ngg, zz->Lc, zz->x, zz->y, zz->z);
+    metrics(zz->mx, zz->my, zz->mz, ngg, zz->Lc, zz->Lg, zz->x, zz->y, zz->z, zz->jac, zz->metric);
+    zone_register(zz);
+  }
+  return M;
+}
+
+void orc_mesh_destroy(orc_mesh *M) {
+  if (!M) return;
+  for (int b = 0; b < M->nb; ++b) orc_zone_destroy(M->z[b]);
+  free(M->z);
+  free(M->nface);
+  free(M->face0);
+  free(M->faces);
+  free(M);
+}
+
+orc_zone *orc_mesh_zone(orc_mesh *M, int b) { return (b >= 0 && b < M->nb) ? M->z[b] : NULL; }
+
+const orc_inner_face *orc_mesh_face(orc_mesh *M, int q) { return &M->faces[q]; }
+
+/* Driver.cu:40-98 over all blocks (data communication without cv after the
+ * boundary conditions) followed by RK.cuh:38-48 */
+void orc_mesh_initialize(orc_mesh *M) {
+  for (int b = 0; b < M->nb; ++b) {
+    orc_zone *zz = M->z[b];
+    if (zz->dim == 2) eliminate_k_gradient(zz, 0);
+  }
+  for (int b = 0; b < M->nb; ++b) {
+    orc_zone *zz = M->z[b];
+    if (zz->cfg.mix) update_properties(zz);
+    apply_bcs(zz, 0);
+  }
+  for (int b = 0; b < M->nb; ++b) {
+    orc_zone *zz = M->z[b];
+    const int g = zz->g;
+    for (int k = -g; k < zz->mz + g; ++k)
+      for (int j = -g; j < zz->my + g; ++j)
+        for (int i = -g; i < zz->mx + g; ++i)
+          S1(vel, i, j, k) = sqrt(BV(1, i, j, k) * BV(1, i, j, k) + BV(2, i, j, k) * BV(2, i, j, k) +
+                                  BV(3, i, j, k) * BV(3, i, j, k));
+    if (zz->nt) initialize_mut(zz);
+  }
+  inner_exchange(M, 0);
+  for (int b = 0; b < M->nb; ++b) {
+    orc_zone *zz = M->z[b];
+    const int g = zz->g;
+    update_properties(zz);
+    store_last_step(zz);
+    for (int k = -g; k < zz->mz + g; ++k)
+      for (int j = -g; j < zz->my + g; ++j)
+        for (int i = -g; i < zz->mx + g; ++i) cv_from_bv_point(zz, i, j, k);
+    zz->step = 0;
+    zz->t_phys = 0;
+  }
+  M->step = 0;
+  M->t_phys = 0;
+}
+
+static double mesh_residual(orc_mesh *M, int step) {
+  double s[4] = {0, 0, 0, 0};
+  int64_t n = 0;
+  for (int b = 0; b < M->nb; ++b) {
+    double sb[4] = {0, 0, 0, 0};
+    residual_sums(M->z[b], sb);
+    for (int l = 0; l < 4; ++l) s[l] += sb[l];
+    n += (int64_t)M->z[b]->mx * M->z[b]->my * M->z[b]->mz;
+  }
+  return residual_finish(s, n, step, M->res, M->res_scale);
+}
+
+static void mesh_hist(orc_mesh *M, int step, int os, int it, double *res_hist) {
+  if (step % os == 0 || step == 1) {
+    const double err = mesh_residual(M, step);
+    if (res_hist) {
+      for (int l = 0; l < 4; ++l) res_hist[it * 5 + l] = M->res[l];
+      res_hist[it * 5 + 4] = err;
+    }
+  } else if (res_hist) {
+    for (int l = 0; l < 5; ++l) res_hist[it * 5 + l] = NAN;
+  }
+}
+
+/* RK.cuh:89-193 over all blocks: dt is the minimum over every block */
+void orc_mesh_run_rk3(orc_mesh *M, int nsteps, double *dt_hist, double *res_hist) {
+  orc_zone *z0 = M->z[0];
+  const int os = z0->cfg.output_screen > 0 ? z0->cfg.output_screen : 1;
+  for (int it = 0; it < nsteps; ++it) {
+    const int step = ++M->step;
+    double dt = z0->cfg.fixed_time_step ? z0->cfg.dt : 1e+6;
+    for (int b = 0; b < M->nb; ++b) {
+      orc_zone *zz = M->z[b];
+      if (step % os == 0) store_last_step(zz);
+      memcpy(zz->qn, zz->cv, sizeof(double) * zz->Lg.sz * zz->nv);
+    }
+    for (int rk = 0; rk < 3; ++rk) {
+      for (int b = 0; b < M->nb; ++b) orc_zone_rhs(M->z[b]);
+      if (!z0->cfg.fixed_time_step) {
+        dt = 1e+6;
+        for (int b = 0; b < M->nb; ++b) {
+          const double d = orc_zone_global_dt(M->z[b]);
+          if (d < dt) dt = d;
+        }
+      }
+      for (int b = 0; b < M->nb; ++b) {
+        rk_update(M->z[b], dt, rk);
+        apply_bcs(M->z[b], 1);
+      }
+      inner_exchange(M, 1);
+      for (int b = 0; b < M->nb; ++b) {
+        if (M->z[b]->dim == 2) eliminate_k_gradient(M->z[b], 1);
+      }
+      for (int b = 0; b < M->nb; ++b) update_properties(M->z[b]);
+      if (dt_hist) dt_hist[it * 3 + rk] = dt;
+    }
+    M->t_phys += dt;
+    for (int b = 0; b < M->nb; ++b) {
+      M->z[b]->t_phys = M->t_phys;
+      M->z[b]->step = step;
+    }
+    mesh_hist(M, step, os, it, res_hist);
+  }
+}
+
+/* SteadySim.cuh:52-124 over all blocks (cv follows at the boundaries and
+ * inner faces for the AWENO path, which reads it) */
+void orc_mesh_run_steady(orc_mesh *M, int nsteps, double *res_hist) {
+  orc_zone *z0 = M->z[0];
+  const int os = z0->cfg.output_screen > 0 ? z0->cfg.output_screen : 1;
+  const int with_cv = z0->inviscid_type == 1;
+  if (M->step == 0)
+    for (int b = 0; b < M->nb; ++b) store_last_step(M->z[b]);
+  for (int it = 0; it < nsteps; ++it) {
+    const int step = ++M->step;
+    for (int b = 0; b < M->nb; ++b) {
+      orc_zone *zz = M->z[b];
+      if (step % os == 0) store_last_step(zz);
+      orc_zone_rhs(zz);
+      (void)orc_zone_global_dt(zz);
+      orc_zone_implicit(zz);
+      update_bv(zz);
+      apply_bcs(zz, with_cv);
+    }
+    inner_exchange(M, with_cv);
+    for (int b = 0; b < M->nb; ++b) {
+      if (M->z[b]->dim == 2) eliminate_k_gradient(M->z[b], with_cv);
+    }
+    for (int b = 0; b < M->nb; ++b) {
+      update_properties(M->z[b]);
+      M->z[b]->step = step;
+    }
+    mesh_hist(M, step, os, it, res_hist);
+  }
+}
+
+void orc_mesh_exchange(orc_mesh *M, int with_cv) { inner_exchange(M, with_cv); }

@@ -161,6 +161,33 @@ void orc_zone_chem_step(orc_zone *zz, double dt);
 void orc_grid_metrics(int mx, int my, int mz, int ngg, const double *x, const double *y, const double *z,
                       double *xg, double *yg, double *zg, double *jac, double *metric);
 
+/* ---- multi-block meshes (one GPU: blocks exchange through inner faces) ---- */
+/* an inner face after Mesh.cpp:34-187 InnerFace::register_boundary (0-based) */
+typedef struct {
+  int rs[3], re[3];           /* this block's range: face layer + ngg ghost layers, full tangential extent */
+  int face, dir;              /* normal direction (0 i, 1 j, 2 k), -1 on the low face */
+  int ts[3], te[3];           /* target block's start (and end) indices */
+  int tblk, tface, tdir;      /* target block (0-based), its face normal and side */
+  int src_tar[3];             /* target direction i takes its offset from this block's direction src_tar[i] */
+  int loop[3], tloop[3];      /* index steps along each direction in this block and the target */
+  int n[3];                   /* points per direction */
+} orc_inner_face;
+int orc_inner_face_register(const int *s7, const int *t7, int ngg, int dim, orc_inner_face *f);
+typedef struct orc_mesh orc_mesh;
+/* nb blocks: dims 3 per block, cfgs one per block (its boundaries), xyz 3
+ * interior coordinate arrays per block, nface[b] inner faces of block b and
+ * lines[14 per face] = source line (7 ints) then target line (7 ints) as in
+ * the reference's inner-face file.  NULL for an unsupported face. */
+orc_mesh *orc_mesh_create(int nb, const int *dims, int ngg, const orc_cfg *cfgs, const orc_mech *mech,
+                          const double *const *xyz, const int *nface, const int *lines);
+void orc_mesh_destroy(orc_mesh *M);
+orc_zone *orc_mesh_zone(orc_mesh *M, int b);
+const orc_inner_face *orc_mesh_face(orc_mesh *M, int q);
+void orc_mesh_initialize(orc_mesh *M);
+void orc_mesh_run_rk3(orc_mesh *M, int nsteps, double *dt_hist, double *res_hist);
+void orc_mesh_run_steady(orc_mesh *M, int nsteps, double *res_hist);
+void orc_mesh_exchange(orc_mesh *M, int with_cv);
+
 /* point-level test hooks (see orc_flow.c) */
 void orc_face_flux_air(int scheme, const double *pl5, const double *pr5, const double *met6, const double *jac2,
                        double *f5);

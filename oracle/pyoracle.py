@@ -93,6 +93,13 @@ if os.environ.get("OSTC_ORACLE_LIB"):
     LIB = os.environ["OSTC_ORACLE_LIB"]
 
 
+class InnerFace(C.Structure):
+    _fields_ = [("rs", C.c_int * 3), ("re", C.c_int * 3), ("face", C.c_int), ("dir", C.c_int),
+                ("ts", C.c_int * 3), ("te", C.c_int * 3), ("tblk", C.c_int), ("tface", C.c_int),
+                ("tdir", C.c_int), ("src_tar", C.c_int * 3), ("loop", C.c_int * 3), ("tloop", C.c_int * 3),
+                ("n", C.c_int * 3)]
+
+
 def lib():
     global _lib
     if _lib is None:
@@ -145,6 +152,18 @@ def lib():
         L.orc_zone_store_last.argtypes = [C.c_void_p]
         L.orc_zone_source.argtypes = [C.c_void_p]
         L.orc_zone_residual.argtypes = [C.c_void_p, C.c_int, dp]
+        ip = C.POINTER(C.c_int)
+        L.orc_inner_face_register.argtypes = [ip, ip, C.c_int, C.c_int, C.POINTER(InnerFace)]
+        L.orc_mesh_create.argtypes = [C.c_int, ip, C.c_int, C.POINTER(Cfg), C.POINTER(Mech),
+                                      C.POINTER(C.POINTER(C.c_double)), ip, ip]
+        L.orc_mesh_create.restype = C.c_void_p
+        L.orc_mesh_destroy.argtypes = [C.c_void_p]
+        L.orc_mesh_zone.argtypes = [C.c_void_p, C.c_int]
+        L.orc_mesh_zone.restype = C.c_void_p
+        L.orc_mesh_initialize.argtypes = [C.c_void_p]
+        L.orc_mesh_run_rk3.argtypes = [C.c_void_p, C.c_int, dp, dp]
+        L.orc_mesh_run_steady.argtypes = [C.c_void_p, C.c_int, dp]
+        L.orc_mesh_exchange.argtypes = [C.c_void_p, C.c_int]
         _lib = L
     return _lib
 
@@ -415,7 +434,8 @@ class Zone:
                                        _p(self._y), _p(self._z))
 
     def __del__(self):
-        if getattr(self, "h", None):
+        # a zone of a Mesh belongs to the mesh, which frees it
+        if getattr(self, "h", None) and getattr(self, "_borrowed", None) is None:
             lib().orc_zone_destroy(self.h)
             self.h = None
 
@@ -468,3 +488,82 @@ class Zone:
         t, s = C.c_double(), C.c_int()
         lib().orc_zone_state(self.h, C.byref(t), C.byref(s))
         return t.value, s.value
+
+
+class Mesh:
+    """Multi-block oracle mesh: blocks joined by inner faces (lines as in the
+    reference's inner-face file), each block an oracle zone."""
+
+    def __init__(self, blocks, ngg, scheme, inner, mech=None):
+        """blocks: list of dict(mx, my, mz, x, y, z, bcs); inner: list of
+        (block, src7, tar7) in file order."""
+        nb = len(blocks)
+        self.mech = mech if mech is not None else Mech()
+        self.cfgs = (Cfg * nb)()
+        self._keep = []
+        dims = np.zeros(3 * nb, dtype=np.int32)
+        ptrs = (C.POINTER(C.c_double) * (3 * nb))()
+        for b, blk in enumerate(blocks):
+            self.cfgs[b] = make_cfg(dict(scheme, bcs=blk["bcs"]))
+            dims[3 * b:3 * b + 3] = (blk["mx"], blk["my"], blk["mz"])
+            for c, key in enumerate("xyz"):
+                a = np.ascontiguousarray(blk[key], dtype=np.float64).ravel()
+                self._keep.append(a)
+                ptrs[3 * b + c] = _p(a)
+        nface = np.zeros(nb, dtype=np.int32)
+        lines = []
+        for b in range(nb):
+            for (fb, src, tar) in inner:
+                if fb == b:
+                    nface[b] += 1
+                    lines.extend(list(src) + list(tar))
+        lines = np.asarray(lines if lines else [0], dtype=np.int32)
+        ip = C.POINTER(C.c_int)
+        self.h = lib().orc_mesh_create(nb, dims.ctypes.data_as(ip), ngg, self.cfgs, C.byref(self.mech), ptrs,
+                                       nface.ctypes.data_as(ip), lines.ctypes.data_as(ip))
+        if not self.h:
+            raise ValueError("unsupported inner face")
+        self.nb = nb
+        self.zones = []
+        for b in range(nb):
+            z = Zone.__new__(Zone)
+            z.mx, z.my, z.mz, z.g = blocks[b]["mx"], blocks[b]["my"], blocks[b]["mz"], ngg
+            z.h = lib().orc_mesh_zone(self.h, b)
+            z._borrowed = self
+            self.zones.append(z)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            for z in self.zones:
+                z.h = None
+            lib().orc_mesh_destroy(self.h)
+            self.h = None
+
+    def initialize(self):
+        lib().orc_mesh_initialize(self.h)
+
+    def run(self, nsteps):
+        dt = np.zeros(nsteps * 3)
+        res = np.zeros(nsteps * 5)
+        lib().orc_mesh_run_rk3(self.h, nsteps, _p(dt), _p(res))
+        return dt.reshape(nsteps, 3), res.reshape(nsteps, 5)
+
+    def run_steady(self, n):
+        res = np.zeros((n, 5))
+        lib().orc_mesh_run_steady(self.h, n, _p(res))
+        return res
+
+    def exchange(self, with_cv=False):
+        lib().orc_mesh_exchange(self.h, 1 if with_cv else 0)
+
+
+def inner_face_register(src7, tar7, ngg, dim):
+    """The oracle's registration of one inner-face pair: dict of fields, or None."""
+    f = InnerFace()
+    a = np.ascontiguousarray(src7, dtype=np.int32)
+    b = np.ascontiguousarray(tar7, dtype=np.int32)
+    ip = C.POINTER(C.c_int)
+    if lib().orc_inner_face_register(a.ctypes.data_as(ip), b.ctypes.data_as(ip), ngg, dim, C.byref(f)) != 0:
+        return None
+    return {k: (list(getattr(f, k)) if isinstance(getattr(f, k), C.Array) else getattr(f, k)) for k, _ in f._fields_}
+

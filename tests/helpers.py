@@ -30,6 +30,35 @@ def oracle_zone(case, mech=None, bv=None, sv=None):
     return z
 
 
+def _oracle_bcs(case, bcs):
+    out = []
+    for b in bcs:
+        d = dict(type=b["type"], raw=b["raw"])
+        if b["type"] == 5:
+            st = b.get("state", case.get("inflow"))
+            d["state"] = dict(st, sv=list(st.get("sv", [])))
+        elif "state" in b:
+            d["state"] = dict(b["state"], sv=list(b["state"].get("sv", [])))
+        if b["type"] == 2:
+            d["t_wall"] = b.get("t_wall", 300.0)
+            d["isothermal"] = b.get("isothermal", True)
+        out.append(d)
+    return out
+
+
+def oracle_mesh(mcase, fields, mech=None):
+    """Oracle mesh for a multi-block case (ostc.cases.split_case) with the
+    given per-block (bv, sv), initialised like the solver."""
+    blocks = [dict(b, bcs=_oracle_bcs(mcase, b["bcs"])) for b in mcase["blocks"]]
+    M = po.Mesh(blocks, mcase["ngg"], mcase["scheme"], mcase["inner"], mech=mech)
+    for z, (bv, sv) in zip(M.zones, fields):
+        z.field("bv")[:] = np.asarray(bv).ravel()
+        if np.asarray(sv).size:
+            z.field("sv")[:] = np.asarray(sv).ravel()
+    M.initialize()
+    return M
+
+
 def relerr(a, b):
     """Normwise relative error ||a-b||_inf / ||b||_inf (0 if both vanish)."""
     a = np.asarray(a, dtype=np.float64).ravel()
