@@ -170,6 +170,58 @@ def cpu_baseline_c5(nx=96, ny=48, nz=32, iters=2):
     return cells * iters / (t1 - t0) / 1e6, f"{nx}x{ny}x{nz} grid, {iters} pseudo-iterations", t1 - t0
 
 
+def c5_inline(s5, cells, a):
+    """The channel with inline (operator-split) chemistry, FP32 fluxes: one
+    chemistry step of a.c5_chem_dt per pseudo-iteration by the FP64
+    point-implicit integrator, then by the surrogate with its fallback
+    (trained first, untimed, on the channel's own states at that dt), both
+    from the same state; times, the fallback fraction and the surrogate
+    run's errors against the integrator run."""
+    import ostc
+    dt = a.c5_chem_dt
+    n = cells
+    st = {k: ostc.DeviceArray(s5, 8 * n * c) for k, c in (("rho", 1), ("T", 1), ("Y", 9))}
+    s5.chem_states(st["rho"], st["T"], st["Y"])
+    log(f"bench: channel inline chemistry: training the surrogate on {n} channel states at dt = {dt:g} s")
+    s5.surrogate_init(31)
+    t_tr = time.perf_counter()
+    hist = s5.surrogate_train(n, st["rho"], st["T"], st["Y"], dt, epochs=a.c5_epochs, batch=8192, seed=31)
+    t_tr = time.perf_counter() - t_tr
+    for v in st.values():
+        v.free()
+    snap = {f: s5.download(f) for f in ("bv", "sv", "cv")}
+    out = {"chem_dt": dt, "train_states": n, "train_epochs": a.c5_epochs, "train_s": t_tr,
+           "train_final_loss": float(hist[-1])}
+    res = {}
+    for mode, tag in ((0, "integrator"), (1, "surrogate")):
+        for f, v in snap.items():
+            s5.upload(f, v)
+        s5.update_properties()
+        s5.set_steady_chem(dt, mode)
+        s5.step_steady(1)
+        s5.sync()
+        t0 = time.perf_counter()
+        s5.step_steady(a.c5_iters)
+        s5.sync()
+        t1 = time.perf_counter()
+        c, fb = s5.steady_chem_stats()
+        res[mode] = (s5.download("bv"), s5.download("sv"))
+        if mode == 1:
+            out["gate_fallback_by_condition_last_iteration"] = s5.surrogate_gate_reasons()
+        out[tag + "_ms_per_iter"] = (t1 - t0) / a.c5_iters * 1e3
+        out[tag + "_mcell_iter_per_s"] = cells * a.c5_iters / (t1 - t0) / 1e6
+        out[tag + "_fallback_fraction"] = fb / max(c, 1)
+    s5.set_steady_chem(0.0)
+    (bv0, sv0), (bv1, sv1) = res[0], res[1]
+    T0, T1 = bv0.reshape(6, -1)[5], bv1.reshape(6, -1)[5]
+    Y0, Y1 = sv0.reshape(11, -1), sv1.reshape(11, -1)
+    out["iters_compared"] = a.c5_iters + 1
+    out["peak_T_rel_err"] = float(abs(T1.max() - T0.max()) / T0.max())
+    out["T_rel_linf"] = float(np.max(np.abs(T1 - T0) / T0))
+    out["major_species_linf"] = float(np.abs(Y1[[0, 2, 7, 8]] - Y0[[0, 2, 7, 8]]).max())
+    return out
+
+
 def cpu_model():
     try:
         with open("/proc/cpuinfo") as f:
@@ -260,6 +312,9 @@ def main():
     ap.add_argument("--split-dt", type=float, default=1e-8)
     ap.add_argument("--split-pairs", type=int, default=1 << 22)
     ap.add_argument("--c5-iters", type=int, default=5)
+    ap.add_argument("--no-c5-inline", dest="c5_inline", action="store_false")
+    ap.add_argument("--c5-chem-dt", type=float, default=1e-7)
+    ap.add_argument("--c5-epochs", type=int, default=4)
     ap.add_argument("--chem-cells", type=int, default=1 << 22)
     ap.add_argument("--train-pairs", type=int, default=1 << 24)
     ap.add_argument("--epochs", type=int, default=16)
@@ -368,12 +423,33 @@ def main():
             run(0, a.steps)
             t1 = time.perf_counter()
             bv_ref, sv_ref = s4.download("bv").reshape(6, -1), s4.download("sv").reshape(9, -1)
-            log(f"bench: surrogate training for the split step on {a.split_pairs} mixing-layer states (untimed)")
+            # training states: the flow's own chemistry inputs at every split
+            # step of the run (replayed untimed from the initial state) plus
+            # synthetic mixing-layer states for coverage
+            s4.upload("bv", bv0)
+            s4.upload("sv", sv0)
+            s4.initialize()
+            g_st = {k: ostc.DeviceArray(s4, 8 * n4 * c) for k, c in (("rho", 1), ("T", 1), ("Y", 9))}
+            hv = {"rho": [], "T": [], "Y": []}
+            for _ in range(a.warmup + a.steps):
+                s4.step(1)
+                s4.chem_states(g_st["rho"], g_st["T"], g_st["Y"])
+                hv["rho"].append(g_st["rho"].to_host((n4,)))
+                hv["T"].append(g_st["T"].to_host((n4,)))
+                hv["Y"].append(g_st["Y"].to_host((9, n4)))
+                s4.chem_step(a.split_dt, mode=0)
+            for v in g_st.values():
+                v.free()
             r2, t2, p2, y2 = cases.mixing_layer_states(a.split_pairs, mw=mw9)
+            r2 = np.concatenate(hv["rho"] + [r2])
+            t2 = np.concatenate(hv["T"] + [t2])
+            y2 = np.ascontiguousarray(np.concatenate(hv["Y"] + [y2], axis=1))
+            n_tr = r2.size
+            log(f"bench: surrogate training for the split step on {n_tr} states ({n_tr - a.split_pairs} from the "
+                f"flow) (untimed)")
             tr = [ostc.DeviceArray.from_host(s4, v) for v in (r2, t2, y2)]
             s4.surrogate_init(13)
-            hist4 = s4.surrogate_train(a.split_pairs, tr[0], tr[1], tr[2], a.split_dt, epochs=8, batch=8192,
-                                       seed=2003)
+            hist4 = s4.surrogate_train(n_tr, tr[0], tr[1], tr[2], a.split_dt, epochs=8, batch=8192, seed=2003)
             for v in tr:
                 v.free()
             s4.upload("bv", bv0)
@@ -381,11 +457,15 @@ def main():
             s4.initialize()
             s4.set_flux_fp32(True)
             nfb = []
+            reasons = {}
 
-            def run_sur(nsteps):
+            def run_sur(nsteps, count=False):
                 for _ in range(nsteps):
                     s4.step(1)
                     nfb.append(s4.chem_step(a.split_dt, mode=1))
+                    if count:
+                        for k, v in s4.surrogate_gate_reasons().items():
+                            reasons[k] = reasons.get(k, 0) + v
                 s4.sync()
 
             run_sur(a.warmup)
@@ -393,6 +473,8 @@ def main():
             run_sur(a.steps)
             t3_ = time.perf_counter()
             bv_s, sv_s = s4.download("bv").reshape(6, -1), s4.download("sv").reshape(9, -1)
+            # gate statistics from one more, untimed step
+            run_sur(1, count=True)
             s4.close()
             Tr, Ts = bv_ref[5], bv_s[5]
             split = {"workload": "2-D reacting H2/air mixing layer 512x256, operator-split step: SSP-RK3 flow step "
@@ -408,7 +490,9 @@ def main():
                      "peak_T_rel_err": float(abs(Ts.max() - Tr.max()) / Tr.max()),
                      "T_rel_linf": float(np.max(np.abs(Ts - Tr) / Tr)),
                      "major_species_linf": float(np.abs(sv_s[[0, 2, 7, 8]] - sv_ref[[0, 2, 7, 8]]).max()),
-                     "train_final_loss": float(hist4[-1])}
+                     "train_states": n_tr, "train_states_from_flow": n_tr - a.split_pairs,
+                     "train_final_loss": float(hist4[-1]),
+                     "gate_fallback_by_condition": reasons}
 
         c5 = None
         if not a.no_c5:
@@ -417,11 +501,13 @@ def main():
             c5_cells = 384 * 192 * 128
             s5.step_steady(2)
             s5.sync()
-            s5.profile(True)
             t0 = time.perf_counter()
             res5 = s5.step_steady(a.c5_iters)
             s5.sync()
             t1 = time.perf_counter()
+            # per-kernel breakdown from a separate profiled pseudo-iteration
+            s5.profile(True)
+            s5.step_steady(1)
             p5 = s5.profile_report()
             s5.profile(False)
             s5.set_flux_fp32(True)
@@ -431,9 +517,9 @@ def main():
             s5.step_steady(a.c5_iters)
             s5.sync()
             t3 = time.perf_counter()
-            s5.close()
             c5 = {"workload": "3-D reacting RANS (k-omega SST) H2/air channel 384x192x128 with 4 sonic H2 wall "
-                              "jets, 9 species + k, omega, HLLC + MUSCL, viscous, steady DPLUR (3 inner sweeps), "
+                              "jets, 9 species + k, omega, HLLC + AWENO-5 (multi-species characteristic "
+                              "interpolation carrying k and omega), viscous, steady DPLUR (3 inner sweeps), "
                               "exact point-implicit chemistry, implicit SST source",
                   "cells": c5_cells, "iters": a.c5_iters,
                   "fp64_mcell_iter_per_s": c5_cells * a.c5_iters / (t1 - t0) / 1e6,
@@ -442,6 +528,9 @@ def main():
                   "mixed_ms_per_iter": (t3 - t2) / a.c5_iters * 1e3,
                   "residual_finite": bool(np.isfinite(res5[:, 4]).all()),
                   "kernel_avg_ms": {k: v[1] / v[0] for k, v in p5.items()}}
+            if a.c5_inline:
+                c5["inline_chemistry"] = c5_inline(s5, c5_cells, a)
+            s5.close()
 
         # chemistry batch: FP64 point-implicit, then the surrogate
         chem = sur = None

@@ -236,6 +236,24 @@ int ostc_set_flux_precision(ostc_solver *s, int fp32);
  * (FiniteRateChem.cu:7-63, 234-256). */
 int ostc_chem_batch(ostc_solver *s, int64_t n, const double *rho, const double *T, const double *Y, double dt,
                     double *dY, double *dT, int mode, int64_t *n_fallback);
+/* Output and restart files in the reference's format and places, under
+ * <root>/output: field/flowfield.plt (Tecplot binary V112, one ordered zone
+ * per block with the interior points; variables x, y, z, density, u, v, w,
+ * pressure, temperature, mach, the species by name, tke, omega and mut for
+ * SST), message/step.txt and, once a step-1 residual has set it,
+ * message/residual_scale.txt.
+ * Replaces: FieldIO::write_header / write_common_data_section / print_field
+ * (FieldIO.h:58-600) and Residual.cuh:123-139. */
+int ostc_write_flowfield(ostc_solver *s, const char *root);
+/* Reads those files back into a finalized solver: variables are matched by
+ * name (density/rho/roe, u, v, w, p/pressure, t/temperature, species names,
+ * k/tke, omega; others ignored), the points the file holds are overwritten
+ * (other points keep their values), species or turbulence variables the file
+ * lacks take the default inflow state, and the next ostc_initialize keeps the
+ * file's step and solution time; the residual scale is restored.
+ * Replaces: read_flowfield + identify_variable_labels
+ * (Initialize.cuh:64-313), Driver.cu:21-27. */
+int ostc_read_flowfield(ostc_solver *s, const char *root);
 /* Operator-split reacting steps.  ostc_set_chem_split(s, 1) leaves the
  * finite-rate source out of the right-hand side (the flow step runs
  * non-reacting); ostc_chem_step then advances every interior cell of a block
@@ -245,6 +263,23 @@ int ostc_chem_batch(ostc_solver *s, int64_t n, const double *rho, const double *
  * the reference, whose RK3 adds the explicit source in every stage
  * (RK.cuh:107-161); this is the split form of "RK3 + point-implicit". */
 int ostc_set_chem_split(ostc_solver *s, int on);
+/* Inline chemistry on the steady path: with dt > 0, ostc_step_steady runs
+ * the flow pseudo-iteration without the chemical source (and without the
+ * chemistry Jacobian in the implicit treatment), then advances every
+ * interior cell by one chemistry step dt at its density and total energy
+ * (mode 0 FP64 point-implicit integrator, mode 1 surrogate + fallback,
+ * as ostc_chem_batch) before the boundary conditions.  dt <= 0 switches it
+ * off (the reference's point-implicit source in the right-hand side).  No
+ * counterpart in the reference, whose steady path keeps the source in the
+ * right-hand side (SteadySim.cuh:73-83, ImplicitTreatmentHPP.cuh:7-52).
+ * ostc_steady_chem_stats: cells integrated and fallback cells since the last
+ * ostc_set_steady_chem. */
+int ostc_set_steady_chem(ostc_solver *s, double dt, int mode);
+/* The interior cells' rho, T and Y (component-major, n_spec x cells, i
+ * fastest) of a block into caller DEVICE arrays of mx*my*mz cells: the
+ * chemistry states of a running flow, e.g. to train the surrogate on. */
+int ostc_chem_states(ostc_solver *s, int blk, double *rho, double *T, double *Y);
+int ostc_steady_chem_stats(ostc_solver *s, int64_t *cells, int64_t *fallback);
 int ostc_chem_step(ostc_solver *s, int blk, double dt, int mode, int64_t *n_fallback);
 
 /* ---- MLP chemistry surrogate (no counterpart in the reference) ----
@@ -305,6 +340,12 @@ int ostc_surrogate_network_outputs(ostc_solver *s, int64_t n, const double *rho,
 /* Last surrogate chem batch: out[4] = fallback cells, network ms (features + MLP),
  * gate ms, FP64 fallback ms (HIP events on the solver stream). */
 int ostc_surrogate_last_stats(ostc_solver *s, double *out);
+/* Why the last inference's cells fell back: out[0] the fallback count,
+ * out[r] the cells whose first failing gate condition was r: 1 input outside
+ * the training range, 2 non-finite output, 3 |dY_k| > dy_max, 4 |dT| >
+ * dT_max, 5 predicted sigma_min(I - dt J) < sig_min, 6 conservation error >
+ * cons_tol, 7 temperature inversion failed, 8 dt other than the training dt. */
+int ostc_surrogate_gate_reasons(ostc_solver *s, int64_t *out);
 
 /* Device memory helpers for callers without their own allocator. */
 int ostc_dev_alloc(ostc_solver *s, int64_t bytes, void **dptr);

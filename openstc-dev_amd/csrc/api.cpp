@@ -6,9 +6,11 @@
 
 #include <algorithm>
 #include <array>
+#include <cctype>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <filesystem>
 #include <fstream>
 #include <map>
 #include <memory>
@@ -95,6 +97,13 @@ struct ostc_solver {
   std::vector<void *> user_mem;
   SurrogateState *sur = nullptr;
   bool chem_split = false;          // finite-rate source left out of the RHS (operator-split chemistry)
+  double steady_chem_dt = 0;        // steady path: inline chemistry step per pseudo-iteration (0: off)
+  int steady_chem_mode = -1;        // 0 FP64 point-implicit, 1 surrogate + fallback
+  int64_t steady_cells = 0, steady_fb = 0;  // inline chemistry cells and fallbacks since it was set
+  bool res_scale_set = false;       // a step-1 residual has set the residual scale
+  bool restart = false;             // fields, step and time came from ostc_read_flowfield
+  int restart_step = 0;
+  double restart_time = 0;
   double *chem_buf = nullptr;       // gather / result buffers of ostc_chem_step (21 doubles per cell)
   int64_t chem_cap = 0;
 
@@ -806,7 +815,7 @@ int ostc_init(const char *case_dir, int device, ostc_solver **out) {
                 " with the characteristic (AWENO) path is not supported: only reconstruction 4 (AWENO-5)");
   // mixtures and k / omega take the multi-species characteristic
   // interpolation (the reference's mixture AWENO is an empty stub,
-  // AWENO.cu:10-15; this build defines it in flux.hip)
+  // AWENO.cu:10-15; this build defines it in flux_device.h)
   if (n_turb && !P.viscous) return fail("SST needs the viscous flux (viscous_order 2)");
   if (P.mix && P.ns != 9) return fail("this build is compiled for 9-species mechanisms");
   if (device < 0) {  // host-only: setup files, tables and grid metrics, no device work
@@ -1445,6 +1454,7 @@ int ostc_residual(ostc_solver *s, int step, double res[4], double *err_max) {
   if (check_blk(s, 0)) return 1;
   HIPCHK(hipSetDevice(s->device));
   if (residual_all(s, 0, step == 1 ? 1 : 0, s->w.res_one)) return 1;
+  if (step == 1) s->res_scale_set = true;
   double h[5];
   HIPCHK(hipMemcpyAsync(h, s->w.res_one, sizeof(h), hipMemcpyDeviceToHost, s->stream));
   HIPCHK(hipStreamSynchronize(s->stream));
@@ -1512,8 +1522,9 @@ int ostc_initialize(ostc_solver *s) {
     HIPCHK(launch_store_last(b.z, s->stream));
     HIPCHK(launch_cv_from_bv(b.z, s->P, s->d_mech, s->stream));
   }
-  s->t_phys = 0;
-  s->step = 0;
+  // a restart continues the step count and time of the file it was read from
+  s->t_phys = s->restart ? s->restart_time : 0;
+  s->step = s->restart ? s->restart_step : 0;
   HIPCHK(hipStreamSynchronize(s->stream));
   return 0;
 }
@@ -1616,6 +1627,7 @@ int ostc_step_rk3(ostc_solver *s, int nsteps, double *dt_hist, double *res_hist)
       if (step % os == 0 || step == 1) {
         ostc_solver::Timer t(s, "residual");
         if (residual_all(s, it, step == 1 ? 1 : 0, nullptr)) return 1;
+        if (step == 1) s->res_scale_set = true;
       }
     }
     return 0;
@@ -1650,16 +1662,22 @@ int ostc_step_rk3(ostc_solver *s, int nsteps, double *dt_hist, double *res_hist)
   return 0;
 }
 
+int chem_interior(ostc_solver *s, Block &b, double dt, int mode, int64_t *nf_out);
+
 static int implicit_block(ostc_solver *s, Block &b) {
   const int method = s->prm.get_int("implicit_method", 0);
   ostc_solver::Timer t(s, "implicit");
+  // split chemistry: the right-hand side has no chemical source, so the
+  // implicit treatment has no chemistry Jacobian either
+  DParams P = s->P;
+  if (s->chem_split) P.reaction = 0;
   if (method == 1) {
     std::vector<BCDesc> walls;
     for (auto &bc : b.bcs)
       if (bc.type == BC_WALL) walls.push_back(bc);
-    HIPCHK(launch_dplur(b.z, s->P, s->d_mech, s->prm.get_int("DPLUR_inner_step", 3), walls, s->stream));
+    HIPCHK(launch_dplur(b.z, P, s->d_mech, s->prm.get_int("DPLUR_inner_step", 3), walls, s->stream));
   } else {
-    HIPCHK(launch_chem_implicit(b.z, s->P, s->stream));
+    HIPCHK(launch_chem_implicit(b.z, P, s->stream));
     if (s->n_turb && s->P.turb_implicit == 1) HIPCHK(launch_turb_implicit(b.z, s->P, s->stream));
   }
   return 0;
@@ -1698,6 +1716,13 @@ int ostc_step_steady(ostc_solver *s, int nsteps, double *res_hist) {
           ostc_solver::Timer t(s, "update_bv");
           HIPCHK(launch_update_bv(b.z, s->P, s->d_mech, s->stream));
         }
+        if (s->steady_chem_dt > 0) {
+          // inline operator-split chemistry after the flow update
+          int64_t nf = 0;
+          if (chem_interior(s, b, s->steady_chem_dt, s->steady_chem_mode, &nf)) return 1;
+          s->steady_cells += (int64_t)b.mx * b.my * b.mz;
+          s->steady_fb += nf;
+        }
         if (run_bcs(s, b, with_cv)) return 1;
       }
       if (exchange_all(s, with_cv)) return 1;
@@ -1708,6 +1733,7 @@ int ostc_step_steady(ostc_solver *s, int nsteps, double *res_hist) {
       if (step % os == 0 || step == 1) {
         ostc_solver::Timer t(s, "residual");
         if (residual_all(s, it, step == 1 ? 1 : 0, nullptr)) return 1;
+        if (step == 1) s->res_scale_set = true;
       }
     }
     return 0;
@@ -1774,6 +1800,227 @@ int ostc_chem_batch(ostc_solver *s, int64_t n, const double *rho, const double *
   return 0;
 }
 
+int ostc_chem_states(ostc_solver *s, int blk, double *rho, double *T, double *Y) {
+  if (check_blk(s, blk)) return 1;
+  if (!rho || !T || !Y) return fail("null argument");
+  if (s->n_spec != 9) return fail("chemistry states need the 9-species mechanism");
+  HIPCHK(hipSetDevice(s->device));
+  HIPCHK(launch_chem_gather(s->blocks[blk].z, s->P, rho, T, Y, s->stream));
+  HIPCHK(hipStreamSynchronize(s->stream));
+  return 0;
+}
+
+int ostc_set_steady_chem(ostc_solver *s, double dt, int mode) {
+  if (!s) return fail("null solver");
+  if (dt > 0) {
+    if (!(s->P.mix && s->P.reaction == 1) || s->n_spec != 9 || s->mech.n_reac <= 0)
+      return fail("inline chemistry needs the 9-species reacting mechanism");
+    if (mode != 0 && mode != 1) return fail("mode must be 0 (point-implicit) or 1 (surrogate)");
+    if (mode == 1 && (!s->sur || !surrogate_ready(s->sur))) return fail("no trained surrogate: train or load one first");
+    s->chem_split = true;
+    s->steady_chem_dt = dt;
+    s->steady_chem_mode = mode;
+  } else {
+    s->chem_split = false;
+    s->steady_chem_dt = 0;
+    s->steady_chem_mode = -1;
+  }
+  s->steady_cells = s->steady_fb = 0;
+  return 0;
+}
+
+int ostc_steady_chem_stats(ostc_solver *s, int64_t *cells, int64_t *fallback) {
+  if (!s) return fail("null solver");
+  if (cells) *cells = s->steady_cells;
+  if (fallback) *fallback = s->steady_fb;
+  return 0;
+}
+
+// ---------------------------------------------------------------- field files
+// Variable names of the output file (FieldIO.h:422-455): coordinates, the
+// basic variables and the Mach number, the species, k and omega, mu_t.
+static std::vector<std::string> output_names(const ostc_solver *s) {
+  std::vector<std::string> v{"x", "y", "z", "density", "u", "v", "w", "pressure", "temperature", "mach"};
+  for (int l = 0; l < s->n_spec; ++l) v.push_back(s->mech.names[l]);
+  if (s->n_turb) {
+    v.push_back("tke");
+    v.push_back("omega");
+    v.push_back("mut");
+  }
+  return v;
+}
+
+int ostc_write_flowfield(ostc_solver *s, const char *root) {
+  if (check_blk(s, 0)) return 1;
+  if (!root) return fail("null output directory");
+  HIPCHK(hipSetDevice(s->device));
+  const std::string base = std::string(root) + "/output";
+  std::error_code ec;
+  std::filesystem::create_directories(base + "/field", ec);
+  std::filesystem::create_directories(base + "/message", ec);
+  if (ec) return fail("cannot create " + base + ": " + ec.message());
+  const auto names = output_names(s);
+  // interior points of every block (the reference writes ngg_output = 0)
+  std::vector<std::vector<double>> store;
+  std::vector<PltZone> zones;
+  for (auto &b : s->blocks) {
+    const int64_t n = (int64_t)b.mx * b.my * b.mz;
+    auto take = [&](const double *dev, int comps) -> int {
+      std::vector<double> h((size_t)b.z.sz * comps);
+      HIPCHK(hipMemcpyAsync(h.data(), dev, h.size() * sizeof(double), hipMemcpyDeviceToHost, s->stream));
+      HIPCHK(hipStreamSynchronize(s->stream));
+      for (int c = 0; c < comps; ++c) {
+        std::vector<double> v((size_t)n);
+        size_t q = 0;
+        for (int k = 0; k < b.mz; ++k)
+          for (int j = 0; j < b.my; ++j)
+            for (int i = 0; i < b.mx; ++i) v[q++] = h[(size_t)c * b.z.sz + b.z.at(i, j, k)];
+        store.push_back(std::move(v));
+      }
+      return 0;
+    };
+    const size_t first = store.size();
+    store.push_back(b.x0);
+    store.push_back(b.y0);
+    store.push_back(b.z0);
+    if (take(b.z.bv, 6) || take(b.z.mach, 1)) return 1;
+    if (s->n_scalar > 0 && take(b.z.sv, s->n_scalar)) return 1;
+    if (s->n_turb && take(b.z.mut, 1)) return 1;
+    PltZone zn;
+    zn.mx = b.mx;
+    zn.my = b.my;
+    zn.mz = b.mz;
+    for (size_t q = first; q < store.size(); ++q) zn.vars.push_back(nullptr);
+    zones.push_back(zn);
+  }
+  // pointers after all arrays are in place
+  size_t q = 0;
+  for (auto &zn : zones)
+    for (auto &p : zn.vars) p = store[q++].data();
+  std::string err;
+  if (!plt_write(base + "/field/flowfield.plt", names, zones, s->t_phys, err)) return fail(err);
+  // FieldIO.h:457-462 and Residual.cuh:123-139: the step and the residual scale
+  std::ofstream st(base + "/message/step.txt");
+  st << s->step;
+  if (!st) return fail("cannot write " + base + "/message/step.txt");
+  if (s->res_scale_set) {
+    double sc[4];
+    HIPCHK(hipMemcpyAsync(sc, s->w.res_scale, sizeof(sc), hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    std::ofstream rs(base + "/message/residual_scale.txt");
+    rs.precision(17);
+    rs << sc[0] << '\n' << sc[1] << '\n' << sc[2] << '\n' << sc[3] << '\n';
+  }
+  return 0;
+}
+
+int ostc_read_flowfield(ostc_solver *s, const char *root) {
+  if (check_blk(s, 0)) return 1;
+  if (!root) return fail("null output directory");
+  HIPCHK(hipSetDevice(s->device));
+  const std::string base = std::string(root) + "/output";
+  PltFile f;
+  std::string err;
+  if (!plt_read(base + "/field/flowfield.plt", f, err)) return fail(err);
+  if (f.dims.size() != s->blocks.size())
+    return fail("the flow-field file has " + std::to_string(f.dims.size()) + " zones, the case " +
+                std::to_string(s->blocks.size()) + " blocks");
+  // Initialize.cuh:238-313 identify_variable_labels: 0-5 basic variables,
+  // 6 + l species l, 6 + n_spec + t turbulence variable t; -1 unused
+  const int ns = s->n_spec, nt = s->n_turb;
+  bool have_spec = false, have_turb = false;
+  std::vector<int> label;
+  for (auto &name : f.names) {
+    std::string n = name;
+    for (char &c : n) c = (char)std::toupper((unsigned char)c);
+    int l = -1;
+    if (n == "DENSITY" || n == "ROE" || n == "RHO") l = 0;
+    else if (n == "U") l = 1;
+    else if (n == "V") l = 2;
+    else if (n == "W") l = 3;
+    else if (n == "P" || n == "PRESSURE") l = 4;
+    else if (n == "T" || n == "TEMPERATURE") l = 5;
+    else {
+      for (int q = 0; q < ns; ++q) {
+        std::string sp = s->mech.names[q];
+        for (char &c : sp) c = (char)std::toupper((unsigned char)c);
+        if (n == sp) {
+          l = 6 + q;
+          have_spec = true;
+          break;
+        }
+      }
+      if (nt == 2 && (n == "K" || n == "TKE")) {
+        l = 6 + ns;
+        have_turb = true;
+      } else if (nt == 2 && n == "OMEGA") {
+        l = 6 + ns + 1;
+        have_turb = true;
+      }
+    }
+    label.push_back(l);
+  }
+  Inflow in;
+  if ((ns && !have_spec) || (nt && !have_turb)) {
+    if (make_inflow(s, s->prm.get_string("default_init", s->prm.get_string("reference_state", "")), in)) return 1;
+  }
+  for (size_t bi = 0; bi < s->blocks.size(); ++bi) {
+    Block &b = s->blocks[bi];
+    const auto &d = f.dims[bi];
+    // a file written with ngg_file ghost layers (FieldIO.h ngg_output)
+    const int gf = (d[0] - b.mx) / 2;
+    if (gf < 0 || gf > s->ngg || d[0] != b.mx + 2 * gf || d[1] != b.my + 2 * gf ||
+        (b.mz > 1 ? d[2] != b.mz + 2 * gf : (d[2] != 1 && d[2] != 1 + 2 * gf)))
+      return fail("zone " + std::to_string(bi) + " of the flow-field file does not fit block " + std::to_string(bi));
+    const int gk = d[2] == b.mz + 2 * gf ? gf : 0;
+    std::vector<double> bv((size_t)b.z.sz * 6), sv((size_t)b.z.sz * std::max(s->n_scalar, 1));
+    HIPCHK(hipMemcpyAsync(bv.data(), b.z.bv, bv.size() * sizeof(double), hipMemcpyDeviceToHost, s->stream));
+    if (s->n_scalar > 0)
+      HIPCHK(hipMemcpyAsync(sv.data(), b.z.sv, sv.size() * sizeof(double), hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    for (size_t v = 0; v < label.size(); ++v) {
+      const int l = label[v];
+      if (l < 0) continue;
+      double *dst = l < 6 ? bv.data() + (size_t)l * b.z.sz : sv.data() + (size_t)(l - 6) * b.z.sz;
+      const std::vector<double> &src = f.data[bi][v];
+      size_t q = 0;
+      for (int k = -gk; k < b.mz + gk; ++k)
+        for (int j = -gf; j < b.my + gf; ++j)
+          for (int i = -gf; i < b.mx + gf; ++i) dst[b.z.at(i, j, k)] = src[q++];
+    }
+    // Initialize.cu:49-121: groups the file lacks take the default inflow state
+    for (int k = 0; k < b.mz; ++k)
+      for (int j = 0; j < b.my; ++j)
+        for (int i = 0; i < b.mx; ++i) {
+          const int64_t p = b.z.at(i, j, k);
+          if (ns && !have_spec)
+            for (int l = 0; l < ns; ++l) sv[(size_t)l * b.z.sz + p] = in.sv[l];
+          if (nt && !have_turb)
+            for (int l = 0; l < nt; ++l) sv[(size_t)(ns + l) * b.z.sz + p] = in.sv[ns + l];
+        }
+    HIPCHK(hipMemcpyAsync(b.z.bv, bv.data(), bv.size() * sizeof(double), hipMemcpyHostToDevice, s->stream));
+    if (s->n_scalar > 0)
+      HIPCHK(hipMemcpyAsync(b.z.sv, sv.data(), sv.size() * sizeof(double), hipMemcpyHostToDevice, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+  }
+  // Initialize.cuh:234-237 step, Driver.cu:21-27 residual scale
+  int step = 0;
+  std::ifstream st(base + "/message/step.txt");
+  if (st) st >> step;
+  std::ifstream rs(base + "/message/residual_scale.txt");
+  if (rs) {
+    double sc[4] = {1e-20, 1e-20, 1e-20, 1e-20};
+    rs >> sc[0] >> sc[1] >> sc[2] >> sc[3];
+    HIPCHK(hipMemcpyAsync(s->w.res_scale, sc, sizeof(sc), hipMemcpyHostToDevice, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    s->res_scale_set = true;
+  }
+  s->restart = true;
+  s->restart_step = step;
+  s->restart_time = f.time;
+  return 0;
+}
+
 int ostc_set_chem_split(ostc_solver *s, int on) {
   if (!s) return fail("null solver");
   if (on && !(s->P.mix && s->P.reaction == 1)) return fail("operator-split chemistry needs a reacting mixture");
@@ -1781,14 +2028,9 @@ int ostc_set_chem_split(ostc_solver *s, int on) {
   return 0;
 }
 
-int ostc_chem_step(ostc_solver *s, int blk, double dt, int mode, int64_t *n_fallback) {
-  if (check_blk(s, blk)) return 1;
-  if (s->n_spec != 9 || s->mech.n_reac <= 0) return fail("the chemistry step needs the 9-species reacting mechanism");
-  if (!(dt > 0)) return fail("dt must be positive");
-  if (mode != 0 && mode != 1) return fail("mode must be 0 (point-implicit) or 1 (surrogate)");
-  if (mode == 1 && (!s->sur || !surrogate_ready(s->sur))) return fail("no trained surrogate: train or load one first");
-  HIPCHK(hipSetDevice(s->device));
-  Block &b = s->blocks[blk];
+// one chemistry step dt of every interior cell of a block (gather, FP64
+// integrator or surrogate + fallback, scatter); *nf: fallback cells
+int chem_interior(ostc_solver *s, Block &b, double dt, int mode, int64_t *nf_out) {
   const int64_t n = (int64_t)b.mx * b.my * b.mz;
   if (n > s->chem_cap) {
     if (s->chem_buf) HIPCHK(hipFree(s->chem_buf));
@@ -1810,6 +2052,20 @@ int ostc_chem_step(ostc_solver *s, int blk, double dt, int mode, int64_t *n_fall
     }
     HIPCHK(launch_chem_scatter(b.z, s->P, s->d_mech, dY, dT, s->stream));
   }
+  *nf_out = nf;
+  return 0;
+}
+
+int ostc_chem_step(ostc_solver *s, int blk, double dt, int mode, int64_t *n_fallback) {
+  if (check_blk(s, blk)) return 1;
+  if (s->n_spec != 9 || s->mech.n_reac <= 0) return fail("the chemistry step needs the 9-species reacting mechanism");
+  if (!(dt > 0)) return fail("dt must be positive");
+  if (mode != 0 && mode != 1) return fail("mode must be 0 (point-implicit) or 1 (surrogate)");
+  if (mode == 1 && (!s->sur || !surrogate_ready(s->sur))) return fail("no trained surrogate: train or load one first");
+  HIPCHK(hipSetDevice(s->device));
+  Block &b = s->blocks[blk];
+  int64_t nf = 0;
+  if (chem_interior(s, b, dt, mode, &nf)) return 1;
   if (run_bcs(s, b, true)) return 1;
   if (s->P.dim == 2) HIPCHK(launch_eliminate_k(b.z, s->P, true, s->stream));
   if (properties_block(s, b, true)) return 1;
@@ -1964,6 +2220,14 @@ int ostc_surrogate_last_stats(ostc_solver *s, double *out) {
   if (need_sur(s)) return 1;
   if (!out) return fail("null argument");
   return surrogate_last_stats(s->sur, out);
+}
+
+int ostc_surrogate_gate_reasons(ostc_solver *s, int64_t *out) {
+  if (need_sur(s)) return 1;
+  if (!out) return fail("null argument");
+  std::string err;
+  if (surrogate_gate_reasons(s->sur, out, s->stream, err)) return fail(err);
+  return 0;
 }
 
 int ostc_dev_alloc(ostc_solver *s, int64_t bytes, void **dptr) {

@@ -45,6 +45,9 @@ struct InnerFaceDesc {
 // one direction (0 = i, 1 = j, 2 = k); init: the sweep writes dq instead of accumulating
 hipError_t launch_flux_sweep(const ZoneView &z, const DParams &P, const DMech *m, int dir, bool init, bool fp32,
                              hipStream_t s);
+// the same for the k-omega SST cases (n_turb == 2), flux_sst.hip
+hipError_t launch_flux_sweep_sst(const ZoneView &z, const DParams &P, const DMech *m, int dir, bool init, bool fp32,
+                                 hipStream_t s);
 // finite-rate source; accumulate: add to dq (already initialised by launch_sst)
 hipError_t launch_source(const ZoneView &z, const DParams &P, const DMech *m, bool accumulate, hipStream_t s);
 hipError_t launch_local_dt(const ZoneView &z, const DParams &P, const StepScratch &w, double *dt_out,

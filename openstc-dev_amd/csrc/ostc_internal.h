@@ -3,6 +3,8 @@
 // kernels.  Nothing here crosses the C ABI.
 #pragma once
 
+#include <algorithm>
+#include <array>
 #include <cstdint>
 #include <map>
 #include <string>
@@ -121,6 +123,23 @@ bool register_inner_face(const int *s7, const int *t7, int ngg, int dim, InnerFa
 // (B[3], N[3]: x, y, z arrays on the layouts LB, LN).
 void inner_ghost_nodes(const InnerFace &f, const Layout &LB, double *const *B, const Layout &LN,
                        const double *const *N);
+
+// ------------------------------------------------------------ field files
+// Tecplot binary (V112) with ordered zones and nodal double data, the
+// reference's output and restart format (FieldIO.h, Initialize.cuh).
+struct PltZone {
+  int mx = 0, my = 0, mz = 0;
+  std::vector<const double *> vars;  // one array of mx*my*mz values (i fastest) per variable
+};
+struct PltFile {
+  std::vector<std::string> names;
+  std::vector<std::array<int, 3>> dims;
+  std::vector<std::vector<std::vector<double>>> data;  // zone -> variable -> values
+  double time = 0;                                     // solution time (of the last zone)
+};
+bool plt_write(const std::string &path, const std::vector<std::string> &names, const std::vector<PltZone> &zones,
+               double time, std::string &err);
+bool plt_read(const std::string &path, PltFile &out, std::string &err);
 
 // ------------------------------------------------------------ boundaries
 enum BcType { BC_WALL = 2, BC_SYMMETRY = 3, BC_FARFIELD = 4, BC_INFLOW = 5, BC_OUTFLOW = 6, BC_SUBSONIC = 7,

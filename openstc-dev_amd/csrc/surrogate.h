@@ -89,8 +89,14 @@ int surrogate_network_outputs(SurrogateState *s, const DMech *m, int64_t n, cons
                               const double *Y, float *host_out, hipStream_t st, std::string &err);
 // fallback count, network ms (features + MLP), gate ms and fallback ms of the last inference
 int surrogate_last_stats(const SurrogateState *s, double *out4);
+// fallback count and the cells each gate condition sent to the fallback in
+// the last inference: 1 input range, 2 non-finite output, 3 |dY| > dy_max,
+// 4 |dT| > dT_max, 5 sigma_min < sig_min, 6 conservation, 7 temperature
+// inversion, 8 dt other than the training dt (out9[0] the total)
+int surrogate_gate_reasons(SurrogateState *s, int64_t *out9, hipStream_t st, std::string &err);
 int surrogate_infer(SurrogateState *s, const DMech *m, int64_t n, const double *rho, const double *T,
                     const double *Y, double dt, double *dY, double *dT, int64_t *n_fallback, hipStream_t stream,
                     std::string &err);
+
 
 }  // namespace ostc

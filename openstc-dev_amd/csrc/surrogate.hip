@@ -189,12 +189,17 @@ __global__ void k_sur_features(const DMech *__restrict__ m, int64_t n, int64_t c
 // T is the Newton temperature at the unchanged internal energy started
 // from T + dT_net, as in the point-implicit batch kernel.
 // Decode one cell's network output o[0..6], apply the gate and, if the cell
-// is accepted, write (dY, dT); returns false when the cell must fall back.
-__device__ inline bool finish_cell(const DMech *__restrict__ m, const SurNorm &nm, int64_t n, int64_t c,
+// is accepted, write (dY, dT).  Returns 0 for an accepted cell, else the
+// first gate condition that failed: 1 input outside the training range,
+// 2 non-finite output, 3 |dY_k| above dy_max, 4 |dT| above dT_max,
+// 5 sigma_min below sig_min, 6 conservation error above cons_tol,
+// 7 temperature inversion failed.
+__device__ inline int finish_cell(const DMech *__restrict__ m, const SurNorm &nm, int64_t n, int64_t c,
                                    const double *__restrict__ rho_a, const double *__restrict__ T_a,
                                    const double *__restrict__ Y_a, const float *o, double *__restrict__ dY,
                                    double *__restrict__ dT) {
   bool ok = true;
+  int code = 0;
   const double rho = rho_a[c], T = T_a[c];
   double Y[kNS];
   load_cell(n, c, Y_a, Y);
@@ -211,6 +216,7 @@ __device__ inline bool finish_cell(const DMech *__restrict__ m, const SurNorm &n
       ok = ok && y >= nm.y_lo[l] && y <= nm.y_hi[l];
     }
     ok = ok && T >= nm.x_lo[9] && T <= nm.x_hi[9] && p >= nm.p_lo && p <= nm.p_hi;
+    if (!ok) code = 1;
   }
   double cj[kSurOut];
 #pragma unroll
@@ -232,8 +238,11 @@ __device__ inline bool finish_cell(const DMech *__restrict__ m, const SurNorm &n
     amax = fmax(amax, fabs(s));
   }
   const double dTn = cj[6];
-  ok = ok && isfinite(amax) && isfinite(dTn) && amax <= nm.dy_max && fabs(dTn) <= nm.dT_max;
-  ok = ok && cj[7] >= nm.sig_min;  // false for NaN as well
+  if (code == 0 && !(isfinite(amax) && isfinite(dTn))) code = 2;
+  if (code == 0 && !(amax <= nm.dy_max)) code = 3;
+  if (code == 0 && !(fabs(dTn) <= nm.dT_max)) code = 4;
+  if (code == 0 && !(cj[7] >= nm.sig_min)) code = 5;  // NaN fails as well
+  ok = code == 0;
   double err = 0, neg = 0;
 #pragma unroll
   for (int r = 0; r < 4; ++r) {
@@ -244,7 +253,8 @@ __device__ inline bool finish_cell(const DMech *__restrict__ m, const SurNorm &n
   }
 #pragma unroll
   for (int k = 0; k < kNS; ++k) neg += fmax(0.0, -(Y[k] + dy[k])) + fmax(0.0, Y[k] + dy[k] - 1.0);
-  ok = ok && fmax(err, neg) <= nm.cons_tol;
+  if (code == 0 && !(fmax(err, neg) <= nm.cons_tol)) code = 6;
+  ok = code == 0;
   if (ok) {
     double mwi = 0, h[kNS];
 #pragma unroll
@@ -267,9 +277,22 @@ __device__ inline bool finish_cell(const DMech *__restrict__ m, const SurNorm &n
 #pragma unroll
       for (int l = 0; l < kNS; ++l) dY[(int64_t)l * n + c] = dy[l];
       dT[c] = Tn - T;
+    } else {
+      code = 7;
     }
   }
-  return ok;
+  return code;
+}
+
+// wave-aggregated counts of the gate codes (index 8: forced fallback, a
+// network trained for another dt); reasons may be null
+__device__ inline void count_reasons(int code, bool active, int *__restrict__ reasons) {
+  if (!reasons) return;
+#pragma unroll
+  for (int r = 1; r <= 8; ++r) {
+    const unsigned long long mask = __ballot(active && code == r);
+    if (mask && (threadIdx.x & 63) == __ffsll((long long)mask) - 1) atomicAdd(reasons + r, (int)__popcll(mask));
+  }
 }
 
 // wave-aggregated append of the cells with fb set to the fallback list
@@ -290,13 +313,15 @@ __global__ void __launch_bounds__(256) k_sur_finish(const DMech *__restrict__ m,
                                                     const double *__restrict__ T_a, const double *__restrict__ Y_a,
                                                     const float *__restrict__ out, int ld, int force_fallback,
                                                     double *__restrict__ dY, double *__restrict__ dT,
-                                                    int *__restrict__ fb_count, int *__restrict__ fb_idx) {
+                                                    int *__restrict__ fb_count, int *__restrict__ fb_idx,
+                                                    int *__restrict__ reasons) {
   const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
   const bool active = t < count;
   const int64_t c = c0 + t;
-  bool ok = false;
-  if (active && !force_fallback) ok = finish_cell(m, nm, n, c, rho_a, T_a, Y_a, out + t * ld, dY, dT);
-  append_fallback(active && !ok, c, fb_count, fb_idx);
+  int code = 8;
+  if (active && !force_fallback) code = finish_cell(m, nm, n, c, rho_a, T_a, Y_a, out + t * ld, dY, dT);
+  count_reasons(code, active, reasons);
+  append_fallback(active && code != 0, c, fb_count, fb_idx);
 }
 
 // ------------------------------------------------------------ fused network
@@ -640,6 +665,7 @@ struct SurrogateState {
   __bf16 *iX = nullptr, *iH[2] = {};
   float *iOut = nullptr;
   int *fb_idx = nullptr, *fb_count = nullptr;
+  int *reasons = nullptr;  // gate codes 1..8 of the last inference (16 ints)
   float *fb_out = nullptr;  // network outputs [fcap x 8] of the fused path
   int kernel = 3;  // inference: 3 fused network, 128 cells per workgroup, halves pipelined (default);
                    // 1 the same unpipelined; 2 fused, 64 cells; 0 layer-by-layer GEMMs
@@ -670,7 +696,7 @@ struct SurrogateState {
     free_train();
     free_infer();
     for (void *p : {(void *)w, (void *)g, (void *)m1, (void *)m2, (void *)wb, (void *)aug, (void *)loss,
-                    (void *)fb_idx, (void *)fb_count, (void *)fb_out})
+                    (void *)fb_idx, (void *)fb_count, (void *)fb_out, (void *)reasons})
       (void)hipFree(p);
     for (auto &e : ev)
       if (e) (void)hipEventDestroy(e);
@@ -922,6 +948,7 @@ int surrogate_create(const Mechanism &mech, int device, SurrogateState **out, st
   SCHK(hipMemcpy(s->aug, aug, sizeof(aug), hipMemcpyHostToDevice));
   SCHK(dalloc(&s->loss, 1));
   SCHK(dalloc(&s->fb_count, 1));
+  SCHK(dalloc(&s->reasons, 16));
   for (auto &e : s->ev) SCHK(hipEventCreate(&e));
   *out = s.release();
   return 0;
@@ -1261,11 +1288,12 @@ int surrogate_infer(SurrogateState *s, const DMech *m, int64_t n, const double *
   if (ensure_infer_ws(s, s->kernel == 0 ? (chunk + 7) / 8 * 8 : 0, n, st, err)) return 1;
   SCHK(hipEventRecord(s->ev[0], st));
   SCHK(hipMemsetAsync(s->fb_count, 0, sizeof(int), st));
+  SCHK(hipMemsetAsync(s->reasons, 0, 16 * sizeof(int), st));
   if (s->kernel != 0) {
     if (!force && launch_net(s, m, n, rho, T, Y, s->fb_out, st, err)) return 1;
     SCHK(hipEventRecord(s->ev[3], st));
     hipLaunchKernelGGL(k_sur_finish, dim3(nb256(n)), dim3(256), 0, st, m, s->norm, n, (int64_t)0, n, rho, T, Y,
-                       (const float *)s->fb_out, kSurOut, force, dY, dT, s->fb_count, s->fb_idx);
+                       (const float *)s->fb_out, kSurOut, force, dY, dT, s->fb_count, s->fb_idx, s->reasons);
     SCHK(hipGetLastError());
   }
   for (int64_t c0 = 0; c0 < n && s->kernel == 0; c0 += chunk) {
@@ -1278,7 +1306,7 @@ int surrogate_infer(SurrogateState *s, const DMech *m, int64_t n, const double *
       if (forward(s, cnt, s->iX, H, nullptr, s->iOut, true, st, err)) return 1;
     }
     hipLaunchKernelGGL(k_sur_finish, dim3(nb256(cnt)), dim3(256), 0, st, m, s->norm, n, c0, cnt, rho, T, Y,
-                       (const float *)s->iOut, kYld, force, dY, dT, s->fb_count, s->fb_idx);
+                       (const float *)s->iOut, kYld, force, dY, dT, s->fb_count, s->fb_idx, s->reasons);
     SCHK(hipGetLastError());
   }
   SCHK(hipEventRecord(s->ev[1], st));
@@ -1397,6 +1425,15 @@ int surrogate_load(SurrogateState *s, const std::string &path, hipStream_t st, s
   s->dt_train = dt;
   finish_norm(s);
   s->ready = true;
+  return 0;
+}
+
+int surrogate_gate_reasons(SurrogateState *s, int64_t *out9, hipStream_t st, std::string &err) {
+  int h[16] = {};
+  SCHK(hipMemcpyAsync(h, s->reasons, sizeof(h), hipMemcpyDeviceToHost, st));
+  SCHK(hipStreamSynchronize(st));
+  out9[0] = s->last_fallback;
+  for (int r = 1; r <= 8; ++r) out9[r] = h[r];
   return 0;
 }
 

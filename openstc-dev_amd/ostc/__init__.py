@@ -80,6 +80,11 @@ def lib():
             "ostc_chem_batch": ([vp, C.c_int64, vp, vp, vp, C.c_double, vp, vp, C.c_int,
                                  C.POINTER(C.c_int64)], C.c_int),
             "ostc_set_chem_split": ([vp, C.c_int], C.c_int),
+            "ostc_write_flowfield": ([vp, C.c_char_p], C.c_int),
+            "ostc_read_flowfield": ([vp, C.c_char_p], C.c_int),
+            "ostc_set_steady_chem": ([vp, C.c_double, C.c_int], C.c_int),
+            "ostc_chem_states": ([vp, C.c_int, vp, vp, vp], C.c_int),
+            "ostc_steady_chem_stats": ([vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64)], C.c_int),
             "ostc_read_grid": ([vp], C.c_int),
             "ostc_add_inner_face": ([vp, C.c_int, ip, ip], C.c_int),
             "ostc_read_inner_file": ([vp], C.c_int),
@@ -110,6 +115,7 @@ def lib():
             "ostc_surrogate_save": ([vp, C.c_char_p], C.c_int),
             "ostc_surrogate_load": ([vp, C.c_char_p], C.c_int),
             "ostc_surrogate_last_stats": ([vp, dp], C.c_int),
+            "ostc_surrogate_gate_reasons": ([vp, C.POINTER(C.c_int64)], C.c_int),
             "ostc_surrogate_set_kernel": ([vp, C.c_int], C.c_int),
             "ostc_surrogate_network_outputs": ([vp, C.c_int64, vp, vp, vp, fp], C.c_int),
         }
@@ -363,6 +369,27 @@ class Solver:
     def set_chem_split(self, on):
         _chk(lib().ostc_set_chem_split(self.h, 1 if on else 0))
 
+    def set_steady_chem(self, dt, mode=0):
+        """Inline chemistry on the steady path (dt <= 0: off)."""
+        _chk(lib().ostc_set_steady_chem(self.h, dt, mode))
+
+    def chem_states(self, rho, T, Y, blk=0):
+        """Interior rho, T, Y of a block into DeviceArrays."""
+        _chk(lib().ostc_chem_states(self.h, blk, rho.ptr, T.ptr, Y.ptr))
+
+    def steady_chem_stats(self):
+        c, f = C.c_int64(), C.c_int64()
+        _chk(lib().ostc_steady_chem_stats(self.h, C.byref(c), C.byref(f)))
+        return c.value, f.value
+
+    def write_flowfield(self, root):
+        """<root>/output/field/flowfield.plt and message/step.txt (+ residual_scale.txt)."""
+        _chk(lib().ostc_write_flowfield(self.h, str(root).encode()))
+
+    def read_flowfield(self, root):
+        """Restart from <root>/output; call initialize() afterwards."""
+        _chk(lib().ostc_read_flowfield(self.h, str(root).encode()))
+
     def chem_step(self, dt, mode=0, blk=0):
         """One operator-split chemistry step on a block; returns the fallback count."""
         nf = C.c_int64()
@@ -455,6 +482,15 @@ class Solver:
         o = np.zeros(4)
         _chk(lib().ostc_surrogate_last_stats(self.h, _dp(o)))
         return dict(n_fallback=int(o[0]), net_ms=o[1], gate_ms=o[2], fallback_ms=o[3])
+
+    def surrogate_gate_reasons(self):
+        """Fallback count of the last inference and the cells per first failing
+        gate condition: dict(total, range, nonfinite, dy, dT, sigma, conservation,
+        newton, dt)."""
+        o = (C.c_int64 * 9)()
+        _chk(lib().ostc_surrogate_gate_reasons(self.h, o))
+        keys = ("total", "range", "nonfinite", "dy", "dT", "sigma", "conservation", "newton", "dt")
+        return dict(zip(keys, (int(v) for v in o)))
 
     def profile(self, on):
         _chk(lib().ostc_profile(self.h, 1 if on else 0))

@@ -2315,7 +2315,9 @@ static void epi_in_place(orc_zone *zz, int i, int j, int k, double diag, double 
   for (int l = 0; l < ns; ++l) DQ(5 + l, i, j, k) = rhs[l];
 }
 
-static int is_fr(const orc_zone *zz) { return zz->cfg.mix && zz->cfg.reaction == 1; }
+/* finite-rate chemistry inside the implicit treatment: off when the
+ * chemistry is operator-split (the right-hand side then has no source) */
+static int is_fr(const orc_zone *zz) { return zz->cfg.mix && zz->cfg.reaction == 1 && !zz->cfg.split_chem; }
 
 /* DPLUR.cuh:14-78 compute_DQ_0 */
 static void dplur_dq0(orc_zone *zz, double *dq0) {
@@ -2555,7 +2557,10 @@ static void update_bv(orc_zone *zz) {
       }
 }
 
-void orc_zone_chem_step(orc_zone *zz, double dt) {
+/* one point-implicit step dt of every interior cell at its density and
+ * total energy: gather, orc_chem_batch, scatter (Y, T, p and the rho Y rows
+ * of cv) */
+static void chem_split_interior(orc_zone *zz, double dt) {
   const int ns = zz->ns;
   const int64_t n = (int64_t)zz->mx * zz->my * zz->mz;
   double *rho = malloc(sizeof(double) * n * 21 + 8);
@@ -2584,6 +2589,10 @@ void orc_zone_chem_step(orc_zone *zz, double dt) {
         BV(4, i, j, k) = BV(0, i, j, k) * BV(5, i, j, k) * R_U * mwi;
       }
   free(rho);
+}
+
+void orc_zone_chem_step(orc_zone *zz, double dt) {
+  chem_split_interior(zz, dt);
   apply_bcs(zz, 1);
   if (zz->dim == 2) eliminate_k_gradient(zz, 1);
   update_properties(zz);
@@ -2600,6 +2609,7 @@ void orc_zone_run_steady(orc_zone *zz, int nsteps, double *res_hist) {
     (void)orc_zone_global_dt(zz);
     orc_zone_implicit(zz);
     update_bv(zz);
+    if (zz->cfg.split_chem && zz->cfg.steady_chem_dt > 0) chem_split_interior(zz, zz->cfg.steady_chem_dt);
     /* ghost cv follows for the AWENO path, which reads it */
     apply_bcs(zz, zz->inviscid_type == 1);
     if (zz->dim == 2) eliminate_k_gradient(zz, zz->inviscid_type == 1);
@@ -3175,6 +3185,7 @@ void orc_mesh_run_steady(orc_mesh *M, int nsteps, double *res_hist) {
       (void)orc_zone_global_dt(zz);
       orc_zone_implicit(zz);
       update_bv(zz);
+      if (zz->cfg.split_chem && zz->cfg.steady_chem_dt > 0) chem_split_interior(zz, zz->cfg.steady_chem_dt);
       apply_bcs(zz, with_cv);
     }
     inner_exchange(M, with_cv);

@@ -16,13 +16,12 @@
  *   disp2 = mx+2g, disp1 = (my+2g)*disp2, sz = (mx+2g)(my+2g)(mz+2g).
  *
  * Parity pins (how each part is checked without trusting this file) are
- * listed in tests/test_oracle_*.py; routines no independent check reaches
- * are marked "parity unpinned" in their comment.  Terms that no check reaches
- * inside otherwise pinned routines: the mixture heat-conduction and
- * species-diffusion terms of the viscous flux (the shear-stress and u . tau
- * work terms are pinned by plane Couette shear), and in the SST source the
- * cross-diffusion term, F1 near a wall and the production limiter's active
- * branch (decay, production and the far-field constants are pinned).
+ * listed in tests/test_oracle_*.py and tests/test_multiblock.py: closed
+ * forms, published formulas, conservation and consistency properties,
+ * finite-difference Jacobians, convergence orders and, for the multi-block
+ * mesh, cut cases that must reproduce their single-block runs.  A routine
+ * that no independent check reaches is marked "parity unpinned" in its
+ * comment.
  */
 #ifndef OSTC_ORACLE_H
 #define OSTC_ORACLE_H
@@ -122,6 +121,9 @@ typedef struct {
   double cfl, Pr, Sc, Prt, Sct, dt, entropy_fix_factor;
   int n_bc;
   orc_bc bc[ORC_MAXBC];
+  /* steady path with split_chem: one point-implicit chemistry step of this
+   * dt per pseudo-iteration, after the flow update (0: none) */
+  double steady_chem_dt;
 } orc_cfg;
 
 typedef struct orc_zone orc_zone;
