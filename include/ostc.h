@@ -236,6 +236,17 @@ int ostc_set_flux_precision(ostc_solver *s, int fp32);
  * (FiniteRateChem.cu:7-63, 234-256). */
 int ostc_chem_batch(ostc_solver *s, int64_t n, const double *rho, const double *T, const double *Y, double dt,
                     double *dY, double *dT, int mode, int64_t *n_fallback);
+/* Allocation-free variant of ostc_chem_batch, for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  No counterpart in the reference.  Same
+ * arguments, kernels and results as ostc_chem_batch; mode 1 needs a fused
+ * inference kernel (the default) and keeps the network outputs, fallback list
+ * and fallback counter in one fixed device array per GPU (36 MiB, held until
+ * the process ends), so n is limited to 1,048,576 cells (larger n is an
+ * error).  Calls on one GPU share that array: calls that overlap on one GPU
+ * (several solvers or threads) race.  Allocates nothing. */
+int ostc_chem_batch_noalloc(ostc_solver *s, int64_t n, const double *rho, const double *T, const double *Y, double dt,
+                            double *dY, double *dT, int mode, int64_t *n_fallback);
 /* Output and restart files in the reference's format and places, under
  * <root>/output: field/flowfield.plt (Tecplot binary V112, one ordered zone
  * per block with the interior points; variables x, y, z, density, u, v, w,
@@ -344,7 +355,8 @@ int ostc_surrogate_last_stats(ostc_solver *s, double *out);
  * out[r] the cells whose first failing gate condition was r: 1 input outside
  * the training range, 2 non-finite output, 3 |dY_k| > dy_max, 4 |dT| >
  * dT_max, 5 predicted sigma_min(I - dt J) < sig_min, 6 conservation error >
- * cons_tol, 7 temperature inversion failed, 8 dt other than the training dt. */
+ * cons_tol, 7 temperature inversion failed, 8 dt other than the training dt
+ * (not counted by ostc_chem_batch_noalloc). */
 int ostc_surrogate_gate_reasons(ostc_solver *s, int64_t *out);
 
 /* Device memory helpers for callers without their own allocator. */

@@ -827,6 +827,11 @@ int ostc_init(const char *case_dir, int device, ostc_solver **out) {
   if (device < 0 || device >= ndev) return fail("HIP device " + std::to_string(device) + " not available");
   HIPCHK(hipSetDevice(device));
   HIPCHK(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+  {
+    // device address of the fixed scratch of ostc_chem_batch_noalloc, once per device
+    std::string err;
+    if (noalloc_scratch_init(device, err)) return fail(err);
+  }
   if (upload_mech(s.get())) return 1;
   *out = s.release();
   return 0;
@@ -1797,6 +1802,32 @@ int ostc_chem_batch(ostc_solver *s, int64_t n, const double *rho, const double *
   }
   HIPCHK(hipStreamSynchronize(s->stream));
   s->drain_profile();
+  return 0;
+}
+
+int ostc_chem_batch_noalloc(ostc_solver *s, int64_t n, const double *rho, const double *T, const double *Y,
+                            double dt, double *dY, double *dT, int mode, int64_t *n_fallback) {
+  if (!s) return fail("null solver");
+  if (s->device < 0) return fail("solver has no device");
+  if (s->n_spec != 9 || s->mech.n_reac <= 0) return fail("chemistry batch needs the 9-species reacting mechanism");
+  if (n < 0) return fail("negative cell count");
+  if (n > kNoAllocCells)
+    return fail("ostc_chem_batch_noalloc takes at most " + std::to_string(kNoAllocCells) + " cells, got " +
+                std::to_string(n));
+  HIPCHK(hipSetDevice(s->device));
+  if (mode == 0) {
+    HIPCHK(launch_chem_batch(s->d_mech, s->mech.n_reac, n, rho, T, Y, dt, dY, dT, s->stream));
+    if (n_fallback) *n_fallback = 0;
+  } else if (mode == 1) {
+    if (!s->sur || !surrogate_ready(s->sur)) return fail("no trained surrogate: train or load one first");
+    int64_t nf = 0;
+    std::string err;
+    if (surrogate_infer_noalloc(s->sur, s->d_mech, n, rho, T, Y, dt, dY, dT, &nf, s->stream, err)) return fail(err);
+    if (n_fallback) *n_fallback = nf;
+  } else {
+    return fail("mode must be 0 (point-implicit) or 1 (surrogate)");
+  }
+  HIPCHK(hipStreamSynchronize(s->stream));
   return 0;
 }
 

@@ -98,5 +98,15 @@ int surrogate_infer(SurrogateState *s, const DMech *m, int64_t n, const double *
                     const double *Y, double dt, double *dY, double *dT, int64_t *n_fallback, hipStream_t stream,
                     std::string &err);
 
+// Allocation-free inference: the same kernels as surrogate_infer with a fused
+// network kernel, on a fixed per-device scratch array (network outputs,
+// fallback list and counter) of at most kNoAllocCells cells.
+// noalloc_scratch_init takes that array's device address once per device
+// (the device must be current); calls that overlap on one GPU share it and race.
+constexpr int64_t kNoAllocCells = 1 << 20;
+int noalloc_scratch_init(int device, std::string &err);
+int surrogate_infer_noalloc(SurrogateState *s, const DMech *m, int64_t n, const double *rho, const double *T,
+                            const double *Y, double dt, double *dY, double *dT, int64_t *n_fallback,
+                            hipStream_t stream, std::string &err);
 
 }  // namespace ostc

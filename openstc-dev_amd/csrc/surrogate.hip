@@ -11,6 +11,7 @@
 #include <cstring>
 #include <fstream>
 #include <memory>
+#include <mutex>
 #include <numeric>
 #include <vector>
 
@@ -1327,6 +1328,118 @@ int surrogate_infer(SurrogateState *s, const DMech *m, int64_t n, const double *
   } else {
     SCHK(hipEventElapsedTime(&a, s->ev[0], s->ev[1]));
   }
+  SCHK(hipEventElapsedTime(&b, s->ev[1], s->ev[2]));
+  s->last_ms[0] = a;
+  s->last_ms[1] = c;
+  s->last_ms[2] = b;
+  s->last_fallback = nf;
+  if (n_fallback) *n_fallback = nf;
+  return 0;
+}
+
+// ------------------------------------------------- allocation-free inference
+// Fixed device scratch of surrogate_infer_noalloc: the network outputs
+// (kSurOut FP32 per cell), the fallback index list and the fallback counter
+// for batches of up to kNoAllocCells cells.  Each GPU the library runs on
+// holds its own copy until the process ends (36 MiB + 256 B).  Host code
+// never passes `scratch` itself to a kernel: its device address is taken
+// once per device with hipGetSymbolAddress (noalloc_scratch_init).
+namespace {
+constexpr int64_t kNoAllocOutBytes = kNoAllocCells * kSurOut * (int64_t)sizeof(float);
+constexpr int64_t kNoAllocIdxBytes = kNoAllocCells * (int64_t)sizeof(int);
+constexpr int64_t kNoAllocBytes = kNoAllocOutBytes + kNoAllocIdxBytes + 256;
+constexpr int kNoAllocMaxDevices = 64;
+
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+std::mutex g_scratch_mu;
+unsigned char *g_scratch_ptr[kNoAllocMaxDevices] = {};
+
+__global__ void k_zero_int(int *__restrict__ p) {
+  if (threadIdx.x == 0) p[0] = 0;
+}
+}  // namespace
+
+int noalloc_scratch_init(int device, std::string &err) {
+  if (device < 0 || device >= kNoAllocMaxDevices) {
+    err = "device index out of range for the fixed scratch";
+    return 1;
+  }
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch_ptr[device]) return 0;
+  int cur = -1;
+  SCHK(hipGetDevice(&cur));
+  if (cur != device) {
+    err = "fixed scratch set-up needs its device current";
+    return 1;
+  }
+  void *ptr = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !ptr) {
+    err = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
+    return 1;
+  }
+  g_scratch_ptr[device] = (unsigned char *)ptr;
+  return 0;
+}
+
+int surrogate_infer_noalloc(SurrogateState *s, const DMech *m, int64_t n, const double *rho, const double *T,
+                            const double *Y, double dt, double *dY, double *dT, int64_t *n_fallback, hipStream_t st,
+                            std::string &err) {
+  if (!surrogate_ready(s)) {
+    err = "surrogate not trained or loaded";
+    return 1;
+  }
+  if (n > kNoAllocCells) {
+    err = "the allocation-free surrogate call takes at most " + std::to_string(kNoAllocCells) + " cells, got " +
+          std::to_string(n);
+    return 1;
+  }
+  if (s->kernel == 0) {
+    err = "the allocation-free surrogate call needs a fused inference kernel (1, 2 or 3), not 0";
+    return 1;
+  }
+  unsigned char *base = nullptr;
+  {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (s->device >= 0 && s->device < kNoAllocMaxDevices) base = g_scratch_ptr[s->device];
+  }
+  if (!base) {
+    err = "fixed scratch not set up on this device";
+    return 1;
+  }
+  SCHK(hipSetDevice(s->device));
+  if (n <= 0) {
+    if (n_fallback) *n_fallback = 0;
+    return 0;
+  }
+  float *out = (float *)base;
+  int *idx = (int *)(base + kNoAllocOutBytes);
+  int *count = (int *)(base + kNoAllocOutBytes + kNoAllocIdxBytes);
+  const int force = std::fabs(dt - s->dt_train) > 1e-12 * s->dt_train ? 1 : 0;
+  SCHK(hipEventRecord(s->ev[0], st));
+  // the counter is cleared by a kernel, not a memset node
+  hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, st, count);
+  SCHK(hipGetLastError());
+  if (!force && launch_net(s, m, n, rho, T, Y, out, st, err)) return 1;
+  SCHK(hipEventRecord(s->ev[3], st));
+  hipLaunchKernelGGL(k_sur_finish, dim3(nb256(n)), dim3(256), 0, st, m, s->norm, n, (int64_t)0, n, rho, T, Y,
+                     (const float *)out, kSurOut, force, dY, dT, count, idx, (int *)nullptr);
+  SCHK(hipGetLastError());
+  SCHK(hipEventRecord(s->ev[1], st));
+  int nf = 0;
+  SCHK(hipMemcpyAsync(&nf, count, sizeof(int), hipMemcpyDeviceToHost, st));
+  SCHK(hipStreamSynchronize(st));
+  if (nf < 0 || nf > n) {
+    err = "fallback count out of range";
+    return 1;
+  }
+  SCHK(launch_chem_batch_subset(m, s->n_reac, n, nf, idx, rho, T, Y, dt, dY, dT, st));
+  SCHK(hipEventRecord(s->ev[2], st));
+  SCHK(hipEventSynchronize(s->ev[2]));
+  float a = 0, b = 0, c = 0;
+  SCHK(hipEventElapsedTime(&a, s->ev[0], s->ev[3]));
+  SCHK(hipEventElapsedTime(&c, s->ev[3], s->ev[1]));
   SCHK(hipEventElapsedTime(&b, s->ev[1], s->ev[2]));
   s->last_ms[0] = a;
   s->last_ms[1] = c;

@@ -79,6 +79,8 @@ def lib():
             "ostc_set_flux_precision": ([vp, C.c_int], C.c_int),
             "ostc_chem_batch": ([vp, C.c_int64, vp, vp, vp, C.c_double, vp, vp, C.c_int,
                                  C.POINTER(C.c_int64)], C.c_int),
+            "ostc_chem_batch_noalloc": ([vp, C.c_int64, vp, vp, vp, C.c_double, vp, vp, C.c_int,
+                                         C.POINTER(C.c_int64)], C.c_int),
             "ostc_set_chem_split": ([vp, C.c_int], C.c_int),
             "ostc_write_flowfield": ([vp, C.c_char_p], C.c_int),
             "ostc_read_flowfield": ([vp, C.c_char_p], C.c_int),
@@ -400,6 +402,13 @@ class Solver:
         """rho/T/Y/dY/dT: DeviceArray (Y, dY component-major n_spec x n)."""
         nf = C.c_int64()
         _chk(lib().ostc_chem_batch(self.h, n, rho.ptr, T.ptr, Y.ptr, dt, dY.ptr, dT.ptr, mode, C.byref(nf)))
+        return nf.value
+
+    def chem_batch_noalloc(self, n, rho, T, Y, dt, dY, dT, mode=0):
+        """As chem_batch, through the allocation-free entry point (n <= 2^20)."""
+        nf = C.c_int64()
+        _chk(lib().ostc_chem_batch_noalloc(self.h, n, rho.ptr, T.ptr, Y.ptr, dt, dY.ptr, dT.ptr, mode,
+                                           C.byref(nf)))
         return nf.value
 
     def sync(self):
