@@ -2,7 +2,7 @@
 Navier-Stokes air case (256x128x128 curvilinear, AWENO-5 + CDS, AUSM+,
 second-order viscous, CFL dt recomputed every stage, residual every step) on
 one MI355X, FP64.  Also reported: the same step with FP32 flux arithmetic;
-the 2-D reacting 9-species mixing layer (512x256, HLLC + MUSCL, viscous,
+the 2-D reacting 9-species mixing layer (512x256, HLLC + AWENO-5, viscous,
 finite-rate source) in FP64 and mixed precision; the chemistry batch (2^22
 cells, dt = 1e-7 s) with the FP64 point-implicit integrator and with the
 BF16-MFMA surrogate + FP64 fallback (trained first, untimed, on 2^24 pairs
@@ -394,8 +394,9 @@ def main():
             s3.set_flux_fp32(True)
             t32 = time_steps(s3, a.steps, a.warmup)
             s3.close()
-            c3 = {"workload": "2-D reacting H2/air mixing layer 512x256, 9 species / 19 reactions, HLLC + MUSCL, "
-                              "viscous, explicit finite-rate source, SSP-RK3",
+            c3 = {"workload": "2-D reacting H2/air mixing layer 512x256, 9 species / 19 reactions, HLLC + AWENO-5 "
+                              "(multi-species characteristic interpolation), viscous, explicit finite-rate source, "
+                              "SSP-RK3",
                   "fp64_mcell_per_s": c3_cells * a.steps / t64 / 1e6, "fp64_ms_per_step": t64 / a.steps * 1e3,
                   "mixed_mcell_per_s": c3_cells * a.steps / t32 / 1e6, "mixed_ms_per_step": t32 / a.steps * 1e3}
 
@@ -478,7 +479,7 @@ def main():
             s4.close()
             Tr, Ts = bv_ref[5], bv_s[5]
             split = {"workload": "2-D reacting H2/air mixing layer 512x256, operator-split step: SSP-RK3 flow step "
-                                 "(HLLC + MUSCL, viscous) + one point-implicit chemistry step, fixed dt",
+                                 "(HLLC + AWENO-5, viscous) + one point-implicit chemistry step, fixed dt",
                      "dt": a.split_dt,
                      "fp64_plain_ms_per_step": (t1 - t0) / a.steps * 1e3,
                      "mixed_surrogate_ms_per_step": (t3_ - t2_) / a.steps * 1e3,

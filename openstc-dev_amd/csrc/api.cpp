@@ -102,6 +102,7 @@ struct ostc_solver {
   int64_t steady_cells = 0, steady_fb = 0;  // inline chemistry cells and fallbacks since it was set
   bool res_scale_set = false;       // a step-1 residual has set the residual scale
   bool restart = false;             // fields, step and time came from ostc_read_flowfield
+  bool conservative_metric = false; // conservative symmetric metrics (uniform flow kept exactly)
   int restart_step = 0;
   double restart_time = 0;
   double *chem_buf = nullptr;       // gather / result buffers of ostc_chem_step (21 doubles per cell)
@@ -807,6 +808,10 @@ int ostc_init(const char *case_dir, int device, ostc_solver **out) {
   P.Sc = p.get_real("schmidt_number", 0.9);
   P.Prt = p.get_real("turbulent_prandtl_number", 0.9);
   P.Sct = p.get_real("turbulent_schmidt_number", 0.9);
+  // not a reference key: metrics in the conservative symmetric form, which
+  // keep a uniform flow exact on curved grids (the reference's are not)
+  s->conservative_metric = p.get_bool("conservative_metric", false);
+  P.jface = s->conservative_metric ? 1 : 0;
   // AWENO.cu:86: the characteristic interpolation reconstructs only for
   // reconstruction 4 (AWENO-5); reconstruction 5 (WENO7) selects the same
   // kernel in the reference but is not that scheme
@@ -909,7 +914,8 @@ int ostc_add_block(ostc_solver *s, int mx, int my, int mz, const double *x, cons
   b.my = my;
   b.mz = mz;
   std::string err;
-  if (!compute_metrics(mx, my, mz, s->ngg, x, y, z, b.jac, b.metric, err, s->n_turb ? &b.xyz : nullptr))
+  if (!compute_metrics(mx, my, mz, s->ngg, x, y, z, b.jac, b.metric, err, s->n_turb ? &b.xyz : nullptr,
+                       s->conservative_metric, s->P.inviscid_type == 1))
     return fail(err);
   const size_t n = (size_t)mx * my * mz;
   b.x0.assign(x, x + n);
@@ -1158,11 +1164,12 @@ int ostc_finalize(ostc_solver *s) {
   if (build_nodes(s, nodes)) return 1;
   int *d_bad_interior = nullptr;
   unsigned char *d_bad = nullptr;
-  double *d_nodes = nullptr;
+  double *d_nodes = nullptr, *d_fsp = nullptr;
   auto free_geo = [&]() {
     (void)hipFree(d_bad_interior);
     (void)hipFree(d_bad);
     (void)hipFree(d_nodes);
+    (void)hipFree(d_fsp);
   };
   for (auto &b : s->blocks) {
     ZoneView &z = b.z;
@@ -1218,19 +1225,22 @@ int ostc_finalize(ostc_solver *s) {
       hipError_t e = hipMalloc(&d_nodes, sizeof(double) * 3 * Lc.sz);
       if (e == hipSuccess) e = hipMalloc(&d_bad, L.sz);
       if (e == hipSuccess) e = hipMalloc(&d_bad_interior, sizeof(int));
+      if (e == hipSuccess && s->conservative_metric) e = hipMalloc(&d_fsp, sizeof(double) * 9 * Lc.sz);
       if (e == hipSuccess) e = hipMemsetAsync(d_bad_interior, 0, sizeof(int), s->stream);
       for (int c = 0; c < 3 && e == hipSuccess; ++c)
         e = hipMemcpyAsync(d_nodes + c * Lc.sz, nodes[bi][c].data(), sizeof(double) * Lc.sz, hipMemcpyHostToDevice,
                            s->stream);
       if (e == hipSuccess)
         e = launch_metrics(b.mx, b.my, b.mz, g, d_nodes, d_nodes + Lc.sz, d_nodes + 2 * Lc.sz, jac, met, d_bad,
-                           d_bad_interior, s->stream);
+                           d_bad_interior, s->stream, s->conservative_metric ? 1 : 0,
+                           s->P.inviscid_type == 1 ? 1 : 0, d_fsp);
       int bad_interior = 0;
       if (e == hipSuccess)
         e = hipMemcpyAsync(&bad_interior, d_bad_interior, sizeof(int), hipMemcpyDeviceToHost, s->stream);
       if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
       free_geo();
       d_nodes = nullptr;
+      d_fsp = nullptr;
       d_bad = nullptr;
       d_bad_interior = nullptr;
       if (e != hipSuccess) return fail(std::string("device metrics: ") + hipGetErrorString(e));

@@ -40,6 +40,7 @@ struct DParams {
   double efix;  // Roe entropy-fix factor
   int turb_implicit, compress;  // SST: implicit source diagonal; compressibility correction 0-3
   int nr;                       // reactions of the mechanism (0 without chemistry)
+  int jface;                    // HLLC / LF face vector (J0 m0 + J1 m1) / 2 (conservative metrics)
 };
 
 // (a, b, c) of linear index t over an a-fastest na x nb x nc box, in 32-bit

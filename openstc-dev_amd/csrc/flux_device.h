@@ -607,10 +607,29 @@ __device__ inline void ausm_plus(const R *pl, const R *pr, const R *mk0, const R
   }
 }
 
+// Face metric and Jacobian of HLLC and LF: the reference's (m0 + m1) / 2 and
+// (J0 + J1) / 2 (HLLC.cu:64-71, LaxFriedrichs.cu:22-35); with conservative
+// metrics (jface) the face vector (J0 m0 + J1 m1) / 2 and the factor 1, as
+// AUSM+ and Roe take it, so that the face vectors of a cell close.
+template <typename R>
+__device__ inline void face_vector(const R *mk0, const R *mk1, R j0, R j1, bool jface, R &kx, R &ky, R &kz, R &ja) {
+  if (jface) {
+    kx = R(0.5) * (j0 * mk0[0] + j1 * mk1[0]);
+    ky = R(0.5) * (j0 * mk0[1] + j1 * mk1[1]);
+    kz = R(0.5) * (j0 * mk0[2] + j1 * mk1[2]);
+    ja = R(1);
+  } else {
+    kx = R(0.5) * (mk0[0] + mk1[0]);
+    ky = R(0.5) * (mk0[1] + mk1[1]);
+    kz = R(0.5) * (mk0[2] + mk1[2]);
+    ja = R(0.5) * (j0 + j1);
+  }
+}
+
 // HLLC.cu:10-143
 template <int NS, int NT, typename R>
 __device__ inline void hllc(const DMech *__restrict__ m, const R *pl, const R *pr, const R *mk0, const R *mk1, R j0,
-                            R j1, R *f) {
+                            R j1, bool jface, R *f) {
   constexpr int NV = 5 + NS + NT;
   const R sl0 = sqrt(pl[0]), sr0 = sqrt(pr[0]);
   const R isum = rcp(sl0 + sr0), ipl = rcp(pl[0]), ipr = rcp(pr[0]);
@@ -640,11 +659,11 @@ __device__ inline void hllc(const DMech *__restrict__ m, const R *pl, const R *p
     }
     ct = (R)sqrt(cpp / cvv * kRuD * mwi * t);
   }
-  const R kx = R(0.5) * (mk0[0] + mk1[0]), ky = R(0.5) * (mk0[1] + mk1[1]), kz = R(0.5) * (mk0[2] + mk1[2]);
+  R kx, ky, kz, ja;
+  face_vector(mk0, mk1, j0, j1, jface, kx, ky, kz, ja);
   const R gK = sqrt(kx * kx + ky * ky + kz * kz);
   const R igK = rcp(gK);
   const R Utb = (kx * ut + ky * vt + kz * wt) * igK;
-  const R ja = R(0.5) * (j0 + j1);
   const R Ul = kx * pl[1] + ky * pl[2] + kz * pl[3];
   const R Ulg = Ul * igK;
   const R cl = sqrt(pl[NV + 1] * pl[4] * ipl);
@@ -826,14 +845,16 @@ __device__ inline void roe(const DMech *__restrict__ m, const R *pl, const R *pr
 
 // LaxFriedrichs.cu:14-49 (air only; the reference's generic version is empty)
 template <int NS, int NT, typename R>
-__device__ inline void lax_friedrichs(const R *pl, const R *pr, const R *mk0, const R *mk1, R j0, R j1, R *f) {
+__device__ inline void lax_friedrichs(const R *pl, const R *pr, const R *mk0, const R *mk1, R j0, R j1, bool jface,
+                                      R *f) {
   constexpr int NV = 5 + NS + NT;
-  const R kx = R(0.5) * (mk0[0] + mk1[0]), ky = R(0.5) * (mk0[1] + mk1[1]), kz = R(0.5) * (mk0[2] + mk1[2]);
+  R kx, ky, kz, ja;
+  face_vector(mk0, mk1, j0, j1, jface, kx, ky, kz, ja);
   const R gK = sqrt(kx * kx + ky * ky + kz * kz);
   const R Ukl = pl[1] * kx + pl[2] * ky + pl[3] * kz, Ukr = pr[1] * kx + pr[2] * ky + pr[3] * kz;
   const R cl = sqrt(R(kGam) * pl[4] / pl[0]), cr = sqrt(R(kGam) * pr[4] / pr[0]);
   const R sr = fmax(fabs(Ukl) + cl * gK, fabs(Ukr) + cr * gK);
-  const R hj = R(0.5) * R(0.5) * (j0 + j1);
+  const R hj = R(0.5) * ja;
   const R rUl = pl[0] * Ukl, rUr = pr[0] * Ukr;
   f[0] = (rUl + rUr - sr * (pr[0] - pl[0])) * hj;
   f[1] = (rUl * pl[1] + rUr * pr[1] + kx * (pl[4] + pr[4]) - sr * (pr[1] * pr[0] - pl[1] * pl[0])) * hj;
@@ -1036,8 +1057,8 @@ __device__ inline void riemann_viscous(const ZoneView &z, const DParams &P, cons
   // inviscid_scheme 2 with AWENO (inviscid_type 1) runs AUSM+, as the
   // reference's AWENO switch has no Roe case (InviscidScheme.cu:290-303)
   switch (P.inviscid_scheme) {
-    case 1: lax_friedrichs<NS, NT, R>(pl, pr, mk0, mk1, j0, j1, F); break;
-    case 4: hllc<NS, NT, R>(m, pl, pr, mk0, mk1, j0, j1, F); break;
+    case 1: lax_friedrichs<NS, NT, R>(pl, pr, mk0, mk1, j0, j1, P.jface != 0, F); break;
+    case 4: hllc<NS, NT, R>(m, pl, pr, mk0, mk1, j0, j1, P.jface != 0, F); break;
     case 2:
       if constexpr (SCHEME == 0) {
         roe<NS, NT, R>(m, pl, pr, mk0, mk1, j0, j1, efix_point<R>(z, P, p0), efix_point<R>(z, P, p0 + st), F);

@@ -80,9 +80,12 @@ hipError_t launch_inner_exchange(const ZoneView &z, const ZoneView &tz, const DP
 // J and metrics of a block from its (g + 1)-ghosted node coordinates, with the
 // reference's ghost fall-back; bad: one byte per metric point (scratch);
 // *interior_bad (device int, zeroed by the caller) set on a non-positive
-// interior Jacobian
+// interior Jacobian.  conservative: then the conservative symmetric rows
+// (AWENO-5 differences when aweno is set), with fsp_scratch 9 node arrays of
+// the (g + 1)-ghosted layout.
 hipError_t launch_metrics(int mx, int my, int mz, int g, const double *X, const double *Y, const double *Z,
-                          double *jac, double *met, unsigned char *bad, int *interior_bad, hipStream_t s);
+                          double *jac, double *met, unsigned char *bad, int *interior_bad, hipStream_t s,
+                          int conservative = 0, int aweno = 0, double *fsp_scratch = nullptr);
 hipError_t launch_chem_batch(const DMech *m, int nr, int64_t n, const double *rho, const double *T, const double *Y,
                              double dt, double *dY, double *dT, hipStream_t s);
 // ---- steady path (implicit.hip)

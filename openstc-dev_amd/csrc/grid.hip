@@ -3,7 +3,7 @@
 // second-order central differences of the (ngg + 1)-ghosted node
 // coordinates, then the reference's fall-back for ghost points with a
 // non-positive Jacobian (copy of the inner neighbour), slab by slab in the
-// reference's order.  Contraction into FMAs is off in this file so the
+// reference's order; optionally the conservative metric rows on top.  Contraction into FMAs is off in this file so the
 // results equal the host computation bit for bit.
 #include <hip/hip_runtime.h>
 
@@ -102,12 +102,93 @@ __global__ void __launch_bounds__(kGTB) k_metric_slab(GridDims G, int axis, int 
   }
 }
 
+// Conservative symmetric metrics (the option conservative_metric), the same
+// arithmetic as the host pass: row r of J d(xi_r)/d(x_c) is
+// 1/2 [D_{r+2}(A^c_{r+1}) - D_{r+1}(A^c_{r+2})] (directions mod 3) with
+// A^c_d = a_d b - a b_d for the coordinates (a, b) following c, and D_d the
+// scheme's difference along d at that index: (f[+1] - f[-1]) / 2, or for
+// AWENO-5 on cells [0, n - 2 g) the face difference plus the CDS
+// correction.  The discrete metric identities then hold to rounding.
+constexpr double kFspW1 = 145.0 / 256, kFspW2 = -13.0 / 320, kFspW3 = 19.0 / 3840;
+
+struct FspDims {
+  GridDims G;
+  int aweno;
+  __device__ int n(int d) const { return d == 0 ? G.mx : (d == 1 ? G.my : G.mz); }
+  __device__ int64_t cstride(int d) const { return d == 0 ? 1 : (d == 1 ? G.cd2 : G.cd1); }
+};
+
+// D_d of F at node (i, j, k) of the (g + 1)-ghosted layout; NaN when the
+// stencil leaves it
+__device__ inline double fsp_diff(const FspDims &S, const double *__restrict__ F, int d, int i, int j, int k) {
+  const int c = d == 0 ? i : (d == 1 ? j : k);
+  const int n = S.n(d), g = S.G.g;
+  const bool wide = S.aweno && c >= 0 && c < n - 2 * g;
+  const int w = wide ? 3 : 1;
+  if (c - w < -(g + 1) || c + w > n + g) return __builtin_nan("");
+  const int64_t st = S.cstride(d);
+  const int64_t p = S.G.cat(i, j, k);
+  if (!wide) return (F[p + st] - F[p - st]) * 0.5;
+  return (F[p + st] - F[p - st]) * kFspW1 + (F[p + 2 * st] - F[p - 2 * st]) * kFspW2 +
+         (F[p + 3 * st] - F[p - 3 * st]) * kFspW3;
+}
+
+// A[(c * 3 + d) * csz + p] at every node of the (g + 1)-ghosted layout
+__global__ void __launch_bounds__(kGTB) k_fsp_products(FspDims S, int64_t csz, const double *__restrict__ X,
+                                                       const double *__restrict__ Y, const double *__restrict__ Z,
+                                                       double *__restrict__ A) {
+  const int64_t t = (int64_t)blockIdx.x * kGTB + threadIdx.x;
+  if (t >= csz) return;
+  const int g1 = S.G.g + 1;
+  const int nx = S.G.mx + 2 * g1, ny = S.G.my + 2 * g1;
+  const int i = (int)(t % nx) - g1;
+  const int j = (int)((t / nx) % ny) - g1;
+  const int k = (int)(t / ((int64_t)nx * ny)) - g1;
+  const double *C[3] = {X, Y, Z};
+  for (int d = 0; d < 3; ++d) {
+    double der[3];
+    for (int c = 0; c < 3; ++c) der[c] = fsp_diff(S, C[c], d, i, j, k);
+    for (int c = 0; c < 3; ++c) {
+      const int a = (c + 1) % 3, b = (c + 2) % 3;
+      A[(int64_t)(c * 3 + d) * csz + t] = der[a] * C[b][t] - C[a][t] * der[b];
+    }
+  }
+}
+
+// the conservative rows of every point of the g-ghosted layout whose
+// stencils stay on the nodes, divided by the point's J
+__global__ void __launch_bounds__(kGTB) k_fsp_rows(FspDims S, int64_t csz, const double *__restrict__ A,
+                                                   const double *__restrict__ jac, double *__restrict__ met) {
+  const int64_t t = (int64_t)blockIdx.x * kGTB + threadIdx.x;
+  const GridDims &G = S.G;
+  if (t >= G.sz) return;
+  const int nx = G.mx + 2 * G.g, ny = G.my + 2 * G.g;
+  const int i = (int)(t % nx) - G.g;
+  const int j = (int)((t / nx) % ny) - G.g;
+  const int k = (int)(t / ((int64_t)nx * ny)) - G.g;
+  const double J = jac[t];
+  for (int r = 0; r < 3; ++r) {
+    const int d1 = (r + 1) % 3, d2 = (r + 2) % 3;
+    double row[3];
+    bool ok = true;
+    for (int c = 0; c < 3; ++c) {
+      row[c] = (fsp_diff(S, A + (int64_t)(c * 3 + d1) * csz, d2, i, j, k) -
+                fsp_diff(S, A + (int64_t)(c * 3 + d2) * csz, d1, i, j, k)) *
+               0.5;
+      if (__builtin_isnan(row[c])) ok = false;
+    }
+    if (!ok) continue;
+    for (int c = 0; c < 3; ++c) met[(int64_t)(r * 3 + c) * G.sz + t] = row[c] / J;
+  }
+}
+
 inline unsigned nblk(int64_t n) { return (unsigned)((n + kGTB - 1) / kGTB); }
 
 }  // namespace
 
 hipError_t launch_metrics(int mx, int my, int mz, int g, const double *X, const double *Y, const double *Z,
-                          double *jac, double *met, unsigned char *bad, int *interior_bad, hipStream_t s) {
+                          double *jac, double *met, unsigned char *bad, int *interior_bad, hipStream_t s,
+                          int conservative, int aweno, double *fsp_scratch) {
   GridDims G;
   G.mx = mx;
   G.my = my;
@@ -138,7 +219,20 @@ hipError_t launch_metrics(int mx, int my, int mz, int g, const double *X, const 
     e = hipGetLastError();
     if (e != hipSuccess) return e;
   }
-  return hipSuccess;
+  if (conservative) {
+    if (!fsp_scratch) return hipErrorInvalidValue;
+    FspDims S;
+    S.G = G;
+    S.aweno = aweno;
+    const int64_t csz = G.cd1 * (mz + 2 * (g + 1));
+    hipLaunchKernelGGL(k_fsp_products, dim3(nblk(csz)), dim3(kGTB), 0, s, S, csz, X, Y, Z, fsp_scratch);
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_fsp_rows, dim3(nblk(G.sz)), dim3(kGTB), 0, s, S, csz, (const double *)fsp_scratch,
+                       (const double *)jac, met);
+    e = hipGetLastError();
+  }
+  return e;
 }
 
 }  // namespace ostc

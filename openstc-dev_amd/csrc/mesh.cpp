@@ -3,6 +3,7 @@
 // k-ghosts at z0+k, Mesh.cpp:848) and the Jacobian / metrics by second-order
 // central differences (Mesh.cpp:259-521).  One-time work; the result is
 // uploaded as component-major arrays for coalesced device reads.
+#include <cmath>
 #include <cstdlib>
 #include <cstring>
 
@@ -116,11 +117,83 @@ bool point_metric(const Layout &Lc, const double *X, const double *Y, const doub
   return true;
 }
 
+// Conservative symmetric metrics (the option conservative_metric; the
+// reference's metrics above are non-conservative and a uniform flow on a
+// curved grid does not stay uniform with them).  Row r of J d(xi_r)/d(x_c):
+//   J xi_c   = 1/2 [D_zeta(A^c_eta) - D_eta(A^c_zeta)]
+//   J eta_c  = 1/2 [D_xi(A^c_zeta)  - D_zeta(A^c_xi)]
+//   J zeta_c = 1/2 [D_eta(A^c_xi)   - D_xi(A^c_eta)]
+// with A^x_d = y_d z - y z_d, A^y_d = z_d x - z x_d, A^z_d = x_d y - x y_d
+// (Thomas & Lombard 1979; Vinokur & Yee 2002).  D_d, also for the inner
+// derivatives, is the difference the flux scheme applies along d at that
+// index: (f[+1] - f[-1]) / 2 (the face average of J * metric differenced),
+// plus for AWENO-5 on cells [0, n - 2 ngg), where the CDS correction acts,
+// c3 (f[+1] - f[-1]) + c2 (f[+2] - f[-2]) + c1 (f[+3] - f[-3]).  Operators
+// along different directions commute, so the discrete metric identities
+// hold to rounding and a uniform flow has a zero right-hand side.  J stays
+// the determinant; the stored row is divided by it.  A row whose stencil
+// leaves the node array keeps the non-conservative row (the fluxes never
+// read such nodes).
+constexpr double kFspW1 = 145.0 / 256, kFspW2 = -13.0 / 320, kFspW3 = 19.0 / 3840;
+
+// D_d of F at (i, j, k) on the (ngg + 1)-ghosted node layout; NaN when the
+// stencil leaves it
+double fsp_diff(const Layout &Lc, const double *F, int d, int i, int j, int k, int n, int ngg, bool aweno) {
+  const int c = d == 0 ? i : (d == 1 ? j : k);
+  const bool wide = aweno && c >= 0 && c < n - 2 * ngg;
+  const int w = wide ? 3 : 1;
+  if (c - w < -(ngg + 1) || c + w > n + ngg) return std::nan("");
+  const int64_t st = d == 0 ? 1 : (d == 1 ? Lc.d2 : Lc.d1);
+  const int64_t p = Lc.at(i, j, k);
+  if (!wide) return (F[p + st] - F[p - st]) * 0.5;
+  return (F[p + st] - F[p - st]) * kFspW1 + (F[p + 2 * st] - F[p - 2 * st]) * kFspW2 +
+         (F[p + 3 * st] - F[p - 3 * st]) * kFspW3;
+}
+
+void fsp_rows(int mx, int my, int mz, int ngg, bool aweno, const Layout &Lc, const Layout &L, const double *X,
+              const double *Y, const double *Z, std::vector<Metric9> &M) {
+  const int n[3] = {mx, my, mz};
+  const double *C[3] = {X, Y, Z};
+  // A[c * 3 + d] on the node layout
+  std::vector<double> A((size_t)9 * Lc.sz);
+  for (int k = -ngg - 1; k <= mz + ngg; ++k)
+    for (int j = -ngg - 1; j <= my + ngg; ++j)
+      for (int i = -ngg - 1; i <= mx + ngg; ++i) {
+        const int64_t p = Lc.at(i, j, k);
+        for (int d = 0; d < 3; ++d) {
+          double der[3];
+          for (int c = 0; c < 3; ++c) der[c] = fsp_diff(Lc, C[c], d, i, j, k, n[d], ngg, aweno);
+          for (int c = 0; c < 3; ++c) {
+            const int a = (c + 1) % 3, b = (c + 2) % 3;
+            A[(size_t)(c * 3 + d) * Lc.sz + p] = der[a] * C[b][p] - C[a][p] * der[b];
+          }
+        }
+      }
+  for (int k = -ngg; k < mz + ngg; ++k)
+    for (int j = -ngg; j < my + ngg; ++j)
+      for (int i = -ngg; i < mx + ngg; ++i) {
+        Metric9 &v = M[L.at(i, j, k)];
+        for (int r = 0; r < 3; ++r) {
+          const int d1 = (r + 1) % 3, d2 = (r + 2) % 3;
+          double row[3];
+          bool ok = true;
+          for (int c = 0; c < 3; ++c) {
+            row[c] = (fsp_diff(Lc, &A[(size_t)(c * 3 + d1) * Lc.sz], d2, i, j, k, n[d2], ngg, aweno) -
+                      fsp_diff(Lc, &A[(size_t)(c * 3 + d2) * Lc.sz], d1, i, j, k, n[d1], ngg, aweno)) *
+                     0.5;
+            if (std::isnan(row[c])) ok = false;
+          }
+          if (!ok) continue;
+          for (int c = 0; c < 3; ++c) v.m[r * 3 + c] = row[c] / v.J;
+        }
+      }
+}
+
 }  // namespace
 
 bool compute_metrics(int mx, int my, int mz, int ngg, const double *x, const double *y, const double *z,
                      std::vector<double> &jac, std::vector<double> &metric, std::string &err,
-                     std::vector<double> *xyz) {
+                     std::vector<double> *xyz, bool conservative, bool aweno) {
   const Layout Lc(mx, my, mz, ngg + 1), L(mx, my, mz, ngg);
   std::vector<double> X(Lc.sz, 0), Y(Lc.sz, 0), Z(Lc.sz, 0);
   for (int k = 0; k < mz; ++k)
@@ -176,6 +249,7 @@ bool compute_metrics(int mx, int my, int mz, int ngg, const double *x, const dou
   for (int i = mx - 1; i < mx + ngg; ++i)
     for (int j = -ngg; j < my + ngg; ++j)
       for (int k = -ngg; k < mz + ngg; ++k) put(i, j, k, -1, 0, 0);
+  if (conservative) fsp_rows(mx, my, mz, ngg, aweno, Lc, L, X.data(), Y.data(), Z.data(), M);
   if (xyz) {
     // node coordinates on the ngg-ghosted layout, component-major
     xyz->assign((size_t)L.sz * 3, 0);

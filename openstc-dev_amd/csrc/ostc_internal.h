@@ -90,11 +90,13 @@ struct Layout {
 // x,y,z interior node coordinates with i fastest.  Outputs: jac (Lg) and
 // metric as 9 component-major arrays (metric[(r*3+c)*Lg.sz + p] =
 // d xi_r / d x_c), both on the ngg-ghosted layout; xyz (optional) gets the
-// ghosted node coordinates.  Returns false on a non-positive Jacobian in the
-// interior.
+// ghosted node coordinates.  conservative: the rows in the conservative
+// symmetric form that keeps a uniform flow exact, with the differences of
+// the AWENO-5 scheme when aweno is set.  Returns false on a non-positive
+// Jacobian in the interior.
 bool compute_metrics(int mx, int my, int mz, int ngg, const double *x, const double *y, const double *z,
                      std::vector<double> &jac, std::vector<double> &metric, std::string &err,
-                     std::vector<double> *xyz = nullptr);
+                     std::vector<double> *xyz = nullptr, bool conservative = false, bool aweno = false);
 
 // Ghost node steps of Mesh.cpp:790-944 on an (ngg + 1)-ghosted node layout:
 // reflected faces first, then (after the inner faces' ghost nodes were taken

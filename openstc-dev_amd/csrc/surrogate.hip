@@ -285,28 +285,28 @@ __device__ inline int finish_cell(const DMech *__restrict__ m, const SurNorm &nm
   return code;
 }
 
-// wave-aggregated counts of the gate codes (index 8: forced fallback, a
-// network trained for another dt); reasons may be null
-__device__ inline void count_reasons(int code, bool active, int *__restrict__ reasons) {
-  if (!reasons) return;
-#pragma unroll
-  for (int r = 1; r <= 8; ++r) {
-    const unsigned long long mask = __ballot(active && code == r);
-    if (mask && (threadIdx.x & 63) == __ffsll((long long)mask) - 1) atomicAdd(reasons + r, (int)__popcll(mask));
-  }
-}
+// Gate codes of one workgroup (index 8: forced fallback, a network trained
+// for another dt) summed in LDS, then added to one of kReasonCopies striped
+// copies of the counters (the host sums the copies), so thousands of
+// workgroups do not serialise on one cache line; reasons may be null.
+constexpr int kReasonCopies = 16;
 
-// wave-aggregated append of the cells with fb set to the fallback list
-__device__ inline void append_fallback(bool fb, int64_t c, int *__restrict__ fb_count, int *__restrict__ fb_idx) {
+// block-aggregated append of the cells with fb set to the fallback list: one
+// global atomic per workgroup of 4 wavefronts
+__device__ inline void append_fallback(bool fb, int64_t c, int *__restrict__ fb_count, int *__restrict__ fb_idx,
+                                       int *s_wave, int &s_base) {
   const unsigned long long mask = __ballot(fb);
-  if (mask) {
-    const int lane = threadIdx.x & 63;
-    const int leader = __ffsll((long long)mask) - 1;
-    int base = 0;
-    if (lane == leader) base = atomicAdd(fb_count, (int)__popcll(mask));
-    base = __shfl(base, leader, 64);
-    if (fb) fb_idx[base + (int)__popcll(mask & ((1ull << lane) - 1ull))] = (int)c;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0) s_wave[wave] = (int)__popcll(mask);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const int total = s_wave[0] + s_wave[1] + s_wave[2] + s_wave[3];
+    s_base = total ? atomicAdd(fb_count, total) : 0;
   }
+  __syncthreads();
+  int base = s_base;
+  for (int w = 0; w < wave; ++w) base += s_wave[w];
+  if (fb) fb_idx[base + (int)__popcll(mask & ((1ull << lane) - 1ull))] = (int)c;
 }
 
 __global__ void __launch_bounds__(256) k_sur_finish(const DMech *__restrict__ m, SurNorm nm, int64_t n, int64_t c0,
@@ -316,13 +316,20 @@ __global__ void __launch_bounds__(256) k_sur_finish(const DMech *__restrict__ m,
                                                     double *__restrict__ dY, double *__restrict__ dT,
                                                     int *__restrict__ fb_count, int *__restrict__ fb_idx,
                                                     int *__restrict__ reasons) {
+  __shared__ int s_codes[9];
+  __shared__ int s_wave[4];
+  __shared__ int s_base;
+  if (threadIdx.x < 9) s_codes[threadIdx.x] = 0;
   const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
   const bool active = t < count;
   const int64_t c = c0 + t;
   int code = 8;
   if (active && !force_fallback) code = finish_cell(m, nm, n, c, rho_a, T_a, Y_a, out + t * ld, dY, dT);
-  count_reasons(code, active, reasons);
-  append_fallback(active && code != 0, c, fb_count, fb_idx);
+  __syncthreads();  // s_codes cleared
+  if (active && code > 0) atomicAdd(&s_codes[code], 1);
+  append_fallback(active && code != 0, c, fb_count, fb_idx, s_wave, s_base);  // synchronises the block
+  if (reasons && threadIdx.x >= 1 && threadIdx.x <= 8 && s_codes[threadIdx.x])
+    atomicAdd(reasons + (blockIdx.x % kReasonCopies) * 16 + threadIdx.x, s_codes[threadIdx.x]);
 }
 
 // ------------------------------------------------------------ fused network
@@ -666,7 +673,7 @@ struct SurrogateState {
   __bf16 *iX = nullptr, *iH[2] = {};
   float *iOut = nullptr;
   int *fb_idx = nullptr, *fb_count = nullptr;
-  int *reasons = nullptr;  // gate codes 1..8 of the last inference (16 ints)
+  int *reasons = nullptr;  // gate codes 1..8 of the last inference (kReasonCopies x 16 ints)
   float *fb_out = nullptr;  // network outputs [fcap x 8] of the fused path
   int kernel = 3;  // inference: 3 fused network, 128 cells per workgroup, halves pipelined (default);
                    // 1 the same unpipelined; 2 fused, 64 cells; 0 layer-by-layer GEMMs
@@ -949,7 +956,7 @@ int surrogate_create(const Mechanism &mech, int device, SurrogateState **out, st
   SCHK(hipMemcpy(s->aug, aug, sizeof(aug), hipMemcpyHostToDevice));
   SCHK(dalloc(&s->loss, 1));
   SCHK(dalloc(&s->fb_count, 1));
-  SCHK(dalloc(&s->reasons, 16));
+  SCHK(dalloc(&s->reasons, kReasonCopies * 16));
   for (auto &e : s->ev) SCHK(hipEventCreate(&e));
   *out = s.release();
   return 0;
@@ -1289,7 +1296,7 @@ int surrogate_infer(SurrogateState *s, const DMech *m, int64_t n, const double *
   if (ensure_infer_ws(s, s->kernel == 0 ? (chunk + 7) / 8 * 8 : 0, n, st, err)) return 1;
   SCHK(hipEventRecord(s->ev[0], st));
   SCHK(hipMemsetAsync(s->fb_count, 0, sizeof(int), st));
-  SCHK(hipMemsetAsync(s->reasons, 0, 16 * sizeof(int), st));
+  SCHK(hipMemsetAsync(s->reasons, 0, kReasonCopies * 16 * sizeof(int), st));
   if (s->kernel != 0) {
     if (!force && launch_net(s, m, n, rho, T, Y, s->fb_out, st, err)) return 1;
     SCHK(hipEventRecord(s->ev[3], st));
@@ -1542,11 +1549,14 @@ int surrogate_load(SurrogateState *s, const std::string &path, hipStream_t st, s
 }
 
 int surrogate_gate_reasons(SurrogateState *s, int64_t *out9, hipStream_t st, std::string &err) {
-  int h[16] = {};
+  int h[kReasonCopies * 16] = {};
   SCHK(hipMemcpyAsync(h, s->reasons, sizeof(h), hipMemcpyDeviceToHost, st));
   SCHK(hipStreamSynchronize(st));
   out9[0] = s->last_fallback;
-  for (int r = 1; r <= 8; ++r) out9[r] = h[r];
+  for (int r = 1; r <= 8; ++r) {
+    out9[r] = 0;
+    for (int k = 0; k < kReasonCopies; ++k) out9[r] += h[k * 16 + r];
+  }
   return 0;
 }
 

@@ -181,6 +181,56 @@ def c2_case(nx=256, ny=128, nz=128, seed=1002, steps=20, mach=2.5, t_inf=300.0, 
                 steps=steps)
 
 
+def wavy_grid(nx, ny, nz, lx=1.0, ly=0.8, lz=0.6, amp=0.03):
+    """Curved grid displaced in all three directions (2-D when nz == 1):
+    x' = x + amp sin(2 pi y/ly) sin(2 pi z/lz), y' = y + amp sin(2 pi x/lx)
+    sin(2 pi z/lz), z' = z + amp sin(2 pi x/lx) sin(2 pi y/ly); in 2-D the
+    z factors are dropped and z = 0."""
+    xi = np.linspace(0.0, lx, nx)
+    yj = np.linspace(0.0, ly, ny)
+    zk = np.linspace(0.0, lz, nz) if nz > 1 else np.zeros(1)
+    Z, Y, X = np.meshgrid(zk, yj, xi, indexing="ij")
+    sx, sy = np.sin(2 * np.pi * X / lx), np.sin(2 * np.pi * Y / ly)
+    sz = np.sin(2 * np.pi * Z / lz) if nz > 1 else np.ones_like(Z)
+    x = X + amp * sy * sz
+    y = Y + amp * sx * sz
+    z = Z + amp * sx * sy if nz > 1 else Z
+    return x.copy(), y.copy(), z.copy()
+
+
+def uniform_flow_case(nx=14, ny=12, nz=10, reconstruction=2, inviscid_scheme=3, conservative_metric=1,
+                      viscous_order=2, amp=0.03):
+    """Free-stream preservation: a uniform Mach 2 air stream at 300 K,
+    101325 Pa along (0.8, 0.48, 0.36) (2-D: (0.8, 0.6, 0)) on wavy_grid, every
+    face a supersonic inflow of that state, no perturbation.  With the
+    conservative metrics the right-hand side vanishes to rounding."""
+    x, y, z = wavy_grid(nx, ny, nz, amp=amp)
+    d = (0.8, 0.48, 0.36) if nz > 1 else (0.8, 0.6, 0.0)
+    inflow = air_inflow(2.0, 101325.0, 300.0)
+    vel = inflow["vel"]
+    inflow = dict(inflow, u=vel * d[0], v=vel * d[1], w=vel * d[2])
+    setup = _setup_common({
+        "int dimension": 3 if nz > 1 else 2, "real cfl": 0.5, "int inviscid_scheme": inviscid_scheme,
+        "int reconstruction": reconstruction, "int limiter": 0, "int viscous_order": viscous_order,
+        "int total_step": 10, "bool conservative_metric": conservative_metric,
+        "array string boundary_conditions": ["freestream"],
+        "string reference_state": "freestream", "string default_init": "freestream",
+        "struct freestream": [("string", "type", "inflow"), ("int", "label", 5), ("real", "mach", 2.0),
+                              ("real", "pressure", 101325.0), ("real", "temperature", 300.0), ("real", "u", d[0]),
+                              ("real", "v", d[1]), ("real", "w", d[2])],
+    })
+    faces = [(0, 0, 0, ny - 1, 0, nz - 1), (nx - 1, nx - 1, 0, ny - 1, 0, nz - 1),
+             (0, nx - 1, 0, 0, 0, nz - 1), (0, nx - 1, ny - 1, ny - 1, 0, nz - 1)]
+    if nz > 1:
+        faces += [(0, nx - 1, 0, ny - 1, 0, 0), (0, nx - 1, 0, ny - 1, nz - 1, nz - 1)]
+    bcs = [dict(type=5, raw=r, label=5) for r in faces]
+    scheme = dict(mix=0, inviscid_scheme=inviscid_scheme, reconstruction=reconstruction, limiter=0,
+                  viscous_order=viscous_order, cfl=0.5, fixed_time_step=0, output_screen=1,
+                  conservative_metric=conservative_metric)
+    return dict(name="uniform", mx=nx, my=ny, mz=nz, ngg=3 if reconstruction == 4 else 2, x=x, y=y, z=z,
+                setup=setup, bcs=bcs, scheme=scheme, inflow=inflow, seed=1, perturb=None, steps=10)
+
+
 def c3_case(nx=512, ny=256, seed=1003, steps=20, dx=1e-4, delta=5e-4, viscous_order=2, inviscid_scheme=4,
             reconstruction=4):
     """2-D reacting H2/air supersonic mixing layer with the 9-species,

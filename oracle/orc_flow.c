@@ -210,6 +210,82 @@ static int metrics(int mx, int my, int mz, int ngg, lay Lc, lay L, const double 
   return 0;
 }
 
+/* Conservative symmetric metrics (build option conservative_metric; the
+ * reference's metrics are the non-conservative differences above and do not
+ * preserve a uniform flow on curved grids).  Row r of J d(xi_r)/d(x_c):
+ *   J xi_c   = 1/2 [D_zeta(A^c_eta) - D_eta(A^c_zeta)]
+ *   J eta_c  = 1/2 [D_xi(A^c_zeta)  - D_zeta(A^c_xi)]
+ *   J zeta_c = 1/2 [D_eta(A^c_xi)   - D_xi(A^c_eta)]
+ * with A^x_d = y_d z - y z_d, A^y_d = z_d x - z x_d, A^z_d = x_d y - x y_d
+ * (Thomas & Lombard 1979; Vinokur & Yee 2002).  D_d, also for the inner
+ * derivatives x_d, is the difference the flux scheme applies along d at that
+ * index: the face average of J * metric differenced, (f[+1] - f[-1]) / 2,
+ * plus, for AWENO-5 on cells [0, n - 2 ngg) where the CDS correction acts,
+ * c3 (f[+1] - f[-1]) + c2 (f[+2] - f[-2]) + c1 (f[+3] - f[-3]).  Operators
+ * along different directions commute, so the discrete metric identities
+ * sum_r D_r(J xi_r,c) = 0 hold to rounding and a uniform flow keeps a zero
+ * right-hand side.  J stays the determinant of metric_point; the stored
+ * metric is the row divided by it.  A row whose stencil leaves the node
+ * array keeps the reference row (such nodes are never read by the fluxes). */
+static const double kFspW1 = 145.0 / 256, kFspW2 = -13.0 / 320, kFspW3 = 19.0 / 3840;
+
+/* D_d of F at (i, j, k) on the (ngg + 1)-ghosted node layout; NaN when the
+ * stencil leaves it */
+static double fsp_diff(lay Lc, const double *F, int d, int i, int j, int k, int n, int ngg, int aweno) {
+  const int c = d == 0 ? i : (d == 1 ? j : k);
+  const int wide = aweno && c >= 0 && c < n - 2 * ngg;
+  const int w = wide ? 3 : 1;
+  if (c - w < -(ngg + 1) || c + w > n + ngg) return NAN;
+  const int di = d == 0, dj = d == 1, dk = d == 2;
+#define F_(s) F[at(Lc, i + (s) * di, j + (s) * dj, k + (s) * dk)]
+  if (!wide) return (F_(1) - F_(-1)) * 0.5;
+  return (F_(1) - F_(-1)) * kFspW1 + (F_(2) - F_(-2)) * kFspW2 + (F_(3) - F_(-3)) * kFspW3;
+#undef F_
+}
+
+static void fsp_metrics(int mx, int my, int mz, int ngg, int aweno, lay Lc, lay L, const double *X, const double *Y,
+                        const double *Z, const double *jac, double *met) {
+  const int n[3] = {mx, my, mz};
+  const double *C[3] = {X, Y, Z};
+  /* A[c * 3 + d] on the node layout */
+  double *A[9];
+  for (int q = 0; q < 9; ++q) A[q] = (double *)malloc(sizeof(double) * Lc.sz);
+  for (int k = -ngg - 1; k <= mz + ngg; ++k)
+    for (int j = -ngg - 1; j <= my + ngg; ++j)
+      for (int i = -ngg - 1; i <= mx + ngg; ++i) {
+        const int64_t p = at(Lc, i, j, k);
+        for (int d = 0; d < 3; ++d) {
+          double der[3];
+          for (int c = 0; c < 3; ++c) der[c] = fsp_diff(Lc, C[c], d, i, j, k, n[d], ngg, aweno);
+          /* column c uses the two other coordinates a = c+1, b = c+2 (mod 3): A = a_d b - a b_d */
+          for (int c = 0; c < 3; ++c) {
+            const int a = (c + 1) % 3, b = (c + 2) % 3;
+            A[c * 3 + d][p] = der[a] * C[b][p] - C[a][p] * der[b];
+          }
+        }
+      }
+  for (int k = -ngg; k < mz + ngg; ++k)
+    for (int j = -ngg; j < my + ngg; ++j)
+      for (int i = -ngg; i < mx + ngg; ++i) {
+        const int64_t p = at(L, i, j, k);
+        for (int r = 0; r < 3; ++r) {
+          /* J xi_r,c = 1/2 [D_{r+2}(A^c_{r+1}) - D_{r+1}(A^c_{r+2})] */
+          const int d1 = (r + 1) % 3, d2 = (r + 2) % 3;
+          double row[3];
+          int ok = 1;
+          for (int c = 0; c < 3; ++c) {
+            row[c] = (fsp_diff(Lc, A[c * 3 + d1], d2, i, j, k, n[d2], ngg, aweno) -
+                      fsp_diff(Lc, A[c * 3 + d2], d1, i, j, k, n[d1], ngg, aweno)) *
+                     0.5;
+            if (isnan(row[c])) ok = 0;
+          }
+          if (!ok) continue;
+          for (int c = 0; c < 3; ++c) met[p * 9 + r * 3 + c] = row[c] / jac[p];
+        }
+      }
+  for (int q = 0; q < 9; ++q) free(A[q]);
+}
+
 void orc_grid_metrics(int mx, int my, int mz, int ngg, const double *x, const double *y, const double *z, double *xg,
                       double *yg, double *zg, double *jac, double *metric) {
   lay Lc = mklay(mx, my, mz, ngg + 1), L = mklay(mx, my, mz, ngg);
@@ -335,6 +411,9 @@ static void zone_register(orc_zone *zz) {
   zz->inviscid_type = 0;
   if (cfg->inviscid_scheme == 2) zz->inviscid_type = 2;
   if (cfg->reconstruction == 4 || cfg->reconstruction == 5) zz->inviscid_type = 1;
+  if (cfg->conservative_metric)
+    fsp_metrics(zz->mx, zz->my, zz->mz, ngg, zz->inviscid_type == 1, zz->Lc, zz->Lg, zz->x, zz->y, zz->z, zz->jac,
+                zz->metric);
   /* boundaries: Mesh.cpp:14-31 register_boundary, BoundCond.cu:369-384 wall trim */
   zz->nbc = cfg->n_bc;
   for (int b = 0; b < cfg->n_bc; ++b) {
@@ -1090,6 +1169,26 @@ static void flux_ausm(const orc_zone *zz, const double *pl, const double *pr, co
   for (int l = 5; l < nv; ++l) f[l] = coeff * s[l];
 }
 
+/* Face metric and Jacobian of HLLC and LF at the face ip + 1/2: the
+ * reference's (m0 + m1) / 2 and (J0 + J1) / 2 (HLLC.cu:64-71,
+ * LaxFriedrichs.cu:22-35); with conservative metrics the face vector is
+ * (J0 m0 + J1 m1) / 2 and the factor 1, as AUSM+ and Roe have it, so that
+ * the face vectors of a cell close. */
+static void face_vector(const orc_zone *zz, const double *met, const double *jac, int ip, double *kx, double *ky,
+                        double *kz, double *ja) {
+  if (zz->cfg.conservative_metric) {
+    *kx = 0.5 * (jac[ip] * met[ip * 3] + jac[ip + 1] * met[(ip + 1) * 3]);
+    *ky = 0.5 * (jac[ip] * met[ip * 3 + 1] + jac[ip + 1] * met[(ip + 1) * 3 + 1]);
+    *kz = 0.5 * (jac[ip] * met[ip * 3 + 2] + jac[ip + 1] * met[(ip + 1) * 3 + 2]);
+    *ja = 1;
+    return;
+  }
+  *kx = 0.5 * (met[ip * 3] + met[(ip + 1) * 3]);
+  *ky = 0.5 * (met[ip * 3 + 1] + met[(ip + 1) * 3 + 1]);
+  *kz = 0.5 * (met[ip * 3 + 2] + met[(ip + 1) * 3 + 2]);
+  *ja = 0.5 * (jac[ip] + jac[ip + 1]);
+}
+
 /* HLLC.cu:10-143 compute_hllc_flux */
 static void flux_hllc(const orc_zone *zz, const double *pl, const double *pr, const double *met, const double *jac,
                       int ip, double *f) {
@@ -1120,12 +1219,10 @@ static void flux_hllc(const orc_zone *zz, const double *pl, const double *pr, co
     gamma = cpp / cvv;
     ct = sqrt(gamma * R_U * mw_inv * t);
   }
-  const double kx = 0.5 * (met[ip * 3] + met[(ip + 1) * 3]);
-  const double ky = 0.5 * (met[ip * 3 + 1] + met[(ip + 1) * 3 + 1]);
-  const double kz = 0.5 * (met[ip * 3 + 2] + met[(ip + 1) * 3 + 2]);
+  double kx, ky, kz, ja;
+  face_vector(zz, met, jac, ip, &kx, &ky, &kz, &ja);
   const double gK = sqrt(kx * kx + ky * ky + kz * kz);
   const double Utb = (kx * ut + ky * vt + kz * wt) / gK;
-  const double ja = 0.5 * (jac[ip] + jac[ip + 1]);
   double gl = GAMMA_AIR, gr = GAMMA_AIR;
   if (zz->cfg.mix) {
     gl = pl[nv + 1];
@@ -1187,14 +1284,13 @@ static void flux_hllc(const orc_zone *zz, const double *pl, const double *pr, co
 static void flux_lf(const orc_zone *zz, const double *pl, const double *pr, const double *met, const double *jac, int ip,
                     double *f) {
   const int nv = zz->nv;
-  const double kx = 0.5 * (met[ip * 3] + met[(ip + 1) * 3]);
-  const double ky = 0.5 * (met[ip * 3 + 1] + met[(ip + 1) * 3 + 1]);
-  const double kz = 0.5 * (met[ip * 3 + 2] + met[(ip + 1) * 3 + 2]);
+  double kx, ky, kz, ja;
+  face_vector(zz, met, jac, ip, &kx, &ky, &kz, &ja);
   const double gK = sqrt(kx * kx + ky * ky + kz * kz);
   const double Ukl = pl[1] * kx + pl[2] * ky + pl[3] * kz, Ukr = pr[1] * kx + pr[2] * ky + pr[3] * kz;
   const double cl = sqrt(GAMMA_AIR * pl[4] / pl[0]), cr = sqrt(GAMMA_AIR * pr[4] / pr[0]);
   const double sr = fmax(fabs(Ukl) + cl * gK, fabs(Ukr) + cr * gK);
-  const double hj = 0.5 * 0.5 * (jac[ip] + jac[ip + 1]);
+  const double hj = 0.5 * ja;
   const double rUl = pl[0] * Ukl, rUr = pr[0] * Ukr;
   f[0] = (rUl + rUr - sr * (pr[0] - pl[0])) * hj;
   f[1] = (rUl * pl[1] + rUr * pr[1] + kx * (pl[4] + pr[4]) - sr * (pr[1] * pr[0] - pl[1] * pl[0])) * hj;

@@ -124,6 +124,9 @@ typedef struct {
   /* steady path with split_chem: one point-implicit chemistry step of this
    * dt per pseudo-iteration, after the flow update (0: none) */
   double steady_chem_dt;
+  /* conservative symmetric metrics that keep a uniform flow exact (0: the
+   * reference's metrics) */
+  int conservative_metric;
 } orc_cfg;
 
 typedef struct orc_zone orc_zone;

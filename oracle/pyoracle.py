@@ -85,7 +85,8 @@ class Cfg(C.Structure):
                 ("turb_implicit", C.c_int), ("compressibility_correction", C.c_int), ("split_chem", C.c_int),
                 ("cfl", C.c_double), ("Pr", C.c_double), ("Sc", C.c_double), ("Prt", C.c_double),
                 ("Sct", C.c_double), ("dt", C.c_double), ("entropy_fix_factor", C.c_double),
-                ("n_bc", C.c_int), ("bc", BC * MAXBC), ("steady_chem_dt", C.c_double)]
+                ("n_bc", C.c_int), ("bc", BC * MAXBC), ("steady_chem_dt", C.c_double),
+                ("conservative_metric", C.c_int)]
 
 
 _lib = None
@@ -396,6 +397,7 @@ def make_cfg(case):
     c.compressibility_correction = case.get("compressibility_correction", 0)
     c.split_chem = case.get("split_chem", 0)
     c.steady_chem_dt = case.get("steady_chem_dt", 0.0)
+    c.conservative_metric = int(case.get("conservative_metric", 0))
     c.cfl = case.get("cfl", 0.5)
     c.Pr = case.get("prandtl_number", 0.72)
     c.Sc = case.get("schmidt_number", 0.9)
