@@ -31,6 +31,9 @@ import numpy as np  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
+# compulsory HBM bytes of one fully fused SSP-RK3 step of a 3-D air cell:
+# 8 (5 n_var reads + 3 n_var writes + 3 n_geom), n_var = 5, n_geom = 10
+STEP_COMPULSORY_BYTES = 8 * (5 * 5 + 3 * 5 + 3 * 10)
 BF16_MFMA_PEAK_TFLOPS = 2500.0
 
 # surrogate network FLOP per cell: 2 (11*256 + 256*256 + 256*256 + 256*8)
@@ -60,17 +63,23 @@ SWEEP_KERNELS = {"sweep_i": "sweep_i_aweno<double>", "sweep_j": "sweep_jk_aweno<
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "air_256x128x128_hbm_traffic.md")
 
 
-def measured_traffic(kernel):
-    """HBM bytes per launch of `kernel` from the committed counter summary
-    (FETCH_SIZE + WRITE_SIZE, copy-calibrated), or None."""
+def measured(key):
+    """A figure recorded in the committed counter summary as a line
+    "<key> = <value>", or None."""
     try:
         with open(TRAFFIC_FILE) as f:
             for line in f:
-                if line.startswith("traffic_bytes_per_launch " + kernel + " = "):
+                if line.startswith(key + " = "):
                     return float(line.rsplit("=", 1)[1])
     except OSError:
         pass
     return None
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed counter summary
+    (FETCH_SIZE + WRITE_SIZE, copy-calibrated), or None."""
+    return measured("traffic_bytes_per_launch " + kernel)
 
 
 def log(msg):
@@ -632,7 +641,23 @@ def main():
     algo_bytes = SWEEP_JK_BYTES_PER_CELL * cells
     achieved = algo_bytes / (avg_ms * 1e-3) / 1e9
     kname = SWEEP_KERNELS.get(dom, dom)
-    traffic = measured_traffic(kname) if (a.nx, a.ny, a.nz) == (256, 128, 128) else None
+    full = (a.nx, a.ny, a.nz) == (256, 128, 128)
+    traffic = measured_traffic(kname) if full else None
+    # FP64 operations of one launch (SQ_INSTS_VALU_FLOPS_FP64 + _TRANS) over
+    # this run's launch time: the sweep's FP64 fraction beside its HBM one
+    flop = measured("fp64_flop_per_launch " + kname) if full else None
+    fp64 = None
+    if flop:
+        fp64 = {"flop_per_launch": flop, "achieved": flop / (avg_ms * 1e-3) / 1e12, "peak": FP64_PEAK_TFLOPS,
+                "unit": "TFLOP/s", "frac": flop / (avg_ms * 1e-3) / 1e12 / FP64_PEAK_TFLOPS}
+    # copy-calibrated HBM bytes of every launch of one whole RK3 step per
+    # cell, against the compulsory bytes of a fully fused step
+    step_b = measured("step_traffic_bytes_per_cell") if full else None
+    step_traffic = None
+    if step_b:
+        step_traffic = {"bytes_per_cell_step": step_b, "compulsory_bytes_per_cell_step": STEP_COMPULSORY_BYTES,
+                        "ratio": step_b / STEP_COMPULSORY_BYTES,
+                        "achieved_GBs": step_b * cells / (ms_per_step * 1e-3) / 1e9}
     stage_ms = {k: v[1] / v[0] for k, v in prof.items()}
 
     cpu = None
@@ -657,7 +682,8 @@ def main():
                    "cells": cells, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "kernel": kname, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "algorithmic_bytes_per_launch": algo_bytes, "avg_launch_ms": avg_ms},
+                     "algorithmic_bytes_per_launch": algo_bytes, "avg_launch_ms": avg_ms, "fp64": fp64,
+                     "step_traffic": step_traffic},
         "cpu_baseline": cpu,
         "c1_air_2d": c1,
         "mixed_fp32_flux": {"value": cells * a.steps / mixed_s / 1e6, "unit": "Mcell-updates/s",

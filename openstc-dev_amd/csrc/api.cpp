@@ -419,6 +419,7 @@ int upload_mech(ostc_solver *s) {
   for (int i = 0; i < m.n_spec; ++i) {
     h.mw[i] = m.mw[i];
     h.mw_inv[i] = 1.0 / m.mw[i];
+    h.r_mw[i] = kRu / m.mw[i];
     h.t_low[i] = m.t_low[i];
     h.t_mid[i] = m.t_mid[i];
     for (int c = 0; c < 7; ++c) {

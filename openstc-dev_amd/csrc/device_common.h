@@ -28,6 +28,9 @@ struct DMech {
   // logarithms of LJ_potent_inv and kb_over_eps_jk: the collision-integral
   // powers T*^-0.145 become exp(-0.145 (log T + log k))
   double log_LJ_potent_inv[kMS], log_kb_over_eps_jk[kMS * kMS];
+  // R_u / W_l, rounded as the quotient itself (so h_l, cp_l equal the
+  // divided forms bit for bit)
+  double r_mw[kMS];
 };
 
 // Scheme switches and constants (the scalar part of DParameter).
@@ -88,6 +91,16 @@ constexpr double kMwAirD = 28.9647;
 // Thermo.cu:5-79.  Below t_low: h extrapolated with cp(t_low), cp and g/RT
 // frozen at t_low.
 
+// The mechanism pointer made opaque to the optimiser: a property evaluation
+// given fresh(m) loads its NASA-7 coefficients where it uses them
+// (scalar-cache hits) instead of sharing loads hoisted once and held in
+// registers for the rest of the kernel (the flux sweeps evaluate the
+// properties at many temperatures per face).
+__device__ inline const DMech *fresh(const DMech *m) {
+  asm volatile("" : "+s"(m));
+  return m;
+}
+
 template <int NS>
 __device__ inline void d_enthalpy(double t, const DMech *__restrict__ m, double *h) {
   const double t2 = t * t, t3 = t2 * t, t4 = t3 * t, t5 = t4 * t;
@@ -104,7 +117,7 @@ __device__ inline void d_enthalpy(double t, const DMech *__restrict__ m, double 
       const double *c = t < m->t_mid[i] ? &m->lo[i * 7] : &m->hi[i * 7];
       h[i] = c[0] * t + 0.5 * c[1] * t2 + c[2] * t3 / 3 + 0.25 * c[3] * t4 + 0.2 * c[4] * t5 + c[5];
     }
-    h[i] *= kRuD / m->mw[i];
+    h[i] *= m->r_mw[i];
   }
 }
 
@@ -125,8 +138,8 @@ __device__ inline void d_enthalpy_cp(double t, const DMech *__restrict__ m, doub
       h[i] = c[0] * t + 0.5 * c[1] * t2 + c[2] * t3 / 3 + 0.25 * c[3] * t4 + 0.2 * c[4] * t5 + c[5];
       cp[i] = c[0] + c[1] * t + c[2] * t2 + c[3] * t3 + c[4] * t4;
     }
-    h[i] *= kRuD / m->mw[i];
-    cp[i] *= kRuD / m->mw[i];
+    h[i] *= m->r_mw[i];
+    cp[i] *= m->r_mw[i];
   }
 }
 
@@ -143,7 +156,7 @@ __device__ inline void d_cp(double t, const DMech *__restrict__ m, double *cp) {
     } else {
       cp[i] = c[0] + c[1] * t + c[2] * t2 + c[3] * t3 + c[4] * t4;
     }
-    cp[i] *= kRuD / m->mw[i];
+    cp[i] *= m->r_mw[i];
   }
 }
 

@@ -88,18 +88,18 @@ __device__ inline void recon_pv(const ZoneView &z, int recon, const DMech *__res
     const double tl = (double)pl[4] / ((double)pl[0] * kRuD * mil), tr = (double)pr[4] / ((double)pr[0] * kRuD * mir);
     double h[NS], c[NS];
     double cpl = 0, cvl = 0, cpr = 0, cvr = 0;
-    d_enthalpy_cp<NS>(tl, m, h, c);
+    d_enthalpy_cp<NS>(tl, fresh(m), h, c);
 #pragma unroll
     for (int l = 0; l < NS; ++l) {
       cpl += c[l] * (double)pl[l + 5];
-      cvl += (double)pl[l + 5] * (c[l] - kRuD / m->mw[l]);
+      cvl += (double)pl[l + 5] * (c[l] - m->r_mw[l]);
       el += h[l] * (double)pl[l + 5];
     }
-    d_enthalpy_cp<NS>(tr, m, h, c);
+    d_enthalpy_cp<NS>(tr, fresh(m), h, c);
 #pragma unroll
     for (int l = 0; l < NS; ++l) {
       cpr += c[l] * (double)pr[l + 5];
-      cvr += (double)pr[l + 5] * (c[l] - kRuD / m->mw[l]);
+      cvr += (double)pr[l + 5] * (c[l] - m->r_mw[l]);
       er += h[l] * (double)pr[l + 5];
     }
     pl[NV] = (R)((double)pl[0] * el - (double)pl[4]);
@@ -346,12 +346,12 @@ __device__ inline void mix_roe_state(const DMech *__restrict__ m, const R *cl, c
     for (int l = 0; l < NS; ++l) mwi += (double)E.y[l] / m->mw[l];
     const double t = (double)((dl * (cl[NV] * il) + dr * (cr[NV] * ir)) * id) / (kRuD * mwi);
     double hi[NS], cpi[NS];
-    d_enthalpy_cp<NS>(t, m, hi, cpi);
+    d_enthalpy_cp<NS>(t, fresh(m), hi, cpi);
     double cp = 0, cv = 0, hs = 0;
 #pragma unroll
     for (int l = 0; l < NS; ++l) {
       cp += cpi[l] * (double)E.y[l];
-      cv += (double)E.y[l] * (cpi[l] - kRuD / m->mw[l]);
+      cv += (double)E.y[l] * (cpi[l] - m->r_mw[l]);
       hs += (double)E.y[l] * hi[l];
     }
     const double gamma = cp / cv;
@@ -436,7 +436,7 @@ __device__ inline double d_energy_temperature(const DMech *__restrict__ m, const
   double t = t0;
   for (int it = 0; it < 30; ++it) {
     double h[NS], cp[NS];
-    d_enthalpy_cp<NS>(t, m, h, cp);
+    d_enthalpy_cp<NS>(t, fresh(m), h, cp);
     double f = 0, fp = 0;
 #pragma unroll
     for (int l = 0; l < NS; ++l) {
@@ -473,12 +473,12 @@ __device__ inline void mix_side(const DMech *__restrict__ m, const MixEig<NS, NT
     const double t = d_energy_temperature<NS>(m, Y, (double)(Q[4] * ri - ek), E.t, mwi);
     pv[4] = (R)((double)rho * kRuD * t * mwi);
     double cpi[NS];
-    d_cp<NS>(t, m, cpi);
+    d_cp<NS>(t, fresh(m), cpi);
     double cp = 0, cv = 0;
 #pragma unroll
     for (int l = 0; l < NS; ++l) {
       cp += cpi[l] * Y[l];
-      cv += Y[l] * (cpi[l] - kRuD / m->mw[l]);
+      cv += Y[l] * (cpi[l] - m->r_mw[l]);
     }
     pv[NV + 1] = (R)(cp / cv);
   } else {
@@ -508,44 +508,76 @@ __device__ inline bool mix_side_ok(const R *pv) {
 // Where either reconstructed side is not admissible (WENO overshoot at a
 // jump no candidate stencil avoids, e.g. a jet port two cells wide), both
 // sides take the first-order values of the two adjacent points.
+// Register use: the scalar waves Q_{5+l} - Y~_l Q_0 are formed and
+// interpolated one scalar at a time (their stencil is read when needed), and
+// the acoustic rows keep only the 6 x 5 flow rows of the stencil, so the
+// 6 x NV amplitude block of the whole stencil is never live at once; the
+// arithmetic and its order are those of mix_left / weno5 per component.
 template <int NS, int NT, typename R>
 __device__ inline void aweno_mix(const ZoneView &z, const DMech *__restrict__ m, int64_t p0, int64_t st,
                                  const R *mk0, const R *mk1, R *pl, R *pr) {
   constexpr int NV = 5 + NS + NT;
-  auto load = [&](int s, R *Q) {
-    const int64_t q = p0 + (s - 2) * st;
-#pragma unroll
-    for (int l = 0; l < NV; ++l) Q[l] = (R)z.cv[(int64_t)l * z.sz + q];
-    Q[NV] = (R)z.bv[4 * z.sz + q];
-  };
   MixEig<NS, NT, R> E;
-  R wv[6][NV];
   {
     R cl[NV + 1], cr[NV + 1];
-    load(2, cl);
-    load(3, cr);
+#pragma unroll
+    for (int l = 0; l < NV; ++l) {
+      cl[l] = (R)z.cv[(int64_t)l * z.sz + p0];
+      cr[l] = (R)z.cv[(int64_t)l * z.sz + p0 + st];
+    }
+    cl[NV] = (R)z.bv[4 * z.sz + p0];
+    cr[NV] = (R)z.bv[4 * z.sz + p0 + st];
     mix_roe_state<NS, NT, R>(m, cl, cr, mk0, mk1, E);
-    mix_left<NS, NT, R>(E, cl, wv[2]);
-    mix_left<NS, NT, R>(E, cr, wv[3]);
+  }
+  // flow rows of the stencil, then (in place) their five wave amplitudes
+  R q[6][5];
+  R c1[6];
+#pragma unroll
+  for (int s = 0; s < 6; ++s) {
+    const int64_t p = p0 + (s - 2) * st;
+#pragma unroll
+    for (int c = 0; c < 5; ++c) q[s][c] = (R)z.cv[(int64_t)c * z.sz + p];
+    c1[s] = E.gm1 * (E.ek * q[s][0] - E.u * q[s][1] - E.v * q[s][2] - E.w * q[s][3] + q[s][4]) * rcp(E.c2);
+  }
+  R wm[NV], wp[NV];
+  // scalar waves, one scalar at a time; the species also complete c1
+#pragma unroll
+  for (int l = 0; l < NS + NT; ++l) {
+    R v[6];
+#pragma unroll
+    for (int s = 0; s < 6; ++s) {
+      const R ql = (R)z.cv[(int64_t)(5 + l) * z.sz + p0 + (s - 2) * st];
+      if (l < NS) c1[s] += E.a[l] * ql;
+      v[s] = ql - E.y[l] * q[s][0];
+    }
+    weno5_vals(v, wm[5 + l], wp[5 + l]);
   }
 #pragma unroll
   for (int s = 0; s < 6; ++s) {
-    if (s == 2 || s == 3) continue;
-    R Q[NV + 1];
-    load(s, Q);
-    mix_left<NS, NT, R>(E, Q, wv[s]);
+    const R *Q = q[s];
+    const R c2v = (E.nx * Q[1] + E.ny * Q[2] + E.nz * Q[3] - E.U * Q[0]) * E.ic;
+    const R c3 = Q[0] - c1[s];
+    const R w0 = R(0.5) * (c1[s] - c2v);
+    const R w1 = E.nx * c3 - ((E.nz * E.v - E.ny * E.w) * Q[0] - E.nz * Q[2] + E.ny * Q[3]) * E.ic;
+    const R w2 = E.ny * c3 - ((E.nx * E.w - E.nz * E.u) * Q[0] - E.nx * Q[3] + E.nz * Q[1]) * E.ic;
+    const R w3 = E.nz * c3 - ((E.ny * E.u - E.nx * E.v) * Q[0] - E.ny * Q[1] + E.nx * Q[2]) * E.ic;
+    const R w4 = R(0.5) * (c1[s] + c2v);
+    q[s][0] = w0;
+    q[s][1] = w1;
+    q[s][2] = w2;
+    q[s][3] = w3;
+    q[s][4] = w4;
   }
-  R wm[NV], wp[NV];
 #pragma unroll
-  for (int l = 0; l < NV; ++l) {
-    const R v[6] = {wv[0][l], wv[1][l], wv[2][l], wv[3][l], wv[4][l], wv[5][l]};
-    weno5_vals(v, wm[l], wp[l]);
+  for (int c = 0; c < 5; ++c) {
+    const R v[6] = {q[0][c], q[1][c], q[2][c], q[3][c], q[4][c], q[5][c]};
+    weno5_vals(v, wm[c], wp[c]);
   }
-  R q[NV];
-  mix_right<NS, NT, R>(E, wm, q);
-  mix_side<NS, NT, R>(m, E, q, pl);
-  mix_right<NS, NT, R>(E, wp, q);
-  mix_side<NS, NT, R>(m, E, q, pr);
+  R qs[NV];
+  mix_right<NS, NT, R>(E, wm, qs);
+  mix_side<NS, NT, R>(m, E, qs, pl);
+  mix_right<NS, NT, R>(E, wp, qs);
+  mix_side<NS, NT, R>(m, E, qs, pr);
   if (!(mix_side_ok<NS, NT, R>(pl) && mix_side_ok<NS, NT, R>(pr))) recon_pv<NS, NT, R>(z, 1, m, p0, st, pl, pr);
 }
 
@@ -651,11 +683,11 @@ __device__ inline void hllc(const DMech *__restrict__ m, const R *pl, const R *p
     }
     const double t = (double)(rlc * (pl[4] * ipl) + rrc * (pr[4] * ipr)) / (kRuD * mwi);
     double h[NS], c[NS], cpp = 0, cvv = 0;
-    d_enthalpy_cp<NS>(t, m, h, c);
+    d_enthalpy_cp<NS>(t, fresh(m), h, c);
 #pragma unroll
     for (int l = 0; l < NS; ++l) {
       cpp += c[l] * svm[l];
-      cvv += svm[l] * (c[l] - kRuD / m->mw[l]);
+      cvv += svm[l] * (c[l] - m->r_mw[l]);
     }
     ct = (R)sqrt(cpp / cvv * kRuD * mwi * t);
   }
@@ -782,11 +814,11 @@ __device__ inline void roe(const DMech *__restrict__ m, const R *pl, const R *pr
     for (int l = 0; l < NS; ++l) mwi += (double)svm[l] / m->mw[l];
     const double t = (double)((dl * (pl[4] / pl[0]) + dr * (pr[4] / pr[0])) * id) / (kRuD * mwi);
     double hh[NS], cc[NS], cp = 0, cv = 0;
-    d_enthalpy_cp<NS>(t, m, hh, cc);
+    d_enthalpy_cp<NS>(t, fresh(m), hh, cc);
 #pragma unroll
     for (int l = 0; l < NS; ++l) {
       cp += cc[l] * (double)svm[l];
-      cv += (double)svm[l] * (cc[l] - kRuD / m->mw[l]);
+      cv += (double)svm[l] * (cc[l] - m->r_mw[l]);
       hi[l] = (R)hh[l];
     }
     gamma = (R)(cp / cv);
@@ -972,7 +1004,7 @@ __device__ inline void viscous_face(const ZoneView &z, const DParams &P, const D
       }
     }
     double h[NS];
-    d_enthalpy<NS>(0.5 * (z.bv[5 * z.sz + p0] + z.bv[5 * z.sz + p1]), m, h);
+    d_enthalpy<NS>(0.5 * (z.bv[5 * z.sz + p0] + z.bv[5 * z.sz + p1]), fresh(m), h);
 #pragma unroll
     for (int l = 0; l < NS; ++l) {
       const R df = dif[l] * (ddf[l] - mw_tot * yk[l] * sum_gy_w) - yk[l] * corr;
@@ -1076,9 +1108,11 @@ __device__ inline void riemann_viscous(const ZoneView &z, const DParams &P, cons
 }
 
 // F(face) = inviscid - viscous; G(face) = CDS flux-form term (AWENO only),
-// every stencil value read from memory.
+// every stencil value read from memory.  Always inlined: as an outlined call
+// (several kernels use it) its arrays and the kernel arguments went to the
+// stack.
 template <int NS, int NT, int SCHEME, typename R>
-__device__ inline void face_flux(const ZoneView &z, const DParams &P, const DMech *__restrict__ m, int dir, int64_t p0,
+__device__ __attribute__((always_inline)) inline void face_flux(const ZoneView &z, const DParams &P, const DMech *__restrict__ m, int dir, int64_t p0,
                                  int64_t st, bool want_g, R *F, R *G) {
   constexpr int NV = 5 + NS + NT;
   constexpr int NG = CdsWidth<NS, NT, SCHEME>::value;

@@ -154,35 +154,12 @@ __global__ void __launch_bounds__(kTB) k_dplur_enthalpy(ZoneView z, const DMech 
   z.dplur_h[(int64_t)NS * z.sz0 + q] = h + 0.5 * z.vel[p] * z.vel[p];
 }
 
-// DPLUR.cuh:139-257: dqk from the neighbours' current dq (interior only)
+// DPLUR.cuh:139-257, the end of one cell's inner sweep: dqk from dq0 and the
+// neighbour sum tot (point-implicit species rows, implicit SST source)
 template <int NS, int NT>
-__global__ void __launch_bounds__(kTB) k_dplur_inner(ZoneView z, const DMech *__restrict__ m, int fr, int chem,
-                                                     int timp) {
+__device__ inline void dplur_finish(const ZoneView &z, int i, int j, int k, const double *tot, int fr, int chem,
+                                    int timp) {
   constexpr int NV = 5 + NS + NT;
-  int i, j, k;
-  if (!cell_index(z, i, j, k)) return;
-  double tot[NV], jd[NV];
-#pragma unroll
-  for (int l = 0; l < NV; ++l) tot[l] = 0;
-  const int c[3] = {i, j, k}, ext[3] = {z.mx, z.my, z.mz};
-#pragma unroll
-  for (int dd = 0; dd < 3; ++dd) {
-    if (c[dd] > 0) {
-      const int ni = i - (dd == 0), nj = j - (dd == 1), nk = k - (dd == 2);
-      jac_times_dq<NS, NT>(z, m, ni, nj, nk, dd, z.inv_sr[(int64_t)dd * z.sz0 + z.at0(ni, nj, nk)], jd);
-#pragma unroll
-      for (int l = 0; l < NV; ++l) tot[l] += 0.5 * jd[l];
-    }
-  }
-#pragma unroll
-  for (int dd = 0; dd < 3; ++dd) {
-    if (c[dd] != ext[dd] - 1) {
-      const int ni = i + (dd == 0), nj = j + (dd == 1), nk = k + (dd == 2);
-      jac_times_dq<NS, NT>(z, m, ni, nj, nk, dd, -z.inv_sr[(int64_t)dd * z.sz0 + z.at0(ni, nj, nk)], jd);
-#pragma unroll
-      for (int l = 0; l < NV; ++l) tot[l] -= 0.5 * jd[l];
-    }
-  }
   const int64_t q = z.at0(i, j, k);
   const double dt = z.dt_local[q];
   const double diag = 1 + dt * (z.inv_sr[q] + z.inv_sr[z.sz0 + q] + z.inv_sr[2 * z.sz0 + q]);
@@ -214,6 +191,43 @@ __global__ void __launch_bounds__(kTB) k_dplur_inner(ZoneView z, const DMech *__
     const double dg = timp ? diag - dt * z.turb_jac[(int64_t)l * z.sz0 + q] : diag;
     z.dqk[(int64_t)c * z.sz0 + q] = z.dq0[(int64_t)c * z.sz0 + q] + dt * tot[c] / dg;
   }
+}
+
+// the i- and j-neighbour term of one side: tot += s 0.5 (A +- lambda I) dq
+template <int NS, int NT>
+__device__ inline void dplur_add(const ZoneView &z, const DMech *__restrict__ m, int i, int j, int k, int dir,
+                                 double sign, double *tot) {
+  constexpr int NV = 5 + NS + NT;
+  double jd[NV];
+  jac_times_dq<NS, NT>(z, m, i, j, k, dir, sign * z.inv_sr[(int64_t)dir * z.sz0 + z.at0(i, j, k)], jd);
+  if (sign > 0) {
+#pragma unroll
+    for (int l = 0; l < NV; ++l) tot[l] += 0.5 * jd[l];
+  } else {
+#pragma unroll
+    for (int l = 0; l < NV; ++l) tot[l] -= 0.5 * jd[l];
+  }
+}
+
+// DPLUR.cuh:139-257: dqk from the neighbours' current dq (interior only), one
+// thread per cell
+template <int NS, int NT>
+__global__ void __launch_bounds__(kTB) k_dplur_inner(ZoneView z, const DMech *__restrict__ m, int fr, int chem,
+                                                     int timp) {
+  constexpr int NV = 5 + NS + NT;
+  int i, j, k;
+  if (!cell_index(z, i, j, k)) return;
+  double tot[NV];
+#pragma unroll
+  for (int l = 0; l < NV; ++l) tot[l] = 0;
+  const int c[3] = {i, j, k}, ext[3] = {z.mx, z.my, z.mz};
+#pragma unroll
+  for (int dd = 0; dd < 3; ++dd)
+    if (c[dd] > 0) dplur_add<NS, NT>(z, m, i - (dd == 0), j - (dd == 1), k - (dd == 2), dd, 1.0, tot);
+#pragma unroll
+  for (int dd = 0; dd < 3; ++dd)
+    if (c[dd] != ext[dd] - 1) dplur_add<NS, NT>(z, m, i + (dd == 0), j + (dd == 1), k + (dd == 2), dd, -1.0, tot);
+  dplur_finish<NS, NT>(z, i, j, k, tot, fr, chem, timp);
 }
 
 // DPLUR.cu:20-52 set_dq_to_0 over one wall range (interior cells of the wall plane)

@@ -714,7 +714,7 @@ __global__ void __launch_bounds__(kTB) k_properties(ZoneView z, DParams P, const
       Yk[l] = z.sv[(int64_t)l * z.sz + p];
       mwi += Yk[l] / m->mw[l];
       cpt += Yk[l] * cp[l];
-      cvt += Yk[l] * (cp[l] - kRuD / m->mw[l]);
+      cvt += Yk[l] * (cp[l] - m->r_mw[l]);
     }
     const double mw = 1 / mwi;
     gam = cpt / cvt;
