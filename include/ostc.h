@@ -61,6 +61,8 @@ const char *ostc_last_error(void);
  * grammar and derived keys, and (species != 0) the CHEMKIN, therm and tran
  * files named there; selects HIP device `device` (device < 0 opens the
  * solver host-only: setup files, tables and grid metrics, no device work).
+ * Multi-species mechanisms of 4 or 9 species (the counts the kernels are
+ * compiled for) with at most 24 reactions; other counts are an error.
  * Replaces: Parameter::Parameter (Parameter.cpp:8-52), Species/Reaction
  * constructors (ChemData.cpp:11-632), DParameter::DParameter (DParameter.cu:7-159). */
 int ostc_init(const char *case_dir, int device, ostc_solver **out);
@@ -231,7 +233,8 @@ int ostc_set_flux_precision(ostc_solver *s, int fp32);
  * rho[n], T[n], Y[n_spec*n] (component-major) in, dY[n_spec*n], dT[n] out.
  * All pointers are DEVICE pointers.  mode 0 = FP64 point-implicit integrator;
  * mode 1 = trained MLP surrogate with per-cell fallback to mode 0
- * (*n_fallback receives the fallback count).  New entry point: the reference
+ * (*n_fallback receives the fallback count; 9-species mechanism only).
+ * Needs a reacting mechanism.  New entry point: the reference
  * reaches this arithmetic only inside compute_source + EPI
  * (FiniteRateChem.cu:7-63, 234-256). */
 int ostc_chem_batch(ostc_solver *s, int64_t n, const double *rho, const double *T, const double *Y, double dt,

@@ -823,7 +823,8 @@ int ostc_init(const char *case_dir, int device, ostc_solver **out) {
   // interpolation (the reference's mixture AWENO is an empty stub,
   // AWENO.cu:10-15; this build defines it in flux_device.h)
   if (n_turb && !P.viscous) return fail("SST needs the viscous flux (viscous_order 2)");
-  if (P.mix && P.ns != 9) return fail("this build is compiled for 9-species mechanisms");
+  if (P.mix && !species_supported(P.ns))
+    return fail("a " + std::to_string(P.ns) + "-species mechanism: the kernels are compiled for 4 and 9 species");
   if (device < 0) {  // host-only: setup files, tables and grid metrics, no device work
     *out = s.release();
     return 0;
@@ -1791,15 +1792,24 @@ int ostc_set_flux_precision(ostc_solver *s, int fp32) {
   return 0;
 }
 
+// the chemistry entry points need a reacting mechanism of a compiled species
+// count; the surrogate (mode 1) is built for the 9-species mechanism
+static int need_chem(const ostc_solver *s, int mode) {
+  if (!(s->n_spec == 4 || s->n_spec == 9) || s->mech.n_reac <= 0)
+    return fail("chemistry needs a reacting mechanism of 4 or 9 species");
+  if (mode == 1 && s->n_spec != 9) return fail("the surrogate is built for the 9-species mechanism");
+  return 0;
+}
+
 int ostc_chem_batch(ostc_solver *s, int64_t n, const double *rho, const double *T, const double *Y, double dt,
                     double *dY, double *dT, int mode, int64_t *n_fallback) {
   if (!s) return fail("null solver");
-  if (s->n_spec != 9 || s->mech.n_reac <= 0) return fail("chemistry batch needs the 9-species reacting mechanism");
+  if (need_chem(s, mode)) return 1;
   if (n < 0) return fail("negative cell count");
   HIPCHK(hipSetDevice(s->device));
   if (mode == 0) {
     ostc_solver::Timer t(s, "chem_batch_fp64");
-    HIPCHK(launch_chem_batch(s->d_mech, s->mech.n_reac, n, rho, T, Y, dt, dY, dT, s->stream));
+    HIPCHK(launch_chem_batch(s->d_mech, s->mech.n_reac, n, rho, T, Y, dt, dY, dT, s->stream, s->n_spec));
     if (n_fallback) *n_fallback = 0;
   } else if (mode == 1) {
     if (!s->sur || !surrogate_ready(s->sur)) return fail("no trained surrogate: train or load one first");
@@ -1820,14 +1830,14 @@ int ostc_chem_batch_noalloc(ostc_solver *s, int64_t n, const double *rho, const 
                             double dt, double *dY, double *dT, int mode, int64_t *n_fallback) {
   if (!s) return fail("null solver");
   if (s->device < 0) return fail("solver has no device");
-  if (s->n_spec != 9 || s->mech.n_reac <= 0) return fail("chemistry batch needs the 9-species reacting mechanism");
+  if (need_chem(s, mode)) return 1;
   if (n < 0) return fail("negative cell count");
   if (n > kNoAllocCells)
     return fail("ostc_chem_batch_noalloc takes at most " + std::to_string(kNoAllocCells) + " cells, got " +
                 std::to_string(n));
   HIPCHK(hipSetDevice(s->device));
   if (mode == 0) {
-    HIPCHK(launch_chem_batch(s->d_mech, s->mech.n_reac, n, rho, T, Y, dt, dY, dT, s->stream));
+    HIPCHK(launch_chem_batch(s->d_mech, s->mech.n_reac, n, rho, T, Y, dt, dY, dT, s->stream, s->n_spec));
     if (n_fallback) *n_fallback = 0;
   } else if (mode == 1) {
     if (!s->sur || !surrogate_ready(s->sur)) return fail("no trained surrogate: train or load one first");
@@ -1845,7 +1855,7 @@ int ostc_chem_batch_noalloc(ostc_solver *s, int64_t n, const double *rho, const 
 int ostc_chem_states(ostc_solver *s, int blk, double *rho, double *T, double *Y) {
   if (check_blk(s, blk)) return 1;
   if (!rho || !T || !Y) return fail("null argument");
-  if (s->n_spec != 9) return fail("chemistry states need the 9-species mechanism");
+  if (!s->P.mix) return fail("chemistry states need a multi-species mechanism");
   HIPCHK(hipSetDevice(s->device));
   HIPCHK(launch_chem_gather(s->blocks[blk].z, s->P, rho, T, Y, s->stream));
   HIPCHK(hipStreamSynchronize(s->stream));
@@ -1855,9 +1865,9 @@ int ostc_chem_states(ostc_solver *s, int blk, double *rho, double *T, double *Y)
 int ostc_set_steady_chem(ostc_solver *s, double dt, int mode) {
   if (!s) return fail("null solver");
   if (dt > 0) {
-    if (!(s->P.mix && s->P.reaction == 1) || s->n_spec != 9 || s->mech.n_reac <= 0)
-      return fail("inline chemistry needs the 9-species reacting mechanism");
+    if (!(s->P.mix && s->P.reaction == 1)) return fail("inline chemistry needs a reacting mechanism");
     if (mode != 0 && mode != 1) return fail("mode must be 0 (point-implicit) or 1 (surrogate)");
+    if (need_chem(s, mode)) return 1;
     if (mode == 1 && (!s->sur || !surrogate_ready(s->sur))) return fail("no trained surrogate: train or load one first");
     s->chem_split = true;
     s->steady_chem_dt = dt;
@@ -2087,7 +2097,7 @@ int chem_interior(ostc_solver *s, Block &b, double dt, int mode, int64_t *nf_out
     ostc_solver::Timer t(s, "chem_step");
     HIPCHK(launch_chem_gather(b.z, s->P, rho, T, Y, s->stream));
     if (mode == 0) {
-      HIPCHK(launch_chem_batch(s->d_mech, s->mech.n_reac, n, rho, T, Y, dt, dY, dT, s->stream));
+      HIPCHK(launch_chem_batch(s->d_mech, s->mech.n_reac, n, rho, T, Y, dt, dY, dT, s->stream, s->n_spec));
     } else {
       std::string err;
       if (surrogate_infer(s->sur, s->d_mech, n, rho, T, Y, dt, dY, dT, &nf, s->stream, err)) return fail(err);
@@ -2100,9 +2110,9 @@ int chem_interior(ostc_solver *s, Block &b, double dt, int mode, int64_t *nf_out
 
 int ostc_chem_step(ostc_solver *s, int blk, double dt, int mode, int64_t *n_fallback) {
   if (check_blk(s, blk)) return 1;
-  if (s->n_spec != 9 || s->mech.n_reac <= 0) return fail("the chemistry step needs the 9-species reacting mechanism");
   if (!(dt > 0)) return fail("dt must be positive");
   if (mode != 0 && mode != 1) return fail("mode must be 0 (point-implicit) or 1 (surrogate)");
+  if (need_chem(s, mode)) return 1;
   if (mode == 1 && (!s->sur || !surrogate_ready(s->sur))) return fail("no trained surrogate: train or load one first");
   HIPCHK(hipSetDevice(s->device));
   Block &b = s->blocks[blk];

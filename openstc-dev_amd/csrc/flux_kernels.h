@@ -3,11 +3,34 @@
 
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
 #include <vector>
 
 #include "device_common.h"
 
 namespace ostc {
+
+// Species counts the mixture kernels are compiled for: the 9-species H2-O2
+// mechanism and the 4-species H2 / O2 / H2O / N2 mixing mechanism; 0 is
+// air.  with_species(ns, f) calls f(std::integral_constant<int, NS>{}) for
+// a compiled count and returns hipErrorNotSupported otherwise;
+// with_mixture(ns, f) the same without air.
+inline bool species_supported(int ns) { return ns == 0 || ns == 4 || ns == 9; }
+
+template <typename F>
+hipError_t with_mixture(int ns, F &&f) {
+  switch (ns) {
+    case 4: return f(std::integral_constant<int, 4>{});
+    case 9: return f(std::integral_constant<int, 9>{});
+    default: return hipErrorNotSupported;
+  }
+}
+
+template <typename F>
+hipError_t with_species(int ns, F &&f) {
+  if (ns == 0) return f(std::integral_constant<int, 0>{});
+  return with_mixture(ns, f);
+}
 
 struct BCDesc {
   int type, face, dir;
@@ -86,8 +109,10 @@ hipError_t launch_inner_exchange(const ZoneView &z, const ZoneView &tz, const DP
 hipError_t launch_metrics(int mx, int my, int mz, int g, const double *X, const double *Y, const double *Z,
                           double *jac, double *met, unsigned char *bad, int *interior_bad, hipStream_t s,
                           int conservative = 0, int aweno = 0, double *fsp_scratch = nullptr);
+// one point-implicit chemistry step of n independent cells of an ns-species
+// mechanism (4 or 9)
 hipError_t launch_chem_batch(const DMech *m, int nr, int64_t n, const double *rho, const double *T, const double *Y,
-                             double dt, double *dY, double *dT, hipStream_t s);
+                             double dt, double *dY, double *dT, hipStream_t s, int ns = 9);
 // ---- steady path (implicit.hip)
 // point-implicit chemistry on dq with the local dt (EPI or DA per chemSrcMethod)
 hipError_t launch_chem_implicit(const ZoneView &z, const DParams &P, hipStream_t s);

@@ -12,21 +12,14 @@ hipError_t launch_flux_sweep_sst(const ZoneView &z, const DParams &P, const DMec
                                  hipStream_t s) {
   const bool aweno = P.inviscid_type == 1;
   if (P.nt != 2) return hipErrorNotSupported;
-  if (P.ns == 0) {
+  return with_species(P.ns, [&](auto c) -> hipError_t {
+    constexpr int NS = decltype(c)::value;
     if (aweno)
-      return fp32 ? launch_sweep_t<0, 2, 1, float>(z, P, m, dir, init, s)
-                  : launch_sweep_t<0, 2, 1, double>(z, P, m, dir, init, s);
-    return fp32 ? launch_sweep_t<0, 2, 0, float>(z, P, m, dir, init, s)
-                : launch_sweep_t<0, 2, 0, double>(z, P, m, dir, init, s);
-  }
-  if (P.ns == 9) {
-    if (aweno)
-      return fp32 ? launch_sweep_t<9, 2, 1, float>(z, P, m, dir, init, s)
-                  : launch_sweep_t<9, 2, 1, double>(z, P, m, dir, init, s);
-    return fp32 ? launch_sweep_t<9, 2, 0, float>(z, P, m, dir, init, s)
-                : launch_sweep_t<9, 2, 0, double>(z, P, m, dir, init, s);
-  }
-  return hipErrorNotSupported;
+      return fp32 ? launch_sweep_t<NS, 2, 1, float>(z, P, m, dir, init, s)
+                  : launch_sweep_t<NS, 2, 1, double>(z, P, m, dir, init, s);
+    return fp32 ? launch_sweep_t<NS, 2, 0, float>(z, P, m, dir, init, s)
+                : launch_sweep_t<NS, 2, 0, double>(z, P, m, dir, init, s);
+  });
 }
 
 

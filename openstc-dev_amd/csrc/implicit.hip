@@ -329,9 +329,10 @@ inline unsigned nb(int64_t n) { return (unsigned)((n + kTB - 1) / kTB); }
 hipError_t launch_chem_implicit(const ZoneView &z, const DParams &P, hipStream_t s) {
   const int64_t n = (int64_t)z.mx * z.my * z.mz;
   if (!(P.mix && P.reaction == 1) || (P.chemSrcMethod != 1 && P.chemSrcMethod != 2)) return hipSuccess;
-  if (P.ns != 9) return hipErrorNotSupported;
-  hipLaunchKernelGGL(k_chem_implicit<9>, dim3(nb(n)), dim3(kTB), 0, s, z, P.chemSrcMethod);
-  return hipGetLastError();
+  return with_mixture(P.ns, [&](auto c) {
+    hipLaunchKernelGGL(k_chem_implicit<decltype(c)::value>, dim3(nb(n)), dim3(kTB), 0, s, z, P.chemSrcMethod);
+    return hipGetLastError();
+  });
 }
 
 hipError_t launch_dplur(const ZoneView &z, const DParams &P, const DMech *m, int inner_steps,
@@ -340,25 +341,23 @@ hipError_t launch_dplur(const ZoneView &z, const DParams &P, const DMech *m, int
   const int nv = P.nv;
   const int fr = P.mix && P.reaction == 1;
   if (!z.inv_sr || !z.dq0 || !z.dqk) return hipErrorInvalidValue;
-  if ((P.ns != 0 && P.ns != 9) || (P.nt != 0 && P.nt != 2)) return hipErrorNotSupported;
+  if (!species_supported(P.ns) || (P.nt != 0 && P.nt != 2)) return hipErrorNotSupported;
   const int timp = P.nt && P.turb_implicit == 1;
   if (timp && !z.turb_jac) return hipErrorInvalidValue;
   const dim3 g(nb(n)), bk(kTB);
-  if (P.nt == 0) {
-    if (P.ns == 0)
-      hipLaunchKernelGGL((k_dplur_dq0<0, 0>), g, bk, 0, s, z, fr, P.chemSrcMethod, 0);
+  hipError_t e = with_species(P.ns, [&](auto c) {
+    constexpr int NS = decltype(c)::value;
+    if (P.nt == 0)
+      hipLaunchKernelGGL((k_dplur_dq0<NS, 0>), g, bk, 0, s, z, fr, P.chemSrcMethod, 0);
     else
-      hipLaunchKernelGGL((k_dplur_dq0<9, 0>), g, bk, 0, s, z, fr, P.chemSrcMethod, 0);
-  } else {
-    if (P.ns == 0)
-      hipLaunchKernelGGL((k_dplur_dq0<0, 2>), g, bk, 0, s, z, fr, P.chemSrcMethod, timp);
-    else
-      hipLaunchKernelGGL((k_dplur_dq0<9, 2>), g, bk, 0, s, z, fr, P.chemSrcMethod, timp);
-  }
-  if (P.ns == 9) {
-    if (!z.dplur_h) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_dplur_enthalpy<9>, g, bk, 0, s, z, m);
-  }
+      hipLaunchKernelGGL((k_dplur_dq0<NS, 2>), g, bk, 0, s, z, fr, P.chemSrcMethod, timp);
+    if constexpr (NS > 0) {
+      if (!z.dplur_h) return hipErrorInvalidValue;
+      hipLaunchKernelGGL(k_dplur_enthalpy<NS>, g, bk, 0, s, z, m);
+    }
+    return hipGetLastError();
+  });
+  if (e != hipSuccess) return e;
   // The inner sweeps alternate between dq and dqk as input and output (the
   // reference copies dqk back into dq after each sweep); k_dq_back reads the
   // last sweep's output array (dqk for an odd count) and writes dq / dt, so
@@ -374,17 +373,15 @@ hipError_t launch_dplur(const ZoneView &z, const DParams &P, const DMech *m, int
       hipLaunchKernelGGL(k_zero_dq_range, dim3(nb(cnt)), dim3(kTB), 0, s, zi, nv, b.rs[0], b.re[0], b.rs[1], b.re[1],
                          b.rs[2], b.re[2]);
     }
-    if (P.nt == 0) {
-      if (P.ns == 0)
-        hipLaunchKernelGGL((k_dplur_inner<0, 0>), g, bk, 0, s, zi, m, fr, P.chemSrcMethod, 0);
+    e = with_species(P.ns, [&](auto c) {
+      constexpr int NS = decltype(c)::value;
+      if (P.nt == 0)
+        hipLaunchKernelGGL((k_dplur_inner<NS, 0>), g, bk, 0, s, zi, m, fr, P.chemSrcMethod, 0);
       else
-        hipLaunchKernelGGL((k_dplur_inner<9, 0>), g, bk, 0, s, zi, m, fr, P.chemSrcMethod, 0);
-    } else {
-      if (P.ns == 0)
-        hipLaunchKernelGGL((k_dplur_inner<0, 2>), g, bk, 0, s, zi, m, fr, P.chemSrcMethod, timp);
-      else
-        hipLaunchKernelGGL((k_dplur_inner<9, 2>), g, bk, 0, s, zi, m, fr, P.chemSrcMethod, timp);
-    }
+        hipLaunchKernelGGL((k_dplur_inner<NS, 2>), g, bk, 0, s, zi, m, fr, P.chemSrcMethod, timp);
+      return hipGetLastError();
+    });
+    if (e != hipSuccess) return e;
   }
   hipLaunchKernelGGL(k_dq_back, dim3(nb(n)), dim3(kTB), 0, s, z, (const double *)((inner_steps & 1) ? z.dqk : z.dq),
                      nv);
@@ -393,13 +390,10 @@ hipError_t launch_dplur(const ZoneView &z, const DParams &P, const DMech *m, int
 
 hipError_t launch_update_bv(const ZoneView &z, const DParams &P, const DMech *m, hipStream_t s) {
   const int64_t n = (int64_t)z.mx * z.my * z.mz;
-  if (P.ns == 0)
-    hipLaunchKernelGGL(k_update_bv<0>, dim3(nb(n)), dim3(kTB), 0, s, z, P, m);
-  else if (P.ns == 9)
-    hipLaunchKernelGGL(k_update_bv<9>, dim3(nb(n)), dim3(kTB), 0, s, z, P, m);
-  else
-    return hipErrorNotSupported;
-  return hipGetLastError();
+  return with_species(P.ns, [&](auto c) {
+    hipLaunchKernelGGL(k_update_bv<decltype(c)::value>, dim3(nb(n)), dim3(kTB), 0, s, z, P, m);
+    return hipGetLastError();
+  });
 }
 
 }  // namespace ostc

@@ -227,26 +227,20 @@ hipError_t launch_wall_distance(const ZoneView &z, const double *xyz, const doub
 hipError_t launch_init_mut(const ZoneView &z, const DParams &P, const DMech *m, hipStream_t s) {
   if (!z.mut || !z.wdist || P.nt != 2) return hipErrorInvalidValue;
   const int64_t n = (int64_t)(z.mx + 2) * (z.my + 2) * (z.mz + 2);
-  if (P.ns == 0)
-    hipLaunchKernelGGL(k_init_mut<0>, dim3(nblk(n, kTB)), dim3(kTB), 0, s, z, m);
-  else if (P.ns == 9)
-    hipLaunchKernelGGL(k_init_mut<9>, dim3(nblk(n, kTB)), dim3(kTB), 0, s, z, m);
-  else
-    return hipErrorNotSupported;
-  return hipGetLastError();
+  return with_species(P.ns, [&](auto c) {
+    hipLaunchKernelGGL(k_init_mut<decltype(c)::value>, dim3(nblk(n, kTB)), dim3(kTB), 0, s, z, m);
+    return hipGetLastError();
+  });
 }
 
 hipError_t launch_sst(const ZoneView &z, const DParams &P, hipStream_t s) {
   if (!z.mut || !z.tcond || !z.wdist || P.nt != 2 || (P.turb_implicit == 1 && !z.turb_jac))
     return hipErrorInvalidValue;
   const int64_t n = (int64_t)z.mx * z.my * z.mz;
-  if (P.ns == 0)
-    hipLaunchKernelGGL(k_sst<0>, dim3(nblk(n, kTB)), dim3(kTB), 0, s, z, P);
-  else if (P.ns == 9)
-    hipLaunchKernelGGL(k_sst<9>, dim3(nblk(n, kTB)), dim3(kTB), 0, s, z, P);
-  else
-    return hipErrorNotSupported;
-  return hipGetLastError();
+  return with_species(P.ns, [&](auto c) {
+    hipLaunchKernelGGL(k_sst<decltype(c)::value>, dim3(nblk(n, kTB)), dim3(kTB), 0, s, z, P);
+    return hipGetLastError();
+  });
 }
 
 hipError_t launch_turb_implicit(const ZoneView &z, const DParams &P, hipStream_t s) {

@@ -985,26 +985,30 @@ inline unsigned nblk(int64_t n, int tb) { return (unsigned)((n + tb - 1) / tb); 
 
 hipError_t launch_chem_gather(const ZoneView &z, const DParams &P, double *rho, double *T, double *Y, hipStream_t s) {
   const int64_t n = (int64_t)z.mx * z.my * z.mz;
-  if (P.ns != 9) return hipErrorNotSupported;
-  hipLaunchKernelGGL(k_chem_gather<9>, dim3(nblk(n, kTB)), dim3(kTB), 0, s, z, rho, T, Y);
-  return hipGetLastError();
+  return with_mixture(P.ns, [&](auto c) {
+    hipLaunchKernelGGL(k_chem_gather<decltype(c)::value>, dim3(nblk(n, kTB)), dim3(kTB), 0, s, z, rho, T, Y);
+    return hipGetLastError();
+  });
 }
 
 hipError_t launch_chem_scatter(const ZoneView &z, const DParams &P, const DMech *m, const double *dY, const double *dT,
                                hipStream_t s) {
   const int64_t n = (int64_t)z.mx * z.my * z.mz;
-  if (P.ns != 9) return hipErrorNotSupported;
-  hipLaunchKernelGGL(k_chem_scatter<9>, dim3(nblk(n, kTB)), dim3(kTB), 0, s, z, m, dY, dT);
-  return hipGetLastError();
+  return with_mixture(P.ns, [&](auto c) {
+    hipLaunchKernelGGL(k_chem_scatter<decltype(c)::value>, dim3(nblk(n, kTB)), dim3(kTB), 0, s, z, m, dY, dT);
+    return hipGetLastError();
+  });
 }
 
 hipError_t launch_source(const ZoneView &z, const DParams &P, const DMech *m, bool accumulate, hipStream_t s) {
   const int64_t n = (int64_t)z.mx * z.my * z.mz;
-  if (P.ns != 9) return hipErrorNotSupported;
   if (P.chemSrcMethod == 1 && (P.nr <= 0 || P.nr > kParkMaxReac)) return hipErrorInvalidValue;
   const size_t park = P.chemSrcMethod == 1 ? (size_t)2 * P.nr * kSrcTB * sizeof(double) : 0;
-  hipLaunchKernelGGL(k_source<9>, dim3(nblk(n, kSrcTB)), dim3(kSrcTB), park, s, z, P, m, accumulate ? 1 : 0);
-  return hipGetLastError();
+  return with_mixture(P.ns, [&](auto c) {
+    hipLaunchKernelGGL(k_source<decltype(c)::value>, dim3(nblk(n, kSrcTB)), dim3(kSrcTB), park, s, z, P, m,
+                       accumulate ? 1 : 0);
+    return hipGetLastError();
+  });
 }
 
 hipError_t launch_local_dt(const ZoneView &z, const DParams &P, const StepScratch &w, double *dt_out, hipStream_t s) {
@@ -1022,13 +1026,10 @@ hipError_t launch_local_dt(const ZoneView &z, const DParams &P, const StepScratc
 hipError_t launch_rk_update(const ZoneView &z, const DParams &P, const DMech *m, const double *dt, int rk,
                             hipStream_t s) {
   const int64_t n = (int64_t)z.mx * z.my * z.mz;
-  if (P.ns == 0)
-    hipLaunchKernelGGL(k_rk_update<0>, dim3(nblk(n, kTB)), dim3(kTB), 0, s, z, P, m, dt, rk);
-  else if (P.ns == 9)
-    hipLaunchKernelGGL(k_rk_update<9>, dim3(nblk(n, kTB)), dim3(kTB), 0, s, z, P, m, dt, rk);
-  else
-    return hipErrorNotSupported;
-  return hipGetLastError();
+  return with_species(P.ns, [&](auto c) {
+    hipLaunchKernelGGL(k_rk_update<decltype(c)::value>, dim3(nblk(n, kTB)), dim3(kTB), 0, s, z, P, m, dt, rk);
+    return hipGetLastError();
+  });
 }
 
 hipError_t launch_bc(const ZoneView &z, const DParams &P, const DMech *m, const BCDesc &bc, bool with_cv,
@@ -1037,20 +1038,14 @@ hipError_t launch_bc(const ZoneView &z, const DParams &P, const DMech *m, const 
   if (n <= 0) return hipSuccess;
   if (n * (z.g + 1) > ((int64_t)1 << 32) - kTB) return hipErrorInvalidValue;  // 32-bit thread index
   const dim3 g(nblk(n * (z.g + 1), kTB)), b(kTB);
-  if (P.ns == 0) {
+  return with_species(P.ns, [&](auto c) {
+    constexpr int NS = decltype(c)::value;
     if (with_cv)
-      hipLaunchKernelGGL((k_bc<0, true>), g, b, 0, s, z, P, m, bc);
+      hipLaunchKernelGGL((k_bc<NS, true>), g, b, 0, s, z, P, m, bc);
     else
-      hipLaunchKernelGGL((k_bc<0, false>), g, b, 0, s, z, P, m, bc);
-  } else if (P.ns == 9) {
-    if (with_cv)
-      hipLaunchKernelGGL((k_bc<9, true>), g, b, 0, s, z, P, m, bc);
-    else
-      hipLaunchKernelGGL((k_bc<9, false>), g, b, 0, s, z, P, m, bc);
-  } else {
-    return hipErrorNotSupported;
-  }
-  return hipGetLastError();
+      hipLaunchKernelGGL((k_bc<NS, false>), g, b, 0, s, z, P, m, bc);
+    return hipGetLastError();
+  });
 }
 
 hipError_t launch_eliminate_k(const ZoneView &z, const DParams &P, bool with_cv, hipStream_t s) {
@@ -1067,20 +1062,14 @@ hipError_t launch_properties(const ZoneView &z, const DParams &P, const DMech *m
   const bool plane = k_planes_copied && P.dim == 2 && z.mz == 1;
   const int64_t n = (int64_t)(z.mx + 2) * (z.my + 2) * (plane ? 1 : z.mz + 2);
   const dim3 g(nblk(n, kTB)), b(kTB);
-  if (P.ns == 0) {
+  return with_species(P.ns, [&](auto c) {
+    constexpr int NS = decltype(c)::value;
     if (plane)
-      hipLaunchKernelGGL((k_properties<0, true>), g, b, 0, s, z, P, m);
+      hipLaunchKernelGGL((k_properties<NS, true>), g, b, 0, s, z, P, m);
     else
-      hipLaunchKernelGGL((k_properties<0, false>), g, b, 0, s, z, P, m);
-  } else if (P.ns == 9) {
-    if (plane)
-      hipLaunchKernelGGL((k_properties<9, true>), g, b, 0, s, z, P, m);
-    else
-      hipLaunchKernelGGL((k_properties<9, false>), g, b, 0, s, z, P, m);
-  } else {
-    return hipErrorNotSupported;
-  }
-  return hipGetLastError();
+      hipLaunchKernelGGL((k_properties<NS, false>), g, b, 0, s, z, P, m);
+    return hipGetLastError();
+  });
 }
 
 hipError_t launch_velocity(const ZoneView &z, hipStream_t s) {
@@ -1091,13 +1080,10 @@ hipError_t launch_velocity(const ZoneView &z, hipStream_t s) {
 
 hipError_t launch_cv_from_bv(const ZoneView &z, const DParams &P, const DMech *m, hipStream_t s) {
   const int64_t n = (int64_t)(z.mx + 2 * z.g) * (z.my + 2 * z.g) * (z.mz + 2 * z.g);
-  if (P.ns == 0)
-    hipLaunchKernelGGL(k_cv_from_bv<0>, dim3(nblk(n, kTB)), dim3(kTB), 0, s, z, m, P.nt);
-  else if (P.ns == 9)
-    hipLaunchKernelGGL(k_cv_from_bv<9>, dim3(nblk(n, kTB)), dim3(kTB), 0, s, z, m, P.nt);
-  else
-    return hipErrorNotSupported;
-  return hipGetLastError();
+  return with_species(P.ns, [&](auto c) {
+    hipLaunchKernelGGL(k_cv_from_bv<decltype(c)::value>, dim3(nblk(n, kTB)), dim3(kTB), 0, s, z, m, P.nt);
+    return hipGetLastError();
+  });
 }
 
 hipError_t launch_store_last(const ZoneView &z, hipStream_t s) {
@@ -1145,20 +1131,14 @@ hipError_t launch_inner_exchange(const ZoneView &z, const ZoneView &tz, const DP
   if (n <= 0) return hipSuccess;
   if (n > ((int64_t)1 << 31)) return hipErrorInvalidValue;
   const dim3 g(nblk(n, kTB)), b(kTB);
-  if (P.ns == 0) {
+  return with_species(P.ns, [&](auto c) {
+    constexpr int NS = decltype(c)::value;
     if (with_cv)
-      hipLaunchKernelGGL((k_inner_exchange<0, true>), g, b, 0, s, z, tz, m, f, P.nsc, P.nt);
+      hipLaunchKernelGGL((k_inner_exchange<NS, true>), g, b, 0, s, z, tz, m, f, P.nsc, P.nt);
     else
-      hipLaunchKernelGGL((k_inner_exchange<0, false>), g, b, 0, s, z, tz, m, f, P.nsc, P.nt);
-  } else if (P.ns == 9) {
-    if (with_cv)
-      hipLaunchKernelGGL((k_inner_exchange<9, true>), g, b, 0, s, z, tz, m, f, P.nsc, P.nt);
-    else
-      hipLaunchKernelGGL((k_inner_exchange<9, false>), g, b, 0, s, z, tz, m, f, P.nsc, P.nt);
-  } else {
-    return hipErrorNotSupported;
-  }
-  return hipGetLastError();
+      hipLaunchKernelGGL((k_inner_exchange<NS, false>), g, b, 0, s, z, tz, m, f, P.nsc, P.nt);
+    return hipGetLastError();
+  });
 }
 
 hipError_t launch_min_of(const double *in, int n, double *out, hipStream_t s) {
@@ -1172,13 +1152,15 @@ hipError_t launch_copy_dt(const double *src, double *dst, hipStream_t s) {
 }
 
 hipError_t launch_chem_batch(const DMech *m, int nr, int64_t n, const double *rho, const double *T, const double *Y,
-                             double dt, double *dY, double *dT, hipStream_t s) {
+                             double dt, double *dY, double *dT, hipStream_t s, int ns) {
   if (n <= 0) return hipSuccess;
   if (nr <= 0 || nr > kParkMaxReac) return hipErrorInvalidValue;
   const size_t park = (size_t)2 * nr * kChemTB * sizeof(double);
-  hipLaunchKernelGGL(k_chem_batch<9>, dim3(nblk(n, kChemTB)), dim3(kChemTB), park, s, m, n, n, (const int *)nullptr, rho,
-                     T, Y, dt, dY, dT);
-  return hipGetLastError();
+  return with_mixture(ns, [&](auto c) {
+    hipLaunchKernelGGL(k_chem_batch<decltype(c)::value>, dim3(nblk(n, kChemTB)), dim3(kChemTB), park, s, m, n, n,
+                       (const int *)nullptr, rho, T, Y, dt, dY, dT);
+    return hipGetLastError();
+  });
 }
 
 hipError_t launch_epi_sigma(const DMech *m, int64_t n, const double *rho, const double *T, const double *Y, double dt,

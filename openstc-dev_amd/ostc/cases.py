@@ -283,6 +283,18 @@ def c3_case(nx=512, ny=256, seed=1003, steps=20, dx=1e-4, delta=5e-4, viscous_or
                 seed=seed, steps=steps, dx=dx, delta=delta, y0=0.5 * (ny - 1) * dx)
 
 
+def mixing4_case(nx=48, ny=24, reconstruction=4, inviscid_scheme=4, steps=20):
+    """The mixing layer of c3_case with the reference's 4-species mixing
+    mechanism (chemistry/H2_mech_mixing.inp: H2, H2O, O2, N2) and no
+    reactions: the same streams, profile, scheme and boundaries, species
+    transport and mixture thermodynamics only."""
+    case = c3_case(nx, ny, steps=steps, inviscid_scheme=inviscid_scheme, reconstruction=reconstruction)
+    case["setup"].update({"int reaction": 0, "string mechanism_file": "chemistry/H2_mech_mixing.inp"})
+    case["scheme"] = dict(case["scheme"], reaction=0)
+    case["name"] = "mixing4"
+    return case
+
+
 def c3_fields(case, states, mw):
     """Initial bv (6, ...) and sv (9, ...) of the mixing layer: with
     f = (1 + tanh((y - y0) / delta)) / 2, u, T and Y blend from the lower

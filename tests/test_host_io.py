@@ -53,6 +53,40 @@ def test_mechanism_tables_match_reference_parser(tmp_path, tables):
     s.close()
 
 
+def test_mixing_mechanism_matches_reference_parser(tmp_path):
+    """The reference's 4-species mixing mechanism (H2, H2O, O2, N2) read
+    without reactions: the species tables equal the reference parser's
+    (tests/golden/h2_mixing_4sp_tables.txt) bit for bit."""
+    tables = po.load_tables(os.path.join(ROOT, "tests", "golden", "h2_mixing_4sp_tables.txt"))
+    case = cases.mixing4_case(24, 12)
+    root = str(tmp_path / "m4")
+    cases.write_case_dir(case, root)
+    s = ostc.Solver(root, device=-1)
+    assert (s.get_int("n_spec"), s.get_int("n_reac")) == (tables["n_spec"], tables["n_reac"]) == (4, 0)
+    assert s.get_int("n_var") == 9
+    for key in ("mw", "t_low", "t_mid", "t_high", "LJ_potent_inv", "vis_coeff", "high_coeff", "low_coeff",
+                "WjDivWi_to_One4th", "sqrt_WiDivWjPl1Mul8", "binary_diffusivity_coeff", "kb_over_eps_jk"):
+        ref = np.asarray(tables[key], dtype=np.float64)
+        np.testing.assert_array_equal(s.table(key, ref.size), ref, err_msg=key)
+    s.close()
+
+
+def test_uncompiled_species_count_is_refused(tmp_path):
+    """A mechanism whose species count has no compiled kernels fails at
+    ostc_init with a message naming the compiled counts."""
+    case = cases.mixing4_case(24, 12)
+    root = str(tmp_path / "m5")
+    cases.write_case_dir(case, root)
+    mech = os.path.join(root, "input_files", "chemistry", "five.inp")
+    with open(mech, "w") as f:
+        f.write("ELEMENTS\nH O N\nEND\nSPECIES\nH2 H2O O2 N2 OH\nEND\nREACTIONS\nEND\n")
+    setup = os.path.join(root, "input_files", "setup", "default_settings.txt")
+    txt = open(setup).read().replace("chemistry/H2_mech_mixing.inp", "chemistry/five.inp")
+    open(setup, "w").write(txt)
+    with pytest.raises(ostc.OstcError, match="compiled for 4 and 9 species"):
+        ostc.Solver(root, device=-1)
+
+
 def test_setup_derivations(tmp_path):
     s = ostc.Solver(_case_dir(tmp_path, species=0, reaction=0), device=-1)
     assert s.get_int("ngg") == 2
