@@ -675,7 +675,7 @@ int run_bcs(ostc_solver *s, Block &b, bool with_cv) {
 // SourceTerm.cuh:11-37 then the fluxes (SteadySim.cuh:73-78, RK.cuh:109-116):
 // the SST kernel (mu_t, lambda_t, k / omega sources) initialises dq, the
 // chemical source adds to it, the sweeps accumulate.
-int rhs_block(ostc_solver *s, Block &b) {
+int rhs_block(ostc_solver *s, Block &b, bool *dt_done = nullptr) {
   bool init = true;
   if (s->n_turb) {
     ostc_solver::Timer t(s, "sst");
@@ -690,7 +690,8 @@ int rhs_block(ostc_solver *s, Block &b) {
   static const char *names[3] = {"sweep_i", "sweep_j", "sweep_k"};
   for (int d = 0; d < s->P.dim; ++d) {
     ostc_solver::Timer t(s, names[d]);
-    HIPCHK(launch_flux_sweep(b.z, s->P, s->d_mech, d, init && d == 0, s->fp32, s->stream));
+    HIPCHK(launch_flux_sweep(b.z, s->P, s->d_mech, d, init && d == 0, s->fp32, s->stream,
+                             dt_done && d == 2 ? &s->w : nullptr, dt_done));
   }
   return 0;
 }
@@ -1208,9 +1209,8 @@ int ostc_finalize(ostc_solver *s) {
         alloc(&met, L.sz * 9) || alloc(&z.dq, L0.sz * nv) || alloc(&z.dt_local, L0.sz) ||
         alloc(&z.bv_last, L0.sz * 4) ||
         alloc(&z.chem_jac, s->P.chemSrcMethod == 1 ? L0.sz * ns * ns : (s->P.chemSrcMethod == 2 ? L0.sz * ns : 1)) ||
-        alloc(&z.inv_sr, L0.sz * 3) ||
         (s->prm.get_int("implicit_method", 0) == 1 &&
-         (alloc(&z.dq0, L0.sz * nv) || alloc(&z.dqk, L0.sz * nv) || (ns && alloc(&z.dplur_h, L0.sz * (ns + 1))))))
+         (alloc(&z.inv_sr, L0.sz * 3) || alloc(&z.dq0, L0.sz * nv) || alloc(&z.dqk, L0.sz * nv) || (ns && alloc(&z.dplur_h, L0.sz * (ns + 1))))))
       return 1;
     // SST: mu_t and lambda_t (ghost lambda_t stays zero: only the source
     // kernel writes it, on the interior), wall distance, source diagonal
@@ -1624,9 +1624,12 @@ int ostc_step_rk3(ostc_solver *s, int nsteps, double *dt_hist, double *res_hist)
                               s->stream));
       }
       for (int rk = 0; rk < 3; ++rk) {
+        // one block: the k-sweep computes the CFL dt where it can
+        bool dt_done = false;
+        const bool fold_dt = !fixed && s->blocks.size() == 1;
         for (auto &b : s->blocks)
-          if (rhs_block(s, b)) return 1;
-        if (!fixed && global_dt_device(s)) return 1;
+          if (rhs_block(s, b, fold_dt ? &dt_done : nullptr)) return 1;
+        if (!fixed && !dt_done && global_dt_device(s)) return 1;
         HIPCHK(launch_copy_dt(s->w.dt, d_dt + it * 3 + rk, s->stream));
         for (auto &b : s->blocks) {
           {

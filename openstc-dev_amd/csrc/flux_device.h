@@ -1383,10 +1383,13 @@ __global__ void __launch_bounds__((64 * JkRows<NS + NT, R>::value)) sweep_jk(Zon
 // variables and pressure (the characteristic stencil) and its CDS node flux
 // -- so a face reads its six stencil rows from LDS instead of six wavefronts
 // loading every row from memory.  Face fluxes pass to the next row as in
-// sweep_jk.
-template <int DIR, int kRows, typename R>
+// sweep_jk.  DT: the cell updates also compute the CFL time step of their
+// cell (cell_dt, written to dt_local) and each workgroup leaves the minimum
+// over its cells in dt_partial[blockIdx.x] -- the stand-alone time-step pass
+// folded into the last sweep of the right-hand side.
+template <int DIR, int kRows, typename R, bool DT = false>
 __global__ void __launch_bounds__(64 * kRows) sweep_jk_aweno(ZoneView z, DParams P, const DMech *__restrict__ m,
-                                                             int init, int gx, int nseg) {
+                                                             int init, int gx, int nseg, double *dt_partial) {
   constexpr int kRing = kRows + 5;
   __shared__ R sQ[kRing][6][64];     // cv 0..4, p
   __shared__ R sN[kRing][5][64];     // node flux
@@ -1398,6 +1401,7 @@ __global__ void __launch_bounds__(64 * kRows) sweep_jk_aweno(ZoneView z, DParams
   const int ncds = n - 2 * z.g;
   const int rb = DIR * 3;
   const bool col = L.i < z.mx;
+  double best = 1e+6;  // DT: smallest time step of this thread's cells
   // row q of the line (q >= -3) into ring slot (q + 3) % kRing
   auto stage = [&](int q) {
     const int64_t p = DIR == 1 ? z.at(L.i, q, L.o) : z.at(L.i, L.o, q);
@@ -1468,11 +1472,29 @@ __global__ void __launch_bounds__(64 * kRows) sweep_jk_aweno(ZoneView z, DParams
       }
       const int64_t q = DIR == 1 ? z.at0(L.i, f, L.o) : z.at0(L.i, L.o, f);
       add_dq<0, 0, R>(z, q, init != 0, F, Fp, f < ncds, G, Gp);
+      if constexpr (DT) {
+        const double dtl = cell_dt<0>(z, P, DIR == 1 ? z.at(L.i, f, L.o) : z.at(L.i, L.o, f), q);
+        z.dt_local[q] = dtl;
+        best = fmin(best, dtl);
+      }
     }
     // no barrier here: the next batch's staging and flux stores come after
     // its first barrier, which every thread reaches only once done here
     // (ring rows overwritten next were last read before this batch's
     // second barrier)
+  }
+  if constexpr (DT) {
+    __shared__ double sm[kRows];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) best = fmin(best, __shfl_xor(best, o, 64));
+    if (lane == 0) sm[r] = best;
+    __syncthreads();
+    if (r == 0 && lane == 0) {
+      double v = sm[0];
+#pragma unroll
+      for (int w = 1; w < kRows; ++w) v = fmin(v, sm[w]);
+      dt_partial[blockIdx.x] = v;
+    }
   }
 }
 
@@ -1512,9 +1534,13 @@ static int sweep_segments(int per_cu, int cross, int n, int rows) {
   return best;
 }
 
+// dt_partial / dt_out (non-null): the k-sweep of the staged air path
+// also computes the stage's CFL time step (sweep_jk_aweno<2, ., ., true>)
+// and reduces it into *dt_out; *dt_done tells the caller whether it did.
 template <int NS, int NT, int SCHEME, typename R>
 static hipError_t launch_sweep_t(const ZoneView &z, const DParams &P, const DMech *m, int dir, bool init,
-                                 hipStream_t s) {
+                                 hipStream_t s, double *dt_partial = nullptr, double *dt_out = nullptr,
+                                 int max_partials = 0, bool *dt_done = nullptr) {
   if (dir == 0) {
     const int64_t waves = ((int64_t)z.my * (z.mx + 1) + 62) / 63;
     const int bpp = (int)((waves + 3) / 4);
@@ -1542,7 +1568,7 @@ static hipError_t launch_sweep_t(const ZoneView &z, const DParams &P, const DMec
   };
   auto kernel = [](int d) -> const void * {
     if constexpr (kStaged)
-      return d == 1 ? (const void *)sweep_jk_aweno<1, kRows, R> : (const void *)sweep_jk_aweno<2, kRows, R>;
+      return d == 1 ? (const void *)sweep_jk_aweno<1, kRows, R> : (const void *)sweep_jk_aweno<2, kRows, R, true>;
     else
       return d == 1 ? (const void *)sweep_jk<1, NS, NT, SCHEME, R> : (const void *)sweep_jk<2, NS, NT, SCHEME, R>;
   };
@@ -1552,10 +1578,23 @@ static hipError_t launch_sweep_t(const ZoneView &z, const DParams &P, const DMec
   const dim3 gr((unsigned)((int64_t)cross * nseg));
   const int in = init ? 1 : 0;
   if constexpr (kStaged) {
-    if (dir == 1)
-      hipLaunchKernelGGL((sweep_jk_aweno<1, kRows, R>), gr, dim3(64, kRows, 1), 0, s, z, P, m, in, gx, nseg);
-    else
-      hipLaunchKernelGGL((sweep_jk_aweno<2, kRows, R>), gr, dim3(64, kRows, 1), 0, s, z, P, m, in, gx, nseg);
+    const bool with_dt = dir == 2 && dt_partial && dt_out && !z.inv_sr && P.dim == 3 &&
+                         (int64_t)gr.x <= (int64_t)max_partials;
+    if (dir == 1) {
+      hipLaunchKernelGGL((sweep_jk_aweno<1, kRows, R>), gr, dim3(64, kRows, 1), 0, s, z, P, m, in, gx, nseg,
+                         nullptr);
+    } else if (with_dt) {
+      hipLaunchKernelGGL((sweep_jk_aweno<2, kRows, R, true>), gr, dim3(64, kRows, 1), 0, s, z, P, m, in, gx, nseg,
+                         dt_partial);
+      hipError_t e = hipGetLastError();
+      if (e != hipSuccess) return e;
+      e = launch_min_of(dt_partial, (int)gr.x, dt_out, s);
+      if (e == hipSuccess && dt_done) *dt_done = true;
+      return e;
+    } else {
+      hipLaunchKernelGGL((sweep_jk_aweno<2, kRows, R>), gr, dim3(64, kRows, 1), 0, s, z, P, m, in, gx, nseg,
+                         nullptr);
+    }
   } else {
     if (dir == 1)
       hipLaunchKernelGGL((sweep_jk<1, NS, NT, SCHEME, R>), gr, dim3(64, kRows, 1), 0, s, z, P, m, in, gx, nseg);

@@ -65,9 +65,12 @@ struct InnerFaceDesc {
   int src_tar[3], loop[3], tloop[3], n[3];
 };
 
-// one direction (0 = i, 1 = j, 2 = k); init: the sweep writes dq instead of accumulating
+// one direction (0 = i, 1 = j, 2 = k); init: the sweep writes dq instead of accumulating.
+// dt_w (non-null, k-direction): where the sweep can, it also computes the
+// stage's CFL time step (dt_local and the minimum into dt_w->dt, through
+// dt_w->partial) and sets *dt_done; otherwise *dt_done is left unchanged.
 hipError_t launch_flux_sweep(const ZoneView &z, const DParams &P, const DMech *m, int dir, bool init, bool fp32,
-                             hipStream_t s);
+                             hipStream_t s, const StepScratch *dt_w = nullptr, bool *dt_done = nullptr);
 // the same for the k-omega SST cases (n_turb == 2), flux_sst.hip
 hipError_t launch_flux_sweep_sst(const ZoneView &z, const DParams &P, const DMech *m, int dir, bool init, bool fp32,
                                  hipStream_t s);

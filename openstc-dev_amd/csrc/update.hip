@@ -97,34 +97,7 @@ __global__ void __launch_bounds__(kTB) k_local_dt(ZoneView z, DParams P, double 
     int i, j, k;
     cell_of(z, t, i, j, k);
     const int64_t p = z.at(i, j, k);
-    double mm[9];
-#pragma unroll
-    for (int c = 0; c < 9; ++c) mm[c] = z.metric[c * z.sz + p];
-    const double gxi = sqrt(mm[0] * mm[0] + mm[1] * mm[1] + mm[2] * mm[2]);
-    const double geta = sqrt(mm[3] * mm[3] + mm[4] * mm[4] + mm[5] * mm[5]);
-    const double gzeta = sqrt(mm[6] * mm[6] + mm[7] * mm[7] + mm[8] * mm[8]);
-    const double u = z.bv[z.sz + p], v = z.bv[2 * z.sz + p], w = z.bv[3 * z.sz + p];
-    const double U = u * mm[0] + v * mm[1] + w * mm[2];
-    const double V = u * mm[3] + v * mm[4] + w * mm[5];
-    const double W = u * mm[6] + v * mm[7] + w * mm[8];
-    const double a = z.acoustic[p];
-    const double r0 = fabs(U) + a * gxi, r1 = fabs(V) + a * geta;
-    const double r2 = P.dim == 3 ? fabs(W) + a * gzeta : 0.0;
-    const double sri = r0 + r1 + r2;
-    if (z.inv_sr) {
-      const int64_t q = z.at0(i, j, k);
-      z.inv_sr[q] = r0;
-      z.inv_sr[z.sz0 + q] = r1;
-      z.inv_sr[2 * z.sz0 + q] = r2;
-    }
-    const double gamma = MIX ? z.gam[p] : kGam;
-    const double c1 = fmax(gamma, 4.0 / 3.0) / z.bv[p];
-    double c2 = z.mul[p] / P.Pr;
-    if (z.mut) c2 += z.mut[p] / P.Prt;
-    double srv = gxi * gxi + geta * geta;
-    if (P.dim == 3) srv += gzeta * gzeta;
-    srv *= c1 * c2;
-    const double dtl = P.cfl / (sri + srv);
+    const double dtl = cell_dt<MIX>(z, P, p, z.at0(i, j, k));
     z.dt_local[z.at0(i, j, k)] = dtl;
     best = fmin(best, dtl);
   }
