@@ -58,8 +58,14 @@ def chem_flop_per_cell(labels, ns=9):
 
 
 # kernel behind each timer name of the air case (AWENO-5 + CDS, FP64)
-SWEEP_KERNELS = {"sweep_i": "sweep_i_aweno<double>", "sweep_j": "sweep_jk_aweno<1, 4, double>",
-                 "sweep_k": "sweep_jk_aweno<2, 4, double>"}
+# (the k-sweep of the one-block RK3 step also computes the CFL time step)
+SWEEP_KERNELS = {"sweep_i": "sweep_i_aweno<double>", "sweep_j": "sweep_jk_aweno<1, 4, double, false>",
+                 "sweep_k": "sweep_jk_aweno<2, 4, double, true>"}
+# algorithmic bytes per cell of each sweep: the i-sweep reads cv 5, bv 6,
+# mu, lambda, J, metric 9 and writes dq 5 (23 + 5 doubles); the j-sweep
+# also reads dq (23 + 10); the k-sweep with the time step reads the other
+# two metric rows and the sound speed and writes dt (23 + 10 + 8)
+SWEEP_BYTES_PER_CELL = {"sweep_i": 8 * (23 + 5), "sweep_j": 8 * (23 + 10), "sweep_k": 8 * (23 + 10 + 8)}
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "air_256x128x128_hbm_traffic.md")
 
 
@@ -638,7 +644,7 @@ def main():
     dom = max(sweeps, key=lambda k: sweeps[k][1])
     calls, total_ms = sweeps[dom]
     avg_ms = total_ms / calls
-    algo_bytes = SWEEP_JK_BYTES_PER_CELL * cells
+    algo_bytes = SWEEP_BYTES_PER_CELL.get(dom, SWEEP_JK_BYTES_PER_CELL) * cells
     achieved = algo_bytes / (avg_ms * 1e-3) / 1e9
     kname = SWEEP_KERNELS.get(dom, dom)
     full = (a.nx, a.ny, a.nz) == (256, 128, 128)

@@ -142,7 +142,7 @@ typedef struct ostc_zone_view {
   double *bv; /* 6 components: rho, u, v, w, p, T (required) */
   double *sv; /* n_scalar components: Y_1..Y_ns, k, omega (required when n_scalar > 0) */
   double *cv; /* n_var components: rho, rho u, rho v, rho w, rho E, rho Y.., rho k, rho omega (required) */
-  double *qn; /* n_var components: cv at t^n (NULL: the library keeps its own) */
+  double *qn; /* n_var components: cv at t^n on the interior cells, stored by RK stage 0 (NULL: the library keeps its own) */
   double *dq; /* n_var components, no ghosts (NULL: the library keeps its own) */
 } ostc_zone_view;
 /* Binds caller device arrays as block blk's state after ostc_finalize: the

@@ -1571,8 +1571,6 @@ int ostc_rk3_stage(ostc_solver *s, int blk, double dt, int rk) {
   if (check_blk(s, blk)) return 1;
   if (rk < 0 || rk > 2) return fail("rk stage must be 0, 1 or 2");
   Block &b = s->blocks[blk];
-  if (rk == 0)
-    HIPCHK(hipMemcpyAsync(b.z.qn, b.z.cv, b.z.sz * s->n_var * sizeof(double), hipMemcpyDeviceToDevice, s->stream));
   HIPCHK(hipMemcpyAsync(s->w.dt, &dt, sizeof(double), hipMemcpyHostToDevice, s->stream));
   HIPCHK(launch_rk_update(b.z, s->P, s->d_mech, s->w.dt, rk, s->stream));
   HIPCHK(hipStreamSynchronize(s->stream));
@@ -1620,8 +1618,7 @@ int ostc_step_rk3(ostc_solver *s, int nsteps, double *dt_hist, double *res_hist)
       const int step = s->step + 1 + it;
       for (auto &b : s->blocks) {
         if (step % os == 0) HIPCHK(launch_store_last(b.z, s->stream));
-        HIPCHK(hipMemcpyAsync(b.z.qn, b.z.cv, b.z.sz * s->n_var * sizeof(double), hipMemcpyDeviceToDevice,
-                              s->stream));
+        // qn: stage 0 of k_rk_update stores the step's initial state
       }
       for (int rk = 0; rk < 3; ++rk) {
         // one block: the k-sweep computes the CFL dt where it can

@@ -145,10 +145,15 @@ __global__ void __launch_bounds__(kTB) k_rk_update(ZoneView z, DParams P, const 
   const double dtj = *dtp / z.jac[p];
   constexpr int NV = 5 + NS;
   double cv[NV];
+  // stage 0 keeps the step's initial state in qn on the way (the stages
+  // after it read qn only on interior cells): qn and cv hold the same value
 #pragma unroll
   for (int l = 0; l < NV; ++l) {
     const int64_t o = (int64_t)l * z.sz + p;
-    cv[l] = A * z.qn[o] + B * z.cv[o] + C * dtj * z.dq[(int64_t)l * z.sz0 + q];
+    const double c0 = z.cv[o];
+    const double q0 = rk == 0 ? c0 : z.qn[o];
+    if (rk == 0) z.qn[o] = c0;
+    cv[l] = A * q0 + B * c0 + C * dtj * z.dq[(int64_t)l * z.sz0 + q];
   }
   if (z.mz == 1) cv[3] = 0;
 #pragma unroll
@@ -157,7 +162,10 @@ __global__ void __launch_bounds__(kTB) k_rk_update(ZoneView z, DParams P, const 
   // turbulence scalars (rho k, rho omega) after the species
   for (int l = NV; l < P.nv; ++l) {
     const int64_t o = (int64_t)l * z.sz + p;
-    const double c = A * z.qn[o] + B * z.cv[o] + C * dtj * z.dq[(int64_t)l * z.sz0 + q];
+    const double c0 = z.cv[o];
+    const double q0 = rk == 0 ? c0 : z.qn[o];
+    if (rk == 0) z.qn[o] = c0;
+    const double c = A * q0 + B * c0 + C * dtj * z.dq[(int64_t)l * z.sz0 + q];
     z.cv[o] = c;
     z.sv[(int64_t)(l - 5) * z.sz + p] = c * ri;
   }
